@@ -1,0 +1,89 @@
+/*
+ * libofw_hip.so -- MI355X (gfx950) free-wake induced-velocity library: C ABI.
+ *
+ * The drop-in boundary for OpenFAWE's vortex batch backends.  Each entry point
+ * replaces one engine of the reference (cited per declaration) and keeps its
+ * argument meaning: targets (n,3), segment starts (m,3), unit directions (m,3),
+ * lengths (m), circulations (m), core radii (m) in; velocities (n,3) and
+ * per-target flags out.  Flags follow vortex_numba.py:32-34 (0 OK, 1 SINGULAR,
+ * 2 NONFINITE; the first flag of a target in its summation order is kept).
+ *
+ * Conventions
+ *   - Every array pointer is a DEVICE pointer (HBM) unless a name says host.
+ *     Arrays are C-contiguous; float64 for geometry, int64 for indices as in
+ *     the reference's NumPy engines.
+ *   - Calls are enqueued on the context's stream and are synchronous on return
+ *     (the host may read the outputs after the call returns), like the
+ *     reference's engines, which synchronize every launch.
+ *   - Errors are returned as status codes, never thrown; ofw_last_error()
+ *     gives the message of the calling thread's last failure.
+ *   - Results are deterministic: the same inputs and settings give the same
+ *     bits on every call (fixed-order reductions, no floating-point atomics).
+ *   - A context owns the scratch memory of its calls and is not shared
+ *     between host threads.
+ */
+#ifndef OFW_HIP_H
+#define OFW_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OFW_ABI_VERSION 1
+
+enum ofw_status {
+  OFW_OK = 0,
+  OFW_ERR_ARG = 1,         /* invalid argument (the reference raises ValueError) */
+  OFW_ERR_HIP = 2,         /* HIP runtime failure */
+  OFW_ERR_UNSUPPORTED = 3, /* e.g. a core model the tree code refuses (vortex_tree.py:593-594) */
+  OFW_ERR_OVERFLOW = 4     /* traversal limit exceeded (vortex_cuda_tree.py:548-549) */
+};
+
+enum ofw_precision {
+  OFW_FP64 = 0,  /* FP64 throughout */
+  OFW_MIXED = 1, /* FP32 far field and far pairs, FP64 near pairs and accumulation */
+  OFW_FP32 = 2   /* as MIXED with an FP32 accumulator */
+};
+
+typedef struct ofw_ctx ofw_ctx;
+
+/* ABI version of the library (OFW_ABI_VERSION). */
+int ofw_abi_version(void);
+
+/* Message of the calling thread's last failed call ("" when none). */
+const char *ofw_last_error(void);
+
+/* Create a context bound to a HIP stream (hipStream_t; NULL = the null stream) on the current device.
+ * Replaces the module-level device state of vortex_cuda.py:53-82 (probe, kernels cache). */
+int ofw_ctx_create(ofw_ctx **ctx, void *stream);
+int ofw_ctx_destroy(ofw_ctx *ctx);
+int ofw_ctx_set_stream(ofw_ctx *ctx, void *stream);
+
+/* Phase timings of the context's last call, in milliseconds (host buffer of `count` floats), measured with
+ * HIP events on the context's stream.  Returns the number of phases written.  The direct sum has three
+ * phases (segment packing + exclusion conversion, the pair kernel, the chunk reduction).  Replaces the
+ * Statistics timings of vortex_cuda.py:372-378 and vortex_cuda_tree.py:429-436. */
+int ofw_ctx_timings(ofw_ctx *ctx, float *ms_host, int count);
+
+/* Device probe: writes the device name (host buffer) and its compute-unit count.
+ * Replaces vortex_cuda.available / probe_error (vortex_cuda.py:58-82). */
+int ofw_device_info(char *name_host, int name_len, int *cu_count_host);
+
+/* Direct sum over all segments for every target, in segment order per target, skipping the pairs of the
+ * exclusion CSR (ex_ptr (n+1), ex_seg (e), per-target sorted unique segment ids; vortex_numba.py:228-243).
+ * Replaces vortex_numba.direct_sum (vortex_numba.py:246-260) and vortex_cuda.cuda_sum (vortex_cuda.py:381-467).
+ *   precision  ofw_precision
+ *   block      threads per workgroup (32, 64, 128 or 256; CudaSettings.block, vortex.py:103)
+ *   model      CoreModel 0..5 (vortex.py:21-27); raise_singular: singularity == "raise"
+ *   flag_seg   optional (may be NULL): per target, the segment index of its first flagged pair, -1 if none
+ * out (n,3) float64, flags (n) int8. */
+int ofw_direct_sum(ofw_ctx *ctx, int precision, int block, int64_t n, const double *points, int64_t m,
+                   const double *starts, const double *directions, const double *lengths, const double *gamma,
+                   const double *radius, int model, int raise_singular, const int64_t *ex_ptr,
+                   const int64_t *ex_seg, double *out, int8_t *flags, int32_t *flag_seg);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OFW_HIP_H */

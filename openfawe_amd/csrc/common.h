@@ -1,0 +1,258 @@
+// Shared definitions of the MI355X (gfx950) free-wake velocity library.
+//
+// Segment records.  The per-segment inputs of the reference engine
+// (starts, directions, lengths, gamma, radius; vortex_numba.py:246) are packed
+// once per call into 16-double records (128 B, one cache line pair) so that a
+// wavefront that walks segments in lock-step reads each record with scalar
+// loads (s_load_dwordx16) into SGPRs: the segment operand of every pair is
+// wave-uniform, the target operand is per lane.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define OFW_FLAG_OK 0
+#define OFW_FLAG_SINGULAR 1
+#define OFW_FLAG_NONFINITE 2
+
+#define OFW_MODEL_NONE 0
+#define OFW_MODEL_RANKINE 1
+#define OFW_MODEL_LAMB 2
+#define OFW_MODEL_VATISTAS 3
+#define OFW_MODEL_OFFSET 4
+#define OFW_MODEL_RM 5
+
+// FP64 segment record slots
+enum {
+  R_SX = 0, R_SY, R_SZ, R_DX, R_DY, R_DZ, R_L, R_INVL, R_G4, R_RC, R_RC2, R_GAMMA, R_RADIUS, R_EXACT,
+  R_PAD0, R_PAD1, REC64 = 16
+};
+// FP32 segment record slots (16 floats, 64 B)
+enum {
+  F_DX = 0, F_DY, F_DZ, F_INVL, F_G4, F_RC2, F_EXACT, F_PAD0, REC32 = 8
+};
+
+#define OFW_AXIS_EPS (8.0 * 2.220446049250313e-16)   // 8 * np.finfo(np.float64).eps, vortex_numba.py:28
+#define OFW_LAMB 1.25643
+#define OFW_FOUR_PI (4.0 * 3.141592653589793)        // 4 * np.pi, vortex_numba.py:30
+
+// Fast FP64 path limits: outside them the pair is evaluated operation by operation as the reference
+// does (pair_exact64).  |a| <= 1e100 segment lengths keeps every square finite and normal; a core of
+// at least 1e-50 segment lengths keeps (h/rc)^2 finite.
+#define OFW_FAST_A2_MAX 1e200
+#define OFW_FAST_RC_MIN 1e-50
+
+// Mixed precision: a pair goes to FP64 when the target is within NEAR_CORES core radii of the
+// segment's line, or within NEAR_AXIS segment lengths (scaled like the axis guard) of it.
+#define OFW_NEAR_CORES2 16.0f          // (4 core radii)^2
+#define OFW_NEAR_AXIS2 1e-6f           // (1e-3)^2
+#define OFW_F32_A2_MAX 1e30f
+
+struct PairOut {
+  double x, y, z;
+  int flag;
+};
+
+__device__ __forceinline__ double ofw_norm3(double x, double y, double z) { return hypot(hypot(x, y), z); }
+
+// Operation-by-operation restatement of vortex_numba.py:42-109 (pair_velocity); the FP64 fallback for
+// extreme ranges (huge |a|, tiny cores).  Accumulates into u when the flag is OK.
+__device__ __noinline__ int pair_exact64(double px, double py, double pz, const double* __restrict__ r,
+                                         int model, int raise_singular, double& ux, double& uy, double& uz) {
+#pragma clang fp contract(off)
+  const double sx = r[R_SX], sy = r[R_SY], sz = r[R_SZ];
+  const double dx = r[R_DX], dy = r[R_DY], dz = r[R_DZ];
+  const double length = r[R_L], gamma = r[R_GAMMA], radius = r[R_RADIUS];
+  double ax = (px - sx) / length, ay = (py - sy) / length, az = (pz - sz) / length;
+  double t = (ax * dx + ay * dy) + az * dz;
+  double cx = dy * az - dz * ay, cy = dz * ax - dx * az, cz = dx * ay - dy * ax;
+  double h = ofw_norm3(cx, cy, cz), na = ofw_norm3(ax, ay, az), nb = ofw_norm3(ax - dx, ay - dy, az - dz);
+  if (!(isfinite(h) && isfinite(na) && isfinite(nb) && isfinite(t))) return OFW_FLAG_NONFINITE;
+  double big = na > nb ? na : nb;
+  double scale = 1.0 > big ? 1.0 : big;
+  if (h <= OFW_AXIS_EPS * scale) {
+    if (raise_singular && t >= 0.0 && t <= 1.0) return OFW_FLAG_SINGULAR;
+    return OFW_FLAG_OK;
+  }
+  if (gamma == 0.0) return OFW_FLAG_OK;
+  double product = na * nb;
+  double dot = h * h + t * (t - 1.0);
+  double plus = product + dot;
+  if (dot < 0.0) plus = (h * h) / (product - dot);
+  double denominator = product * plus;
+  double rc = radius / length;
+  double factor = 1.0;
+  if (radius > 0.0 && model != 0) {
+    if (model == OFW_MODEL_RM) {
+      na = hypot(na, rc);
+      nb = hypot(nb, rc);
+      product = na * nb;
+      double transverse2 = h * h + rc * rc;
+      dot = transverse2 + t * (t - 1.0);
+      plus = product + dot;
+      if (dot < 0.0) plus = transverse2 / (product - dot);
+      denominator = product * plus;
+    } else if (model == OFW_MODEL_OFFSET) {
+      denominator = denominator + rc * rc;
+    } else {
+      double ratio = h / rc;
+      if (!isfinite(ratio)) return OFW_FLAG_NONFINITE;
+      if (model == OFW_MODEL_RANKINE) {
+        double limited = ratio < 1.0 ? ratio : 1.0;
+        factor = limited * limited;
+      } else if (model == OFW_MODEL_LAMB) {
+        if (ratio <= sqrt(10.0 / OFW_LAMB)) factor = -expm1(-OFW_LAMB * (ratio * ratio));
+      } else if (model == OFW_MODEL_VATISTAS) {
+        if (ratio < 1.0) {
+          double q = ratio * ratio;
+          factor = q / hypot(1.0, q);
+        } else {
+          double inverse = 1.0 / ratio;
+          factor = 1.0 / hypot(1.0, inverse * inverse);
+        }
+      }
+    }
+  }
+  double coefficient = (gamma / length) / OFW_FOUR_PI * factor * (na + nb) / denominator;
+  if (!(isfinite(denominator) && isfinite(coefficient) && isfinite(product) && isfinite(dot)))
+    return OFW_FLAG_NONFINITE;
+  ux += coefficient * cx;
+  uy += coefficient * cy;
+  uz += coefficient * cz;
+  return OFW_FLAG_OK;
+}
+
+// Fast FP64 pair: the reference formula (vortex_numba.py:46-109) reorganised for the VALU: a per-segment
+// reciprocal length and circulation factor, squared norms (no hypot), the axis test and the core ratios
+// on squares, one division per pair.  Same branches and flags; results agree with the operation-by-
+// operation form to a few ulp.  Falls back to pair_exact64 outside the fast range.
+template <int MODEL>
+__device__ __forceinline__ int pair_fast64(double px, double py, double pz, const double* __restrict__ r,
+                                           int raise_singular, double& ux, double& uy, double& uz) {
+  const double invL = r[R_INVL];
+  const double dx = r[R_DX], dy = r[R_DY], dz = r[R_DZ];
+  const double ax = (px - r[R_SX]) * invL, ay = (py - r[R_SY]) * invL, az = (pz - r[R_SZ]) * invL;
+  const double cx = dy * az - dz * ay, cy = dz * ax - dx * az, cz = dx * ay - dy * ax;
+  const double na2 = ax * ax + ay * ay + az * az;
+  const double bx = ax - dx, by = ay - dy, bz = az - dz;
+  const double nb2 = bx * bx + by * by + bz * bz;
+  const bool exact_seg = (MODEL == OFW_MODEL_RANKINE || MODEL == OFW_MODEL_LAMB || MODEL == OFW_MODEL_VATISTAS)
+                             ? r[R_EXACT] != 0.0 : false;
+  if (!(fmax(na2, nb2) <= OFW_FAST_A2_MAX) || exact_seg)
+    return pair_exact64(px, py, pz, r, MODEL, raise_singular, ux, uy, uz);
+  const double t = (ax * dx + ay * dy) + az * dz;
+  const double h2 = cx * cx + cy * cy + cz * cz;
+  const double na = __builtin_sqrt(na2), nb = __builtin_sqrt(nb2);
+  const double mx = fmax(1.0, fmax(na, nb));
+  if (h2 <= (OFW_AXIS_EPS * OFW_AXIS_EPS) * (mx * mx)) {
+    if (raise_singular && t >= 0.0 && t <= 1.0) return OFW_FLAG_SINGULAR;
+    return OFW_FLAG_OK;
+  }
+  const double g4 = r[R_G4];
+  if (g4 == 0.0) return OFW_FLAG_OK;
+  double sa = na, sb = nb;
+  double product = na * nb;
+  double dot = h2 + t * (t - 1.0);
+  double plus = dot < 0.0 ? h2 / (product - dot) : product + dot;
+  double den = product * plus;
+  double factor = 1.0;
+  const double radius = r[R_RADIUS];
+  if (MODEL != OFW_MODEL_NONE && radius > 0.0) {
+    const double rc2 = r[R_RC2];
+    if (MODEL == OFW_MODEL_RANKINE) {
+      factor = fmin(h2 / rc2, 1.0);
+    } else if (MODEL == OFW_MODEL_LAMB) {
+      const double q = h2 / rc2;
+      if (q <= 10.0 / OFW_LAMB) factor = -expm1(-OFW_LAMB * q);
+    } else if (MODEL == OFW_MODEL_VATISTAS) {
+      const double q = h2 / rc2;
+      if (q < 1.0) {
+        factor = q / __builtin_sqrt(1.0 + q * q);
+      } else {
+        const double iq = 1.0 / q;
+        factor = 1.0 / __builtin_sqrt(1.0 + iq * iq);
+      }
+    } else if (MODEL == OFW_MODEL_OFFSET) {
+      den = den + rc2;
+    } else if (MODEL == OFW_MODEL_RM) {
+      sa = __builtin_sqrt(na2 + rc2);
+      sb = __builtin_sqrt(nb2 + rc2);
+      product = sa * sb;
+      const double tr2 = h2 + rc2;
+      dot = tr2 + t * (t - 1.0);
+      plus = dot < 0.0 ? tr2 / (product - dot) : product + dot;
+      den = product * plus;
+    }
+  }
+  const double coef = g4 * factor * (sa + sb) / den;
+  if (!isfinite(coef)) return OFW_FLAG_NONFINITE;
+  ux += coef * cx;
+  uy += coef * cy;
+  uz += coef * cz;
+  return OFW_FLAG_OK;
+}
+
+// FP32 pair for the mixed / fp32 precisions.  rel = p - s rounded from FP64.  Returns true when the pair
+// is near (must be evaluated in FP64): within 4 core radii or 1e-3 (scaled) segment lengths of the
+// segment's line, or outside the FP32 range.  Otherwise accumulates the FP32 velocity.
+template <int MODEL>
+__device__ __forceinline__ bool pair_f32(float rx, float ry, float rz, const float* __restrict__ f,
+                                         float& ux, float& uy, float& uz) {
+  const float invL = f[F_INVL];
+  const float dx = f[F_DX], dy = f[F_DY], dz = f[F_DZ];
+  const float ax = rx * invL, ay = ry * invL, az = rz * invL;
+  const float cx = dy * az - dz * ay, cy = dz * ax - dx * az, cz = dx * ay - dy * ax;
+  const float na2 = ax * ax + ay * ay + az * az;
+  const float bx = ax - dx, by = ay - dy, bz = az - dz;
+  const float nb2 = bx * bx + by * by + bz * bz;
+  const float h2 = cx * cx + cy * cy + cz * cz;
+  const float m2 = fmaxf(1.0f, fmaxf(na2, nb2));
+  const float rc2 = f[F_RC2];
+  bool near = !(m2 <= OFW_F32_A2_MAX) || h2 <= OFW_NEAR_AXIS2 * m2 || f[F_EXACT] != 0.0f;
+  if (MODEL != OFW_MODEL_NONE) near = near || h2 < OFW_NEAR_CORES2 * rc2;
+  if (near) return true;
+  const float g4 = f[F_G4];
+  if (g4 == 0.0f) return false;
+  const float t = (ax * dx + ay * dy) + az * dz;
+  float sa = __builtin_amdgcn_sqrtf(na2), sb = __builtin_amdgcn_sqrtf(nb2);
+  float product = sa * sb;
+  float dot = h2 + t * (t - 1.0f);
+  float plus = dot < 0.0f ? h2 * __builtin_amdgcn_rcpf(product - dot) : product + dot;
+  float den = product * plus;
+  float factor = 1.0f;
+  if (MODEL == OFW_MODEL_VATISTAS && rc2 > 0.0f) {
+    // beyond 4 core radii q >= 16: factor = 1/sqrt(1 + q^-2)
+    const float iq = rc2 * __builtin_amdgcn_rcpf(h2);
+    factor = __builtin_amdgcn_rsqf(1.0f + iq * iq);
+  } else if (MODEL == OFW_MODEL_OFFSET) {
+    den = den + rc2;
+  } else if (MODEL == OFW_MODEL_RM && rc2 > 0.0f) {
+    sa = __builtin_amdgcn_sqrtf(na2 + rc2);
+    sb = __builtin_amdgcn_sqrtf(nb2 + rc2);
+    product = sa * sb;
+    const float tr2 = h2 + rc2;
+    dot = tr2 + t * (t - 1.0f);
+    plus = dot < 0.0f ? tr2 * __builtin_amdgcn_rcpf(product - dot) : product + dot;
+    den = product * plus;
+  }
+  // Rankine and Lamb-Oseen factors are exactly 1 beyond 4 core radii (ratio^2 >= 16 > 10/1.25643).
+  const float coef = g4 * factor * (sa + sb) * __builtin_amdgcn_rcpf(den);
+  ux += coef * cx;
+  uy += coef * cy;
+  uz += coef * cz;
+  return false;
+}
+
+// Deterministic first-flag bookkeeping: a lane keeps the first flagged segment in its own order.
+struct FlagState {
+  int flag;
+  int seg;
+};
+
+__device__ __forceinline__ void note_flag(FlagState& st, int f, int s) {
+  if (f != OFW_FLAG_OK && st.flag == OFW_FLAG_OK) {
+    st.flag = f;
+    st.seg = s;
+  }
+}
+
+#define OFW_CHECK_LAUNCH() (hipGetLastError())

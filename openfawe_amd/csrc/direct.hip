@@ -1,0 +1,301 @@
+// Direct Biot-Savart sum on gfx950.
+//
+// Work decomposition: a 2-D grid of (target block) x (segment chunk).  A wavefront owns 64 targets (one
+// per lane) and walks its chunk of segments in input order; each segment record is wave-uniform and is
+// read with scalar loads, so the only per-lane operands are the target and the accumulators.  20,570
+// targets are only 322 wavefronts, far too few for 256 CUs, so the segments are cut into chunks and the
+// per-chunk partial sums are added in chunk order by a second kernel (fixed order: deterministic).
+// Per target, the result is the reference's ordered sum over segments (vortex_numba.py:112-147),
+// re-associated at chunk boundaries.
+#include <climits>
+
+#include "common.h"
+#include "internal.h"
+
+namespace ofw {
+
+// ---------------------------------------------------------------- segment records
+__global__ __launch_bounds__(256) void k_pack_segments(int m, const double* __restrict__ starts,
+                                                       const double* __restrict__ directions,
+                                                       const double* __restrict__ lengths,
+                                                       const double* __restrict__ gamma,
+                                                       const double* __restrict__ radius, int model,
+                                                       double* __restrict__ rec64, float* __restrict__ rec32) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= m) return;
+  const double L = lengths[s], g = gamma[s], rad = radius[s];
+  const double invL = 1.0 / L;
+  const double g4 = (g / L) / OFW_FOUR_PI;
+  const double rc = rad / L;
+  const double rc2 = rc * rc;
+  const bool cored = rad > 0.0 && model != OFW_MODEL_NONE;
+  const bool ratio_model = model == OFW_MODEL_RANKINE || model == OFW_MODEL_LAMB || model == OFW_MODEL_VATISTAS;
+  const double exact = (cored && ratio_model && !(rc >= OFW_FAST_RC_MIN)) ? 1.0 : 0.0;
+  double* r = rec64 + (size_t)s * REC64;
+  r[R_SX] = starts[3 * (size_t)s];
+  r[R_SY] = starts[3 * (size_t)s + 1];
+  r[R_SZ] = starts[3 * (size_t)s + 2];
+  r[R_DX] = directions[3 * (size_t)s];
+  r[R_DY] = directions[3 * (size_t)s + 1];
+  r[R_DZ] = directions[3 * (size_t)s + 2];
+  r[R_L] = L;
+  r[R_INVL] = invL;
+  r[R_G4] = g4;
+  r[R_RC] = rc;
+  r[R_RC2] = cored ? rc2 : 0.0;
+  r[R_GAMMA] = g;
+  r[R_RADIUS] = rad;
+  r[R_EXACT] = exact;
+  r[R_PAD0] = 0.0;
+  r[R_PAD1] = 0.0;
+  // FP32 record: a segment whose factors leave the normal FP32 range always takes the FP64 path.
+  const double ag = fabs(g4);
+  const bool f32_ok = exact == 0.0 && invL < 1e30 && invL > 1e-30 && ag < 1e30 && (ag == 0.0 || ag > 1e-30) &&
+                      (!cored || (rc2 < 1e30 && rc2 > 1e-30));
+  float* f = rec32 + (size_t)s * REC32;
+  f[F_DX] = (float)r[R_DX];
+  f[F_DY] = (float)r[R_DY];
+  f[F_DZ] = (float)r[R_DZ];
+  f[F_INVL] = (float)invL;
+  f[F_G4] = (float)g4;
+  f[F_RC2] = cored ? (float)rc2 : 0.0f;
+  f[F_EXACT] = f32_ok ? 0.0f : 1.0f;
+  f[F_PAD0] = 0.0f;
+}
+
+int pack_segments(Ctx& c, int64_t m, const double* starts, const double* directions, const double* lengths,
+                  const double* gamma, const double* radius, int model, double* rec64, float* rec32) {
+  if (m == 0) return OFW_OK;
+  hipLaunchKernelGGL(k_pack_segments, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, c.stream, (int)m, starts,
+                     directions, lengths, gamma, radius, model, rec64, rec32);
+  OFW_HIP(hipGetLastError());
+  return OFW_OK;
+}
+
+// int64 CSR -> int32 CSR (the kernels index with 32-bit integers)
+__global__ void k_i64_to_i32(int64_t count, const int64_t* __restrict__ src, int32_t* __restrict__ dst) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) dst[i] = (int32_t)src[i];
+}
+
+int csr_to_i32(Ctx& c, int64_t n, const int64_t* ex_ptr, const int64_t* ex_seg, int32_t** ptr32, int32_t** seg32) {
+  int64_t e = 0;
+  OFW_HIP(hipMemcpyAsync(&e, ex_ptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
+  OFW_HIP(hipStreamSynchronize(c.stream));
+  if (e < 0 || e > INT_MAX) {
+    set_error("exclusion count out of range");
+    return OFW_ERR_ARG;
+  }
+  OFW_HIP(c.exptr32.reserve((size_t)(n + 1) * sizeof(int32_t)));
+  OFW_HIP(c.exseg32.reserve((size_t)(e > 0 ? e : 1) * sizeof(int32_t)));
+  *ptr32 = (int32_t*)c.exptr32.ptr;
+  *seg32 = (int32_t*)c.exseg32.ptr;
+  hipLaunchKernelGGL(k_i64_to_i32, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, c.stream, n + 1, ex_ptr,
+                     *ptr32);
+  if (e > 0)
+    hipLaunchKernelGGL(k_i64_to_i32, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, c.stream, e, ex_seg, *seg32);
+  OFW_HIP(hipGetLastError());
+  return OFW_OK;
+}
+
+// ---------------------------------------------------------------- direct kernels
+// code of a chunk's first flag: -1 when none, else seg * 4 + flag
+__device__ __forceinline__ int flag_code(const FlagState& st) { return st.flag ? st.seg * 4 + st.flag : -1; }
+
+template <int MODEL>
+__global__ __launch_bounds__(256) void k_direct64(int n, const double* __restrict__ pts, int m,
+                                                  const double* __restrict__ rec, int chunk,
+                                                  const int32_t* __restrict__ ex_ptr,
+                                                  const int32_t* __restrict__ ex_seg, int raise_singular,
+                                                  double* __restrict__ part, int32_t* __restrict__ pflag) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  const int s0 = c * chunk;
+  const int s1 = min(m, s0 + chunk);
+  const bool active = p < n;
+  double px = 0.0, py = 0.0, pz = 0.0;
+  int k = 0, stop = 0, next = INT_MAX;
+  if (active) {
+    px = pts[3 * (size_t)p];
+    py = pts[3 * (size_t)p + 1];
+    pz = pts[3 * (size_t)p + 2];
+    k = ex_ptr[p];
+    stop = ex_ptr[p + 1];
+    while (k < stop && ex_seg[k] < s0) k++;
+    next = k < stop ? ex_seg[k] : INT_MAX;
+  }
+  double ux = 0.0, uy = 0.0, uz = 0.0;
+  FlagState st{OFW_FLAG_OK, INT_MAX};
+  for (int s = s0; s < s1; ++s) {
+    const double* r = rec + (size_t)s * REC64;
+    if (s == next) {
+      ++k;
+      next = k < stop ? ex_seg[k] : INT_MAX;
+      continue;
+    }
+    const int f = pair_fast64<MODEL>(px, py, pz, r, raise_singular, ux, uy, uz);
+    note_flag(st, f, s);
+  }
+  if (active) {
+    const size_t o = (size_t)c * n + p;
+    part[3 * o] = ux;
+    part[3 * o + 1] = uy;
+    part[3 * o + 2] = uz;
+    pflag[o] = flag_code(st);
+  }
+}
+
+// Mixed / fp32: FP32 pair arithmetic on the target-minus-start vector formed in FP64; near pairs in FP64.
+template <int MODEL, bool WIDE>
+__global__ __launch_bounds__(256) void k_direct_mixed(int n, const double* __restrict__ pts, int m,
+                                                      const double* __restrict__ rec,
+                                                      const float* __restrict__ rec32, int chunk,
+                                                      const int32_t* __restrict__ ex_ptr,
+                                                      const int32_t* __restrict__ ex_seg, int raise_singular,
+                                                      double* __restrict__ part, int32_t* __restrict__ pflag) {
+  typedef typename std::conditional<WIDE, double, float>::type acc_t;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  const int s0 = c * chunk;
+  const int s1 = min(m, s0 + chunk);
+  const bool active = p < n;
+  double px = 0.0, py = 0.0, pz = 0.0;
+  int k = 0, stop = 0, next = INT_MAX;
+  if (active) {
+    px = pts[3 * (size_t)p];
+    py = pts[3 * (size_t)p + 1];
+    pz = pts[3 * (size_t)p + 2];
+    k = ex_ptr[p];
+    stop = ex_ptr[p + 1];
+    while (k < stop && ex_seg[k] < s0) k++;
+    next = k < stop ? ex_seg[k] : INT_MAX;
+  }
+  acc_t ux = 0, uy = 0, uz = 0;
+  FlagState st{OFW_FLAG_OK, INT_MAX};
+  for (int s = s0; s < s1; ++s) {
+    const double* r = rec + (size_t)s * REC64;
+    const float* f = rec32 + (size_t)s * REC32;
+    if (s == next) {
+      ++k;
+      next = k < stop ? ex_seg[k] : INT_MAX;
+      continue;
+    }
+    const float rx = (float)(px - r[R_SX]), ry = (float)(py - r[R_SY]), rz = (float)(pz - r[R_SZ]);
+    float vx = 0.0f, vy = 0.0f, vz = 0.0f;
+    if (pair_f32<MODEL>(rx, ry, rz, f, vx, vy, vz)) {
+      double wx = 0.0, wy = 0.0, wz = 0.0;
+      const int fl = pair_fast64<MODEL>(px, py, pz, r, raise_singular, wx, wy, wz);
+      note_flag(st, fl, s);
+      ux += (acc_t)wx;
+      uy += (acc_t)wy;
+      uz += (acc_t)wz;
+    } else {
+      ux += (acc_t)vx;
+      uy += (acc_t)vy;
+      uz += (acc_t)vz;
+    }
+  }
+  if (active) {
+    const size_t o = (size_t)c * n + p;
+    part[3 * o] = (double)ux;
+    part[3 * o + 1] = (double)uy;
+    part[3 * o + 2] = (double)uz;
+    pflag[o] = flag_code(st);
+  }
+}
+
+// Sum of the chunk partials in chunk order; first flag in segment order; the reference's final
+// non-finite check (vortex_numba.py:142-143).
+template <bool WIDE>
+__global__ __launch_bounds__(256) void k_direct_reduce(int n, int chunks, const double* __restrict__ part,
+                                                       const int32_t* __restrict__ pflag, double* __restrict__ out,
+                                                       int8_t* __restrict__ flags, int32_t* __restrict__ flag_seg) {
+  typedef typename std::conditional<WIDE, double, float>::type acc_t;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  acc_t ux = 0, uy = 0, uz = 0;
+  int code = -1;
+  for (int c = 0; c < chunks; ++c) {
+    const size_t o = (size_t)c * n + p;
+    ux += (acc_t)part[3 * o];
+    uy += (acc_t)part[3 * o + 1];
+    uz += (acc_t)part[3 * o + 2];
+    if (code < 0) code = pflag[o];
+  }
+  int flag = code < 0 ? OFW_FLAG_OK : (code & 3);
+  int seg = code < 0 ? -1 : (code >> 2);
+  if (!(isfinite((double)ux) && isfinite((double)uy) && isfinite((double)uz)) && flag == OFW_FLAG_OK)
+    flag = OFW_FLAG_NONFINITE;
+  out[3 * (size_t)p] = (double)ux;
+  out[3 * (size_t)p + 1] = (double)uy;
+  out[3 * (size_t)p + 2] = (double)uz;
+  flags[p] = (int8_t)flag;
+  if (flag_seg) flag_seg[p] = seg;
+}
+
+template <int MODEL>
+static void launch_direct(int precision, dim3 grid, dim3 block, hipStream_t st, int n, const double* pts, int m,
+                          const double* rec64, const float* rec32, int chunk, const int32_t* ex_ptr,
+                          const int32_t* ex_seg, int raise, double* part, int32_t* pflag) {
+  if (precision == OFW_FP64)
+    hipLaunchKernelGGL(k_direct64<MODEL>, grid, block, 0, st, n, pts, m, rec64, chunk, ex_ptr, ex_seg, raise, part,
+                       pflag);
+  else if (precision == OFW_MIXED)
+    hipLaunchKernelGGL((k_direct_mixed<MODEL, true>), grid, block, 0, st, n, pts, m, rec64, rec32, chunk, ex_ptr,
+                       ex_seg, raise, part, pflag);
+  else
+    hipLaunchKernelGGL((k_direct_mixed<MODEL, false>), grid, block, 0, st, n, pts, m, rec64, rec32, chunk, ex_ptr,
+                       ex_seg, raise, part, pflag);
+}
+
+int direct_sum(Ctx& c, int precision, int block, int64_t n, const double* points, int64_t m, const double* rec64,
+               const float* rec32, int model, int raise_singular, const int32_t* ex_ptr, const int32_t* ex_seg,
+               double* out, int8_t* flags, int32_t* flag_seg) {
+  if (n == 0) return OFW_OK;
+  const int64_t bx = (n + block - 1) / block;
+  const int64_t waves_per_block = (block + 63) / 64;
+  // enough wavefronts for every SIMD of every CU to hold several (MI355X: 256 CUs x 4 SIMDs)
+  const int64_t want_waves = (int64_t)c.cu_count * 4 * 8;
+  int64_t chunks = (want_waves + bx * waves_per_block - 1) / (bx * waves_per_block);
+  const int64_t min_chunk = 64;
+  int64_t max_chunks = (m + min_chunk - 1) / min_chunk;
+  if (chunks > max_chunks) chunks = max_chunks;
+  if (chunks < 1) chunks = 1;
+  if (chunks > 65535) chunks = 65535;
+  int64_t chunk = (m + chunks - 1) / chunks;
+  if (chunk < 1) chunk = 1;
+  chunks = m > 0 ? (m + chunk - 1) / chunk : 1;
+  OFW_HIP(c.part.reserve((size_t)chunks * n * 3 * sizeof(double)));
+  OFW_HIP(c.pflag.reserve((size_t)chunks * n * sizeof(int32_t)));
+  double* part = (double*)c.part.ptr;
+  int32_t* pflag = (int32_t*)c.pflag.ptr;
+  if (m > 0) {
+    dim3 grid((unsigned)bx, (unsigned)chunks);
+    dim3 blk((unsigned)block);
+    switch (model) {
+      case 0: launch_direct<0>(precision, grid, blk, c.stream, (int)n, points, (int)m, rec64, rec32, (int)chunk, ex_ptr, ex_seg, raise_singular, part, pflag); break;
+      case 1: launch_direct<1>(precision, grid, blk, c.stream, (int)n, points, (int)m, rec64, rec32, (int)chunk, ex_ptr, ex_seg, raise_singular, part, pflag); break;
+      case 2: launch_direct<2>(precision, grid, blk, c.stream, (int)n, points, (int)m, rec64, rec32, (int)chunk, ex_ptr, ex_seg, raise_singular, part, pflag); break;
+      case 3: launch_direct<3>(precision, grid, blk, c.stream, (int)n, points, (int)m, rec64, rec32, (int)chunk, ex_ptr, ex_seg, raise_singular, part, pflag); break;
+      case 4: launch_direct<4>(precision, grid, blk, c.stream, (int)n, points, (int)m, rec64, rec32, (int)chunk, ex_ptr, ex_seg, raise_singular, part, pflag); break;
+      default: launch_direct<5>(precision, grid, blk, c.stream, (int)n, points, (int)m, rec64, rec32, (int)chunk, ex_ptr, ex_seg, raise_singular, part, pflag); break;
+    }
+    OFW_HIP(hipGetLastError());
+  } else {
+    chunks = 0;
+  }
+  OFW_HIP(c.mark());
+  if (precision == OFW_FP32)
+    hipLaunchKernelGGL(k_direct_reduce<false>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, (int)n,
+                       (int)chunks, part, pflag, out, flags, flag_seg);
+  else
+    hipLaunchKernelGGL(k_direct_reduce<true>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, (int)n,
+                       (int)chunks, part, pflag, out, flags, flag_seg);
+  OFW_HIP(hipGetLastError());
+  OFW_HIP(c.mark());
+  c.stats_i[0] = chunks;
+  c.stats_i[1] = chunk;
+  return OFW_OK;
+}
+
+}  // namespace ofw

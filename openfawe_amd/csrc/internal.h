@@ -1,0 +1,54 @@
+// Host-side internals shared by the translation units of libofw_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/ofw_hip.h"
+
+namespace ofw {
+
+void set_error(const std::string& msg);
+
+// A growable device buffer owned by a context.  Every call of the library that needs scratch takes it
+// from its context; nothing is kept across calls except the capacity.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  hipError_t reserve(size_t need);
+  void release();
+};
+
+struct Ctx {
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int cu_count = 256;
+  DevBuf rec64, rec32, part, pflag, expos, exptr32, exseg32;
+  DevBuf tree[24];           // tree build / moments / evaluation scratch
+  std::vector<char> host;    // host staging
+  hipEvent_t ev[12] = {};
+  int nev = 0;               // events recorded by the last call (phase k = ev[k] -> ev[k+1])
+  int64_t stats_i[8] = {};
+  hipError_t mark() { return nev < 12 ? hipEventRecord(ev[nev++], stream) : hipSuccess; }
+};
+
+#define OFW_HIP(expr)                                                                              \
+  do {                                                                                             \
+    hipError_t _e = (expr);                                                                        \
+    if (_e != hipSuccess) {                                                                        \
+      ::ofw::set_error(std::string(#expr " failed: ") + hipGetErrorString(_e));                    \
+      return OFW_ERR_HIP;                                                                          \
+    }                                                                                              \
+  } while (0)
+
+// direct.hip
+int pack_segments(Ctx& c, int64_t m, const double* starts, const double* directions, const double* lengths,
+                  const double* gamma, const double* radius, int model, double* rec64, float* rec32);
+int direct_sum(Ctx& c, int precision, int block, int64_t n, const double* points, int64_t m, const double* rec64,
+               const float* rec32, int model, int raise_singular, const int32_t* ex_ptr, const int32_t* ex_seg,
+               double* out, int8_t* flags, int32_t* flag_seg);
+int csr_to_i32(Ctx& c, int64_t n, const int64_t* ex_ptr, const int64_t* ex_seg, int32_t** ptr32, int32_t** seg32);
+
+}  // namespace ofw

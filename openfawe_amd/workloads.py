@@ -1,0 +1,205 @@
+"""Seeded synthetic wakes: the inputs of the parity tests and of bench.py.
+
+* ``sheet`` -- the reference's synthetic sheet (apps/studio/openfawe_studio/selftest.py:243-272), the
+  generator of the reference's GPU tree speed table: 110 nodes per row, spanwise and trailing segments,
+  20,570 off-node targets; 468 / 936 / 1,872 / 3,744 rows give 102,382 / 204,874 / 409,858 / 819,826
+  segments.
+* ``loop`` -- a crosswind-loop wake: rows of wing shed and trailing segments along a circular loop of
+  radius 145 m convected downwind, plus 8 rotor tip-vortex helices; targets are the first 187 rows of
+  wake nodes with their incident segments excluded (the wake-on-wake convection call of wake.py:404-491).
+* ``helix`` -- 40 rows x 256 spanwise segments on a helix of radius 150 m and pitch 40 m, Vatistas cores.
+* ``random_network`` -- the reference's backend-equivalence network (tests/test_vortex_numba.py:38-54).
+* ``lattice`` and ``ring`` -- the reference's tree-test geometries (tests/test_vortex_tree.py:18-40).
+
+Every generator returns a dict with the engine inputs: points, starts, directions, lengths, gamma,
+radius (float64), pairs ((e,2) int64 exclusions), model.
+"""
+import numpy as np
+
+
+def _segments(nodes, edges):
+    starts = nodes[edges[:, 0]]
+    vector = nodes[edges[:, 1]] - starts
+    length = np.hypot(np.hypot(vector[:, 0], vector[:, 1]), vector[:, 2])
+    return starts, vector / length[:, None], length
+
+
+def incident_pairs(point_nodes, edges):
+    """(point, segment) pairs that exclude every segment incident to a target node.
+
+    ``point_nodes[i]`` is the node index of target i.  The rule of wake.py:427-438 (incident_pairs).
+    """
+    lookup = {int(n): i for i, n in enumerate(point_nodes)}
+    rows = []
+    for s, (a, b) in enumerate(edges):
+        for node in (int(a), int(b)):
+            i = lookup.get(node)
+            if i is not None:
+                rows.append((i, s))
+    return np.asarray(rows, dtype=np.int64).reshape(-1, 2)
+
+
+def sheet(rows, nodes_per_row=110, span=26.0, dx=0.45, seed=None):
+    """The reference's synthetic sheet; seed defaults to ``rows`` as in its GPU self-test."""
+    rng = np.random.default_rng(rows if seed is None else seed)
+    y = np.linspace(-span / 2, span / 2, nodes_per_row)
+    grid = np.zeros((rows, nodes_per_row, 3))
+    for r in range(rows):
+        s = r * dx
+        grid[r, :, 0] = s + 0.02 * rng.standard_normal(nodes_per_row)
+        grid[r, :, 1] = y * (1 + 0.002 * r) + 0.5 * np.sin(0.01 * r)
+        grid[r, :, 2] = -0.03 * s + 0.3 * np.sin(y / 3.0 + 0.02 * r)
+    starts, ends = [], []
+    for r in range(rows):
+        starts.append(grid[r, :-1])
+        ends.append(grid[r, 1:])
+        if r + 1 < rows:
+            starts.append(grid[r])
+            ends.append(grid[r + 1])
+    starts, ends = np.concatenate(starts), np.concatenate(ends)
+    vector = ends - starts
+    length = np.linalg.norm(vector, axis=1)
+    targets = grid[:min(rows, 187)].reshape(-1, 3) + np.array([0.0, 0.0, 0.011])
+    return dict(points=targets, starts=starts, directions=vector / length[:, None], lengths=length,
+                gamma=np.linspace(0.5, 1.5, len(starts)), radius=np.full(len(starts), 0.075),
+                pairs=np.zeros((0, 2), dtype=np.int64), model=2, name=f"sheet-{rows}")
+
+
+def loop(rows=468, seed=0, nodes_per_row=110, span=26.0, radius_loop=145.0, wind=10.0, rows_per_loop=468,
+         phase=0.0, kite_speed=50.0, rotor_segments=64, target_rows=187, core=0.1, rotor_core=0.05):
+    """Crosswind-loop wake (see the module docstring); ``rows / rows_per_loop`` loops long."""
+    rng = np.random.default_rng(seed)
+    period = 2 * np.pi * radius_loop / kite_speed
+    dt = period / rows_per_loop
+    # row r was shed r steps ago at loop angle phase - omega * age and has drifted downwind since
+    age = np.arange(rows) * dt
+    angle = phase - 2 * np.pi * age / period
+    centre = np.stack([wind * age, radius_loop * np.cos(angle), radius_loop * np.sin(angle)], axis=1)
+    radial = np.stack([np.zeros(rows), np.cos(angle), np.sin(angle)], axis=1)
+    eta = np.linspace(-0.5, 0.5, nodes_per_row)
+    nodes = centre[:, None, :] + span * eta[None, :, None] * radial[:, None, :]
+    nodes = nodes + 0.02 * rng.standard_normal(nodes.shape)
+    nodes = nodes.reshape(-1, 3)
+    # panel circulation: seeded elliptic spanwise profile, peak 8 m^2/s, small per-row modulation
+    mid_eta = 0.5 * (eta[:-1] + eta[1:])
+    profile = 8.0 * np.sqrt(np.clip(1 - (2 * mid_eta) ** 2, 0, None))
+    panel = profile[None, :] * (1 + 0.05 * rng.standard_normal((rows, nodes_per_row - 1)))
+    edges, gamma = [], []
+    for r in range(rows):
+        base = r * nodes_per_row
+        older = panel[r + 1] if r + 1 < rows else np.zeros(nodes_per_row - 1)
+        for j in range(nodes_per_row - 1):            # shed (spanwise) segments
+            edges.append((base + j, base + j + 1))
+            gamma.append(panel[r, j] - older[j])
+        if r + 1 < rows:                                # trailing segments towards the older row
+            padded = np.concatenate([[0.0], panel[r], [0.0]])
+            for j in range(nodes_per_row):
+                edges.append((base + j, base + nodes_per_row + j))
+                gamma.append(padded[j + 1] - padded[j])
+    edges = np.asarray(edges, dtype=np.int64)
+    gamma = np.asarray(gamma)
+    radius = np.full(len(edges), core)
+    # 8 rotors x 4 blades: tip-vortex helices of radius 1.15 m trailing from the rotors
+    rotor_nodes, rotor_edges, rotor_gamma = [], [], []
+    offset = len(nodes)
+    flight = np.stack([np.zeros(rows), -np.sin(angle), np.cos(angle)], axis=1)
+    hub_eta = np.linspace(-0.35, 0.35, 8)
+    k = np.arange(rotor_segments + 1)
+    for rotor in range(8):
+        for blade in range(4):
+            idx = np.minimum(k // 2, rows - 1)
+            hub = centre[idx] + span * hub_eta[rotor] * radial[idx] + 1.5 * flight[idx]
+            spin = 0.7 * k + 0.5 * np.pi * blade + rng.uniform(0, 2 * np.pi)
+            e1, e2 = radial[idx], np.cross(flight[idx], radial[idx])
+            tip = hub + 1.15 * (np.cos(spin)[:, None] * e1 + np.sin(spin)[:, None] * e2)
+            rotor_nodes.append(tip)
+            first = offset + sum(len(v) for v in rotor_nodes[:-1])
+            for j in range(rotor_segments):
+                rotor_edges.append((first + j, first + j + 1))
+            rotor_gamma.append(np.full(rotor_segments, rng.uniform(0, 2)))
+    nodes = np.concatenate([nodes, *rotor_nodes])
+    edges = np.concatenate([edges, np.asarray(rotor_edges, dtype=np.int64)])
+    gamma = np.concatenate([gamma, *rotor_gamma])
+    radius = np.concatenate([radius, np.full(len(rotor_edges), rotor_core)])
+    starts, directions, lengths = _segments(nodes, edges)
+    target_nodes = np.arange(min(rows, target_rows) * nodes_per_row)
+    return dict(points=nodes[target_nodes].copy(), starts=starts, directions=directions, lengths=lengths,
+                gamma=gamma, radius=radius, pairs=incident_pairs(target_nodes, edges), model=2,
+                nodes=nodes, edges=edges, name=f"loop-{rows}")
+
+
+def helix(seed=0, rows=40, spanwise=256, radius_helix=150.0, pitch=40.0, span=26.0, row_spacing=2.0,
+          jitter=0.2, targets=2048, wing_points=100, core=0.05):
+    """40 rows x 256 spanwise-connected segments on a helix, Vatistas n=2 cores, 2,048 targets."""
+    rng = np.random.default_rng(seed)
+    dphi = row_spacing / np.hypot(radius_helix, pitch / (2 * np.pi))
+    phi = dphi * np.arange(rows)
+    centre = np.stack([radius_helix * np.cos(phi), radius_helix * np.sin(phi), pitch * phi / (2 * np.pi)], axis=1)
+    radial = np.stack([np.cos(phi), np.sin(phi), np.zeros(rows)], axis=1)
+    eta = np.linspace(-0.5, 0.5, spanwise + 1)
+    nodes = centre[:, None, :] + span * eta[None, :, None] * radial[:, None, :]
+    nodes = (nodes + rng.normal(0.0, jitter, nodes.shape)).reshape(-1, 3)
+    edges = np.asarray([(r * (spanwise + 1) + j, r * (spanwise + 1) + j + 1)
+                        for r in range(rows) for j in range(spanwise)], dtype=np.int64)
+    gamma = rng.uniform(0.0, 8.0, len(edges))
+    node_targets = np.sort(rng.choice(len(nodes), targets - wing_points, replace=False))
+    # the wing: a straight 26 m line one row ahead of the helix start
+    wing_centre = np.array([radius_helix * np.cos(-dphi), radius_helix * np.sin(-dphi), pitch * -dphi / (2 * np.pi)])
+    wing_dir = np.array([np.cos(-dphi), np.sin(-dphi), 0.0])
+    wing = wing_centre + span * np.linspace(-0.5, 0.5, wing_points)[:, None] * wing_dir
+    points = np.concatenate([nodes[node_targets], wing])
+    starts, directions, lengths = _segments(nodes, edges)
+    return dict(points=points, starts=starts, directions=directions, lengths=lengths, gamma=gamma,
+                radius=np.full(len(edges), core), pairs=incident_pairs(node_targets, edges), model=3,
+                nodes=nodes, edges=edges, name="helix")
+
+
+def random_network(seed, nodes=60, segments=90, points=70):
+    """The reference's random network with near-axis targets (tests/test_vortex_numba.py:38-54)."""
+    rng = np.random.default_rng(seed)
+    xyz = rng.normal(size=(nodes, 3))
+    edges = rng.integers(0, nodes, size=(segments, 2))
+    edges = edges[edges[:, 0] != edges[:, 1]]
+    gamma = rng.normal(size=len(edges))
+    gamma[::7] = 0.
+    radius = rng.uniform(.01, .6, size=len(edges))
+    targets = rng.normal(size=(points, 3))
+    a, b = xyz[edges[:5, 0]], xyz[edges[:5, 1]]
+    normal = np.cross(b - a, [0.3, -0.2, 0.9])
+    normal /= np.linalg.norm(normal, axis=1)[:, None]
+    near = [a + .4 * (b - a) + 1e-9 * normal, a + .4 * (b - a) + 1e-13 * normal,
+            a + 1.7 * (b - a), a - .5 * (b - a), a + .5 * (b - a) + .05 * normal]
+    targets = np.vstack([targets, *near])
+    return xyz, edges, gamma, radius, targets
+
+
+def lattice(seed=11, rows=24, span=17):
+    """Wavy wake-like ring lattice with shed circulation differences (tests/test_vortex_tree.py:25-40)."""
+    rng = np.random.default_rng(seed)
+    x = np.linspace(0., 12., rows)[:, None] * np.ones((1, span))
+    y = np.ones((rows, 1)) * np.linspace(-4., 4., span)[None, :]
+    z = .3 * np.sin(.7 * x) * np.cos(.5 * y) + .05 * rng.normal(size=x.shape)
+    nodes = np.column_stack((x.ravel(), y.ravel(), z.ravel()))
+    edges, gamma = [], []
+    strength = np.cos(np.linspace(-1.3, 1.3, span - 1))[None, :] * np.exp(-.05 * np.arange(rows - 1))[:, None]
+    for i in range(rows - 1):
+        for j in range(span - 1):
+            a, b, c, d = i * span + j, i * span + j + 1, (i + 1) * span + j + 1, (i + 1) * span + j
+            for p, q in ((a, b), (b, c), (c, d), (d, a)):
+                edges.append((p, q))
+                gamma.append(strength[i, j])
+    return nodes, np.asarray(edges), np.asarray(gamma)
+
+
+def ring(radius=1.0, count=64, centre=(0., 0., 0.)):
+    """Polygonal ring of ``count`` segments (tests/test_vortex_tree.py:18-22)."""
+    angle = np.linspace(0., 2 * np.pi, count, endpoint=False)
+    nodes = np.column_stack((radius * np.cos(angle), radius * np.sin(angle), np.zeros(count))) + centre
+    edges = np.column_stack((np.arange(count), (np.arange(count) + 1) % count))
+    return nodes, edges
+
+
+def engine_args(case):
+    """(points, starts, directions, lengths, gamma, radius, model) of a generator's case."""
+    return (case["points"], case["starts"], case["directions"], case["lengths"], case["gamma"], case["radius"],
+            case["model"])
