@@ -1,15 +1,21 @@
 """Benchmark of the free-wake induced-velocity path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
 
 A step is one induced-velocity call over one synthetic wake whose inputs are already resident in HBM:
-targets (n,3), segment starts / directions / lengths / circulations / core radii.  The timed region is
+targets (n,3), segment starts / directions / lengths / circulations / core radii and the exclusion CSR.
+For the tree code a call is the whole pipeline (tree build, node moments, traversal).  The timed region is
 K calls bracketed by a device synchronisation on both sides.  One JSON line is printed.
 
 Workloads (--workload):
-  direct-fp64-102k  FP64 direct sum, 20,570 targets x 102,382 segments (the reference's 468-row sheet)
+  tree-mixed-820k   (default) mixed-precision tree, 20,570 targets x 821,k segments: crosswind-loop wake of
+                    3,744 rows over 8 loops + 8 rotor helices, theta 0.5, order 2, leaf 32, Lamb-Oseen cores
+  tree-mixed-410k   the same generator with 1,872 rows
+  tree-fp64-102k    FP64 tree, 468 rows (one loop)
+  sheet-mixed-820k  the reference's synthetic sheet (3,744 rows, 819,826 segments), theta 0.2
+  direct-fp64-102k  FP64 direct sum over the reference's 468-row sheet (20,570 x 102,382)
   direct-mixed-102k mixed-precision direct sum, same wake
-The headline metric is pair-interactions/s = n*m / (time per call).
+The headline metric is pair-interactions/s = n * m / (time per call), with ms per call.
 """
 import argparse
 import json
@@ -27,7 +33,22 @@ sys.path.insert(0, str(ROOT))
 FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 vector
 FP32_PEAK_TFLOPS = 157.3    # MI355X FP32 vector (packed); non-packed issue rate is half of it
 HBM_PEAK_GBS = 8000.0
-OPS_PER_PAIR = {0: 67, 1: 71, 2: 69, 3: 75, 4: 70, 5: 86}   # algorithmic ops per far pair by core model
+# algorithmic ops per pair by core model (far pairs, outside the Lamb-Oseen cutoff), per accepted order-2
+# node, per visited node (acceptance test)
+OPS_PER_PAIR = {0: 67, 1: 71, 2: 69, 3: 75, 4: 70, 5: 86}
+OPS_PER_FAR = 178 + 4
+OPS_PER_VISIT = 13
+
+WORKLOADS = {
+    "tree-mixed-820k": dict(kind="tree", gen="loop", rows=3744, precision="mixed", theta=0.5),
+    "tree-mixed-410k": dict(kind="tree", gen="loop", rows=1872, precision="mixed", theta=0.5),
+    "tree-fp64-102k": dict(kind="tree", gen="loop", rows=468, precision="fp64", theta=0.5),
+    "tree-fp64-820k": dict(kind="tree", gen="loop", rows=3744, precision="fp64", theta=0.5),
+    "sheet-mixed-820k": dict(kind="tree", gen="sheet", rows=3744, precision="mixed", theta=0.2),
+    "sheet-fp64-102k": dict(kind="tree", gen="sheet", rows=468, precision="fp64", theta=0.2),
+    "direct-fp64-102k": dict(kind="direct", gen="sheet", rows=468, precision="fp64"),
+    "direct-mixed-102k": dict(kind="direct", gen="sheet", rows=468, precision="mixed"),
+}
 
 
 def usable_cpus():
@@ -56,36 +77,41 @@ def cpu_model():
     return "unknown"
 
 
-def make_case(workload):
+def make_case(spec):
     from openfawe_amd import workloads
-    if workload.endswith("102k"):
-        return workloads.sheet(468)
-    if workload.endswith("410k"):
-        return workloads.sheet(1872)
-    if workload.endswith("820k"):
-        return workloads.sheet(3744)
-    raise SystemExit(f"unknown workload {workload}")
+    return workloads.sheet(spec["rows"]) if spec["gen"] == "sheet" else workloads.loop(rows=spec["rows"])
 
 
-def cpu_baseline(case, workload, threads):
+def cpu_baseline(case, spec, threads):
     """The oracle (the reference algorithm restated in C, OpenMP over targets) on a bounded sample."""
     from oracle import oracle as O
     n, m = len(case["points"]), len(case["starts"])
     args = (case["starts"], case["directions"], case["lengths"], case["gamma"], case["radius"], case["model"])
+    csr = O.exclusion_csr(n, case["pairs"])
+    if spec["kind"] == "tree":
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            O.tree_sum(case["points"], *args, csr, "raise", spec["theta"], 2, 32, 3.0, threads=threads)
+            times.append(time.perf_counter() - t0)
+        dt = float(np.median(times))
+        return dict(value=n * m / dt, unit="pair-interactions/s", cores=threads, kind="port", ms_per_call=1e3 * dt,
+                    sample=f"oracle tree_sum (C, -O2, OpenMP over targets; build, moments, traversal) on the whole "
+                           f"workload, median of 3 calls; {cpu_model()}")
     sample = 512
     while True:
         pts = case["points"][:sample]
+        sub = O.exclusion_csr(sample, case["pairs"][case["pairs"][:, 0] < sample])
         t0 = time.perf_counter()
-        O.direct_sum(pts, *args, None, "raise", threads=threads)
+        O.direct_sum(pts, *args, sub, "raise", threads=threads)
         dt = time.perf_counter() - t0
         if dt > 4.0 or sample >= n:
             break
         sample = min(n, sample * 4)
     rate = sample * m / dt
-    return dict(value=rate, unit="pair-interactions/s", cores=threads, kind="port",
+    return dict(value=rate, unit="pair-interactions/s", cores=threads, kind="port", ms_per_call=1e3 * n * m / rate,
                 sample=f"oracle direct sum (C, -O2, OpenMP) over the first {sample} of {n} targets x all {m} "
-                       f"segments, {dt:.2f} s; {cpu_model()}",
-                ms_per_call_extrapolated=1e3 * n * m / rate)
+                       f"segments, {dt:.2f} s, extrapolated to the whole workload; {cpu_model()}")
 
 
 def main():
@@ -93,23 +119,26 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="direct-fp64-102k")
+    ap.add_argument("--workload", default="tree-mixed-820k", choices=sorted(WORKLOADS))
+    ap.add_argument("--split-depth", type=int, default=-1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this build "
-              f"(targets shard without communication; see DESIGN.md)", file=sys.stderr)
+              f"(targets shard across GPUs without communication; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
     import torch
     from openfawe_amd import _lib
+    from openfawe_amd import tree as T
     from openfawe_amd import vortex as V
 
+    spec = WORKLOADS[args.workload]
     torch.cuda.set_device(0)
-    case = make_case(args.workload)
+    case = make_case(spec)
     n, m = len(case["points"]), len(case["starts"])
-    precision = "mixed" if "mixed" in args.workload else "fp64"
-    settings = V.HipSettings(precision, 256)
+    settings = V.HipSettings(spec["precision"], 256)
+    tree = V.TreeSettings(spec.get("theta", 0.5), 2, 32, 3.0)
     dev = lambda a, dt=torch.float64: torch.from_numpy(np.ascontiguousarray(a)).to("cuda", dt)  # noqa: E731
     pts, starts, dirs = dev(case["points"]), dev(case["starts"]), dev(case["directions"])
     lens, gam, rad = dev(case["lengths"]), dev(case["gamma"]), dev(case["radius"])
@@ -118,42 +147,65 @@ def main():
     seg = dev(seg if len(seg) else np.zeros(1, np.int64), torch.int64)
     out = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
     flags = torch.zeros(n, dtype=torch.int8, device="cuda")
+    far = torch.zeros(n, dtype=torch.int64, device="cuda")
+    near = torch.zeros(n, dtype=torch.int64, device="cuda")
     fseg = torch.zeros(n, dtype=torch.int32, device="cuda")
 
-    def step():
-        V.hip_sum_device(pts, starts, dirs, lens, gam, rad, case["model"], ptr, seg, "raise", settings,
-                         out, flags, fseg)
+    if spec["kind"] == "tree":
+        def step():
+            T.hip_tree_sum_device(pts, starts, dirs, lens, gam, rad, case["model"], ptr, seg, "raise", tree, settings,
+                                  out, flags, far, near, split_depth=args.split_depth)
+    else:
+        def step():
+            V.hip_sum_device(pts, starts, dirs, lens, gam, rad, case["model"], ptr, seg, "raise", settings,
+                             out, flags, fseg)
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    kernel_ms = []
+    phases = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
-        kernel_ms.append(_lib.timings(V._ctx())[1])
+        phases.append(_lib.timings(V._ctx()))
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ms = 1e3 * elapsed / args.steps
-    pairs = n * m - len(case["pairs"])
     value = n * m / (ms * 1e-3)
-    kms = float(np.mean(kernel_ms))
-    flops = pairs * OPS_PER_PAIR[case["model"]]
+    phase_ms = np.mean(np.array(phases), axis=0).tolist()
+    fp64 = spec["precision"] == "fp64"
+    peak = FP64_PEAK_TFLOPS if fp64 else FP32_PEAK_TFLOPS / 2
+    if spec["kind"] == "tree":
+        far_n = int(far.sum().item())
+        near_n = int(near.sum().item())
+        flops = far_n * OPS_PER_FAR + near_n * OPS_PER_PAIR[case["model"]] + (far_n + near_n // 16) * OPS_PER_VISIT
+        kms = phase_ms[3]
+        names = ["build (prep + level build)", "node moments", "exclusions", "traversal", "reduce"]
+        metric = "pair-interactions/s (tree code, direct-sum equivalent n*m/t)"
+        kernel = "k_tree_eval"
+        extra = dict(far_nodes_per_target=far_n / n, near_pairs_per_target=near_n / n,
+                     phases_ms=dict(zip(names, phase_ms)))
+    else:
+        pairs = n * m - len(case["pairs"])
+        flops = pairs * OPS_PER_PAIR[case["model"]]
+        kms = phase_ms[1]
+        metric = "pair-interactions/s (direct sum, n*m/t)"
+        kernel = "k_direct64" if fp64 else "k_direct_mixed"
+        extra = dict(phases_ms=dict(zip(["pack", "pair kernel", "chunk reduce"], phase_ms)))
     achieved = flops / (kms * 1e-3) / 1e12
-    peak = FP64_PEAK_TFLOPS if precision == "fp64" else FP32_PEAK_TFLOPS / 2
     result = dict(
-        metric="pair-interactions/s (direct sum, 20,570 targets x 102k segments)",
-        value=value, unit="pair-interactions/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
+        metric=metric, value=value, unit="pair-interactions/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
         ms_per_step=ms, higher_is_better=True, scaling="weak", vs_baseline=None,
-        dtype="f64" if precision == "fp64" else "f32/f64", data="synthetic (seeded sheet wake of the reference)",
-        config=dict(workload=args.workload, targets=n, segments=m, core_model="lamb-oseen", precision=precision),
-        roofline=dict(bound="valu-fp64" if precision == "fp64" else "valu-fp32", achieved=achieved, peak=peak,
-                      unit="TFLOP/s", frac=achieved / peak, traffic=None,
-                      kernel="k_direct64" if precision == "fp64" else "k_direct_mixed", kernel_ms=kms,
-                      ops_per_pair=OPS_PER_PAIR[case["model"]]),
+        dtype="f64" if fp64 else "f32+f64", data=f"synthetic ({case['name']} wake, seeded)",
+        config=dict(workload=args.workload, targets=n, segments=m, core_model="lamb-oseen",
+                    precision=spec["precision"], **({"theta": spec["theta"], "order": 2, "leaf_size": 32,
+                                                     "core_clearance": 3.0} if spec["kind"] == "tree" else {})),
+        roofline=dict(bound="valu-fp64" if fp64 else "valu-fp32", achieved=achieved, peak=peak, unit="TFLOP/s",
+                      frac=achieved / peak, traffic=None, kernel=kernel, kernel_ms=kms),
+        **extra,
     )
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(case, args.workload, usable_cpus())
+        result["cpu_baseline"] = cpu_baseline(case, spec, usable_cpus())
     print(json.dumps(result), flush=True)
 
 

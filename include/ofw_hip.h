@@ -83,6 +83,42 @@ int ofw_direct_sum(ofw_ctx *ctx, int precision, int block, int64_t n, const doub
                    const double *radius, int model, int raise_singular, const int64_t *ex_ptr,
                    const int64_t *ex_seg, double *out, int8_t *flags, int32_t *flag_seg);
 
+/* Counters of the context's last call (host buffer of `count` int64): [0] direct-sum segment chunks,
+ * [1] segments per chunk, [2] traversal subtrees, [3] traversal split depth, [4] tree nodes, [5] tree depth.
+ * Returns the number written. */
+int ofw_ctx_stats(ofw_ctx *ctx, int64_t *stats_host, int count);
+
+/* Median-split binary tree over segment midpoints mid (m,3), element for element the reference's
+ * vortex_tree.build_tree (vortex_tree.py:208-241; serial form :51-115): order (m), and lo, hi, left,
+ * right (node arrays of capacity 4*(m//leaf_size+1)+1, reference numbering, -1 children for leaves).
+ * The node count is written to *count_host. */
+int ofw_build_tree(ofw_ctx *ctx, int64_t m, const double *mid, int64_t leaf_size, int64_t *order, int64_t *lo,
+                   int64_t *hi, int64_t *left, int64_t *right, int64_t *count_host);
+
+/* Node quantities of the tree built on the segments: centre (c,3), bound (c), core (c), m0 (c,3),
+ * m1 (c,3,3), m2 (c,3,3,3), w1 (c,3), t2 (c,3), w2 (c,3,3) in the reference numbering (arrays of the
+ * capacity of ofw_build_tree).  Replaces vortex_tree.node_moments + node_contractions
+ * (vortex_tree.py:309-376, 425-447) and vortex_cuda_moments.device_node_quantities
+ * (vortex_cuda_moments.py:206-230).  Centre, bound and core are bit-exact; the moments are sums in
+ * another order (equal to rounding). */
+int ofw_tree_nodes(ofw_ctx *ctx, int64_t m, const double *starts, const double *directions, const double *lengths,
+                   const double *gamma, const double *radius, int64_t leaf_size, double *centre, double *bound,
+                   double *core, double *m0, double *m1, double *m2, double *w1, double *t2, double *w2,
+                   int64_t *count_host);
+
+/* Tree-code sum: build, node quantities and traversal on the GPU.  Replaces vortex_tree.tree_sum
+ * (vortex_tree.py:588-626) and vortex_cuda_tree.cuda_tree_sum (vortex_cuda_tree.py:439-595).
+ * theta, expansion_order, leaf_size, clearance: TreeSettings (vortex.py:30-55).  model must be 0, 1 or 2
+ * (OFW_ERR_UNSUPPORTED otherwise, as vortex_tree.py:593-594 when a core is present).  The accepted nodes
+ * are the reference's (FP64 acceptance test in every precision).  split_depth: depth of the traversal's
+ * subtree frontier (-1 = chosen from the target count).  far_counts / near_counts (optional, may be NULL):
+ * accepted nodes and evaluated near pairs per target. */
+int ofw_tree_sum(ofw_ctx *ctx, int precision, int64_t n, const double *points, int64_t m, const double *starts,
+                 const double *directions, const double *lengths, const double *gamma, const double *radius,
+                 int model, int raise_singular, const int64_t *ex_ptr, const int64_t *ex_seg, double theta,
+                 int expansion_order, int64_t leaf_size, double clearance, int split_depth, double *out,
+                 int8_t *flags, int64_t *far_counts, int64_t *near_counts);
+
 #ifdef __cplusplus
 }
 #endif

@@ -5,10 +5,6 @@
 #include "common.h"
 #include "internal.h"
 
-struct ofw_ctx {
-  ofw::Ctx c;
-};
-
 namespace ofw {
 
 static thread_local std::string g_error;
@@ -106,6 +102,16 @@ int ofw_ctx_timings(ofw_ctx* x, float* ms_host, int count) {
     if (k < phases) OFW_HIP(hipEventElapsedTime(&ms_host[k], c.ev[k], c.ev[k + 1]));
   }
   return phases;
+}
+
+int ofw_ctx_stats(ofw_ctx* x, int64_t* stats_host, int count) {
+  if (!x || !stats_host) {
+    set_error("ofw_ctx_stats: null argument");
+    return OFW_ERR_ARG;
+  }
+  int k = 0;
+  for (; k < count && k < 8; ++k) stats_host[k] = x->c.stats_i[k];
+  return k;
 }
 
 int ofw_device_info(char* name_host, int name_len, int* cu_count_host) {

@@ -11,6 +11,7 @@
 
 #include "common.h"
 #include "internal.h"
+#include "tree_records.h"
 
 namespace ofw {
 
@@ -23,44 +24,8 @@ __global__ __launch_bounds__(256) void k_pack_segments(int m, const double* __re
                                                        double* __restrict__ rec64, float* __restrict__ rec32) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= m) return;
-  const double L = lengths[s], g = gamma[s], rad = radius[s];
-  const double invL = 1.0 / L;
-  const double g4 = (g / L) / OFW_FOUR_PI;
-  const double rc = rad / L;
-  const double rc2 = rc * rc;
-  const bool cored = rad > 0.0 && model != OFW_MODEL_NONE;
-  const bool ratio_model = model == OFW_MODEL_RANKINE || model == OFW_MODEL_LAMB || model == OFW_MODEL_VATISTAS;
-  const double exact = (cored && ratio_model && !(rc >= OFW_FAST_RC_MIN)) ? 1.0 : 0.0;
-  double* r = rec64 + (size_t)s * REC64;
-  r[R_SX] = starts[3 * (size_t)s];
-  r[R_SY] = starts[3 * (size_t)s + 1];
-  r[R_SZ] = starts[3 * (size_t)s + 2];
-  r[R_DX] = directions[3 * (size_t)s];
-  r[R_DY] = directions[3 * (size_t)s + 1];
-  r[R_DZ] = directions[3 * (size_t)s + 2];
-  r[R_L] = L;
-  r[R_INVL] = invL;
-  r[R_G4] = g4;
-  r[R_RC] = rc;
-  r[R_RC2] = cored ? rc2 : 0.0;
-  r[R_GAMMA] = g;
-  r[R_RADIUS] = rad;
-  r[R_EXACT] = exact;
-  r[R_PAD0] = 0.0;
-  r[R_PAD1] = 0.0;
-  // FP32 record: a segment whose factors leave the normal FP32 range always takes the FP64 path.
-  const double ag = fabs(g4);
-  const bool f32_ok = exact == 0.0 && invL < 1e30 && invL > 1e-30 && ag < 1e30 && (ag == 0.0 || ag > 1e-30) &&
-                      (!cored || (rc2 < 1e30 && rc2 > 1e-30));
-  float* f = rec32 + (size_t)s * REC32;
-  f[F_DX] = (float)r[R_DX];
-  f[F_DY] = (float)r[R_DY];
-  f[F_DZ] = (float)r[R_DZ];
-  f[F_INVL] = (float)invL;
-  f[F_G4] = (float)g4;
-  f[F_RC2] = cored ? (float)rc2 : 0.0f;
-  f[F_EXACT] = f32_ok ? 0.0f : 1.0f;
-  f[F_PAD0] = 0.0f;
+  pack_record(s, starts, directions, lengths, gamma, radius, model, rec64 + (size_t)s * REC64,
+              rec32 + (size_t)s * REC32);
 }
 
 int pack_segments(Ctx& c, int64_t m, const double* starts, const double* directions, const double* lengths,

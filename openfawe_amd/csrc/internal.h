@@ -52,3 +52,7 @@ int direct_sum(Ctx& c, int precision, int block, int64_t n, const double* points
 int csr_to_i32(Ctx& c, int64_t n, const int64_t* ex_ptr, const int64_t* ex_seg, int32_t** ptr32, int32_t** seg32);
 
 }  // namespace ofw
+
+struct ofw_ctx {
+  ofw::Ctx c;
+};

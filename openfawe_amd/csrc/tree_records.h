@@ -1,0 +1,98 @@
+// Device-side record layouts of the direct sum and the tree code.
+#pragma once
+#include <stdint.h>
+
+#include <vector>
+
+#include "common.h"
+
+namespace ofw {
+
+// Pack one segment's FP64 and FP32 pair records (slots in common.h) from the reference's engine inputs.
+__device__ __forceinline__ void pack_record(int s, const double* __restrict__ starts,
+                                            const double* __restrict__ directions,
+                                            const double* __restrict__ lengths, const double* __restrict__ gamma,
+                                            const double* __restrict__ radius, int model, double* __restrict__ r,
+                                            float* __restrict__ f) {
+  const double L = lengths[s], g = gamma[s], rad = radius[s];
+  const double invL = 1.0 / L;
+  const double g4 = (g / L) / OFW_FOUR_PI;
+  const double rc = rad / L;
+  const double rc2 = rc * rc;
+  const bool cored = rad > 0.0 && model != OFW_MODEL_NONE;
+  const bool ratio_model = model == OFW_MODEL_RANKINE || model == OFW_MODEL_LAMB || model == OFW_MODEL_VATISTAS;
+  const double exact = (cored && ratio_model && !(rc >= OFW_FAST_RC_MIN)) ? 1.0 : 0.0;
+  r[R_SX] = starts[3 * (size_t)s];
+  r[R_SY] = starts[3 * (size_t)s + 1];
+  r[R_SZ] = starts[3 * (size_t)s + 2];
+  r[R_DX] = directions[3 * (size_t)s];
+  r[R_DY] = directions[3 * (size_t)s + 1];
+  r[R_DZ] = directions[3 * (size_t)s + 2];
+  r[R_L] = L;
+  r[R_INVL] = invL;
+  r[R_G4] = g4;
+  r[R_RC] = rc;
+  r[R_RC2] = cored ? rc2 : 0.0;
+  r[R_GAMMA] = g;
+  r[R_RADIUS] = rad;
+  r[R_EXACT] = exact;
+  r[R_PAD0] = 0.0;
+  r[R_PAD1] = 0.0;
+  // a segment whose factors leave the normal FP32 range always takes the FP64 path
+  const double ag = fabs(g4);
+  const bool f32_ok = exact == 0.0 && invL < 1e30 && invL > 1e-30 && ag < 1e30 && (ag == 0.0 || ag > 1e-30) &&
+                      (!cored || (rc2 < 1e30 && rc2 > 1e-30));
+  f[F_DX] = (float)r[R_DX];
+  f[F_DY] = (float)r[R_DY];
+  f[F_DZ] = (float)r[R_DZ];
+  f[F_INVL] = (float)invL;
+  f[F_G4] = (float)g4;
+  f[F_RC2] = cored ? (float)rc2 : 0.0f;
+  f[F_EXACT] = f32_ok ? 0.0f : 1.0f;
+  f[F_PAD0] = 0.0f;
+}
+
+// Moment inputs in tree order (SoA): start, end, alpha = gamma * length vector, length vector, radius, gamma
+struct MomentInputs {
+  double* s[3];
+  double* e[3];
+  double* a[3];
+  double* l[3];
+  double* rad;
+  double* gam;
+};
+
+// Optional full per-node outputs (pre-order node index), as node_moments / node_contractions return them
+struct NodeOutputs {
+  double* bound = nullptr;
+  double* m0 = nullptr;
+  double* m1 = nullptr;
+  double* m2 = nullptr;
+  double* w1 = nullptr;
+  double* t2 = nullptr;
+  double* w2 = nullptr;
+};
+
+// Acceptance record of a node (8 doubles, one 64-byte scalar load): centre, bounding radius,
+// clearance * largest core, and (lo, hi, skip, 0) as int32 in the last two slots but one.
+enum { A_CX = 0, A_CY, A_CZ, A_BOUND, A_CLEAR, A_INTS, A_INTS2, A_PAD, ACC_REC = 8 };
+
+// Far-field record (48 values): m0 (3), m1 (9, row-major), w1 (3), symmetric m2 (3 x 6: 00 01 02 11 12 22,
+// the off-diagonal entries doubled), t2 (3), w2 (9, row-major), pad
+enum { FR_M0 = 0, FR_M1 = 3, FR_W1 = 12, FR_M2 = 15, FR_T2 = 33, FR_W2 = 36, FR_PAD0 = 45, FR_PAD1 = 46, FR_PAD2 = 47,
+       FAR_REC = 48 };
+
+struct TreeDevice {
+  int count = 0;
+  double* rec64 = nullptr;  // pair records in tree order
+  float* rec32 = nullptr;
+  double* centre = nullptr;  // pre-order
+  double* core = nullptr;
+  double* acc = nullptr;
+  double* far64 = nullptr;
+  float* far32 = nullptr;
+  NodeOutputs out;
+  std::vector<int32_t> h_items[3], h_nodes[5];  // host staging kept alive until the call synchronises
+};
+
+}  // namespace ofw

@@ -1,0 +1,107 @@
+"""Tree code (Barnes-Hut type) on MI355X: the engine behind ``segments_velocity(backend="hip_tree")``.
+
+``hip_tree_sum`` has the signature of the reference's ``vortex_tree.tree_sum`` / ``vortex_cuda_tree.
+cuda_tree_sum`` (vortex_tree.py:588, vortex_cuda_tree.py:439) and returns (velocity, any_flag).  Build,
+node quantities and traversal all run in libofw_hip.so on the GPU.  ``build_tree`` and ``tree_nodes``
+expose the tree and its node quantities for the bit-exactness checks.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .vortex import HipSettings, Statistics, TREE_MODELS, TreeSettings, _ctx, _dev, _ptr, _torch, _upload
+
+
+def capacity(m, leaf_size):
+    """Node capacity of a tree over m segments (vortex_tree.py:59)."""
+    return 4 * (m // max(int(leaf_size), 1) + 1) + 1
+
+
+def build_tree(mid, leaf_size):
+    """vortex_tree.build_tree on the GPU: (order, lo, hi, left, right, node_count) as host int64 arrays."""
+    torch = _torch()
+    mid = np.ascontiguousarray(mid, dtype=np.float64).reshape(-1, 3)
+    m = len(mid)
+    cap = capacity(m, leaf_size)
+    d_mid = _dev(torch, mid if m else np.zeros((1, 3)), np.float64)
+    order = torch.zeros(max(m, 1), dtype=torch.int64, device="cuda")
+    arrays = [torch.full((cap,), -1, dtype=torch.int64, device="cuda") for _ in range(4)]
+    count = ctypes.c_int64(0)
+    status = _lib.load().ofw_build_tree(_ctx(), m, d_mid.data_ptr(), int(leaf_size), order.data_ptr(),
+                                        *[a.data_ptr() for a in arrays], ctypes.byref(count))
+    _lib.check(status, "ofw_build_tree")
+    c = count.value
+    return (order[:m].cpu().numpy(), *[a[:c].cpu().numpy() for a in arrays], c)
+
+
+def tree_nodes(starts, directions, lengths, gamma, radius, leaf_size):
+    """Node quantities (centre, bound, core, m0, m1, m2, w1, t2, w2) in the reference numbering."""
+    torch = _torch()
+    m = len(starts)
+    cap = capacity(m, leaf_size)
+    dev = [_dev(torch, a, np.float64) for a in (starts, directions, lengths, gamma, radius)]
+    shapes = [(cap, 3), (cap,), (cap,), (cap, 3), (cap, 3, 3), (cap, 3, 3, 3), (cap, 3), (cap, 3), (cap, 3, 3)]
+    outs = [torch.zeros(s, dtype=torch.float64, device="cuda") for s in shapes]
+    count = ctypes.c_int64(0)
+    status = _lib.load().ofw_tree_nodes(_ctx(), m, *[d.data_ptr() for d in dev], int(leaf_size),
+                                        *[o.data_ptr() for o in outs], ctypes.byref(count))
+    _lib.check(status, "ofw_tree_nodes")
+    c = count.value
+    return tuple(o[:c].cpu().numpy() for o in outs)
+
+
+def hip_tree_sum_device(points, starts, directions, lengths, gamma, radius, model, ex_ptr, ex_seg, singularity,
+                        tree, settings, out=None, flags=None, far_counts=None, near_counts=None, split_depth=-1):
+    """Tree sum on device tensors; returns (out, flags, far_counts, near_counts) device tensors."""
+    torch = _torch()
+    n, m = int(points.shape[0]), int(starts.shape[0])
+    if out is None:
+        out = torch.zeros((n, 3), dtype=torch.float64, device=points.device)
+    if flags is None:
+        flags = torch.zeros(n, dtype=torch.int8, device=points.device)
+    if far_counts is None:
+        far_counts = torch.zeros(n, dtype=torch.int64, device=points.device)
+    if near_counts is None:
+        near_counts = torch.zeros(n, dtype=torch.int64, device=points.device)
+    if n == 0:
+        return out, flags, far_counts, near_counts
+    status = _lib.load().ofw_tree_sum(
+        _ctx(), _lib.PRECISIONS[settings.precision], n, _ptr(points), m, _ptr(starts), _ptr(directions),
+        _ptr(lengths), _ptr(gamma), _ptr(radius), int(model), int(singularity == "raise"), _ptr(ex_ptr), _ptr(ex_seg),
+        float(tree.theta), int(tree.order), int(tree.leaf_size), float(tree.core_clearance), int(split_depth),
+        _ptr(out), _ptr(flags), _ptr(far_counts), _ptr(near_counts))
+    _lib.check(status, "ofw_tree_sum")
+    return out, flags, far_counts, near_counts
+
+
+def hip_tree_sum(points, starts, directions, lengths, gamma, radius, model, csr, singularity, tree, settings, *,
+                 split_depth=-1):
+    """Return (velocity, any_flag) like vortex_tree.tree_sum for validated inputs."""
+    if not isinstance(tree, TreeSettings):
+        raise ValueError("tree must be TreeSettings")
+    if settings is None:
+        settings = HipSettings("fp64")
+    radius = np.asarray(radius, dtype=np.float64)
+    if int(model) not in TREE_MODELS:
+        if np.any(radius > 0):
+            raise ValueError("tree backend supports core models NONE, RANKINE and LAMB_OSEEN only")
+        model = 0   # without a core every model is the ideal filament
+    if tree.theta >= 1.0:
+        raise ValueError("tree theta must be below one")
+    n, m = len(points), len(starts)
+    if n == 0 or m == 0:
+        Statistics.last = dict(points=n, segments=m)
+        return np.zeros((n, 3)), False
+    torch, *dev = _upload(points, starts, directions, lengths, gamma, radius, csr)
+    out, flags, far, near = hip_tree_sum_device(*dev[:6], model, dev[6], dev[7], singularity, tree, settings,
+                                                split_depth=split_depth)
+    host_flags = flags.cpu().numpy()
+    far = far.cpu().numpy()
+    stats = (ctypes.c_int64 * 8)()
+    _lib.load().ofw_ctx_stats(_ctx(), stats, 8)
+    Statistics.last = dict(points=n, segments=m, precision=settings.precision, flags=host_flags, flag_seg=None,
+                           far_counts=far, near_counts=near.cpu().numpy(), mean_far_nodes=float(far.mean()),
+                           nodes=int(stats[4]), depth=int(stats[5]), subtrees=int(stats[2]),
+                           split_depth=int(stats[3]), phases_ms=_lib.timings(_ctx()))
+    return out.cpu().numpy(), bool(np.any(host_flags))

@@ -1,0 +1,192 @@
+"""Tree code on the GPU against the oracle.
+
+Bars: the tree (order, lo, hi, left, right, count) bit-exact; node centre, bound and core bit-exact, the
+moments within 1e-12 node-relative (the reference's GPU-moments bound, tests/test_cuda_moments.py:90-105);
+the accepted-node and near-pair counts of every target identical to the oracle's (same traversal); FP64
+velocities within 1e-10 * max|u| of the oracle tree (the reference's GPU self-test bound,
+apps/studio/openfawe_studio/selftest.py:413-414); mixed precision within 0.1 x the tree's own error against
+the direct sum, in max and rms (selftest.py:415-417).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from openfawe_amd import tree as T
+from openfawe_amd import vortex as V
+from openfawe_amd import workloads
+from tests.helpers import midpoint_cases, prepare
+
+pytestmark = pytest.mark.gpu
+
+
+def tree_cases():
+    cases = [(name, mid, leaf) for name, mid, leaf in midpoint_cases()]
+    zeros = np.zeros((9, 3))
+    zeros[:, 0] = [0.0, -0.0, 1.0, -0.0, 0.0, 2.0, -1.0, 0.0, -0.0]
+    cases.append(("signed-zeros", zeros, 2))
+    for rows, leaf in ((60, 32), (468, 32), (200, 1), (468, 8)):
+        c = workloads.sheet(rows)
+        cases.append((f"sheet-{rows}-leaf-{leaf}", O.segment_arrays(c["starts"], c["directions"], c["lengths"],
+                                                                    c["gamma"])[3], leaf))
+    loop = workloads.loop(rows=60)
+    cases.append(("loop-60", O.segment_arrays(loop["starts"], loop["directions"], loop["lengths"], loop["gamma"])[3],
+                  16))
+    rng = np.random.default_rng(3)
+    ties = np.round(rng.random((9000, 3)) * 4) / 4   # many equal coordinates across big (merged) nodes
+    cases.append(("ties-9000", ties, 4))
+    return cases
+
+
+@pytest.mark.parametrize("case", tree_cases(), ids=lambda c: c[0])
+def test_build_element_for_element(case):
+    name, mid, leaf = case
+    ref = O.build_tree(mid, leaf)
+    got = T.build_tree(mid, leaf)
+    for a, b, what in zip(ref[:5], got[:5], ("order", "lo", "hi", "left", "right")):
+        assert np.array_equal(a, b), what
+    assert ref[5] == got[5]
+
+
+def segments(seed, count=400, zero_every=7):
+    rng = np.random.default_rng(seed)
+    starts = rng.random((count, 3)) * [20.0, 5.0, 2.0]
+    d = rng.normal(size=(count, 3))
+    directions = d / np.linalg.norm(d, axis=1)[:, None]
+    lengths = 0.2 + 0.3 * rng.random(count)
+    gamma = rng.normal(size=count)
+    if zero_every:
+        gamma[::zero_every] = 0.0
+    radius = rng.uniform(0.0, 0.08, count)
+    return starts, directions, lengths, gamma, radius
+
+
+def node_relative(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    scale = np.abs(a).reshape(len(a), -1).max(axis=1)
+    diff = np.abs(a - b).reshape(len(a), -1).max(axis=1)
+    return np.where(scale > 0, diff / np.where(scale > 0, scale, 1.0), diff)
+
+
+@pytest.mark.parametrize("seed,leaf,count", [(1, 8, 300), (2, 32, 300), (3, 1, 120), (4, 16, 20000)])
+def test_node_quantities(seed, leaf, count):
+    s, d, L, g, r = segments(seed, count)
+    lv, ends, alpha, mid = O.segment_arrays(s, d, L, g)
+    order, lo, hi, _, _, _ = O.build_tree(mid, leaf)
+    centre, bound, core, m0, m1, m2 = O.node_moments(order, lo, hi, s, ends, alpha, lv, r, g)
+    w1, t2, w2 = O.node_contractions(m1, m2)
+    got = T.tree_nodes(s, d, L, g, r, leaf)
+    assert np.array_equal(got[0], centre)
+    assert np.array_equal(got[1], bound)
+    assert np.array_equal(got[2], core)
+    for name, a, b in (("m0", m0, got[3]), ("m1", m1, got[4]), ("m2", m2, got[5])):
+        assert node_relative(a, b).max() <= 1e-12, name
+    for name, a, b, made_of in (("w1", w1, got[6], m1), ("t2", t2, got[7], m2), ("w2", w2, got[8], m2)):
+        scale = np.abs(made_of).reshape(len(made_of), -1).max(axis=1)
+        diff = np.abs(a - b).reshape(len(a), -1).max(axis=1)
+        assert np.all(diff <= 1e-12 * scale), name
+
+
+def run_tree(case, theta, order=2, leaf=32, clearance=3.0, precision="fp64", split_depth=-1, singularity="raise"):
+    args = workloads.engine_args(case)
+    n = len(case["points"])
+    csr = V.exclusion_csr(n, case["pairs"])
+    out, flagged = T.hip_tree_sum(*args, csr, singularity, V.TreeSettings(theta, order, leaf, clearance),
+                                  V.HipSettings(precision), split_depth=split_depth)
+    return out, flagged, dict(V.Statistics.last)
+
+
+def oracle_tree(case, theta, order=2, leaf=32, clearance=3.0, singularity="raise"):
+    args = workloads.engine_args(case)
+    csr = O.exclusion_csr(len(case["points"]), case["pairs"])
+    return O.tree_sum(*args, csr, singularity, theta, order, leaf, clearance)
+
+
+def small_cases():
+    nodes, edges, gamma = workloads.lattice()
+    s, d, L = prepare(nodes, edges)
+    rng = np.random.default_rng(2)
+    lattice = dict(points=nodes + .07 * rng.normal(size=nodes.shape), starts=s, directions=d, lengths=L, gamma=gamma,
+                   radius=np.full(len(s), .1), pairs=np.zeros((0, 2), np.int64), model=2, name="lattice")
+    incident = np.asarray([(n, k) for k, (a, b) in enumerate(edges) for n in (a, b)], dtype=np.int64)
+    lattice_nodes = dict(lattice, points=nodes, pairs=incident, name="lattice-nodes")
+    return [lattice, lattice_nodes, workloads.sheet(60), workloads.loop(rows=40)]
+
+
+@pytest.mark.parametrize("case", small_cases(), ids=lambda c: c["name"])
+@pytest.mark.parametrize("theta,order,leaf", [(0.5, 2, 32), (0.2, 2, 32), (0.4, 1, 8), (0.6, 0, 16)])
+def test_fp64_tree_matches_oracle(case, theta, order, leaf):
+    ref, fref, far_ref, near_ref = oracle_tree(case, theta, order, leaf)
+    out, flagged, stats = run_tree(case, theta, order, leaf)
+    assert not flagged and not fref.any()
+    assert np.array_equal(stats["far_counts"], far_ref)
+    assert np.array_equal(stats["near_counts"], near_ref)
+    scale = np.linalg.norm(ref, axis=1).max()
+    assert np.abs(out - ref).max() <= 1e-10 * scale
+
+
+def test_split_depth_only_reassociates():
+    case = workloads.sheet(60)
+    base, _, st0 = run_tree(case, 0.3, split_depth=0)
+    for depth in (1, 3, 6, -1):
+        out, _, st = run_tree(case, 0.3, split_depth=depth)
+        assert np.array_equal(st["far_counts"], st0["far_counts"])
+        scale = np.linalg.norm(base, axis=1).max()
+        assert np.abs(out - base).max() <= 1e-13 * scale
+
+
+def test_tiny_theta_equals_direct_and_excluded_never_leak():
+    nodes, edges, gamma = workloads.lattice(rows=12, span=9)
+    s, d, L = prepare(nodes, edges)
+    incident = np.asarray([(n, k) for k, (a, b) in enumerate(edges) for n in (a, b)], dtype=np.int64)
+    case = dict(points=nodes, starts=s, directions=d, lengths=L, gamma=gamma, radius=np.full(len(s), .1),
+                pairs=incident, model=2)
+    direct, _ = O.direct_sum(nodes, s, d, L, gamma, case["radius"], 2, O.exclusion_csr(len(nodes), incident))
+    exact, _, st = run_tree(case, 1e-9, leaf=4)
+    assert st["far_counts"].sum() == 0
+    scale = np.linalg.norm(direct, axis=1).max()
+    np.testing.assert_allclose(exact, direct, rtol=0, atol=1e-13 * scale)
+    loud, silent = gamma.copy(), gamma.copy()
+    loud[-1], silent[-1] = 1e6, 0.0
+    far_pair = np.array([[0, len(edges) - 1]], dtype=np.int64)
+    one = dict(case, points=nodes[:1], pairs=far_pair)
+    a, _, _ = run_tree(dict(one, gamma=loud), 0.6, leaf=4, singularity="zero")
+    b, _, _ = run_tree(dict(one, gamma=silent), 0.6, leaf=4, singularity="zero")
+    assert np.array_equal(a, b)
+    leaked, _, _ = run_tree(dict(one, gamma=loud, pairs=np.zeros((0, 2), np.int64)), 0.6, leaf=4, singularity="zero")
+    assert np.linalg.norm(leaked - b) > 1e3 * np.linalg.norm(b)
+
+
+def test_mixed_within_a_tenth_of_the_tree_error():
+    """selftest.py:415-417 on the reference's sheet: mixed vs FP64 tree within 0.1 x tree-vs-direct error."""
+    case = workloads.sheet(120)
+    args = workloads.engine_args(case)
+    direct, _ = O.direct_sum(*args, None, "raise")
+    ref, _, _, _ = oracle_tree(case, 0.2)
+    err_tree = np.linalg.norm(ref - direct, axis=1)
+    out, flagged, _ = run_tree(case, 0.2, precision="mixed")
+    assert not flagged
+    err = np.linalg.norm(out - ref, axis=1)
+    assert err.max() <= 0.1 * err_tree.max()
+    assert np.sqrt(np.mean(err ** 2)) <= 0.1 * np.sqrt(np.mean(err_tree ** 2))
+
+
+def test_api_errors_flags_and_determinism():
+    nodes, edges, gamma = workloads.lattice(rows=6, span=5)
+    point = .5 * (nodes[edges[3, 0]] + nodes[edges[3, 1]])[None]
+    settings = V.TreeSettings(theta=.4, order=2, leaf_size=2)
+    with pytest.raises(V.FilamentSingularity):
+        V.segments_velocity(point, nodes, edges, gamma, backend="hip_tree", tree=settings)
+    for model in (3, 4, 5):
+        with pytest.raises(ValueError):
+            V.segments_velocity(point + 1., nodes, edges, gamma, model=model, core_radius=.1, backend="hip_tree",
+                                tree=settings)
+    ok = V.segments_velocity(point + 1., nodes, edges, gamma, model=3, core_radius=0.0, backend="hip_tree",
+                             tree=settings)
+    ideal = V.segments_velocity(point + 1., nodes, edges, gamma, backend="hip_tree", tree=settings)
+    assert np.array_equal(ok, ideal)
+    case = workloads.sheet(40)
+    a, _, _ = run_tree(case, 0.5, precision="mixed")
+    b, _, _ = run_tree(case, 0.5, precision="mixed")
+    assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+    empty = V.segments_velocity(np.empty((0, 3)), nodes, edges, gamma, backend="hip_tree", tree=settings)
+    assert empty.shape == (0, 3)
