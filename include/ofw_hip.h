@@ -119,6 +119,33 @@ int ofw_tree_sum(ofw_ctx *ctx, int precision, int64_t n, const double *points, i
                  int expansion_order, int64_t leaf_size, double clearance, int split_depth, double *out,
                  int8_t *flags, int64_t *far_counts, int64_t *near_counts);
 
+/* Unit-circulation velocity of every segment at every target: out (n,m,3), flags (n) (first flag in
+ * segment order).  Replaces vortex_numba.pair_matrix (vortex_numba.py:271-287).  n <= 65,535. */
+int ofw_pair_matrix(ofw_ctx *ctx, int64_t n, const double *points, int64_t m, const double *starts,
+                    const double *directions, const double *lengths, const double *radius, int model,
+                    int raise_singular, double *out, int8_t *flags);
+
+/* out[p, j, :] = sum over k in [column_ptr[j], column_ptr[j+1]) of column_sign[k] * pairs[p, column_segment[k], :],
+ * added sequentially from 0.0 in k order.  Replaces vortex_numba.assemble_columns (vortex_numba.py:290-308). */
+int ofw_assemble_columns(ofw_ctx *ctx, int64_t n, int64_t m, const double *pairs, int64_t columns,
+                         const int64_t *column_ptr, const int64_t *column_segment, const double *column_sign,
+                         double *out);
+
+/* Segment geometry from end points (the preparation of vortex.py:460-465): directions (m,3) and lengths (m)
+ * from starts (m,3) and ends (m,3).  Returns OFW_ERR_ARG ("zero-length segment") when a length is 0. */
+int ofw_segment_geometry(ofw_ctx *ctx, int64_t m, const double *starts, const double *ends, double *directions,
+                         double *lengths);
+
+/* Adjoint of the direct sum (no counterpart in the reference, which has no derivatives of the sum; its
+ * VSM Jacobian vsm.py:333-349 consumes dV/dGamma built from pair_matrix).  Given grad_out = dL/dV (n,3):
+ * grad_gamma (m) = dL/dGamma, grad_starts / grad_ends (m,3) = dL/d(segment end points), and optionally
+ * grad_points (n,3) = dL/d(targets) and the fused forward velocities out (n,3) (either may be NULL).
+ * Segment s runs from starts[s] to ends[s]; models 0-4 (the Rosenhead-Moore core is refused). */
+int ofw_direct_vjp(ofw_ctx *ctx, int64_t n, const double *points, int64_t m, const double *starts, const double *ends,
+                   const double *gamma, const double *radius, int model, const int64_t *ex_ptr,
+                   const int64_t *ex_seg, const double *grad_out, double *grad_gamma, double *grad_starts,
+                   double *grad_ends, double *grad_points, double *out);
+
 #ifdef __cplusplus
 }
 #endif

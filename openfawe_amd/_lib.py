@@ -34,6 +34,11 @@ SIGNATURES = {
                        + [ctypes.POINTER(_c_i64)]),
     "ofw_tree_sum": (_c_i, [_c_p, _c_i, _c_i64, _c_p, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i, _c_i, _c_p, _c_p,
                             _c_d, _c_i, _c_i64, _c_d, _c_i, _c_p, _c_p, _c_p, _c_p]),
+    "ofw_pair_matrix": (_c_i, [_c_p, _c_i64, _c_p, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_i, _c_i, _c_p, _c_p]),
+    "ofw_assemble_columns": (_c_i, [_c_p, _c_i64, _c_i64, _c_p, _c_i64, _c_p, _c_p, _c_p, _c_p]),
+    "ofw_segment_geometry": (_c_i, [_c_p, _c_i64, _c_p, _c_p, _c_p, _c_p]),
+    "ofw_direct_vjp": (_c_i, [_c_p, _c_i64, _c_p, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_i, _c_p, _c_p, _c_p, _c_p, _c_p,
+                              _c_p, _c_p, _c_p]),
 }
 
 _lib = None

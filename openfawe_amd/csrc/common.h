@@ -40,6 +40,7 @@ enum {
 // at least 1e-50 segment lengths keeps (h/rc)^2 finite.
 #define OFW_FAST_A2_MAX 1e200
 #define OFW_FAST_RC_MIN 1e-50
+#define OFW_NEAR_LINE2 1e-4   // (1e-2)^2: pairs closer to the line than 1e-2 |a| take the exact sequence
 
 // Mixed precision: a pair goes to FP64 when the target is within NEAR_CORES core radii of the
 // segment's line, or within NEAR_AXIS segment lengths (scaled like the axis guard) of it.
@@ -137,10 +138,13 @@ __device__ __forceinline__ int pair_fast64(double px, double py, double pz, cons
   const double nb2 = bx * bx + by * by + bz * bz;
   const bool exact_seg = (MODEL == OFW_MODEL_RANKINE || MODEL == OFW_MODEL_LAMB || MODEL == OFW_MODEL_VATISTAS)
                              ? r[R_EXACT] != 0.0 : false;
-  if (!(fmax(na2, nb2) <= OFW_FAST_A2_MAX) || exact_seg)
+  const double h2 = cx * cx + cy * cy + cz * cz;
+  // Near a segment's line the cross product d x a cancels: its rounding error (about eps |a|) is then a large
+  // fraction of h, and the result depends on the exact operation sequence.  Such pairs (h < 1e-2 |a|) take
+  // the reference's operation sequence, so that they agree with it to a few ulp of the pair velocity.
+  if (!(fmax(na2, nb2) <= OFW_FAST_A2_MAX) || exact_seg || h2 <= OFW_NEAR_LINE2 * na2)
     return pair_exact64(px, py, pz, r, MODEL, raise_singular, ux, uy, uz);
   const double t = (ax * dx + ay * dy) + az * dz;
-  const double h2 = cx * cx + cy * cy + cz * cz;
   const double na = __builtin_sqrt(na2), nb = __builtin_sqrt(nb2);
   const double mx = fmax(1.0, fmax(na, nb));
   if (h2 <= (OFW_AXIS_EPS * OFW_AXIS_EPS) * (mx * mx)) {

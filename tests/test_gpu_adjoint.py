@@ -1,0 +1,192 @@
+"""Pair matrix, ordered column assembly and the adjoint kernel on the GPU.
+
+Bars: pair matrix within 1e-13 relative per pair of the oracle's pair_matrix and the same flags; column
+assembly bit for bit the oracle's on the same pairs and bit for bit the GPU direct sum of each column's
+signed segments (the reference's bitwise check, tests/test_vsm_wake_batched.py:50-70); the adjoint's
+dL/dGamma equal to pair_matrix^T dL/dV (linearity) to 1e-12; dL/d(end points) and dL/d(targets) against
+central finite differences (h = 1e-6) of the forward, rtol 2e-6; the VSM Newton Jacobian built from the GPU
+influence operator against central differences of the closure (rtol 2e-7, atol 2e-9, the reference's
+tests/test_vsm_jacobian.py:67-88).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from openfawe_amd import adjoint as AD
+from openfawe_amd import vortex as V
+from openfawe_amd import workloads
+from tests.helpers import prepare
+
+pytestmark = pytest.mark.gpu
+
+
+def network(seed=3):
+    nodes, edges, gamma, radius, points = workloads.random_network(seed)
+    s, d, L = prepare(nodes, edges)
+    return nodes, edges, gamma, radius, points, s, d, L
+
+
+@pytest.mark.parametrize("model", range(6))
+def test_pair_matrix_matches_oracle(model):
+    _, _, _, radius, points, s, d, L = network()
+    ref, fref = O.pair_matrix(points, s, d, L, radius, model, "zero")
+    got, flags = AD.pair_matrix(points, s, d, L, radius, model, "zero")
+    assert np.array_equal(flags, fref)
+    scale = np.linalg.norm(ref, axis=2, keepdims=True)
+    assert np.all(np.abs(got - ref) <= 1e-13 * scale + 1e-300)
+    _, fr = O.pair_matrix(points, s, d, L, radius, model, "raise")
+    _, fg = AD.pair_matrix(points, s, d, L, radius, model, "raise")
+    assert np.array_equal(fg, fr)
+
+
+def ring_layout(rows, cols):
+    """Panels of a lattice as rings of 4 signed edges over shared segments (as vsm_wake._ring_layout)."""
+    nodes = np.array([[i * 0.4, j * 1.0, 0.05 * np.sin(j)] for i in range(rows + 1) for j in range(cols + 1)])
+    index = {}
+    segments = []
+
+    def edge(a, b):
+        key = (min(a, b), max(a, b))
+        if key not in index:
+            index[key] = len(segments)
+            segments.append(key)
+        return index[key], (1.0 if (a, b) == key else -1.0)
+    ptr, seg, sign = [0], [], []
+    for i in range(rows):
+        for j in range(cols):
+            a, b = i * (cols + 1) + j, i * (cols + 1) + j + 1
+            c, dd = (i + 1) * (cols + 1) + j + 1, (i + 1) * (cols + 1) + j
+            for p, q in ((a, b), (b, c), (c, dd), (dd, a)):
+                k, sg = edge(p, q)
+                seg.append(k)
+                sign.append(sg)
+            ptr.append(len(seg))
+    return nodes, np.array(segments), np.array(ptr), np.array(seg), np.array(sign)
+
+
+def test_columns_bitwise():
+    nodes, edges, ptr, seg, sign = ring_layout(4, 25)
+    s, d, L = prepare(nodes, edges)
+    radius = np.full(len(s), 0.02)
+    rng = np.random.default_rng(1)
+    points = rng.random((100, 3)) * [1.6, 25, 0.5] + [0, 0, 0.3]
+    pairs, _ = O.pair_matrix(points, s, d, L, radius, 2, "raise")
+    ref = O.assemble_columns(pairs, ptr, seg, sign)
+    got = AD.assemble_columns(pairs, ptr, seg, sign).cpu().numpy()
+    assert np.array_equal(ref.view(np.uint64), got.view(np.uint64))
+    gp, _ = AD.pair_matrix(points, s, d, L, radius, 2)
+    matrix = AD.assemble_columns(gp, ptr, seg, sign).cpu().numpy()
+    for j in range(0, len(ptr) - 1, 7):   # each column is the direct sum of its signed segments
+        k = slice(ptr[j], ptr[j + 1])
+        col, _ = V.hip_sum(points, s[seg[k]], d[seg[k]], L[seg[k]], sign[k], radius[seg[k]], 2,
+                           V.exclusion_csr(len(points), np.zeros((0, 2), np.int64)), "raise", V.HipSettings("fp64"))
+        assert np.array_equal(col.view(np.uint64), matrix[:, j].view(np.uint64))
+
+
+def forward(points, a, b, gamma, radius, model):
+    import torch
+    t = lambda v: torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64)).cuda()  # noqa: E731
+    d, L = AD.segment_geometry(t(a), t(b))
+    out, _ = V.hip_sum(points, a, d.cpu().numpy(), L.cpu().numpy(), gamma, radius, model,
+                       V.exclusion_csr(len(points), np.zeros((0, 2), np.int64)), "zero", V.HipSettings("fp64"))
+    return out
+
+
+@pytest.mark.parametrize("model", [0, 1, 2, 3, 4])
+def test_vjp_against_linearity_and_finite_differences(model):
+    nodes, edges, gamma, radius, points, s, d, L = network(4)
+    points = points[:40]
+    a = nodes[edges[:, 0]]
+    b = nodes[edges[:, 1]]
+    rng = np.random.default_rng(7)
+    g = rng.normal(size=(len(points), 3))
+    gg, ga, gb, gx, out = (v.cpu().numpy() if v is not None else None
+                           for v in AD.direct_vjp(points, a, b, gamma, radius, model, g, fused_forward=True))
+    # fused forward equals the direct sum
+    ref, _, scale = O.direct_sum(points, s, d, L, gamma, radius, model, None, "zero", with_scale=True)
+    assert np.all(np.linalg.norm(out - ref, axis=1) <= 1e-12 * scale)
+    # dL/dGamma = pair_matrix^T g (the velocity is linear in Gamma)
+    pm, _ = O.pair_matrix(points, s, d, L, radius, model, "zero")
+    lin = np.einsum("psc,pc->s", pm, g)
+    bound = 1e-12 * np.einsum("psc->s", np.abs(pm) * np.abs(g)[:, None, :]) + 1e-300
+    assert np.all(np.abs(gg - lin) <= bound)
+    # central differences of L = sum(g * V) in a sample of coordinates
+    h = 1e-6
+    for which, grad, base in (("a", ga, a), ("b", gb, b), ("x", gx, points)):
+        for k in rng.choice(len(base), 6, replace=False):
+            for c in range(3):
+                plus, minus = base.copy(), base.copy()
+                plus[k, c] += h
+                minus[k, c] -= h
+                args = dict(a=a, b=b, x=points)
+                args[which] = plus
+                vp = forward(args["x"], args["a"], args["b"], gamma, radius, model)
+                args[which] = minus
+                vm = forward(args["x"], args["a"], args["b"], gamma, radius, model)
+                fd = np.sum(g * (vp - vm)) / (2 * h)
+                assert abs(fd - grad[k, c]) <= 2e-6 * max(abs(fd), abs(grad[k, c])) + 1e-7, (which, k, c, fd,
+                                                                                               grad[k, c])
+
+
+def test_autograd_function():
+    import torch
+    nodes, edges, gamma, radius, points, s, d, L = network(5)
+    t = lambda v: torch.from_numpy(np.ascontiguousarray(v, dtype=np.float64)).cuda()  # noqa: E731
+    x = t(points[:30]).requires_grad_(True)
+    a = t(nodes[edges[:, 0]]).requires_grad_(True)
+    b = t(nodes[edges[:, 1]]).requires_grad_(True)
+    g = t(gamma).requires_grad_(True)
+    v = AD.induced_velocity(x, a, b, g, t(radius), model=2, singularity="zero")
+    ref, _ = O.direct_sum(points[:30], s, d, L, gamma, radius, 2, None, "zero")
+    assert np.abs(v.detach().cpu().numpy() - ref).max() <= 1e-12 * np.abs(ref).max()
+    w = t(np.random.default_rng(0).normal(size=(30, 3)))
+    (v * w).sum().backward()
+    gg, ga, gb, gx, _ = AD.direct_vjp(x.detach(), a.detach(), b.detach(), g.detach(), t(radius), 2, w)
+    for auto, direct in ((g.grad, gg), (a.grad, ga), (b.grad, gb), (x.grad, gx)):
+        assert torch.equal(auto, direct)
+
+
+def test_vsm_closure_jacobian():
+    """The reference's Newton Jacobian (vsm.py:333-349) from the GPU influence operator, against central
+    differences of the closure (vsm.py:320-330) with h = 1e-6 (tests/test_vsm_jacobian.py:67-88)."""
+    nodes, edges, ptr, seg, sign = ring_layout(1, 20)
+    s, d, L = prepare(nodes, edges)
+    radius = np.full(len(s), 0.05)
+    n = len(ptr) - 1
+    cp = np.array([[0.3, j + 0.5, 0.05 * np.sin(j + 0.5)] for j in range(n)])   # 3/4-chord points
+    pairs, flags = AD.pair_matrix(cp, s, d, L, radius, 2)
+    assert not flags.any()
+    matrix = AD.assemble_columns(pairs, ptr, seg, sign).cpu().numpy()
+    chord = np.full(n, 0.4)
+    tangent = np.tile([1.0, 0.0, 0.0], (n, 1))
+    normal = np.tile([0.0, 0.0, 1.0], (n, 1))
+    span = np.tile([0.0, 1.0, 0.0], (n, 1))
+    ambient = np.tile([20.0, 0.0, 1.5], (n, 1))
+    known = np.zeros((n, 3))
+    cross = np.linalg.norm(np.cross(ambient, span), axis=1)
+    slope = 2 * np.pi
+
+    def closure(gamma):
+        v = ambient + np.einsum("ijk,j->ik", matrix, gamma) + known
+        vt, vn = np.sum(v * tangent, 1), np.sum(v * normal, 1)
+        alpha = np.arctan2(vn, vt)
+        speed2 = np.linalg.norm(np.cross(v, span), axis=1) ** 2
+        return .5 * chord * (slope * alpha) * speed2 / cross - gamma
+
+    def jacobian(gamma):
+        v = ambient + np.einsum("ijk,j->ik", matrix, gamma) + known
+        vt, vn = np.sum(v * tangent, 1), np.sum(v * normal, 1)
+        alpha = np.arctan2(vn, vt)
+        cl = slope * alpha
+        speed2 = np.linalg.norm(np.cross(v, span), axis=1) ** 2
+        an = np.einsum("ijk,ik->ij", matrix, normal)
+        at = np.einsum("ijk,ik->ij", matrix, tangent)
+        da = (vt[:, None] * an - vn[:, None] * at) / (vt ** 2 + vn ** 2)[:, None]
+        dh = 2 * np.einsum("ik,ijk->ij", np.cross(v, span), np.cross(matrix, span[:, None, :]))
+        return (.5 * chord / cross)[:, None] * (slope * speed2[:, None] * da + cl[:, None] * dh) - np.eye(n)
+
+    gamma = np.linspace(1.0, 3.0, n)
+    J = jacobian(gamma)
+    h = 1e-6
+    fd = np.stack([(closure(gamma + h * e) - closure(gamma - h * e)) / (2 * h) for e in np.eye(n)], axis=1)
+    np.testing.assert_allclose(J, fd, rtol=2e-7, atol=2e-9)
