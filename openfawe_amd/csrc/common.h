@@ -24,7 +24,7 @@
 // FP64 segment record slots
 enum {
   R_SX = 0, R_SY, R_SZ, R_DX, R_DY, R_DZ, R_L, R_INVL, R_G4, R_RC, R_RC2, R_GAMMA, R_RADIUS, R_EXACT,
-  R_PAD0, R_PAD1, REC64 = 16
+  R_IRC2, R_PAD1, REC64 = 16
 };
 // FP32 segment record slots (16 floats, 64 B)
 enum {
@@ -39,8 +39,8 @@ enum {
 // does (pair_exact64).  |a| <= 1e100 segment lengths keeps every square finite and normal; a core of
 // at least 1e-50 segment lengths keeps (h/rc)^2 finite.
 #define OFW_FAST_A2_MAX 1e200
+#define OFW_FAST_A2_MIN 1e-200
 #define OFW_FAST_RC_MIN 1e-50
-#define OFW_NEAR_LINE2 1e-4   // (1e-2)^2: pairs closer to the line than 1e-2 |a| take the exact sequence
 
 // Mixed precision: a pair goes to FP64 when the target is within NEAR_CORES core radii of the
 // segment's line, or within NEAR_AXIS segment lengths (scaled like the axis guard) of it.
@@ -122,30 +122,72 @@ __device__ __noinline__ int pair_exact64(double px, double py, double pz, const 
   return OFW_FLAG_OK;
 }
 
+// Reciprocal and square root for normal, positive FP64 operands: the hardware estimates (v_rcp_f64,
+// v_rsq_f64) refined by Newton-Raphson steps to within an ulp or two, instead of the IEEE-exact division and
+// square-root sequences with their range scaling.  Every operation is written out (no contraction), so the
+// result does not depend on the kernel the function is inlined into.
+__device__ __forceinline__ double ofw_rcp(double x) {
+#pragma clang fp contract(off)
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+
+__device__ __forceinline__ double ofw_sqrt(double x) {
+#pragma clang fp contract(off)
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  const double e = __builtin_fma(-h, g, 0.5);
+  g = __builtin_fma(g, e, g);
+  h = __builtin_fma(h, e, h);
+  const double d = __builtin_fma(-g, g, x);
+  return __builtin_fma(d, h, g);
+}
+
+__device__ __forceinline__ double ofw_rsqrt(double x) {
+#pragma clang fp contract(off)
+  double y = __builtin_amdgcn_rsq(x);
+  double xy = x * y;
+  double e = __builtin_fma(-xy, y, 1.0);  // 1 - x y^2
+  y = __builtin_fma(0.5 * y, e, y);
+  xy = x * y;
+  e = __builtin_fma(-xy, y, 1.0);
+  return __builtin_fma(0.5 * y, e, y);
+}
+
 // Fast FP64 pair: the reference formula (vortex_numba.py:46-109) reorganised for the VALU: a per-segment
-// reciprocal length and circulation factor, squared norms (no hypot), the axis test and the core ratios
-// on squares, one division per pair.  Same branches and flags; results agree with the operation-by-
-// operation form to a few ulp.  Falls back to pair_exact64 outside the fast range.
+// reciprocal length and circulation factor, squared norms instead of hypot, the axis test and the core
+// ratios on squares, one reciprocal per pair.  Same branches and flags.  a = (p - s)/L and the cross product
+// d x a are rounded exactly as the reference rounds them (they decide h near a segment's line, where the
+// cross product cancels); the other quantities agree with it to an ulp or two.  The arithmetic is written
+// out operation by operation (no contraction), so every kernel that inlines it (direct sum, tree near field,
+// pair matrix) gives the same bits for the same pair.  Pairs outside the fast range take pair_exact64.
 template <int MODEL>
 __device__ __forceinline__ int pair_fast64(double px, double py, double pz, const double* __restrict__ r,
                                            int raise_singular, double& ux, double& uy, double& uz) {
-  const double invL = r[R_INVL];
+#pragma clang fp contract(off)
+  const double L = r[R_L], invL = r[R_INVL];
   const double dx = r[R_DX], dy = r[R_DY], dz = r[R_DZ];
-  const double ax = (px - r[R_SX]) * invL, ay = (py - r[R_SY]) * invL, az = (pz - r[R_SZ]) * invL;
+  // the correctly rounded quotient: the product with the correctly rounded reciprocal, corrected once by
+  // its exact remainder (Markstein)
+  const double qx = px - r[R_SX], qy = py - r[R_SY], qz = pz - r[R_SZ];
+  double ax = qx * invL, ay = qy * invL, az = qz * invL;
+  ax = __builtin_fma(__builtin_fma(-ax, L, qx), invL, ax);
+  ay = __builtin_fma(__builtin_fma(-ay, L, qy), invL, ay);
+  az = __builtin_fma(__builtin_fma(-az, L, qz), invL, az);
   const double cx = dy * az - dz * ay, cy = dz * ax - dx * az, cz = dx * ay - dy * ax;
-  const double na2 = ax * ax + ay * ay + az * az;
+  const double na2 = __builtin_fma(az, az, __builtin_fma(ay, ay, ax * ax));
   const double bx = ax - dx, by = ay - dy, bz = az - dz;
-  const double nb2 = bx * bx + by * by + bz * bz;
+  const double nb2 = __builtin_fma(bz, bz, __builtin_fma(by, by, bx * bx));
   const bool exact_seg = (MODEL == OFW_MODEL_RANKINE || MODEL == OFW_MODEL_LAMB || MODEL == OFW_MODEL_VATISTAS)
                              ? r[R_EXACT] != 0.0 : false;
-  const double h2 = cx * cx + cy * cy + cz * cz;
-  // Near a segment's line the cross product d x a cancels: its rounding error (about eps |a|) is then a large
-  // fraction of h, and the result depends on the exact operation sequence.  Such pairs (h < 1e-2 |a|) take
-  // the reference's operation sequence, so that they agree with it to a few ulp of the pair velocity.
-  if (!(fmax(na2, nb2) <= OFW_FAST_A2_MAX) || exact_seg || h2 <= OFW_NEAR_LINE2 * na2)
+  if (!(fmax(na2, nb2) <= OFW_FAST_A2_MAX) || !(fmin(na2, nb2) >= OFW_FAST_A2_MIN) || exact_seg)
     return pair_exact64(px, py, pz, r, MODEL, raise_singular, ux, uy, uz);
-  const double t = (ax * dx + ay * dy) + az * dz;
-  const double na = __builtin_sqrt(na2), nb = __builtin_sqrt(nb2);
+  const double h2 = __builtin_fma(cz, cz, __builtin_fma(cy, cy, cx * cx));
+  const double t = __builtin_fma(az, dz, __builtin_fma(ay, dy, ax * dx));
+  const double na = ofw_sqrt(na2), nb = ofw_sqrt(nb2);
   const double mx = fmax(1.0, fmax(na, nb));
   if (h2 <= (OFW_AXIS_EPS * OFW_AXIS_EPS) * (mx * mx)) {
     if (raise_singular && t >= 0.0 && t <= 1.0) return OFW_FLAG_SINGULAR;
@@ -155,40 +197,40 @@ __device__ __forceinline__ int pair_fast64(double px, double py, double pz, cons
   if (g4 == 0.0) return OFW_FLAG_OK;
   double sa = na, sb = nb;
   double product = na * nb;
-  double dot = h2 + t * (t - 1.0);
-  double plus = dot < 0.0 ? h2 / (product - dot) : product + dot;
+  double dot = __builtin_fma(t, t - 1.0, h2);
+  double plus = dot < 0.0 ? h2 * ofw_rcp(product - dot) : product + dot;
   double den = product * plus;
   double factor = 1.0;
   const double radius = r[R_RADIUS];
   if (MODEL != OFW_MODEL_NONE && radius > 0.0) {
     const double rc2 = r[R_RC2];
+    const double q = h2 * r[R_IRC2];  // (h / rc)^2
     if (MODEL == OFW_MODEL_RANKINE) {
-      factor = fmin(h2 / rc2, 1.0);
+      factor = fmin(q, 1.0);
     } else if (MODEL == OFW_MODEL_LAMB) {
-      const double q = h2 / rc2;
       if (q <= 10.0 / OFW_LAMB) factor = -expm1(-OFW_LAMB * q);
     } else if (MODEL == OFW_MODEL_VATISTAS) {
-      const double q = h2 / rc2;
       if (q < 1.0) {
-        factor = q / __builtin_sqrt(1.0 + q * q);
+        factor = q * ofw_rsqrt(__builtin_fma(q, q, 1.0));
       } else {
-        const double iq = 1.0 / q;
-        factor = 1.0 / __builtin_sqrt(1.0 + iq * iq);
+        const double iq = ofw_rcp(q);
+        factor = ofw_rsqrt(__builtin_fma(iq, iq, 1.0));
       }
     } else if (MODEL == OFW_MODEL_OFFSET) {
       den = den + rc2;
     } else if (MODEL == OFW_MODEL_RM) {
-      sa = __builtin_sqrt(na2 + rc2);
-      sb = __builtin_sqrt(nb2 + rc2);
+      sa = ofw_sqrt(na2 + rc2);
+      sb = ofw_sqrt(nb2 + rc2);
       product = sa * sb;
       const double tr2 = h2 + rc2;
-      dot = tr2 + t * (t - 1.0);
-      plus = dot < 0.0 ? tr2 / (product - dot) : product + dot;
+      dot = __builtin_fma(t, t - 1.0, tr2);
+      plus = dot < 0.0 ? tr2 * ofw_rcp(product - dot) : product + dot;
       den = product * plus;
     }
   }
-  const double coef = g4 * factor * (sa + sb) / den;
+  const double coef = ((g4 * factor) * (sa + sb)) * ofw_rcp(den);
   if (!isfinite(coef)) return OFW_FLAG_NONFINITE;
+  // the pair velocity is rounded before it is added, as in the reference's ordered sum
   ux += coef * cx;
   uy += coef * cy;
   uz += coef * cz;

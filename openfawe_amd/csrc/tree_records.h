@@ -36,7 +36,7 @@ __device__ __forceinline__ void pack_record(int s, const double* __restrict__ st
   r[R_GAMMA] = g;
   r[R_RADIUS] = rad;
   r[R_EXACT] = exact;
-  r[R_PAD0] = 0.0;
+  r[R_IRC2] = cored ? 1.0 / rc2 : 0.0;
   r[R_PAD1] = 0.0;
   // a segment whose factors leave the normal FP32 range always takes the FP64 path
   const double ag = fabs(g4);
