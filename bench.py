@@ -114,6 +114,49 @@ def cpu_baseline(case, spec, threads):
                        f"segments, {dt:.2f} s, extrapolated to the whole workload; {cpu_model()}")
 
 
+def adjoint_bench(steps):
+    """Differentiation case: 100-panel VSM wing in a ~205k-segment wake.  Times the FP64 forward direct sum at
+    the 3/4-chord points, the fused forward + adjoint (dV -> dGamma, d(end points), d(points)), and the
+    100 x 100 attached-ring influence Jacobian (pair matrix + ordered column sums)."""
+    import torch
+    from openfawe_amd import adjoint as AD
+    from openfawe_amd import vortex as V
+    from openfawe_amd import workloads
+
+    case = workloads.vsm_wing()
+    n, m = len(case["points"]), len(case["starts"])
+    dev = lambda a, dt=torch.float64: torch.from_numpy(np.ascontiguousarray(a)).to("cuda", dt)  # noqa: E731
+    pts, starts, dirs = dev(case["points"]), dev(case["starts"]), dev(case["directions"])
+    lens, gam, rad = dev(case["lengths"]), dev(case["gamma"]), dev(case["radius"])
+    ends = starts + dirs * lens[:, None]
+    ptr, seg = dev(np.zeros(n + 1, np.int64), torch.int64), dev(np.zeros(1, np.int64), torch.int64)
+    settings = V.HipSettings("fp64", 128)
+    grad = dev(np.random.default_rng(0).normal(size=(n, 3)))
+    ws, wd, wl, wr = (dev(case[k]) for k in ("wing_starts", "wing_directions", "wing_lengths", "wing_radius"))
+    cptr, cseg, csign = dev(case["column_ptr"], torch.int64), dev(case["column_segment"], torch.int64), \
+        dev(case["column_sign"])
+
+    def timed(fn):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return 1e3 * (time.perf_counter() - t0) / steps
+
+    fwd = timed(lambda: V.hip_sum_device(pts, starts, dirs, lens, gam, rad, 2, ptr, seg, "zero", settings))
+    fused = timed(lambda: AD.direct_vjp(pts, starts, ends, gam, rad, 2, grad, ptr, seg, fused_forward=True))
+
+    def jac():
+        pairs, _ = AD.pair_matrix_device(pts, ws, wd, wl, wr, 2)
+        return AD.assemble_columns(pairs, cptr, cseg, csign)
+    jms = timed(jac)
+    return dict(workload=case["name"], targets=n, segments=m, forward_ms=fwd, fused_forward_adjoint_ms=fused,
+                adjoint_fwd_ratio=fused / fwd, influence_jacobian_ms=jms, jacobian_shape=[n, n, 3])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +165,7 @@ def main():
     ap.add_argument("--workload", default="tree-mixed-820k", choices=sorted(WORKLOADS))
     ap.add_argument("--split-depth", type=int, default=-1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extra", action="store_true", help="skip the adjoint and batch legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this build "
@@ -204,6 +248,8 @@ def main():
                       frac=achieved / peak, traffic=None, kernel=kernel, kernel_ms=kms),
         **extra,
     )
+    if not args.no_extra and spec["kind"] == "tree":
+        result["adjoint"] = adjoint_bench(max(3, args.steps // 2))
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(case, spec, usable_cpus())
     print(json.dumps(result), flush=True)

@@ -154,6 +154,68 @@ def helix(seed=0, rows=40, spanwise=256, radius_helix=150.0, pitch=40.0, span=26
                 nodes=nodes, edges=edges, name="helix")
 
 
+def vsm_wing(panels=100, span=26.0, chord=1.3, rotors=8, ring_segments=32, rotor_radius=1.15, wake_rows=936,
+             seed=5, core=0.05):
+    """A VSM wing in a free wake: the differentiation case.
+
+    Each of ``panels`` spanwise panels is a vortex ring over its bound vortex at 1/4 chord: quarter-chord
+    segment, two chordwise legs to the trailing edge and the trailing-edge segment; neighbouring panels
+    share their legs with opposite signs (the ring layout of vsm_wake._ring_layout, vsm_wake.py:196-220).
+    Collocation points are at 3/4 chord, panel mid-span.  ``rotors`` rotor rings of ``ring_segments``
+    segments and radius ``rotor_radius`` sit ahead of the wing, and a crosswind-loop wake of ``wake_rows``
+    rows (about 205k segments at 936 rows) trails behind it.  Returns the collocation points, the wing ring
+    segments and their column layout (ptr, segment, sign), and the wake segments (rotor rings included).
+    """
+    rng = np.random.default_rng(seed)
+    y = np.linspace(-span / 2, span / 2, panels + 1)
+    sweep = 0.02 * np.abs(y)
+    qc = np.stack([sweep + 0.25 * chord, y, 0.01 * y ** 2 / span], axis=1)
+    te = np.stack([sweep + chord, y, 0.01 * y ** 2 / span], axis=1)
+    nodes = np.concatenate([qc, te])
+    index, segments, ptr, col_seg, col_sign = {}, [], [0], [], []
+
+    def edge(a, b):
+        key = (min(a, b), max(a, b))
+        if key not in index:
+            index[key] = len(segments)
+            segments.append(key)
+        return index[key], (1.0 if (a, b) == key else -1.0)
+    n1 = panels + 1
+    for j in range(panels):
+        for a, b in ((j, j + 1), (j + 1, n1 + j + 1), (n1 + j + 1, n1 + j), (n1 + j, j)):
+            k, sign = edge(a, b)
+            col_seg.append(k)
+            col_sign.append(sign)
+        ptr.append(len(col_seg))
+    wing_edges = np.asarray(segments, dtype=np.int64)
+    ws, wd, wl = _segments(nodes, wing_edges)
+    ymid = 0.5 * (y[:-1] + y[1:])
+    points = np.stack([0.02 * np.abs(ymid) + 0.75 * chord, ymid, 0.01 * ymid ** 2 / span], axis=1)
+    # rotor rings ahead of the wing
+    hubs = np.linspace(-0.35 * span, 0.35 * span, rotors)
+    phi = np.linspace(0.0, 2 * np.pi, ring_segments + 1)
+    rnodes, redges = [], []
+    for k, yh in enumerate(hubs):
+        ring = np.stack([np.full_like(phi, -1.5), yh + rotor_radius * np.cos(phi), rotor_radius * np.sin(phi)], axis=1)
+        base = sum(len(v) for v in rnodes)
+        rnodes.append(ring)
+        redges += [(base + i, base + i + 1) for i in range(ring_segments)]
+    rnodes = np.concatenate(rnodes)
+    rs, rd, rl = _segments(rnodes, np.asarray(redges, dtype=np.int64))
+    wake = loop(rows=wake_rows, seed=seed, target_rows=1)
+    # place the wake's newest row (the kite's current position on the loop) one chord behind the wing
+    shift = np.array([chord + 0.5, 0.0, 0.0]) - wake["points"].mean(axis=0)
+    starts = np.concatenate([rs, wake["starts"] + shift])
+    directions = np.concatenate([rd, wake["directions"]])
+    lengths = np.concatenate([rl, wake["lengths"]])
+    gamma = np.concatenate([rng.uniform(0.0, 2.0, len(rs)), wake["gamma"]])
+    return dict(points=points, wing_starts=ws, wing_directions=wd, wing_lengths=wl,
+                column_ptr=np.asarray(ptr, dtype=np.int64), column_segment=np.asarray(col_seg, dtype=np.int64),
+                column_sign=np.asarray(col_sign), starts=starts, directions=directions, lengths=lengths,
+                gamma=gamma, radius=np.full(len(starts), core), wing_radius=np.full(len(ws), core), model=2,
+                name=f"vsm-{panels}-wake-{len(starts)}")
+
+
 def random_network(seed, nodes=60, segments=90, points=70):
     """The reference's random network with near-axis targets (tests/test_vortex_numba.py:38-54)."""
     rng = np.random.default_rng(seed)

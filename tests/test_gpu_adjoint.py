@@ -190,3 +190,33 @@ def test_vsm_closure_jacobian():
     h = 1e-6
     fd = np.stack([(closure(gamma + h * e) - closure(gamma - h * e)) / (2 * h) for e in np.eye(n)], axis=1)
     np.testing.assert_allclose(J, fd, rtol=2e-7, atol=2e-9)
+
+
+def test_vsm_wing_in_wake_influence_and_adjoint():
+    """The differentiation case at a reduced wake: the 100 x 100 ring influence Jacobian equals the direct sums
+    of each panel's signed ring (bitwise), and the adjoint's dL/dGamma over the wake equals the unit-circulation
+    pair velocities contracted with dL/dV (linearity)."""
+    case = workloads.vsm_wing(wake_rows=12)
+    pts = case["points"]
+    pairs, flags = AD.pair_matrix(pts, case["wing_starts"], case["wing_directions"], case["wing_lengths"],
+                                  case["wing_radius"], 2)
+    assert not flags.any()
+    jac = AD.assemble_columns(pairs, case["column_ptr"], case["column_segment"], case["column_sign"]).cpu().numpy()
+    assert jac.shape == (100, 100, 3)
+    ptr, seg, sign = case["column_ptr"], case["column_segment"], case["column_sign"]
+    for j in (0, 37, 99):
+        k = slice(ptr[j], ptr[j + 1])
+        col, _ = V.hip_sum(pts, case["wing_starts"][seg[k]], case["wing_directions"][seg[k]],
+                           case["wing_lengths"][seg[k]], sign[k], case["wing_radius"][seg[k]], 2,
+                           V.exclusion_csr(len(pts), np.zeros((0, 2), np.int64)), "raise", V.HipSettings("fp64"))
+        assert np.array_equal(col, jac[:, j])
+    s, d, L, g, r = (case[k] for k in ("starts", "directions", "lengths", "gamma", "radius"))
+    ends = s + d * L[:, None]
+    grad = np.random.default_rng(3).normal(size=pts.shape)
+    gg, ga, gb, gx, out = AD.direct_vjp(pts, s, ends, g, r, 2, grad, fused_forward=True)
+    ref, _, scale = O.direct_sum(pts, s, d, L, g, r, 2, None, "raise", with_scale=True)
+    assert np.all(np.linalg.norm(out.cpu().numpy() - ref, axis=1) <= 1e-12 * scale)
+    pm, _ = O.pair_matrix(pts, s, d, L, r, 2, "raise")
+    lin = np.einsum("psc,pc->s", pm, grad)
+    bound = 1e-12 * np.einsum("psc->s", np.abs(pm) * np.abs(grad)[:, None, :]) + 1e-300
+    assert np.all(np.abs(gg.cpu().numpy() - lin) <= bound)
