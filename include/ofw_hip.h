@@ -119,6 +119,32 @@ int ofw_tree_sum(ofw_ctx *ctx, int precision, int64_t n, const double *points, i
                  int expansion_order, int64_t leaf_size, double clearance, int split_depth, double *out,
                  int8_t *flags, int64_t *far_counts, int64_t *near_counts);
 
+/* Batched tree sum over K independent wake snapshots packed back to back (no counterpart in the reference,
+ * whose engines take one snapshot per call, vortex_tree.py:588 and vortex_cuda_tree.py:439).  Snapshot k owns
+ * targets [target_offsets_host[k], target_offsets_host[k+1]) of points and segments
+ * [segment_offsets_host[k], segment_offsets_host[k+1]) of the segment arrays (host int64 arrays of K+1
+ * non-decreasing offsets from 0).  ex_ptr (n+1) / ex_seg: optional exclusion CSR over all targets with global
+ * segment ids (NULL for none).  Other arguments as ofw_tree_sum; split_depth -1 = chosen from the total target
+ * count.  Each snapshot's result equals ofw_tree_sum on that snapshot alone with the same split_depth, bit for
+ * bit.  Creation runs the call once (sizing all scratch and checking every capacity) and, when use_graph is
+ * set, records its whole launch sequence (build, node quantities, traversal and reduction of every snapshot)
+ * in a HIP graph.  The pointers are fixed at creation; ofw_tree_batch_run replays the call on the current
+ * contents of the same arrays, on the context's stream, synchronously. */
+typedef struct ofw_tree_batch ofw_tree_batch;
+int ofw_tree_batch_create(ofw_ctx *ctx, ofw_tree_batch **plan, int precision, int64_t K,
+                          const int64_t *target_offsets_host, const int64_t *segment_offsets_host,
+                          const double *points, const double *starts, const double *directions,
+                          const double *lengths, const double *gamma, const double *radius, int model,
+                          int raise_singular, const int64_t *ex_ptr, const int64_t *ex_seg, double theta,
+                          int expansion_order, int64_t leaf_size, double clearance, int split_depth, int use_graph,
+                          double *out, int8_t *flags, int64_t *far_counts, int64_t *near_counts);
+int ofw_tree_batch_run(ofw_tree_batch *plan);
+/* Timings (ms) of the plan's last run: one phase (the graph) when recorded, else the phases of ofw_tree_sum. */
+int ofw_tree_batch_timings(ofw_tree_batch *plan, float *ms_host, int count);
+/* Counters of the plan's last run, as ofw_ctx_stats. */
+int ofw_tree_batch_stats(ofw_tree_batch *plan, int64_t *stats_host, int count);
+int ofw_tree_batch_destroy(ofw_tree_batch *plan);
+
 /* Unit-circulation velocity of every segment at every target: out (n,m,3), flags (n) (first flag in
  * segment order).  Replaces vortex_numba.pair_matrix (vortex_numba.py:271-287).  n <= 65,535. */
 int ofw_pair_matrix(ofw_ctx *ctx, int64_t n, const double *points, int64_t m, const double *starts,

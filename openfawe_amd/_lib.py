@@ -34,6 +34,13 @@ SIGNATURES = {
                        + [ctypes.POINTER(_c_i64)]),
     "ofw_tree_sum": (_c_i, [_c_p, _c_i, _c_i64, _c_p, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i, _c_i, _c_p, _c_p,
                             _c_d, _c_i, _c_i64, _c_d, _c_i, _c_p, _c_p, _c_p, _c_p]),
+    "ofw_tree_batch_create": (_c_i, [_c_p, ctypes.POINTER(_c_p), _c_i, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p,
+                                     _c_p, _c_p, _c_i, _c_i, _c_p, _c_p, _c_d, _c_i, _c_i64, _c_d, _c_i, _c_i, _c_p,
+                                     _c_p, _c_p, _c_p]),
+    "ofw_tree_batch_run": (_c_i, [_c_p]),
+    "ofw_tree_batch_timings": (_c_i, [_c_p, ctypes.POINTER(ctypes.c_float), _c_i]),
+    "ofw_tree_batch_stats": (_c_i, [_c_p, ctypes.POINTER(_c_i64), _c_i]),
+    "ofw_tree_batch_destroy": (_c_i, [_c_p]),
     "ofw_pair_matrix": (_c_i, [_c_p, _c_i64, _c_p, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_i, _c_i, _c_p, _c_p]),
     "ofw_assemble_columns": (_c_i, [_c_p, _c_i64, _c_i64, _c_p, _c_i64, _c_p, _c_p, _c_p, _c_p]),
     "ofw_segment_geometry": (_c_i, [_c_p, _c_i64, _c_p, _c_p, _c_p, _c_p]),

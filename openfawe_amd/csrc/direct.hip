@@ -43,7 +43,7 @@ __global__ void k_i64_to_i32(int64_t count, const int64_t* __restrict__ src, int
   if (i < count) dst[i] = (int32_t)src[i];
 }
 
-int csr_to_i32(Ctx& c, int64_t n, const int64_t* ex_ptr, const int64_t* ex_seg, int32_t** ptr32, int32_t** seg32) {
+int csr_count(Ctx& c, int64_t n, const int64_t* ex_ptr, int64_t* e_out) {
   int64_t e = 0;
   OFW_HIP(hipMemcpyAsync(&e, ex_ptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
   OFW_HIP(hipStreamSynchronize(c.stream));
@@ -51,6 +51,12 @@ int csr_to_i32(Ctx& c, int64_t n, const int64_t* ex_ptr, const int64_t* ex_seg, 
     set_error("exclusion count out of range");
     return OFW_ERR_ARG;
   }
+  *e_out = e;
+  return OFW_OK;
+}
+
+int csr_convert(Ctx& c, int64_t n, int64_t e, const int64_t* ex_ptr, const int64_t* ex_seg, int32_t** ptr32,
+                int32_t** seg32) {
   OFW_HIP(c.exptr32.reserve((size_t)(n + 1) * sizeof(int32_t)));
   OFW_HIP(c.exseg32.reserve((size_t)(e > 0 ? e : 1) * sizeof(int32_t)));
   *ptr32 = (int32_t*)c.exptr32.ptr;
@@ -61,6 +67,13 @@ int csr_to_i32(Ctx& c, int64_t n, const int64_t* ex_ptr, const int64_t* ex_seg, 
     hipLaunchKernelGGL(k_i64_to_i32, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, c.stream, e, ex_seg, *seg32);
   OFW_HIP(hipGetLastError());
   return OFW_OK;
+}
+
+int csr_to_i32(Ctx& c, int64_t n, const int64_t* ex_ptr, const int64_t* ex_seg, int32_t** ptr32, int32_t** seg32) {
+  int64_t e = 0;
+  int st = csr_count(c, n, ex_ptr, &e);
+  if (st) return st;
+  return csr_convert(c, n, e, ex_ptr, ex_seg, ptr32, seg32);
 }
 
 // ---------------------------------------------------------------- direct kernels

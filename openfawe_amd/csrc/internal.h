@@ -31,7 +31,8 @@ struct Ctx {
   hipEvent_t ev[12] = {};
   int nev = 0;               // events recorded by the last call (phase k = ev[k] -> ev[k+1])
   int64_t stats_i[8] = {};
-  hipError_t mark() { return nev < 12 ? hipEventRecord(ev[nev++], stream) : hipSuccess; }
+  bool capturing = false;    // enqueuing into a graph capture: no events are recorded
+  hipError_t mark() { return (!capturing && nev < 12) ? hipEventRecord(ev[nev++], stream) : hipSuccess; }
 };
 
 #define OFW_HIP(expr)                                                                              \
@@ -50,6 +51,10 @@ int direct_sum(Ctx& c, int precision, int block, int64_t n, const double* points
                const float* rec32, int model, int raise_singular, const int32_t* ex_ptr, const int32_t* ex_seg,
                double* out, int8_t* flags, int32_t* flag_seg);
 int csr_to_i32(Ctx& c, int64_t n, const int64_t* ex_ptr, const int64_t* ex_seg, int32_t** ptr32, int32_t** seg32);
+// the two halves of csr_to_i32: the entry count (read back, synchronising) and the enqueued conversion
+int csr_count(Ctx& c, int64_t n, const int64_t* ex_ptr, int64_t* e_out);
+int csr_convert(Ctx& c, int64_t n, int64_t e, const int64_t* ex_ptr, const int64_t* ex_seg, int32_t** ptr32,
+                int32_t** seg32);
 
 }  // namespace ofw
 

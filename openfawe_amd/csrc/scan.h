@@ -1,0 +1,45 @@
+// Workgroup-wide exclusive scan shared by the build, moment and traversal planning kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace ofw {
+
+// Exclusive scan of one int per thread over the workgroup (blockDim a multiple of 64, at most 1024); every
+// thread of the workgroup must call it.  *total receives the workgroup's sum.
+__device__ inline int block_exclusive_scan(int v, int* total) {
+  __shared__ int wave_sums[17];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) wave_sums[w] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int k = 0; k < nw; ++k) {
+      const int s = wave_sums[k];
+      wave_sums[k] = acc;
+      acc += s;
+    }
+    wave_sums[16] = acc;
+  }
+  __syncthreads();
+  const int result = x - v + wave_sums[w];
+  *total = wave_sums[16];
+  __syncthreads();  // the next call may overwrite wave_sums
+  return result;
+}
+
+// Workgroup index remapped so that each XCD (workgroups are dispatched to the 8 XCDs round-robin) receives a
+// contiguous range of logical indices: neighbouring work shares one XCD's L2.  A bijection on [0, nb).
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  const int per = nb >> 3;
+  if (b >= (per << 3)) return b;
+  return (b & 7) * per + (b >> 3);
+}
+
+}  // namespace ofw

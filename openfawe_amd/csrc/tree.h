@@ -1,4 +1,4 @@
-// Host-side tree structures shared by the tree build, moments and traversal.
+// Device-side tree (a forest of one or more trees over disjoint segment ranges) and its build.
 #pragma once
 #include <stdint.h>
 
@@ -8,19 +8,74 @@
 
 namespace ofw {
 
-struct HostTree {
-  int64_t m = 0, count = 0;
-  int depth = 0;
-  // creation numbering of the level build (vortex_tree.py:244-268)
-  std::vector<int32_t> lo, hi, left, right;
-  // the reference's numbering (vortex_tree.py:271-306)
-  std::vector<int32_t> ref_lo, ref_hi, ref_left, ref_right;
-  // pre-order layout (left child first): the visit order of the reference's stack walk
-  std::vector<int32_t> pre_ref, pre_skip, pre_parent, pre_depth;
+// Device counters of a build (one int32 array; graph-capturable: nothing is read back during the build).
+enum {
+  C_NODES = 0,      // nodes created (global ids are dense from 0; some heap slots may stay unused)
+  C_LEVEL,          // nodes in the current top level list
+  C_NEXT,           // nodes in the next top level list
+  C_TILES,          // tiles of the current level
+  C_BIG,            // nodes of the current level sorted by tiles
+  C_BOTTOM,         // subtree roots of the bottom phase
+  C_HEAP,           // heap slots reserved by the bottom phase
+  C_OVERFLOW,       // capacity exceeded (checked by the host after the call)
+  C_FINAL,          // existing nodes (after the finish)
+  C_ITEMS,          // moment items
+  C_TASKS,          // traversal subtrees (all trees)
+  C_WORK,           // traversal work items
+  C_DEPTH,          // largest node depth
+  C_COUNT = 16
 };
 
-int build_tree(Ctx& c, int64_t m, const double* mx, const double* my, const double* mz, int64_t leaf,
-               int32_t* order, HostTree& t);
-int finish_tree(HostTree& t);
+// A device count, or 0 once a capacity has overflowed: every kernel after an overflow does no work, and the
+// host reports the overflow when the call has finished.
+__device__ __forceinline__ int live_count(const int32_t* counters, int which) {
+  return counters[C_OVERFLOW] ? 0 : counters[which];
+}
+
+struct TreeArrays {
+  // build inputs
+  int K = 1;                     // trees in the forest
+  int64_t m = 0;
+  int leaf = 32;
+  int cap = 0;                   // node capacity
+  int tile_cap = 0;
+  int bottom_cap = 0;
+  int32_t* roots_lo = nullptr;   // (K+1) segment offsets of the trees
+  // node table (global ids)
+  int32_t *lo, *hi, *left, *right, *parent, *depth, *tree;
+  int8_t *axis, *split;
+  int32_t* counters;
+  int32_t *level_a, *level_b, *big, *big_t0, *p_flag;
+  int32_t *tile_node, *tile_p0, *tile_p1;
+  double* box_partial;
+  uint64_t *keyA, *keyB;
+  uint32_t *posA, *posB;
+  int32_t* tmp;
+  int32_t *bottom, *heap_base;
+  // finish: pre-order layout
+  int32_t *lo_cnt, *lo_mind, *lo_base, *scan_part;
+  int32_t* pre;                  // node id -> pre-order index
+  int32_t *p_lo, *p_hi, *p_skip, *p_parent, *p_depth, *p_tree, *p_node, *p_irank;
+  int32_t* tree_first;           // (K+1) first pre-order index of each tree
+  int32_t* order = nullptr;      // segment permutation (tree order -> segment)
+};
+
+// Copy host int64 values (each below 2^31) to a device int32 array through kernel arguments: no host
+// memory is read at execution time, so the copy can be captured in a graph.
+int upload_i32(Ctx& c, const std::vector<int64_t>& values, int32_t* dst);
+
+// Exclusive scan of n int32 values (three kernels); the total goes to *total_out when it is not null.
+void scan_exclusive(Ctx& c, int n, const int32_t* in, int32_t* out, int32_t* part, int32_t* total_out);
+
+// Reserve and lay out every array of a build of m segments in K trees with leaf size `leaf`.
+int tree_layout(Ctx& c, int buf, int64_t m, int K, int leaf, TreeArrays& t);
+// Enqueue the whole build (no host synchronisation): midpoints mx/my/mz (m), tree segment offsets on the
+// host (K+1, for the launch bounds) and on the device (t.roots_lo).
+int tree_build(Ctx& c, TreeArrays& t, const double* mx, const double* my, const double* mz,
+               const std::vector<int64_t>& offsets);
+// Reference numbering of the nodes of tree k (host arrays of its node count), from the device layout.
+int tree_reference_arrays(Ctx& c, const TreeArrays& t, std::vector<int64_t>& lo, std::vector<int64_t>& hi,
+                          std::vector<int64_t>& left, std::vector<int64_t>& right, std::vector<int32_t>& ref_of_pre,
+                          int64_t& count);
 
 }  // namespace ofw

@@ -1,5 +1,7 @@
-// C ABI of the tree code: build, node quantities and the tree sum (include/ofw_hip.h).
+// C ABI of the tree code: build, node quantities, the tree sum and the batched (graph-replayed) tree sum
+// (include/ofw_hip.h).
 #include <algorithm>
+#include <climits>
 #include <vector>
 
 #include "common.h"
@@ -33,26 +35,132 @@ __global__ void k_scatter_nodes(int count, int width, const int32_t* __restrict_
   for (int q = 0; q < width; ++q) out[r * width + q] = in[(size_t)i * width + q];
 }
 
+template <typename T>
+__global__ void k_fill(int64_t count, T* __restrict__ a, T v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) a[i] = v;
+}
+
 static size_t al(size_t b) { return (b + 255) & ~(size_t)255; }
 
-// Shared front half of every tree entry point: midpoints, the build, and the tree order.
-static int build_common(Ctx& c, int64_t m, const double* starts, const double* directions, const double* lengths,
-                        int64_t leaf, HostTree& t, int32_t** order_out) {
-  OFW_HIP(c.tree[3].reserve(al((size_t)(m > 0 ? m : 1) * 8) * 3 + al((size_t)(m > 0 ? m : 1) * 4)));
+// Midpoints of the segments (scratch buffer 3 of the context).
+static int midpoints(Ctx& c, int64_t m, const double* starts, const double* directions, const double* lengths,
+                     double** mx, double** my, double** mz) {
+  const size_t M = (size_t)std::max<int64_t>(m, 1);
+  OFW_HIP(c.tree[3].reserve(3 * al(M * 8)));
   char* base = (char*)c.tree[3].ptr;
-  double* mx = (double*)base;
-  double* my = (double*)(base + al((size_t)m * 8));
-  double* mz = (double*)(base + 2 * al((size_t)m * 8));
-  int32_t* order = (int32_t*)(base + 3 * al((size_t)m * 8));
-  int st = tree_prep(c, m, starts, directions, lengths, mx, my, mz);
+  *mx = (double*)base;
+  *my = (double*)(base + al(M * 8));
+  *mz = (double*)(base + 2 * al(M * 8));
+  return tree_prep(c, m, starts, directions, lengths, *mx, *my, *mz);
+}
+
+// One tree sum over K independent snapshots (K = 1 for ofw_tree_sum).  Enqueue only: no host
+// synchronisation, no memset, no allocation once the context's buffers have the sizes of an earlier
+// identical call, so the whole sequence can be captured in a graph.
+struct TreeCall {
+  int precision = OFW_MIXED;
+  int K = 1;
+  int64_t n = 0, m = 0;
+  std::vector<int64_t> toff, soff;
+  const double *points = nullptr, *starts = nullptr, *directions = nullptr, *lengths = nullptr;
+  const double *gamma = nullptr, *radius = nullptr;
+  int model = 0, raise_singular = 0;
+  const int64_t *ex_ptr = nullptr, *ex_seg = nullptr;
+  int64_t n_ex = 0;
+  double theta = 0.5;
+  int order = 2;
+  int leaf = 32;
+  double clearance = 0.0;
+  int split_depth = 0;
+  double* out = nullptr;
+  int8_t* flags = nullptr;
+  int64_t *far_counts = nullptr, *near_counts = nullptr;
+};
+
+static int tree_pipeline(Ctx& c, const TreeCall& tc, TreeArrays& t) {
+  int32_t *p32 = nullptr, *s32 = nullptr;
+  if (tc.ex_ptr && tc.n_ex > 0) {
+    int st = csr_convert(c, tc.n, tc.n_ex, tc.ex_ptr, tc.ex_seg, &p32, &s32);
+    if (st) return st;
+  }
+  OFW_HIP(c.mark());
+  double *mx, *my, *mz;
+  int st = midpoints(c, tc.m, tc.starts, tc.directions, tc.lengths, &mx, &my, &mz);
   if (st) return st;
-  st = build_tree(c, m, mx, my, mz, leaf, order, t);
+  st = tree_layout(c, 0, tc.m, tc.K, tc.leaf, t);
   if (st) return st;
-  *order_out = order;
+  st = upload_i32(c, tc.soff, t.roots_lo);
+  if (st) return st;
+  st = tree_build(c, t, mx, my, mz, tc.soff);
+  if (st) return st;
+  OFW_HIP(c.mark());
+  TreeDevice td;
+  st = tree_moments(c, 1, t, tc.starts, tc.directions, tc.lengths, tc.gamma, tc.radius, tc.model, tc.clearance, td,
+                    false);
+  if (st) return st;
+  OFW_HIP(c.mark());
+  EvalShape shape;
+  shape.K = tc.K;
+  shape.n = tc.n;
+  shape.target_off = tc.toff;
+  shape.seg_off = tc.soff;
+  shape.split_depth = tc.split_depth;
+  st = tree_evaluate(c, 2, t, td, shape, tc.precision, tc.points, tc.model, tc.raise_singular, p32, s32,
+                     p32 ? tc.n_ex : 0, tc.theta, tc.order, tc.out, tc.flags, tc.far_counts, tc.near_counts);
+  if (st) return st;
+  OFW_HIP(c.mark());
   return OFW_OK;
 }
 
+// After the stream has finished: the build's counters, and an error when a capacity was exceeded.
+static int read_counters(Ctx& c, const TreeArrays& t, int32_t* counters) {
+  OFW_HIP(hipMemcpyAsync(counters, t.counters, C_COUNT * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+  OFW_HIP(hipStreamSynchronize(c.stream));
+  if (counters[C_OVERFLOW]) {
+    set_error("tree code exceeded a capacity of its build or traversal plan");
+    return OFW_ERR_OVERFLOW;
+  }
+  c.stats_i[2] = counters[C_TASKS];
+  c.stats_i[4] = counters[C_FINAL];
+  c.stats_i[5] = counters[C_DEPTH];
+  return OFW_OK;
+}
+
+static int check_tree_args(const char* what, int precision, int64_t n, int64_t m, int64_t leaf_size, double theta,
+                           int expansion_order, double clearance, int model, int split_depth) {
+  if (precision < 0 || precision > 2 || n < 0 || m < 0 || leaf_size < 1 || leaf_size > (1 << 20) ||
+      expansion_order < 0 || expansion_order > 2 || !(theta > 0.0 && theta < 1.0) || !(clearance >= 0.0) ||
+      n > ((int64_t)1 << 30) || m > ((int64_t)1 << 28) || split_depth < -1) {
+    set_error(std::string(what) + ": invalid arguments");
+    return OFW_ERR_ARG;
+  }
+  if (model < 0 || model > 2) {
+    set_error("tree backend supports core models NONE, RANKINE and LAMB_OSEEN only");
+    return OFW_ERR_UNSUPPORTED;
+  }
+  return OFW_OK;
+}
+
+static int64_t target_groups(const std::vector<int64_t>& toff) {
+  int64_t g = 0;
+  for (size_t k = 0; k + 1 < toff.size(); ++k) g += (toff[k + 1] - toff[k] + 63) / 64;
+  return g;
+}
+
 }  // namespace ofw
+
+// The batched plan: its own scratch (so that the recorded graph's buffers stay valid), the call it replays,
+// and the graph.
+struct ofw_tree_batch {
+  ofw::Ctx c;
+  ofw::TreeCall call;
+  ofw::TreeArrays t;
+  hipStream_t user_stream = nullptr;
+  hipStream_t capture_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
 
 using ofw::set_error;
 
@@ -60,39 +168,50 @@ extern "C" {
 
 int ofw_build_tree(ofw_ctx* x, int64_t m, const double* mid, int64_t leaf_size, int64_t* order, int64_t* lo,
                    int64_t* hi, int64_t* left, int64_t* right, int64_t* count_host) {
-  if (!x || m < 0 || leaf_size < 1 || m > ((int64_t)1 << 30) || !count_host) {
+  if (!x || m < 0 || leaf_size < 1 || leaf_size > (1 << 20) || m > ((int64_t)1 << 28) || !count_host) {
     set_error("ofw_build_tree: invalid arguments");
     return OFW_ERR_ARG;
   }
   ofw::Ctx& c = x->c;
   c.nev = 0;
-  OFW_HIP(c.tree[3].reserve(ofw::al((size_t)(m > 0 ? m : 1) * 8) * 3 + ofw::al((size_t)(m > 0 ? m : 1) * 4)));
+  if (m == 0) {
+    // the reference's tree of no segments: one leaf root over the empty range
+    const int64_t root[4] = {0, 0, -1, -1};
+    OFW_HIP(hipMemcpyAsync(lo, &root[0], 8, hipMemcpyHostToDevice, c.stream));
+    OFW_HIP(hipMemcpyAsync(hi, &root[1], 8, hipMemcpyHostToDevice, c.stream));
+    OFW_HIP(hipMemcpyAsync(left, &root[2], 8, hipMemcpyHostToDevice, c.stream));
+    OFW_HIP(hipMemcpyAsync(right, &root[3], 8, hipMemcpyHostToDevice, c.stream));
+    OFW_HIP(hipStreamSynchronize(c.stream));
+    *count_host = 1;
+    return OFW_OK;
+  }
+  OFW_HIP(c.tree[3].reserve(3 * ofw::al((size_t)m * 8)));
   char* base = (char*)c.tree[3].ptr;
   double* mx = (double*)base;
   double* my = (double*)(base + ofw::al((size_t)m * 8));
   double* mz = (double*)(base + 2 * ofw::al((size_t)m * 8));
-  int32_t* ord32 = (int32_t*)(base + 3 * ofw::al((size_t)m * 8));
-  if (m > 0)
-    hipLaunchKernelGGL(ofw::k_aos_to_soa, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, c.stream, (int)m, mid, mx,
-                       my, mz);
-  ofw::HostTree t;
-  int st = ofw::build_tree(c, m, mx, my, mz, leaf_size, ord32, t);
+  hipLaunchKernelGGL(ofw::k_aos_to_soa, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, c.stream, (int)m, mid, mx, my,
+                     mz);
+  ofw::TreeArrays t;
+  const std::vector<int64_t> offsets = {0, m};
+  int st = ofw::tree_layout(c, 0, m, 1, (int)leaf_size, t);
   if (st) return st;
-  if (m > 0)
-    hipLaunchKernelGGL(ofw::k_i32_to_i64, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, c.stream, (int)m, ord32,
-                       order);
-  const int64_t count = t.count;
-  std::vector<int64_t> buf((size_t)count * 4);
-  for (int64_t i = 0; i < count; ++i) {
-    buf[i] = t.ref_lo[i];
-    buf[count + i] = t.ref_hi[i];
-    buf[2 * count + i] = t.ref_left[i];
-    buf[3 * count + i] = t.ref_right[i];
-  }
-  OFW_HIP(hipMemcpyAsync(lo, buf.data(), count * 8, hipMemcpyHostToDevice, c.stream));
-  OFW_HIP(hipMemcpyAsync(hi, buf.data() + count, count * 8, hipMemcpyHostToDevice, c.stream));
-  OFW_HIP(hipMemcpyAsync(left, buf.data() + 2 * count, count * 8, hipMemcpyHostToDevice, c.stream));
-  OFW_HIP(hipMemcpyAsync(right, buf.data() + 3 * count, count * 8, hipMemcpyHostToDevice, c.stream));
+  st = ofw::upload_i32(c, offsets, t.roots_lo);
+  if (st) return st;
+  st = ofw::tree_build(c, t, mx, my, mz, offsets);
+  if (st) return st;
+  hipLaunchKernelGGL(ofw::k_i32_to_i64, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, c.stream, (int)m, t.order,
+                     order);
+  OFW_HIP(hipGetLastError());
+  std::vector<int64_t> vlo, vhi, vleft, vright;
+  std::vector<int32_t> ref_of_pre;
+  int64_t count = 0;
+  st = ofw::tree_reference_arrays(c, t, vlo, vhi, vleft, vright, ref_of_pre, count);
+  if (st) return st;
+  OFW_HIP(hipMemcpyAsync(lo, vlo.data(), count * 8, hipMemcpyHostToDevice, c.stream));
+  OFW_HIP(hipMemcpyAsync(hi, vhi.data(), count * 8, hipMemcpyHostToDevice, c.stream));
+  OFW_HIP(hipMemcpyAsync(left, vleft.data(), count * 8, hipMemcpyHostToDevice, c.stream));
+  OFW_HIP(hipMemcpyAsync(right, vright.data(), count * 8, hipMemcpyHostToDevice, c.stream));
   OFW_HIP(hipStreamSynchronize(c.stream));
   *count_host = count;
   return OFW_OK;
@@ -102,28 +221,40 @@ int ofw_tree_nodes(ofw_ctx* x, int64_t m, const double* starts, const double* di
                    const double* gamma, const double* radius, int64_t leaf_size, double* centre, double* bound,
                    double* core, double* m0, double* m1, double* m2, double* w1, double* t2, double* w2,
                    int64_t* count_host) {
-  if (!x || m < 1 || leaf_size < 1 || m > ((int64_t)1 << 30) || !count_host) {
+  if (!x || m < 1 || leaf_size < 1 || leaf_size > (1 << 20) || m > ((int64_t)1 << 28) || !count_host) {
     set_error("ofw_tree_nodes: invalid arguments");
     return OFW_ERR_ARG;
   }
   ofw::Ctx& c = x->c;
   c.nev = 0;
-  ofw::HostTree t;
-  int32_t* order = nullptr;
-  int st = ofw::build_common(c, m, starts, directions, lengths, leaf_size, t, &order);
+  double *mx, *my, *mz;
+  int st = ofw::midpoints(c, m, starts, directions, lengths, &mx, &my, &mz);
   if (st) return st;
-  const int count = (int)t.count;
+  ofw::TreeArrays t;
+  const std::vector<int64_t> offsets = {0, m};
+  st = ofw::tree_layout(c, 0, m, 1, (int)leaf_size, t);
+  if (st) return st;
+  st = ofw::upload_i32(c, offsets, t.roots_lo);
+  if (st) return st;
+  st = ofw::tree_build(c, t, mx, my, mz, offsets);
+  if (st) return st;
+  std::vector<int64_t> vlo, vhi, vleft, vright;
+  std::vector<int32_t> ref_of_pre;
+  int64_t count64 = 0;
+  st = ofw::tree_reference_arrays(c, t, vlo, vhi, vleft, vright, ref_of_pre, count64);
+  if (st) return st;
+  const int count = (int)count64;
   // pre-order outputs in a scratch block, then scattered to the reference numbering
   const int widths[9] = {3, 1, 1, 3, 9, 27, 3, 3, 9};
   double* outs[9] = {centre, bound, core, m0, m1, m2, w1, t2, w2};
   size_t total = 0;
-  for (int w : widths) total += ofw::al((size_t)count * w * 8);
+  for (int w : widths) total += ofw::al((size_t)t.cap * w * 8);
   OFW_HIP(c.tree[4].reserve(total + ofw::al((size_t)count * 4)));
   char* base = (char*)c.tree[4].ptr;
   double* pre[9];
   for (int q = 0; q < 9; ++q) {
     pre[q] = (double*)base;
-    base += ofw::al((size_t)count * widths[q] * 8);
+    base += ofw::al((size_t)t.cap * widths[q] * 8);
   }
   int32_t* d_pre_ref = (int32_t*)base;
   ofw::TreeDevice td;
@@ -134,16 +265,18 @@ int ofw_tree_nodes(ofw_ctx* x, int64_t m, const double* starts, const double* di
   td.out.w1 = pre[6];
   td.out.t2 = pre[7];
   td.out.w2 = pre[8];
-  st = ofw::tree_moments(c, t, order, starts, directions, lengths, gamma, radius, OFW_MODEL_NONE, 0.0, td, true);
+  st = ofw::tree_moments(c, 1, t, starts, directions, lengths, gamma, radius, OFW_MODEL_NONE, 0.0, td, true);
   if (st) return st;
   OFW_HIP(hipMemcpyAsync(pre[0], td.centre, (size_t)count * 3 * 8, hipMemcpyDeviceToDevice, c.stream));
   OFW_HIP(hipMemcpyAsync(pre[2], td.core, (size_t)count * 8, hipMemcpyDeviceToDevice, c.stream));
-  OFW_HIP(hipMemcpyAsync(d_pre_ref, t.pre_ref.data(), (size_t)count * 4, hipMemcpyHostToDevice, c.stream));
+  OFW_HIP(hipMemcpyAsync(d_pre_ref, ref_of_pre.data(), (size_t)count * 4, hipMemcpyHostToDevice, c.stream));
   for (int q = 0; q < 9; ++q)
     hipLaunchKernelGGL(ofw::k_scatter_nodes, dim3((count + 127) / 128), dim3(128), 0, c.stream, count, widths[q],
                        d_pre_ref, pre[q], outs[q]);
   OFW_HIP(hipGetLastError());
-  OFW_HIP(hipStreamSynchronize(c.stream));
+  int32_t counters[ofw::C_COUNT];
+  st = ofw::read_counters(c, t, counters);
+  if (st) return st;
   *count_host = count;
   return OFW_OK;
 }
@@ -153,48 +286,243 @@ int ofw_tree_sum(ofw_ctx* x, int precision, int64_t n, const double* points, int
                  int model, int raise_singular, const int64_t* ex_ptr, const int64_t* ex_seg, double theta,
                  int expansion_order, int64_t leaf_size, double clearance, int split_depth, double* out,
                  int8_t* flags, int64_t* far_counts, int64_t* near_counts) {
-  if (!x || precision < 0 || precision > 2 || n < 0 || m < 0 || leaf_size < 1 || expansion_order < 0 ||
-      expansion_order > 2 || !(theta > 0.0 && theta < 1.0) || !(clearance >= 0.0) || n > ((int64_t)1 << 30) ||
-      m > ((int64_t)1 << 28)) {
-    set_error("ofw_tree_sum: invalid arguments");
+  if (!x) {
+    set_error("null context");
     return OFW_ERR_ARG;
   }
-  if (model < 0 || model > 2) {
-    set_error("tree backend supports core models NONE, RANKINE and LAMB_OSEEN only");
-    return OFW_ERR_UNSUPPORTED;
-  }
+  int st = ofw::check_tree_args("ofw_tree_sum", precision, n, m, leaf_size, theta, expansion_order, clearance, model,
+                                split_depth);
+  if (st) return st;
   ofw::Ctx& c = x->c;
   c.nev = 0;
   if (n == 0) return OFW_OK;
   if (m == 0) {
-    OFW_HIP(hipMemsetAsync(out, 0, (size_t)n * 3 * 8, c.stream));
-    OFW_HIP(hipMemsetAsync(flags, 0, (size_t)n, c.stream));
-    if (far_counts) OFW_HIP(hipMemsetAsync(far_counts, 0, (size_t)n * 8, c.stream));
-    if (near_counts) OFW_HIP(hipMemsetAsync(near_counts, 0, (size_t)n * 8, c.stream));
+    const unsigned g3 = (unsigned)((3 * n + 255) / 256), g1 = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(ofw::k_fill<double>, dim3(g3), dim3(256), 0, c.stream, 3 * n, out, 0.0);
+    hipLaunchKernelGGL(ofw::k_fill<int8_t>, dim3(g1), dim3(256), 0, c.stream, n, flags, (int8_t)0);
+    if (far_counts) hipLaunchKernelGGL(ofw::k_fill<int64_t>, dim3(g1), dim3(256), 0, c.stream, n, far_counts, (int64_t)0);
+    if (near_counts)
+      hipLaunchKernelGGL(ofw::k_fill<int64_t>, dim3(g1), dim3(256), 0, c.stream, n, near_counts, (int64_t)0);
+    OFW_HIP(hipGetLastError());
     OFW_HIP(hipStreamSynchronize(c.stream));
     return OFW_OK;
   }
-  OFW_HIP(c.mark());
-  ofw::HostTree t;
-  int32_t* order = nullptr;
-  int st = ofw::build_common(c, m, starts, directions, lengths, leaf_size, t, &order);
+  ofw::TreeCall tc;
+  tc.precision = precision;
+  tc.K = 1;
+  tc.n = n;
+  tc.m = m;
+  tc.toff = {0, n};
+  tc.soff = {0, m};
+  tc.points = points;
+  tc.starts = starts;
+  tc.directions = directions;
+  tc.lengths = lengths;
+  tc.gamma = gamma;
+  tc.radius = radius;
+  tc.model = model;
+  tc.raise_singular = raise_singular;
+  tc.ex_ptr = ex_ptr;
+  tc.ex_seg = ex_seg;
+  if (ex_ptr) {
+    st = ofw::csr_count(c, n, ex_ptr, &tc.n_ex);
+    if (st) return st;
+  }
+  tc.theta = theta;
+  tc.order = expansion_order;
+  tc.leaf = (int)leaf_size;
+  tc.clearance = clearance;
+  tc.split_depth = split_depth >= 0 ? split_depth : ofw::choose_split_depth(c, ofw::target_groups(tc.toff), 1);
+  tc.out = out;
+  tc.flags = flags;
+  tc.far_counts = far_counts;
+  tc.near_counts = near_counts;
+  ofw::TreeArrays t;
+  st = ofw::tree_pipeline(c, tc, t);
   if (st) return st;
-  OFW_HIP(c.mark());
-  ofw::TreeDevice td;
-  st = ofw::tree_moments(c, t, order, starts, directions, lengths, gamma, radius, model, clearance, td, false);
+  int32_t counters[ofw::C_COUNT];
+  return ofw::read_counters(c, t, counters);
+}
+
+int ofw_tree_batch_create(ofw_ctx* x, ofw_tree_batch** plan, int precision, int64_t K,
+                          const int64_t* target_offsets_host, const int64_t* segment_offsets_host,
+                          const double* points, const double* starts, const double* directions,
+                          const double* lengths, const double* gamma, const double* radius, int model,
+                          int raise_singular, const int64_t* ex_ptr, const int64_t* ex_seg, double theta,
+                          int expansion_order, int64_t leaf_size, double clearance, int split_depth, int use_graph,
+                          double* out, int8_t* flags, int64_t* far_counts, int64_t* near_counts) {
+  if (!x || !plan || K < 1 || K > (1 << 20) || !target_offsets_host || !segment_offsets_host) {
+    set_error("ofw_tree_batch_create: invalid arguments");
+    return OFW_ERR_ARG;
+  }
+  *plan = nullptr;
+  std::vector<int64_t> toff(target_offsets_host, target_offsets_host + K + 1);
+  std::vector<int64_t> soff(segment_offsets_host, segment_offsets_host + K + 1);
+  if (toff[0] != 0 || soff[0] != 0) {
+    set_error("ofw_tree_batch_create: offsets must start at 0");
+    return OFW_ERR_ARG;
+  }
+  for (int64_t k = 0; k < K; ++k)
+    if (toff[k + 1] < toff[k] || soff[k + 1] < soff[k]) {
+      set_error("ofw_tree_batch_create: offsets must not decrease");
+      return OFW_ERR_ARG;
+    }
+  const int64_t n = toff[K], m = soff[K];
+  int st = ofw::check_tree_args("ofw_tree_batch_create", precision, n, m, leaf_size, theta, expansion_order,
+                                clearance, model, split_depth);
   if (st) return st;
-  OFW_HIP(c.mark());
-  int32_t *p32 = nullptr, *s32 = nullptr;
-  st = ofw::csr_to_i32(c, n, ex_ptr, ex_seg, &p32, &s32);
-  if (st) return st;
-  std::vector<ofw::EvalTask> tasks;
-  st = ofw::tree_evaluate(c, t, td, order, precision, n, points, model, raise_singular, p32, s32, theta,
-                          expansion_order, split_depth, out, flags, far_counts, near_counts, tasks);
-  if (st) return st;
-  OFW_HIP(c.mark());
-  OFW_HIP(hipStreamSynchronize(c.stream));
-  c.stats_i[4] = t.count;
-  c.stats_i[5] = t.depth;
+  if (n == 0 || m == 0) {
+    set_error("ofw_tree_batch_create: the batch needs targets and segments");
+    return OFW_ERR_ARG;
+  }
+  ofw_tree_batch* b = new ofw_tree_batch();
+  ofw::Ctx& c = b->c;
+  c.device = x->c.device;
+  c.cu_count = x->c.cu_count;
+  c.stream = x->c.stream;
+  b->user_stream = x->c.stream;
+  auto fail = [&](int code) {
+    ofw_tree_batch_destroy(b);
+    return code;
+  };
+  for (auto& e : c.ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      set_error("ofw_tree_batch_create: hipEventCreate failed");
+      return fail(OFW_ERR_HIP);
+    }
+  ofw::TreeCall& tc = b->call;
+  tc.precision = precision;
+  tc.K = (int)K;
+  tc.n = n;
+  tc.m = m;
+  tc.toff = toff;
+  tc.soff = soff;
+  tc.points = points;
+  tc.starts = starts;
+  tc.directions = directions;
+  tc.lengths = lengths;
+  tc.gamma = gamma;
+  tc.radius = radius;
+  tc.model = model;
+  tc.raise_singular = raise_singular;
+  tc.ex_ptr = ex_ptr;
+  tc.ex_seg = ex_seg;
+  if (ex_ptr) {
+    st = ofw::csr_count(c, n, ex_ptr, &tc.n_ex);
+    if (st) return fail(st);
+  }
+  tc.theta = theta;
+  tc.order = expansion_order;
+  tc.leaf = (int)leaf_size;
+  tc.clearance = clearance;
+  tc.split_depth = split_depth >= 0 ? split_depth : ofw::choose_split_depth(c, ofw::target_groups(toff), (int)K);
+  tc.out = out;
+  tc.flags = flags;
+  tc.far_counts = far_counts;
+  tc.near_counts = near_counts;
+  // one eager call sizes every buffer and checks the capacities
+  c.nev = 0;
+  st = ofw::tree_pipeline(c, tc, b->t);
+  if (st) return fail(st);
+  int32_t counters[ofw::C_COUNT];
+  st = ofw::read_counters(c, b->t, counters);
+  if (st) return fail(st);
+  if (use_graph) {
+    if (hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking) != hipSuccess) {
+      set_error("ofw_tree_batch_create: hipStreamCreate failed");
+      return fail(OFW_ERR_HIP);
+    }
+    c.stream = b->capture_stream;
+    c.capturing = true;
+    if (hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      set_error("ofw_tree_batch_create: hipStreamBeginCapture failed");
+      return fail(OFW_ERR_HIP);
+    }
+    const int st_rec = ofw::tree_pipeline(c, tc, b->t);
+    const hipError_t e_end = hipStreamEndCapture(c.stream, &b->graph);
+    c.capturing = false;
+    c.stream = b->user_stream;
+    if (st_rec) return fail(st_rec);
+    if (e_end != hipSuccess) {
+      set_error(std::string("ofw_tree_batch_create: hipStreamEndCapture failed: ") + hipGetErrorString(e_end));
+      return fail(OFW_ERR_HIP);
+    }
+    const hipError_t e_inst = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    if (e_inst != hipSuccess) {
+      set_error(std::string("ofw_tree_batch_create: hipGraphInstantiate failed: ") + hipGetErrorString(e_inst));
+      return fail(OFW_ERR_HIP);
+    }
+  }
+  *plan = b;
+  return OFW_OK;
+}
+
+int ofw_tree_batch_run(ofw_tree_batch* b) {
+  if (!b) {
+    set_error("ofw_tree_batch_run: null plan");
+    return OFW_ERR_ARG;
+  }
+  ofw::Ctx& c = b->c;
+  c.stream = b->user_stream;
+  c.nev = 0;
+  if (b->exec) {
+    OFW_HIP(c.mark());
+    OFW_HIP(hipGraphLaunch(b->exec, c.stream));
+    OFW_HIP(c.mark());
+  } else {
+    int st = ofw::tree_pipeline(c, b->call, b->t);
+    if (st) return st;
+  }
+  int32_t counters[ofw::C_COUNT];
+  return ofw::read_counters(c, b->t, counters);
+}
+
+int ofw_tree_batch_timings(ofw_tree_batch* b, float* ms_host, int count) {
+  if (!b || !ms_host) {
+    set_error("ofw_tree_batch_timings: null argument");
+    return OFW_ERR_ARG;
+  }
+  ofw::Ctx& c = b->c;
+  const int phases = c.nev > 1 ? c.nev - 1 : 0;
+  for (int k = 0; k < count; ++k) {
+    ms_host[k] = 0.0f;
+    if (k < phases) OFW_HIP(hipEventElapsedTime(&ms_host[k], c.ev[k], c.ev[k + 1]));
+  }
+  return phases;
+}
+
+int ofw_tree_batch_stats(ofw_tree_batch* b, int64_t* stats_host, int count) {
+  if (!b || !stats_host) {
+    set_error("ofw_tree_batch_stats: null argument");
+    return OFW_ERR_ARG;
+  }
+  int k = 0;
+  for (; k < count && k < 8; ++k) stats_host[k] = b->c.stats_i[k];
+  return k;
+}
+
+int ofw_tree_batch_destroy(ofw_tree_batch* b) {
+  if (!b) return OFW_OK;
+  ofw::Ctx& c = b->c;
+  if (b->user_stream) (void)hipStreamSynchronize(b->user_stream);
+  else (void)hipDeviceSynchronize();
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->capture_stream) {
+    (void)hipStreamSynchronize(b->capture_stream);
+    (void)hipStreamDestroy(b->capture_stream);
+  }
+  c.rec64.release();
+  c.rec32.release();
+  c.part.release();
+  c.pflag.release();
+  c.expos.release();
+  c.exptr32.release();
+  c.exseg32.release();
+  for (auto& buf : c.tree) buf.release();
+  for (auto& e : c.ev)
+    if (e) (void)hipEventDestroy(e);
+  delete b;
   return OFW_OK;
 }
 

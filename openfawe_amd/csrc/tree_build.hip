@@ -1,54 +1,167 @@
 // Median-split binary tree over segment midpoints, built on the GPU element for element like the
-// reference (vortex_tree.py:51-115 serial, :170-306 level build + renumbering).
+// reference (vortex_tree.py:51-115 serial build, :170-306 level build + renumbering), for one tree or a
+// forest of trees over disjoint segment ranges.  The whole build is enqueued without any host
+// synchronisation (fixed launch sequence, counts kept on the device), so it can be captured in a graph.
 //
-// Level-synchronous build.  For every node of a level with more than `leaf` segments:
-//   1. box of its midpoints -> split axis (longest extent, the lower axis on ties) and split flag
-//      (extent > 0), exactly the reference's comparisons;
-//   2. stable sort of the node's range of `order` by the midpoint coordinate on that axis.  A stable sort
-//      by key equals an ordinary sort by the unique pair (key, position), so the sort runs on
-//      (order-preserving 64-bit key of the coordinate with -0.0 folded into +0.0, position) pairs:
-//      bitonic networks in LDS for tiles of up to 4,096 elements, merge-path merges between tiles;
-//   3. the children [a, (a+b)//2) and [(a+b)//2, b).
-// The host keeps the node table (integer bookkeeping of a few arrays per level) and numbers the nodes
-// like the reference's _renumber at the end.
+// Every node with more than `leaf` segments: the box of its midpoints gives the split axis (longest extent,
+// the lower axis on ties) and the split flag (extent > 0), with the reference's comparisons; its range of
+// `order` is stably sorted by the midpoint coordinate on that axis; the range is cut at (a+b)//2.  A stable
+// sort by key equals an ordinary sort by the unique pair (key, position), with the key the order-preserving
+// 64-bit image of the double and -0.0 folded into +0.0 (np.argsort compares with <).
+//
+//  * Top phase, nodes of more than S_BOTTOM segments, level by level: box reduction over tiles, LDS
+//    bitonic sort of 8,192-element tiles, merge-path passes between tiles.  A one-workgroup plan kernel
+//    lists each level's nodes, tiles and deferred subtrees; a one-workgroup kernel creates the children.
+//  * Bottom phase: each node of at most S_BOTTOM segments is finished by one workgroup entirely in LDS,
+//    level after level (box, composite (node, key, position) bitonic sort, children), into a heap block.
+//  * Finish: in a median-split tree the pre-order (the visit order of the reference's stack walk) is the
+//    order of the nodes by (lo, depth), so pre(N) = #nodes with lo < N.lo + depth(N) - min depth at N.lo and
+//    the escape index is #nodes with lo < N.hi: a histogram over lo and one scan give the whole layout.
 #include <algorithm>
 #include <climits>
 #include <vector>
 
 #include "common.h"
 #include "internal.h"
+#include "scan.h"
 #include "tree.h"
 
 namespace ofw {
 
-static constexpr int TILE = 4096;       // elements per LDS sort tile
-static constexpr int SORT_THREADS = 256;
+static constexpr int TILE = 8192;           // elements per LDS tile of the top phase
+static constexpr int TILE_THREADS = 1024;
+static constexpr int S_BOTTOM = 2048;       // largest subtree finished in LDS
+static constexpr int BOTTOM_THREADS = 512;
+static constexpr int HCAP = 2 * S_BOTTOM - 1;
+static constexpr int PLAN_THREADS = 1024;
 
 __device__ __forceinline__ uint64_t ordered_key(double x) {
-  // order-preserving map of a finite double to uint64, with -0.0 == +0.0 (np.argsort compares with <)
   uint64_t u = (x == 0.0) ? 0ull : (uint64_t)__double_as_longlong(x);
   return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
 }
 
+__host__ __device__ __forceinline__ int heap_slots(int size, int leaf) {
+  int d = 0;
+  while (size > leaf) {
+    size = (size + 1) / 2;
+    ++d;
+  }
+  return (2 << d) - 1;
+}
+
+// ------------------------------------------------------------------ init
 __global__ void k_iota(int m, int32_t* __restrict__ order) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < m) order[i] = i;
 }
 
-// ------------------------------------------------------------------ node boxes
-struct Box {
-  double lo[3], hi[3];
+__global__ void k_init_roots(TreeArrays t) {
+  for (int k = threadIdx.x; k < t.K; k += blockDim.x) {
+    // the root of an empty tree is no node (lo = -1): it takes no place in the pre-order layout
+    const bool empty = t.roots_lo[k] == t.roots_lo[k + 1];
+    t.lo[k] = empty ? -1 : t.roots_lo[k];
+    t.hi[k] = empty ? -1 : t.roots_lo[k + 1];
+    t.left[k] = -1;
+    t.right[k] = -1;
+    t.parent[k] = -1;
+    t.depth[k] = 0;
+    t.tree[k] = k;
+    t.level_a[k] = k;
+  }
+  if (threadIdx.x < C_COUNT) t.counters[threadIdx.x] = 0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    t.counters[C_NODES] = t.K;
+    t.counters[C_LEVEL] = t.K;
+  }
+}
+
+__global__ void k_fill_i32(int count, int32_t* __restrict__ a, int32_t v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) a[i] = v;
+}
+
+struct I32Chunk {
+  int count;
+  int32_t v[240];
 };
 
+__global__ void k_store_chunk(I32Chunk ch, int32_t* __restrict__ dst) {
+  if ((int)threadIdx.x < ch.count) dst[threadIdx.x] = ch.v[threadIdx.x];
+}
+
+int upload_i32(Ctx& c, const std::vector<int64_t>& values, int32_t* dst) {
+  const int total = (int)values.size();
+  for (int base = 0; base < total; base += 240) {
+    I32Chunk ch{};
+    ch.count = std::min(240, total - base);
+    for (int q = 0; q < ch.count; ++q) ch.v[q] = (int32_t)values[base + q];
+    hipLaunchKernelGGL(k_store_chunk, dim3(1), dim3(256), 0, c.stream, ch, dst + base);
+  }
+  OFW_HIP(hipGetLastError());
+  return OFW_OK;
+}
+
+// ------------------------------------------------------------------ top phase
+// One workgroup: nodes of the level with more than S_BOTTOM (and `leaf`) segments get tiles; nodes with
+// leaf < size <= S_BOTTOM are deferred to the bottom phase.
+__global__ __launch_bounds__(PLAN_THREADS) void k_level_plan(TreeArrays t, const int32_t* __restrict__ level) {
+  const int count = live_count(t.counters, C_LEVEL);
+  int nbig = 0, ntiles = 0, nsmall = 0;
+  const int bottom0 = t.counters[C_BOTTOM];
+  for (int base = 0; base < count; base += blockDim.x) {
+    const int i = base + threadIdx.x;
+    int node = -1, size = 0;
+    if (i < count) {
+      node = level[i];
+      size = t.hi[node] - t.lo[node];
+    }
+    const bool big = node >= 0 && size > S_BOTTOM && size > t.leaf;
+    const bool small = node >= 0 && size > t.leaf && size <= S_BOTTOM;
+    const int tiles = big ? (size + TILE - 1) / TILE : 0;
+    int tb, tt, ts;
+    const int rb = block_exclusive_scan(big ? 1 : 0, &tb);
+    const int rt = block_exclusive_scan(tiles, &tt);
+    const int rs = block_exclusive_scan(small ? 1 : 0, &ts);
+    if (big) {
+      const int bi = nbig + rb;
+      const int t0 = ntiles + rt;
+      if (bi < t.cap && t0 + tiles <= t.tile_cap) {
+        t.big[bi] = node;
+        t.big_t0[bi] = t0;  // first tile of this big node
+        for (int q = 0; q < tiles; ++q) {
+          t.tile_node[t0 + q] = node;
+          t.tile_p0[t0 + q] = t.lo[node] + q * TILE;
+          t.tile_p1[t0 + q] = min(t.hi[node], t.lo[node] + (q + 1) * TILE);
+        }
+      } else {
+        t.counters[C_OVERFLOW] = 1;
+      }
+    }
+    if (small) {
+      const int b = bottom0 + nsmall + rs;
+      if (b < t.bottom_cap) t.bottom[b] = node;
+      else t.counters[C_OVERFLOW] = 1;
+    }
+    nbig += tb;
+    ntiles += tt;
+    nsmall += ts;
+  }
+  if (threadIdx.x == 0) {
+    t.counters[C_BIG] = nbig;
+    t.counters[C_TILES] = ntiles;
+    t.counters[C_BOTTOM] = bottom0 + nsmall;
+  }
+}
+
 __device__ __forceinline__ void box_wave_reduce(double* v) {
-  // v[0..2] mins, v[3..5] maxes; butterfly over the 64 lanes (min/max are exact in any order)
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      double o = __shfl_xor(v[a], off, 64);
+      const double o = __shfl_xor(v[a], off, 64);
       v[a] = o < v[a] ? o : v[a];
-      double q = __shfl_xor(v[3 + a], off, 64);
+      const double q = __shfl_xor(v[3 + a], off, 64);
       v[3 + a] = q > v[3 + a] ? q : v[3 + a];
     }
   }
@@ -59,7 +172,7 @@ __device__ __forceinline__ void choose_axis(const double* v, int8_t* axis_out, i
   int axis = 0;
   double extent = v[3] - v[0];
   for (int c = 1; c < 3; ++c) {
-    double e = v[3 + c] - v[c];
+    const double e = v[3 + c] - v[c];
     if (e > extent) {
       extent = e;
       axis = c;
@@ -69,509 +182,702 @@ __device__ __forceinline__ void choose_axis(const double* v, int8_t* axis_out, i
   *split_out = (int8_t)(extent > 0.0 ? 1 : 0);
 }
 
-// one wavefront per node (nodes of at most TILE segments)
-__global__ __launch_bounds__(256) void k_box_small(int count, const int32_t* __restrict__ list,
-                                                    const int32_t* __restrict__ clo, const int32_t* __restrict__ chi,
-                                                    const int32_t* __restrict__ order, const double* __restrict__ mx,
-                                                    const double* __restrict__ my, const double* __restrict__ mz,
-                                                    int8_t* __restrict__ axis, int8_t* __restrict__ split) {
-  const int w = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
-  const int lane = threadIdx.x & 63;
-  if (w >= count) return;
-  const int node = list[w];
-  const int a = clo[node], b = chi[node];
-  double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-  for (int p = a + lane; p < b; p += 64) {
-    const int s = order[p];
-    const double c[3] = {mx[s], my[s], mz[s]};
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      v[q] = c[q] < v[q] ? c[q] : v[q];
-      v[3 + q] = c[q] > v[3 + q] ? c[q] : v[3 + q];
-    }
-  }
-  box_wave_reduce(v);
-  if (lane == 0) choose_axis(v, &axis[node], &split[node]);
-}
-
-// big nodes: one workgroup per tile, a partial box per tile
-__global__ __launch_bounds__(256) void k_box_tiles(const int32_t* __restrict__ tile_node,
-                                                    const int32_t* __restrict__ tile_p0,
-                                                    const int32_t* __restrict__ tile_p1,
-                                                    const int32_t* __restrict__ order, const double* __restrict__ mx,
-                                                    const double* __restrict__ my, const double* __restrict__ mz,
-                                                    double* __restrict__ partial) {
+__global__ __launch_bounds__(256) void k_box_tiles(TreeArrays t, const double* __restrict__ mx,
+                                                   const double* __restrict__ my, const double* __restrict__ mz) {
   __shared__ double red[4][6];
-  const int t = blockIdx.x;
-  const int p0 = tile_p0[t], p1 = tile_p1[t];
-  double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-  for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-    const int s = order[p];
-    const double c[3] = {mx[s], my[s], mz[s]};
+  const int ntiles = live_count(t.counters, C_TILES);
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int p0 = t.tile_p0[tile], p1 = t.tile_p1[tile];
+    double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+      const int s = t.order[p];
+      const double c[3] = {mx[s], my[s], mz[s]};
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      v[q] = c[q] < v[q] ? c[q] : v[q];
-      v[3 + q] = c[q] > v[3 + q] ? c[q] : v[3 + q];
-    }
-  }
-  box_wave_reduce(v);
-  const int wv = threadIdx.x / 64;
-  if ((threadIdx.x & 63) == 0)
-    for (int q = 0; q < 6; ++q) red[wv][q] = v[q];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int k = 1; k < (int)(blockDim.x / 64); ++k)
       for (int q = 0; q < 3; ++q) {
-        v[q] = red[k][q] < v[q] ? red[k][q] : v[q];
-        v[3 + q] = red[k][3 + q] > v[3 + q] ? red[k][3 + q] : v[3 + q];
+        v[q] = c[q] < v[q] ? c[q] : v[q];
+        v[3 + q] = c[q] > v[3 + q] ? c[q] : v[3 + q];
       }
-    for (int q = 0; q < 6; ++q) partial[6 * (size_t)t + q] = v[q];
+    }
+    box_wave_reduce(v);
+    const int wv = threadIdx.x / 64;
+    if ((threadIdx.x & 63) == 0)
+      for (int q = 0; q < 6; ++q) red[wv][q] = v[q];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int k = 1; k < (int)(blockDim.x / 64); ++k)
+        for (int q = 0; q < 3; ++q) {
+          v[q] = red[k][q] < v[q] ? red[k][q] : v[q];
+          v[3 + q] = red[k][3 + q] > v[3 + q] ? red[k][3 + q] : v[3 + q];
+        }
+      for (int q = 0; q < 6; ++q) t.box_partial[6 * (size_t)tile + q] = v[q];
+    }
+    __syncthreads();
   }
 }
 
-__global__ void k_box_combine(int count, const int32_t* __restrict__ list, const int32_t* __restrict__ first_tile,
-                              const int32_t* __restrict__ tiles, const double* __restrict__ partial,
-                              int8_t* __restrict__ axis, int8_t* __restrict__ split) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
-  double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-  for (int t = first_tile[i]; t < first_tile[i] + tiles[i]; ++t)
-    for (int q = 0; q < 3; ++q) {
-      double lo = partial[6 * (size_t)t + q], hi = partial[6 * (size_t)t + 3 + q];
-      v[q] = lo < v[q] ? lo : v[q];
-      v[3 + q] = hi > v[3 + q] ? hi : v[3 + q];
-    }
-  choose_axis(v, &axis[list[i]], &split[list[i]]);
+__global__ void k_box_combine(TreeArrays t) {
+  const int nbig = live_count(t.counters, C_BIG);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nbig; i += gridDim.x * blockDim.x) {
+    const int node = t.big[i];
+    const int t0 = t.big_t0[i];
+    const int tiles = (t.hi[node] - t.lo[node] + TILE - 1) / TILE;
+    double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (int q = t0; q < t0 + tiles; ++q)
+      for (int c = 0; c < 3; ++c) {
+        const double lo = t.box_partial[6 * (size_t)q + c], hi = t.box_partial[6 * (size_t)q + 3 + c];
+        v[c] = lo < v[c] ? lo : v[c];
+        v[3 + c] = hi > v[3 + c] ? hi : v[3 + c];
+      }
+    choose_axis(v, &t.axis[node], &t.split[node]);
+  }
 }
 
-// ------------------------------------------------------------------ sorting
 __device__ __forceinline__ bool pair_less(uint64_t ka, uint32_t pa, uint64_t kb, uint32_t pb) {
   return ka < kb || (ka == kb && pa < pb);
 }
 
-// Bitonic sort of (key, pos) in LDS over `len` elements padded to the power of two `width`.
-__device__ void lds_bitonic(uint64_t* key, uint32_t* pos, int width) {
-  for (int size = 2; size <= width; size <<= 1) {
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int i = threadIdx.x; i < (width >> 1); i += blockDim.x) {
-        const int lo = 2 * i - (i & (stride - 1));
-        const int hi = lo + stride;
-        const bool up = (lo & size) == 0;
-        const uint64_t ka = key[lo], kb = key[hi];
-        const uint32_t pa = pos[lo], pb = pos[hi];
-        const bool swap = up ? pair_less(kb, pb, ka, pa) : pair_less(ka, pa, kb, pb);
-        if (swap) {
-          key[lo] = kb;
-          key[hi] = ka;
-          pos[lo] = pb;
-          pos[hi] = pa;
-        }
-      }
-      __syncthreads();
-    }
-  }
-}
-
-// Sort one node's whole range (at most TILE elements) in LDS and write the new order in place.
-__global__ __launch_bounds__(SORT_THREADS) void k_sort_small(const int32_t* __restrict__ list,
-                                                             const int32_t* __restrict__ clo,
-                                                             const int32_t* __restrict__ chi,
-                                                             const int8_t* __restrict__ axis_of,
-                                                             int32_t* __restrict__ order,
-                                                             const double* __restrict__ mx,
+__global__ __launch_bounds__(TILE_THREADS) void k_sort_tiles(TreeArrays t, const double* __restrict__ mx,
                                                              const double* __restrict__ my,
                                                              const double* __restrict__ mz) {
   __shared__ uint64_t key[TILE];
   __shared__ uint32_t pos[TILE];
-  __shared__ int32_t seg[TILE];
-  const int node = list[blockIdx.x];
-  const int a = clo[node], b = chi[node];
-  const int len = b - a;
-  int width = 1;
-  while (width < len) width <<= 1;
-  const int axis = axis_of[node];
-  const double* coord = axis == 0 ? mx : (axis == 1 ? my : mz);
-  for (int i = threadIdx.x; i < width; i += blockDim.x) {
-    if (i < len) {
-      const int s = order[a + i];
-      seg[i] = s;
-      key[i] = ordered_key(coord[s]);
-      pos[i] = (uint32_t)i;
-    } else {
-      key[i] = ~0ull;
-      pos[i] = 0xffffffffu;
+  const int ntiles = live_count(t.counters, C_TILES);
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int node = t.tile_node[tile];
+    if (!t.split[node]) continue;  // uniform over the workgroup
+    const int p0 = t.tile_p0[tile], p1 = t.tile_p1[tile];
+    const int len = p1 - p0;
+    int width = 1;
+    while (width < len) width <<= 1;
+    const int axis = t.axis[node];
+    const double* coord = axis == 0 ? mx : (axis == 1 ? my : mz);
+    for (int i = threadIdx.x; i < width; i += blockDim.x) {
+      if (i < len) {
+        key[i] = ordered_key(coord[t.order[p0 + i]]);
+        pos[i] = (uint32_t)(p0 + i);
+      } else {
+        key[i] = ~0ull;
+        pos[i] = 0xffffffffu;
+      }
     }
-  }
-  __syncthreads();
-  lds_bitonic(key, pos, width);
-  for (int i = threadIdx.x; i < len; i += blockDim.x) order[a + i] = seg[pos[i]];
-}
-
-// Sort one tile of a big node; writes the sorted (key, global position) pairs.
-__global__ __launch_bounds__(SORT_THREADS) void k_sort_tiles(const int32_t* __restrict__ tile_node,
-                                                             const int32_t* __restrict__ tile_p0,
-                                                             const int32_t* __restrict__ tile_p1,
-                                                             const int8_t* __restrict__ axis_of,
-                                                             const int32_t* __restrict__ order,
-                                                             const double* __restrict__ mx,
-                                                             const double* __restrict__ my,
-                                                             const double* __restrict__ mz,
-                                                             uint64_t* __restrict__ out_key,
-                                                             uint32_t* __restrict__ out_pos) {
-  __shared__ uint64_t key[TILE];
-  __shared__ uint32_t pos[TILE];
-  const int t = blockIdx.x;
-  const int p0 = tile_p0[t], p1 = tile_p1[t];
-  const int len = p1 - p0;
-  int width = 1;
-  while (width < len) width <<= 1;
-  const int axis = axis_of[tile_node[t]];
-  const double* coord = axis == 0 ? mx : (axis == 1 ? my : mz);
-  for (int i = threadIdx.x; i < width; i += blockDim.x) {
-    if (i < len) {
-      key[i] = ordered_key(coord[order[p0 + i]]);
-      pos[i] = (uint32_t)(p0 + i);
-    } else {
-      key[i] = ~0ull;
-      pos[i] = 0xffffffffu;
+    __syncthreads();
+    for (int size = 2; size <= width; size <<= 1) {
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        for (int i = threadIdx.x; i < (width >> 1); i += blockDim.x) {
+          const int lo = 2 * i - (i & (stride - 1));
+          const int hi = lo + stride;
+          const bool up = (lo & size) == 0;
+          const uint64_t ka = key[lo], kb = key[hi];
+          const uint32_t pa = pos[lo], pb = pos[hi];
+          if (up ? pair_less(kb, pb, ka, pa) : pair_less(ka, pa, kb, pb)) {
+            key[lo] = kb;
+            key[hi] = ka;
+            pos[lo] = pb;
+            pos[hi] = pa;
+          }
+        }
+        __syncthreads();
+      }
     }
-  }
-  __syncthreads();
-  lds_bitonic(key, pos, width);
-  for (int i = threadIdx.x; i < len; i += blockDim.x) {
-    out_key[p0 + i] = key[i];
-    out_pos[p0 + i] = pos[i];
+    for (int i = threadIdx.x; i < len; i += blockDim.x) {
+      t.keyA[p0 + i] = key[i];
+      t.posA[p0 + i] = pos[i];
+    }
+    __syncthreads();
   }
 }
 
-// One merge pass: within each big node, runs of `width` elements (from the node start) are merged in
-// pairs.  A workgroup produces one output tile; each thread finds its own start on the merge path by
-// binary search and merges PER_THREAD outputs.
-static constexpr int PER_THREAD = TILE / SORT_THREADS;
+// One merge pass: within each big split node, runs of `width` elements (from the node start) are merged in
+// pairs; a workgroup produces one output tile, each thread PER_THREAD outputs from its own merge-path start.
+static constexpr int MERGE_THREADS = 512;
+static constexpr int PER_THREAD = TILE / MERGE_THREADS;
 
-__global__ __launch_bounds__(SORT_THREADS) void k_merge_pass(const int32_t* __restrict__ tile_node,
-                                                             const int32_t* __restrict__ tile_p0,
-                                                             const int32_t* __restrict__ tile_p1,
-                                                             const int32_t* __restrict__ clo,
-                                                             const int32_t* __restrict__ chi, int width,
-                                                             const uint64_t* __restrict__ in_key,
-                                                             const uint32_t* __restrict__ in_pos,
-                                                             uint64_t* __restrict__ out_key,
-                                                             uint32_t* __restrict__ out_pos) {
-  const int t = blockIdx.x;
-  const int node = tile_node[t];
-  const int a = clo[node], b = chi[node];
-  const int p0 = tile_p0[t], p1 = tile_p1[t];
-  const int pair_start = a + ((p0 - a) / (2 * width)) * (2 * width);
-  const int a0 = pair_start, a1 = min(b, pair_start + width);
-  const int b0 = a1, b1 = min(b, a1 + width);
-  const int la = a1 - a0, lb = b1 - b0;
-  const int o_begin = p0 + threadIdx.x * PER_THREAD;
-  if (o_begin >= p1) return;
-  const int o_end = min(p1, o_begin + PER_THREAD);
-  const int diag = o_begin - pair_start;  // outputs before this thread's first within the pair
-  // find i (taken from A) with i + j = diag such that A[i-1] < B[j] and B[j-1] < A[i]
-  int lo = max(0, diag - lb), hi = min(diag, la);
-  while (lo < hi) {
-    const int i = (lo + hi) >> 1;
-    const int j = diag - i - 1;  // compare A[i] with B[diag - i - 1]
-    if (pair_less(in_key[b0 + j], in_pos[b0 + j], in_key[a0 + i], in_pos[a0 + i]))
-      hi = i;
-    else
-      lo = i + 1;
-  }
-  int i = lo, j = diag - lo;
-  for (int o = o_begin; o < o_end; ++o) {
-    bool take_a;
-    if (i >= la) take_a = false;
-    else if (j >= lb) take_a = true;
-    else take_a = !pair_less(in_key[b0 + j], in_pos[b0 + j], in_key[a0 + i], in_pos[a0 + i]);
-    if (take_a) {
-      out_key[o] = in_key[a0 + i];
-      out_pos[o] = in_pos[a0 + i];
-      ++i;
-    } else {
-      out_key[o] = in_key[b0 + j];
-      out_pos[o] = in_pos[b0 + j];
-      ++j;
+__global__ __launch_bounds__(MERGE_THREADS) void k_merge_pass(TreeArrays t, int width,
+                                                              const uint64_t* __restrict__ in_key,
+                                                              const uint32_t* __restrict__ in_pos,
+                                                              uint64_t* __restrict__ out_key,
+                                                              uint32_t* __restrict__ out_pos) {
+  const int ntiles = live_count(t.counters, C_TILES);
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int node = t.tile_node[tile];
+    if (!t.split[node]) continue;
+    const int a = t.lo[node], b = t.hi[node];
+    const int p0 = t.tile_p0[tile], p1 = t.tile_p1[tile];
+    const int pair_start = a + ((p0 - a) / (2 * width)) * (2 * width);
+    const int a0 = pair_start, a1 = min(b, pair_start + width);
+    const int b0 = a1, b1 = min(b, a1 + width);
+    const int la = a1 - a0, lb = b1 - b0;
+    const int o_begin = p0 + threadIdx.x * PER_THREAD;
+    if (o_begin >= p1) continue;
+    const int o_end = min(p1, o_begin + PER_THREAD);
+    const int diag = o_begin - pair_start;
+    int lo = max(0, diag - lb), hi = min(diag, la);
+    while (lo < hi) {
+      const int i = (lo + hi) >> 1;
+      const int j = diag - i - 1;
+      if (pair_less(in_key[b0 + j], in_pos[b0 + j], in_key[a0 + i], in_pos[a0 + i]))
+        hi = i;
+      else
+        lo = i + 1;
+    }
+    int i = lo, j = diag - lo;
+    for (int o = o_begin; o < o_end; ++o) {
+      bool take_a;
+      if (i >= la) take_a = false;
+      else if (j >= lb) take_a = true;
+      else take_a = !pair_less(in_key[b0 + j], in_pos[b0 + j], in_key[a0 + i], in_pos[a0 + i]);
+      if (take_a) {
+        out_key[o] = in_key[a0 + i];
+        out_pos[o] = in_pos[a0 + i];
+        ++i;
+      } else {
+        out_key[o] = in_key[b0 + j];
+        out_pos[o] = in_pos[b0 + j];
+        ++j;
+      }
     }
   }
 }
 
-__global__ __launch_bounds__(256) void k_apply_perm(const int32_t* __restrict__ tile_p0,
-                                                    const int32_t* __restrict__ tile_p1,
-                                                    const uint32_t* __restrict__ sorted_pos,
-                                                    const int32_t* __restrict__ order, int32_t* __restrict__ tmp) {
-  const int t = blockIdx.x;
-  const int p0 = tile_p0[t], p1 = tile_p1[t];
-  for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) tmp[p] = order[sorted_pos[p]];
+__global__ __launch_bounds__(256) void k_apply_perm(TreeArrays t, const uint32_t* __restrict__ sorted_pos) {
+  const int ntiles = live_count(t.counters, C_TILES);
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    if (!t.split[t.tile_node[tile]]) continue;
+    const int p0 = t.tile_p0[tile], p1 = t.tile_p1[tile];
+    for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) t.tmp[p] = t.order[sorted_pos[p]];
+  }
 }
 
-__global__ __launch_bounds__(256) void k_copy_tiles(const int32_t* __restrict__ tile_p0,
-                                                    const int32_t* __restrict__ tile_p1,
-                                                    const int32_t* __restrict__ tmp, int32_t* __restrict__ order) {
-  const int t = blockIdx.x;
-  const int p0 = tile_p0[t], p1 = tile_p1[t];
-  for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) order[p] = tmp[p];
+__global__ __launch_bounds__(256) void k_copy_tiles(TreeArrays t) {
+  const int ntiles = live_count(t.counters, C_TILES);
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    if (!t.split[t.tile_node[tile]]) continue;
+    const int p0 = t.tile_p0[tile], p1 = t.tile_p1[tile];
+    for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) t.order[p] = t.tmp[p];
+  }
+}
+
+// One workgroup: children of the big nodes that split (vortex_tree.py:244-268); they form the next level.
+__global__ __launch_bounds__(PLAN_THREADS) void k_level_children(TreeArrays t, int32_t* __restrict__ next) {
+  const int nbig = live_count(t.counters, C_BIG);
+  const int nodes0 = t.counters[C_NODES];
+  int made = 0;
+  for (int base = 0; base < nbig; base += blockDim.x) {
+    const int i = base + threadIdx.x;
+    const int node = i < nbig ? t.big[i] : -1;
+    const bool sp = node >= 0 && t.split[node];
+    int total;
+    const int r = block_exclusive_scan(sp ? 1 : 0, &total);
+    if (sp) {
+      const int first = nodes0 + 2 * (made + r);
+      if (first + 1 < t.cap) {
+        const int a = t.lo[node], b = t.hi[node], middle = (int)(((int64_t)a + b) / 2);
+        const int d = t.depth[node] + 1, tr = t.tree[node];
+        t.lo[first] = a;
+        t.hi[first] = middle;
+        t.lo[first + 1] = middle;
+        t.hi[first + 1] = b;
+        for (int q = 0; q < 2; ++q) {
+          t.left[first + q] = -1;
+          t.right[first + q] = -1;
+          t.parent[first + q] = node;
+          t.depth[first + q] = d;
+          t.tree[first + q] = tr;
+        }
+        t.left[node] = first;
+        t.right[node] = first + 1;
+        next[2 * (made + r)] = first;
+        next[2 * (made + r) + 1] = first + 1;
+      } else {
+        t.counters[C_OVERFLOW] = 1;
+      }
+    }
+    made += total;
+  }
+  if (threadIdx.x == 0) {
+    t.counters[C_NODES] = nodes0 + 2 * made;
+    t.counters[C_LEVEL] = 2 * made;
+  }
+}
+
+// ------------------------------------------------------------------ bottom phase
+__global__ __launch_bounds__(PLAN_THREADS) void k_bottom_bases(TreeArrays t) {
+  const int count = live_count(t.counters, C_BOTTOM);
+  const int nodes0 = t.counters[C_NODES];
+  int acc = 0;
+  for (int base = 0; base < count; base += blockDim.x) {
+    const int b = base + threadIdx.x;
+    int slots = 0;
+    if (b < count) {
+      const int node = t.bottom[b];
+      slots = heap_slots(t.hi[node] - t.lo[node], t.leaf) - 1;  // the heap root is the node itself
+    }
+    int total;
+    const int r = block_exclusive_scan(slots, &total);
+    if (b < count) t.heap_base[b] = nodes0 + acc + r;
+    acc += total;
+  }
+  if (threadIdx.x == 0) {
+    if ((int64_t)nodes0 + acc > t.cap) t.counters[C_OVERFLOW] = 1;
+    t.counters[C_HEAP] = acc;
+    t.counters[C_NODES] = nodes0 + acc;
+  }
+}
+
+__device__ __forceinline__ bool key3_less(uint32_t ta, uint64_t ka, uint32_t tb, uint64_t kb) {
+  // tags are (start position of the element's node << 11) | position: nodes of every depth partition the
+  // range in position order, so ordering by node start, then key, then position sorts each splitting
+  // node's range and leaves every other node's range as it is
+  const uint32_t na = ta >> 11, nb = tb >> 11;
+  if (na != nb) return na < nb;
+  if (ka != kb) return ka < kb;
+  return (ta & 2047u) < (tb & 2047u);
+}
+
+// One workgroup per subtree of at most S_BOTTOM segments: every remaining level in LDS.
+__global__ __launch_bounds__(BOTTOM_THREADS) void k_bottom(TreeArrays t, const double* __restrict__ mx,
+                                                           const double* __restrict__ my,
+                                                           const double* __restrict__ mz) {
+  __shared__ double cx[S_BOTTOM], cy[S_BOTTOM], cz[S_BOTTOM];
+  __shared__ int32_t seg[S_BOTTOM], itmp[S_BOTTOM];
+  __shared__ uint64_t key[S_BOTTOM];
+  __shared__ uint32_t tag[S_BOTTOM];
+  __shared__ uint16_t nodeof[S_BOTTOM];
+  __shared__ uint16_t hlo[HCAP], hhi[HCAP];
+  __shared__ uint8_t hstate[HCAP], haxis[HCAP];  // state: 0 absent, 1 leaf so far, 2 split
+  __shared__ int any_split;
+  const int count = live_count(t.counters, C_BOTTOM);
+  const int nwaves = blockDim.x / 64, wv = threadIdx.x / 64, lane = threadIdx.x & 63;
+  for (int b = blockIdx.x; b < count; b += gridDim.x) {
+    const int root = t.bottom[b];
+    const int a = t.lo[root];
+    const int s = t.hi[root] - a;
+    const int slots = heap_slots(s, t.leaf);
+    for (int i = threadIdx.x; i < s; i += blockDim.x) {
+      const int g = t.order[a + i];
+      seg[i] = g;
+      cx[i] = mx[g];
+      cy[i] = my[g];
+      cz[i] = mz[g];
+      nodeof[i] = 0;
+    }
+    for (int j = threadIdx.x; j < slots; j += blockDim.x) {
+      hstate[j] = j == 0 ? 1 : 0;
+      hlo[j] = 0;
+      hhi[j] = j == 0 ? (uint16_t)s : 0;
+      haxis[j] = 0;
+    }
+    __syncthreads();
+    for (int k = 0;; ++k) {
+      const int first = (1 << k) - 1;
+      if (first >= slots) break;
+      const int last = min(slots, (2 << k) - 1);
+      if (threadIdx.x == 0) any_split = 0;
+      __syncthreads();
+      // boxes: one wavefront per node of this level
+      for (int j = first + wv; j < last; j += nwaves) {
+        if (hstate[j] == 0) continue;
+        const int lo = hlo[j], hi = hhi[j];
+        if (hi - lo <= t.leaf) continue;
+        double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+        for (int i = lo + lane; i < hi; i += 64) {
+          const double c[3] = {cx[i], cy[i], cz[i]};
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            v[q] = c[q] < v[q] ? c[q] : v[q];
+            v[3 + q] = c[q] > v[3 + q] ? c[q] : v[3 + q];
+          }
+        }
+        box_wave_reduce(v);
+        if (lane == 0) {
+          int8_t ax, sp;
+          choose_axis(v, &ax, &sp);
+          haxis[j] = (uint8_t)ax;
+          if (sp) {
+            hstate[j] = 2;
+            any_split = 1;
+          }
+        }
+      }
+      __syncthreads();
+      if (!any_split) break;
+      // sort every splitting node's range by (node, key, position); other nodes keep their order
+      int width = 1;
+      while (width < s) width <<= 1;
+      for (int i = threadIdx.x; i < width; i += blockDim.x) {
+        if (i < s) {
+          const int j = nodeof[i];
+          uint64_t kk = 0;
+          if (hstate[j] == 2) {
+            const int ax = haxis[j];
+            kk = ordered_key(ax == 0 ? cx[i] : (ax == 1 ? cy[i] : cz[i]));
+          }
+          key[i] = kk;
+          tag[i] = ((uint32_t)hlo[j] << 11) | (uint32_t)i;
+        } else {
+          key[i] = ~0ull;
+          tag[i] = 0xffffffffu;
+        }
+      }
+      __syncthreads();
+      for (int size = 2; size <= width; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+          for (int i = threadIdx.x; i < (width >> 1); i += blockDim.x) {
+            const int lo = 2 * i - (i & (stride - 1));
+            const int hi = lo + stride;
+            const bool up = (lo & size) == 0;
+            const uint64_t ka = key[lo], kb = key[hi];
+            const uint32_t ta = tag[lo], tb = tag[hi];
+            if (up ? key3_less(tb, kb, ta, ka) : key3_less(ta, ka, tb, kb)) {
+              key[lo] = kb;
+              key[hi] = ka;
+              tag[lo] = tb;
+              tag[hi] = ta;
+            }
+          }
+          __syncthreads();
+        }
+      }
+      // permute segments and midpoints by the sorted positions (one array at a time through a scratch)
+      double* dtmp = reinterpret_cast<double*>(key);
+      for (int i = threadIdx.x; i < s; i += blockDim.x) itmp[i] = seg[tag[i] & 2047u];
+      __syncthreads();
+      for (int i = threadIdx.x; i < s; i += blockDim.x) seg[i] = itmp[i];
+      __syncthreads();
+      double* arrays[3] = {cx, cy, cz};
+      for (int q = 0; q < 3; ++q) {
+        double* arr = arrays[q];
+        for (int i = threadIdx.x; i < s; i += blockDim.x) dtmp[i] = arr[tag[i] & 2047u];
+        __syncthreads();
+        for (int i = threadIdx.x; i < s; i += blockDim.x) arr[i] = dtmp[i];
+        __syncthreads();
+      }
+      // children of the splitting nodes, then every element's node for the next level
+      for (int j = first + threadIdx.x; j < last; j += blockDim.x) {
+        if (hstate[j] != 2) continue;
+        const int lo = hlo[j], hi = hhi[j], middle = (lo + hi) / 2;
+        hstate[2 * j + 1] = 1;
+        hlo[2 * j + 1] = (uint16_t)lo;
+        hhi[2 * j + 1] = (uint16_t)middle;
+        hstate[2 * j + 2] = 1;
+        hlo[2 * j + 2] = (uint16_t)middle;
+        hhi[2 * j + 2] = (uint16_t)hi;
+      }
+      __syncthreads();
+      for (int i = threadIdx.x; i < s; i += blockDim.x) {
+        const int j = nodeof[i];
+        if (hstate[j] == 2) nodeof[i] = (uint16_t)(i < hhi[2 * j + 1] ? 2 * j + 1 : 2 * j + 2);
+      }
+      __syncthreads();
+    }
+    // results
+    for (int i = threadIdx.x; i < s; i += blockDim.x) t.order[a + i] = seg[i];
+    const int hb = t.heap_base[b];
+    const int d0 = t.depth[root], tr = t.tree[root];
+    for (int j = threadIdx.x; j < slots; j += blockDim.x) {
+      const int gid = j == 0 ? root : hb + j - 1;
+      const bool exists = hstate[j] != 0;
+      const bool sp = hstate[j] == 2;
+      if (j > 0) {
+        t.lo[gid] = exists ? a + hlo[j] : -1;
+        t.hi[gid] = exists ? a + hhi[j] : -1;
+        const int pj = (j - 1) / 2;
+        t.parent[gid] = pj == 0 ? root : hb + pj - 1;
+        t.tree[gid] = tr;
+        int d = 0;
+        for (int q = j; q > 0; q = (q - 1) / 2) ++d;
+        t.depth[gid] = d0 + d;
+      }
+      t.left[gid] = sp ? hb + 2 * j : -1;       // heap slot 2j+1 -> hb + 2j
+      t.right[gid] = sp ? hb + 2 * j + 1 : -1;  // heap slot 2j+2 -> hb + 2j+1
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ finish: pre-order layout
+__global__ void k_node_hist(TreeArrays t) {
+  const int count = live_count(t.counters, C_NODES);
+  for (int id = blockIdx.x * blockDim.x + threadIdx.x; id < count; id += gridDim.x * blockDim.x) {
+    const int lo = t.lo[id];
+    if (lo < 0) continue;
+    atomicAdd(&t.lo_cnt[lo], 1);
+    atomicMin(&t.lo_mind[lo], t.depth[id]);
+  }
+}
+
+// exclusive scan of n int32 values in blocks of SCAN_ITEMS (three kernels)
+static constexpr int SCAN_THREADS = 256, SCAN_PER = 16, SCAN_ITEMS = SCAN_THREADS * SCAN_PER;
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(int n, const int32_t* __restrict__ in,
+                                                              int32_t* __restrict__ part) {
+  const int base = blockIdx.x * SCAN_ITEMS;
+  int s = 0;
+  for (int q = 0; q < SCAN_PER; ++q) {
+    const int i = base + threadIdx.x * SCAN_PER + q;
+    if (i < n) s += in[i];
+  }
+  int total;
+  (void)block_exclusive_scan(s, &total);
+  if (threadIdx.x == 0) part[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(PLAN_THREADS) void k_scan_parts(int blocks, int32_t* __restrict__ part,
+                                                            int32_t* __restrict__ total_out) {
+  int acc = 0;
+  for (int base = 0; base < blocks; base += blockDim.x) {
+    const int i = base + threadIdx.x;
+    const int v = i < blocks ? part[i] : 0;
+    int total;
+    const int r = block_exclusive_scan(v, &total);
+    if (i < blocks) part[i] = acc + r;
+    acc += total;
+  }
+  if (threadIdx.x == 0 && total_out) *total_out = acc;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(int n, const int32_t* __restrict__ in,
+                                                             const int32_t* __restrict__ part,
+                                                             int32_t* __restrict__ out) {
+  const int base = blockIdx.x * SCAN_ITEMS;
+  int v[SCAN_PER];
+  int s = 0;
+  for (int q = 0; q < SCAN_PER; ++q) {
+    const int i = base + threadIdx.x * SCAN_PER + q;
+    v[q] = i < n ? in[i] : 0;
+    s += v[q];
+  }
+  int total;
+  int r = block_exclusive_scan(s, &total) + part[blockIdx.x];
+  for (int q = 0; q < SCAN_PER; ++q) {
+    const int i = base + threadIdx.x * SCAN_PER + q;
+    if (i < n) out[i] = r;
+    r += v[q];
+  }
+}
+
+void scan_exclusive(Ctx& c, int n, const int32_t* in, int32_t* out, int32_t* part, int32_t* total_out) {
+  const int blocks = (n + SCAN_ITEMS - 1) / SCAN_ITEMS;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(blocks), dim3(SCAN_THREADS), 0, c.stream, n, in, part);
+  hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(PLAN_THREADS), 0, c.stream, blocks, part, total_out);
+  hipLaunchKernelGGL(k_scan_apply, dim3(blocks), dim3(SCAN_THREADS), 0, c.stream, n, in, part, out);
+}
+
+__global__ void k_node_pre(TreeArrays t) {
+  const int count = live_count(t.counters, C_NODES);
+  for (int id = blockIdx.x * blockDim.x + threadIdx.x; id < count; id += gridDim.x * blockDim.x) {
+    const int lo = t.lo[id];
+    if (lo < 0) {
+      t.pre[id] = -1;
+      continue;
+    }
+    const int p = t.lo_base[lo] + t.depth[id] - t.lo_mind[lo];
+    t.pre[id] = p;
+    t.p_lo[p] = lo;
+    t.p_hi[p] = t.hi[id];
+    t.p_skip[p] = t.lo_base[t.hi[id]];
+    t.p_depth[p] = t.depth[id];
+    t.p_tree[p] = t.tree[id];
+    t.p_node[p] = id;
+    atomicMax(&t.counters[C_DEPTH], t.depth[id]);
+  }
+}
+
+__global__ void k_node_pre_links(TreeArrays t) {
+  const int count = live_count(t.counters, C_FINAL);
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < t.cap; p += gridDim.x * blockDim.x) {
+    int internal = 0;
+    if (p < count) {
+      const int id = t.p_node[p];
+      const int par = t.parent[id];
+      t.p_parent[p] = par >= 0 ? t.pre[par] : -1;
+      internal = t.left[id] >= 0 ? 1 : 0;
+    }
+    t.p_flag[p] = internal;  // internal flags in pre-order, zero beyond the node count
+  }
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k <= t.K; k += gridDim.x * blockDim.x)
+    t.tree_first[k] = t.lo_base[t.roots_lo[k]];
 }
 
 // ------------------------------------------------------------------ host driver
-namespace {
-
-template <typename T>
-T* carve(char*& base, size_t count) {
-  T* p = reinterpret_cast<T*>(base);
-  size_t bytes = (count * sizeof(T) + 255) & ~(size_t)255;
-  base += bytes;
-  return p;
-}
-
-}  // namespace
-
-int build_tree(Ctx& c, int64_t m64, const double* mx, const double* my, const double* mz, int64_t leaf64,
-               int32_t* order, HostTree& t) {
-  const int m = (int)m64;
-  const int leaf = (int)std::min<int64_t>(leaf64, INT_MAX);
+int tree_layout(Ctx& c, int buf, int64_t m64, int K, int leaf, TreeArrays& t) {
+  const int64_t m = m64;
+  t.K = K;
   t.m = m;
-  t.lo.assign(1, 0);
-  t.hi.assign(1, m);
-  t.left.assign(1, -1);
-  t.right.assign(1, -1);
-  if (m == 0) {
-    t.count = 1;
-    return finish_tree(t);
+  t.leaf = leaf;
+  const int64_t cap = 8 * (m / std::max(leaf, 1) + 1) + 16 * (int64_t)K + 1024;
+  if (cap > INT_MAX / 8 || m > (1 << 28)) {
+    set_error("tree too large for one build");
+    return OFW_ERR_ARG;
   }
-  hipLaunchKernelGGL(k_iota, dim3((m + 255) / 256), dim3(256), 0, c.stream, m, order);
-  OFW_HIP(hipGetLastError());
-  // scratch: sort buffers (2 x (key, pos)), tmp order, per-level tables
-  const size_t cap_nodes = (size_t)4 * (m / std::max(leaf, 1) + 1) + 1;
-  const size_t max_tiles = (size_t)m / TILE + cap_nodes + 8;
-  size_t bytes = 0;
-  bytes += 2 * (((size_t)m * 8 + 255) & ~(size_t)255) + 2 * (((size_t)m * 4 + 255) & ~(size_t)255);
-  bytes += (((size_t)m * 4 + 255) & ~(size_t)255);
-  bytes += 8 * ((cap_nodes * 4 + 255) & ~(size_t)255) + 4 * ((max_tiles * 4 + 255) & ~(size_t)255);
-  bytes += ((max_tiles * 6 * 8 + 255) & ~(size_t)255);
-  OFW_HIP(c.tree[0].reserve(bytes));
-  char* base = (char*)c.tree[0].ptr;
-  uint64_t* keyA = carve<uint64_t>(base, m);
-  uint64_t* keyB = carve<uint64_t>(base, m);
-  uint32_t* posA = carve<uint32_t>(base, m);
-  uint32_t* posB = carve<uint32_t>(base, m);
-  int32_t* tmp = carve<int32_t>(base, m);
-  int32_t* d_clo = carve<int32_t>(base, cap_nodes);
-  int32_t* d_chi = carve<int32_t>(base, cap_nodes);
-  int32_t* d_list = carve<int32_t>(base, cap_nodes);
-  int32_t* d_first = carve<int32_t>(base, cap_nodes);
-  int32_t* d_ntiles = carve<int32_t>(base, cap_nodes);
-  int8_t* d_axis = carve<int8_t>(base, cap_nodes);
-  int8_t* d_split = carve<int8_t>(base, cap_nodes);
-  int32_t* d_slist = carve<int32_t>(base, cap_nodes);
-  int32_t* d_tnode = carve<int32_t>(base, max_tiles);
-  int32_t* d_tp0 = carve<int32_t>(base, max_tiles);
-  int32_t* d_tp1 = carve<int32_t>(base, max_tiles);
-  int32_t* d_unused = carve<int32_t>(base, max_tiles);
-  (void)d_unused;
-  double* d_partial = carve<double>(base, max_tiles * 6);
-
-  std::vector<int32_t> level{0};
-  std::vector<int32_t> cand, clo, chi, small, big, tnode, tp0, tp1, first, ntiles, slist_small, slist_big;
-  std::vector<int8_t> axis, split;
-  std::vector<int32_t> bt_node, bt_p0, bt_p1;  // alive until the next synchronisation (async uploads)
-  while (!level.empty()) {
-    cand.clear();
-    for (int node : level)
-      if (t.hi[node] - t.lo[node] > leaf) cand.push_back(node);
-    if (cand.empty()) break;
-    const int nc = (int)cand.size();
-    clo.resize(nc);
-    chi.resize(nc);
-    small.clear();
-    big.clear();
-    tnode.clear();
-    tp0.clear();
-    tp1.clear();
-    first.clear();
-    ntiles.clear();
-    for (int i = 0; i < nc; ++i) {
-      clo[i] = t.lo[cand[i]];
-      chi[i] = t.hi[cand[i]];
-      if (chi[i] - clo[i] <= TILE) {
-        small.push_back(i);
-      } else {
-        big.push_back(i);
-        first.push_back((int)tnode.size());
-        int k = 0;
-        for (int p = clo[i]; p < chi[i]; p += TILE, ++k) {
-          tnode.push_back(i);
-          tp0.push_back(p);
-          tp1.push_back(std::min(chi[i], p + TILE));
-        }
-        ntiles.push_back(k);
-      }
-    }
-    // upload the level's tables
-    OFW_HIP(hipMemcpyAsync(d_clo, clo.data(), nc * 4, hipMemcpyHostToDevice, c.stream));
-    OFW_HIP(hipMemcpyAsync(d_chi, chi.data(), nc * 4, hipMemcpyHostToDevice, c.stream));
-    if (!small.empty())
-      OFW_HIP(hipMemcpyAsync(d_list, small.data(), small.size() * 4, hipMemcpyHostToDevice, c.stream));
-    const int nt = (int)tnode.size();
-    if (!big.empty()) {
-      OFW_HIP(hipMemcpyAsync(d_slist, big.data(), big.size() * 4, hipMemcpyHostToDevice, c.stream));
-      OFW_HIP(hipMemcpyAsync(d_first, first.data(), big.size() * 4, hipMemcpyHostToDevice, c.stream));
-      OFW_HIP(hipMemcpyAsync(d_ntiles, ntiles.data(), big.size() * 4, hipMemcpyHostToDevice, c.stream));
-      OFW_HIP(hipMemcpyAsync(d_tnode, tnode.data(), nt * 4, hipMemcpyHostToDevice, c.stream));
-      OFW_HIP(hipMemcpyAsync(d_tp0, tp0.data(), nt * 4, hipMemcpyHostToDevice, c.stream));
-      OFW_HIP(hipMemcpyAsync(d_tp1, tp1.data(), nt * 4, hipMemcpyHostToDevice, c.stream));
-    }
-    // 1. boxes -> axis and split flag of every candidate
-    if (!small.empty())
-      hipLaunchKernelGGL(k_box_small, dim3(((int)small.size() + 3) / 4), dim3(256), 0, c.stream, (int)small.size(),
-                         d_list, d_clo, d_chi, order, mx, my, mz, d_axis, d_split);
-    if (!big.empty()) {
-      hipLaunchKernelGGL(k_box_tiles, dim3(nt), dim3(256), 0, c.stream, d_tnode, d_tp0, d_tp1, order, mx, my, mz,
-                         d_partial);
-      hipLaunchKernelGGL(k_box_combine, dim3(((int)big.size() + 63) / 64), dim3(64), 0, c.stream, (int)big.size(),
-                         d_slist, d_first, d_ntiles, d_partial, d_axis, d_split);
-    }
-    OFW_HIP(hipGetLastError());
-    axis.resize(nc);
-    split.resize(nc);
-    OFW_HIP(hipMemcpyAsync(split.data(), d_split, nc, hipMemcpyDeviceToHost, c.stream));
-    OFW_HIP(hipStreamSynchronize(c.stream));
-    // 2. sort the ranges of the nodes that split
-    slist_small.clear();
-    for (int i : small)
-      if (split[i]) slist_small.push_back(i);
-    bool any_big = false;
-    for (int i : big) any_big = any_big || split[i];
-    if (!slist_small.empty()) {
-      OFW_HIP(hipMemcpyAsync(d_list, slist_small.data(), slist_small.size() * 4, hipMemcpyHostToDevice, c.stream));
-      hipLaunchKernelGGL(k_sort_small, dim3((unsigned)slist_small.size()), dim3(SORT_THREADS), 0, c.stream, d_list,
-                         d_clo, d_chi, d_axis, order, mx, my, mz);
-      OFW_HIP(hipGetLastError());
-    }
-    if (any_big) {
-      // big nodes that do not split keep their order: restrict the tile list to splitting nodes
-      bt_node.clear();
-      bt_p0.clear();
-      bt_p1.clear();
-      int maxlen = 0;
-      for (size_t k = 0; k < big.size(); ++k) {
-        const int i = big[k];
-        if (!split[i]) continue;
-        maxlen = std::max(maxlen, chi[i] - clo[i]);
-        for (int q = 0; q < ntiles[k]; ++q) {
-          bt_node.push_back(i);
-          bt_p0.push_back(tp0[first[k] + q]);
-          bt_p1.push_back(tp1[first[k] + q]);
-        }
-      }
-      const int nbt = (int)bt_node.size();
-      OFW_HIP(hipMemcpyAsync(d_tnode, bt_node.data(), nbt * 4, hipMemcpyHostToDevice, c.stream));
-      OFW_HIP(hipMemcpyAsync(d_tp0, bt_p0.data(), nbt * 4, hipMemcpyHostToDevice, c.stream));
-      OFW_HIP(hipMemcpyAsync(d_tp1, bt_p1.data(), nbt * 4, hipMemcpyHostToDevice, c.stream));
-      hipLaunchKernelGGL(k_sort_tiles, dim3(nbt), dim3(SORT_THREADS), 0, c.stream, d_tnode, d_tp0, d_tp1, d_axis,
-                         order, mx, my, mz, keyA, posA);
-      uint64_t *kin = keyA, *kout = keyB;
-      uint32_t *pin = posA, *pout = posB;
-      for (int width = TILE; width < maxlen; width <<= 1) {
-        hipLaunchKernelGGL(k_merge_pass, dim3(nbt), dim3(SORT_THREADS), 0, c.stream, d_tnode, d_tp0, d_tp1, d_clo,
-                           d_chi, width, kin, pin, kout, pout);
-        std::swap(kin, kout);
-        std::swap(pin, pout);
-      }
-      hipLaunchKernelGGL(k_apply_perm, dim3(nbt), dim3(256), 0, c.stream, d_tp0, d_tp1, pin, order, tmp);
-      hipLaunchKernelGGL(k_copy_tiles, dim3(nbt), dim3(256), 0, c.stream, d_tp0, d_tp1, tmp, order);
-      OFW_HIP(hipGetLastError());
-    }
-    // 3. children (vortex_tree.py:244-268)
-    std::vector<int32_t> next;
-    for (int i = 0; i < nc; ++i) {
-      if (!split[i]) continue;
-      const int node = cand[i];
-      const int a = clo[i], b = chi[i], middle = (int)(((int64_t)a + b) / 2);
-      const int first_child = (int)t.lo.size();
-      t.lo.push_back(a);
-      t.hi.push_back(middle);
-      t.lo.push_back(middle);
-      t.hi.push_back(b);
-      t.left.push_back(-1);
-      t.left.push_back(-1);
-      t.right.push_back(-1);
-      t.right.push_back(-1);
-      t.left[node] = first_child;
-      t.right[node] = first_child + 1;
-      next.push_back(first_child);
-      next.push_back(first_child + 1);
-    }
-    level.swap(next);
-  }
-  t.count = (int64_t)t.lo.size();
-  return finish_tree(t);
+  t.cap = (int)cap;
+  t.tile_cap = (int)(m / TILE + 2 * (m / (S_BOTTOM / 2) + 1) + K + 64);
+  t.bottom_cap = (int)(m / (std::max(leaf, 1) + 1) + K + 64);
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t M = (size_t)std::max<int64_t>(m, 1);
+  const size_t C = (size_t)t.cap;
+  size_t need = 0;
+  need += al((K + 1) * 4) + al(M * 4);                                    // roots_lo, order
+  need += 7 * al(C * 4) + 2 * al(C);                                      // node table, axis, split
+  need += al(C_COUNT * 4) + 5 * al(C * 4);                                // counters, levels, big, big_t0, flags
+  need += 3 * al((size_t)t.tile_cap * 4) + al((size_t)t.tile_cap * 48);   // tiles, box partials
+  need += 2 * al(M * 8) + 2 * al(M * 4) + al(std::max(M, C) * 4);         // sort buffers, tmp
+  need += 2 * al((size_t)t.bottom_cap * 4);                               // bottom, heap base
+  need += 3 * al((M + 1 + C) * 4) + al(((M + C) / 4096 + 8) * 4);         // lo_cnt, lo_mind, lo_base, parts
+  need += al(C * 4) + 8 * al(C * 4) + al((K + 1) * 4);                    // pre, p_*, tree_first
+  OFW_HIP(c.tree[buf].reserve(need));
+  char* base = (char*)c.tree[buf].ptr;
+  auto carve = [&](size_t bytes) {
+    char* p = base;
+    base += al(bytes);
+    return p;
+  };
+  t.roots_lo = (int32_t*)carve((K + 1) * 4);
+  t.order = (int32_t*)carve(M * 4);
+  t.lo = (int32_t*)carve(C * 4);
+  t.hi = (int32_t*)carve(C * 4);
+  t.left = (int32_t*)carve(C * 4);
+  t.right = (int32_t*)carve(C * 4);
+  t.parent = (int32_t*)carve(C * 4);
+  t.depth = (int32_t*)carve(C * 4);
+  t.tree = (int32_t*)carve(C * 4);
+  t.axis = (int8_t*)carve(C);
+  t.split = (int8_t*)carve(C);
+  t.counters = (int32_t*)carve(C_COUNT * 4);
+  t.level_a = (int32_t*)carve(C * 4);
+  t.level_b = (int32_t*)carve(C * 4);
+  t.big = (int32_t*)carve(C * 4);
+  t.big_t0 = (int32_t*)carve(C * 4);
+  t.p_flag = (int32_t*)carve(C * 4);
+  t.tile_node = (int32_t*)carve((size_t)t.tile_cap * 4);
+  t.tile_p0 = (int32_t*)carve((size_t)t.tile_cap * 4);
+  t.tile_p1 = (int32_t*)carve((size_t)t.tile_cap * 4);
+  t.box_partial = (double*)carve((size_t)t.tile_cap * 48);
+  t.keyA = (uint64_t*)carve(M * 8);
+  t.keyB = (uint64_t*)carve(M * 8);
+  t.posA = (uint32_t*)carve(M * 4);
+  t.posB = (uint32_t*)carve(M * 4);
+  t.tmp = (int32_t*)carve(std::max(M, C) * 4);
+  t.bottom = (int32_t*)carve((size_t)t.bottom_cap * 4);
+  t.heap_base = (int32_t*)carve((size_t)t.bottom_cap * 4);
+  t.lo_cnt = (int32_t*)carve((M + 1 + C) * 4);
+  t.lo_mind = (int32_t*)carve((M + 1 + C) * 4);
+  t.lo_base = (int32_t*)carve((M + 1 + C) * 4);
+  t.scan_part = (int32_t*)carve(((M + C) / 4096 + 8) * 4);
+  t.pre = (int32_t*)carve(C * 4);
+  t.p_lo = (int32_t*)carve(C * 4);
+  t.p_hi = (int32_t*)carve(C * 4);
+  t.p_skip = (int32_t*)carve(C * 4);
+  t.p_parent = (int32_t*)carve(C * 4);
+  t.p_depth = (int32_t*)carve(C * 4);
+  t.p_tree = (int32_t*)carve(C * 4);
+  t.p_node = (int32_t*)carve(C * 4);
+  t.p_irank = (int32_t*)carve(C * 4);
+  t.tree_first = (int32_t*)carve((K + 1) * 4);
+  return OFW_OK;
 }
 
-// Reference numbering (vortex_tree.py:271-306, _renumber) and the pre-order layout used by the traversal.
-int finish_tree(HostTree& t) {
-  const int count = (int)t.count;
-  std::vector<int32_t> number(count, 0), stack;
-  t.ref_lo.assign(count, 0);
-  t.ref_hi.assign(count, 0);
-  t.ref_left.assign(count, -1);
-  t.ref_right.assign(count, -1);
-  t.ref_lo[0] = t.lo[0];
-  t.ref_hi[0] = t.hi[0];
-  int assigned = 1;
-  stack.push_back(0);
-  while (!stack.empty()) {
-    const int node = stack.back();
-    stack.pop_back();
-    if (t.left[node] < 0) continue;
-    const int f = assigned, s = assigned + 1;
-    assigned += 2;
-    number[t.left[node]] = f;
-    number[t.right[node]] = s;
-    t.ref_left[number[node]] = f;
-    t.ref_right[number[node]] = s;
-    t.ref_lo[f] = t.lo[t.left[node]];
-    t.ref_hi[f] = t.hi[t.left[node]];
-    t.ref_lo[s] = t.lo[t.right[node]];
-    t.ref_hi[s] = t.hi[t.right[node]];
-    stack.push_back(t.right[node]);
-    stack.push_back(t.left[node]);
+int tree_build(Ctx& c, TreeArrays& t, const double* mx, const double* my, const double* mz,
+               const std::vector<int64_t>& offsets) {
+  const int m = (int)t.m;
+  int64_t max_root = 0;
+  for (int k = 0; k < t.K; ++k) max_root = std::max(max_root, offsets[k + 1] - offsets[k]);
+  const int grid = c.cu_count * 2;
+  if (m > 0) hipLaunchKernelGGL(k_iota, dim3((m + 255) / 256), dim3(256), 0, c.stream, m, t.order);
+  hipLaunchKernelGGL(k_init_roots, dim3(1), dim3(256), 0, c.stream, t);
+  OFW_HIP(hipGetLastError());
+  // top levels: while the largest node of a level can hold more than S_BOTTOM segments
+  int32_t* level = t.level_a;
+  int32_t* next = t.level_b;
+  for (int lvl = 0;; ++lvl) {
+    const int64_t biggest = (max_root + ((int64_t)1 << lvl) - 1) >> lvl;  // ceil(max_root / 2^lvl)
+    hipLaunchKernelGGL(k_level_plan, dim3(1), dim3(PLAN_THREADS), 0, c.stream, t, level);
+    if (biggest <= S_BOTTOM || biggest <= t.leaf) break;  // the plan above sent every node to the bottom
+    hipLaunchKernelGGL(k_box_tiles, dim3(grid), dim3(256), 0, c.stream, t, mx, my, mz);
+    const int64_t nodes_bound = std::min<int64_t>((int64_t)t.K << std::min(lvl, 30), t.cap);
+    hipLaunchKernelGGL(k_box_combine, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(grid, (nodes_bound + 255) / 256))),
+                       dim3(256), 0, c.stream, t);
+    hipLaunchKernelGGL(k_sort_tiles, dim3(grid), dim3(TILE_THREADS), 0, c.stream, t, mx, my, mz);
+    uint64_t *kin = t.keyA, *kout = t.keyB;
+    uint32_t *pin = t.posA, *pout = t.posB;
+    for (int64_t width = TILE; width < biggest; width <<= 1) {
+      hipLaunchKernelGGL(k_merge_pass, dim3(grid), dim3(MERGE_THREADS), 0, c.stream, t, (int)width, kin, pin, kout,
+                         pout);
+      std::swap(kin, kout);
+      std::swap(pin, pout);
+    }
+    hipLaunchKernelGGL(k_apply_perm, dim3(grid), dim3(256), 0, c.stream, t, pin);
+    hipLaunchKernelGGL(k_copy_tiles, dim3(grid), dim3(256), 0, c.stream, t);
+    hipLaunchKernelGGL(k_level_children, dim3(1), dim3(PLAN_THREADS), 0, c.stream, t, next);
+    OFW_HIP(hipGetLastError());
+    std::swap(level, next);
   }
-  // pre-order (left first) of the reference-numbered tree: the visit order of its stack walk
-  t.pre_ref.assign(count, 0);   // pre-order index -> reference node number
-  t.pre_skip.assign(count, 0);  // pre-order index -> pre-order index after its subtree
-  t.pre_parent.assign(count, -1);
-  t.pre_depth.assign(count, 0);
-  std::vector<int32_t> pstack;  // pre-order index of nodes on the stack
-  std::vector<int32_t> size(count, 1);
-  int next = 0;
-  stack.clear();
-  stack.push_back(0);
-  pstack.push_back(-1);
-  std::vector<int32_t> pre_of_ref(count, 0);
-  while (!stack.empty()) {
-    const int r = stack.back();
-    stack.pop_back();
-    const int parent = pstack.back();
-    pstack.pop_back();
-    const int i = next++;
-    pre_of_ref[r] = i;
-    t.pre_ref[i] = r;
-    t.pre_parent[i] = parent;
-    t.pre_depth[i] = parent < 0 ? 0 : t.pre_depth[parent] + 1;
-    if (t.ref_left[r] >= 0) {
-      stack.push_back(t.ref_right[r]);
-      pstack.push_back(i);
-      stack.push_back(t.ref_left[r]);
-      pstack.push_back(i);
+  // bottom phase
+  hipLaunchKernelGGL(k_bottom_bases, dim3(1), dim3(PLAN_THREADS), 0, c.stream, t);
+  hipLaunchKernelGGL(k_bottom, dim3(grid), dim3(BOTTOM_THREADS), 0, c.stream, t, mx, my, mz);
+  OFW_HIP(hipGetLastError());
+  // finish: pre-order layout from (lo, depth)
+  const int n1 = m + 1;
+  hipLaunchKernelGGL(k_fill_i32, dim3((n1 + 255) / 256), dim3(256), 0, c.stream, n1, t.lo_cnt, 0);
+  hipLaunchKernelGGL(k_fill_i32, dim3((n1 + 255) / 256), dim3(256), 0, c.stream, n1, t.lo_mind, INT_MAX);
+  hipLaunchKernelGGL(k_node_hist, dim3(grid), dim3(256), 0, c.stream, t);
+  scan_exclusive(c, n1, t.lo_cnt, t.lo_base, t.scan_part, t.counters + C_FINAL);
+  hipLaunchKernelGGL(k_node_pre, dim3(grid), dim3(256), 0, c.stream, t);
+  hipLaunchKernelGGL(k_node_pre_links, dim3(grid), dim3(256), 0, c.stream, t);
+  // internal ranks in pre-order (for the reference numbering)
+  scan_exclusive(c, t.cap, t.p_flag, t.p_irank, t.scan_part, nullptr);
+  OFW_HIP(hipGetLastError());
+  return OFW_OK;
+}
+
+// Reference numbering (vortex_tree.py:271-306): root 0; the children of the split node with internal rank
+// j in pre-order are 2j+1 and 2j+2.  Host arrays for tests and ofw_build_tree (single tree).
+int tree_reference_arrays(Ctx& c, const TreeArrays& t, std::vector<int64_t>& lo, std::vector<int64_t>& hi,
+                          std::vector<int64_t>& left, std::vector<int64_t>& right, std::vector<int32_t>& ref_of_pre,
+                          int64_t& count) {
+  int32_t counters[C_COUNT];
+  OFW_HIP(hipMemcpyAsync(counters, t.counters, sizeof(counters), hipMemcpyDeviceToHost, c.stream));
+  OFW_HIP(hipStreamSynchronize(c.stream));
+  if (counters[C_OVERFLOW]) {
+    set_error("tree build exceeded its node capacity");
+    return OFW_ERR_OVERFLOW;
+  }
+  const int n = counters[C_FINAL];
+  std::vector<int32_t> plo(n), phi(n), pnode(n), pparent(n), irank(n), left_id(t.cap);
+  OFW_HIP(hipMemcpyAsync(plo.data(), t.p_lo, (size_t)n * 4, hipMemcpyDeviceToHost, c.stream));
+  OFW_HIP(hipMemcpyAsync(phi.data(), t.p_hi, (size_t)n * 4, hipMemcpyDeviceToHost, c.stream));
+  OFW_HIP(hipMemcpyAsync(pnode.data(), t.p_node, (size_t)n * 4, hipMemcpyDeviceToHost, c.stream));
+  OFW_HIP(hipMemcpyAsync(pparent.data(), t.p_parent, (size_t)n * 4, hipMemcpyDeviceToHost, c.stream));
+  OFW_HIP(hipMemcpyAsync(irank.data(), t.p_irank, (size_t)n * 4, hipMemcpyDeviceToHost, c.stream));
+  OFW_HIP(hipMemcpyAsync(left_id.data(), t.left, (size_t)t.cap * 4, hipMemcpyDeviceToHost, c.stream));
+  OFW_HIP(hipStreamSynchronize(c.stream));
+  count = n;
+  ref_of_pre.assign(n, 0);
+  lo.assign(n, 0);
+  hi.assign(n, 0);
+  left.assign(n, -1);
+  right.assign(n, -1);
+  for (int p = 0; p < n; ++p) {
+    const int par = pparent[p];
+    int r;
+    if (par < 0) {
+      r = 0;
+    } else {
+      // this node is the left child of its parent when it directly follows it in pre-order
+      r = 2 * irank[par] + (par + 1 == p ? 1 : 2);
+    }
+    ref_of_pre[p] = r;
+  }
+  for (int p = 0; p < n; ++p) {
+    const int r = ref_of_pre[p];
+    lo[r] = plo[p];
+    hi[r] = phi[p];
+    const int id = pnode[p];
+    if (left_id[id] >= 0) {
+      left[r] = 2 * irank[p] + 1;
+      right[r] = 2 * irank[p] + 2;
     }
   }
-  for (int i = count - 1; i > 0; --i) size[t.pre_parent[i]] += size[i];
-  for (int i = 0; i < count; ++i) t.pre_skip[i] = i + size[i];
-  t.depth = 0;
-  for (int i = 0; i < count; ++i) t.depth = std::max(t.depth, (int)t.pre_depth[i]);
   return OFW_OK;
 }
 

@@ -12,23 +12,33 @@ namespace ofw {
 
 static constexpr int MAX_ANC = 14;
 
-// One frontier subtree of the traversal: its root (pre-order index) and the root's ancestors, root first.
+// One frontier subtree of the traversal: its root (pre-order index), the tree it belongs to, and the
+// root's ancestors, root of the tree first.
 struct EvalTask {
   int32_t node;
   int32_t nanc;
   int32_t anc[MAX_ANC];
 };
 
-void frontier_tasks(const HostTree& t, int depth, std::vector<EvalTask>& tasks);
+// Per-call shape of the targets of a forest: targets of tree k are [target_off[k], target_off[k+1]).
+struct EvalShape {
+  int K = 1;
+  int64_t n = 0;
+  std::vector<int64_t> target_off;  // K+1
+  std::vector<int64_t> seg_off;     // K+1
+  int split_depth = 0;
+};
 
 int tree_prep(Ctx& c, int64_t m, const double* starts, const double* directions, const double* lengths, double* mx,
               double* my, double* mz);
-int tree_moments(Ctx& c, const HostTree& t, const int32_t* order, const double* starts, const double* directions,
+int tree_moments(Ctx& c, int buf, const TreeArrays& t, const double* starts, const double* directions,
                  const double* lengths, const double* gamma, const double* radius, int model, double clearance,
                  TreeDevice& td, bool want_outputs);
-int tree_evaluate(Ctx& c, const HostTree& t, const TreeDevice& td, const int32_t* order, int precision, int64_t n,
-                  const double* points, int model, int raise_singular, const int32_t* ex_ptr,
-                  const int32_t* ex_seg, double theta, int expansion_order, int split_depth, double* out,
-                  int8_t* flags, int64_t* far_counts, int64_t* near_counts, std::vector<EvalTask>& tasks);
+int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, const EvalShape& shape, int precision,
+                  const double* points, int model, int raise_singular, const int32_t* ex_ptr, const int32_t* ex_seg,
+                  int64_t n_ex, double theta, int expansion_order, double* out, int8_t* flags, int64_t* far_counts,
+                  int64_t* near_counts);
+// Frontier depth that gives the traversal enough wavefronts: `groups` = target groups of 64 over all trees.
+int choose_split_depth(const Ctx& c, int64_t groups, int K);
 
 }  // namespace ofw

@@ -23,6 +23,7 @@
 #include "internal.h"
 #include "tree.h"
 #include "tree_eval.h"
+#include "scan.h"
 #include "tree_records.h"
 
 namespace ofw {
@@ -135,8 +136,29 @@ __device__ __forceinline__ bool wave_any(bool v) { return __builtin_amdgcn_ballo
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Per-tree tables of a forest evaluation (K+1 entries each, int32, device).
+struct ForestTables {
+  int K;
+  const int32_t* toff;        // first target of each tree
+  const int32_t* task_first;  // first traversal task of each tree
+  const int32_t* wave_off;    // first work item (wavefront) of each tree
+  const int32_t* part_off;    // first partial-sum slot of each tree
+  const int32_t* counters;
+};
+
+// the tree k of a wavefront / target: the last k with table[k] <= v (table[0] = 0 <= v < table[K])
+__device__ __forceinline__ int find_tree(const int32_t* __restrict__ table, int K, int v) {
+  int lo = 0, hi = K;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (table[mid] <= v) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 template <int MODEL, int PREC>
-__global__ __launch_bounds__(256) void k_tree_eval(int n, const double* __restrict__ pts,
+__global__ __launch_bounds__(256) void k_tree_eval(ForestTables ft, const double* __restrict__ pts,
                                                    const int32_t* __restrict__ ex_ptr,
                                                    const int32_t* __restrict__ ex_pos,
                                                    const EvalTask* __restrict__ tasks,
@@ -147,20 +169,32 @@ __global__ __launch_bounds__(256) void k_tree_eval(int n, const double* __restri
                                                    int8_t* __restrict__ pflag, int32_t* __restrict__ pfar,
                                                    int32_t* __restrict__ pnear) {
   typedef typename std::conditional<PREC == OFW_FP32, float, double>::type acc_t;
-  const int group = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int wave = uniform(xcd_block(blockIdx.x, gridDim.x) * 4 + (int)(threadIdx.x >> 6));
+  // whole wavefront beyond the work, or a plan that overflowed its capacity (reported by the host)
+  if (wave >= ft.counters[C_WORK] || ft.counters[C_OVERFLOW]) return;
   const int lane = threadIdx.x & 63;
-  const int task = blockIdx.y;
-  const int j = group * 64 + lane;
-  if (group * 64 >= n) return;  // whole wave beyond the targets
-  const bool valid = j < n;
+  // work item -> (tree, task of the tree, group of 64 targets): the groups of one task are adjacent
+  const int k = uniform(find_tree(ft.wave_off, ft.K, wave));
+  const int t0 = uniform(ft.toff[k]);
+  const int nt = uniform(ft.toff[k + 1]) - t0;
+  const int groups = (nt + 63) >> 6;
+  const int local = wave - uniform(ft.wave_off[k]);
+  const int q = local / groups;
+  const int g = local - q * groups;
+  const int task = uniform(ft.task_first[k]) + q;
+  const int jl = g * 64 + lane;
+  const bool valid = jl < nt;
+  const int j = t0 + jl;
   double x0 = 0.0, x1 = 0.0, x2 = 0.0;
   int cur = 0, cend = 0;
   if (valid) {
     x0 = pts[3 * (size_t)j];
     x1 = pts[3 * (size_t)j + 1];
     x2 = pts[3 * (size_t)j + 2];
-    cur = ex_ptr[j];
-    cend = ex_ptr[j + 1];
+    if (ex_ptr) {
+      cur = ex_ptr[j];
+      cend = ex_ptr[j + 1];
+    }
   }
   acc_t ux = 0, uy = 0, uz = 0;
   int flag = OFW_FLAG_OK, far = 0, near = 0;
@@ -172,8 +206,8 @@ __global__ __launch_bounds__(256) void k_tree_eval(int n, const double* __restri
   const int flo = uniform(fi[0]);
 
   // ---- ancestors of the frontier node
-  for (int q = 0; q < nanc; ++q) {
-    const int A = uniform(T.anc[q]);
+  for (int a_i = 0; a_i < nanc; ++a_i) {
+    const int A = uniform(T.anc[a_i]);
     const double* a = accr + (size_t)A * ACC_REC;
     const int32_t* ai = reinterpret_cast<const int32_t*>(a + A_INTS);
     const int alo = ai[0], ahi = ai[1];
@@ -243,9 +277,9 @@ __global__ __launch_bounds__(256) void k_tree_eval(int n, const double* __restri
     if (skip == i + 1) {
       // leaf: the near field of the lanes that did not accept it (vortex_tree.py:538-557)
       if (wave_any(open)) {
-        for (int k = lo; k < hi; ++k) {
+        for (int s = lo; s < hi; ++s) {
           bool use = open;
-          if (use && cur < cend && ex_pos[cur] == k) {
+          if (use && cur < cend && ex_pos[cur] == s) {
             ++cur;
             use = false;
           }
@@ -254,7 +288,7 @@ __global__ __launch_bounds__(256) void k_tree_eval(int n, const double* __restri
             ++near;
             if (PREC == OFW_FP64) {
               double vx = 0.0, vy = 0.0, vz = 0.0;
-              const int f = pair_fast64<MODEL>(x0, x1, x2, rec64 + (size_t)k * REC64, raise_singular, vx, vy, vz);
+              const int f = pair_fast64<MODEL>(x0, x1, x2, rec64 + (size_t)s * REC64, raise_singular, vx, vy, vz);
               if (f != OFW_FLAG_OK) {
                 if (flag == OFW_FLAG_OK) flag = f;
               } else {
@@ -263,10 +297,10 @@ __global__ __launch_bounds__(256) void k_tree_eval(int n, const double* __restri
                 uz += vz;
               }
             } else {
-              const double* r = rec64 + (size_t)k * REC64;
+              const double* r = rec64 + (size_t)s * REC64;
               const float sx = (float)(x0 - r[R_SX]), sy = (float)(x1 - r[R_SY]), sz = (float)(x2 - r[R_SZ]);
               float vx = 0.0f, vy = 0.0f, vz = 0.0f;
-              if (pair_f32<MODEL>(sx, sy, sz, rec32 + (size_t)k * REC32, vx, vy, vz)) {
+              if (pair_f32<MODEL>(sx, sy, sz, rec32 + (size_t)s * REC32, vx, vy, vz)) {
                 double wx = 0.0, wy = 0.0, wz = 0.0;
                 const int f = pair_fast64<MODEL>(x0, x1, x2, r, raise_singular, wx, wy, wz);
                 if (f != OFW_FLAG_OK) {
@@ -292,19 +326,21 @@ __global__ __launch_bounds__(256) void k_tree_eval(int n, const double* __restri
     }
   }
   if (valid) {
-    const size_t o = (size_t)task * n + j;
+    const size_t o = (size_t)uniform(ft.part_off[k]) + (size_t)q * nt + jl;
     part[3 * o] = (double)ux;
     part[3 * o + 1] = (double)uy;
     part[3 * o + 2] = (double)uz;
     pflag[o] = (int8_t)flag;
-    pfar[o] = far;
-    pnear[o] = near;
+    if (pfar) {
+      pfar[o] = far;
+      pnear[o] = near;
+    }
   }
 }
 
+// Per target: its tree's subtree partials added in pre-order of the subtrees.
 template <bool WIDE>
-__global__ __launch_bounds__(256) void k_tree_reduce(int n, int ntasks, const int32_t* __restrict__ perm,
-                                                     const double* __restrict__ part,
+__global__ __launch_bounds__(256) void k_tree_reduce(ForestTables ft, int n, const double* __restrict__ part,
                                                      const int8_t* __restrict__ pflag,
                                                      const int32_t* __restrict__ pfar,
                                                      const int32_t* __restrict__ pnear, double* __restrict__ out,
@@ -313,44 +349,52 @@ __global__ __launch_bounds__(256) void k_tree_reduce(int n, int ntasks, const in
   typedef typename std::conditional<WIDE, double, float>::type acc_t;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
+  const int k = find_tree(ft.toff, ft.K, j);
+  const int t0 = ft.toff[k], nt = ft.toff[k + 1] - t0;
+  const int ntasks = ft.counters[C_OVERFLOW] ? 0 : ft.task_first[k + 1] - ft.task_first[k];
+  const size_t base = (size_t)ft.part_off[k] + (j - t0);
   acc_t ux = 0, uy = 0, uz = 0;
   int flag = 0;
   int64_t far = 0, near = 0;
   for (int q = 0; q < ntasks; ++q) {
-    const size_t o = (size_t)q * n + j;
+    const size_t o = base + (size_t)q * nt;
     ux += (acc_t)part[3 * o];
     uy += (acc_t)part[3 * o + 1];
     uz += (acc_t)part[3 * o + 2];
     if (flag == 0) flag = pflag[o];
-    far += pfar[o];
-    near += pnear[o];
+    if (pfar) {
+      far += pfar[o];
+      near += pnear[o];
+    }
   }
   if (!(isfinite((double)ux) && isfinite((double)uy) && isfinite((double)uz)) && flag == 0) flag = OFW_FLAG_NONFINITE;
-  const int t = perm ? perm[j] : j;
-  out[3 * (size_t)t] = (double)ux;
-  out[3 * (size_t)t + 1] = (double)uy;
-  out[3 * (size_t)t + 2] = (double)uz;
-  flags[t] = (int8_t)flag;
-  if (far_counts) far_counts[t] = far;
-  if (near_counts) near_counts[t] = near;
+  out[3 * (size_t)j] = (double)ux;
+  out[3 * (size_t)j + 1] = (double)uy;
+  out[3 * (size_t)j + 2] = (double)uz;
+  flags[j] = (int8_t)flag;
+  if (far_counts) far_counts[j] = far;
+  if (near_counts) near_counts[j] = near;
 }
 
 // ------------------------------------------------------------------ exclusions in tree positions
 __global__ void k_inverse(int m, const int32_t* __restrict__ order, int32_t* __restrict__ inverse) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < m) inverse[order[k]] = k;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < m) inverse[order[s]] = s;
 }
 
 // _positions (vortex_tree.py:571-580): per target, the sorted tree positions of its excluded segments
-__global__ void k_ex_positions(int n, const int32_t* __restrict__ ex_ptr, const int32_t* __restrict__ ex_seg,
+__global__ void k_ex_positions(int n, int m, const int32_t* __restrict__ ex_ptr, const int32_t* __restrict__ ex_seg,
                                const int32_t* __restrict__ inverse, int32_t* __restrict__ pos) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const int a = ex_ptr[p], b = ex_ptr[p + 1];
-  for (int k = a; k < b; ++k) pos[k] = inverse[ex_seg[k]];
-  for (int k = a + 1; k < b; ++k) {
-    const int v = pos[k];
-    int q = k - 1;
+  for (int s = a; s < b; ++s) {
+    const int seg = ex_seg[s];
+    pos[s] = (seg >= 0 && seg < m) ? inverse[seg] : INT_MAX;  // an id out of range excludes nothing
+  }
+  for (int s = a + 1; s < b; ++s) {
+    const int v = pos[s];
+    int q = s - 1;
     while (q >= a && pos[q] > v) {
       pos[q + 1] = pos[q];
       --q;
@@ -359,104 +403,194 @@ __global__ void k_ex_positions(int n, const int32_t* __restrict__ ex_ptr, const 
   }
 }
 
-// ------------------------------------------------------------------ host driver
-void frontier_tasks(const HostTree& t, int depth, std::vector<EvalTask>& tasks) {
-  tasks.clear();
-  const int count = (int)t.count;
-  for (int i = 0; i < count; ++i) {
-    const int d = t.pre_depth[i];
-    const bool leaf = t.pre_skip[i] == i + 1;
-    if (d == depth || (d < depth && leaf)) {
-      EvalTask task{};
-      task.node = i;
-      task.nanc = d;
-      int p = t.pre_parent[i];
-      for (int q = d - 1; q >= 0; --q) {
-        task.anc[q] = p;
-        p = t.pre_parent[p];
-      }
-      tasks.push_back(task);
+// ------------------------------------------------------------------ traversal plan on the device
+// frontier nodes: depth == D, or leaves shallower than D (pre-order flags; zero past the node count)
+__global__ void k_task_flags(TreeArrays t, int D, int32_t* __restrict__ flag) {
+  const int count = live_count(t.counters, C_FINAL);
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p <= t.cap; p += gridDim.x * blockDim.x) {
+    int f = 0;
+    if (p < count) {
+      const int d = t.p_depth[p];
+      f = (d == D || (d < D && t.p_skip[p] == p + 1)) ? 1 : 0;
     }
+    flag[p] = f;
   }
 }
 
+__global__ void k_task_fill(TreeArrays t, const int32_t* __restrict__ flag, const int32_t* __restrict__ index,
+                            int task_cap, EvalTask* __restrict__ tasks) {
+  const int count = t.counters[C_OVERFLOW] ? 0 : t.counters[C_FINAL];
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < count; p += gridDim.x * blockDim.x) {
+    if (!flag[p]) continue;
+    const int idx = index[p];
+    if (idx >= task_cap) {
+      t.counters[C_OVERFLOW] = 1;
+      continue;
+    }
+    EvalTask task;
+    const int d = t.p_depth[p];
+    task.node = p;
+    task.nanc = d;
+    int a = t.p_parent[p];
+    for (int q = d - 1; q >= 0; --q) {
+      task.anc[q] = a;
+      a = t.p_parent[a];
+    }
+    for (int q = d; q < MAX_ANC; ++q) task.anc[q] = -1;
+    tasks[idx] = task;
+  }
+}
+
+// One workgroup: per tree, its tasks (from the flag scan at the tree's first node), work items and partials.
+__global__ __launch_bounds__(1024) void k_work_plan(TreeArrays t, const int32_t* __restrict__ index,
+                                                    const int32_t* __restrict__ toff, int32_t* __restrict__ task_first,
+                                                    int32_t* __restrict__ wave_off, int32_t* __restrict__ part_off,
+                                                    int wave_cap, int part_cap) {
+  int waves = 0, parts = 0;
+  for (int base = 0; base <= t.K; base += blockDim.x) {
+    const int k = base + threadIdx.x;
+    int w = 0, pp = 0;
+    if (k < t.K) {
+      const int first = index[t.tree_first[k]];
+      const int ntasks = index[t.tree_first[k + 1]] - first;
+      const int nt = toff[k + 1] - toff[k];
+      task_first[k] = first;
+      w = ntasks * ((nt + 63) >> 6);
+      pp = ntasks * nt;
+    } else if (k == t.K) {
+      task_first[k] = index[t.tree_first[k]];
+    }
+    int tw, tp;
+    const int rw = block_exclusive_scan(w, &tw);
+    const int rp = block_exclusive_scan(pp, &tp);
+    if (k <= t.K) {
+      wave_off[k] = waves + rw;
+      part_off[k] = parts + rp;
+    }
+    waves += tw;
+    parts += tp;
+  }
+  if (threadIdx.x == 0) {
+    if (waves > wave_cap || parts > part_cap) {
+      t.counters[C_OVERFLOW] = 1;
+      waves = 0;
+    }
+    t.counters[C_WORK] = waves;
+    t.counters[C_TASKS] = index[t.tree_first[t.K]];
+  }
+}
+
+// ------------------------------------------------------------------ host driver
 template <int MODEL>
-static void launch_eval(int precision, dim3 grid, hipStream_t st, int n, const double* pts, const int32_t* ex_ptr,
-                        const int32_t* ex_pos, const EvalTask* tasks, const TreeDevice& td, double theta, int order,
-                        int raise, double* part, int8_t* pflag, int32_t* pfar, int32_t* pnear) {
+static void launch_eval(int precision, dim3 grid, hipStream_t st, const ForestTables& ft, const double* pts,
+                        const int32_t* ex_ptr, const int32_t* ex_pos, const EvalTask* tasks, const TreeDevice& td,
+                        double theta, int order, int raise, double* part, int8_t* pflag, int32_t* pfar,
+                        int32_t* pnear) {
   if (precision == OFW_FP64)
-    hipLaunchKernelGGL((k_tree_eval<MODEL, OFW_FP64>), grid, dim3(256), 0, st, n, pts, ex_ptr, ex_pos, tasks, td.acc,
+    hipLaunchKernelGGL((k_tree_eval<MODEL, OFW_FP64>), grid, dim3(256), 0, st, ft, pts, ex_ptr, ex_pos, tasks, td.acc,
                        td.far64, td.far32, td.rec64, td.rec32, theta, order, raise, part, pflag, pfar, pnear);
   else if (precision == OFW_MIXED)
-    hipLaunchKernelGGL((k_tree_eval<MODEL, OFW_MIXED>), grid, dim3(256), 0, st, n, pts, ex_ptr, ex_pos, tasks, td.acc,
-                       td.far64, td.far32, td.rec64, td.rec32, theta, order, raise, part, pflag, pfar, pnear);
+    hipLaunchKernelGGL((k_tree_eval<MODEL, OFW_MIXED>), grid, dim3(256), 0, st, ft, pts, ex_ptr, ex_pos, tasks,
+                       td.acc, td.far64, td.far32, td.rec64, td.rec32, theta, order, raise, part, pflag, pfar, pnear);
   else
-    hipLaunchKernelGGL((k_tree_eval<MODEL, OFW_FP32>), grid, dim3(256), 0, st, n, pts, ex_ptr, ex_pos, tasks, td.acc,
+    hipLaunchKernelGGL((k_tree_eval<MODEL, OFW_FP32>), grid, dim3(256), 0, st, ft, pts, ex_ptr, ex_pos, tasks, td.acc,
                        td.far64, td.far32, td.rec64, td.rec32, theta, order, raise, part, pflag, pfar, pnear);
 }
 
-int tree_evaluate(Ctx& c, const HostTree& t, const TreeDevice& td, const int32_t* order, int precision, int64_t n64,
-                  const double* points, int model, int raise_singular, const int32_t* ex_ptr,
-                  const int32_t* ex_seg, double theta, int expansion_order, int split_depth, double* out,
-                  int8_t* flags, int64_t* far_counts, int64_t* near_counts, std::vector<EvalTask>& tasks) {
-  const int n = (int)n64, m = (int)t.m;
-  const int groups = (n + 63) / 64;
-  int depth = split_depth;
-  if (depth < 0) {
-    depth = 0;
-    const int want = c.cu_count * 4 * 8;
-    while ((groups << depth) < want && depth < 8) ++depth;
+int choose_split_depth(const Ctx& c, int64_t groups, int K) {
+  (void)K;
+  const int64_t want = (int64_t)c.cu_count * 4 * 8;
+  int depth = 0;
+  while ((groups << depth) < want && depth < 8) ++depth;
+  return std::min(depth, MAX_ANC);
+}
+
+int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, const EvalShape& shape, int precision,
+                  const double* points, int model, int raise_singular, const int32_t* ex_ptr, const int32_t* ex_seg,
+                  int64_t n_ex, double theta, int expansion_order, double* out, int8_t* flags, int64_t* far_counts,
+                  int64_t* near_counts) {
+  const int K = shape.K;
+  const int64_t n = shape.n, m = t.m;
+  const int D = std::max(0, std::min(shape.split_depth, MAX_ANC));
+  // host bounds: a tree has at most min(2^D, m_k) frontier subtrees
+  int64_t task_cap = 0, wave_cap = 0, part_cap = 0;
+  for (int k = 0; k < K; ++k) {
+    const int64_t mk = shape.seg_off[k + 1] - shape.seg_off[k];
+    const int64_t nk = shape.target_off[k + 1] - shape.target_off[k];
+    const int64_t tasks_k = mk > 0 ? std::min<int64_t>((int64_t)1 << D, mk) : 0;
+    task_cap += tasks_k;
+    wave_cap += tasks_k * ((nk + 63) / 64);
+    part_cap += tasks_k * nk;
   }
-  depth = std::min(std::min(depth, t.depth), MAX_ANC);
-  frontier_tasks(t, depth, tasks);
-  const int ntasks = (int)tasks.size();
-  // scratch
-  int64_t e = 0;
-  OFW_HIP(hipMemcpyAsync(&e, ex_ptr + n, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
-  OFW_HIP(hipStreamSynchronize(c.stream));
-  e = (int32_t)e;
+  if (part_cap > INT_MAX || wave_cap > INT_MAX / 4) {
+    set_error("tree traversal: split depth too large for the target count");
+    return OFW_ERR_ARG;
+  }
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-  size_t need = al((size_t)m * 4) + al((size_t)(e > 0 ? e : 1) * 4) + al((size_t)ntasks * sizeof(EvalTask)) +
-                al((size_t)ntasks * n * 3 * 8) + al((size_t)ntasks * n) + 2 * al((size_t)ntasks * n * 4);
-  OFW_HIP(c.tree[2].reserve(need));
-  char* base = (char*)c.tree[2].ptr;
+  const size_t P = (size_t)std::max<int64_t>(part_cap, 1);
+  const bool counts = far_counts || near_counts;
+  size_t need = al((size_t)std::max<int64_t>(m, 1) * 4) + al((size_t)std::max<int64_t>(n_ex, 1) * 4);
+  need += 2 * al(((size_t)t.cap + 1) * 4) + al((size_t)std::max<int64_t>(task_cap, 1) * sizeof(EvalTask));
+  need += 4 * al((size_t)(K + 1) * 4) + al(P * 24) + al(P) + (counts ? 2 * al(P * 4) : 0);
+  OFW_HIP(c.tree[buf].reserve(need));
+  char* base = (char*)c.tree[buf].ptr;
   auto carve = [&](size_t bytes) {
     char* p = base;
     base += al(bytes);
     return (void*)p;
   };
-  int32_t* inverse = (int32_t*)carve((size_t)m * 4);
-  int32_t* ex_pos = (int32_t*)carve((size_t)(e > 0 ? e : 1) * 4);
-  EvalTask* d_tasks = (EvalTask*)carve((size_t)ntasks * sizeof(EvalTask));
-  double* part = (double*)carve((size_t)ntasks * n * 3 * 8);
-  int8_t* pflag = (int8_t*)carve((size_t)ntasks * n);
-  int32_t* pfar = (int32_t*)carve((size_t)ntasks * n * 4);
-  int32_t* pnear = (int32_t*)carve((size_t)ntasks * n * 4);
-  OFW_HIP(hipMemcpyAsync(d_tasks, tasks.data(), (size_t)ntasks * sizeof(EvalTask), hipMemcpyHostToDevice, c.stream));
-  if (e > 0) {
-    hipLaunchKernelGGL(k_inverse, dim3((m + 255) / 256), dim3(256), 0, c.stream, m, order, inverse);
-    hipLaunchKernelGGL(k_ex_positions, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, ex_ptr, ex_seg, inverse,
-                       ex_pos);
+  int32_t* inverse = (int32_t*)carve((size_t)std::max<int64_t>(m, 1) * 4);
+  int32_t* ex_pos = (int32_t*)carve((size_t)std::max<int64_t>(n_ex, 1) * 4);
+  int32_t* flag = (int32_t*)carve(((size_t)t.cap + 1) * 4);
+  int32_t* index = (int32_t*)carve(((size_t)t.cap + 1) * 4);
+  EvalTask* tasks = (EvalTask*)carve((size_t)std::max<int64_t>(task_cap, 1) * sizeof(EvalTask));
+  int32_t* toff = (int32_t*)carve((size_t)(K + 1) * 4);
+  int32_t* task_first = (int32_t*)carve((size_t)(K + 1) * 4);
+  int32_t* wave_off = (int32_t*)carve((size_t)(K + 1) * 4);
+  int32_t* part_off = (int32_t*)carve((size_t)(K + 1) * 4);
+  double* part = (double*)carve(P * 24);
+  int8_t* pflag = (int8_t*)carve(P);
+  int32_t* pfar = counts ? (int32_t*)carve(P * 4) : nullptr;
+  int32_t* pnear = counts ? (int32_t*)carve(P * 4) : nullptr;
+
+  int st = upload_i32(c, shape.target_off, toff);
+  if (st) return st;
+  const int grid = c.cu_count * 2;
+  if (n_ex > 0 && ex_ptr) {
+    hipLaunchKernelGGL(k_inverse, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, c.stream, (int)m, t.order,
+                       inverse);
+    hipLaunchKernelGGL(k_ex_positions, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, (int)n, (int)m,
+                       ex_ptr, ex_seg, inverse, ex_pos);
+  }
+  const int32_t* ex_ptr_used = (n_ex > 0) ? ex_ptr : nullptr;
+  hipLaunchKernelGGL(k_task_flags, dim3(grid), dim3(256), 0, c.stream, t, D, flag);
+  scan_exclusive(c, t.cap + 1, flag, index, t.scan_part, nullptr);
+  hipLaunchKernelGGL(k_task_fill, dim3(grid), dim3(256), 0, c.stream, t, flag, index, (int)task_cap, tasks);
+  hipLaunchKernelGGL(k_work_plan, dim3(1), dim3(1024), 0, c.stream, t, index, toff, task_first, wave_off, part_off,
+                     (int)wave_cap, (int)part_cap);
+  OFW_HIP(hipGetLastError());
+  OFW_HIP(c.mark());
+  ForestTables ft{K, toff, task_first, wave_off, part_off, t.counters};
+  if (wave_cap > 0) {
+    dim3 egrid((unsigned)((wave_cap + 3) / 4));
+    switch (model) {
+      case 0: launch_eval<0>(precision, egrid, c.stream, ft, points, ex_ptr_used, ex_pos, tasks, td, theta, expansion_order, raise_singular, part, pflag, pfar, pnear); break;
+      case 1: launch_eval<1>(precision, egrid, c.stream, ft, points, ex_ptr_used, ex_pos, tasks, td, theta, expansion_order, raise_singular, part, pflag, pfar, pnear); break;
+      default: launch_eval<2>(precision, egrid, c.stream, ft, points, ex_ptr_used, ex_pos, tasks, td, theta, expansion_order, raise_singular, part, pflag, pfar, pnear); break;
+    }
   }
   OFW_HIP(hipGetLastError());
   OFW_HIP(c.mark());
-  dim3 grid((unsigned)((groups + 3) / 4), (unsigned)ntasks);
-  switch (model) {
-    case 0: launch_eval<0>(precision, grid, c.stream, n, points, ex_ptr, ex_pos, d_tasks, td, theta, expansion_order, raise_singular, part, pflag, pfar, pnear); break;
-    case 1: launch_eval<1>(precision, grid, c.stream, n, points, ex_ptr, ex_pos, d_tasks, td, theta, expansion_order, raise_singular, part, pflag, pfar, pnear); break;
-    default: launch_eval<2>(precision, grid, c.stream, n, points, ex_ptr, ex_pos, d_tasks, td, theta, expansion_order, raise_singular, part, pflag, pfar, pnear); break;
+  if (n > 0) {
+    if (precision == OFW_FP32)
+      hipLaunchKernelGGL(k_tree_reduce<false>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, ft, (int)n,
+                         part, pflag, pfar, pnear, out, flags, far_counts, near_counts);
+    else
+      hipLaunchKernelGGL(k_tree_reduce<true>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, ft, (int)n,
+                         part, pflag, pfar, pnear, out, flags, far_counts, near_counts);
   }
   OFW_HIP(hipGetLastError());
-  OFW_HIP(c.mark());
-  if (precision == OFW_FP32)
-    hipLaunchKernelGGL(k_tree_reduce<false>, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, ntasks,
-                       (const int32_t*)nullptr, part, pflag, pfar, pnear, out, flags, far_counts, near_counts);
-  else
-    hipLaunchKernelGGL(k_tree_reduce<true>, dim3((n + 255) / 256), dim3(256), 0, c.stream, n, ntasks,
-                       (const int32_t*)nullptr, part, pflag, pfar, pnear, out, flags, far_counts, near_counts);
-  OFW_HIP(hipGetLastError());
-  c.stats_i[2] = ntasks;
-  c.stats_i[3] = depth;
+  c.stats_i[3] = D;
   return OFW_OK;
 }
 

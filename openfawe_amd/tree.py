@@ -10,7 +10,8 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from .vortex import HipSettings, Statistics, TREE_MODELS, TreeSettings, _ctx, _dev, _ptr, _torch, _upload
+from .vortex import (HipSettings, Statistics, TREE_MODELS, TreeSettings, _ctx, _dev, _ptr, _torch, _upload,
+                     exclusion_csr)
 
 
 def capacity(m, leaf_size):
@@ -105,3 +106,89 @@ def hip_tree_sum(points, starts, directions, lengths, gamma, radius, model, csr,
                            nodes=int(stats[4]), depth=int(stats[5]), subtrees=int(stats[2]),
                            split_depth=int(stats[3]), phases_ms=_lib.timings(_ctx()))
     return out.cpu().numpy(), bool(np.any(host_flags))
+
+
+class TreeBatch:
+    """Many independent snapshots in one tree-code call, recorded once and replayed (ofw_tree_batch_*).
+
+    ``snapshots`` is a list of dicts with the engine arrays of one snapshot each (``points``, ``starts``,
+    ``directions``, ``lengths``, ``gamma``, ``radius``; host or device).  They are packed back to back on the
+    device; ``arrays`` holds the packed device tensors, whose contents may be changed in place between runs
+    (the recorded call reads them at every replay).  Each snapshot's result equals ``hip_tree_sum`` on the
+    snapshot alone with the same ``split_depth``, bit for bit.
+    """
+
+    def __init__(self, snapshots, tree, *, model=0, precision="mixed", split_depth=-1, use_graph=True,
+                 singularity="zero", counts=False):
+        torch = _torch()
+        if not isinstance(tree, TreeSettings):
+            raise ValueError("tree must be TreeSettings")
+        names = ("points", "starts", "directions", "lengths", "gamma", "radius")
+        self.K = len(snapshots)
+        if self.K == 0:
+            raise ValueError("a batch needs at least one snapshot")
+        n_k = [int(len(s["points"])) for s in snapshots]
+        m_k = [int(len(s["starts"])) for s in snapshots]
+        self.target_offsets = np.concatenate([[0], np.cumsum(n_k)]).astype(np.int64)
+        self.segment_offsets = np.concatenate([[0], np.cumsum(m_k)]).astype(np.int64)
+        packed = {}
+        for name in names:
+            parts = [torch.as_tensor(s[name], dtype=torch.float64).to("cuda").reshape(
+                (-1, 3) if name in ("points", "starts", "directions") else (-1,)) for s in snapshots]
+            packed[name] = torch.cat(parts).contiguous()
+        self.arrays = packed
+        n = int(self.target_offsets[-1])
+        # exclusions: each snapshot's (target, segment) pairs, local to the snapshot, shifted to batch ids
+        pairs = [np.asarray(s.get("pairs", np.zeros((0, 2))), dtype=np.int64).reshape(-1, 2)
+                 + [self.target_offsets[k], self.segment_offsets[k]] for k, s in enumerate(snapshots)]
+        pairs = np.concatenate(pairs) if pairs else np.zeros((0, 2), np.int64)
+        self.ex_ptr = self.ex_seg = None
+        if len(pairs):
+            ptr, seg = exclusion_csr(n, pairs)
+            self.ex_ptr = torch.as_tensor(ptr).to("cuda")
+            self.ex_seg = torch.as_tensor(seg).to("cuda")
+        self.out = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
+        self.flags = torch.zeros(n, dtype=torch.int8, device="cuda")
+        self.far_counts = torch.zeros(n, dtype=torch.int64, device="cuda") if counts else None
+        self.near_counts = torch.zeros(n, dtype=torch.int64, device="cuda") if counts else None
+        self._lib = _lib.load()
+        self._plan = ctypes.c_void_p()
+        toff = (ctypes.c_int64 * (self.K + 1))(*self.target_offsets.tolist())
+        soff = (ctypes.c_int64 * (self.K + 1))(*self.segment_offsets.tolist())
+        status = self._lib.ofw_tree_batch_create(
+            _ctx(), ctypes.byref(self._plan), _lib.PRECISIONS[precision], self.K, toff, soff,
+            *[_ptr(packed[name]) for name in names], int(model), int(singularity == "raise"), _ptr(self.ex_ptr),
+            _ptr(self.ex_seg),
+            float(tree.theta), int(tree.order), int(tree.leaf_size), float(tree.core_clearance), int(split_depth),
+            int(bool(use_graph)), _ptr(self.out), _ptr(self.flags), _ptr(self.far_counts), _ptr(self.near_counts))
+        _lib.check(status, "ofw_tree_batch_create")
+
+    def run(self):
+        """Replay the call; returns the packed device velocities (n, 3)."""
+        _lib.check(self._lib.ofw_tree_batch_run(self._plan), "ofw_tree_batch_run")
+        return self.out
+
+    def split(self, array):
+        """Per-snapshot views of a packed per-target array."""
+        return [array[a:b] for a, b in zip(self.target_offsets[:-1], self.target_offsets[1:])]
+
+    def timings(self):
+        buf = (ctypes.c_float * 8)()
+        phases = self._lib.ofw_tree_batch_timings(self._plan, buf, 8)
+        return [float(buf[k]) for k in range(max(phases, 0))]
+
+    def stats(self):
+        buf = (ctypes.c_int64 * 8)()
+        self._lib.ofw_tree_batch_stats(self._plan, buf, 8)
+        return dict(subtrees=int(buf[2]), split_depth=int(buf[3]), nodes=int(buf[4]), depth=int(buf[5]))
+
+    def close(self):
+        if self._plan:
+            self._lib.ofw_tree_batch_destroy(self._plan)
+            self._plan = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -7,6 +7,7 @@
 * ``loop`` -- a crosswind-loop wake: rows of wing shed and trailing segments along a circular loop of
   radius 145 m convected downwind, plus 8 rotor tip-vortex helices; targets are the first 187 rows of
   wake nodes with their incident segments excluded (the wake-on-wake convection call of wake.py:404-491).
+* ``batch`` -- uneven snapshots of ``loop`` at random loop phases and wind speeds, for the batched call.
 * ``helix`` -- 40 rows x 256 spanwise segments on a helix of radius 150 m and pitch 40 m, Vatistas cores.
 * ``random_network`` -- the reference's backend-equivalence network (tests/test_vortex_numba.py:38-54).
 * ``lattice`` and ``ring`` -- the reference's tree-test geometries (tests/test_vortex_tree.py:18-40).
@@ -29,14 +30,15 @@ def incident_pairs(point_nodes, edges):
 
     ``point_nodes[i]`` is the node index of target i.  The rule of wake.py:427-438 (incident_pairs).
     """
-    lookup = {int(n): i for i, n in enumerate(point_nodes)}
-    rows = []
-    for s, (a, b) in enumerate(edges):
-        for node in (int(a), int(b)):
-            i = lookup.get(node)
-            if i is not None:
-                rows.append((i, s))
-    return np.asarray(rows, dtype=np.int64).reshape(-1, 2)
+    point_nodes = np.asarray(point_nodes, dtype=np.int64)
+    edges = np.asarray(edges, dtype=np.int64).reshape(-1, 2)
+    size = int(max(edges.max(initial=-1), point_nodes.max(initial=-1))) + 1
+    lookup = np.full(size, -1, dtype=np.int64)
+    lookup[point_nodes[::-1]] = np.arange(len(point_nodes))[::-1]   # the first target of a repeated node
+    target = lookup[edges]                                            # (m, 2): start, end
+    seg = np.broadcast_to(np.arange(len(edges))[:, None], target.shape)
+    keep = target >= 0                                                # segment order, start before end
+    return np.stack([target[keep], seg[keep]], axis=1).astype(np.int64).reshape(-1, 2)
 
 
 def sheet(rows, nodes_per_row=110, span=26.0, dx=0.45, seed=None):
@@ -84,20 +86,18 @@ def loop(rows=468, seed=0, nodes_per_row=110, span=26.0, radius_loop=145.0, wind
     mid_eta = 0.5 * (eta[:-1] + eta[1:])
     profile = 8.0 * np.sqrt(np.clip(1 - (2 * mid_eta) ** 2, 0, None))
     panel = profile[None, :] * (1 + 0.05 * rng.standard_normal((rows, nodes_per_row - 1)))
-    edges, gamma = [], []
-    for r in range(rows):
-        base = r * nodes_per_row
-        older = panel[r + 1] if r + 1 < rows else np.zeros(nodes_per_row - 1)
-        for j in range(nodes_per_row - 1):            # shed (spanwise) segments
-            edges.append((base + j, base + j + 1))
-            gamma.append(panel[r, j] - older[j])
-        if r + 1 < rows:                                # trailing segments towards the older row
-            padded = np.concatenate([[0.0], panel[r], [0.0]])
-            for j in range(nodes_per_row):
-                edges.append((base + j, base + nodes_per_row + j))
-                gamma.append(padded[j + 1] - padded[j])
-    edges = np.asarray(edges, dtype=np.int64)
-    gamma = np.asarray(gamma)
+    # per row: the shed (spanwise) segments, then the trailing segments towards the older row (none after
+    # the oldest row)
+    npr = nodes_per_row
+    base = (np.arange(rows) * npr)[:, None]
+    j1, j2 = np.arange(npr - 1)[None, :], np.arange(npr)[None, :]
+    shed = np.stack([base + j1, base + j1 + 1], axis=-1)
+    trail = np.stack([base + j2, base + npr + j2], axis=-1)
+    older = np.vstack([panel[1:], np.zeros((1, npr - 1))])
+    padded = np.pad(panel, ((0, 0), (1, 1)))
+    kept = rows * (2 * npr - 1) - npr
+    edges = np.concatenate([shed, trail], axis=1).reshape(-1, 2)[:kept].astype(np.int64)
+    gamma = np.concatenate([panel - older, padded[:, 1:] - padded[:, :-1]], axis=1).reshape(-1)[:kept]
     radius = np.full(len(edges), core)
     # 8 rotors x 4 blades: tip-vortex helices of radius 1.15 m trailing from the rotors
     rotor_nodes, rotor_edges, rotor_gamma = [], [], []
@@ -126,6 +126,29 @@ def loop(rows=468, seed=0, nodes_per_row=110, span=26.0, radius_loop=145.0, wind
     return dict(points=nodes[target_nodes].copy(), starts=starts, directions=directions, lengths=lengths,
                 gamma=gamma, radius=radius, pairs=incident_pairs(target_nodes, edges), model=2,
                 nodes=nodes, edges=edges, name=f"loop-{rows}")
+
+
+def batch(count=64, seed=4, m_lo=10_000, m_hi=410_000, n_lo=2_000, n_hi=20_000):
+    """``count`` independent snapshots of the loop wake at random loop phases and wind speeds of 6-14 m/s.
+
+    Segment counts are drawn log-uniform in [m_lo, m_hi] (the loop gives 219 segments a row plus 2,048
+    rotor segments, so each snapshot has the row count nearest the draw); target counts follow the segment
+    counts log-linearly from n_lo (at m_lo) to n_hi (at m_hi).  Targets are the first wake nodes with their
+    incident segments excluded, as in ``loop``.
+    """
+    rng = np.random.default_rng(seed)
+    snaps = []
+    for k in range(count):
+        m_want = float(np.exp(rng.uniform(np.log(m_lo), np.log(m_hi))))
+        frac = (np.log(m_want) - np.log(m_lo)) / (np.log(m_hi) - np.log(m_lo))
+        n = int(round(np.exp(np.log(n_lo) + frac * (np.log(n_hi) - np.log(n_lo)))))
+        rows = max(2, int(round((m_want - 2048 + 110) / 219)))
+        target_rows = int(np.ceil(n / 110))
+        case = loop(rows=max(rows, target_rows + 1), seed=seed * 1000 + k, phase=float(rng.uniform(0, 2 * np.pi)),
+                    wind=float(rng.uniform(6.0, 14.0)), target_rows=target_rows)
+        pairs = case["pairs"][case["pairs"][:, 0] < n]
+        snaps.append(dict(case, points=case["points"][:n].copy(), pairs=pairs, name=f"batch-{k}"))
+    return snaps
 
 
 def helix(seed=0, rows=40, spanwise=256, radius_helix=150.0, pitch=40.0, span=26.0, row_spacing=2.0,

@@ -190,3 +190,83 @@ def test_api_errors_flags_and_determinism():
     assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
     empty = V.segments_velocity(np.empty((0, 3)), nodes, edges, gamma, backend="hip_tree", tree=settings)
     assert empty.shape == (0, 3)
+
+
+def batch_snapshots(sizes, seed=11):
+    """Snapshots of (targets, segments) from the loop generator at several phases; empty ones included."""
+    rng = np.random.default_rng(seed)
+    snaps = []
+    for k, (n, m) in enumerate(sizes):
+        rows = max(2, int(np.ceil(m / 220)))
+        case = workloads.loop(rows=rows, phase=float(rng.uniform(0, 2 * np.pi)), wind=float(rng.uniform(6, 14)))
+        take = slice(0, m)
+        pts = case["points"][rng.choice(len(case["points"]), size=n, replace=n > len(case["points"]))]
+        snaps.append(dict(points=pts + 0.01 * rng.normal(size=pts.shape) if n else np.zeros((0, 3)),
+                          starts=case["starts"][take], directions=case["directions"][take],
+                          lengths=case["lengths"][take], gamma=case["gamma"][take],
+                          radius=case["radius"][take]))
+    return snaps
+
+
+def single_call(snap, settings, precision, split_depth):
+    torch = V._torch()
+    n, m = len(snap["points"]), len(snap["starts"])
+    if n == 0:
+        return np.zeros((0, 3)), np.zeros(0, np.int64), np.zeros(0, np.int64)
+    dev = [torch.as_tensor(np.ascontiguousarray(snap[k]), dtype=torch.float64).cuda()
+           for k in ("points", "starts", "directions", "lengths", "gamma", "radius")]
+    ptr, seg = V.exclusion_csr(n, snap.get("pairs", np.zeros((0, 2), np.int64)))
+    ex_ptr = torch.as_tensor(ptr).cuda()
+    ex_seg = torch.as_tensor(seg if len(seg) else np.zeros(1, np.int64)).cuda()
+    out, _, far, near = T.hip_tree_sum_device(*dev, 2, ex_ptr, ex_seg, "zero", settings, V.HipSettings(precision),
+                                              split_depth=split_depth)
+    return out.cpu().numpy(), far.cpu().numpy(), near.cpu().numpy()
+
+
+@pytest.mark.parametrize("use_graph", [False, True], ids=["eager", "graph"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_equals_single_calls_bitwise(use_graph, precision):
+    """Every snapshot of a batch (empty ones included) equals its own single call, bit for bit, on every
+    replay, also after the inputs change in place between replays."""
+    sizes = [(300, 1500), (0, 700), (40, 0), (900, 6000), (64, 65), (1, 3000)]
+    snaps = batch_snapshots(sizes)
+    settings = V.TreeSettings(theta=0.5, order=2, leaf_size=16, core_clearance=3.0)
+    batch = T.TreeBatch(snaps, settings, model=2, precision=precision, split_depth=3, use_graph=use_graph,
+                        counts=True)
+    try:
+        for replay in range(3):
+            if replay == 2:
+                batch.arrays["gamma"].mul_(-0.75)
+                for s in snaps:
+                    s["gamma"] = s["gamma"] * -0.75
+            out = batch.run().cpu().numpy()
+            far = batch.far_counts.cpu().numpy()
+            near = batch.near_counts.cpu().numpy()
+            for k, snap in enumerate(snaps):
+                a, b = batch.target_offsets[k], batch.target_offsets[k + 1]
+                ref, far_ref, near_ref = single_call(snap, settings, precision, 3)
+                assert np.array_equal(out[a:b].view(np.uint64), ref.view(np.uint64)), (replay, k)
+                if len(snap["starts"]):
+                    assert np.array_equal(far[a:b], far_ref) and np.array_equal(near[a:b], near_ref), (replay, k)
+                else:
+                    assert not out[a:b].any()
+        assert batch.stats()["split_depth"] == 3
+    finally:
+        batch.close()
+
+
+def test_batch_with_exclusions_and_default_depth():
+    """The batch workload generator (uneven loop snapshots, incident segments excluded) at a reduced size:
+    graph replay equals the single calls bit for bit at the batch's own split depth."""
+    snaps = workloads.batch(count=5, m_lo=3_000, m_hi=30_000, n_lo=150, n_hi=1_200)
+    settings = V.TreeSettings(theta=0.5, order=2, leaf_size=32, core_clearance=3.0)
+    batch = T.TreeBatch(snaps, settings, model=2, precision="mixed", use_graph=True, counts=True)
+    try:
+        out = batch.run().cpu().numpy()
+        depth = batch.stats()["split_depth"]
+        for k, snap in enumerate(snaps):
+            a, b = batch.target_offsets[k], batch.target_offsets[k + 1]
+            ref, _, _ = single_call(snap, settings, "mixed", depth)
+            assert np.array_equal(out[a:b].view(np.uint64), ref.view(np.uint64)), k
+    finally:
+        batch.close()
