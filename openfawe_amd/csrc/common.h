@@ -42,10 +42,16 @@ enum {
 #define OFW_FAST_A2_MIN 1e-200
 #define OFW_FAST_RC_MIN 1e-50
 
-// Mixed precision: a pair goes to FP64 when the target is within NEAR_CORES core radii of the
-// segment's line, or within NEAR_AXIS segment lengths (scaled like the axis guard) of it.
+// Mixed precision: a pair goes to FP64 when the target is within NEAR_CORES core radii of the segment
+// (its closest point), beside the segment within NEAR_BESIDE of its line, or within NEAR_AXIS of its line
+// anywhere (distances relative to max(L, |p - s|, |p - e|), as the axis guard scales them).  NEAR_AXIS is
+// 1e4 times the FP32 rounding of the cross product, so every pair the reference's FP64 axis guard catches
+// (8 eps64 relative) reaches the FP64 path and takes the same branch there.  A target on the line's
+// extension beyond either end needs no FP64: there the denominator does not cancel and the pair velocity
+// vanishes with the distance to the line.
 #define OFW_NEAR_CORES2 16.0f          // (4 core radii)^2
-#define OFW_NEAR_AXIS2 1e-6f           // (1e-3)^2
+#define OFW_NEAR_BESIDE2 1e-6f         // (1e-3)^2
+#define OFW_NEAR_AXIS2 1e-8f           // (1e-4)^2
 #define OFW_F32_A2_MAX 1e30f
 
 struct PairOut {
@@ -56,9 +62,10 @@ struct PairOut {
 __device__ __forceinline__ double ofw_norm3(double x, double y, double z) { return hypot(hypot(x, y), z); }
 
 // Operation-by-operation restatement of vortex_numba.py:42-109 (pair_velocity); the FP64 fallback for
-// extreme ranges (huge |a|, tiny cores).  Accumulates into u when the flag is OK.
-__device__ __noinline__ int pair_exact64(double px, double py, double pz, const double* __restrict__ r,
-                                         int model, int raise_singular, double& ux, double& uy, double& uz) {
+// extreme ranges (huge |a|, tiny cores).  Returns the pair velocity and its flag by value (nothing passes
+// through memory on the call).
+__device__ __noinline__ PairOut pair_exact64(double px, double py, double pz, const double* __restrict__ r,
+                                             int model, int raise_singular) {
 #pragma clang fp contract(off)
   const double sx = r[R_SX], sy = r[R_SY], sz = r[R_SZ];
   const double dx = r[R_DX], dy = r[R_DY], dz = r[R_DZ];
@@ -67,14 +74,14 @@ __device__ __noinline__ int pair_exact64(double px, double py, double pz, const 
   double t = (ax * dx + ay * dy) + az * dz;
   double cx = dy * az - dz * ay, cy = dz * ax - dx * az, cz = dx * ay - dy * ax;
   double h = ofw_norm3(cx, cy, cz), na = ofw_norm3(ax, ay, az), nb = ofw_norm3(ax - dx, ay - dy, az - dz);
-  if (!(isfinite(h) && isfinite(na) && isfinite(nb) && isfinite(t))) return OFW_FLAG_NONFINITE;
+  if (!(isfinite(h) && isfinite(na) && isfinite(nb) && isfinite(t))) return PairOut{0.0, 0.0, 0.0, OFW_FLAG_NONFINITE};
   double big = na > nb ? na : nb;
   double scale = 1.0 > big ? 1.0 : big;
   if (h <= OFW_AXIS_EPS * scale) {
-    if (raise_singular && t >= 0.0 && t <= 1.0) return OFW_FLAG_SINGULAR;
-    return OFW_FLAG_OK;
+    if (raise_singular && t >= 0.0 && t <= 1.0) return PairOut{0.0, 0.0, 0.0, OFW_FLAG_SINGULAR};
+    return PairOut{0.0, 0.0, 0.0, OFW_FLAG_OK};
   }
-  if (gamma == 0.0) return OFW_FLAG_OK;
+  if (gamma == 0.0) return PairOut{0.0, 0.0, 0.0, OFW_FLAG_OK};
   double product = na * nb;
   double dot = h * h + t * (t - 1.0);
   double plus = product + dot;
@@ -96,7 +103,7 @@ __device__ __noinline__ int pair_exact64(double px, double py, double pz, const 
       denominator = denominator + rc * rc;
     } else {
       double ratio = h / rc;
-      if (!isfinite(ratio)) return OFW_FLAG_NONFINITE;
+      if (!isfinite(ratio)) return PairOut{0.0, 0.0, 0.0, OFW_FLAG_NONFINITE};
       if (model == OFW_MODEL_RANKINE) {
         double limited = ratio < 1.0 ? ratio : 1.0;
         factor = limited * limited;
@@ -115,11 +122,8 @@ __device__ __noinline__ int pair_exact64(double px, double py, double pz, const 
   }
   double coefficient = (gamma / length) / OFW_FOUR_PI * factor * (na + nb) / denominator;
   if (!(isfinite(denominator) && isfinite(coefficient) && isfinite(product) && isfinite(dot)))
-    return OFW_FLAG_NONFINITE;
-  ux += coefficient * cx;
-  uy += coefficient * cy;
-  uz += coefficient * cz;
-  return OFW_FLAG_OK;
+    return PairOut{0.0, 0.0, 0.0, OFW_FLAG_NONFINITE};
+  return PairOut{coefficient * cx, coefficient * cy, coefficient * cz, OFW_FLAG_OK};
 }
 
 // Reciprocal and square root for normal, positive FP64 operands: the hardware estimates (v_rcp_f64,
@@ -183,8 +187,15 @@ __device__ __forceinline__ int pair_fast64(double px, double py, double pz, cons
   const double nb2 = __builtin_fma(bz, bz, __builtin_fma(by, by, bx * bx));
   const bool exact_seg = (MODEL == OFW_MODEL_RANKINE || MODEL == OFW_MODEL_LAMB || MODEL == OFW_MODEL_VATISTAS)
                              ? r[R_EXACT] != 0.0 : false;
-  if (!(fmax(na2, nb2) <= OFW_FAST_A2_MAX) || !(fmin(na2, nb2) >= OFW_FAST_A2_MIN) || exact_seg)
-    return pair_exact64(px, py, pz, r, MODEL, raise_singular, ux, uy, uz);
+  if (!(fmax(na2, nb2) <= OFW_FAST_A2_MAX) || !(fmin(na2, nb2) >= OFW_FAST_A2_MIN) || exact_seg) {
+    const PairOut e = pair_exact64(px, py, pz, r, MODEL, raise_singular);
+    if (e.flag == OFW_FLAG_OK) {
+      ux += e.x;
+      uy += e.y;
+      uz += e.z;
+    }
+    return e.flag;
+  }
   const double h2 = __builtin_fma(cz, cz, __builtin_fma(cy, cy, cx * cx));
   const double t = __builtin_fma(az, dz, __builtin_fma(ay, dy, ax * dx));
   const double na = ofw_sqrt(na2), nb = ofw_sqrt(nb2);
@@ -238,8 +249,10 @@ __device__ __forceinline__ int pair_fast64(double px, double py, double pz, cons
 }
 
 // FP32 pair for the mixed / fp32 precisions.  rel = p - s rounded from FP64.  Returns true when the pair
-// is near (must be evaluated in FP64): within 4 core radii or 1e-3 (scaled) segment lengths of the
-// segment's line, or outside the FP32 range.  Otherwise accumulates the FP32 velocity.
+// is near (must be evaluated in FP64, see OFW_NEAR_*) or outside the FP32 range; otherwise adds the FP32
+// velocity to u.  The arithmetic runs unconditionally and the result is selected, so that a wavefront
+// evaluating one segment against 64 targets takes no divergent branch except the rare core factor of a
+// target close to the line beyond an end of the segment.
 template <int MODEL>
 __device__ __forceinline__ bool pair_f32(float rx, float ry, float rz, const float* __restrict__ f,
                                          float& ux, float& uy, float& uz) {
@@ -253,20 +266,33 @@ __device__ __forceinline__ bool pair_f32(float rx, float ry, float rz, const flo
   const float h2 = cx * cx + cy * cy + cz * cz;
   const float m2 = fmaxf(1.0f, fmaxf(na2, nb2));
   const float rc2 = f[F_RC2];
-  bool near = !(m2 <= OFW_F32_A2_MAX) || h2 <= OFW_NEAR_AXIS2 * m2 || f[F_EXACT] != 0.0f;
-  if (MODEL != OFW_MODEL_NONE) near = near || h2 < OFW_NEAR_CORES2 * rc2;
-  if (near) return true;
-  const float g4 = f[F_G4];
-  if (g4 == 0.0f) return false;
   const float t = (ax * dx + ay * dy) + az * dz;
+  const bool beside = t >= 0.0f && t <= 1.0f;
+  bool near = !(m2 <= OFW_F32_A2_MAX) || h2 <= OFW_NEAR_AXIS2 * m2 || f[F_EXACT] != 0.0f ||
+              (beside && h2 <= OFW_NEAR_BESIDE2 * m2);
+  if (MODEL != OFW_MODEL_NONE) {
+    // squared distance to the segment itself (in segment lengths)
+    const float dseg2 = beside ? h2 : (t < 0.0f ? na2 : nb2);
+    near = near || dseg2 < OFW_NEAR_CORES2 * rc2;
+  }
+  const float g4 = f[F_G4];
   float sa = __builtin_amdgcn_sqrtf(na2), sb = __builtin_amdgcn_sqrtf(nb2);
   float product = sa * sb;
   float dot = h2 + t * (t - 1.0f);
   float plus = dot < 0.0f ? h2 * __builtin_amdgcn_rcpf(product - dot) : product + dot;
   float den = product * plus;
   float factor = 1.0f;
-  if (MODEL == OFW_MODEL_VATISTAS && rc2 > 0.0f) {
-    // beyond 4 core radii q >= 16: factor = 1/sqrt(1 + q^-2)
+  if (MODEL == OFW_MODEL_RANKINE || MODEL == OFW_MODEL_LAMB) {
+    // core factors of q = (h / rc)^2 (vortex_numba.py:79-101): an FP32 pair is more than 4 core radii from
+    // the segment but may be closer than that to its line, beyond one of its ends
+    const bool core = !near && rc2 > 0.0f && h2 < OFW_NEAR_CORES2 * rc2;
+    if (__builtin_amdgcn_ballot_w64(core) != 0 && core) {
+      const float q = h2 * __builtin_amdgcn_rcpf(rc2);
+      if (MODEL == OFW_MODEL_RANKINE) factor = fminf(q, 1.0f);
+      else if (q <= (float)(10.0 / OFW_LAMB)) factor = -expm1f(-(float)OFW_LAMB * q);
+    }
+  } else if (MODEL == OFW_MODEL_VATISTAS && rc2 > 0.0f) {
+    // q / sqrt(1 + q^2) = 1 / sqrt(1 + q^-2) for every q > 0
     const float iq = rc2 * __builtin_amdgcn_rcpf(h2);
     factor = __builtin_amdgcn_rsqf(1.0f + iq * iq);
   } else if (MODEL == OFW_MODEL_OFFSET) {
@@ -280,12 +306,12 @@ __device__ __forceinline__ bool pair_f32(float rx, float ry, float rz, const flo
     plus = dot < 0.0f ? tr2 * __builtin_amdgcn_rcpf(product - dot) : product + dot;
     den = product * plus;
   }
-  // Rankine and Lamb-Oseen factors are exactly 1 beyond 4 core radii (ratio^2 >= 16 > 10/1.25643).
   const float coef = g4 * factor * (sa + sb) * __builtin_amdgcn_rcpf(den);
-  ux += coef * cx;
-  uy += coef * cy;
-  uz += coef * cz;
-  return false;
+  const bool add = !near && g4 != 0.0f;
+  ux = add ? ux + coef * cx : ux;
+  uy = add ? uy + coef * cy : uy;
+  uz = add ? uz + coef * cz : uz;
+  return near;
 }
 
 // Deterministic first-flag bookkeeping: a lane keeps the first flagged segment in its own order.

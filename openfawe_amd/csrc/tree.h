@@ -23,6 +23,7 @@ enum {
   C_TASKS,          // traversal subtrees (all trees)
   C_WORK,           // traversal work items
   C_DEPTH,          // largest node depth
+  C_CHUNKS,         // near-field work items of the traversal
   C_COUNT = 16
 };
 

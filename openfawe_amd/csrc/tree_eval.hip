@@ -42,6 +42,17 @@ __device__ __forceinline__ bool accept_test(double x0, double x1, double x2, con
   return bound < theta * dist && dist - bound >= a[A_CLEAR];
 }
 
+// The same test on a record given as values (centre, bound, clear).
+__device__ __forceinline__ bool accept_test5(double x0, double x1, double x2, const double* rec, double theta,
+                                             double& rx, double& ry, double& rz, double& dist) {
+#pragma clang fp contract(off)
+  rx = x0 - rec[0];
+  ry = x1 - rec[1];
+  rz = x2 - rec[2];
+  dist = __builtin_sqrt(rx * rx + ry * ry + rz * rz);
+  return rec[3] < theta * dist && dist - rec[3] >= rec[4];
+}
+
 // far_velocity (vortex_tree.py:450-497) in FP64, from the contracted moments of one node record
 __device__ __forceinline__ void far64(double rx, double ry, double rz, double dist, const double* __restrict__ f,
                                       int order, double& ux, double& uy, double& uz) {
@@ -136,15 +147,37 @@ __device__ __forceinline__ bool wave_any(bool v) { return __builtin_amdgcn_ballo
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// value of lane `src` (wave-uniform index) for every lane
+__device__ __forceinline__ float bcast(float v, int src) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+__device__ __forceinline__ double bcast(double v, int src) {
+  const long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffll), src);
+  const int hi = __builtin_amdgcn_readlane((int)(u >> 32), src);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // Per-tree tables of a forest evaluation (K+1 entries each, int32, device).
 struct ForestTables {
   int K;
+  int node_cap;               // entries of the node record arrays
+  int seg_count;              // entries of the segment record arrays
   const int32_t* toff;        // first target of each tree
   const int32_t* task_first;  // first traversal task of each tree
   const int32_t* wave_off;    // first work item (wavefront) of each tree
   const int32_t* part_off;    // first partial-sum slot of each tree
+  const int32_t* list_base;   // first near-list entry of each tree (below FP64)
+  const int32_t* list_cap;    // near-list entries per wavefront of each tree
   const int32_t* counters;
 };
+
+// A near-field work entry: a leaf (its segment range) and the lanes of the wavefront that opened it.
+struct NearEntry {
+  int32_t lo, hi;
+  uint32_t mask_lo, mask_hi;
+};
+static constexpr int CHUNK = 4;  // leaves per near-field work item
 
 // the tree k of a wavefront / target: the last k with table[k] <= v (table[0] = 0 <= v < table[K])
 __device__ __forceinline__ int find_tree(const int32_t* __restrict__ table, int K, int v) {
@@ -157,6 +190,13 @@ __device__ __forceinline__ int find_tree(const int32_t* __restrict__ table, int 
   return lo;
 }
 
+// Node records are read through a per-wavefront window of WIN consecutive pre-order nodes in LDS (the
+// acceptance record and, below FP64, the FP32 far-field record): the walk only moves forward in pre-order,
+// so one batch of coalesced loads serves many visits.  Segments of the near field are staged one per lane
+// in a window of 64 consecutive tree positions and broadcast with readlane.
+static constexpr int WIN = 32;
+static constexpr int EVAL_WAVES = 4;
+
 template <int MODEL, int PREC>
 __global__ __launch_bounds__(256) void k_tree_eval(ForestTables ft, const double* __restrict__ pts,
                                                    const int32_t* __restrict__ ex_ptr,
@@ -167,9 +207,14 @@ __global__ __launch_bounds__(256) void k_tree_eval(ForestTables ft, const double
                                                    const float* __restrict__ rec32, double theta, int order,
                                                    int raise_singular, double* __restrict__ part,
                                                    int8_t* __restrict__ pflag, int32_t* __restrict__ pfar,
-                                                   int32_t* __restrict__ pnear) {
+                                                   int32_t* __restrict__ pnear, int8_t* __restrict__ pnz,
+                                                   NearEntry* __restrict__ list, int32_t* __restrict__ list_count) {
   typedef typename std::conditional<PREC == OFW_FP32, float, double>::type acc_t;
-  const int wave = uniform(xcd_block(blockIdx.x, gridDim.x) * 4 + (int)(threadIdx.x >> 6));
+  constexpr int FARW = PREC == OFW_FP64 ? 1 : WIN * FAR_REC;
+  __shared__ double s_acc[EVAL_WAVES][WIN * ACC_REC];
+  __shared__ float s_far[EVAL_WAVES][FARW];
+  const int wv = threadIdx.x >> 6;
+  const int wave = uniform(xcd_block(blockIdx.x, gridDim.x) * EVAL_WAVES + wv);
   // whole wavefront beyond the work, or a plan that overflowed its capacity (reported by the host)
   if (wave >= ft.counters[C_WORK] || ft.counters[C_OVERFLOW]) return;
   const int lane = threadIdx.x & 63;
@@ -186,6 +231,7 @@ __global__ __launch_bounds__(256) void k_tree_eval(ForestTables ft, const double
   const bool valid = jl < nt;
   const int j = t0 + jl;
   double x0 = 0.0, x1 = 0.0, x2 = 0.0;
+  // exclusion cursor: the lane's next excluded tree position (INT_MAX when none is left)
   int cur = 0, cend = 0;
   if (valid) {
     x0 = pts[3 * (size_t)j];
@@ -196,97 +242,167 @@ __global__ __launch_bounds__(256) void k_tree_eval(ForestTables ft, const double
       cend = ex_ptr[j + 1];
     }
   }
+  // the next four excluded positions are held in registers (nextex, e1, e2, e3), so that passing one costs
+  // no memory access; they are reloaded four at a time when a lane has more
+  int nextex = INT_MAX, e1 = INT_MAX, e2 = INT_MAX, e3 = INT_MAX;
+  auto refill = [&]() {
+    nextex = cur < cend ? ex_pos[cur] : INT_MAX;
+    e1 = cur + 1 < cend ? ex_pos[cur + 1] : INT_MAX;
+    e2 = cur + 2 < cend ? ex_pos[cur + 2] : INT_MAX;
+    e3 = cur + 3 < cend ? ex_pos[cur + 3] : INT_MAX;
+  };
+  refill();
+  auto advance_to = [&](int pos) {
+    while (nextex < pos) {
+      ++cur;
+      nextex = e1;
+      e1 = e2;
+      e2 = e3;
+      e3 = INT_MAX;
+      if (nextex == INT_MAX && cur < cend) refill();
+    }
+  };
   acc_t ux = 0, uy = 0, uz = 0;
   int flag = OFW_FLAG_OK, far = 0, near = 0;
   bool alive = valid;
   const EvalTask& T = tasks[task];
   const int F = uniform(T.node);
   const int nanc = uniform(T.nanc);
-  const int32_t* fi = reinterpret_cast<const int32_t*>(accr + (size_t)F * ACC_REC + A_INTS);
-  const int flo = uniform(fi[0]);
 
-  // ---- ancestors of the frontier node
-  for (int a_i = 0; a_i < nanc; ++a_i) {
-    const int A = uniform(T.anc[a_i]);
-    const double* a = accr + (size_t)A * ACC_REC;
-    const int32_t* ai = reinterpret_cast<const int32_t*>(a + A_INTS);
-    const int alo = ai[0], ahi = ai[1];
-    double rx, ry, rz, dist;
-    bool acc = alive && accept_test(x0, x1, x2, a, theta, rx, ry, rz, dist);
-    if (acc) {
-      while (cur < cend && ex_pos[cur] < alo) ++cur;
-      acc = !(cur < cend && ex_pos[cur] < ahi);
+  auto add_far = [&](double rx, double ry, double rz, double dist, int node, const float* f32) {
+    if (PREC == OFW_FP64) {
+      double vx = 0.0, vy = 0.0, vz = 0.0;
+      far64(rx, ry, rz, dist, fr64 + (size_t)node * FAR_REC, order, vx, vy, vz);
+      ux += vx;
+      uy += vy;
+      uz += vz;
+    } else {
+      float vx, vy, vz;
+      far32((float)rx, (float)ry, (float)rz, f32, order, vx, vy, vz);
+      ux += (acc_t)vx;
+      uy += (acc_t)vy;
+      uz += (acc_t)vz;
     }
-    if (acc) {
-      alive = false;
-      if (alo == flo) {
-        if (PREC == OFW_FP64) {
-          double vx = 0.0, vy = 0.0, vz = 0.0;
-          far64(rx, ry, rz, dist, fr64 + (size_t)A * FAR_REC, order, vx, vy, vz);
-          ux += vx;
-          uy += vy;
-          uz += vz;
-        } else {
-          float vx, vy, vz;
-          far32((float)rx, (float)ry, (float)rz, fr32 + (size_t)A * FAR_REC, order, vx, vy, vz);
-          ux += (acc_t)vx;
-          uy += (acc_t)vy;
-          uz += (acc_t)vz;
+    ++far;
+  };
+
+  // ---- ancestors of the frontier node: their acceptance records loaded together, one per lane
+  {
+    double acx = 0.0, acy = 0.0, acz = 0.0, abound = 0.0, aclear = 0.0;
+    int alo_l = 0, ahi_l = 0;
+    if (lane < nanc) {
+      const double* a = accr + (size_t)T.anc[lane] * ACC_REC;
+      acx = a[A_CX];
+      acy = a[A_CY];
+      acz = a[A_CZ];
+      abound = a[A_BOUND];
+      aclear = a[A_CLEAR];
+      const int32_t* ai = reinterpret_cast<const int32_t*>(a + A_INTS);
+      alo_l = ai[0];
+      ahi_l = ai[1];
+    }
+    const int flo = uniform(reinterpret_cast<const int32_t*>(accr + (size_t)F * ACC_REC + A_INTS)[0]);
+    for (int a_i = 0; a_i < nanc; ++a_i) {
+      const double rec[5] = {bcast(acx, a_i), bcast(acy, a_i), bcast(acz, a_i), bcast(abound, a_i),
+                             bcast(aclear, a_i)};
+      const int alo = __builtin_amdgcn_readlane(alo_l, a_i), ahi = __builtin_amdgcn_readlane(ahi_l, a_i);
+      double rx, ry, rz, dist;
+      bool acc = alive && accept_test5(x0, x1, x2, rec, theta, rx, ry, rz, dist);
+      if (acc) {
+        advance_to(alo);
+        acc = !(nextex < ahi);
+      }
+      if (acc) {
+        alive = false;
+        // the subtree that holds the ancestor's leftmost segment adds it, where the reference's walk does
+        if (alo == flo) {
+          const int A = uniform(T.anc[a_i]);
+          add_far(rx, ry, rz, dist, A, fr32 + (size_t)A * FAR_REC);
         }
-        ++far;
       }
     }
   }
 
-  // ---- stackless walk of the frontier subtree
+  // ---- stackless walk of the frontier subtree through the node window
+  const int end = uniform(reinterpret_cast<const int32_t*>(accr + (size_t)F * ACC_REC + A_INTS)[2]);
+  int wbase = INT_MIN / 2;
+  auto load_window = [&](int base) {
+    wbase = base;
+    // the window is one contiguous block of each record array: each lane copies 16 bytes per chunk,
+    // stored with the element type the window is read with
+    const double* src = accr + (size_t)base * ACC_REC;
+    double v[4];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int e = 2 * lane + 128 * c;  // first double of this lane's chunk
+      const bool in = base + e / ACC_REC < ft.node_cap;
+      const double2 d = in ? *reinterpret_cast<const double2*>(src + e) : make_double2(0.0, 0.0);
+      v[2 * c] = d.x;
+      v[2 * c + 1] = d.y;
+    }
+    if (PREC != OFW_FP64) {
+      const float* fsrc = fr32 + (size_t)base * FAR_REC;
+      float w[24];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        const int e = 4 * lane + 256 * c;  // first float of this lane's chunk
+        const bool in = base + e / FAR_REC < ft.node_cap;
+        const float4 d = in ? *reinterpret_cast<const float4*>(fsrc + e) : make_float4(0.f, 0.f, 0.f, 0.f);
+        w[4 * c] = d.x;
+        w[4 * c + 1] = d.y;
+        w[4 * c + 2] = d.z;
+        w[4 * c + 3] = d.w;
+      }
+#pragma unroll
+      for (int c = 0; c < 6; ++c)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) s_far[wv][4 * lane + 256 * c + e] = w[4 * c + e];
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      s_acc[wv][2 * lane + 128 * c] = v[2 * c];
+      s_acc[wv][2 * lane + 128 * c + 1] = v[2 * c + 1];
+    }
+  };
+  // near-field list of this wavefront (below FP64)
+  const int lcap = PREC == OFW_FP64 ? 0 : uniform(ft.list_cap[k]);
+  const size_t lbase = PREC == OFW_FP64 ? 0 : (size_t)uniform(ft.list_base[k]) + (size_t)local * lcap;
+  int nlist = 0;
   int i = F;
-  const int end = uniform(fi[2]);
   int skip_until = F;
   while (i < end) {
     const bool act = alive && i >= skip_until;
     if (!wave_any(act)) break;
-    const double* a = accr + (size_t)i * ACC_REC;
-    const int32_t* ai = reinterpret_cast<const int32_t*>(a + A_INTS);
-    const int lo = uniform(ai[0]), hi = uniform(ai[1]), skip = uniform(ai[2]);
+    if (i >= wbase + WIN) load_window(i);
+    const double* a = &s_acc[wv][(i - wbase) * ACC_REC];
+    // (lo, hi) and (skip, tree) are int32 pairs in two record slots
+    const long long ints0 = __double_as_longlong(a[A_INTS]), ints1 = __double_as_longlong(a[A_INTS2]);
+    const int lo = uniform((int)(ints0 & 0xffffffffll)), hi = uniform((int)(ints0 >> 32));
+    const int skip = uniform((int)(ints1 & 0xffffffffll));
     double rx = 0.0, ry = 0.0, rz = 0.0, dist = 1.0;
     bool acc = false;
     if (act) {
       acc = accept_test(x0, x1, x2, a, theta, rx, ry, rz, dist);
-      while (cur < cend && ex_pos[cur] < lo) ++cur;
-      if (acc) acc = !(cur < cend && ex_pos[cur] < hi);
+      advance_to(lo);
+      if (acc) acc = !(nextex < hi);
     }
     if (wave_any(acc)) {
-      if (acc) {
-        if (PREC == OFW_FP64) {
-          double vx = 0.0, vy = 0.0, vz = 0.0;
-          far64(rx, ry, rz, dist, fr64 + (size_t)i * FAR_REC, order, vx, vy, vz);
-          ux += vx;
-          uy += vy;
-          uz += vz;
-        } else {
-          float vx, vy, vz;
-          far32((float)rx, (float)ry, (float)rz, fr32 + (size_t)i * FAR_REC, order, vx, vy, vz);
-          ux += (acc_t)vx;
-          uy += (acc_t)vy;
-          uz += (acc_t)vz;
-        }
-        ++far;
-      }
+      if (acc) add_far(rx, ry, rz, dist, i, &s_far[wv][(PREC == OFW_FP64) ? 0 : (i - wbase) * FAR_REC]);
     }
     const bool open = act && !acc;
     if (skip == i + 1) {
       // leaf: the near field of the lanes that did not accept it (vortex_tree.py:538-557)
       if (wave_any(open)) {
-        for (int s = lo; s < hi; ++s) {
-          bool use = open;
-          if (use && cur < cend && ex_pos[cur] == s) {
-            ++cur;
-            use = false;
-          }
-          if (!wave_any(use)) continue;
-          if (use) {
-            ++near;
-            if (PREC == OFW_FP64) {
+        if (PREC == OFW_FP64) {
+          for (int s = lo; s < hi; ++s) {
+            bool use = open;
+            if (use && nextex == s) {
+              advance_to(s + 1);
+              use = false;
+            }
+            if (!wave_any(use)) continue;
+            if (use) {
+              ++near;
               double vx = 0.0, vy = 0.0, vz = 0.0;
               const int f = pair_fast64<MODEL>(x0, x1, x2, rec64 + (size_t)s * REC64, raise_singular, vx, vy, vz);
               if (f != OFW_FLAG_OK) {
@@ -296,26 +412,17 @@ __global__ __launch_bounds__(256) void k_tree_eval(ForestTables ft, const double
                 uy += vy;
                 uz += vz;
               }
-            } else {
-              const double* r = rec64 + (size_t)s * REC64;
-              const float sx = (float)(x0 - r[R_SX]), sy = (float)(x1 - r[R_SY]), sz = (float)(x2 - r[R_SZ]);
-              float vx = 0.0f, vy = 0.0f, vz = 0.0f;
-              if (pair_f32<MODEL>(sx, sy, sz, rec32 + (size_t)s * REC32, vx, vy, vz)) {
-                double wx = 0.0, wy = 0.0, wz = 0.0;
-                const int f = pair_fast64<MODEL>(x0, x1, x2, r, raise_singular, wx, wy, wz);
-                if (f != OFW_FLAG_OK) {
-                  if (flag == OFW_FLAG_OK) flag = f;
-                } else {
-                  ux += (acc_t)wx;
-                  uy += (acc_t)wy;
-                  uz += (acc_t)wz;
-                }
-              } else {
-                ux += (acc_t)vx;
-                uy += (acc_t)vy;
-                uz += (acc_t)vz;
-              }
             }
+          }
+        } else {
+          // the leaf goes to the wavefront's near-field list with the lanes that opened it; the near-field
+          // kernel evaluates the lists in short chunks spread over the whole GPU
+          const unsigned long long mask = __builtin_amdgcn_ballot_w64(open);
+          if (nlist < lcap) {
+            if (lane == 0) list[lbase + nlist] = NearEntry{lo, hi, (uint32_t)mask, (uint32_t)(mask >> 32)};
+            ++nlist;
+          } else if (lane == 0) {
+            atomicOr(const_cast<int32_t*>(ft.counters) + C_OVERFLOW, 1);
           }
         }
       }
@@ -325,7 +432,13 @@ __global__ __launch_bounds__(256) void k_tree_eval(ForestTables ft, const double
       i = wave_any(open) ? i + 1 : skip;
     }
   }
-  if (valid) {
+  // partial sums only from wavefronts that added anything (the reduction reads the flag)
+  const bool any = wave_any(far > 0 || near > 0);
+  if (lane == 0) {
+    pnz[wave] = any ? 1 : 0;
+    if (PREC != OFW_FP64) list_count[wave] = nlist;
+  }
+  if (any && valid) {
     const size_t o = (size_t)uniform(ft.part_off[k]) + (size_t)q * nt + jl;
     part[3 * o] = (double)ux;
     part[3 * o + 1] = (double)uy;
@@ -338,12 +451,167 @@ __global__ __launch_bounds__(256) void k_tree_eval(ForestTables ft, const double
   }
 }
 
-// Per target: its tree's subtree partials added in pre-order of the subtrees.
+// Near field of the traversal (below FP64): one wavefront per work item, CHUNK consecutive entries of one
+// walk wavefront's list, for the same 64 targets.  The leaf's segments are staged one per lane in a window
+// of 64 consecutive tree positions and broadcast with readlane; every lane evaluates the FP32 pair and the
+// lanes that opened the leaf select the result (vortex_tree.py:538-557 for each of them).  A grid-stride
+// loop covers the work items, whose count is known on the device only.
+template <int MODEL, int PREC>
+__global__ __launch_bounds__(256) void k_tree_near(ForestTables ft, const double* __restrict__ pts,
+                                                   const int32_t* __restrict__ ex_ptr,
+                                                   const int32_t* __restrict__ ex_pos,
+                                                   const double* __restrict__ rec64, const float* __restrict__ rec32,
+                                                   int raise_singular, const NearEntry* __restrict__ list,
+                                                   const int32_t* __restrict__ list_count,
+                                                   const int32_t* __restrict__ chunk_off, int chunk_slots,
+                                                   double* __restrict__ cpart, int8_t* __restrict__ cflag,
+                                                   int32_t* __restrict__ cnear) {
+  typedef typename std::conditional<PREC == OFW_FP32, float, double>::type acc_t;
+  const int lane = threadIdx.x & 63;
+  const int works = min(live_count(ft.counters, C_CHUNKS), chunk_slots);
+  const int nwaves = live_count(ft.counters, C_WORK);
+  const int stride = gridDim.x * EVAL_WAVES;
+  for (int item = uniform(blockIdx.x * EVAL_WAVES + (int)(threadIdx.x >> 6)); item < works; item += stride) {
+    // the walk wavefront of this item: the last w with chunk_off[w] <= item
+    const int w = uniform(find_tree(chunk_off, nwaves, item));
+    const int c = item - uniform(chunk_off[w]);
+    const int k = uniform(find_tree(ft.wave_off, ft.K, w));
+    const int t0 = uniform(ft.toff[k]);
+    const int nt = uniform(ft.toff[k + 1]) - t0;
+    const int groups = (nt + 63) >> 6;
+    const int local = w - uniform(ft.wave_off[k]);
+    const int g = local % groups;
+    const int jl = g * 64 + lane;
+    const bool valid = jl < nt;
+    const int j = t0 + jl;
+    double x0 = 0.0, x1 = 0.0, x2 = 0.0;
+    int cur = 0, cend = 0;
+    if (valid) {
+      x0 = pts[3 * (size_t)j];
+      x1 = pts[3 * (size_t)j + 1];
+      x2 = pts[3 * (size_t)j + 2];
+      if (ex_ptr) {
+        cur = ex_ptr[j];
+        cend = ex_ptr[j + 1];
+      }
+    }
+    // the next four excluded positions in registers (see k_tree_eval)
+    int nextex = INT_MAX, e1 = INT_MAX, e2 = INT_MAX, e3 = INT_MAX;
+    auto refill = [&]() {
+      nextex = cur < cend ? ex_pos[cur] : INT_MAX;
+      e1 = cur + 1 < cend ? ex_pos[cur + 1] : INT_MAX;
+      e2 = cur + 2 < cend ? ex_pos[cur + 2] : INT_MAX;
+      e3 = cur + 3 < cend ? ex_pos[cur + 3] : INT_MAX;
+    };
+    refill();
+    auto advance_to = [&](int pos) {
+      while (nextex < pos) {
+        ++cur;
+        nextex = e1;
+        e1 = e2;
+        e2 = e3;
+        e3 = INT_MAX;
+        if (nextex == INT_MAX && cur < cend) refill();
+      }
+    };
+    acc_t ux = 0, uy = 0, uz = 0;
+    int flag = OFW_FLAG_OK, near = 0;
+    int sbase = INT_MIN / 2;
+    float4 fa = make_float4(0.f, 0.f, 0.f, 0.f), fb = make_float4(0.f, 0.f, 0.f, 0.f);
+    double qx = 0.0, qy = 0.0, qz = 0.0;
+    const size_t lbase = (size_t)uniform(ft.list_base[k]) + (size_t)local * uniform(ft.list_cap[k]);
+    const int e_end = min(uniform(list_count[w]), (c + 1) * CHUNK);
+    for (int e = c * CHUNK; e < e_end; ++e) {
+      const NearEntry en = list[lbase + e];
+      const int lo = uniform(en.lo), hi = uniform(en.hi);
+      const unsigned long long mask =
+          ((unsigned long long)(uint32_t)uniform((int)en.mask_hi) << 32) | (uint32_t)uniform((int)en.mask_lo);
+      const bool open = valid && ((mask >> lane) & 1ull);
+      if (open) advance_to(lo);
+      for (int base = lo; base < hi;) {
+        if (base < sbase || base >= sbase + 64) {
+          sbase = base;
+          const int s = base + lane;
+          if (s < ft.seg_count) {
+            const float4* f4 = reinterpret_cast<const float4*>(rec32 + (size_t)s * REC32);
+            fa = f4[0];
+            fb = f4[1];
+            const double* r = rec64 + (size_t)s * REC64;
+            qx = r[R_SX];
+            qy = r[R_SY];
+            qz = r[R_SZ];
+          }
+        }
+        const int stop = min(hi, sbase + 64);
+        for (int s = base; s < stop; ++s) {
+          const int tl = s - sbase;
+          // every lane evaluates the FP32 pair; the lanes that use it select the result, and only the rare
+          // excluded segment and the FP64 pairs branch (on the wavefront's ballot)
+          const bool hit = open && nextex == s;
+          if (wave_any(hit)) {
+            if (hit) advance_to(s + 1);
+          }
+          const bool use = open && !hit;
+          const float f[REC32] = {bcast(fa.x, tl), bcast(fa.y, tl), bcast(fa.z, tl), bcast(fa.w, tl),
+                                  bcast(fb.x, tl), bcast(fb.y, tl), bcast(fb.z, tl), 0.0f};
+          const double bx = bcast(qx, tl), by = bcast(qy, tl), bz = bcast(qz, tl);
+          const float sx = (float)(x0 - bx), sy = (float)(x1 - by), sz = (float)(x2 - bz);
+          float vx = 0.0f, vy = 0.0f, vz = 0.0f;
+          const bool need64 = pair_f32<MODEL>(sx, sy, sz, f, vx, vy, vz);
+          const bool add = use && !need64;
+          ux = add ? ux + (acc_t)vx : ux;
+          uy = add ? uy + (acc_t)vy : uy;
+          uz = add ? uz + (acc_t)vz : uz;
+          near += use ? 1 : 0;
+          const bool slow = use && need64;
+          if (wave_any(slow)) {
+            if (slow) {
+              double wx = 0.0, wy = 0.0, wz = 0.0;
+              const int fl = pair_fast64<MODEL>(x0, x1, x2, rec64 + (size_t)s * REC64, raise_singular, wx, wy, wz);
+              if (fl != OFW_FLAG_OK) {
+                if (flag == OFW_FLAG_OK) flag = fl;
+              } else {
+                ux += (acc_t)wx;
+                uy += (acc_t)wy;
+                uz += (acc_t)wz;
+              }
+            }
+          }
+        }
+        base = stop;
+      }
+    }
+    if (valid) {
+      const size_t o = (size_t)item * 64 + lane;
+      cpart[3 * o] = (double)ux;
+      cpart[3 * o + 1] = (double)uy;
+      cpart[3 * o + 2] = (double)uz;
+      cflag[o] = (int8_t)flag;
+      if (cnear) cnear[o] = near;
+    }
+  }
+}
+
+// near-field work items per walk wavefront: ceil(list entries / CHUNK), zero past the planned waves
+__global__ void k_chunk_count(ForestTables ft, int wave_cap, const int32_t* __restrict__ list_count,
+                              int32_t* __restrict__ nch) {
+  const int nwaves = live_count(ft.counters, C_WORK);
+  for (int w = blockIdx.x * blockDim.x + threadIdx.x; w <= wave_cap; w += gridDim.x * blockDim.x)
+    nch[w] = w < nwaves ? (list_count[w] + CHUNK - 1) / CHUNK : 0;
+}
+
+// Per target: its tree's subtree partials added in pre-order of the subtrees (those of wavefronts that
+// added nothing are skipped: they are zero).
 template <bool WIDE>
 __global__ __launch_bounds__(256) void k_tree_reduce(ForestTables ft, int n, const double* __restrict__ part,
                                                      const int8_t* __restrict__ pflag,
                                                      const int32_t* __restrict__ pfar,
-                                                     const int32_t* __restrict__ pnear, double* __restrict__ out,
+                                                     const int32_t* __restrict__ pnear,
+                                                     const int8_t* __restrict__ pnz,
+                                                     const int32_t* __restrict__ chunk_off,
+                                                     const double* __restrict__ cpart,
+                                                     const int8_t* __restrict__ cflag,
+                                                     const int32_t* __restrict__ cnear, double* __restrict__ out,
                                                      int8_t* __restrict__ flags, int64_t* __restrict__ far_counts,
                                                      int64_t* __restrict__ near_counts) {
   typedef typename std::conditional<WIDE, double, float>::type acc_t;
@@ -352,19 +620,35 @@ __global__ __launch_bounds__(256) void k_tree_reduce(ForestTables ft, int n, con
   const int k = find_tree(ft.toff, ft.K, j);
   const int t0 = ft.toff[k], nt = ft.toff[k + 1] - t0;
   const int ntasks = ft.counters[C_OVERFLOW] ? 0 : ft.task_first[k + 1] - ft.task_first[k];
+  const int groups = (nt + 63) >> 6;
+  const int w0 = ft.wave_off[k] + ((j - t0) >> 6);
   const size_t base = (size_t)ft.part_off[k] + (j - t0);
   acc_t ux = 0, uy = 0, uz = 0;
   int flag = 0;
   int64_t far = 0, near = 0;
   for (int q = 0; q < ntasks; ++q) {
-    const size_t o = base + (size_t)q * nt;
-    ux += (acc_t)part[3 * o];
-    uy += (acc_t)part[3 * o + 1];
-    uz += (acc_t)part[3 * o + 2];
-    if (flag == 0) flag = pflag[o];
-    if (pfar) {
-      far += pfar[o];
-      near += pnear[o];
+    const int w = w0 + q * groups;
+    if (pnz[w]) {
+      const size_t o = base + (size_t)q * nt;
+      ux += (acc_t)part[3 * o];
+      uy += (acc_t)part[3 * o + 1];
+      uz += (acc_t)part[3 * o + 2];
+      if (flag == 0) flag = pflag[o];
+      if (pfar) {
+        far += pfar[o];
+        near += pnear[o];
+      }
+    }
+    if (chunk_off) {
+      // the subtree's near field, in the order of its walk
+      for (int ch = chunk_off[w]; ch < chunk_off[w + 1]; ++ch) {
+        const size_t o = (size_t)ch * 64 + ((j - t0) & 63);
+        ux += (acc_t)cpart[3 * o];
+        uy += (acc_t)cpart[3 * o + 1];
+        uz += (acc_t)cpart[3 * o + 2];
+        if (flag == 0) flag = cflag[o];
+        if (cnear) near += cnear[o];
+      }
     }
   }
   if (!(isfinite((double)ux) && isfinite((double)uy) && isfinite((double)uz)) && flag == 0) flag = OFW_FLAG_NONFINITE;
@@ -485,16 +769,32 @@ template <int MODEL>
 static void launch_eval(int precision, dim3 grid, hipStream_t st, const ForestTables& ft, const double* pts,
                         const int32_t* ex_ptr, const int32_t* ex_pos, const EvalTask* tasks, const TreeDevice& td,
                         double theta, int order, int raise, double* part, int8_t* pflag, int32_t* pfar,
-                        int32_t* pnear) {
+                        int32_t* pnear, int8_t* pnz, NearEntry* list, int32_t* list_count) {
   if (precision == OFW_FP64)
     hipLaunchKernelGGL((k_tree_eval<MODEL, OFW_FP64>), grid, dim3(256), 0, st, ft, pts, ex_ptr, ex_pos, tasks, td.acc,
-                       td.far64, td.far32, td.rec64, td.rec32, theta, order, raise, part, pflag, pfar, pnear);
+                       td.far64, td.far32, td.rec64, td.rec32, theta, order, raise, part, pflag, pfar, pnear, pnz,
+                       list, list_count);
   else if (precision == OFW_MIXED)
     hipLaunchKernelGGL((k_tree_eval<MODEL, OFW_MIXED>), grid, dim3(256), 0, st, ft, pts, ex_ptr, ex_pos, tasks,
-                       td.acc, td.far64, td.far32, td.rec64, td.rec32, theta, order, raise, part, pflag, pfar, pnear);
+                       td.acc, td.far64, td.far32, td.rec64, td.rec32, theta, order, raise, part, pflag, pfar, pnear,
+                       pnz, list, list_count);
   else
     hipLaunchKernelGGL((k_tree_eval<MODEL, OFW_FP32>), grid, dim3(256), 0, st, ft, pts, ex_ptr, ex_pos, tasks, td.acc,
-                       td.far64, td.far32, td.rec64, td.rec32, theta, order, raise, part, pflag, pfar, pnear);
+                       td.far64, td.far32, td.rec64, td.rec32, theta, order, raise, part, pflag, pfar, pnear, pnz,
+                       list, list_count);
+}
+
+template <int MODEL>
+static void launch_near(int precision, dim3 grid, hipStream_t st, const ForestTables& ft, const double* pts,
+                        const int32_t* ex_ptr, const int32_t* ex_pos, const TreeDevice& td, int raise,
+                        const NearEntry* list, const int32_t* list_count, const int32_t* chunk_off, int slots,
+                        double* cpart, int8_t* cflag, int32_t* cnear) {
+  if (precision == OFW_MIXED)
+    hipLaunchKernelGGL((k_tree_near<MODEL, OFW_MIXED>), grid, dim3(256), 0, st, ft, pts, ex_ptr, ex_pos, td.rec64,
+                       td.rec32, raise, list, list_count, chunk_off, slots, cpart, cflag, cnear);
+  else
+    hipLaunchKernelGGL((k_tree_near<MODEL, OFW_FP32>), grid, dim3(256), 0, st, ft, pts, ex_ptr, ex_pos, td.rec64,
+                       td.rec32, raise, list, list_count, chunk_off, slots, cpart, cflag, cnear);
 }
 
 int choose_split_depth(const Ctx& c, int64_t groups, int K) {
@@ -512,26 +812,46 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
   const int K = shape.K;
   const int64_t n = shape.n, m = t.m;
   const int D = std::max(0, std::min(shape.split_depth, MAX_ANC));
-  // host bounds: a tree has at most min(2^D, m_k) frontier subtrees
-  int64_t task_cap = 0, wave_cap = 0, part_cap = 0;
+  const bool lists = precision != OFW_FP64;
+  // host bounds: a tree has at most min(2^D, m_k) frontier subtrees of at most ceil(m_k / 2^D) segments; a
+  // subtree of S segments has at most S / floor((leaf + 1) / 2) + 1 leaves
+  const int64_t min_leaf = std::max(1, (t.leaf + 1) / 2);
+  int64_t task_cap = 0, wave_cap = 0, part_cap = 0, entry_cap = 0, chunk_cap = 0;
+  std::vector<int64_t> list_base(K + 1, 0), list_cap(K + 1, 0);
   for (int k = 0; k < K; ++k) {
     const int64_t mk = shape.seg_off[k + 1] - shape.seg_off[k];
     const int64_t nk = shape.target_off[k + 1] - shape.target_off[k];
     const int64_t tasks_k = mk > 0 ? std::min<int64_t>((int64_t)1 << D, mk) : 0;
+    const int64_t waves_k = tasks_k * ((nk + 63) / 64);
     task_cap += tasks_k;
-    wave_cap += tasks_k * ((nk + 63) / 64);
+    wave_cap += waves_k;
     part_cap += tasks_k * nk;
+    const int64_t sub = (mk + ((int64_t)1 << D) - 1) >> D;
+    list_cap[k] = lists && mk > 0 ? sub / min_leaf + 2 : 0;
+    list_base[k] = entry_cap;
+    entry_cap += waves_k * list_cap[k];
+    chunk_cap += waves_k * ((list_cap[k] + CHUNK - 1) / CHUNK);
   }
-  if (part_cap > INT_MAX || wave_cap > INT_MAX / 4) {
+  list_base[K] = entry_cap;
+  if (part_cap > INT_MAX || wave_cap > INT_MAX / 4 || entry_cap > INT_MAX || chunk_cap > INT_MAX / 64) {
     set_error("tree traversal: split depth too large for the target count");
     return OFW_ERR_ARG;
   }
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const size_t P = (size_t)std::max<int64_t>(part_cap, 1);
   const bool counts = far_counts || near_counts;
+  const size_t W1 = (size_t)wave_cap + 1;
+  const size_t E = (size_t)std::max<int64_t>(entry_cap, 1);
+  // near-field partials: one slot per work item that can exist, bounded by the slots the entries can fill
+  const size_t CH = (size_t)std::max<int64_t>(std::min<int64_t>(chunk_cap, entry_cap / CHUNK + wave_cap), 1);
   size_t need = al((size_t)std::max<int64_t>(m, 1) * 4) + al((size_t)std::max<int64_t>(n_ex, 1) * 4);
   need += 2 * al(((size_t)t.cap + 1) * 4) + al((size_t)std::max<int64_t>(task_cap, 1) * sizeof(EvalTask));
-  need += 4 * al((size_t)(K + 1) * 4) + al(P * 24) + al(P) + (counts ? 2 * al(P * 4) : 0);
+  need += 6 * al((size_t)(K + 1) * 4) + al(P * 24) + al(P) + (counts ? 2 * al(P * 4) : 0);
+  need += al(W1);
+  if (lists) {
+    need += al(E * sizeof(NearEntry)) + 3 * al(W1 * 4) + al(((W1 + 4095) / 4096 + 8) * 4);
+    need += al(CH * 64 * 24) + al(CH * 64) + (counts ? al(CH * 64 * 4) : 0);
+  }
   OFW_HIP(c.tree[buf].reserve(need));
   char* base = (char*)c.tree[buf].ptr;
   auto carve = [&](size_t bytes) {
@@ -548,13 +868,37 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
   int32_t* task_first = (int32_t*)carve((size_t)(K + 1) * 4);
   int32_t* wave_off = (int32_t*)carve((size_t)(K + 1) * 4);
   int32_t* part_off = (int32_t*)carve((size_t)(K + 1) * 4);
+  int32_t* d_list_base = (int32_t*)carve((size_t)(K + 1) * 4);
+  int32_t* d_list_cap = (int32_t*)carve((size_t)(K + 1) * 4);
   double* part = (double*)carve(P * 24);
   int8_t* pflag = (int8_t*)carve(P);
   int32_t* pfar = counts ? (int32_t*)carve(P * 4) : nullptr;
   int32_t* pnear = counts ? (int32_t*)carve(P * 4) : nullptr;
+  int8_t* pnz = (int8_t*)carve(W1);
+  NearEntry* list = nullptr;
+  int32_t *list_count = nullptr, *nch = nullptr, *chunk_off = nullptr, *chunk_part = nullptr;
+  double* cpart = nullptr;
+  int8_t* cflag = nullptr;
+  int32_t* cnear = nullptr;
+  if (lists) {
+    list = (NearEntry*)carve(E * sizeof(NearEntry));
+    list_count = (int32_t*)carve(W1 * 4);
+    nch = (int32_t*)carve(W1 * 4);
+    chunk_off = (int32_t*)carve(W1 * 4);
+    chunk_part = (int32_t*)carve(((W1 + 4095) / 4096 + 8) * 4);
+    cpart = (double*)carve(CH * 64 * 24);
+    cflag = (int8_t*)carve(CH * 64);
+    cnear = counts ? (int32_t*)carve(CH * 64 * 4) : nullptr;
+  }
 
   int st = upload_i32(c, shape.target_off, toff);
   if (st) return st;
+  if (lists) {
+    st = upload_i32(c, list_base, d_list_base);
+    if (st) return st;
+    st = upload_i32(c, list_cap, d_list_cap);
+    if (st) return st;
+  }
   const int grid = c.cu_count * 2;
   if (n_ex > 0 && ex_ptr) {
     hipLaunchKernelGGL(k_inverse, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, c.stream, (int)m, t.order,
@@ -570,24 +914,39 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
                      (int)wave_cap, (int)part_cap);
   OFW_HIP(hipGetLastError());
   OFW_HIP(c.mark());
-  ForestTables ft{K, toff, task_first, wave_off, part_off, t.counters};
+  ForestTables ft{K, td.count, (int)m, toff, task_first, wave_off, part_off, d_list_base, d_list_cap, t.counters};
   if (wave_cap > 0) {
     dim3 egrid((unsigned)((wave_cap + 3) / 4));
     switch (model) {
-      case 0: launch_eval<0>(precision, egrid, c.stream, ft, points, ex_ptr_used, ex_pos, tasks, td, theta, expansion_order, raise_singular, part, pflag, pfar, pnear); break;
-      case 1: launch_eval<1>(precision, egrid, c.stream, ft, points, ex_ptr_used, ex_pos, tasks, td, theta, expansion_order, raise_singular, part, pflag, pfar, pnear); break;
-      default: launch_eval<2>(precision, egrid, c.stream, ft, points, ex_ptr_used, ex_pos, tasks, td, theta, expansion_order, raise_singular, part, pflag, pfar, pnear); break;
+      case 0: launch_eval<0>(precision, egrid, c.stream, ft, points, ex_ptr_used, ex_pos, tasks, td, theta, expansion_order, raise_singular, part, pflag, pfar, pnear, pnz, list, list_count); break;
+      case 1: launch_eval<1>(precision, egrid, c.stream, ft, points, ex_ptr_used, ex_pos, tasks, td, theta, expansion_order, raise_singular, part, pflag, pfar, pnear, pnz, list, list_count); break;
+      default: launch_eval<2>(precision, egrid, c.stream, ft, points, ex_ptr_used, ex_pos, tasks, td, theta, expansion_order, raise_singular, part, pflag, pfar, pnear, pnz, list, list_count); break;
     }
   }
   OFW_HIP(hipGetLastError());
+  if (lists && wave_cap > 0) {
+    // near-field work items: CHUNK list entries each, numbered in walk-wavefront order
+    hipLaunchKernelGGL(k_chunk_count, dim3((unsigned)((W1 + 255) / 256)), dim3(256), 0, c.stream, ft, (int)wave_cap,
+                       list_count, nch);
+    scan_exclusive(c, (int)W1, nch, chunk_off, chunk_part, t.counters + C_CHUNKS);
+    const dim3 ngrid((unsigned)std::max<int64_t>(1, std::min<int64_t>((int64_t)c.cu_count * 16, (CH + 3) / 4)));
+    switch (model) {
+      case 0: launch_near<0>(precision, ngrid, c.stream, ft, points, ex_ptr_used, ex_pos, td, raise_singular, list, list_count, chunk_off, (int)CH, cpart, cflag, cnear); break;
+      case 1: launch_near<1>(precision, ngrid, c.stream, ft, points, ex_ptr_used, ex_pos, td, raise_singular, list, list_count, chunk_off, (int)CH, cpart, cflag, cnear); break;
+      default: launch_near<2>(precision, ngrid, c.stream, ft, points, ex_ptr_used, ex_pos, td, raise_singular, list, list_count, chunk_off, (int)CH, cpart, cflag, cnear); break;
+    }
+    OFW_HIP(hipGetLastError());
+  }
   OFW_HIP(c.mark());
   if (n > 0) {
     if (precision == OFW_FP32)
       hipLaunchKernelGGL(k_tree_reduce<false>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, ft, (int)n,
-                         part, pflag, pfar, pnear, out, flags, far_counts, near_counts);
+                         part, pflag, pfar, pnear, pnz, (const int32_t*)chunk_off, cpart, cflag, cnear, out, flags,
+                         far_counts, near_counts);
     else
       hipLaunchKernelGGL(k_tree_reduce<true>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, ft, (int)n,
-                         part, pflag, pfar, pnear, out, flags, far_counts, near_counts);
+                         part, pflag, pfar, pnear, pnz, (const int32_t*)chunk_off, cpart, cflag, cnear, out, flags,
+                         far_counts, near_counts);
   }
   OFW_HIP(hipGetLastError());
   c.stats_i[3] = D;
