@@ -163,14 +163,17 @@ __device__ __forceinline__ double ofw_rsqrt(double x) {
 
 // Fast FP64 pair: the reference formula (vortex_numba.py:46-109) reorganised for the VALU: a per-segment
 // reciprocal length and circulation factor, squared norms instead of hypot, the axis test and the core
-// ratios on squares, one reciprocal per pair.  Same branches and flags.  a = (p - s)/L and the cross product
+// ratios on squares, one reciprocal per pair.  The same decisions and flags, taken with selects so that a
+// wavefront evaluating one segment for 64 targets does not diverge; only the rare paths (the Lamb-Oseen core
+// inside its cutoff, extreme ranges) branch, on the wavefront's ballot.  A lane with use == false evaluates
+// the pair but adds nothing (an excluded pair) and reports no flag.  a = (p - s)/L and the cross product
 // d x a are rounded exactly as the reference rounds them (they decide h near a segment's line, where the
 // cross product cancels); the other quantities agree with it to an ulp or two.  The arithmetic is written
 // out operation by operation (no contraction), so every kernel that inlines it (direct sum, tree near field,
 // pair matrix) gives the same bits for the same pair.  Pairs outside the fast range take pair_exact64.
 template <int MODEL>
 __device__ __forceinline__ int pair_fast64(double px, double py, double pz, const double* __restrict__ r,
-                                           int raise_singular, double& ux, double& uy, double& uz) {
+                                           int raise_singular, bool use, double& ux, double& uy, double& uz) {
 #pragma clang fp contract(off)
   const double L = r[R_L], invL = r[R_INVL];
   const double dx = r[R_DX], dy = r[R_DY], dz = r[R_DZ];
@@ -187,65 +190,86 @@ __device__ __forceinline__ int pair_fast64(double px, double py, double pz, cons
   const double nb2 = __builtin_fma(bz, bz, __builtin_fma(by, by, bx * bx));
   const bool exact_seg = (MODEL == OFW_MODEL_RANKINE || MODEL == OFW_MODEL_LAMB || MODEL == OFW_MODEL_VATISTAS)
                              ? r[R_EXACT] != 0.0 : false;
-  if (!(fmax(na2, nb2) <= OFW_FAST_A2_MAX) || !(fmin(na2, nb2) >= OFW_FAST_A2_MIN) || exact_seg) {
+  const bool slow = use && (!(fmax(na2, nb2) <= OFW_FAST_A2_MAX) || !(fmin(na2, nb2) >= OFW_FAST_A2_MIN) || exact_seg);
+  const double h2 = __builtin_fma(cz, cz, __builtin_fma(cy, cy, cx * cx));
+  const double t = __builtin_fma(az, dz, __builtin_fma(ay, dy, ax * dx));
+  const double na = ofw_sqrt(na2), nb = ofw_sqrt(nb2);
+  const double mx = fmax(1.0, fmax(na, nb));
+  const bool axis = h2 <= (OFW_AXIS_EPS * OFW_AXIS_EPS) * (mx * mx);
+  const double g4 = r[R_G4];
+  double sa = na, sb = nb;
+  double product = na * nb;
+  double dot = __builtin_fma(t, t - 1.0, h2);
+  double factor = 1.0;
+  double coef;
+  if (MODEL == OFW_MODEL_OFFSET || MODEL == OFW_MODEL_RM) {
+    double plus = dot < 0.0 ? h2 * ofw_rcp(product - dot) : product + dot;
+    double den = product * plus;
+    const double rc2 = r[R_RC2];
+    if (r[R_RADIUS] > 0.0) {
+      if (MODEL == OFW_MODEL_OFFSET) {
+        den = den + rc2;
+      } else {
+        sa = ofw_sqrt(na2 + rc2);
+        sb = ofw_sqrt(nb2 + rc2);
+        product = sa * sb;
+        const double tr2 = h2 + rc2;
+        dot = __builtin_fma(t, t - 1.0, tr2);
+        plus = dot < 0.0 ? tr2 * ofw_rcp(product - dot) : product + dot;
+        den = product * plus;
+      }
+    }
+    coef = ((g4 * factor) * (sa + sb)) * ofw_rcp(den);
+  } else {
+    if (MODEL != OFW_MODEL_NONE) {
+      const bool cored = r[R_RADIUS] > 0.0;
+      const double q = h2 * r[R_IRC2];  // (h / rc)^2
+      if (MODEL == OFW_MODEL_RANKINE) {
+        factor = cored ? fmin(q, 1.0) : 1.0;
+      } else if (MODEL == OFW_MODEL_LAMB) {
+        // inside the cutoff only (a few pairs per target at most): a branch on the wavefront's ballot
+        const bool inside = use && cored && q <= 10.0 / OFW_LAMB;
+        if (__builtin_amdgcn_ballot_w64(inside) != 0 && inside) factor = -expm1(-OFW_LAMB * q);
+      } else {  // Vatistas: q / sqrt(1 + q^2) below one core radius, 1 / sqrt(1 + q^-2) beyond
+        const double iq = q < 1.0 ? q : ofw_rcp(q);
+        const double rs = ofw_rsqrt(__builtin_fma(iq, iq, 1.0));
+        factor = cored ? (q < 1.0 ? q * rs : rs) : 1.0;
+      }
+    }
+    // den = product * plus with plus = h^2 / (product - dot) when dot < 0 (the reference's rationalised form):
+    // one reciprocal of product * h^2 there, of product * (product + dot) elsewhere
+    const bool rat = dot < 0.0;
+    const double dd = rat ? product * h2 : product * (product + dot);
+    const double xm = rat ? product - dot : 1.0;
+    coef = (((g4 * factor) * (sa + sb)) * xm) * ofw_rcp(dd);
+  }
+  bool add = use && !slow && !axis && g4 != 0.0;
+  int flag = OFW_FLAG_OK;
+  if (use && !slow && axis && raise_singular && t >= 0.0 && t <= 1.0) flag = OFW_FLAG_SINGULAR;
+  if (add && !isfinite(coef)) {
+    add = false;
+    flag = OFW_FLAG_NONFINITE;
+  }
+  // the pair velocity is rounded before it is added, as in the reference's ordered sum
+  ux = add ? ux + coef * cx : ux;
+  uy = add ? uy + coef * cy : uy;
+  uz = add ? uz + coef * cz : uz;
+  if (__builtin_amdgcn_ballot_w64(slow) != 0 && slow) {
     const PairOut e = pair_exact64(px, py, pz, r, MODEL, raise_singular);
     if (e.flag == OFW_FLAG_OK) {
       ux += e.x;
       uy += e.y;
       uz += e.z;
     }
-    return e.flag;
+    flag = e.flag;
   }
-  const double h2 = __builtin_fma(cz, cz, __builtin_fma(cy, cy, cx * cx));
-  const double t = __builtin_fma(az, dz, __builtin_fma(ay, dy, ax * dx));
-  const double na = ofw_sqrt(na2), nb = ofw_sqrt(nb2);
-  const double mx = fmax(1.0, fmax(na, nb));
-  if (h2 <= (OFW_AXIS_EPS * OFW_AXIS_EPS) * (mx * mx)) {
-    if (raise_singular && t >= 0.0 && t <= 1.0) return OFW_FLAG_SINGULAR;
-    return OFW_FLAG_OK;
-  }
-  const double g4 = r[R_G4];
-  if (g4 == 0.0) return OFW_FLAG_OK;
-  double sa = na, sb = nb;
-  double product = na * nb;
-  double dot = __builtin_fma(t, t - 1.0, h2);
-  double plus = dot < 0.0 ? h2 * ofw_rcp(product - dot) : product + dot;
-  double den = product * plus;
-  double factor = 1.0;
-  const double radius = r[R_RADIUS];
-  if (MODEL != OFW_MODEL_NONE && radius > 0.0) {
-    const double rc2 = r[R_RC2];
-    const double q = h2 * r[R_IRC2];  // (h / rc)^2
-    if (MODEL == OFW_MODEL_RANKINE) {
-      factor = fmin(q, 1.0);
-    } else if (MODEL == OFW_MODEL_LAMB) {
-      if (q <= 10.0 / OFW_LAMB) factor = -expm1(-OFW_LAMB * q);
-    } else if (MODEL == OFW_MODEL_VATISTAS) {
-      if (q < 1.0) {
-        factor = q * ofw_rsqrt(__builtin_fma(q, q, 1.0));
-      } else {
-        const double iq = ofw_rcp(q);
-        factor = ofw_rsqrt(__builtin_fma(iq, iq, 1.0));
-      }
-    } else if (MODEL == OFW_MODEL_OFFSET) {
-      den = den + rc2;
-    } else if (MODEL == OFW_MODEL_RM) {
-      sa = ofw_sqrt(na2 + rc2);
-      sb = ofw_sqrt(nb2 + rc2);
-      product = sa * sb;
-      const double tr2 = h2 + rc2;
-      dot = __builtin_fma(t, t - 1.0, tr2);
-      plus = dot < 0.0 ? tr2 * ofw_rcp(product - dot) : product + dot;
-      den = product * plus;
-    }
-  }
-  const double coef = ((g4 * factor) * (sa + sb)) * ofw_rcp(den);
-  if (!isfinite(coef)) return OFW_FLAG_NONFINITE;
-  // the pair velocity is rounded before it is added, as in the reference's ordered sum
-  ux += coef * cx;
-  uy += coef * cy;
-  uz += coef * cz;
-  return OFW_FLAG_OK;
+  return flag;
+}
+
+template <int MODEL>
+__device__ __forceinline__ int pair_fast64(double px, double py, double pz, const double* __restrict__ r,
+                                           int raise_singular, double& ux, double& uy, double& uz) {
+  return pair_fast64<MODEL>(px, py, pz, r, raise_singular, true, ux, uy, uz);
 }
 
 // FP32 pair for the mixed / fp32 precisions.  rel = p - s rounded from FP64.  Returns true when the pair

@@ -80,46 +80,75 @@ int csr_to_i32(Ctx& c, int64_t n, const int64_t* ex_ptr, const int64_t* ex_seg, 
 // code of a chunk's first flag: -1 when none, else seg * 4 + flag
 __device__ __forceinline__ int flag_code(const FlagState& st) { return st.flag ? st.seg * 4 + st.flag : -1; }
 
+// FP64 direct sum: each lane owns two targets (blockDim apart, so loads stay coalesced) and walks the chunk's
+// segments in order; the segment record is wave-uniform (scalar loads) and shared by the 128 targets of the
+// wavefront, and the two independent pair evaluations of a lane interleave.  An excluded pair is evaluated
+// and not added; the lane's next excluded segment advances on the wavefront's ballot.
 template <int MODEL>
 __global__ __launch_bounds__(256) void k_direct64(int n, const double* __restrict__ pts, int m,
                                                   const double* __restrict__ rec, int chunk,
                                                   const int32_t* __restrict__ ex_ptr,
                                                   const int32_t* __restrict__ ex_seg, int raise_singular,
                                                   double* __restrict__ part, int32_t* __restrict__ pflag) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int pa = blockIdx.x * (2 * blockDim.x) + threadIdx.x;
+  const int pb = pa + blockDim.x;
   const int c = blockIdx.y;
   const int s0 = c * chunk;
   const int s1 = min(m, s0 + chunk);
-  const bool active = p < n;
-  double px = 0.0, py = 0.0, pz = 0.0;
-  int k = 0, stop = 0, next = INT_MAX;
-  if (active) {
-    px = pts[3 * (size_t)p];
-    py = pts[3 * (size_t)p + 1];
-    pz = pts[3 * (size_t)p + 2];
-    k = ex_ptr[p];
-    stop = ex_ptr[p + 1];
-    while (k < stop && ex_seg[k] < s0) k++;
-    next = k < stop ? ex_seg[k] : INT_MAX;
+  const bool act_a = pa < n, act_b = pb < n;
+  double ax = 0.0, ay = 0.0, az = 0.0, bx = 0.0, by = 0.0, bz = 0.0;
+  int ka = 0, stop_a = 0, next_a = INT_MAX, kb = 0, stop_b = 0, next_b = INT_MAX;
+  if (act_a) {
+    ax = pts[3 * (size_t)pa];
+    ay = pts[3 * (size_t)pa + 1];
+    az = pts[3 * (size_t)pa + 2];
+    ka = ex_ptr[pa];
+    stop_a = ex_ptr[pa + 1];
+    while (ka < stop_a && ex_seg[ka] < s0) ka++;
+    next_a = ka < stop_a ? ex_seg[ka] : INT_MAX;
   }
-  double ux = 0.0, uy = 0.0, uz = 0.0;
-  FlagState st{OFW_FLAG_OK, INT_MAX};
+  if (act_b) {
+    bx = pts[3 * (size_t)pb];
+    by = pts[3 * (size_t)pb + 1];
+    bz = pts[3 * (size_t)pb + 2];
+    kb = ex_ptr[pb];
+    stop_b = ex_ptr[pb + 1];
+    while (kb < stop_b && ex_seg[kb] < s0) kb++;
+    next_b = kb < stop_b ? ex_seg[kb] : INT_MAX;
+  }
+  double uax = 0.0, uay = 0.0, uaz = 0.0, ubx = 0.0, uby = 0.0, ubz = 0.0;
+  FlagState sta{OFW_FLAG_OK, INT_MAX}, stb{OFW_FLAG_OK, INT_MAX};
   for (int s = s0; s < s1; ++s) {
     const double* r = rec + (size_t)s * REC64;
-    if (s == next) {
-      ++k;
-      next = k < stop ? ex_seg[k] : INT_MAX;
-      continue;
+    const bool hit_a = s == next_a, hit_b = s == next_b;
+    if (__builtin_amdgcn_ballot_w64(hit_a || hit_b) != 0) {
+      if (hit_a) {
+        ++ka;
+        next_a = ka < stop_a ? ex_seg[ka] : INT_MAX;
+      }
+      if (hit_b) {
+        ++kb;
+        next_b = kb < stop_b ? ex_seg[kb] : INT_MAX;
+      }
     }
-    const int f = pair_fast64<MODEL>(px, py, pz, r, raise_singular, ux, uy, uz);
-    note_flag(st, f, s);
+    const int fa = pair_fast64<MODEL>(ax, ay, az, r, raise_singular, act_a && !hit_a, uax, uay, uaz);
+    const int fb = pair_fast64<MODEL>(bx, by, bz, r, raise_singular, act_b && !hit_b, ubx, uby, ubz);
+    note_flag(sta, fa, s);
+    note_flag(stb, fb, s);
   }
-  if (active) {
-    const size_t o = (size_t)c * n + p;
-    part[3 * o] = ux;
-    part[3 * o + 1] = uy;
-    part[3 * o + 2] = uz;
-    pflag[o] = flag_code(st);
+  if (act_a) {
+    const size_t o = (size_t)c * n + pa;
+    part[3 * o] = uax;
+    part[3 * o + 1] = uay;
+    part[3 * o + 2] = uaz;
+    pflag[o] = flag_code(sta);
+  }
+  if (act_b) {
+    const size_t o = (size_t)c * n + pb;
+    part[3 * o] = ubx;
+    part[3 * o + 1] = uby;
+    part[3 * o + 2] = ubz;
+    pflag[o] = flag_code(stb);
   }
 }
 
@@ -230,7 +259,9 @@ int direct_sum(Ctx& c, int precision, int block, int64_t n, const double* points
                const float* rec32, int model, int raise_singular, const int32_t* ex_ptr, const int32_t* ex_seg,
                double* out, int8_t* flags, int32_t* flag_seg) {
   if (n == 0) return OFW_OK;
-  const int64_t bx = (n + block - 1) / block;
+  // the FP64 kernel gives each lane two targets
+  const int per_block = precision == OFW_FP64 ? 2 * block : block;
+  const int64_t bx = (n + per_block - 1) / per_block;
   const int64_t waves_per_block = (block + 63) / 64;
   // enough wavefronts for every SIMD of every CU to hold several (MI355X: 256 CUs x 4 SIMDs)
   const int64_t want_waves = (int64_t)c.cu_count * 4 * 8;

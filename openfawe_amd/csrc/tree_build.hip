@@ -288,54 +288,81 @@ __global__ __launch_bounds__(TILE_THREADS) void k_sort_tiles(TreeArrays t, const
 }
 
 // One merge pass: within each big split node, runs of `width` elements (from the node start) are merged in
-// pairs; a workgroup produces one output tile, each thread PER_THREAD outputs from its own merge-path start.
+// pairs.  A workgroup produces one output tile: two merge-path searches in global memory give the tile's input
+// ranges of both runs, which are staged in LDS; each thread then finds its own start in LDS and merges
+// PER_THREAD outputs.
 static constexpr int MERGE_THREADS = 512;
 static constexpr int PER_THREAD = TILE / MERGE_THREADS;
+
+// number of elements taken from run A among the first `diag` outputs of merging A (la) and B (lb)
+__device__ __forceinline__ int merge_split(const uint64_t* __restrict__ ka, const uint32_t* __restrict__ pa, int la,
+                                           const uint64_t* __restrict__ kb, const uint32_t* __restrict__ pb, int lb,
+                                           int diag) {
+  int lo = max(0, diag - lb), hi = min(diag, la);
+  while (lo < hi) {
+    const int i = (lo + hi) >> 1;
+    const int j = diag - i - 1;
+    if (pair_less(kb[j], pb[j], ka[i], pa[i])) hi = i;
+    else lo = i + 1;
+  }
+  return lo;
+}
 
 __global__ __launch_bounds__(MERGE_THREADS) void k_merge_pass(TreeArrays t, int width,
                                                               const uint64_t* __restrict__ in_key,
                                                               const uint32_t* __restrict__ in_pos,
                                                               uint64_t* __restrict__ out_key,
                                                               uint32_t* __restrict__ out_pos) {
+  __shared__ uint64_t s_key[TILE];
+  __shared__ uint32_t s_pos[TILE];
+  __shared__ int s_split[2];
   const int ntiles = live_count(t.counters, C_TILES);
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int node = t.tile_node[tile];
-    if (!t.split[node]) continue;
+    if (!t.split[node]) continue;  // uniform over the workgroup
     const int a = t.lo[node], b = t.hi[node];
     const int p0 = t.tile_p0[tile], p1 = t.tile_p1[tile];
     const int pair_start = a + ((p0 - a) / (2 * width)) * (2 * width);
     const int a0 = pair_start, a1 = min(b, pair_start + width);
     const int b0 = a1, b1 = min(b, a1 + width);
     const int la = a1 - a0, lb = b1 - b0;
-    const int o_begin = p0 + threadIdx.x * PER_THREAD;
-    if (o_begin >= p1) continue;
-    const int o_end = min(p1, o_begin + PER_THREAD);
-    const int diag = o_begin - pair_start;
-    int lo = max(0, diag - lb), hi = min(diag, la);
-    while (lo < hi) {
-      const int i = (lo + hi) >> 1;
-      const int j = diag - i - 1;
-      if (pair_less(in_key[b0 + j], in_pos[b0 + j], in_key[a0 + i], in_pos[a0 + i]))
-        hi = i;
-      else
-        lo = i + 1;
+    const int d0 = p0 - pair_start, d1 = p1 - pair_start;
+    if (threadIdx.x < 2) {
+      const int d = threadIdx.x == 0 ? d0 : d1;
+      s_split[threadIdx.x] = merge_split(in_key + a0, in_pos + a0, la, in_key + b0, in_pos + b0, lb, d);
     }
-    int i = lo, j = diag - lo;
-    for (int o = o_begin; o < o_end; ++o) {
-      bool take_a;
-      if (i >= la) take_a = false;
-      else if (j >= lb) take_a = true;
-      else take_a = !pair_less(in_key[b0 + j], in_pos[b0 + j], in_key[a0 + i], in_pos[a0 + i]);
-      if (take_a) {
-        out_key[o] = in_key[a0 + i];
-        out_pos[o] = in_pos[a0 + i];
-        ++i;
-      } else {
-        out_key[o] = in_key[b0 + j];
-        out_pos[o] = in_pos[b0 + j];
-        ++j;
+    __syncthreads();
+    const int i0 = s_split[0], i1 = s_split[1];
+    const int j0 = d0 - i0, j1 = d1 - i1;
+    const int na = i1 - i0, nb = j1 - j0;  // na + nb = p1 - p0 <= TILE
+    for (int e = threadIdx.x; e < na + nb; e += blockDim.x) {
+      const int src = e < na ? a0 + i0 + e : b0 + j0 + (e - na);
+      s_key[e] = in_key[src];
+      s_pos[e] = in_pos[src];
+    }
+    __syncthreads();
+    const int o_begin = threadIdx.x * PER_THREAD;
+    if (o_begin < na + nb) {
+      const int o_end = min(na + nb, o_begin + PER_THREAD);
+      int i = merge_split(s_key, s_pos, na, s_key + na, s_pos + na, nb, o_begin);
+      int j = o_begin - i;
+      for (int o = o_begin; o < o_end; ++o) {
+        bool take_a;
+        if (i >= na) take_a = false;
+        else if (j >= nb) take_a = true;
+        else take_a = !pair_less(s_key[na + j], s_pos[na + j], s_key[i], s_pos[i]);
+        if (take_a) {
+          out_key[p0 + o] = s_key[i];
+          out_pos[p0 + o] = s_pos[i];
+          ++i;
+        } else {
+          out_key[p0 + o] = s_key[na + j];
+          out_pos[p0 + o] = s_pos[na + j];
+          ++j;
+        }
       }
     }
+    __syncthreads();  // the next tile reuses the LDS
   }
 }
 

@@ -156,13 +156,16 @@ def test_vsm_closure_jacobian():
     cp = np.array([[0.3, j + 0.5, 0.05 * np.sin(j + 0.5)] for j in range(n)])   # 3/4-chord points
     pairs, flags = AD.pair_matrix(cp, s, d, L, radius, 2)
     assert not flags.any()
-    matrix = AD.assemble_columns(pairs, ptr, seg, sign).cpu().numpy()
-    chord = np.full(n, 0.4)
-    tangent = np.tile([1.0, 0.0, 0.0], (n, 1))
-    normal = np.tile([0.0, 0.0, 1.0], (n, 1))
-    span = np.tile([0.0, 1.0, 0.0], (n, 1))
-    ambient = np.tile([20.0, 0.0, 1.5], (n, 1))
-    known = np.zeros((n, 3))
+    # the closure and its Jacobian are evaluated in extended precision from the FP64 operator, so that the
+    # comparison measures the Jacobian's formula and not the rounding noise of the differences (|closure| / h)
+    ld = np.longdouble
+    matrix = AD.assemble_columns(pairs, ptr, seg, sign).cpu().numpy().astype(ld)
+    chord = np.full(n, 0.4, dtype=ld)
+    tangent = np.tile(np.array([1.0, 0.0, 0.0], dtype=ld), (n, 1))
+    normal = np.tile(np.array([0.0, 0.0, 1.0], dtype=ld), (n, 1))
+    span = np.tile(np.array([0.0, 1.0, 0.0], dtype=ld), (n, 1))
+    ambient = np.tile(np.array([20.0, 0.0, 1.5], dtype=ld), (n, 1))
+    known = np.zeros((n, 3), dtype=ld)
     cross = np.linalg.norm(np.cross(ambient, span), axis=1)
     slope = 2 * np.pi
 
@@ -185,11 +188,11 @@ def test_vsm_closure_jacobian():
         dh = 2 * np.einsum("ik,ijk->ij", np.cross(v, span), np.cross(matrix, span[:, None, :]))
         return (.5 * chord / cross)[:, None] * (slope * speed2[:, None] * da + cl[:, None] * dh) - np.eye(n)
 
-    gamma = np.linspace(1.0, 3.0, n)
+    gamma = np.linspace(1.0, 3.0, n).astype(ld)
     J = jacobian(gamma)
-    h = 1e-6
-    fd = np.stack([(closure(gamma + h * e) - closure(gamma - h * e)) / (2 * h) for e in np.eye(n)], axis=1)
-    np.testing.assert_allclose(J, fd, rtol=2e-7, atol=2e-9)
+    h = ld(1e-6)
+    fd = np.stack([(closure(gamma + h * e) - closure(gamma - h * e)) / (2 * h) for e in np.eye(n, dtype=ld)], axis=1)
+    np.testing.assert_allclose(J.astype(np.float64), fd.astype(np.float64), rtol=2e-7, atol=2e-9)
 
 
 def test_vsm_wing_in_wake_influence_and_adjoint():
