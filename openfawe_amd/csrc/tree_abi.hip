@@ -55,9 +55,10 @@ static int midpoints(Ctx& c, int64_t m, const double* starts, const double* dire
   return tree_prep(c, m, starts, directions, lengths, *mx, *my, *mz);
 }
 
-// One tree sum over K independent snapshots (K = 1 for ofw_tree_sum).  Enqueue only: no host
-// synchronisation, no memset, no allocation once the context's buffers have the sizes of an earlier
-// identical call, so the whole sequence can be captured in a graph.
+// One tree sum over K independent snapshots (K = 1 for ofw_tree_sum).  With a fixed near-field capacity
+// (chunk_slots >= 0) it only enqueues: no host synchronisation, no memset, and no allocation once the
+// context's buffers have the sizes of an earlier identical call, so the whole sequence can be captured in a
+// graph.  With chunk_slots < 0 the capacity is read from the device once, after the walk.
 struct TreeCall {
   int precision = OFW_MIXED;
   int K = 1;
@@ -73,6 +74,7 @@ struct TreeCall {
   int leaf = 32;
   double clearance = 0.0;
   int split_depth = 0;
+  int64_t chunk_slots = -1;  // near-field work-item slots (< 0: sized from this call's count)
   double* out = nullptr;
   int8_t* flags = nullptr;
   int64_t *far_counts = nullptr, *near_counts = nullptr;
@@ -106,6 +108,7 @@ static int tree_pipeline(Ctx& c, const TreeCall& tc, TreeArrays& t) {
   shape.target_off = tc.toff;
   shape.seg_off = tc.soff;
   shape.split_depth = tc.split_depth;
+  shape.chunk_slots = tc.chunk_slots;
   st = tree_evaluate(c, 2, t, td, shape, tc.precision, tc.points, tc.model, tc.raise_singular, p32, s32,
                      p32 ? tc.n_ex : 0, tc.theta, tc.order, tc.out, tc.flags, tc.far_counts, tc.near_counts);
   if (st) return st;
@@ -420,11 +423,19 @@ int ofw_tree_batch_create(ofw_ctx* x, ofw_tree_batch** plan, int precision, int6
   tc.flags = flags;
   tc.far_counts = far_counts;
   tc.near_counts = near_counts;
-  // one eager call sizes every buffer and checks the capacities
+  // one eager call measures the near-field work, a second one sizes every buffer for the fixed capacity
+  // the recorded call keeps (half as much again as measured: the work varies with the contents of the inputs)
   c.nev = 0;
   st = ofw::tree_pipeline(c, tc, b->t);
   if (st) return fail(st);
   int32_t counters[ofw::C_COUNT];
+  st = ofw::read_counters(c, b->t, counters);
+  if (st) return fail(st);
+  const int64_t measured = c.stats_i[6];
+  tc.chunk_slots = measured + measured / 2 + 1024;
+  c.nev = 0;
+  st = ofw::tree_pipeline(c, tc, b->t);
+  if (st) return fail(st);
   st = ofw::read_counters(c, b->t, counters);
   if (st) return fail(st);
   if (use_graph) {

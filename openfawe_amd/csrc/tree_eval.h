@@ -27,6 +27,9 @@ struct EvalShape {
   std::vector<int64_t> target_off;  // K+1
   std::vector<int64_t> seg_off;     // K+1
   int split_depth = 0;
+  // near-field work-item slots: < 0 = sized from the device count of this call (one synchronisation, the
+  // count is left in the context's stats[6]); >= 0 = fixed (no synchronisation; more work items overflow)
+  int64_t chunk_slots = -1;
 };
 
 int tree_prep(Ctx& c, int64_t m, const double* starts, const double* directions, const double* lengths, double* mx,

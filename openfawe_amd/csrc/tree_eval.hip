@@ -592,6 +592,11 @@ __global__ __launch_bounds__(256) void k_tree_near(ForestTables ft, const double
   }
 }
 
+// a fixed number of near-field partial slots that the work items exceed is reported as an overflow
+__global__ void k_check_slots(int32_t* __restrict__ counters, int slots) {
+  if (threadIdx.x == 0 && counters[C_CHUNKS] > slots) counters[C_OVERFLOW] = 1;
+}
+
 // near-field work items per walk wavefront: ceil(list entries / CHUNK), zero past the planned waves
 __global__ void k_chunk_count(ForestTables ft, int wave_cap, const int32_t* __restrict__ list_count,
                               int32_t* __restrict__ nch) {
@@ -833,8 +838,12 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
     chunk_cap += waves_k * ((list_cap[k] + CHUNK - 1) / CHUNK);
   }
   list_base[K] = entry_cap;
-  if (part_cap > INT_MAX || wave_cap > INT_MAX / 4 || entry_cap > INT_MAX || chunk_cap > INT_MAX / 64) {
+  if (part_cap > INT_MAX || wave_cap > INT_MAX / 4) {
     set_error("tree traversal: split depth too large for the target count");
+    return OFW_ERR_ARG;
+  }
+  if (entry_cap > INT_MAX) {
+    set_error("tree traversal: too many targets and segments for one call (near-field list bound)");
     return OFW_ERR_ARG;
   }
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
@@ -842,16 +851,12 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
   const bool counts = far_counts || near_counts;
   const size_t W1 = (size_t)wave_cap + 1;
   const size_t E = (size_t)std::max<int64_t>(entry_cap, 1);
-  // near-field partials: one slot per work item that can exist, bounded by the slots the entries can fill
-  const size_t CH = (size_t)std::max<int64_t>(std::min<int64_t>(chunk_cap, entry_cap / CHUNK + wave_cap), 1);
+  (void)chunk_cap;
   size_t need = al((size_t)std::max<int64_t>(m, 1) * 4) + al((size_t)std::max<int64_t>(n_ex, 1) * 4);
   need += 2 * al(((size_t)t.cap + 1) * 4) + al((size_t)std::max<int64_t>(task_cap, 1) * sizeof(EvalTask));
   need += 6 * al((size_t)(K + 1) * 4) + al(P * 24) + al(P) + (counts ? 2 * al(P * 4) : 0);
   need += al(W1);
-  if (lists) {
-    need += al(E * sizeof(NearEntry)) + 3 * al(W1 * 4) + al(((W1 + 4095) / 4096 + 8) * 4);
-    need += al(CH * 64 * 24) + al(CH * 64) + (counts ? al(CH * 64 * 4) : 0);
-  }
+  if (lists) need += al(E * sizeof(NearEntry)) + 3 * al(W1 * 4) + al(((W1 + 4095) / 4096 + 8) * 4);
   OFW_HIP(c.tree[buf].reserve(need));
   char* base = (char*)c.tree[buf].ptr;
   auto carve = [&](size_t bytes) {
@@ -886,9 +891,6 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
     nch = (int32_t*)carve(W1 * 4);
     chunk_off = (int32_t*)carve(W1 * 4);
     chunk_part = (int32_t*)carve(((W1 + 4095) / 4096 + 8) * 4);
-    cpart = (double*)carve(CH * 64 * 24);
-    cflag = (int8_t*)carve(CH * 64);
-    cnear = counts ? (int32_t*)carve(CH * 64 * 4) : nullptr;
   }
 
   int st = upload_i32(c, shape.target_off, toff);
@@ -929,6 +931,26 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
     hipLaunchKernelGGL(k_chunk_count, dim3((unsigned)((W1 + 255) / 256)), dim3(256), 0, c.stream, ft, (int)wave_cap,
                        list_count, nch);
     scan_exclusive(c, (int)W1, nch, chunk_off, chunk_part, t.counters + C_CHUNKS);
+    // partial sums of the near-field work items (scratch buffer 5 of the context)
+    int64_t slots = shape.chunk_slots;
+    if (slots < 0) {
+      int32_t count = 0;
+      OFW_HIP(hipMemcpyAsync(&count, t.counters + C_CHUNKS, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+      OFW_HIP(hipStreamSynchronize(c.stream));
+      slots = count;
+      c.stats_i[6] = count;
+    }
+    const size_t CH = (size_t)std::max<int64_t>(slots, 1);
+    OFW_HIP(c.tree[5].reserve(al(CH * 64 * 24) + al(CH * 64) + (counts ? al(CH * 64 * 4) : 0)));
+    {
+      char* cb = (char*)c.tree[5].ptr;
+      cpart = (double*)cb;
+      cb += al(CH * 64 * 24);
+      cflag = (int8_t*)cb;
+      cb += al(CH * 64);
+      cnear = counts ? (int32_t*)cb : nullptr;
+    }
+    hipLaunchKernelGGL(k_check_slots, dim3(1), dim3(64), 0, c.stream, t.counters, (int)std::min<int64_t>(slots, INT_MAX));
     const dim3 ngrid((unsigned)std::max<int64_t>(1, std::min<int64_t>((int64_t)c.cu_count * 16, (CH + 3) / 4)));
     switch (model) {
       case 0: launch_near<0>(precision, ngrid, c.stream, ft, points, ex_ptr_used, ex_pos, td, raise_singular, list, list_count, chunk_off, (int)CH, cpart, cflag, cnear); break;
