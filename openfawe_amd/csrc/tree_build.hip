@@ -30,8 +30,10 @@ namespace ofw {
 
 static constexpr int TILE = 8192;           // elements per LDS tile of the top phase
 static constexpr int TILE_THREADS = 1024;
+static constexpr int TILE_E = TILE / TILE_THREADS;  // elements per thread of the tile sort
 static constexpr int S_BOTTOM = 2048;       // largest subtree finished in LDS
 static constexpr int BOTTOM_THREADS = 512;
+static constexpr int BOTTOM_E = S_BOTTOM / BOTTOM_THREADS;  // elements per thread of the subtree sort
 static constexpr int HCAP = 2 * S_BOTTOM - 1;
 static constexpr int PLAN_THREADS = 1024;
 
@@ -236,6 +238,99 @@ __device__ __forceinline__ bool pair_less(uint64_t ka, uint32_t pa, uint64_t kb,
   return ka < kb || (ka == kb && pa < pb);
 }
 
+// Workgroup sort of `width` (key, tag) elements in LDS (width a power of two, E <= width <= blockDim * E):
+// each thread sorts E consecutive elements in registers (bitonic network), then runs of E, 2E, ... are
+// merged in place: every thread finds its first output with a merge-path search in LDS, merges its E outputs
+// into registers, and writes them back after a barrier.  Every thread of the workgroup must call it.
+template <int E, typename Less>
+__device__ void block_merge_sort(uint64_t* s_key, uint32_t* s_tag, int width, Less less) {
+  const int tid = threadIdx.x;
+  const bool active = tid * E < width;
+  uint64_t k[E];
+  uint32_t p[E];
+  if (active) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      k[e] = s_key[tid * E + e];
+      p[e] = s_tag[tid * E + e];
+    }
+#pragma unroll
+    for (int size = 2; size <= E; size <<= 1)
+#pragma unroll
+      for (int stride = size >> 1; stride > 0; stride >>= 1)
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+          const int j = i ^ stride;
+          if (j > i) {
+            const bool up = (i & size) == 0;
+            const bool swap = up ? less(k[j], p[j], k[i], p[i]) : less(k[i], p[i], k[j], p[j]);
+            if (swap) {
+              const uint64_t tk = k[i];
+              k[i] = k[j];
+              k[j] = tk;
+              const uint32_t tp = p[i];
+              p[i] = p[j];
+              p[j] = tp;
+            }
+          }
+        }
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      s_key[tid * E + e] = k[e];
+      s_tag[tid * E + e] = p[e];
+    }
+  }
+  __syncthreads();
+  for (int run = E; run < width; run <<= 1) {
+    if (active) {
+      const int out0 = tid * E;
+      const int pair0 = out0 & ~(2 * run - 1);
+      const int diag = out0 - pair0;
+      const uint64_t* ka = s_key + pair0;
+      const uint32_t* pa = s_tag + pair0;
+      const uint64_t* kb = ka + run;
+      const uint32_t* pb = pa + run;
+      int lo = max(0, diag - run), hi = min(diag, run);
+      while (lo < hi) {
+        const int i = (lo + hi) >> 1;
+        const int j = diag - i - 1;
+        if (less(kb[j], pb[j], ka[i], pa[i])) hi = i;
+        else lo = i + 1;
+      }
+      int i = lo, j = diag - lo;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        // outputs of one thread never pass the end of both runs: i + j < 2 * run
+        const bool take_a = j >= run || (i < run && !less(kb[j], pb[j], ka[i], pa[i]));
+        if (take_a) {
+          k[e] = ka[i];
+          p[e] = pa[i];
+          ++i;
+        } else {
+          k[e] = kb[j];
+          p[e] = pb[j];
+          ++j;
+        }
+      }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        s_key[tid * E + e] = k[e];
+        s_tag[tid * E + e] = p[e];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+struct PairLess {
+  __device__ bool operator()(uint64_t ka, uint32_t pa, uint64_t kb, uint32_t pb) const {
+    return pair_less(ka, pa, kb, pb);
+  }
+};
+
 __global__ __launch_bounds__(TILE_THREADS) void k_sort_tiles(TreeArrays t, const double* __restrict__ mx,
                                                              const double* __restrict__ my,
                                                              const double* __restrict__ mz) {
@@ -247,7 +342,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_sort_tiles(TreeArrays t, const
     if (!t.split[node]) continue;  // uniform over the workgroup
     const int p0 = t.tile_p0[tile], p1 = t.tile_p1[tile];
     const int len = p1 - p0;
-    int width = 1;
+    int width = TILE_E;
     while (width < len) width <<= 1;
     const int axis = t.axis[node];
     const double* coord = axis == 0 ? mx : (axis == 1 ? my : mz);
@@ -261,24 +356,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_sort_tiles(TreeArrays t, const
       }
     }
     __syncthreads();
-    for (int size = 2; size <= width; size <<= 1) {
-      for (int stride = size >> 1; stride > 0; stride >>= 1) {
-        for (int i = threadIdx.x; i < (width >> 1); i += blockDim.x) {
-          const int lo = 2 * i - (i & (stride - 1));
-          const int hi = lo + stride;
-          const bool up = (lo & size) == 0;
-          const uint64_t ka = key[lo], kb = key[hi];
-          const uint32_t pa = pos[lo], pb = pos[hi];
-          if (up ? pair_less(kb, pb, ka, pa) : pair_less(ka, pa, kb, pb)) {
-            key[lo] = kb;
-            key[hi] = ka;
-            pos[lo] = pb;
-            pos[hi] = pa;
-          }
-        }
-        __syncthreads();
-      }
-    }
+    block_merge_sort<TILE_E>(key, pos, width, PairLess());
     for (int i = threadIdx.x; i < len; i += blockDim.x) {
       t.keyA[p0 + i] = key[i];
       t.posA[p0 + i] = pos[i];
@@ -461,6 +539,12 @@ __device__ __forceinline__ bool key3_less(uint32_t ta, uint64_t ka, uint32_t tb,
   return (ta & 2047u) < (tb & 2047u);
 }
 
+struct Key3Less {
+  __device__ bool operator()(uint64_t ka, uint32_t ta, uint64_t kb, uint32_t tb) const {
+    return key3_less(ta, ka, tb, kb);
+  }
+};
+
 // One workgroup per subtree of at most S_BOTTOM segments: every remaining level in LDS.
 __global__ __launch_bounds__(BOTTOM_THREADS) void k_bottom(TreeArrays t, const double* __restrict__ mx,
                                                            const double* __restrict__ my,
@@ -529,7 +613,7 @@ __global__ __launch_bounds__(BOTTOM_THREADS) void k_bottom(TreeArrays t, const d
       __syncthreads();
       if (!any_split) break;
       // sort every splitting node's range by (node, key, position); other nodes keep their order
-      int width = 1;
+      int width = BOTTOM_E;
       while (width < s) width <<= 1;
       for (int i = threadIdx.x; i < width; i += blockDim.x) {
         if (i < s) {
@@ -547,24 +631,7 @@ __global__ __launch_bounds__(BOTTOM_THREADS) void k_bottom(TreeArrays t, const d
         }
       }
       __syncthreads();
-      for (int size = 2; size <= width; size <<= 1) {
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-          for (int i = threadIdx.x; i < (width >> 1); i += blockDim.x) {
-            const int lo = 2 * i - (i & (stride - 1));
-            const int hi = lo + stride;
-            const bool up = (lo & size) == 0;
-            const uint64_t ka = key[lo], kb = key[hi];
-            const uint32_t ta = tag[lo], tb = tag[hi];
-            if (up ? key3_less(tb, kb, ta, ka) : key3_less(ta, ka, tb, kb)) {
-              key[lo] = kb;
-              key[hi] = ka;
-              tag[lo] = tb;
-              tag[hi] = ta;
-            }
-          }
-          __syncthreads();
-        }
-      }
+      block_merge_sort<BOTTOM_E>(key, tag, width, Key3Less());
       // permute segments and midpoints by the sorted positions (one array at a time through a scratch)
       double* dtmp = reinterpret_cast<double*>(key);
       for (int i = threadIdx.x; i < s; i += blockDim.x) itmp[i] = seg[tag[i] & 2047u];
