@@ -366,11 +366,14 @@ __global__ __launch_bounds__(TILE_THREADS) void k_sort_tiles(TreeArrays t, const
 }
 
 // One merge pass: within each big split node, runs of `width` elements (from the node start) are merged in
-// pairs.  A workgroup produces one output tile: two merge-path searches in global memory give the tile's input
-// ranges of both runs, which are staged in LDS; each thread then finds its own start in LDS and merges
-// PER_THREAD outputs.
-static constexpr int MERGE_THREADS = 512;
-static constexpr int PER_THREAD = TILE / MERGE_THREADS;
+// pairs.  A workgroup produces one slice of MERGE_OUT outputs: two merge-path searches in global memory (one
+// wavefront each) give the slice's input ranges of both runs, which are staged in LDS; each thread finds its
+// own start in LDS, merges MERGE_E outputs into registers, and the slice leaves through LDS with coalesced
+// stores.
+static constexpr int MERGE_THREADS = 256;
+static constexpr int MERGE_E = 8;                                // outputs per thread
+static constexpr int MERGE_OUT = MERGE_THREADS * MERGE_E;        // outputs per work item
+static constexpr int MERGE_SLICES = TILE / MERGE_OUT;            // work items per sort tile
 
 // number of elements taken from run A among the first `diag` outputs of merging A (la) and B (lb)
 __device__ __forceinline__ int merge_split(const uint64_t* __restrict__ ka, const uint32_t* __restrict__ pa, int la,
@@ -386,61 +389,118 @@ __device__ __forceinline__ int merge_split(const uint64_t* __restrict__ ka, cons
   return lo;
 }
 
+// merge_split by one wavefront: 64 probes per round trip narrow the range 65-fold (the predicate
+// "B[diag - i - 1] < A[i]" is monotone in i), instead of one dependent load pair per halving
+__device__ int wave_merge_split(const uint64_t* __restrict__ ka, const uint32_t* __restrict__ pa, int la,
+                                const uint64_t* __restrict__ kb, const uint32_t* __restrict__ pb, int lb, int diag) {
+  const int lane = threadIdx.x & 63;
+  int lo = max(0, diag - lb), hi = min(diag, la);  // the answer (first i with the predicate, else hi) is in [lo, hi]
+  while (true) {
+    const int span = hi - lo;
+    if (span <= 64) {
+      int best = hi;
+      const int q = lo + lane;
+      if (q < hi) {
+        const int j = diag - q - 1;
+        if (pair_less(kb[j], pb[j], ka[q], pa[q])) best = q;
+      }
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) best = min(best, __shfl_xor(best, off, 64));
+      return best;
+    }
+    // 64 distinct probes i_k = lo + floor((k + 1) * span / 65) < hi
+    const int i = lo + (int)(((int64_t)(lane + 1) * span) / 65);
+    const int j = diag - i - 1;
+    const bool pred = pair_less(kb[j], pb[j], ka[i], pa[i]);
+    const unsigned long long ballot = __builtin_amdgcn_ballot_w64(pred);
+    if (ballot == 0) {
+      lo = lo + (int)(((int64_t)64 * span) / 65) + 1;  // above the last probe
+    } else {
+      const int first = __builtin_ctzll(ballot);
+      const int i_first = lo + (int)(((int64_t)(first + 1) * span) / 65);
+      lo = first == 0 ? lo : lo + (int)(((int64_t)first * span) / 65) + 1;  // above the last false probe
+      hi = i_first;
+    }
+  }
+}
+
 __global__ __launch_bounds__(MERGE_THREADS) void k_merge_pass(TreeArrays t, int width,
                                                               const uint64_t* __restrict__ in_key,
                                                               const uint32_t* __restrict__ in_pos,
                                                               uint64_t* __restrict__ out_key,
                                                               uint32_t* __restrict__ out_pos) {
-  __shared__ uint64_t s_key[TILE];
-  __shared__ uint32_t s_pos[TILE];
+  __shared__ uint64_t s_key[MERGE_OUT];
+  __shared__ uint32_t s_pos[MERGE_OUT];
   __shared__ int s_split[2];
   const int ntiles = live_count(t.counters, C_TILES);
-  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  // work item = one MERGE_OUT slice of a sort tile's output range
+  for (int w = blockIdx.x; w < ntiles * MERGE_SLICES; w += gridDim.x) {
+    const int tile = w / MERGE_SLICES, slice = w - tile * MERGE_SLICES;
     const int node = t.tile_node[tile];
     if (!t.split[node]) continue;  // uniform over the workgroup
     const int a = t.lo[node], b = t.hi[node];
-    const int p0 = t.tile_p0[tile], p1 = t.tile_p1[tile];
-    const int pair_start = a + ((p0 - a) / (2 * width)) * (2 * width);
+    const int q0 = t.tile_p0[tile] + slice * MERGE_OUT;
+    const int q1 = min(t.tile_p1[tile], q0 + MERGE_OUT);
+    if (q0 >= q1) continue;
+    const int pair_start = a + ((q0 - a) / (2 * width)) * (2 * width);
     const int a0 = pair_start, a1 = min(b, pair_start + width);
     const int b0 = a1, b1 = min(b, a1 + width);
     const int la = a1 - a0, lb = b1 - b0;
-    const int d0 = p0 - pair_start, d1 = p1 - pair_start;
-    if (threadIdx.x < 2) {
-      const int d = threadIdx.x == 0 ? d0 : d1;
-      s_split[threadIdx.x] = merge_split(in_key + a0, in_pos + a0, la, in_key + b0, in_pos + b0, lb, d);
+    const int d0 = q0 - pair_start, d1 = q1 - pair_start;
+    if (threadIdx.x < 128) {
+      // wavefront 0 finds the slice's first split, wavefront 1 its end
+      const int wv = threadIdx.x >> 6;
+      const int split = wave_merge_split(in_key + a0, in_pos + a0, la, in_key + b0, in_pos + b0, lb, wv ? d1 : d0);
+      if ((threadIdx.x & 63) == 0) s_split[wv] = split;
     }
     __syncthreads();
     const int i0 = s_split[0], i1 = s_split[1];
     const int j0 = d0 - i0, j1 = d1 - i1;
-    const int na = i1 - i0, nb = j1 - j0;  // na + nb = p1 - p0 <= TILE
+    const int na = i1 - i0, nb = j1 - j0;  // na + nb = q1 - q0 <= MERGE_OUT
     for (int e = threadIdx.x; e < na + nb; e += blockDim.x) {
       const int src = e < na ? a0 + i0 + e : b0 + j0 + (e - na);
       s_key[e] = in_key[src];
       s_pos[e] = in_pos[src];
     }
     __syncthreads();
-    const int o_begin = threadIdx.x * PER_THREAD;
-    if (o_begin < na + nb) {
-      const int o_end = min(na + nb, o_begin + PER_THREAD);
+    // each thread merges MERGE_E consecutive outputs into registers
+    uint64_t k[MERGE_E];
+    uint32_t p[MERGE_E];
+    const int o_begin = threadIdx.x * MERGE_E;
+    const int cnt = max(0, min(na + nb, o_begin + MERGE_E) - o_begin);
+    if (cnt > 0) {
       int i = merge_split(s_key, s_pos, na, s_key + na, s_pos + na, nb, o_begin);
       int j = o_begin - i;
-      for (int o = o_begin; o < o_end; ++o) {
-        bool take_a;
-        if (i >= na) take_a = false;
-        else if (j >= nb) take_a = true;
-        else take_a = !pair_less(s_key[na + j], s_pos[na + j], s_key[i], s_pos[i]);
-        if (take_a) {
-          out_key[p0 + o] = s_key[i];
-          out_pos[p0 + o] = s_pos[i];
-          ++i;
-        } else {
-          out_key[p0 + o] = s_key[na + j];
-          out_pos[p0 + o] = s_pos[na + j];
-          ++j;
+#pragma unroll
+      for (int e = 0; e < MERGE_E; ++e) {
+        if (e < cnt) {
+          const bool take_a = j >= nb || (i < na && !pair_less(s_key[na + j], s_pos[na + j], s_key[i], s_pos[i]));
+          if (take_a) {
+            k[e] = s_key[i];
+            p[e] = s_pos[i];
+            ++i;
+          } else {
+            k[e] = s_key[na + j];
+            p[e] = s_pos[na + j];
+            ++j;
+          }
         }
       }
     }
-    __syncthreads();  // the next tile reuses the LDS
+    __syncthreads();
+    // back through LDS, so that the global stores are coalesced
+#pragma unroll
+    for (int e = 0; e < MERGE_E; ++e)
+      if (e < cnt) {
+        s_key[o_begin + e] = k[e];
+        s_pos[o_begin + e] = p[e];
+      }
+    __syncthreads();
+    for (int e = threadIdx.x; e < na + nb; e += blockDim.x) {
+      out_key[q0 + e] = s_key[e];
+      out_pos[q0 + e] = s_pos[e];
+    }
+    __syncthreads();  // the next work item reuses the LDS
   }
 }
 
@@ -895,8 +955,8 @@ int tree_build(Ctx& c, TreeArrays& t, const double* mx, const double* my, const 
     uint64_t *kin = t.keyA, *kout = t.keyB;
     uint32_t *pin = t.posA, *pout = t.posB;
     for (int64_t width = TILE; width < biggest; width <<= 1) {
-      hipLaunchKernelGGL(k_merge_pass, dim3(grid), dim3(MERGE_THREADS), 0, c.stream, t, (int)width, kin, pin, kout,
-                         pout);
+      hipLaunchKernelGGL(k_merge_pass, dim3(grid * 2), dim3(MERGE_THREADS), 0, c.stream, t, (int)width, kin, pin,
+                         kout, pout);
       std::swap(kin, kout);
       std::swap(pin, pout);
     }
