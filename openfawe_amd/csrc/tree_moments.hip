@@ -1,13 +1,16 @@
 // Node quantities of the tree on the GPU: centre, bounding radius, largest core, moments m0, m1, m2 and
 // their contractions (vortex_tree.py:118-167 node_moments, :425-447 node_contractions).
 //
-// Segments are first permuted into tree order, so that every node's segments are one contiguous range and
-// every pass reads HBM with coalesced loads.  Each node's range is cut into items of at most ITEM segments
-// (planned on the device with one scan); one wavefront reduces one item (lanes stride, then a fixed
-// butterfly), and one wavefront per node combines the node's items (lanes stride, fixed butterfly).  Box,
-// centre and core are exact (min / max), so they equal the reference's bit for bit; the bounding radius is
-// the square root of an exact maximum of squared distances computed operation by operation as the reference
-// does, so it is bit-exact too.  The moments are sums in another order: equal to the reference to rounding.
+// Segments are first permuted into tree order, so that every node's segments are one contiguous range.
+//  * Leaves: one thread per leaf computes its box, centre, core, squared bound and moments from its
+//    segments, in the reference's operation order (so a leaf's quantities equal the reference's bit for bit).
+//  * Internal nodes, bottom-up, one launch per depth: a thread computes a node from its two children -- box
+//    and core by exact min / max (bit-exact), moments shifted to the node's centre by the parallel-axis
+//    identity (equal to the reference's direct sums to rounding).
+//  * The bounding radius of an internal node needs its own centre and every end point below it: its range is
+//    cut into items of at most ITEM segments (planned on the device with one scan), one wavefront takes the
+//    exact maximum over an item, and one wavefront per node combines the items.  The square root of an exact
+//    maximum of squared distances computed operation by operation as the reference does: bit-exact.
 #include <algorithm>
 #include <climits>
 #include <vector>
@@ -21,7 +24,6 @@
 namespace ofw {
 
 static constexpr int ITEM = 1024;
-static constexpr int MPARTS = 31;  // r2, m0 (3), m1 (9), m2 (18 entries j <= l)
 
 // vortex_tree.py:608-611, elementwise in FP64 without contraction: the midpoints the tree is built on
 __global__ __launch_bounds__(256) void k_tree_prep(int m, const double* __restrict__ starts,
@@ -75,19 +77,6 @@ __global__ __launch_bounds__(256) void k_tree_permute(int m, const int32_t* __re
               rec32 + (size_t)k * REC32);
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-__device__ __forceinline__ double wave_min(double v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    const double o = __shfl_xor(v, off, 64);
-    v = o < v ? o : v;
-  }
-  return v;
-}
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
@@ -97,12 +86,14 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
-// ---------------------------------------------------------------- item plan
+// ---------------------------------------------------------------- item plan (internal nodes)
+__device__ __forceinline__ bool is_leaf(const TreeArrays& t, int p) { return t.p_skip[p] == p + 1; }
+
 __global__ void k_items_count(TreeArrays t, int32_t* __restrict__ cnt) {
   const int count = live_count(t.counters, C_FINAL);
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < t.cap; p += gridDim.x * blockDim.x) {
     int c = 0;
-    if (p < count) c = max(1, (t.p_hi[p] - t.p_lo[p] + ITEM - 1) / ITEM);
+    if (p < count && !is_leaf(t, p)) c = (t.p_hi[p] - t.p_lo[p] + ITEM - 1) / ITEM;
     cnt[p] = c;
   }
 }
@@ -112,7 +103,7 @@ __global__ void k_items_fill(TreeArrays t, const int32_t* __restrict__ first, in
   const int count = live_count(t.counters, C_FINAL);
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < count; p += gridDim.x * blockDim.x) {
     const int a = t.p_lo[p], b = t.p_hi[p];
-    const int n = max(1, (b - a + ITEM - 1) / ITEM);
+    const int n = is_leaf(t, p) ? 0 : (b - a + ITEM - 1) / ITEM;
     const int f = first[p];
     if (f + n > item_cap) {
       t.counters[C_OVERFLOW] = 1;
@@ -127,77 +118,128 @@ __global__ void k_items_fill(TreeArrays t, const int32_t* __restrict__ first, in
   }
 }
 
-// ---------------------------------------------------------------- passes
-// pass 1: box of start and end points, largest core (7 values per item)
-__global__ __launch_bounds__(256) void k_moments_box(const int32_t* __restrict__ counters,
-                                                     const int32_t* __restrict__ item_p0,
-                                                     const int32_t* __restrict__ item_p1, MomentInputs mi,
-                                                     double* __restrict__ part) {
-  const int items = live_count(counters, C_ITEMS);
-  const int lane = threadIdx.x & 63;
-  for (int w = blockIdx.x * 4 + threadIdx.x / 64; w < items; w += gridDim.x * 4) {
-    const int p0 = item_p0[w], p1 = item_p1[w];
-    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    double big = 0.0;
-    for (int k = p0 + lane; k < p1; k += 64) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const double a = mi.s[c][k], b = mi.e[c][k];
-        lo[c] = a < lo[c] ? a : lo[c];
-        hi[c] = a > hi[c] ? a : hi[c];
-        lo[c] = b < lo[c] ? b : lo[c];
-        hi[c] = b > hi[c] ? b : hi[c];
-      }
-      const double r = mi.rad[k];
-      big = r > big ? r : big;
-    }
-    double out[7];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      out[c] = wave_min(lo[c]);
-      out[3 + c] = wave_max(hi[c]);
-    }
-    out[6] = wave_max(big);
-    if (lane == 0)
-#pragma unroll
-      for (int q = 0; q < 7; ++q) part[7 * (size_t)w + q] = out[q];
-  }
-}
+// Per-node scratch of the bottom-up pass (pre-order index): box of the end points (low xyz, high xyz),
+// moments (m0 3, m1 9, m2 as 3 x 6 entries j <= l), squared bound of a leaf.
+struct NodeScratch {
+  double* box;
+  double* mom;
+  double* r2;
+};
 
-// one wavefront per node: combine its item boxes -> centre and largest core
-__global__ __launch_bounds__(256) void k_moments_centre(const int32_t* __restrict__ counters,
-                                                        const int32_t* __restrict__ first,
-                                                        const int32_t* __restrict__ ncount,
-                                                        const double* __restrict__ part, double* __restrict__ centre,
-                                                        double* __restrict__ core) {
+// The parent's moments from its two children's, about its own centre: with delta = c_child - c_parent,
+// m1 += m0 (x) delta, m2_ijl += m1_ij delta_l + m1_il delta_j + m0_i delta_j delta_l (the length terms do
+// not depend on the centre).
+__device__ __forceinline__ void shift_add(const double* __restrict__ cm, const double* dl, double* pm) {
 #pragma clang fp contract(off)
-  const int count = live_count(counters, C_FINAL);
-  const int lane = threadIdx.x & 63;
-  for (int node = blockIdx.x * 4 + threadIdx.x / 64; node < count; node += gridDim.x * 4) {
-    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    double big = 0.0;
-    for (int it = first[node] + lane; it < first[node] + ncount[node]; it += 64) {
-      const double* p = part + 7 * (size_t)it;
-      for (int c = 0; c < 3; ++c) {
-        lo[c] = p[c] < lo[c] ? p[c] : lo[c];
-        hi[c] = p[3 + c] > hi[c] ? p[3 + c] : hi[c];
-      }
-      big = p[6] > big ? p[6] : big;
-    }
-    for (int c = 0; c < 3; ++c) {
-      lo[c] = wave_min(lo[c]);
-      hi[c] = wave_max(hi[c]);
-    }
-    big = wave_max(big);
-    if (lane == 0) {
-      for (int c = 0; c < 3; ++c) centre[3 * (size_t)node + c] = 0.5 * (lo[c] + hi[c]);
-      core[node] = big;
+  const int jj[6] = {0, 0, 0, 1, 1, 2}, ll[6] = {0, 1, 2, 1, 2, 2};
+  for (int i = 0; i < 3; ++i) {
+    const double a0 = cm[i];
+    pm[i] += a0;
+    for (int j = 0; j < 3; ++j) pm[3 + 3 * i + j] += cm[3 + 3 * i + j] + a0 * dl[j];
+    for (int u = 0; u < 6; ++u) {
+      const int j = jj[u], l = ll[u];
+      pm[12 + 6 * i + u] += cm[12 + 6 * i + u] + cm[3 + 3 * i + j] * dl[l] + cm[3 + 3 * i + l] * dl[j] +
+                            a0 * (dl[j] * dl[l]);
     }
   }
 }
 
-// pass 2: squared bounding radius and the moments about the node centre
-__global__ __launch_bounds__(256) void k_moments_sum(const int32_t* __restrict__ counters,
+// One thread per leaf: its box, centre, largest core, squared bound and moments from its segments in tree
+// order, operation by operation as node_moments does (vortex_tree.py:118-167).
+__global__ __launch_bounds__(256) void k_leaves(TreeArrays t, MomentInputs mi, NodeScratch ns,
+                                                double* __restrict__ centre, double* __restrict__ core) {
+#pragma clang fp contract(off)
+  const int count = live_count(t.counters, C_FINAL);
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= count || !is_leaf(t, p)) return;
+  const int a = t.p_lo[p], b = t.p_hi[p];
+  double low[3] = {INFINITY, INFINITY, INFINITY}, high[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int k = a; k < b; ++k)
+    for (int axis = 0; axis < 3; ++axis) {
+      const double v0 = mi.s[axis][k], v1 = mi.e[axis][k];
+      if (v0 < low[axis]) low[axis] = v0;
+      if (v0 > high[axis]) high[axis] = v0;
+      if (v1 < low[axis]) low[axis] = v1;
+      if (v1 > high[axis]) high[axis] = v1;
+    }
+  double c[3];
+  for (int axis = 0; axis < 3; ++axis) c[axis] = 0.5 * (low[axis] + high[axis]);
+  double m[30];
+  for (int q = 0; q < 30; ++q) m[q] = 0.0;
+  double r2 = 0.0, largest = 0.0;
+  for (int k = a; k < b; ++k) {
+    const double s0 = mi.s[0][k] - c[0], s1 = mi.s[1][k] - c[1], s2 = mi.s[2][k] - c[2];
+    const double qs = s0 * s0 + s1 * s1 + s2 * s2;
+    if (qs > r2) r2 = qs;
+    const double e0 = mi.e[0][k] - c[0], e1 = mi.e[1][k] - c[1], e2 = mi.e[2][k] - c[2];
+    const double qe = e0 * e0 + e1 * e1 + e2 * e2;
+    if (qe > r2) r2 = qe;
+    const double rad = mi.rad[k];
+    if (rad > largest) largest = rad;
+    if (mi.gam[k] == 0.0) continue;
+    double d[3], al[3], lv[3];
+    for (int axis = 0; axis < 3; ++axis) {
+      d[axis] = 0.5 * (mi.s[axis][k] + mi.e[axis][k]) - c[axis];
+      al[axis] = mi.a[axis][k];
+      lv[axis] = mi.l[axis][k];
+    }
+    for (int i = 0; i < 3; ++i) {
+      m[i] += al[i];
+      for (int j = 0; j < 3; ++j) m[3 + 3 * i + j] += al[i] * d[j];
+      int u = 0;
+      for (int j = 0; j < 3; ++j)
+        for (int l = j; l < 3; ++l) m[12 + 6 * i + u++] += al[i] * (d[j] * d[l] + lv[j] * lv[l] / 12.0);
+    }
+  }
+  for (int q = 0; q < 3; ++q) {
+    ns.box[6 * (size_t)p + q] = low[q];
+    ns.box[6 * (size_t)p + 3 + q] = high[q];
+    centre[3 * (size_t)p + q] = c[q];
+  }
+  for (int q = 0; q < 30; ++q) ns.mom[30 * (size_t)p + q] = m[q];
+  core[p] = largest;
+  ns.r2[p] = r2;
+}
+
+// The internal nodes of one depth from their two children (computed by the previous launch or by k_leaves):
+// box and core exactly by min / max, moments by shift_add, in the fixed order left then right.
+__global__ __launch_bounds__(256) void k_internal_level(TreeArrays t, int depth, NodeScratch ns,
+                                                        double* __restrict__ centre, double* __restrict__ core) {
+#pragma clang fp contract(off)
+  const int count = live_count(t.counters, C_FINAL);
+  for (int par = blockIdx.x * blockDim.x + threadIdx.x; par < count; par += gridDim.x * blockDim.x) {
+    if (t.p_depth[par] != depth || is_leaf(t, par)) continue;
+    const int left = par + 1, right = t.p_skip[left];
+    double pl[3], ph[3], pc[3];
+    for (int q = 0; q < 3; ++q) {
+      const double l0 = ns.box[6 * (size_t)left + q], r0 = ns.box[6 * (size_t)right + q];
+      const double l1 = ns.box[6 * (size_t)left + 3 + q], r1 = ns.box[6 * (size_t)right + 3 + q];
+      // the left child's segments come first: on equal values its bound is kept, as a sequential scan keeps it
+      pl[q] = r0 < l0 ? r0 : l0;
+      ph[q] = r1 > l1 ? r1 : l1;
+      pc[q] = 0.5 * (pl[q] + ph[q]);
+    }
+    double pm[30];
+    for (int q = 0; q < 30; ++q) pm[q] = 0.0;
+    double dl[3];
+    for (int q = 0; q < 3; ++q) dl[q] = centre[3 * (size_t)left + q] - pc[q];
+    shift_add(ns.mom + 30 * (size_t)left, dl, pm);
+    for (int q = 0; q < 3; ++q) dl[q] = centre[3 * (size_t)right + q] - pc[q];
+    shift_add(ns.mom + 30 * (size_t)right, dl, pm);
+    const double cl = core[left], cr = core[right];
+    for (int q = 0; q < 3; ++q) {
+      ns.box[6 * (size_t)par + q] = pl[q];
+      ns.box[6 * (size_t)par + 3 + q] = ph[q];
+      centre[3 * (size_t)par + q] = pc[q];
+    }
+    for (int q = 0; q < 30; ++q) ns.mom[30 * (size_t)par + q] = pm[q];
+    core[par] = cr > cl ? cr : cl;
+  }
+}
+
+// Squared bounding radius of internal nodes: one wavefront per item (lanes stride, fixed butterfly),
+// the same operations as node_moments.
+__global__ __launch_bounds__(256) void k_bound_items(const int32_t* __restrict__ counters,
                                                      const int32_t* __restrict__ item_node,
                                                      const int32_t* __restrict__ item_p0,
                                                      const int32_t* __restrict__ item_p1, MomentInputs mi,
@@ -209,60 +251,31 @@ __global__ __launch_bounds__(256) void k_moments_sum(const int32_t* __restrict__
     const int p0 = item_p0[w], p1 = item_p1[w];
     const double c0 = centre[3 * (size_t)node], c1 = centre[3 * (size_t)node + 1], c2 = centre[3 * (size_t)node + 2];
     double r2 = 0.0;
-    double acc[30];
-#pragma unroll
-    for (int q = 0; q < 30; ++q) acc[q] = 0.0;
     for (int k = p0 + lane; k < p1; k += 64) {
-      {
 #pragma clang fp contract(off)
-        const double s0 = mi.s[0][k] - c0, s1 = mi.s[1][k] - c1, s2 = mi.s[2][k] - c2;
-        const double qs = s0 * s0 + s1 * s1 + s2 * s2;
-        const double e0 = mi.e[0][k] - c0, e1 = mi.e[1][k] - c1, e2 = mi.e[2][k] - c2;
-        const double qe = e0 * e0 + e1 * e1 + e2 * e2;
-        r2 = qs > r2 ? qs : r2;
-        r2 = qe > r2 ? qe : r2;
-      }
-      if (mi.gam[k] == 0.0) continue;
-      const double d[3] = {0.5 * (mi.s[0][k] + mi.e[0][k]) - c0, 0.5 * (mi.s[1][k] + mi.e[1][k]) - c1,
-                           0.5 * (mi.s[2][k] + mi.e[2][k]) - c2};
-      const double al[3] = {mi.a[0][k], mi.a[1][k], mi.a[2][k]};
-      const double lv[3] = {mi.l[0][k], mi.l[1][k], mi.l[2][k]};
-      double dd[6];
-      int q = 0;
-#pragma unroll
-      for (int j = 0; j < 3; ++j)
-#pragma unroll
-        for (int l = j; l < 3; ++l) dd[q++] = d[j] * d[l] + lv[j] * lv[l] / 12.0;
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        acc[i] += al[i];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) acc[3 + 3 * i + j] += al[i] * d[j];
-#pragma unroll
-        for (int u = 0; u < 6; ++u) acc[12 + 6 * i + u] += al[i] * dd[u];
-      }
+      const double s0 = mi.s[0][k] - c0, s1 = mi.s[1][k] - c1, s2 = mi.s[2][k] - c2;
+      const double qs = s0 * s0 + s1 * s1 + s2 * s2;
+      const double e0 = mi.e[0][k] - c0, e1 = mi.e[1][k] - c1, e2 = mi.e[2][k] - c2;
+      const double qe = e0 * e0 + e1 * e1 + e2 * e2;
+      r2 = qs > r2 ? qs : r2;
+      r2 = qe > r2 ? qe : r2;
     }
     r2 = wave_max(r2);
-    double* out = part + MPARTS * (size_t)w;
-    if (lane == 0) out[0] = r2;
-#pragma unroll
-    for (int q = 0; q < 30; ++q) {
-      const double v = wave_sum(acc[q]);
-      if (lane == 0) out[1 + q] = v;
-    }
+    if (lane == 0) part[w] = r2;
   }
 }
 
+// index of the symmetric pair (j, l) among 00 01 02 11 12 22
 __device__ __forceinline__ int sym6(int j, int l) {
   const int a = j < l ? j : l, b = j < l ? l : j;
   return a == 0 ? b : (a == 1 ? 2 + b : 5);
 }
 
-// One wavefront per node: combine its item partials (lanes stride, fixed butterfly); contractions
-// (vortex_tree.py:425-447); full outputs (optional) and the packed traversal records.
+// One wavefront per node: the squared bound (a leaf's own, or the maximum over the node's items), the
+// contractions (vortex_tree.py:425-447), full outputs (optional) and the packed traversal records.
 __global__ __launch_bounds__(256) void k_moments_final(TreeArrays t, const int32_t* __restrict__ first,
                                                        const int32_t* __restrict__ ncount,
-                                                       const double* __restrict__ part,
+                                                       const double* __restrict__ part, NodeScratch ns,
                                                        const double* __restrict__ centre,
                                                        const double* __restrict__ core, double clearance,
                                                        NodeOutputs no, double* __restrict__ acc_rec,
@@ -270,24 +283,21 @@ __global__ __launch_bounds__(256) void k_moments_final(TreeArrays t, const int32
   const int count = live_count(t.counters, C_FINAL);
   const int lane = threadIdx.x & 63;
   for (int node = blockIdx.x * 4 + threadIdx.x / 64; node < count; node += gridDim.x * 4) {
-    double r2 = 0.0, v[30];
-    for (int q = 0; q < 30; ++q) v[q] = 0.0;
-    for (int it = first[node] + lane; it < first[node] + ncount[node]; it += 64) {
-      const double* p = part + MPARTS * (size_t)it;
-      r2 = p[0] > r2 ? p[0] : r2;
-      for (int q = 0; q < 30; ++q) v[q] += p[1 + q];
-    }
+    double r2 = 0.0;
+    for (int it = first[node] + lane; it < first[node] + ncount[node]; it += 64) r2 = part[it] > r2 ? part[it] : r2;
     r2 = wave_max(r2);
-    for (int q = 0; q < 30; ++q) v[q] = wave_sum(v[q]);
     if (lane != 0) continue;
+    if (ncount[node] == 0) r2 = ns.r2[node];  // a leaf
     double bound;
     {
 #pragma clang fp contract(off)
       bound = __builtin_sqrt(r2) * (1.0 + 1e-12);
     }
-    const double* m0 = v;
-    const double* m1 = v + 3;
-    const double* m2s = v + 12;
+    const double* v = ns.mom + 30 * (size_t)node;
+    double m0[3], m1[9], m2s[18];
+    for (int q = 0; q < 3; ++q) m0[q] = v[q];
+    for (int q = 0; q < 9; ++q) m1[q] = v[3 + q];
+    for (int q = 0; q < 18; ++q) m2s[q] = v[12 + q];
     double m2[27];
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j)
@@ -355,10 +365,12 @@ int tree_moments(Ctx& c, int buf, const TreeArrays& t, const double* starts, con
   const int item_cap = (int)std::min<int64_t>(item_cap64, INT_MAX / 64);
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const size_t M = (size_t)std::max(m, 1);
+  const size_t C = (size_t)cap;
   size_t need = 14 * al(M * 8) + al(M * REC64 * 8) + al(M * REC32 * 4);
-  need += 3 * al((size_t)item_cap * 4) + 2 * al((size_t)cap * 4);
-  need += al((size_t)item_cap * MPARTS * 8) + al((size_t)cap * 3 * 8) + al((size_t)cap * 8);
-  need += al((size_t)cap * ACC_REC * 8) + al((size_t)cap * FAR_REC * 8) + al((size_t)cap * FAR_REC * 4);
+  need += 3 * al((size_t)item_cap * 4) + 2 * al(C * 4);
+  need += al((size_t)item_cap * 8) + al(C * 3 * 8) + al(C * 8);
+  need += al(C * ACC_REC * 8) + al(C * FAR_REC * 8) + al(C * FAR_REC * 4);
+  need += al(C * 6 * 8) + al(C * 30 * 8) + al(C * 8);
   OFW_HIP(c.tree[buf].reserve(need));
   char* base = (char*)c.tree[buf].ptr;
   auto carve = [&](size_t bytes) {
@@ -378,14 +390,18 @@ int tree_moments(Ctx& c, int buf, const TreeArrays& t, const double* starts, con
   int32_t* d_inode = (int32_t*)carve((size_t)item_cap * 4);
   int32_t* d_ip0 = (int32_t*)carve((size_t)item_cap * 4);
   int32_t* d_ip1 = (int32_t*)carve((size_t)item_cap * 4);
-  int32_t* d_first = (int32_t*)carve((size_t)cap * 4);
-  int32_t* d_ncount = (int32_t*)carve((size_t)cap * 4);
-  double* d_part = (double*)carve((size_t)item_cap * MPARTS * 8);
-  td.centre = (double*)carve((size_t)cap * 3 * 8);
-  td.core = (double*)carve((size_t)cap * 8);
-  td.acc = (double*)carve((size_t)cap * ACC_REC * 8);
-  td.far64 = (double*)carve((size_t)cap * FAR_REC * 8);
-  td.far32 = (float*)carve((size_t)cap * FAR_REC * 4);
+  int32_t* d_first = (int32_t*)carve(C * 4);
+  int32_t* d_ncount = (int32_t*)carve(C * 4);
+  double* d_part = (double*)carve((size_t)item_cap * 8);
+  td.centre = (double*)carve(C * 3 * 8);
+  td.core = (double*)carve(C * 8);
+  td.acc = (double*)carve(C * ACC_REC * 8);
+  td.far64 = (double*)carve(C * FAR_REC * 8);
+  td.far32 = (float*)carve(C * FAR_REC * 4);
+  NodeScratch ns;
+  ns.box = (double*)carve(C * 6 * 8);
+  ns.mom = (double*)carve(C * 30 * 8);
+  ns.r2 = (double*)carve(C * 8);
   td.count = cap;
   const int grid = c.cu_count * 4;
   if (m > 0)
@@ -394,13 +410,17 @@ int tree_moments(Ctx& c, int buf, const TreeArrays& t, const double* starts, con
   hipLaunchKernelGGL(k_items_count, dim3(grid), dim3(256), 0, c.stream, t, d_ncount);
   scan_exclusive(c, cap, d_ncount, d_first, t.scan_part, nullptr);
   hipLaunchKernelGGL(k_items_fill, dim3(grid), dim3(256), 0, c.stream, t, d_first, item_cap, d_inode, d_ip0, d_ip1);
-  hipLaunchKernelGGL(k_moments_box, dim3(grid), dim3(256), 0, c.stream, t.counters, d_ip0, d_ip1, mi, d_part);
-  hipLaunchKernelGGL(k_moments_centre, dim3(grid), dim3(256), 0, c.stream, t.counters, d_first, d_ncount, d_part,
-                     td.centre, td.core);
-  hipLaunchKernelGGL(k_moments_sum, dim3(grid), dim3(256), 0, c.stream, t.counters, d_inode, d_ip0, d_ip1, mi,
+  hipLaunchKernelGGL(k_leaves, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, c.stream, t, mi, ns, td.centre,
+                     td.core);
+  // internal nodes level by level, deepest first (the depth of a median-split tree of s segments is at most
+  // ceil(log2 s))
+  for (int d = depth_bound; d >= 0; --d)
+    hipLaunchKernelGGL(k_internal_level, dim3((unsigned)std::max(1, std::min(grid, (cap + 255) / 256))), dim3(256), 0,
+                       c.stream, t, d, ns, td.centre, td.core);
+  hipLaunchKernelGGL(k_bound_items, dim3(grid), dim3(256), 0, c.stream, t.counters, d_inode, d_ip0, d_ip1, mi,
                      td.centre, d_part);
   NodeOutputs no = want_outputs ? td.out : NodeOutputs{};
-  hipLaunchKernelGGL(k_moments_final, dim3(grid), dim3(256), 0, c.stream, t, d_first, d_ncount, d_part, td.centre,
+  hipLaunchKernelGGL(k_moments_final, dim3(grid), dim3(256), 0, c.stream, t, d_first, d_ncount, d_part, ns, td.centre,
                      td.core, clearance, no, td.acc, td.far64, td.far32);
   OFW_HIP(hipGetLastError());
   return OFW_OK;
