@@ -167,7 +167,7 @@ struct ForestTables {
   const int32_t* task_first;  // first traversal task of each tree
   const int32_t* wave_off;    // first work item (wavefront) of each tree
   const int32_t* part_off;    // first partial-sum slot of each tree
-  const int32_t* list_base;   // first near-list entry of each tree (below FP64)
+  const int32_t* list_base;   // first near-list entry of each tree
   const int32_t* list_cap;    // near-list entries per wavefront of each tree
   const int32_t* counters;
 };
@@ -364,9 +364,9 @@ __global__ __launch_bounds__(256) void k_tree_eval(ForestTables ft, const double
       s_acc[wv][2 * lane + 128 * c + 1] = v[2 * c + 1];
     }
   };
-  // near-field list of this wavefront (below FP64)
-  const int lcap = PREC == OFW_FP64 ? 0 : uniform(ft.list_cap[k]);
-  const size_t lbase = PREC == OFW_FP64 ? 0 : (size_t)uniform(ft.list_base[k]) + (size_t)local * lcap;
+  // near-field list of this wavefront
+  const int lcap = uniform(ft.list_cap[k]);
+  const size_t lbase = (size_t)uniform(ft.list_base[k]) + (size_t)local * lcap;
   int nlist = 0;
   int i = F;
   int skip_until = F;
@@ -393,37 +393,14 @@ __global__ __launch_bounds__(256) void k_tree_eval(ForestTables ft, const double
     if (skip == i + 1) {
       // leaf: the near field of the lanes that did not accept it (vortex_tree.py:538-557)
       if (wave_any(open)) {
-        if (PREC == OFW_FP64) {
-          for (int s = lo; s < hi; ++s) {
-            bool use = open;
-            if (use && nextex == s) {
-              advance_to(s + 1);
-              use = false;
-            }
-            if (!wave_any(use)) continue;
-            if (use) {
-              ++near;
-              double vx = 0.0, vy = 0.0, vz = 0.0;
-              const int f = pair_fast64<MODEL>(x0, x1, x2, rec64 + (size_t)s * REC64, raise_singular, vx, vy, vz);
-              if (f != OFW_FLAG_OK) {
-                if (flag == OFW_FLAG_OK) flag = f;
-              } else {
-                ux += vx;
-                uy += vy;
-                uz += vz;
-              }
-            }
-          }
-        } else {
-          // the leaf goes to the wavefront's near-field list with the lanes that opened it; the near-field
-          // kernel evaluates the lists in short chunks spread over the whole GPU
-          const unsigned long long mask = __builtin_amdgcn_ballot_w64(open);
-          if (nlist < lcap) {
-            if (lane == 0) list[lbase + nlist] = NearEntry{lo, hi, (uint32_t)mask, (uint32_t)(mask >> 32)};
-            ++nlist;
-          } else if (lane == 0) {
-            atomicOr(const_cast<int32_t*>(ft.counters) + C_OVERFLOW, 1);
-          }
+        // the leaf goes to the wavefront's near-field list with the lanes that opened it; the near-field
+        // kernel evaluates the lists in short chunks spread over the whole GPU
+        const unsigned long long mask = __builtin_amdgcn_ballot_w64(open);
+        if (nlist < lcap) {
+          if (lane == 0) list[lbase + nlist] = NearEntry{lo, hi, (uint32_t)mask, (uint32_t)(mask >> 32)};
+          ++nlist;
+        } else if (lane == 0) {
+          atomicOr(const_cast<int32_t*>(ft.counters) + C_OVERFLOW, 1);
         }
       }
       i = i + 1;
@@ -436,7 +413,7 @@ __global__ __launch_bounds__(256) void k_tree_eval(ForestTables ft, const double
   const bool any = wave_any(far > 0 || near > 0);
   if (lane == 0) {
     pnz[wave] = any ? 1 : 0;
-    if (PREC != OFW_FP64) list_count[wave] = nlist;
+    list_count[wave] = nlist;
   }
   if (any && valid) {
     const size_t o = (size_t)uniform(ft.part_off[k]) + (size_t)q * nt + jl;
@@ -451,10 +428,11 @@ __global__ __launch_bounds__(256) void k_tree_eval(ForestTables ft, const double
   }
 }
 
-// Near field of the traversal (below FP64): one wavefront per work item, CHUNK consecutive entries of one
-// walk wavefront's list, for the same 64 targets.  The leaf's segments are staged one per lane in a window
-// of 64 consecutive tree positions and broadcast with readlane; every lane evaluates the FP32 pair and the
-// lanes that opened the leaf select the result (vortex_tree.py:538-557 for each of them).  A grid-stride
+// Near field of the traversal: one wavefront per work item, CHUNK consecutive entries of one walk
+// wavefront's list, for the same 64 targets.  Below FP64 the leaf's segments are staged one per lane in a
+// window of 64 consecutive tree positions and broadcast with readlane; in FP64 the 128-byte record is read
+// with scalar loads.  Every lane evaluates the pair and the lanes that opened the leaf select the result
+// (vortex_tree.py:538-557 for each of them).  A grid-stride
 // loop covers the work items, whose count is known on the device only.
 template <int MODEL, int PREC>
 __global__ __launch_bounds__(256) void k_tree_near(ForestTables ft, const double* __restrict__ pts,
@@ -528,6 +506,24 @@ __global__ __launch_bounds__(256) void k_tree_near(ForestTables ft, const double
           ((unsigned long long)(uint32_t)uniform((int)en.mask_hi) << 32) | (uint32_t)uniform((int)en.mask_lo);
       const bool open = valid && ((mask >> lane) & 1ull);
       if (open) advance_to(lo);
+      if (PREC == OFW_FP64) {
+        // FP64: the segment record is wave-uniform (scalar loads); an excluded pair is evaluated, not added
+        for (int s = lo; s < hi; ++s) {
+          const bool hit = open && nextex == s;
+          if (wave_any(hit)) {
+            if (hit) advance_to(s + 1);
+          }
+          const bool use = open && !hit;
+          double vx = 0.0, vy = 0.0, vz = 0.0;
+          const int fl = pair_fast64<MODEL>(x0, x1, x2, rec64 + (size_t)s * REC64, raise_singular, use, vx, vy, vz);
+          ux = use ? ux + (acc_t)vx : ux;
+          uy = use ? uy + (acc_t)vy : uy;
+          uz = use ? uz + (acc_t)vz : uz;
+          near += use ? 1 : 0;
+          if (fl != OFW_FLAG_OK && flag == OFW_FLAG_OK) flag = fl;
+        }
+        continue;
+      }
       for (int base = lo; base < hi;) {
         if (base < sbase || base >= sbase + 64) {
           sbase = base;
@@ -794,7 +790,10 @@ static void launch_near(int precision, dim3 grid, hipStream_t st, const ForestTa
                         const int32_t* ex_ptr, const int32_t* ex_pos, const TreeDevice& td, int raise,
                         const NearEntry* list, const int32_t* list_count, const int32_t* chunk_off, int slots,
                         double* cpart, int8_t* cflag, int32_t* cnear) {
-  if (precision == OFW_MIXED)
+  if (precision == OFW_FP64)
+    hipLaunchKernelGGL((k_tree_near<MODEL, OFW_FP64>), grid, dim3(256), 0, st, ft, pts, ex_ptr, ex_pos, td.rec64,
+                       td.rec32, raise, list, list_count, chunk_off, slots, cpart, cflag, cnear);
+  else if (precision == OFW_MIXED)
     hipLaunchKernelGGL((k_tree_near<MODEL, OFW_MIXED>), grid, dim3(256), 0, st, ft, pts, ex_ptr, ex_pos, td.rec64,
                        td.rec32, raise, list, list_count, chunk_off, slots, cpart, cflag, cnear);
   else
@@ -817,7 +816,7 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
   const int K = shape.K;
   const int64_t n = shape.n, m = t.m;
   const int D = std::max(0, std::min(shape.split_depth, MAX_ANC));
-  const bool lists = precision != OFW_FP64;
+  const bool lists = true;
   // host bounds: a tree has at most min(2^D, m_k) frontier subtrees of at most ceil(m_k / 2^D) segments; a
   // subtree of S segments has at most S / floor((leaf + 1) / 2) + 1 leaves
   const int64_t min_leaf = std::max(1, (t.leaf + 1) / 2);
