@@ -33,13 +33,13 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 vector
-FP32_PEAK_TFLOPS = 157.3    # MI355X FP32 vector (packed); non-packed issue rate is half of it
+FP32_PEAK_TFLOPS = 157.3    # MI355X FP32 vector (published, packed dual issue; the kernels issue unpacked FP32)
 HBM_PEAK_GBS = 8000.0
-# algorithmic ops per pair by core model (far pairs, outside the Lamb-Oseen cutoff), per accepted order-2
-# node, per visited node (acceptance test)
+# algorithmic ops per pair by core model (the operations of the reference's pair formula, vortex_numba.py:42-109,
+# each add, multiply, divide or square root counted once, outside the Lamb-Oseen cutoff) and per accepted order-2
+# node (far_velocity, vortex_tree.py:450-497, plus the acceptance test)
 OPS_PER_PAIR = {0: 67, 1: 71, 2: 69, 3: 75, 4: 70, 5: 86}
 OPS_PER_FAR = 178 + 4
-OPS_PER_VISIT = 13
 
 WORKLOADS = {
     # theta 0.11 keeps the relative L2 error against the FP64 direct sum below 1e-5 on this wake
@@ -161,6 +161,30 @@ def adjoint_bench(steps):
                 adjoint_fwd_ratio=fused / fwd, influence_jacobian_ms=jms, jacobian_shape=[n, n, 3])
 
 
+def traffic(workload, kernel):
+    """HBM bytes per launch of the kernel from the committed PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by
+    the copy-kernel calibration), or None when none was measured for this workload."""
+    path = ROOT / "profiles" / "traffic.json"
+    try:
+        table = json.loads(path.read_text())
+    except (OSError, ValueError):
+        return None
+    entry = table.get(f"{workload}:{kernel}")
+    return entry.get("bytes_per_launch") if entry else None
+
+
+def copy_calibration(mib=1024, repeats=5):
+    """A plain device copy of a known size, for scaling the PMC byte counters (FETCH_SIZE, WRITE_SIZE)."""
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(repeats):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps(dict(copy_bytes=n * 8, repeats=repeats)), flush=True)
+
+
 def accuracy(case, out, spec, sample=1024, seed=0):
     """Relative L2 and max-abs error of the measured call against the FP64 direct sum (HIP kernel) at a seeded
     sample of targets, with the same exclusions."""
@@ -231,6 +255,8 @@ def main():
     ap.add_argument("--workload", default="tree-mixed-820k", choices=sorted(WORKLOADS))
     ap.add_argument("--split-depth", type=int, default=-1)
     ap.add_argument("--theta", type=float, default=None, help="override the workload's opening angle")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (PMC byte-counter calibration)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the adjoint and batch legs")
     args = ap.parse_args()
@@ -238,6 +264,10 @@ def main():
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this build "
               f"(targets shard across GPUs without communication; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
     import torch
     from openfawe_amd import _lib
@@ -287,18 +317,24 @@ def main():
     value = n * m / (ms * 1e-3)
     phase_ms = np.mean(np.array(phases), axis=0).tolist()
     fp64 = spec["precision"] == "fp64"
-    peak = FP64_PEAK_TFLOPS if fp64 else FP32_PEAK_TFLOPS / 2
+    peak = FP64_PEAK_TFLOPS if fp64 else FP32_PEAK_TFLOPS
     if spec["kind"] == "tree":
         far_n = int(far.sum().item())
         near_n = int(near.sum().item())
-        flops = far_n * OPS_PER_FAR + near_n * OPS_PER_PAIR[case["model"]] + (far_n + near_n // 16) * OPS_PER_VISIT
-        kms = phase_ms[3]
-        names = ["build (midpoints + tree)", "node moments", "traversal plan (exclusions, tasks)", "traversal",
-                 "reduce"]
+        # the roofline kernel: the near-field pair kernel (the pair interactions of the tree code); the walk
+        # (acceptance tests and far-field expansions) is reported beside it
+        flops = near_n * OPS_PER_PAIR[case["model"]]
+        kms = phase_ms[5]
+        names = ["build (midpoints + tree)", "node moments", "traversal plan (exclusions, tasks)",
+                 "walk (k_tree_eval)", "near-field plan", "near field (k_tree_near)", "reduce"]
         metric = "pair-interactions/s (tree code, direct-sum equivalent n*m/t)"
-        kernel = "k_tree_eval"
+        kernel = "k_tree_near"
+        walk_flops = far_n * OPS_PER_FAR
         extra = dict(far_nodes_per_target=far_n / n, near_pairs_per_target=near_n / n,
-                     phases_ms=dict(zip(names, phase_ms)))
+                     phases_ms=dict(zip(names, phase_ms)),
+                     walk=dict(kernel="k_tree_eval", kernel_ms=phase_ms[3],
+                               far_field_tflops=walk_flops / (phase_ms[3] * 1e-3) / 1e12,
+                               note="latency-bound pre-order walk; far-field flops only"))
     else:
         pairs = n * m - len(case["pairs"])
         flops = pairs * OPS_PER_PAIR[case["model"]]
@@ -315,7 +351,8 @@ def main():
                     precision=spec["precision"], **({"theta": spec["theta"], "order": 2, "leaf_size": 32,
                                                      "core_clearance": 3.0} if spec["kind"] == "tree" else {})),
         roofline=dict(bound="valu-fp64" if fp64 else "valu-fp32", achieved=achieved, peak=peak, unit="TFLOP/s",
-                      frac=achieved / peak, traffic=None, kernel=kernel, kernel_ms=kms),
+                      frac=achieved / peak, traffic=traffic(args.workload, kernel), kernel=kernel, kernel_ms=kms,
+                      flops_per_pair=OPS_PER_PAIR[case["model"]]),
         **extra,
     )
     if spec["kind"] == "tree":

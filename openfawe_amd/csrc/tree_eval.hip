@@ -925,6 +925,7 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
     }
   }
   OFW_HIP(hipGetLastError());
+  OFW_HIP(c.mark());
   if (lists && wave_cap > 0) {
     // near-field work items: CHUNK list entries each, numbered in walk-wavefront order
     hipLaunchKernelGGL(k_chunk_count, dim3((unsigned)((W1 + 255) / 256)), dim3(256), 0, c.stream, ft, (int)wave_cap,
@@ -950,6 +951,7 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
       cnear = counts ? (int32_t*)cb : nullptr;
     }
     hipLaunchKernelGGL(k_check_slots, dim3(1), dim3(64), 0, c.stream, t.counters, (int)std::min<int64_t>(slots, INT_MAX));
+    OFW_HIP(c.mark());
     const dim3 ngrid((unsigned)std::max<int64_t>(1, std::min<int64_t>((int64_t)c.cu_count * 16, (CH + 3) / 4)));
     switch (model) {
       case 0: launch_near<0>(precision, ngrid, c.stream, ft, points, ex_ptr_used, ex_pos, td, raise_singular, list, list_count, chunk_off, (int)CH, cpart, cflag, cnear); break;
