@@ -34,6 +34,12 @@ __device__ inline int block_exclusive_scan(int v, int* total) {
   return result;
 }
 
+// true when the predicate holds on any / every active lane of the wavefront
+__device__ __forceinline__ bool wave_any(bool v) { return __builtin_amdgcn_ballot_w64(v) != 0; }
+__device__ __forceinline__ bool wave_all(bool v) {
+  return __builtin_amdgcn_ballot_w64(v) == __builtin_amdgcn_read_exec();
+}
+
 // Workgroup index remapped so that each XCD (workgroups are dispatched to the 8 XCDs round-robin) receives a
 // contiguous range of logical indices: neighbouring work shares one XCD's L2.  A bijection on [0, nb).
 __device__ __forceinline__ int xcd_block(int b, int nb) {

@@ -24,6 +24,7 @@ enum {
   C_WORK,           // traversal work items
   C_DEPTH,          // largest node depth
   C_CHUNKS,         // near-field work items of the traversal
+  C_SUBTREES,       // subtrees of the moment pass
   C_COUNT = 16
 };
 

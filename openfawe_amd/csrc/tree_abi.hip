@@ -98,8 +98,8 @@ static int tree_pipeline(Ctx& c, const TreeCall& tc, TreeArrays& t) {
   if (st) return st;
   OFW_HIP(c.mark());
   TreeDevice td;
-  st = tree_moments(c, 1, t, tc.starts, tc.directions, tc.lengths, tc.gamma, tc.radius, tc.model, tc.clearance, td,
-                    false);
+  st = tree_moments(c, 1, t, tc.soff, tc.starts, tc.directions, tc.lengths, tc.gamma, tc.radius, tc.model,
+                    tc.clearance, td, false);
   if (st) return st;
   OFW_HIP(c.mark());
   EvalShape shape;
@@ -268,7 +268,7 @@ int ofw_tree_nodes(ofw_ctx* x, int64_t m, const double* starts, const double* di
   td.out.w1 = pre[6];
   td.out.t2 = pre[7];
   td.out.w2 = pre[8];
-  st = ofw::tree_moments(c, 1, t, starts, directions, lengths, gamma, radius, OFW_MODEL_NONE, 0.0, td, true);
+  st = ofw::tree_moments(c, 1, t, offsets, starts, directions, lengths, gamma, radius, OFW_MODEL_NONE, 0.0, td, true);
   if (st) return st;
   OFW_HIP(hipMemcpyAsync(pre[0], td.centre, (size_t)count * 3 * 8, hipMemcpyDeviceToDevice, c.stream));
   OFW_HIP(hipMemcpyAsync(pre[2], td.core, (size_t)count * 8, hipMemcpyDeviceToDevice, c.stream));

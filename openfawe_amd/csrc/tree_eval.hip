@@ -143,8 +143,6 @@ __device__ __forceinline__ void far32(float rx, float ry, float rz, const float*
   uz = k * u2;
 }
 
-__device__ __forceinline__ bool wave_any(bool v) { return __builtin_amdgcn_ballot_w64(v) != 0; }
-
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // value of lane `src` (wave-uniform index) for every lane

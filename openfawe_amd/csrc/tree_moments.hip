@@ -2,28 +2,27 @@
 // their contractions (vortex_tree.py:118-167 node_moments, :425-447 node_contractions).
 //
 // Segments are first permuted into tree order, so that every node's segments are one contiguous range.
-//  * Leaves: one thread per leaf computes its box, centre, core, squared bound and moments from its
-//    segments, in the reference's operation order (so a leaf's quantities equal the reference's bit for bit).
-//  * Internal nodes, bottom-up, one launch per depth: a thread computes a node from its two children -- box
-//    and core by exact min / max (bit-exact), moments shifted to the node's centre by the parallel-axis
-//    identity (equal to the reference's direct sums to rounding).
-//  * The bounding radius of an internal node needs its own centre and every end point below it: its range is
-//    cut into items of at most ITEM segments (planned on the device with one scan), one wavefront takes the
-//    exact maximum over an item, and one wavefront per node combines the items.  The square root of an exact
-//    maximum of squared distances computed operation by operation as the reference does: bit-exact.
+//  * Below a cut depth, one workgroup finishes a whole subtree in LDS: each leaf on half a wavefront (box,
+//    centre, core, squared bound and moments in the reference's operation order, so a leaf's quantities equal
+//    the reference's bit for bit); the internal nodes level by level from their two children (box and core by
+//    exact min / max, bit-exact; moments shifted to the node's centre by the parallel-axis identity, equal to
+//    the reference's direct sums to rounding); the exact squared bound of each internal node in one sweep
+//    over the subtree's end points.
+//  * Above the cut, one workgroup per tree finishes the internal nodes level by level, and one pass over all
+//    segments gives their exact squared bounds by integer atomic maxima on the bits.
+//  * A last pass takes the square roots, the contractions and packs the traversal records.
 #include <algorithm>
 #include <climits>
 #include <vector>
 
 #include "common.h"
 #include "internal.h"
+#include "scan.h"
 #include "tree.h"
 #include "tree_eval.h"
 #include "tree_records.h"
 
 namespace ofw {
-
-static constexpr int ITEM = 1024;
 
 // vortex_tree.py:608-611, elementwise in FP64 without contraction: the midpoints the tree is built on
 __global__ __launch_bounds__(256) void k_tree_prep(int m, const double* __restrict__ starts,
@@ -86,44 +85,55 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
-// ---------------------------------------------------------------- item plan (internal nodes)
+// ---------------------------------------------------------------- bottom-up node quantities
 __device__ __forceinline__ bool is_leaf(const TreeArrays& t, int p) { return t.p_skip[p] == p + 1; }
 
-__global__ void k_items_count(TreeArrays t, int32_t* __restrict__ cnt) {
-  const int count = live_count(t.counters, C_FINAL);
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < t.cap; p += gridDim.x * blockDim.x) {
-    int c = 0;
-    if (p < count && !is_leaf(t, p)) c = (t.p_hi[p] - t.p_lo[p] + ITEM - 1) / ITEM;
-    cnt[p] = c;
-  }
-}
+// Each tree is cut at depth (its depth bound - MB_DEPTH): one workgroup finishes each subtree below the cut
+// (k_moments_bottom, at most 2^(MB_DEPTH+1) - 1 nodes, held in LDS); one workgroup per tree finishes the
+// internal nodes of the MB_MAX_CUT levels under the root (k_moments_top), and a launch per level the
+// internal nodes between those and the cut (trees deeper than MB_MAX_CUT + MB_DEPTH).
+static constexpr int MB_DEPTH = 5;
+static constexpr int SUB_CAP = 64;
+static constexpr int MB_MAX_CUT = 12;
+static constexpr int LEAF_LANES = 32;                  // lanes of one leaf job (half a wavefront)
+static constexpr int BOTTOM_MT = 256;                  // threads of k_moments_bottom
+static constexpr int HALVES = BOTTOM_MT / LEAF_LANES;  // leaf jobs at a time
 
-__global__ void k_items_fill(TreeArrays t, const int32_t* __restrict__ first, int item_cap,
-                             int32_t* __restrict__ inode, int32_t* __restrict__ ip0, int32_t* __restrict__ ip1) {
-  const int count = live_count(t.counters, C_FINAL);
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < count; p += gridDim.x * blockDim.x) {
-    const int a = t.p_lo[p], b = t.p_hi[p];
-    const int n = is_leaf(t, p) ? 0 : (b - a + ITEM - 1) / ITEM;
-    const int f = first[p];
-    if (f + n > item_cap) {
-      t.counters[C_OVERFLOW] = 1;
-      continue;
-    }
-    for (int q = 0; q < n; ++q) {
-      inode[f + q] = p;
-      ip0[f + q] = a + q * ITEM;
-      ip1[f + q] = min(b, a + (q + 1) * ITEM);
-    }
-    if (p == count - 1) t.counters[C_ITEMS] = f + n;
+// Per-tree tables and lists of the moment pass (device, int32).
+struct MomentForest {
+  int K;
+  const int32_t* cut;  // K: the cut depth of each tree
+  int32_t* top_count;  // K x MB_MAX_CUT: internal nodes listed per tree and level (cleared by the table upload)
+  int32_t* top_list;   // K x 2^MB_MAX_CUT: level d of tree k at k * 2^MB_MAX_CUT + 2^d - 1
+  int32_t* roots;      // subtree roots, in any order (their count is C_SUBTREES)
+};
+
+// the last k with table[k] <= v (table[0] = 0 <= v < table[K])
+__device__ __forceinline__ int last_le(const int32_t* __restrict__ table, int K, int v) {
+  int lo = 0, hi = K;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (table[mid] <= v) lo = mid;
+    else hi = mid;
   }
+  return lo;
 }
 
 // Per-node scratch of the bottom-up pass (pre-order index): box of the end points (low xyz, high xyz),
-// moments (m0 3, m1 9, m2 as 3 x 6 entries j <= l), squared bound of a leaf.
+// moments (m0 3, m1 9, m2 as 3 x 6 entries j <= l), squared bound.
 struct NodeScratch {
   double* box;
   double* mom;
   double* r2;
+};
+
+// Node quantities addressed by a node index, in global memory (pre-order index) or in LDS (index within a
+// subtree).
+struct NodeRef {
+  double* box;     // 6 per node
+  double* centre;  // 3 per node
+  double* mom;     // 30 per node
+  double* core;    // 1 per node
 };
 
 // The parent's moments from its two children's, about its own centre: with delta = c_child - c_parent,
@@ -144,124 +154,407 @@ __device__ __forceinline__ void shift_add(const double* __restrict__ cm, const d
   }
 }
 
-// One thread per leaf: its box, centre, largest core, squared bound and moments from its segments in tree
-// order, operation by operation as node_moments does (vortex_tree.py:118-167).
-__global__ __launch_bounds__(256) void k_leaves(TreeArrays t, MomentInputs mi, NodeScratch ns,
-                                                double* __restrict__ centre, double* __restrict__ core) {
+// An internal node from its two children: box and core exactly by min / max, the centre from the box,
+// moments by shift_add in the fixed order left then right.
+__device__ __forceinline__ void combine_children(NodeRef v, int par, int left, int right) {
 #pragma clang fp contract(off)
-  const int count = live_count(t.counters, C_FINAL);
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= count || !is_leaf(t, p)) return;
-  const int a = t.p_lo[p], b = t.p_hi[p];
-  double low[3] = {INFINITY, INFINITY, INFINITY}, high[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int k = a; k < b; ++k)
-    for (int axis = 0; axis < 3; ++axis) {
-      const double v0 = mi.s[axis][k], v1 = mi.e[axis][k];
-      if (v0 < low[axis]) low[axis] = v0;
-      if (v0 > high[axis]) high[axis] = v0;
-      if (v1 < low[axis]) low[axis] = v1;
-      if (v1 > high[axis]) high[axis] = v1;
-    }
-  double c[3];
-  for (int axis = 0; axis < 3; ++axis) c[axis] = 0.5 * (low[axis] + high[axis]);
-  double m[30];
-  for (int q = 0; q < 30; ++q) m[q] = 0.0;
-  double r2 = 0.0, largest = 0.0;
-  for (int k = a; k < b; ++k) {
-    const double s0 = mi.s[0][k] - c[0], s1 = mi.s[1][k] - c[1], s2 = mi.s[2][k] - c[2];
-    const double qs = s0 * s0 + s1 * s1 + s2 * s2;
-    if (qs > r2) r2 = qs;
-    const double e0 = mi.e[0][k] - c[0], e1 = mi.e[1][k] - c[1], e2 = mi.e[2][k] - c[2];
-    const double qe = e0 * e0 + e1 * e1 + e2 * e2;
-    if (qe > r2) r2 = qe;
-    const double rad = mi.rad[k];
-    if (rad > largest) largest = rad;
-    if (mi.gam[k] == 0.0) continue;
-    double d[3], al[3], lv[3];
-    for (int axis = 0; axis < 3; ++axis) {
-      d[axis] = 0.5 * (mi.s[axis][k] + mi.e[axis][k]) - c[axis];
-      al[axis] = mi.a[axis][k];
-      lv[axis] = mi.l[axis][k];
-    }
-    for (int i = 0; i < 3; ++i) {
-      m[i] += al[i];
-      for (int j = 0; j < 3; ++j) m[3 + 3 * i + j] += al[i] * d[j];
-      int u = 0;
-      for (int j = 0; j < 3; ++j)
-        for (int l = j; l < 3; ++l) m[12 + 6 * i + u++] += al[i] * (d[j] * d[l] + lv[j] * lv[l] / 12.0);
-    }
-  }
+  double pl[3], ph[3], pc[3];
   for (int q = 0; q < 3; ++q) {
-    ns.box[6 * (size_t)p + q] = low[q];
-    ns.box[6 * (size_t)p + 3 + q] = high[q];
-    centre[3 * (size_t)p + q] = c[q];
+    const double l0 = v.box[6 * (size_t)left + q], r0 = v.box[6 * (size_t)right + q];
+    const double l1 = v.box[6 * (size_t)left + 3 + q], r1 = v.box[6 * (size_t)right + 3 + q];
+    // the left child's segments come first: on equal values its bound is kept, as a sequential scan keeps it
+    pl[q] = r0 < l0 ? r0 : l0;
+    ph[q] = r1 > l1 ? r1 : l1;
+    pc[q] = 0.5 * (pl[q] + ph[q]);
   }
-  for (int q = 0; q < 30; ++q) ns.mom[30 * (size_t)p + q] = m[q];
-  core[p] = largest;
-  ns.r2[p] = r2;
+  double pm[30];
+  for (int q = 0; q < 30; ++q) pm[q] = 0.0;
+  double dl[3];
+  for (int q = 0; q < 3; ++q) dl[q] = v.centre[3 * (size_t)left + q] - pc[q];
+  shift_add(v.mom + 30 * (size_t)left, dl, pm);
+  for (int q = 0; q < 3; ++q) dl[q] = v.centre[3 * (size_t)right + q] - pc[q];
+  shift_add(v.mom + 30 * (size_t)right, dl, pm);
+  const double cl = v.core[left], cr = v.core[right];
+  for (int q = 0; q < 3; ++q) {
+    v.box[6 * (size_t)par + q] = pl[q];
+    v.box[6 * (size_t)par + 3 + q] = ph[q];
+    v.centre[3 * (size_t)par + q] = pc[q];
+  }
+  for (int q = 0; q < 30; ++q) v.mom[30 * (size_t)par + q] = pm[q];
+  v.core[par] = cr > cl ? cr : cl;
 }
 
-// The internal nodes of one depth from their two children (computed by the previous launch or by k_leaves):
-// box and core exactly by min / max, moments by shift_add, in the fixed order left then right.
-__global__ __launch_bounds__(256) void k_internal_level(TreeArrays t, int depth, NodeScratch ns,
-                                                        double* __restrict__ centre, double* __restrict__ core) {
-#pragma clang fp contract(off)
-  const int count = live_count(t.counters, C_FINAL);
-  for (int par = blockIdx.x * blockDim.x + threadIdx.x; par < count; par += gridDim.x * blockDim.x) {
-    if (t.p_depth[par] != depth || is_leaf(t, par)) continue;
-    const int left = par + 1, right = t.p_skip[left];
-    double pl[3], ph[3], pc[3];
-    for (int q = 0; q < 3; ++q) {
-      const double l0 = ns.box[6 * (size_t)left + q], r0 = ns.box[6 * (size_t)right + q];
-      const double l1 = ns.box[6 * (size_t)left + 3 + q], r1 = ns.box[6 * (size_t)right + 3 + q];
-      // the left child's segments come first: on equal values its bound is kept, as a sequential scan keeps it
-      pl[q] = r0 < l0 ? r0 : l0;
-      ph[q] = r1 > l1 ? r1 : l1;
-      pc[q] = 0.5 * (pl[q] + ph[q]);
-    }
-    double pm[30];
-    for (int q = 0; q < 30; ++q) pm[q] = 0.0;
-    double dl[3];
-    for (int q = 0; q < 3; ++q) dl[q] = centre[3 * (size_t)left + q] - pc[q];
-    shift_add(ns.mom + 30 * (size_t)left, dl, pm);
-    for (int q = 0; q < 3; ++q) dl[q] = centre[3 * (size_t)right + q] - pc[q];
-    shift_add(ns.mom + 30 * (size_t)right, dl, pm);
-    const double cl = core[left], cr = core[right];
-    for (int q = 0; q < 3; ++q) {
-      ns.box[6 * (size_t)par + q] = pl[q];
-      ns.box[6 * (size_t)par + 3 + q] = ph[q];
-      centre[3 * (size_t)par + q] = pc[q];
-    }
-    for (int q = 0; q < 30; ++q) ns.mom[30 * (size_t)par + q] = pm[q];
-    core[par] = cr > cl ? cr : cl;
-  }
+// LDS written by some lanes of a wavefront and read by others: the wavefront's LDS operations execute in
+// order, so only the compiler must keep them in program order.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Squared bounding radius of internal nodes: one wavefront per item (lanes stride, fixed butterfly),
-// the same operations as node_moments.
-__global__ __launch_bounds__(256) void k_bound_items(const int32_t* __restrict__ counters,
-                                                     const int32_t* __restrict__ item_node,
-                                                     const int32_t* __restrict__ item_p0,
-                                                     const int32_t* __restrict__ item_p1, MomentInputs mi,
-                                                     const double* __restrict__ centre, double* __restrict__ part) {
-  const int items = live_count(counters, C_ITEMS);
-  const int lane = threadIdx.x & 63;
-  for (int w = blockIdx.x * 4 + threadIdx.x / 64; w < items; w += gridDim.x * 4) {
-    const int node = item_node[w];
-    const int p0 = item_p0[w], p1 = item_p1[w];
-    const double c0 = centre[3 * (size_t)node], c1 = centre[3 * (size_t)node + 1], c2 = centre[3 * (size_t)node + 2];
-    double r2 = 0.0;
-    for (int k = p0 + lane; k < p1; k += 64) {
+__device__ __forceinline__ double half_max(double v) {
+#pragma unroll
+  for (int off = 16; off >= 1; off >>= 1) {
+    const double o = __shfl_xor(v, off, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ unsigned long long dbits(double v) { return (unsigned long long)__double_as_longlong(v); }
+
+// One leaf [a, b) on 32 lanes (hl = lane within the half wavefront), operation by operation as node_moments
+// does (vortex_tree.py:118-167): every accumulator is updated in the order of the leaf's segments, each by
+// its own lane, so the leaf's quantities equal the sequential loop's bit for bit.  The segments are staged
+// in LDS 32 at a time by coalesced loads.
+//  * box: lanes 0-2 the low bound of axis hl, lanes 3-5 the high bound of axis hl - 3 (start, then end);
+//  * squared bound and largest core: one segment per lane, then the maximum over the lanes (exact);
+//  * moments: lane q < 30 accumulates m[q] (m0 3, m1 9, m2 as 3 x 6).
+// The results go to node i of v; the squared bound to r2b[i] as its bits.
+__device__ void leaf_job(const MomentInputs& mi, NodeRef v, unsigned long long* r2b, int i, int a, int b,
+                         double (*sh)[LEAF_LANES], int hl) {
 #pragma clang fp contract(off)
-      const double s0 = mi.s[0][k] - c0, s1 = mi.s[1][k] - c1, s2 = mi.s[2][k] - c2;
+  const int axis = hl < 3 ? hl : (hl < 6 ? hl - 3 : 0);
+  const bool low = hl < 3;
+  double bnd = low ? INFINITY : -INFINITY;
+  for (int base = a; base < b; base += LEAF_LANES) {
+    const int n = min(LEAF_LANES, b - base);
+    if (hl < n) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        sh[c][hl] = mi.s[c][base + hl];
+        sh[3 + c][hl] = mi.e[c][base + hl];
+      }
+    }
+    wave_lds_sync();
+    if (hl < 6) {
+#pragma unroll 4
+      for (int j = 0; j < n; ++j) {
+        const double v0 = sh[axis][j], v1 = sh[3 + axis][j];
+        if (low) {
+          if (v0 < bnd) bnd = v0;
+          if (v1 < bnd) bnd = v1;
+        } else {
+          if (v0 > bnd) bnd = v0;
+          if (v1 > bnd) bnd = v1;
+        }
+      }
+    }
+    wave_lds_sync();
+  }
+  const int h0 = threadIdx.x & 32;  // first lane of this half wavefront
+  double c[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) c[q] = 0.5 * (__shfl(bnd, h0 + q, 64) + __shfl(bnd, h0 + 3 + q, 64));
+  // accumulator of this lane: kind 0 = m0_i, 1 = m1_ij, 2 = m2_ijl (j <= l)
+  const int kind = hl < 3 ? 0 : (hl < 12 ? 1 : 2);
+  const int ai = hl < 3 ? hl : (hl < 12 ? (hl - 3) / 3 : min((hl - 12) / 6, 2));
+  const int u = hl < 12 ? 0 : (hl - 12) % 6;
+  const int aj = kind == 1 ? (hl - 3) % 3 : (u < 3 ? 0 : (u < 5 ? 1 : 2));
+  const int al = kind == 2 ? (u == 0 ? 0 : (u == 1 || u == 3) ? 1 : 2) : 0;
+  double acc = 0.0, r2 = 0.0, largest = 0.0;
+  // staged per segment: d (0-2), alpha (3-5), length vector (6-8), gamma (9)
+  for (int base = a; base < b; base += LEAF_LANES) {
+    const int n = min(LEAF_LANES, b - base);
+    if (hl < n) {
+      const int k = base + hl;
+      double sv[3], ev[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        sv[q] = mi.s[q][k];
+        ev[q] = mi.e[q][k];
+      }
+      const double s0 = sv[0] - c[0], s1 = sv[1] - c[1], s2 = sv[2] - c[2];
       const double qs = s0 * s0 + s1 * s1 + s2 * s2;
-      const double e0 = mi.e[0][k] - c0, e1 = mi.e[1][k] - c1, e2 = mi.e[2][k] - c2;
+      if (qs > r2) r2 = qs;
+      const double e0 = ev[0] - c[0], e1 = ev[1] - c[1], e2 = ev[2] - c[2];
       const double qe = e0 * e0 + e1 * e1 + e2 * e2;
-      r2 = qs > r2 ? qs : r2;
-      r2 = qe > r2 ? qe : r2;
+      if (qe > r2) r2 = qe;
+      const double rad = mi.rad[k];
+      if (rad > largest) largest = rad;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        sh[q][hl] = 0.5 * (sv[q] + ev[q]) - c[q];
+        sh[3 + q][hl] = mi.a[q][k];
+        sh[6 + q][hl] = mi.l[q][k];
+      }
+      sh[9][hl] = mi.gam[k];
     }
-    r2 = wave_max(r2);
-    if (lane == 0) part[w] = r2;
+    wave_lds_sync();
+    if (hl < 30) {
+#pragma unroll 4
+      for (int j = 0; j < n; ++j) {
+        const double alpha = sh[3 + ai][j];
+        const double dj = sh[aj][j], dl = sh[al][j];
+        const double lj = sh[6 + aj][j], ll = sh[6 + al][j];
+        const double x2 = dj * dl + lj * ll / 12.0;
+        // alpha * 1.0 == alpha exactly; a segment without circulation leaves the sum as it is
+        const double x = kind == 0 ? 1.0 : (kind == 1 ? dj : x2);
+        const double next = acc + alpha * x;
+        acc = sh[9][j] == 0.0 ? acc : next;
+      }
+    }
+    wave_lds_sync();
+  }
+  r2 = half_max(r2);
+  largest = half_max(largest);
+  if (hl < 30) v.mom[30 * i + hl] = acc;
+  if (hl < 6) v.box[6 * i + hl] = bnd;
+  if (hl < 3) v.centre[3 * i + hl] = c[hl == 0 ? 0 : (hl == 1 ? 1 : 2)];
+  if (hl == 0) {
+    v.core[i] = largest;
+    r2b[i] = dbits(r2);
+  }
+}
+
+// Steps of halving a node of s segments until it holds at most `leaf`: a bound on the depth below it.
+__device__ __host__ __forceinline__ int halving_depth(int64_t s, int leaf) {
+  int d = 0;
+  while (s > leaf) {
+    s = (s + 1) / 2;
+    ++d;
+  }
+  return d;
+}
+
+// One pass over the nodes: the subtree roots of k_moments_bottom (the nodes at their tree's cut depth and
+// the leaves above it) and, per tree and level, the internal nodes above min(cut, MB_MAX_CUT) for
+// k_moments_top.  The order of the lists does not matter: every entry is finished on its own.
+__global__ __launch_bounds__(256) void k_moment_lists(TreeArrays t, MomentForest mf) {
+  const int count = live_count(t.counters, C_FINAL);
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < count; p += gridDim.x * blockDim.x) {
+    const int k = t.p_tree[p], d = t.p_depth[p], cut = mf.cut[k];
+    const bool leaf = is_leaf(t, p);
+    if (d == cut || (d < cut && leaf)) mf.roots[atomicAdd(t.counters + C_SUBTREES, 1)] = p;
+    if (d < min(cut, MB_MAX_CUT) && !leaf) {
+      const int slot = atomicAdd(mf.top_count + (size_t)k * MB_MAX_CUT + d, 1);
+      mf.top_list[((size_t)k << MB_MAX_CUT) + (1 << d) - 1 + slot] = p;
+    }
+  }
+}
+
+// One workgroup per subtree below the cut depth, all of it in LDS: its leaves (a half wavefront each), its
+// internal nodes level by level from the deepest, then the exact squared bound of each internal node in one
+// sweep over the subtree's segments (each segment against the centres of the internal nodes on its path;
+// the maximum over a wavefront's lanes that share a node, then an LDS atomic maximum on the bits, which is
+// exact and independent of the order: non-negative doubles order as their bit patterns).
+__global__ __launch_bounds__(BOTTOM_MT) void k_moments_bottom(TreeArrays t, MomentForest mf, MomentInputs mi,
+                                                              NodeScratch ns, double* __restrict__ centre,
+                                                              double* __restrict__ core) {
+#pragma clang fp contract(off)
+  __shared__ double s_box[SUB_CAP * 6], s_cen[SUB_CAP * 3], s_mom[SUB_CAP * 30], s_core[SUB_CAP];
+  __shared__ unsigned long long s_r2[SUB_CAP];
+  __shared__ int s_lo[SUB_CAP], s_hi[SUB_CAP], s_skip[SUB_CAP], s_depth[SUB_CAP], s_leaves[SUB_CAP];
+  __shared__ double stage[HALVES][10][LEAF_LANES];
+  if (t.counters[C_OVERFLOW] || (int)blockIdx.x >= t.counters[C_SUBTREES]) return;
+  const int r = mf.roots[blockIdx.x];
+  const int nn = t.p_skip[r] - r;  // at most 2^(MB_DEPTH+1) - 1 <= SUB_CAP
+  const int tid = threadIdx.x;
+  int leaf_flag = 0;
+  if (tid < nn) {
+    const int p = r + tid;
+    const int skip = t.p_skip[p];
+    s_lo[tid] = t.p_lo[p];
+    s_hi[tid] = t.p_hi[p];
+    s_skip[tid] = skip - r;
+    s_depth[tid] = t.p_depth[p];
+    s_r2[tid] = 0ull;
+    leaf_flag = skip == p + 1 ? 1 : 0;
+  }
+  int nleaves;
+  const int rank = block_exclusive_scan(leaf_flag, &nleaves);
+  if (leaf_flag) s_leaves[rank] = tid;
+  __syncthreads();
+  const NodeRef lds{s_box, s_cen, s_mom, s_core};
+  const int hw = tid >> 5, hl = tid & 31;
+  for (int q = hw; q < nleaves; q += HALVES) {
+    const int i = s_leaves[q];
+    leaf_job(mi, lds, s_r2, i, s_lo[i], s_hi[i], stage[hw], hl);
+  }
+  __syncthreads();
+  if (nn > 1) {
+    const int d0 = s_depth[0];
+    const int dmax = d0 + halving_depth(s_hi[0] - s_lo[0], t.leaf);
+    for (int lev = dmax - 1; lev >= d0; --lev) {
+      if (tid < nn && s_depth[tid] == lev && s_skip[tid] != tid + 1) combine_children(lds, tid, tid + 1, s_skip[tid + 1]);
+      __syncthreads();
+    }
+    const int lane = tid & 63;
+    const int a = s_lo[0], b = s_hi[0];
+    for (int base = a; base < b; base += BOTTOM_MT) {
+      const int kk = base + tid;
+      bool live = kk < b;
+      double sv[3] = {0.0, 0.0, 0.0}, ev[3] = {0.0, 0.0, 0.0};
+      if (live)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          sv[q] = mi.s[q][kk];
+          ev[q] = mi.e[q][kk];
+        }
+      int i = 0;
+      while (wave_any(live)) {
+        if (live && s_skip[i] == i + 1) live = false;
+        double v = 0.0;
+        int id = -1;
+        if (live) {
+          const double c0 = s_cen[3 * i], c1 = s_cen[3 * i + 1], c2 = s_cen[3 * i + 2];
+          const double s0 = sv[0] - c0, s1 = sv[1] - c1, s2 = sv[2] - c2;
+          const double qs = s0 * s0 + s1 * s1 + s2 * s2;
+          const double e0 = ev[0] - c0, e1 = ev[1] - c1, e2 = ev[2] - c2;
+          const double qe = e0 * e0 + e1 * e1 + e2 * e2;
+          v = qs > qe ? qs : qe;
+          id = i;
+        }
+        const int first = __shfl(id, 0, 64);
+        if (wave_all(id == first || id < 0) && first >= 0) {
+          v = wave_max(v);
+          if (lane == 0) atomicMax(&s_r2[first], dbits(v));
+        } else if (id >= 0) {
+          atomicMax(&s_r2[id], dbits(v));
+        }
+        if (live) i = kk < s_hi[i + 1] ? i + 1 : s_skip[i + 1];
+      }
+    }
+    __syncthreads();
+  }
+  // the subtree's nodes are the pre-order indices r .. r + nn - 1: contiguous rows of every array
+  for (int x = tid; x < 30 * nn; x += BOTTOM_MT) ns.mom[30 * (size_t)r + x] = s_mom[x];
+  for (int x = tid; x < 6 * nn; x += BOTTOM_MT) ns.box[6 * (size_t)r + x] = s_box[x];
+  for (int x = tid; x < 3 * nn; x += BOTTOM_MT) centre[3 * (size_t)r + x] = s_cen[x];
+  if (tid < nn) {
+    core[r + tid] = s_core[tid];
+    ns.r2[r + tid] = __longlong_as_double((long long)s_r2[tid]);
+  }
+}
+
+// The internal nodes of one depth between MB_MAX_CUT and their tree's cut (trees deeper than
+// MB_MAX_CUT + MB_DEPTH), from their children; their squared bounds are cleared for k_bound_top.
+__global__ __launch_bounds__(256) void k_internal_level(TreeArrays t, MomentForest mf, int depth, NodeScratch ns,
+                                                        double* __restrict__ centre, double* __restrict__ core) {
+  const int count = live_count(t.counters, C_FINAL);
+  const NodeRef g{ns.box, centre, ns.mom, core};
+  for (int par = blockIdx.x * blockDim.x + threadIdx.x; par < count; par += gridDim.x * blockDim.x) {
+    if (t.p_depth[par] != depth || is_leaf(t, par) || depth >= mf.cut[t.p_tree[par]]) continue;
+    combine_children(g, par, par + 1, t.p_skip[par + 1]);
+    ns.r2[par] = 0.0;
+  }
+}
+
+// One workgroup per tree: the internal nodes of the first min(cut, MB_MAX_CUT) levels (listed by
+// k_moment_lists), level by level from the deepest; their children were finished by k_moments_bottom or
+// k_internal_level.  Their squared bounds are cleared for k_bound_top.
+__global__ __launch_bounds__(512) void k_moments_top(TreeArrays t, MomentForest mf, NodeScratch ns,
+                                                     double* __restrict__ centre, double* __restrict__ core) {
+  __shared__ int s_node[1 << MB_MAX_CUT], s_right[1 << MB_MAX_CUT];
+  if (t.counters[C_OVERFLOW]) return;
+  const int k = blockIdx.x;
+  const int D = min(mf.cut[k], MB_MAX_CUT);
+  if (D == 0 || t.tree_first[k] == t.tree_first[k + 1]) return;
+  const int32_t* list = mf.top_list + ((size_t)k << MB_MAX_CUT);
+  const int32_t* cnt = mf.top_count + (size_t)k * MB_MAX_CUT;
+  // every listed node and its right child, loaded together
+  for (int d = 0; d < D; ++d)
+    for (int i = (int)threadIdx.x; i < cnt[d]; i += blockDim.x) {
+      const int p = list[(1 << d) - 1 + i];
+      s_node[(1 << d) - 1 + i] = p;
+      s_right[(1 << d) - 1 + i] = t.p_skip[p + 1];
+    }
+  __syncthreads();
+  const NodeRef g{ns.box, centre, ns.mom, core};
+  for (int d = D - 1; d >= 0; --d) {
+    for (int i = (int)threadIdx.x; i < cnt[d]; i += blockDim.x) {
+      const int p = s_node[(1 << d) - 1 + i];
+      combine_children(g, p, p + 1, s_right[(1 << d) - 1 + i]);
+      ns.r2[p] = 0.0;
+    }
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ void max_bits(unsigned long long* a, double v) {
+  const unsigned long long b = dbits(v);
+  // the stored value only grows: a stale read can only cause an unneeded atomic, and skipping the others
+  // keeps thousands of wavefronts from queueing on the root's bound
+  if (b > *reinterpret_cast<volatile unsigned long long*>(a)) atomicMax(a, b);
+}
+
+// Exact squared bounds of the internal nodes above the cut depth in one pass over the segments (four per
+// lane): each segment's end points against the centre of each such node on its root path; the maximum over
+// a wavefront's segments that share the node, then an integer atomic maximum on the bits.
+__global__ __launch_bounds__(256) void k_bound_top(TreeArrays t, MomentForest mf, MomentInputs mi,
+                                                   const double* __restrict__ centre, double* __restrict__ r2out) {
+#pragma clang fp contract(off)
+  constexpr int J = 4;
+  if (t.counters[C_OVERFLOW]) return;
+  const int m = (int)t.m;
+  const int lane = threadIdx.x & 63;
+  unsigned long long* bits = reinterpret_cast<unsigned long long*>(r2out);
+  const int stride = gridDim.x * 4 * 64 * J;
+  for (int base = (blockIdx.x * 4 + (int)(threadIdx.x >> 6)) * 64 * J; base < m; base += stride) {
+    int p[J], kk[J], D[J];
+    bool live[J];
+    double s[J][3], e[J][3];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      kk[j] = base + 64 * j + lane;
+      const bool valid = kk[j] < m;
+      const int tr = last_le(t.roots_lo, t.K, valid ? kk[j] : base);
+      D[j] = valid ? mf.cut[tr] : 0;
+      p[j] = valid ? t.tree_first[tr] : -1;
+      live[j] = valid && D[j] > 0;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        s[j][q] = valid ? mi.s[q][kk[j]] : 0.0;
+        e[j][q] = valid ? mi.e[q][kk[j]] : 0.0;
+      }
+    }
+    for (int lev = 0;; ++lev) {
+      bool any = false;
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        if (live[j] && (lev >= D[j] || is_leaf(t, p[j]))) live[j] = false;
+        any = any || live[j];
+      }
+      if (!wave_any(any)) break;
+      double v[J];
+      int id[J];
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        v[j] = 0.0;
+        id[j] = -1;
+        if (live[j]) {
+          const double c0 = centre[3 * (size_t)p[j]], c1 = centre[3 * (size_t)p[j] + 1],
+                       c2 = centre[3 * (size_t)p[j] + 2];
+          const double s0 = s[j][0] - c0, s1 = s[j][1] - c1, s2 = s[j][2] - c2;
+          const double qs = s0 * s0 + s1 * s1 + s2 * s2;
+          const double e0 = e[j][0] - c0, e1 = e[j][1] - c1, e2 = e[j][2] - c2;
+          const double qe = e0 * e0 + e1 * e1 + e2 * e2;
+          v[j] = qs > qe ? qs : qe;
+          id[j] = p[j];
+        }
+      }
+      const int first = __shfl(id[0], 0, 64);
+      bool same = first >= 0;
+#pragma unroll
+      for (int j = 0; j < J; ++j) same = same && (id[j] == first || id[j] < 0);
+      if (wave_all(same)) {
+        double vm = v[0];
+#pragma unroll
+        for (int j = 1; j < J; ++j) vm = v[j] > vm ? v[j] : vm;
+        vm = wave_max(vm);
+        if (lane == 0) max_bits(bits + first, vm);
+      } else {
+#pragma unroll
+        for (int j = 0; j < J; ++j)
+          if (id[j] >= 0) max_bits(bits + id[j], v[j]);
+      }
+#pragma unroll
+      for (int j = 0; j < J; ++j)
+        if (live[j]) {
+          const int left = p[j] + 1;
+          p[j] = kk[j] < t.p_hi[left] ? left : t.p_skip[left];
+        }
+    }
   }
 }
 
@@ -271,23 +564,15 @@ __device__ __forceinline__ int sym6(int j, int l) {
   return a == 0 ? b : (a == 1 ? 2 + b : 5);
 }
 
-// One wavefront per node: the squared bound (a leaf's own, or the maximum over the node's items), the
-// contractions (vortex_tree.py:425-447), full outputs (optional) and the packed traversal records.
-__global__ __launch_bounds__(256) void k_moments_final(TreeArrays t, const int32_t* __restrict__ first,
-                                                       const int32_t* __restrict__ ncount,
-                                                       const double* __restrict__ part, NodeScratch ns,
-                                                       const double* __restrict__ centre,
+// One thread per node: the bounding radius from the exact squared bound, the contractions
+// (vortex_tree.py:425-447), full outputs (optional) and the packed traversal records.
+__global__ __launch_bounds__(256) void k_moments_final(TreeArrays t, NodeScratch ns, const double* __restrict__ centre,
                                                        const double* __restrict__ core, double clearance,
                                                        NodeOutputs no, double* __restrict__ acc_rec,
                                                        double* __restrict__ far64, float* __restrict__ far32) {
   const int count = live_count(t.counters, C_FINAL);
-  const int lane = threadIdx.x & 63;
-  for (int node = blockIdx.x * 4 + threadIdx.x / 64; node < count; node += gridDim.x * 4) {
-    double r2 = 0.0;
-    for (int it = first[node] + lane; it < first[node] + ncount[node]; it += 64) r2 = part[it] > r2 ? part[it] : r2;
-    r2 = wave_max(r2);
-    if (lane != 0) continue;
-    if (ncount[node] == 0) r2 = ns.r2[node];  // a leaf
+  for (int node = blockIdx.x * blockDim.x + threadIdx.x; node < count; node += gridDim.x * blockDim.x) {
+    const double r2 = ns.r2[node];
     double bound;
     {
 #pragma clang fp contract(off)
@@ -355,20 +640,36 @@ __global__ __launch_bounds__(256) void k_moments_final(TreeArrays t, const int32
   }
 }
 
-int tree_moments(Ctx& c, int buf, const TreeArrays& t, const double* starts, const double* directions,
-                 const double* lengths, const double* gamma, const double* radius, int model, double clearance,
-                 TreeDevice& td, bool want_outputs) {
-  const int m = (int)t.m, cap = t.cap;
-  int depth_bound = 2;
-  for (int64_t s = std::max<int64_t>(m, 1); s > 1; s = (s + 1) / 2) ++depth_bound;
-  const int64_t item_cap64 = (int64_t)m * depth_bound / ITEM + 2 * (int64_t)cap + 64;
-  const int item_cap = (int)std::min<int64_t>(item_cap64, INT_MAX / 64);
+int tree_moments(Ctx& c, int buf, const TreeArrays& t, const std::vector<int64_t>& offsets, const double* starts,
+                 const double* directions, const double* lengths, const double* gamma, const double* radius, int model,
+                 double clearance, TreeDevice& td, bool want_outputs) {
+  const int m = (int)t.m, cap = t.cap, K = t.K;
+  // per tree: the cut depth and the subtrees below it (at most one per path of that depth); the table also
+  // clears the per-level counters of the top lists
+  std::vector<int64_t> table((size_t)K * (1 + MB_MAX_CUT), 0);
+  int64_t wgs = 0;
+  int cut_max = 0;
+  for (int k = 0; k < K; ++k) {
+    const int64_t mk = offsets[k + 1] - offsets[k];
+    const int cut = std::max(0, halving_depth(mk, t.leaf) - MB_DEPTH);
+    if (cut > 30) {
+      set_error("tree too deep for the moment pass");
+      return OFW_ERR_ARG;
+    }
+    cut_max = std::max(cut_max, cut);
+    table[k] = cut;
+    wgs += mk > 0 ? ((int64_t)1 << cut) : 0;
+  }
+  if (wgs > INT_MAX / 2) {
+    set_error("tree too large for one moment pass");
+    return OFW_ERR_ARG;
+  }
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const size_t M = (size_t)std::max(m, 1);
   const size_t C = (size_t)cap;
-  size_t need = 14 * al(M * 8) + al(M * REC64 * 8) + al(M * REC32 * 4);
-  need += 3 * al((size_t)item_cap * 4) + 2 * al(C * 4);
-  need += al((size_t)item_cap * 8) + al(C * 3 * 8) + al(C * 8);
+  size_t need = 14 * al(M * 8) + al(M * REC64 * 8) + al(M * REC32 * 4) + al(table.size() * 4);
+  need += al(((size_t)K << MB_MAX_CUT) * 4) + al((size_t)std::max<int64_t>(wgs, 1) * 4);
+  need += al(C * 3 * 8) + al(C * 8);
   need += al(C * ACC_REC * 8) + al(C * FAR_REC * 8) + al(C * FAR_REC * 4);
   need += al(C * 6 * 8) + al(C * 30 * 8) + al(C * 8);
   OFW_HIP(c.tree[buf].reserve(need));
@@ -387,12 +688,9 @@ int tree_moments(Ctx& c, int buf, const TreeArrays& t, const double* starts, con
   mi.gam = (double*)carve(M * 8);
   td.rec64 = (double*)carve(M * REC64 * 8);
   td.rec32 = (float*)carve(M * REC32 * 4);
-  int32_t* d_inode = (int32_t*)carve((size_t)item_cap * 4);
-  int32_t* d_ip0 = (int32_t*)carve((size_t)item_cap * 4);
-  int32_t* d_ip1 = (int32_t*)carve((size_t)item_cap * 4);
-  int32_t* d_first = (int32_t*)carve(C * 4);
-  int32_t* d_ncount = (int32_t*)carve(C * 4);
-  double* d_part = (double*)carve((size_t)item_cap * 8);
+  int32_t* d_table = (int32_t*)carve(table.size() * 4);
+  int32_t* d_top = (int32_t*)carve(((size_t)K << MB_MAX_CUT) * 4);
+  int32_t* d_roots = (int32_t*)carve((size_t)std::max<int64_t>(wgs, 1) * 4);
   td.centre = (double*)carve(C * 3 * 8);
   td.core = (double*)carve(C * 8);
   td.acc = (double*)carve(C * ACC_REC * 8);
@@ -403,25 +701,27 @@ int tree_moments(Ctx& c, int buf, const TreeArrays& t, const double* starts, con
   ns.mom = (double*)carve(C * 30 * 8);
   ns.r2 = (double*)carve(C * 8);
   td.count = cap;
+  int st = upload_i32(c, table, d_table);
+  if (st) return st;
+  MomentForest mf{K, d_table, d_table + K, d_top, d_roots};
   const int grid = c.cu_count * 4;
-  if (m > 0)
+  if (m > 0) {
     hipLaunchKernelGGL(k_tree_permute, dim3((m + 255) / 256), dim3(256), 0, c.stream, m, t.order, starts, directions,
                        lengths, gamma, radius, model, mi, td.rec64, td.rec32);
-  hipLaunchKernelGGL(k_items_count, dim3(grid), dim3(256), 0, c.stream, t, d_ncount);
-  scan_exclusive(c, cap, d_ncount, d_first, t.scan_part, nullptr);
-  hipLaunchKernelGGL(k_items_fill, dim3(grid), dim3(256), 0, c.stream, t, d_first, item_cap, d_inode, d_ip0, d_ip1);
-  hipLaunchKernelGGL(k_leaves, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, c.stream, t, mi, ns, td.centre,
-                     td.core);
-  // internal nodes level by level, deepest first (the depth of a median-split tree of s segments is at most
-  // ceil(log2 s))
-  for (int d = depth_bound; d >= 0; --d)
-    hipLaunchKernelGGL(k_internal_level, dim3((unsigned)std::max(1, std::min(grid, (cap + 255) / 256))), dim3(256), 0,
-                       c.stream, t, d, ns, td.centre, td.core);
-  hipLaunchKernelGGL(k_bound_items, dim3(grid), dim3(256), 0, c.stream, t.counters, d_inode, d_ip0, d_ip1, mi,
-                     td.centre, d_part);
+    hipLaunchKernelGGL(k_moment_lists, dim3((unsigned)std::max(1, std::min(grid, (cap + 255) / 256))), dim3(256), 0,
+                       c.stream, t, mf);
+    hipLaunchKernelGGL(k_moments_bottom, dim3((unsigned)wgs), dim3(BOTTOM_MT), 0, c.stream, t, mf, mi, ns,
+                       td.centre, td.core);
+    for (int d = cut_max - 1; d >= MB_MAX_CUT; --d)
+      hipLaunchKernelGGL(k_internal_level, dim3((unsigned)std::max(1, std::min(grid, (cap + 255) / 256))), dim3(256),
+                         0, c.stream, t, mf, d, ns, td.centre, td.core);
+    hipLaunchKernelGGL(k_moments_top, dim3(K), dim3(512), 0, c.stream, t, mf, ns, td.centre, td.core);
+    hipLaunchKernelGGL(k_bound_top, dim3((unsigned)std::min<int64_t>(grid, (m + 255) / 256)), dim3(256), 0, c.stream,
+                       t, mf, mi, td.centre, ns.r2);
+  }
   NodeOutputs no = want_outputs ? td.out : NodeOutputs{};
-  hipLaunchKernelGGL(k_moments_final, dim3(grid), dim3(256), 0, c.stream, t, d_first, d_ncount, d_part, ns, td.centre,
-                     td.core, clearance, no, td.acc, td.far64, td.far32);
+  hipLaunchKernelGGL(k_moments_final, dim3((unsigned)std::max(1, std::min(grid, (cap + 255) / 256))), dim3(256), 0,
+                     c.stream, t, ns, td.centre, td.core, clearance, no, td.acc, td.far64, td.far32);
   OFW_HIP(hipGetLastError());
   return OFW_OK;
 }

@@ -67,7 +67,7 @@ def node_relative(a, b):
     return np.where(scale > 0, diff / np.where(scale > 0, scale, 1.0), diff)
 
 
-@pytest.mark.parametrize("seed,leaf,count", [(1, 8, 300), (2, 32, 300), (3, 1, 120), (4, 16, 20000)])
+@pytest.mark.parametrize("seed,leaf,count", [(1, 8, 300), (2, 32, 300), (3, 1, 120), (4, 16, 20000), (5, 1, 150000)])
 def test_node_quantities(seed, leaf, count):
     s, d, L, g, r = segments(seed, count)
     lv, ends, alpha, mid = O.segment_arrays(s, d, L, g)
