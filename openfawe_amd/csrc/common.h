@@ -273,13 +273,13 @@ __device__ __forceinline__ int pair_fast64(double px, double py, double pz, cons
 }
 
 // FP32 pair for the mixed / fp32 precisions.  rel = p - s rounded from FP64.  Returns true when the pair
-// is near (must be evaluated in FP64, see OFW_NEAR_*) or outside the FP32 range; otherwise adds the FP32
-// velocity to u.  The arithmetic runs unconditionally and the result is selected, so that a wavefront
-// evaluating one segment against 64 targets takes no divergent branch except the rare core factor of a
-// target close to the line beyond an end of the segment.
+// is near (must be evaluated in FP64, see OFW_NEAR_*) or outside the FP32 range; otherwise v is the FP32
+// pair velocity (zero for a segment without circulation).  The arithmetic runs unconditionally, so that a
+// wavefront evaluating one segment against 64 targets takes no divergent branch except the rare core factor
+// of a target close to the line beyond an end of the segment.
 template <int MODEL>
 __device__ __forceinline__ bool pair_f32(float rx, float ry, float rz, const float* __restrict__ f,
-                                         float& ux, float& uy, float& uz) {
+                                         float& vx, float& vy, float& vz) {
   const float invL = f[F_INVL];
   const float dx = f[F_DX], dy = f[F_DY], dz = f[F_DZ];
   const float ax = rx * invL, ay = ry * invL, az = rz * invL;
@@ -331,10 +331,9 @@ __device__ __forceinline__ bool pair_f32(float rx, float ry, float rz, const flo
     den = product * plus;
   }
   const float coef = g4 * factor * (sa + sb) * __builtin_amdgcn_rcpf(den);
-  const bool add = !near && g4 != 0.0f;
-  ux = add ? ux + coef * cx : ux;
-  uy = add ? uy + coef * cy : uy;
-  uz = add ? uz + coef * cz : uz;
+  vx = coef * cx;
+  vy = coef * cy;
+  vz = coef * cz;
   return near;
 }
 

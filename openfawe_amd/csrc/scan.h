@@ -40,6 +40,14 @@ __device__ __forceinline__ bool wave_all(bool v) {
   return __builtin_amdgcn_ballot_w64(v) == __builtin_amdgcn_read_exec();
 }
 
+// LDS written by some lanes of a wavefront and read by others: the wavefront's LDS operations execute in
+// order, so only the compiler must keep them in program order.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Workgroup index remapped so that each XCD (workgroups are dispatched to the 8 XCDs round-robin) receives a
 // contiguous range of logical indices: neighbouring work shares one XCD's L2.  A bijection on [0, nb).
 __device__ __forceinline__ int xcd_block(int b, int nb) {

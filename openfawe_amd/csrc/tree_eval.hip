@@ -42,6 +42,27 @@ __device__ __forceinline__ bool accept_test(double x0, double x1, double x2, con
   return bound < theta * dist && dist - bound >= a[A_CLEAR];
 }
 
+// The acceptance test decided from the squared distance d2 where its answer is certain.  Both conditions are
+// monotone in dist = sqrt(d2), and so is their floating-point evaluation (rounding is monotone): with
+// b2 = (bound / theta)^2 and c2 = (bound + clear)^2, d2 above both by a relative 1e-9 (far more than the
+// rounding of any of these steps) accepts as the reference's operations do, and d2 below either by 1e-9
+// rejects.  Returns 1 (accept), 0 (reject) or -1: d2 within 1e-9 of a threshold or outside the range where
+// these bounds hold, to be decided by accept_test.
+__device__ __forceinline__ int accept_sure(double x0, double x1, double x2, const double* __restrict__ a,
+                                           double inv_theta2) {
+#pragma clang fp contract(off)
+  const double rx = x0 - a[A_CX], ry = x1 - a[A_CY], rz = x2 - a[A_CZ];
+  const double d2 = rx * rx + ry * ry + rz * rz;
+  const double bound = a[A_BOUND];
+  const double b2 = (bound * bound) * inv_theta2;
+  const double sum = bound + a[A_CLEAR];
+  const double c2 = sum * sum;
+  const bool sane = d2 >= 1e-200 && d2 <= 1e200 && b2 <= 1e200 && c2 <= 1e200;
+  const bool yes = d2 > b2 * (1.0 + 1e-9) && d2 > c2 * (1.0 + 1e-9);
+  const bool no = d2 < b2 * (1.0 - 1e-9) || d2 < c2 * (1.0 - 1e-9);
+  return !sane ? -1 : (yes ? 1 : (no ? 0 : -1));
+}
+
 // The same test on a record given as values (centre, bound, clear).
 __device__ __forceinline__ bool accept_test5(double x0, double x1, double x2, const double* rec, double theta,
                                              double& rx, double& ry, double& rz, double& dist) {
@@ -171,11 +192,13 @@ struct ForestTables {
 };
 
 // A near-field work entry: a leaf (its segment range) and the lanes of the wavefront that opened it.
+// An interaction-list entry of a walk wavefront: an opened leaf (its segment range lo < hi) or an accepted
+// node (lo = -1 - node), and the lanes of the wavefront that opened / accepted it.
 struct NearEntry {
   int32_t lo, hi;
   uint32_t mask_lo, mask_hi;
 };
-static constexpr int CHUNK = 4;  // leaves per near-field work item
+static constexpr int CHUNK = 8;  // list entries per work item of the list kernel
 
 // the tree k of a wavefront / target: the last k with table[k] <= v (table[0] = 0 <= v < table[K])
 __device__ __forceinline__ int find_tree(const int32_t* __restrict__ table, int K, int v) {
@@ -188,10 +211,11 @@ __device__ __forceinline__ int find_tree(const int32_t* __restrict__ table, int 
   return lo;
 }
 
-// Node records are read through a per-wavefront window of WIN consecutive pre-order nodes in LDS (the
-// acceptance record and, below FP64, the FP32 far-field record): the walk only moves forward in pre-order,
-// so one batch of coalesced loads serves many visits.  Segments of the near field are staged one per lane
-// in a window of 64 consecutive tree positions and broadcast with readlane.
+// Acceptance records are read through a per-wavefront window of WIN consecutive pre-order nodes in LDS: the
+// walk only moves forward in pre-order, so one batch of coalesced loads serves many visits.  The walk only
+// decides: accepted nodes and opened leaves go to the wavefront's interaction list, whose far fields and
+// pair sums the list kernel evaluates spread over the whole GPU.  Segments of the near field are staged
+// one per lane in a window of 64 consecutive tree positions and broadcast with readlane.
 static constexpr int WIN = 32;
 static constexpr int EVAL_WAVES = 4;
 
@@ -208,9 +232,7 @@ __global__ __launch_bounds__(256) void k_tree_eval(ForestTables ft, const double
                                                    int32_t* __restrict__ pnear, int8_t* __restrict__ pnz,
                                                    NearEntry* __restrict__ list, int32_t* __restrict__ list_count) {
   typedef typename std::conditional<PREC == OFW_FP32, float, double>::type acc_t;
-  constexpr int FARW = PREC == OFW_FP64 ? 1 : WIN * FAR_REC;
   __shared__ double s_acc[EVAL_WAVES][WIN * ACC_REC];
-  __shared__ float s_far[EVAL_WAVES][FARW];
   const int wv = threadIdx.x >> 6;
   const int wave = uniform(xcd_block(blockIdx.x, gridDim.x) * EVAL_WAVES + wv);
   // whole wavefront beyond the work, or a plan that overflowed its capacity (reported by the host)
@@ -266,6 +288,7 @@ __global__ __launch_bounds__(256) void k_tree_eval(ForestTables ft, const double
   const EvalTask& T = tasks[task];
   const int F = uniform(T.node);
   const int nanc = uniform(T.nanc);
+  const double inv_theta2 = 1.0 / (theta * theta);
 
   auto add_far = [&](double rx, double ry, double rz, double dist, int node, const float* f32) {
     if (PREC == OFW_FP64) {
@@ -338,24 +361,6 @@ __global__ __launch_bounds__(256) void k_tree_eval(ForestTables ft, const double
       v[2 * c] = d.x;
       v[2 * c + 1] = d.y;
     }
-    if (PREC != OFW_FP64) {
-      const float* fsrc = fr32 + (size_t)base * FAR_REC;
-      float w[24];
-#pragma unroll
-      for (int c = 0; c < 6; ++c) {
-        const int e = 4 * lane + 256 * c;  // first float of this lane's chunk
-        const bool in = base + e / FAR_REC < ft.node_cap;
-        const float4 d = in ? *reinterpret_cast<const float4*>(fsrc + e) : make_float4(0.f, 0.f, 0.f, 0.f);
-        w[4 * c] = d.x;
-        w[4 * c + 1] = d.y;
-        w[4 * c + 2] = d.z;
-        w[4 * c + 3] = d.w;
-      }
-#pragma unroll
-      for (int c = 0; c < 6; ++c)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) s_far[wv][4 * lane + 256 * c + e] = w[4 * c + e];
-    }
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
       s_acc[wv][2 * lane + 128 * c] = v[2 * c];
@@ -377,15 +382,30 @@ __global__ __launch_bounds__(256) void k_tree_eval(ForestTables ft, const double
     const long long ints0 = __double_as_longlong(a[A_INTS]), ints1 = __double_as_longlong(a[A_INTS2]);
     const int lo = uniform((int)(ints0 & 0xffffffffll)), hi = uniform((int)(ints0 >> 32));
     const int skip = uniform((int)(ints1 & 0xffffffffll));
-    double rx = 0.0, ry = 0.0, rz = 0.0, dist = 1.0;
-    bool acc = false;
+    int sure = 0;
+    if (act) sure = accept_sure(x0, x1, x2, a, inv_theta2);
+    bool acc = sure > 0;
+    if (wave_any(act && sure < 0)) {
+      if (act && sure < 0) {
+        double rx, ry, rz, dist;
+        acc = accept_test(x0, x1, x2, a, theta, rx, ry, rz, dist);
+      }
+    }
     if (act) {
-      acc = accept_test(x0, x1, x2, a, theta, rx, ry, rz, dist);
       advance_to(lo);
       if (acc) acc = !(nextex < hi);
     }
     if (wave_any(acc)) {
-      if (acc) add_far(rx, ry, rz, dist, i, &s_far[wv][(PREC == OFW_FP64) ? 0 : (i - wbase) * FAR_REC]);
+      // the accepted node goes to the wavefront's interaction list with the lanes that accepted it; the list
+      // kernel adds its far field (vortex_tree.py:526-537 for each of them)
+      const unsigned long long amask = __builtin_amdgcn_ballot_w64(acc);
+      if (nlist < lcap) {
+        if (lane == 0) list[lbase + nlist] = NearEntry{-1 - i, 0, (uint32_t)amask, (uint32_t)(amask >> 32)};
+        ++nlist;
+      } else if (lane == 0) {
+        atomicOr(const_cast<int32_t*>(ft.counters) + C_OVERFLOW, 1);
+      }
+      if (acc) ++far;
     }
     const bool open = act && !acc;
     if (skip == i + 1) {
@@ -433,23 +453,36 @@ __global__ __launch_bounds__(256) void k_tree_eval(ForestTables ft, const double
 // (vortex_tree.py:538-557 for each of them).  A grid-stride
 // loop covers the work items, whose count is known on the device only.
 template <int MODEL, int PREC>
-__global__ __launch_bounds__(256) void k_tree_near(ForestTables ft, const double* __restrict__ pts,
+__global__ __launch_bounds__(256, 4) void k_tree_near(ForestTables ft, const double* __restrict__ pts,
                                                    const int32_t* __restrict__ ex_ptr,
                                                    const int32_t* __restrict__ ex_pos,
                                                    const double* __restrict__ rec64, const float* __restrict__ rec32,
+                                                   const double* __restrict__ accr, const double* __restrict__ fr64,
+                                                   const float* __restrict__ fr32, int order,
                                                    int raise_singular, const NearEntry* __restrict__ list,
                                                    const int32_t* __restrict__ list_count,
-                                                   const int32_t* __restrict__ chunk_off, int chunk_slots,
+                                                   const int32_t* __restrict__ chunk_off,
+                                                   const int32_t* __restrict__ item_wave, int chunk_slots,
                                                    double* __restrict__ cpart, int8_t* __restrict__ cflag,
                                                    int32_t* __restrict__ cnear) {
   typedef typename std::conditional<PREC == OFW_FP32, float, double>::type acc_t;
+  typedef typename std::conditional<PREC == OFW_FP64, double, float>::type far_t;
+  // far-field records (and centres) of the accepted nodes of one work item, per wavefront
+  __shared__ far_t s_far[EVAL_WAVES][CHUNK][FAR_REC];
+  __shared__ double s_cen[EVAL_WAVES][CHUNK][4];
+  // below FP64, a window of 64 consecutive segments: FP32 records and FP64 starts
+  constexpr int SEGW = PREC == OFW_FP64 ? 1 : 64;
+  __shared__ float s_seg[EVAL_WAVES][SEGW][REC32];
+  __shared__ double s_start[EVAL_WAVES][SEGW][4];
+  const far_t* __restrict__ frec = PREC == OFW_FP64 ? (const far_t*)fr64 : (const far_t*)fr32;
+  const int wv = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int works = min(live_count(ft.counters, C_CHUNKS), chunk_slots);
   const int nwaves = live_count(ft.counters, C_WORK);
   const int stride = gridDim.x * EVAL_WAVES;
   for (int item = uniform(blockIdx.x * EVAL_WAVES + (int)(threadIdx.x >> 6)); item < works; item += stride) {
-    // the walk wavefront of this item: the last w with chunk_off[w] <= item
-    const int w = uniform(find_tree(chunk_off, nwaves, item));
+    // the walk wavefront of this item
+    const int w = uniform(item_wave[item]);
     const int c = item - uniform(chunk_off[w]);
     const int k = uniform(find_tree(ft.wave_off, ft.K, w));
     const int t0 = uniform(ft.toff[k]);
@@ -493,16 +526,55 @@ __global__ __launch_bounds__(256) void k_tree_near(ForestTables ft, const double
     acc_t ux = 0, uy = 0, uz = 0;
     int flag = OFW_FLAG_OK, near = 0;
     int sbase = INT_MIN / 2;
-    float4 fa = make_float4(0.f, 0.f, 0.f, 0.f), fb = make_float4(0.f, 0.f, 0.f, 0.f);
-    double qx = 0.0, qy = 0.0, qz = 0.0;
     const size_t lbase = (size_t)uniform(ft.list_base[k]) + (size_t)local * uniform(ft.list_cap[k]);
-    const int e_end = min(uniform(list_count[w]), (c + 1) * CHUNK);
-    for (int e = c * CHUNK; e < e_end; ++e) {
-      const NearEntry en = list[lbase + e];
-      const int lo = uniform(en.lo), hi = uniform(en.hi);
+    const int e0 = c * CHUNK;
+    const int ne = min(uniform(list_count[w]), e0 + CHUNK) - e0;
+    // the item's entries, one per lane, and the records of its accepted nodes staged in LDS together (lane
+    // 8 q + u copies part u of entry q's record), so that no entry waits for its own loads
+    NearEntry mine{0, 0, 0u, 0u};
+    if (lane < ne) mine = list[lbase + e0 + lane];
+    wave_lds_sync();  // the previous item's reads of the stage come first
+    {
+      const int q = lane >> 3, u = lane & 7;
+      const int lo_q = __shfl(mine.lo, q, 64);
+      if (q < ne && lo_q < 0) {
+        const int node = -1 - lo_q;
+        const far_t* src = frec + (size_t)node * FAR_REC;
+#pragma unroll
+        for (int v = u; v < FAR_REC; v += 8) s_far[wv][q][v] = src[v];
+        if (u < 3) s_cen[wv][q][u] = accr[(size_t)node * ACC_REC + A_CX + u];
+      }
+    }
+    wave_lds_sync();
+    for (int e = 0; e < ne; ++e) {
+      const int lo = __builtin_amdgcn_readlane(mine.lo, e), hi = __builtin_amdgcn_readlane(mine.hi, e);
       const unsigned long long mask =
-          ((unsigned long long)(uint32_t)uniform((int)en.mask_hi) << 32) | (uint32_t)uniform((int)en.mask_lo);
+          ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)mine.mask_hi, e) << 32) |
+          (uint32_t)__builtin_amdgcn_readlane((int)mine.mask_lo, e);
       const bool open = valid && ((mask >> lane) & 1ull);
+      if (lo < 0) {
+        // an accepted node: its far field for the lanes that accepted it, as the walk's acceptance test formed
+        // R and its length (vortex_tree.py:526-537)
+        if (open) {
+#pragma clang fp contract(off)
+          const double rx = x0 - s_cen[wv][e][0], ry = x1 - s_cen[wv][e][1], rz = x2 - s_cen[wv][e][2];
+          if (PREC == OFW_FP64) {
+            const double dist = __builtin_sqrt(rx * rx + ry * ry + rz * rz);
+            double vx = 0.0, vy = 0.0, vz = 0.0;
+            far64(rx, ry, rz, dist, (const double*)&s_far[wv][e][0], order, vx, vy, vz);
+            ux += vx;
+            uy += vy;
+            uz += vz;
+          } else {
+            float vx, vy, vz;
+            far32((float)rx, (float)ry, (float)rz, (const float*)&s_far[wv][e][0], order, vx, vy, vz);
+            ux += (acc_t)vx;
+            uy += (acc_t)vy;
+            uz += (acc_t)vz;
+          }
+        }
+        continue;
+      }
       if (open) advance_to(lo);
       if (PREC == OFW_FP64) {
         // FP64: the segment record is wave-uniform (scalar loads); an excluded pair is evaluated, not added
@@ -526,15 +598,20 @@ __global__ __launch_bounds__(256) void k_tree_near(ForestTables ft, const double
         if (base < sbase || base >= sbase + 64) {
           sbase = base;
           const int s = base + lane;
+          wave_lds_sync();  // the previous window's reads come first
           if (s < ft.seg_count) {
             const float4* f4 = reinterpret_cast<const float4*>(rec32 + (size_t)s * REC32);
-            fa = f4[0];
-            fb = f4[1];
             const double* r = rec64 + (size_t)s * REC64;
-            qx = r[R_SX];
-            qy = r[R_SY];
-            qz = r[R_SZ];
+            const float4 fa = f4[0], fb = f4[1];
+            const double qx = r[R_SX], qy = r[R_SY], qz = r[R_SZ];
+            float4* d4 = reinterpret_cast<float4*>(&s_seg[wv][lane][0]);
+            d4[0] = fa;
+            d4[1] = fb;
+            s_start[wv][lane][0] = qx;
+            s_start[wv][lane][1] = qy;
+            s_start[wv][lane][2] = qz;
           }
+          wave_lds_sync();
         }
         const int stop = min(hi, sbase + 64);
         for (int s = base; s < stop; ++s) {
@@ -546,16 +623,15 @@ __global__ __launch_bounds__(256) void k_tree_near(ForestTables ft, const double
             if (hit) advance_to(s + 1);
           }
           const bool use = open && !hit;
-          const float f[REC32] = {bcast(fa.x, tl), bcast(fa.y, tl), bcast(fa.z, tl), bcast(fa.w, tl),
-                                  bcast(fb.x, tl), bcast(fb.y, tl), bcast(fb.z, tl), 0.0f};
-          const double bx = bcast(qx, tl), by = bcast(qy, tl), bz = bcast(qz, tl);
-          const float sx = (float)(x0 - bx), sy = (float)(x1 - by), sz = (float)(x2 - bz);
-          float vx = 0.0f, vy = 0.0f, vz = 0.0f;
-          const bool need64 = pair_f32<MODEL>(sx, sy, sz, f, vx, vy, vz);
+          // the segment's record and start: the same LDS address on every lane (a broadcast read)
+          const float* f = &s_seg[wv][tl][0];
+          const double sx = x0 - s_start[wv][tl][0], sy = x1 - s_start[wv][tl][1], sz = x2 - s_start[wv][tl][2];
+          float vx, vy, vz;
+          const bool need64 = pair_f32<MODEL>((float)sx, (float)sy, (float)sz, f, vx, vy, vz);
           const bool add = use && !need64;
-          ux = add ? ux + (acc_t)vx : ux;
-          uy = add ? uy + (acc_t)vy : uy;
-          uz = add ? uz + (acc_t)vz : uz;
+          ux += (acc_t)(add ? vx : 0.0f);
+          uy += (acc_t)(add ? vy : 0.0f);
+          uz += (acc_t)(add ? vz : 0.0f);
           near += use ? 1 : 0;
           const bool slow = use && need64;
           if (wave_any(slow)) {
@@ -584,6 +660,14 @@ __global__ __launch_bounds__(256) void k_tree_near(ForestTables ft, const double
       if (cnear) cnear[o] = near;
     }
   }
+}
+
+// the walk wavefront of every work item of the list kernel (within the partial slots)
+__global__ void k_item_waves(const int32_t* __restrict__ counters, const int32_t* __restrict__ chunk_off, int slots,
+                             int32_t* __restrict__ item_wave) {
+  const int nwaves = live_count(counters, C_WORK);
+  for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < nwaves; w += gridDim.x * blockDim.x)
+    for (int it = chunk_off[w]; it < min(chunk_off[w + 1], slots); ++it) item_wave[it] = w;
 }
 
 // a fixed number of near-field partial slots that the work items exceed is reported as an overflow
@@ -785,18 +869,21 @@ static void launch_eval(int precision, dim3 grid, hipStream_t st, const ForestTa
 
 template <int MODEL>
 static void launch_near(int precision, dim3 grid, hipStream_t st, const ForestTables& ft, const double* pts,
-                        const int32_t* ex_ptr, const int32_t* ex_pos, const TreeDevice& td, int raise,
-                        const NearEntry* list, const int32_t* list_count, const int32_t* chunk_off, int slots,
-                        double* cpart, int8_t* cflag, int32_t* cnear) {
+                        const int32_t* ex_ptr, const int32_t* ex_pos, const TreeDevice& td, int order, int raise,
+                        const NearEntry* list, const int32_t* list_count, const int32_t* chunk_off,
+                        const int32_t* item_wave, int slots, double* cpart, int8_t* cflag, int32_t* cnear) {
   if (precision == OFW_FP64)
     hipLaunchKernelGGL((k_tree_near<MODEL, OFW_FP64>), grid, dim3(256), 0, st, ft, pts, ex_ptr, ex_pos, td.rec64,
-                       td.rec32, raise, list, list_count, chunk_off, slots, cpart, cflag, cnear);
+                       td.rec32, td.acc, td.far64, td.far32, order, raise, list, list_count, chunk_off, item_wave, slots, cpart,
+                       cflag, cnear);
   else if (precision == OFW_MIXED)
     hipLaunchKernelGGL((k_tree_near<MODEL, OFW_MIXED>), grid, dim3(256), 0, st, ft, pts, ex_ptr, ex_pos, td.rec64,
-                       td.rec32, raise, list, list_count, chunk_off, slots, cpart, cflag, cnear);
+                       td.rec32, td.acc, td.far64, td.far32, order, raise, list, list_count, chunk_off, item_wave, slots, cpart,
+                       cflag, cnear);
   else
     hipLaunchKernelGGL((k_tree_near<MODEL, OFW_FP32>), grid, dim3(256), 0, st, ft, pts, ex_ptr, ex_pos, td.rec64,
-                       td.rec32, raise, list, list_count, chunk_off, slots, cpart, cflag, cnear);
+                       td.rec32, td.acc, td.far64, td.far32, order, raise, list, list_count, chunk_off, item_wave, slots, cpart,
+                       cflag, cnear);
 }
 
 int choose_split_depth(const Ctx& c, int64_t groups, int K) {
@@ -829,7 +916,9 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
     wave_cap += waves_k;
     part_cap += tasks_k * nk;
     const int64_t sub = (mk + ((int64_t)1 << D) - 1) >> D;
-    list_cap[k] = lists && mk > 0 ? sub / min_leaf + 2 : 0;
+    // per visited node one entry (accepted or opened), a leaf two (accepted by some lanes, opened by others):
+    // at most 3 x the subtree's leaves
+    list_cap[k] = lists && mk > 0 ? 3 * (sub / min_leaf + 2) : 0;
     list_base[k] = entry_cap;
     entry_cap += waves_k * list_cap[k];
     chunk_cap += waves_k * ((list_cap[k] + CHUNK - 1) / CHUNK);
@@ -939,22 +1028,27 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
       c.stats_i[6] = count;
     }
     const size_t CH = (size_t)std::max<int64_t>(slots, 1);
-    OFW_HIP(c.tree[5].reserve(al(CH * 64 * 24) + al(CH * 64) + (counts ? al(CH * 64 * 4) : 0)));
+    OFW_HIP(c.tree[5].reserve(al(CH * 64 * 24) + al(CH * 64) + al(CH * 4) + (counts ? al(CH * 64 * 4) : 0)));
+    int32_t* item_wave;
     {
       char* cb = (char*)c.tree[5].ptr;
       cpart = (double*)cb;
       cb += al(CH * 64 * 24);
       cflag = (int8_t*)cb;
       cb += al(CH * 64);
+      item_wave = (int32_t*)cb;
+      cb += al(CH * 4);
       cnear = counts ? (int32_t*)cb : nullptr;
     }
     hipLaunchKernelGGL(k_check_slots, dim3(1), dim3(64), 0, c.stream, t.counters, (int)std::min<int64_t>(slots, INT_MAX));
+    hipLaunchKernelGGL(k_item_waves, dim3((unsigned)((W1 + 255) / 256)), dim3(256), 0, c.stream, t.counters, chunk_off,
+                       (int)CH, item_wave);
     OFW_HIP(c.mark());
     const dim3 ngrid((unsigned)std::max<int64_t>(1, std::min<int64_t>((int64_t)c.cu_count * 16, (CH + 3) / 4)));
     switch (model) {
-      case 0: launch_near<0>(precision, ngrid, c.stream, ft, points, ex_ptr_used, ex_pos, td, raise_singular, list, list_count, chunk_off, (int)CH, cpart, cflag, cnear); break;
-      case 1: launch_near<1>(precision, ngrid, c.stream, ft, points, ex_ptr_used, ex_pos, td, raise_singular, list, list_count, chunk_off, (int)CH, cpart, cflag, cnear); break;
-      default: launch_near<2>(precision, ngrid, c.stream, ft, points, ex_ptr_used, ex_pos, td, raise_singular, list, list_count, chunk_off, (int)CH, cpart, cflag, cnear); break;
+      case 0: launch_near<0>(precision, ngrid, c.stream, ft, points, ex_ptr_used, ex_pos, td, expansion_order, raise_singular, list, list_count, chunk_off, item_wave, (int)CH, cpart, cflag, cnear); break;
+      case 1: launch_near<1>(precision, ngrid, c.stream, ft, points, ex_ptr_used, ex_pos, td, expansion_order, raise_singular, list, list_count, chunk_off, item_wave, (int)CH, cpart, cflag, cnear); break;
+      default: launch_near<2>(precision, ngrid, c.stream, ft, points, ex_ptr_used, ex_pos, td, expansion_order, raise_singular, list, list_count, chunk_off, item_wave, (int)CH, cpart, cflag, cnear); break;
     }
     OFW_HIP(hipGetLastError());
   }

@@ -184,14 +184,6 @@ __device__ __forceinline__ void combine_children(NodeRef v, int par, int left, i
   v.core[par] = cr > cl ? cr : cl;
 }
 
-// LDS written by some lanes of a wavefront and read by others: the wavefront's LDS operations execute in
-// order, so only the compiler must keep them in program order.
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 __device__ __forceinline__ double half_max(double v) {
 #pragma unroll
   for (int off = 16; off >= 1; off >>= 1) {
