@@ -37,6 +37,7 @@ static constexpr int BOTTOM_THREADS = 512;
 static constexpr int BOTTOM_E = S_BOTTOM / BOTTOM_THREADS;  // elements per thread of the subtree sort
 static constexpr int HCAP = 2 * S_BOTTOM - 1;
 static constexpr int PLAN_THREADS = 1024;
+static constexpr int SUBS = 8;              // workgroups per tile of the elementwise top-phase kernels
 
 __device__ __forceinline__ uint64_t ordered_key(double x) {
   uint64_t u = (x == 0.0) ? 0ull : (uint64_t)__double_as_longlong(x);
@@ -185,12 +186,21 @@ __device__ __forceinline__ void choose_axis(const double* v, int8_t* axis_out, i
   *split_out = (int8_t)(extent > 0.0 ? 1 : 0);
 }
 
+// Part `sub` of SUBS equal parts of the tile range [p0, p1).
+__device__ __forceinline__ void sub_range(int p0, int p1, int sub, int& a, int& b) {
+  const int chunk = (p1 - p0 + SUBS - 1) / SUBS;
+  a = min(p1, p0 + sub * chunk);
+  b = min(p1, a + chunk);
+}
+
 __global__ __launch_bounds__(256) void k_box_tiles(TreeArrays t, const double* __restrict__ mx,
                                                    const double* __restrict__ my, const double* __restrict__ mz) {
   __shared__ double red[4][6];
   const int ntiles = live_count(t.counters, C_TILES);
-  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int p0 = t.tile_p0[tile], p1 = t.tile_p1[tile];
+  for (int wu = blockIdx.x; wu < ntiles * SUBS; wu += gridDim.x) {
+    const int tile = wu / SUBS;
+    int p0, p1;
+    sub_range(t.tile_p0[tile], t.tile_p1[tile], wu % SUBS, p0, p1);
     double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
     for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
       const int s = t.order[p];
@@ -212,26 +222,29 @@ __global__ __launch_bounds__(256) void k_box_tiles(TreeArrays t, const double* _
           v[q] = red[k][q] < v[q] ? red[k][q] : v[q];
           v[3 + q] = red[k][3 + q] > v[3 + q] ? red[k][3 + q] : v[3 + q];
         }
-      for (int q = 0; q < 6; ++q) t.box_partial[6 * (size_t)tile + q] = v[q];
+      for (int q = 0; q < 6; ++q) t.box_partial[6 * (size_t)wu + q] = v[q];
     }
     __syncthreads();
   }
 }
 
-__global__ void k_box_combine(TreeArrays t) {
+// One wavefront per big node: its box from the partial boxes of its tiles (exact min / max, in any order).
+__global__ __launch_bounds__(256) void k_box_combine(TreeArrays t) {
   const int nbig = live_count(t.counters, C_BIG);
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nbig; i += gridDim.x * blockDim.x) {
+  const int lane = threadIdx.x & 63;
+  for (int i = blockIdx.x * 4 + (int)(threadIdx.x >> 6); i < nbig; i += gridDim.x * 4) {
     const int node = t.big[i];
     const int t0 = t.big_t0[i];
     const int tiles = (t.hi[node] - t.lo[node] + TILE - 1) / TILE;
     double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-    for (int q = t0; q < t0 + tiles; ++q)
+    for (int q = t0 * SUBS + lane; q < (t0 + tiles) * SUBS; q += 64)
       for (int c = 0; c < 3; ++c) {
         const double lo = t.box_partial[6 * (size_t)q + c], hi = t.box_partial[6 * (size_t)q + 3 + c];
         v[c] = lo < v[c] ? lo : v[c];
         v[3 + c] = hi > v[3 + c] ? hi : v[3 + c];
       }
-    choose_axis(v, &t.axis[node], &t.split[node]);
+    box_wave_reduce(v);
+    if (lane == 0) choose_axis(v, &t.axis[node], &t.split[node]);
   }
 }
 
@@ -507,18 +520,22 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_merge_pass(TreeArrays t, int 
 
 __global__ __launch_bounds__(256) void k_apply_perm(TreeArrays t, const uint32_t* __restrict__ sorted_pos) {
   const int ntiles = live_count(t.counters, C_TILES);
-  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  for (int wu = blockIdx.x; wu < ntiles * SUBS; wu += gridDim.x) {
+    const int tile = wu / SUBS;
     if (!t.split[t.tile_node[tile]]) continue;
-    const int p0 = t.tile_p0[tile], p1 = t.tile_p1[tile];
+    int p0, p1;
+    sub_range(t.tile_p0[tile], t.tile_p1[tile], wu % SUBS, p0, p1);
     for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) t.tmp[p] = t.order[sorted_pos[p]];
   }
 }
 
 __global__ __launch_bounds__(256) void k_copy_tiles(TreeArrays t) {
   const int ntiles = live_count(t.counters, C_TILES);
-  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  for (int wu = blockIdx.x; wu < ntiles * SUBS; wu += gridDim.x) {
+    const int tile = wu / SUBS;
     if (!t.split[t.tile_node[tile]]) continue;
-    const int p0 = t.tile_p0[tile], p1 = t.tile_p1[tile];
+    int p0, p1;
+    sub_range(t.tile_p0[tile], t.tile_p1[tile], wu % SUBS, p0, p1);
     for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) t.order[p] = t.tmp[p];
   }
 }
@@ -875,7 +892,7 @@ int tree_layout(Ctx& c, int buf, int64_t m64, int K, int leaf, TreeArrays& t) {
   need += al((K + 1) * 4) + al(M * 4);                                    // roots_lo, order
   need += 7 * al(C * 4) + 2 * al(C);                                      // node table, axis, split
   need += al(C_COUNT * 4) + 5 * al(C * 4);                                // counters, levels, big, big_t0, flags
-  need += 3 * al((size_t)t.tile_cap * 4) + al((size_t)t.tile_cap * 48);   // tiles, box partials
+  need += 3 * al((size_t)t.tile_cap * 4) + al((size_t)t.tile_cap * SUBS * 48);   // tiles, box partials
   need += 2 * al(M * 8) + 2 * al(M * 4) + al(std::max(M, C) * 4);         // sort buffers, tmp
   need += 2 * al((size_t)t.bottom_cap * 4);                               // bottom, heap base
   need += 3 * al((M + 1 + C) * 4) + al(((M + C) / 4096 + 8) * 4);         // lo_cnt, lo_mind, lo_base, parts
@@ -907,7 +924,7 @@ int tree_layout(Ctx& c, int buf, int64_t m64, int K, int leaf, TreeArrays& t) {
   t.tile_node = (int32_t*)carve((size_t)t.tile_cap * 4);
   t.tile_p0 = (int32_t*)carve((size_t)t.tile_cap * 4);
   t.tile_p1 = (int32_t*)carve((size_t)t.tile_cap * 4);
-  t.box_partial = (double*)carve((size_t)t.tile_cap * 48);
+  t.box_partial = (double*)carve((size_t)t.tile_cap * SUBS * 48);
   t.keyA = (uint64_t*)carve(M * 8);
   t.keyB = (uint64_t*)carve(M * 8);
   t.posA = (uint32_t*)carve(M * 4);
@@ -950,7 +967,7 @@ int tree_build(Ctx& c, TreeArrays& t, const double* mx, const double* my, const 
     if (biggest <= S_BOTTOM || biggest <= t.leaf) break;  // the plan above sent every node to the bottom
     hipLaunchKernelGGL(k_box_tiles, dim3(grid), dim3(256), 0, c.stream, t, mx, my, mz);
     const int64_t nodes_bound = std::min<int64_t>((int64_t)t.K << std::min(lvl, 30), t.cap);
-    hipLaunchKernelGGL(k_box_combine, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(grid, (nodes_bound + 255) / 256))),
+    hipLaunchKernelGGL(k_box_combine, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(grid, (nodes_bound + 3) / 4))),
                        dim3(256), 0, c.stream, t);
     hipLaunchKernelGGL(k_sort_tiles, dim3(grid), dim3(TILE_THREADS), 0, c.stream, t, mx, my, mz);
     uint64_t *kin = t.keyA, *kout = t.keyB;
