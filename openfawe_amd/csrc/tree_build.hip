@@ -33,7 +33,7 @@ static constexpr int TILE = 8192;           // elements per LDS tile of the top 
 static constexpr int TILE_THREADS = 1024;
 static constexpr int TILE_E = TILE / TILE_THREADS;  // elements per thread of the tile sort
 static constexpr int S_BOTTOM = 2048;       // largest subtree finished in LDS
-static constexpr int BOTTOM_THREADS = 512;
+static constexpr int BOTTOM_THREADS = 1024;
 static constexpr int BOTTOM_E = S_BOTTOM / BOTTOM_THREADS;  // elements per thread of the subtree sort
 static constexpr int HCAP = 2 * S_BOTTOM - 1;
 static constexpr int PLAN_THREADS = 1024;
@@ -384,7 +384,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_sort_tiles(TreeArrays t, const
 // wavefront each) give the slice's input ranges of both runs, which are staged in LDS; each thread finds its
 // own start in LDS, merges MERGE_E outputs into registers, and the slice leaves through LDS with coalesced
 // stores.
-static constexpr int MERGE_THREADS = 256;
+static constexpr int MERGE_THREADS = 512;
 static constexpr int MERGE_E = 8;                                // outputs per thread
 static constexpr int MERGE_OUT = MERGE_THREADS * MERGE_E;        // outputs per work item
 static constexpr int MERGE_SLICES = TILE / MERGE_OUT;            // work items per sort tile
