@@ -92,7 +92,7 @@ def timings(ctx, count=8):
     phases = load().ofw_ctx_timings(ctx, buf, count)
     if phases < 0:
         check(phases, "ofw_ctx_timings")
-    return [float(buf[k]) for k in range(phases)]
+    return [float(buf[k]) for k in range(min(phases, count))]
 
 
 _contexts = {}

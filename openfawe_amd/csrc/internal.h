@@ -31,6 +31,7 @@ struct Ctx {
   hipEvent_t ev[12] = {};
   int nev = 0;               // events recorded by the last call (phase k = ev[k] -> ev[k+1])
   int64_t stats_i[8] = {};
+  int64_t list_slots = 0;    // interaction-list work-item slots of the next tree call (0: measure)
   bool capturing = false;    // enqueuing into a graph capture: no events are recorded
   hipError_t mark() { return (!capturing && nev < 12) ? hipEventRecord(ev[nev++], stream) : hipSuccess; }
 };

@@ -116,10 +116,8 @@ static int tree_pipeline(Ctx& c, const TreeCall& tc, TreeArrays& t) {
   return OFW_OK;
 }
 
-// After the stream has finished: the build's counters, and an error when a capacity was exceeded.
-static int read_counters(Ctx& c, const TreeArrays& t, int32_t* counters) {
-  OFW_HIP(hipMemcpyAsync(counters, t.counters, C_COUNT * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
-  OFW_HIP(hipStreamSynchronize(c.stream));
+// The counters of a finished call: an error when a capacity was exceeded, else the statistics.
+static int check_counters(Ctx& c, const int32_t* counters) {
   if (counters[C_OVERFLOW]) {
     set_error("tree code exceeded a capacity of its build or traversal plan");
     return OFW_ERR_OVERFLOW;
@@ -128,6 +126,13 @@ static int read_counters(Ctx& c, const TreeArrays& t, int32_t* counters) {
   c.stats_i[4] = counters[C_FINAL];
   c.stats_i[5] = counters[C_DEPTH];
   return OFW_OK;
+}
+
+// After the stream has finished: the build's counters, and an error when a capacity was exceeded.
+static int read_counters(Ctx& c, const TreeArrays& t, int32_t* counters) {
+  OFW_HIP(hipMemcpyAsync(counters, t.counters, C_COUNT * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+  OFW_HIP(hipStreamSynchronize(c.stream));
+  return check_counters(c, counters);
 }
 
 static int check_tree_args(const char* what, int precision, int64_t n, int64_t m, int64_t leaf_size, double theta,
@@ -340,11 +345,24 @@ int ofw_tree_sum(ofw_ctx* x, int precision, int64_t n, const double* points, int
   tc.flags = flags;
   tc.far_counts = far_counts;
   tc.near_counts = near_counts;
+  // the interaction-list slots of the previous call, with margin, spare this call's read-back of the count;
+  // a call that needs more runs again with the count it measured
   ofw::TreeArrays t;
-  st = ofw::tree_pipeline(c, tc, t);
-  if (st) return st;
   int32_t counters[ofw::C_COUNT];
-  return ofw::read_counters(c, t, counters);
+  for (int attempt = 0;; ++attempt) {
+    c.nev = 0;  // the phase timings describe the last attempt
+    tc.chunk_slots = c.list_slots > 0 ? c.list_slots : -1;
+    st = ofw::tree_pipeline(c, tc, t);
+    if (st) return st;
+    OFW_HIP(hipMemcpyAsync(counters, t.counters, sizeof(counters), hipMemcpyDeviceToHost, c.stream));
+    OFW_HIP(hipStreamSynchronize(c.stream));
+    const int64_t used = counters[ofw::C_CHUNKS];
+    const bool short_slots = counters[ofw::C_OVERFLOW] && tc.chunk_slots >= 0 && used > tc.chunk_slots;
+    c.list_slots = used + used / 4 + 1024;
+    if (!short_slots || attempt > 0) break;
+  }
+  c.stats_i[6] = counters[ofw::C_CHUNKS];
+  return ofw::check_counters(c, counters);
 }
 
 int ofw_tree_batch_create(ofw_ctx* x, ofw_tree_batch** plan, int precision, int64_t K,
