@@ -244,12 +244,14 @@ __device__ __forceinline__ int pair_fast64(double px, double py, double pz, cons
     coef = (((g4 * factor) * (sa + sb)) * xm) * ofw_rcp(dd);
   }
   bool add = use && !slow && !axis && g4 != 0.0;
+  const bool finite = isfinite(coef);
   int flag = OFW_FLAG_OK;
-  if (use && !slow && axis && raise_singular && t >= 0.0 && t <= 1.0) flag = OFW_FLAG_SINGULAR;
-  if (add && !isfinite(coef)) {
-    add = false;
-    flag = OFW_FLAG_NONFINITE;
+  // flags are rare (a target on a segment's line, a non-finite coefficient): worked out on the ballot only
+  if (__builtin_amdgcn_ballot_w64((use && !slow && axis) || (add && !finite)) != 0) {
+    if (use && !slow && axis && raise_singular && t >= 0.0 && t <= 1.0) flag = OFW_FLAG_SINGULAR;
+    if (add && !finite) flag = OFW_FLAG_NONFINITE;
   }
+  add = add && finite;
   // the pair velocity is rounded before it is added, as in the reference's ordered sum
   ux = add ? ux + coef * cx : ux;
   uy = add ? uy + coef * cy : uy;

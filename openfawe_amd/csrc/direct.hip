@@ -133,8 +133,10 @@ __global__ __launch_bounds__(256) void k_direct64(int n, const double* __restric
     }
     const int fa = pair_fast64<MODEL>(ax, ay, az, r, raise_singular, act_a && !hit_a, uax, uay, uaz);
     const int fb = pair_fast64<MODEL>(bx, by, bz, r, raise_singular, act_b && !hit_b, ubx, uby, ubz);
-    note_flag(sta, fa, s);
-    note_flag(stb, fb, s);
+    if (__builtin_amdgcn_ballot_w64(fa != OFW_FLAG_OK || fb != OFW_FLAG_OK) != 0) {
+      note_flag(sta, fa, s);
+      note_flag(stb, fb, s);
+    }
   }
   if (act_a) {
     const size_t o = (size_t)c * n + pa;
