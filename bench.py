@@ -161,6 +161,33 @@ def adjoint_bench(steps):
                 adjoint_fwd_ratio=fused / fwd, influence_jacobian_ms=jms, jacobian_shape=[n, n, 3])
 
 
+def network_bench(case, steps, cpu=True):
+    """Incident-exclusion pairs of the whole wake (the rotor-ring call, first = 0, network_numba.py:242-253) on
+    the GPU with nodes and edges resident in HBM, and the oracle (NumPy, one core) on the same input."""
+    import torch
+    from openfawe_amd import network as NET
+    nodes, edges = case["nodes"], case["edges"]
+    dn = torch.from_numpy(np.ascontiguousarray(nodes)).cuda()
+    de = torch.from_numpy(np.ascontiguousarray(edges)).cuda()
+    pairs = NET.incident_pairs_device(dn, de, 0)
+    cap = int(pairs.shape[0])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        NET.incident_pairs_device(dn, de, 0, capacity=cap)
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    result = dict(call="incident_pairs(nodes, edges, first=0)", nodes=len(nodes), segments=len(edges),
+                  pairs=cap, ms_per_call=ms, note="includes the one stream synchronisation that reads the count")
+    if cpu:
+        from oracle import network_np as NP
+        t0 = time.perf_counter()
+        NP.incident_pairs(nodes, edges, 0)
+        result["cpu_oracle_ms"] = 1e3 * (time.perf_counter() - t0)
+        result["cpu_oracle"] = "oracle/network_np.py incident_pairs (NumPy sort and search, one core)"
+    return result
+
+
 def traffic(workload, kernel):
     """HBM bytes per launch of the kernel from the committed PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by
     the copy-kernel calibration), or None when none was measured for this workload."""
@@ -258,7 +285,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (PMC byte-counter calibration)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-extra", action="store_true", help="skip the adjoint and batch legs")
+    ap.add_argument("--no-extra", action="store_true", help="skip the adjoint, batch and network legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this build "
@@ -359,6 +386,8 @@ def main():
     if not args.no_extra and spec["kind"] == "tree":
         result["adjoint"] = adjoint_bench(max(3, args.steps // 2))
         result["batch"] = batch_bench(max(3, args.steps // 2))
+        if "nodes" in case:
+            result["network"] = network_bench(case, max(3, args.steps // 2), cpu=not args.no_cpu)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(case, spec, usable_cpus())
     print(json.dumps(result), flush=True)

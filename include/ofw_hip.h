@@ -173,6 +173,18 @@ int ofw_direct_vjp(ofw_ctx *ctx, int64_t n, const double *points, int64_t m, con
                    const int64_t *ex_seg, const double *grad_out, double *grad_gamma, double *grad_starts,
                    double *grad_ends, double *grad_points, double *out);
 
+/* Incident-exclusion pairs of a wake network (replaces network_numba.incident_pairs, network_numba.py:242-253,
+ * and the loop of wake.py:427-438): for the free points nodes[first:] (n,3), every (point, segment) pair in
+ * which the point's coordinates equal an end of segment s (edges (m,2)) bit for bit, in segment order, the
+ * start end before the end end, and for one end the coinciding points in increasing order; a node with a
+ * NaN coordinate never matches.  Point indices count from `first`.  Sets *count_out (host) to the number of
+ * pairs and, when it is at most capacity, writes them to pairs (capacity,2) int64 (device); returns
+ * OFW_ERR_OVERFLOW when count > capacity (nothing written: call again with room for *count_out pairs) and
+ * OFW_ERR_ARG when an edge names a node outside [0, n), n or m exceeds 2^29 or the count exceeds 2^31 - 1.
+ * Synchronises the stream once (to read the count). */
+int ofw_incident_pairs(ofw_ctx *ctx, int64_t n, const double *nodes, int64_t m, const int64_t *edges, int64_t first,
+                       int64_t *pairs, int64_t capacity, int64_t *count_out);
+
 #ifdef __cplusplus
 }
 #endif

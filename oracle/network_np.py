@@ -1,0 +1,70 @@
+"""NumPy restatement of the reference's incident-exclusion pairs.  TEST INFRASTRUCTURE ONLY.
+
+``incident_pairs`` (network_numba.py:242-253, with ``_incident_pairs_sorted`` :177-239 and the dictionary
+form ``_incident_pairs`` :121-167 that it reproduces element for element) lists, for the free points
+``nodes[first:]`` of a wake network, every (point, segment) pair in which the point coincides with an end of
+the segment.  Coordinates match by bit pattern (-0.0 and 0.0 differ, network_numba.py:16-21), a node with a
+NaN coordinate never matches, and the pairs come in segment order, the start end before the end end, and for
+one end the coinciding points in increasing order (:229-239).  Point indices count from ``first``.
+
+``incident_pairs_loop`` follows the reference's loops with a dictionary keyed by the bit patterns;
+``incident_pairs`` is a vectorised form of the same rule, checked against the loop form by the tests.
+"""
+import numpy as np
+
+
+def _check(nodes, edges, first):
+    nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+    edges = np.ascontiguousarray(edges, dtype=np.int64)
+    if nodes.ndim != 2 or nodes.shape[1] != 3 or edges.ndim != 2 or edges.shape[1] != 2:
+        raise ValueError("nodes must be (n, 3) and edges (e, 2)")
+    if not 0 <= int(first) <= len(nodes) or (edges.size and (edges.min() < 0 or edges.max() >= len(nodes))):
+        raise ValueError("first or an edge index is out of range")
+    return nodes, edges, int(first)
+
+
+def incident_pairs_loop(nodes, edges, first):
+    """network_numba.py:121-167 (dictionary form), keyed by the coordinates' bit patterns."""
+    nodes, edges, first = _check(nodes, edges, first)
+    bits = nodes.view(np.int64)
+    nan = np.isnan(nodes).any(axis=1)
+    groups = {}
+    for point in range(first, len(nodes)):
+        if not nan[point]:
+            groups.setdefault(tuple(bits[point]), []).append(point - first)
+    pairs = []
+    for segment, (a, b) in enumerate(edges):
+        for node in (a, b):
+            if not nan[node]:
+                pairs.extend((member, segment) for member in groups.get(tuple(bits[node]), ()))
+    return np.asarray(pairs, dtype=np.int64).reshape(-1, 2)
+
+
+def incident_pairs(nodes, edges, first):
+    """The same pairs, vectorised: groups of equal bit patterns by a sort, segment ends matched by search."""
+    nodes, edges, first = _check(nodes, edges, first)
+    bits = np.ascontiguousarray(nodes.view(np.int64))
+    nan = np.isnan(nodes).any(axis=1)
+    free = np.arange(first, len(nodes))
+    free = free[~nan[free]]
+    if len(free) == 0 or len(edges) == 0:
+        return np.zeros((0, 2), dtype=np.int64)
+    key = bits.view([("x", "<i8"), ("y", "<i8"), ("z", "<i8")]).reshape(-1)
+    uniq, group = np.unique(key[free], return_inverse=True)
+    # members of each group in increasing point order (a stable sort keeps the order of `free`)
+    by_group = np.argsort(group, kind="stable")
+    members = free[by_group] - first
+    start = np.searchsorted(group[by_group], np.arange(len(uniq)))
+    size = np.bincount(group, minlength=len(uniq))
+    ends = edges.reshape(-1)                                   # segment order, start end first
+    pos = np.searchsorted(uniq, key[ends])
+    hit = (pos < len(uniq)) & ~nan[ends]
+    hit[hit] = uniq[pos[hit]] == key[ends[hit]]
+    g = np.where(hit, pos, 0)
+    counts = np.where(hit, size[g], 0)
+    total = int(counts.sum())
+    seg = np.repeat(np.arange(len(ends)) // 2, counts)
+    offset = np.repeat(np.cumsum(counts) - counts, counts)
+    first_member = np.repeat(start[g], counts)
+    point = members[first_member + (np.arange(total) - offset)]
+    return np.stack([point, seg], axis=1).astype(np.int64).reshape(-1, 2)

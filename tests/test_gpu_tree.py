@@ -170,6 +170,26 @@ def test_mixed_within_a_tenth_of_the_tree_error():
     assert np.sqrt(np.mean(err ** 2)) <= 0.1 * np.sqrt(np.mean(err_tree ** 2))
 
 
+def test_headline_820k_mixed_tree_accuracy():
+    """The 20,570-target x 821,874-segment loop wake in mixed precision at theta 0.11: relative L2 and max-abs
+    (relative to max |u|) error at most 1e-5 against the FP64 direct sum at 2,048 seeded sample targets.  The
+    FP64 direct kernel is pinned to the oracle by the A1 tests of test_gpu_direct.py."""
+    from openfawe_amd import autotune as AT
+    case = workloads.loop(rows=3744)
+    n = len(case["points"])
+    assert n == 20570 and len(case["starts"]) == 821874
+    out, flagged, stats = run_tree(case, 0.11, precision="mixed", singularity="raise")
+    assert not flagged
+    idx = np.sort(np.random.default_rng(5).choice(n, size=2048, replace=False))
+    ref, ref_flagged = V.hip_sum(case["points"][idx], case["starts"], case["directions"], case["lengths"],
+                                 case["gamma"], case["radius"], case["model"], AT._sample_csr(case, idx), "raise",
+                                 V.HipSettings("fp64", 256))
+    assert not ref_flagged
+    err = out[idx] - ref
+    assert np.linalg.norm(err) <= 1e-5 * np.linalg.norm(ref)
+    assert np.abs(err).max() <= 1e-5 * np.abs(ref).max()
+
+
 def test_api_errors_flags_and_determinism():
     nodes, edges, gamma = workloads.lattice(rows=6, span=5)
     point = .5 * (nodes[edges[3, 0]] + nodes[edges[3, 1]])[None]
