@@ -179,12 +179,31 @@ def network_bench(case, steps, cpu=True):
     ms = 1e3 * (time.perf_counter() - t0) / steps
     result = dict(call="incident_pairs(nodes, edges, first=0)", nodes=len(nodes), segments=len(edges),
                   pairs=cap, ms_per_call=ms, note="includes the one stream synchronisation that reads the count")
+    # ring-edge aggregation of a lattice of the wake's size: 3,744 rows x 109 panels, seeded strengths
+    rng = np.random.default_rng(2)
+    strengths = rng.normal(size=(3743, 109))
+    strengths[1:, ::3] = strengths[:-1, ::3]
+    panels = np.column_stack([np.arange(109), np.arange(1, 110)])
+    dp, ds = torch.from_numpy(panels).cuda(), torch.from_numpy(strengths).cuda()
+    edges_n = int(NET.pack_ring_edges_device(3744, 110, dp, ds)[0].shape[0])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        NET.pack_ring_edges_device(3744, 110, dp, ds, capacity=edges_n)
+    torch.cuda.synchronize()
+    rings = dict(call="pack_rings (3,744 x 110 nodes, 109 panels)", contributions=4 * 3743 * 109, edges=edges_n,
+                 ms_per_call=1e3 * (time.perf_counter() - t0) / steps)
     if cpu:
         from oracle import network_np as NP
         t0 = time.perf_counter()
         NP.incident_pairs(nodes, edges, 0)
         result["cpu_oracle_ms"] = 1e3 * (time.perf_counter() - t0)
         result["cpu_oracle"] = "oracle/network_np.py incident_pairs (NumPy sort and search, one core)"
+        t0 = time.perf_counter()
+        NP.pack_ring_edges(3744, 110, panels, strengths)
+        rings["cpu_oracle_ms"] = 1e3 * (time.perf_counter() - t0)
+        rings["cpu_oracle"] = "oracle/network_np.py pack_ring_edges (NumPy sort and ordered sums, one core)"
+    result["pack_rings"] = rings
     return result
 
 

@@ -173,6 +173,18 @@ int ofw_direct_vjp(ofw_ctx *ctx, int64_t n, const double *points, int64_t m, con
                    const int64_t *ex_seg, const double *grad_out, double *grad_gamma, double *grad_starts,
                    double *grad_ends, double *grad_points, double *out);
 
+/* Directed ring-edge aggregation of a wake lattice (replaces network_numba.pack_ring_edges, network_numba.py:
+ * 112-117, and the loop of wake.pack_rings, wake.py:300-307).  Ring (i, j), 0 <= i < depth - 1, of panel j with
+ * columns panel_edges[j] = (left, right) runs (i,left) -> (i,right) -> (i+1,right) -> (i+1,left) over node ids
+ * row * width + column with circulation strengths[i * panels + j].  Every edge is stored once as (lo, hi), lo < hi,
+ * with the sum, from 0.0 and in ring order, of +strength for each ring that runs it lo -> hi and -strength
+ * otherwise; edges come in the order of their first appearance and exact-zero sums are omitted.  Sets *count_out
+ * (host) to the number of edges and, when it is at most capacity, writes edges (capacity,2) int64 and gamma
+ * (capacity) float64 (device); OFW_ERR_OVERFLOW when the count exceeds capacity (nothing written), OFW_ERR_ARG for
+ * a panel column outside [0, width) or a panel joining a column to itself.  Synchronises the stream once. */
+int ofw_pack_ring_edges(ofw_ctx *ctx, int64_t depth, int64_t width, int64_t panels, const int64_t *panel_edges,
+                        const double *strengths, int64_t *edges, double *gamma, int64_t capacity, int64_t *count_out);
+
 /* Incident-exclusion pairs of a wake network (replaces network_numba.incident_pairs, network_numba.py:242-253,
  * and the loop of wake.py:427-438): for the free points nodes[first:] (n,3), every (point, segment) pair in
  * which the point's coordinates equal an end of segment s (edges (m,2)) bit for bit, in segment order, the

@@ -214,6 +214,30 @@ __global__ void k_write_pairs(int total, int ends, const int32_t* __restrict__ e
   }
 }
 
+// Stable LSD radix sort of (key, val) by the low 8 * passes bits of key; returns the buffer index (0 or 1)
+// that holds the result.  hist and hist_off hold RADIX * tiles entries, part the scan's partials.
+static int radix_sort(Ctx& c, int n, int passes, int32_t* key[2], int32_t* val[2], int32_t* hist, int32_t* hist_off,
+                      int32_t* part) {
+  const int tiles = (n + SORT_TILE - 1) / SORT_TILE;
+  int cur = 0;
+  for (int p = 0; p < passes; ++p) {
+    hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(SORT_THREADS), 0, c.stream, n, p * RADIX_BITS, key[cur],
+                       hist);
+    scan_exclusive(c, RADIX * tiles, hist, hist_off, part, nullptr);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(SORT_THREADS), 0, c.stream, n, p * RADIX_BITS, key[cur],
+                       val[cur], hist_off, key[cur ^ 1], val[cur ^ 1]);
+    cur ^= 1;
+  }
+  return cur;
+}
+
+// Number of 8-bit passes for keys in [0, top].
+static int radix_passes(int64_t top) {
+  int bits = 1;
+  while ((1ll << bits) <= top) ++bits;
+  return (bits + RADIX_BITS - 1) / RADIX_BITS;
+}
+
 int incident_pairs(Ctx& c, int64_t n, const double* nodes, int64_t m, const int64_t* edges, int64_t first,
                    int64_t* pairs, int64_t capacity, int64_t* count_out) {
   *count_out = 0;
@@ -225,9 +249,7 @@ int incident_pairs(Ctx& c, int64_t n, const double* nodes, int64_t m, const int6
   }
   uint32_t H = 2;
   while ((int64_t)H < 2 * nfree) H <<= 1;
-  int key_bits = 1;  // keys run to H inclusive
-  while ((1ll << key_bits) <= (int64_t)H) ++key_bits;
-  const int passes = (key_bits + RADIX_BITS - 1) / RADIX_BITS;
+  const int passes = radix_passes(H);  // keys run to H inclusive
   const int F = (int)nfree, E = (int)(2 * m);
   const int tiles = (F + SORT_TILE - 1) / SORT_TILE;
   const int hist_n = RADIX * tiles;
@@ -263,16 +285,7 @@ int incident_pairs(Ctx& c, int64_t n, const double* nodes, int64_t m, const int6
   hipLaunchKernelGGL(k_group_points, blocks(F), dim3(256), 0, c.stream, F, (int)first, nodes, H, table, cnt, key[0],
                      val[0]);
   scan_exclusive(c, (int)H, cnt, start, part, nullptr);
-  int cur = 0;
-  for (int p = 0; p < passes; ++p) {
-    hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(SORT_THREADS), 0, c.stream, F, p * RADIX_BITS, key[cur],
-                       hist);
-    scan_exclusive(c, hist_n, hist, hist_off, part, nullptr);
-    hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(SORT_THREADS), 0, c.stream, F, p * RADIX_BITS, key[cur],
-                       val[cur], hist_off, key[cur ^ 1], val[cur ^ 1]);
-    cur ^= 1;
-  }
-  const int32_t* members = val[cur];
+  const int32_t* members = val[radix_sort(c, F, passes, key, val, hist, hist_off, part)];
   hipLaunchKernelGGL(k_end_groups, blocks(E), dim3(256), 0, c.stream, (int)m, (int)n, edges, nodes, H, table, cnt,
                      end_group, end_count, status);
   OFW_HIP(hipGetLastError());
@@ -300,7 +313,208 @@ int incident_pairs(Ctx& c, int64_t n, const double* nodes, int64_t m, const int6
   return OFW_OK;
 }
 
+// ---------------------------------------------------------------------------------------------------------------
+// Ring-edge aggregation (network_numba.py:36-117, pack_ring_edges; the loop of wake.py:300-307).  Ring (i, j) of the
+// (depth - 1) x panels lattice runs (i, left) -> (i, right) -> (i+1, right) -> (i+1, left) -> (i, left) over the
+// node ids row * width + column; contribution c = 4 (i panels + j) + k is its k-th edge, stored undirected as
+// (lo, hi) with +strength when it runs lo -> hi and -strength otherwise.  Contributions sharing (lo, hi) are summed
+// from 0.0 in increasing c; the edges come in the order of their first contribution, exact zeros omitted.
+//
+// Same machinery as the incident pairs: groups by a hash of the 64-bit key lo * nodes + hi, members in increasing c
+// by the stable radix sort, one thread per group for the ordered sum, and an exclusive scan over the first
+// contributions for the output positions.
+
+struct RingEdge {
+  int64_t lo, hi;
+  double value;
+};
+
+__device__ __forceinline__ RingEdge ring_contribution(int64_t c, int64_t width, int panels,
+                                                      const int64_t* __restrict__ panel_edges,
+                                                      const double* __restrict__ strengths) {
+  const int64_t ring = c >> 2;
+  const int k = (int)(c & 3);
+  const int64_t i = ring / panels;
+  const int j = (int)(ring - i * panels);
+  const int64_t left = panel_edges[2 * j], right = panel_edges[2 * j + 1];
+  const int64_t row = i * width, next = (i + 1) * width;
+  const int64_t a = k == 0 ? row + left : k == 1 ? row + right : k == 2 ? next + right : next + left;
+  const int64_t b = k == 0 ? row + right : k == 1 ? next + right : k == 2 ? next + left : row + left;
+  const double g = strengths[ring];
+  return a < b ? RingEdge{a, b, g} : RingEdge{b, a, -g};
+}
+
+constexpr unsigned long long EMPTY_KEY = ~0ull;
+
+// status[1] is set when a panel names a column outside [0, width) or joins a column to itself.
+__global__ void k_ring_group(int64_t total, int64_t width, int64_t nodes, int panels,
+                             const int64_t* __restrict__ panel_edges, const double* __restrict__ strengths,
+                             uint32_t H, unsigned long long* __restrict__ table, int32_t* __restrict__ count,
+                             int32_t* __restrict__ key, int32_t* __restrict__ val,
+                             unsigned long long* __restrict__ status) {
+  const uint32_t mask = H - 1;
+  bool bad = false;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < total; c += (int64_t)gridDim.x * blockDim.x) {
+    const int j = (int)((c >> 2) % panels);
+    const int64_t left = panel_edges[2 * j], right = panel_edges[2 * j + 1];
+    val[c] = (int32_t)c;
+    if (left < 0 || left >= width || right < 0 || right >= width || left == right) {
+      bad = true;
+      key[c] = (int32_t)H;
+      continue;
+    }
+    const RingEdge e = ring_contribution(c, width, panels, panel_edges, strengths);
+    const unsigned long long k = (unsigned long long)e.lo * (unsigned long long)nodes + (unsigned long long)e.hi;
+    uint32_t h = (uint32_t)mix64(k) & mask;
+    while (true) {
+      unsigned long long cur = table[h];
+      if (cur == EMPTY_KEY) {
+        const unsigned long long prev = atomicCAS(&table[h], EMPTY_KEY, k);
+        if (prev == EMPTY_KEY || prev == k) break;
+        cur = prev;
+      }
+      if (cur == k) break;
+      h = (h + 1) & mask;
+    }
+    key[c] = (int32_t)h;
+    atomicAdd(&count[h], 1);
+  }
+  if (__builtin_amdgcn_ballot_w64(bad) != 0 && (threadIdx.x & 63) == 0) atomicOr(&status[1], 1ull);
+}
+
+// One thread per table slot: the group's sum in member order from 0.0; a nonzero sum marks the group's first
+// contribution.
+__global__ void k_ring_sums(uint32_t H, const int32_t* __restrict__ count, const int32_t* __restrict__ start,
+                            const int32_t* __restrict__ members, int64_t width, int panels,
+                            const int64_t* __restrict__ panel_edges, const double* __restrict__ strengths,
+                            int32_t* __restrict__ mark, double* __restrict__ sums) {
+  for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < H; h += gridDim.x * blockDim.x) {
+    const int size = count[h];
+    if (size == 0) continue;
+    const int a = start[h];
+    double total = 0.0;
+    for (int q = a; q < a + size; ++q) total = total + ring_contribution(members[q], width, panels, panel_edges,
+                                                                           strengths).value;
+    if (total != 0.0) {
+      const int first = members[a];
+      mark[first] = 1;
+      sums[first] = total;
+    }
+  }
+}
+
+__global__ void k_ring_emit(int64_t total, const int32_t* __restrict__ mark, const int32_t* __restrict__ pos,
+                            const double* __restrict__ sums, int64_t width, int panels,
+                            const int64_t* __restrict__ panel_edges, const double* __restrict__ strengths,
+                            int64_t* __restrict__ edges, double* __restrict__ gamma) {
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < total; c += (int64_t)gridDim.x * blockDim.x) {
+    if (!mark[c]) continue;
+    const RingEdge e = ring_contribution(c, width, panels, panel_edges, strengths);
+    const int64_t o = pos[c];
+    edges[2 * o] = e.lo;
+    edges[2 * o + 1] = e.hi;
+    gamma[o] = sums[c];
+  }
+}
+
+int pack_ring_edges(Ctx& c, int64_t depth, int64_t width, int64_t panels, const int64_t* panel_edges,
+                    const double* strengths, int64_t* edges, double* gamma, int64_t capacity, int64_t* count_out) {
+  *count_out = 0;
+  if (depth < 2 || width < 2 || panels == 0) return OFW_OK;
+  if (depth > INT_MAX || width > INT_MAX || depth * width > INT_MAX || panels > (1 << 28) ||
+      4 * (depth - 1) * panels > (1 << 28)) {
+    set_error("ofw_pack_ring_edges: more than 2^31 - 1 nodes or 2^28 ring-edge contributions");
+    return OFW_ERR_ARG;
+  }
+  const int T = (int)(4 * (depth - 1) * panels);
+  uint32_t H = 2;
+  while ((int64_t)H < 2 * (int64_t)T) H <<= 1;
+  const int passes = radix_passes(H);  // keys run to H inclusive (H marks a bad panel)
+  const int tiles = (T + SORT_TILE - 1) / SORT_TILE;
+  const int hist_n = RADIX * tiles;
+  const int scan_max = std::max<int>(std::max<int>((int)H, T), hist_n);
+  const size_t spart = ((size_t)(scan_max + 4095) / 4096 + 8) * 4;
+  const size_t need = al((size_t)H * 8) + 2 * al((size_t)H * 4) + 4 * al((size_t)T * 4) + 2 * al((size_t)T * 4) +
+                      al((size_t)T * 8) + 2 * al((size_t)hist_n * 4) + al(spart) + al(16) + al(4);
+  OFW_HIP(c.tree[7].reserve(need));
+  char* base = (char*)c.tree[7].ptr;
+  auto carve = [&](size_t bytes) {
+    char* p = base;
+    base += al(bytes);
+    return (void*)p;
+  };
+  unsigned long long* table = (unsigned long long*)carve((size_t)H * 8);
+  int32_t* cnt = (int32_t*)carve((size_t)H * 4);
+  int32_t* start = (int32_t*)carve((size_t)H * 4);
+  int32_t* key[2] = {(int32_t*)carve((size_t)T * 4), (int32_t*)carve((size_t)T * 4)};
+  int32_t* val[2] = {(int32_t*)carve((size_t)T * 4), (int32_t*)carve((size_t)T * 4)};
+  int32_t* mark = (int32_t*)carve((size_t)T * 4);
+  int32_t* pos = (int32_t*)carve((size_t)T * 4);
+  double* sums = (double*)carve((size_t)T * 8);
+  int32_t* hist = (int32_t*)carve((size_t)hist_n * 4);
+  int32_t* hist_off = (int32_t*)carve((size_t)hist_n * 4);
+  int32_t* part = (int32_t*)carve(spart);
+  unsigned long long* status = (unsigned long long*)carve(16);
+  int32_t* kept = (int32_t*)carve(4);
+
+  const int grid = c.cu_count * 8;
+  auto blocks = [&](int64_t items) {
+    return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(grid, (items + 255) / 256)));
+  };
+  hipLaunchKernelGGL(k_net_fill, blocks(2 * (int64_t)H), dim3(256), 0, c.stream, (int)(2 * H), (int32_t*)table, -1);
+  hipLaunchKernelGGL(k_net_fill, blocks(H), dim3(256), 0, c.stream, (int)H, cnt, 0);
+  hipLaunchKernelGGL(k_net_fill, blocks(T), dim3(256), 0, c.stream, T, mark, 0);
+  hipLaunchKernelGGL(k_net_fill, dim3(1), dim3(64), 0, c.stream, 4, (int32_t*)status, 0);
+  hipLaunchKernelGGL(k_ring_group, blocks(T), dim3(256), 0, c.stream, (int64_t)T, width, depth * width, (int)panels,
+                     panel_edges, strengths, H, table, cnt, key[0], val[0], status);
+  scan_exclusive(c, (int)H, cnt, start, part, nullptr);
+  const int32_t* members = val[radix_sort(c, T, passes, key, val, hist, hist_off, part)];
+  hipLaunchKernelGGL(k_ring_sums, blocks(H), dim3(256), 0, c.stream, H, cnt, start, members, width, (int)panels,
+                     panel_edges, strengths, mark, sums);
+  scan_exclusive(c, T, mark, pos, part, kept);
+  OFW_HIP(hipGetLastError());
+  unsigned long long host[2] = {0, 0};
+  int32_t host_kept = 0;
+  OFW_HIP(hipMemcpyAsync(host, status, 16, hipMemcpyDeviceToHost, c.stream));
+  OFW_HIP(hipMemcpyAsync(&host_kept, kept, 4, hipMemcpyDeviceToHost, c.stream));
+  OFW_HIP(hipStreamSynchronize(c.stream));
+  if (host[1]) {
+    set_error("ofw_pack_ring_edges: invalid spanwise panel connectivity");
+    return OFW_ERR_ARG;
+  }
+  *count_out = host_kept;
+  if (host_kept > capacity) {
+    set_error("ofw_pack_ring_edges: the edge buffer is too small (count_out holds the number needed)");
+    return OFW_ERR_OVERFLOW;
+  }
+  if (host_kept == 0) return OFW_OK;
+  hipLaunchKernelGGL(k_ring_emit, blocks(T), dim3(256), 0, c.stream, (int64_t)T, mark, pos, sums, width, (int)panels,
+                     panel_edges, strengths, edges, gamma);
+  OFW_HIP(hipGetLastError());
+  return OFW_OK;
+}
+
 }  // namespace ofw
+
+extern "C" int ofw_pack_ring_edges(ofw_ctx* x, int64_t depth, int64_t width, int64_t panels,
+                                   const int64_t* panel_edges, const double* strengths, int64_t* edges,
+                                   double* gamma, int64_t capacity, int64_t* count_out) {
+  if (count_out) *count_out = 0;
+  if (!x || !count_out || depth < 0 || width < 0 || panels < 0 || capacity < 0 ||
+      (depth >= 2 && width >= 2 && panels > 0 && (!panel_edges || !strengths)) ||
+      (capacity > 0 && (!edges || !gamma))) {
+    ofw::set_error("ofw_pack_ring_edges: invalid arguments");
+    return OFW_ERR_ARG;
+  }
+  ofw::Ctx& c = x->c;
+  c.nev = 0;
+  OFW_HIP(c.mark());
+  const int st = ofw::pack_ring_edges(c, depth, width, panels, panel_edges, strengths, edges, gamma, capacity,
+                                      count_out);
+  if (st) return st;
+  OFW_HIP(c.mark());
+  return OFW_OK;
+}
 
 extern "C" int ofw_incident_pairs(ofw_ctx* x, int64_t n, const double* nodes, int64_t m, const int64_t* edges,
                                   int64_t first, int64_t* pairs, int64_t capacity, int64_t* count_out) {

@@ -9,6 +9,13 @@ one end the coinciding points in increasing order (:229-239).  Point indices cou
 
 ``incident_pairs_loop`` follows the reference's loops with a dictionary keyed by the bit patterns;
 ``incident_pairs`` is a vectorised form of the same rule, checked against the loop form by the tests.
+
+``pack_ring_edges`` (network_numba.py:36-117, the compiled form of the loop of wake.pack_rings, wake.py:300-307)
+aggregates the directed edges of a lattice of vortex rings: ring (i, j) of panel j = (left, right) runs
+(i, left) -> (i, right) -> (i+1, right) -> (i+1, left) over node ids row * width + column; each edge is kept once as
+(lo, hi) with the sum, from 0.0 in ring order, of +strength per ring that runs it lo -> hi and -strength otherwise;
+edges come in order of first appearance and exact-zero sums are dropped.  ``pack_ring_edges_loop`` is the
+dictionary form; ``pack_ring_edges`` the vectorised one.
 """
 import numpy as np
 
@@ -68,3 +75,61 @@ def incident_pairs(nodes, edges, first):
     first_member = np.repeat(start[g], counts)
     point = members[first_member + (np.arange(total) - offset)]
     return np.stack([point, seg], axis=1).astype(np.int64).reshape(-1, 2)
+
+
+def _ring_check(depth, width, panel_edges, strengths):
+    depth, width = int(depth), int(width)
+    panel_edges = np.ascontiguousarray(panel_edges, dtype=np.int64).reshape(-1, 2)
+    strengths = np.ascontiguousarray(strengths, dtype=np.float64)
+    if depth < 2 or width < 2:
+        raise ValueError("rows must have shape (depth+1,span+1,3)")
+    if (np.any(panel_edges < 0) or np.any(panel_edges >= width)
+            or np.any(panel_edges[:, 0] == panel_edges[:, 1])):
+        raise ValueError("invalid spanwise panel connectivity")
+    if strengths.shape != (depth - 1, len(panel_edges)):
+        raise ValueError("ring strength shape mismatch")
+    return depth, width, panel_edges, strengths
+
+
+def ring_contributions(depth, width, panel_edges, strengths):
+    """(lo, hi, value) of every ring edge in ring order (network_numba.py:36-64): 4 per ring."""
+    depth, width, panel_edges, strengths = _ring_check(depth, width, panel_edges, strengths)
+    i = np.repeat(np.arange(depth - 1), len(panel_edges))
+    left = np.tile(panel_edges[:, 0], depth - 1)
+    right = np.tile(panel_edges[:, 1], depth - 1)
+    vertices = np.stack([i * width + left, i * width + right, (i + 1) * width + right, (i + 1) * width + left],
+                        axis=1)
+    a, b = vertices.reshape(-1), np.roll(vertices, -1, axis=1).reshape(-1)
+    g = np.repeat(strengths.reshape(-1), 4)
+    forward = a < b
+    return np.where(forward, a, b), np.where(forward, b, a), np.where(forward, g, -g)
+
+
+def pack_ring_edges_loop(depth, width, panel_edges, strengths):
+    """The dictionary form (wake.py:300-307): (edges (e,2) int64, gamma (e,) float64)."""
+    lo, hi, value = ring_contributions(depth, width, panel_edges, strengths)
+    sums = {}
+    for key in zip(lo.tolist(), hi.tolist(), value.tolist()):
+        sums[key[:2]] = sums.get(key[:2], 0.0) + key[2]
+    kept = [key for key, total in sums.items() if total != 0.0]
+    return (np.asarray(kept, dtype=np.int64).reshape(-1, 2),
+            np.asarray([sums[key] for key in kept], dtype=np.float64))
+
+
+def pack_ring_edges(depth, width, panel_edges, strengths):
+    """The same edges, vectorised: a stable sort by key, the group sums added member by member in ring order."""
+    lo, hi, value = ring_contributions(depth, width, panel_edges, strengths)
+    if len(lo) == 0:
+        return np.zeros((0, 2), dtype=np.int64), np.zeros(0)
+    key = lo * (int(depth) * int(width)) + hi
+    order = np.argsort(key, kind="stable")
+    starts = np.flatnonzero(np.r_[True, key[order][1:] != key[order][:-1]])
+    sizes = np.diff(np.r_[starts, len(order)])
+    total = np.zeros(len(starts))
+    for r in range(int(sizes.max())):           # member r of every group that has one, in ring order
+        has = sizes > r
+        total[has] = total[has] + value[order[starts[has] + r]]
+    first = order[starts]
+    keep = np.argsort(first, kind="stable")
+    keep = keep[total[keep] != 0.0]
+    return np.stack([lo[first[keep]], hi[first[keep]]], axis=1).astype(np.int64), total[keep]

@@ -1,6 +1,8 @@
 """GPU incident-exclusion pairs (ofw_incident_pairs) against the oracle, element for element: the reference's
 test networks (tests/test_network_numba.py), signed zeros, NaN and infinite coordinates, self-loops, a group of
-20,000 coincident points, the 819,826-segment crosswind-loop wake, and the C ABI's overflow and range errors."""
+20,000 coincident points, the 819,826-segment crosswind-loop wake, and the C ABI's overflow and range errors.
+The ring-edge aggregation (ofw_pack_ring_edges) bitwise against the oracle on the reference's pack_rings cases and a
+3,744 x 110 lattice, with its overflow and panel errors."""
 import ctypes
 
 import numpy as np
@@ -102,3 +104,51 @@ def test_c_abi_overflow_range_and_determinism():
         NET.incident_pairs_device(dn, bad, 0)
     # the context still works after the errors
     assert torch.equal(NET.incident_pairs_device(dn, de, 0), b)
+
+
+# ------------------------------------------------------------------ ring-edge aggregation
+from tests.test_oracle_network import ring_cases  # noqa: E402
+
+
+def check_rings(depth, width, panels, strengths):
+    print(f"pack rings: depth={depth} width={width} panels={len(panels)}", flush=True)
+    rows = np.zeros((depth, width, 3))
+    rows[..., 0] = np.arange(depth)[:, None]
+    rows[..., 1] = np.arange(width)[None, :]
+    nodes, edges, gamma = NET.pack_rings(rows, strengths, panels)
+    want_e, want_g = NP.pack_ring_edges_loop(depth, width, panels, strengths)
+    np.testing.assert_array_equal(nodes, rows.reshape(-1, 3))
+    assert edges.dtype == np.int64 and gamma.dtype == np.float64
+    np.testing.assert_array_equal(edges, want_e)
+    np.testing.assert_array_equal(gamma.view(np.int64), want_g.view(np.int64))   # bitwise
+    return edges, gamma
+
+
+@pytest.mark.parametrize("case", list(ring_cases()), ids=lambda c: c[0])
+def test_pack_rings_bitwise(case):
+    check_rings(*case[1:])
+
+
+def test_pack_rings_full_wake_and_errors():
+    import torch
+    rng = np.random.default_rng(2)
+    strengths = rng.normal(size=(3743, 109))
+    strengths[1:, ::3] = strengths[:-1, ::3]          # cancelling shed edges
+    strengths[::11] = 0.0
+    panels = np.column_stack([np.arange(109), np.arange(1, 110)])
+    edges, _ = check_rings(3744, 110, panels, strengths)
+    assert len(edges) < 4 * 3743 * 109
+    dp = torch.from_numpy(panels).cuda()
+    ds = torch.from_numpy(strengths).cuda()
+    e1, g1 = NET.pack_ring_edges_device(3744, 110, dp, ds, capacity=5)    # retried with the exact count
+    e2, g2 = NET.pack_ring_edges_device(3744, 110, dp, ds)
+    assert torch.equal(e1, e2) and torch.equal(g1.view(torch.int64), g2.view(torch.int64))
+    bad = dp.clone()
+    bad[7, 1] = 110
+    with pytest.raises(ValueError):
+        NET.pack_ring_edges_device(3744, 110, bad, ds)
+    bad[7, 1] = bad[7, 0]
+    with pytest.raises(ValueError):
+        NET.pack_ring_edges_device(3744, 110, bad, ds)
+    e3, g3 = NET.pack_ring_edges_device(3744, 110, dp, ds)
+    assert torch.equal(e3, e2) and torch.equal(g3.view(torch.int64), g2.view(torch.int64))

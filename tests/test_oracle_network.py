@@ -1,7 +1,9 @@
 """The oracle of the incident-exclusion pairs (oracle/network_np.py) against the reference's own test rule and
 against brute force, on the networks of the reference's tests (tests/test_network_numba.py:18-95): wavy ring
 rows of 2x3 to 400x37 nodes, a coincident column of nodes, signed zeros, NaN and infinite coordinates, first
-at 0 and at n, self-loop segments.  CPU only."""
+at 0 and at n, self-loop segments.  The ring-edge aggregation (pack_ring_edges) against a hand-derived lattice,
+the closure of every ring (the circulation into each node equals the circulation out) and lone rings, on the
+strengths of the same tests.  CPU only."""
 import numpy as np
 import pytest
 
@@ -140,3 +142,69 @@ def test_host_validation_before_any_device_work():
             NET.incident_pairs(*args)
         with pytest.raises(ValueError):
             NP.incident_pairs(*args)
+
+
+# ------------------------------------------------------------------ ring-edge aggregation (pack_ring_edges)
+def ring_cases():
+    """The strengths of the reference's pack_rings tests (tests/test_network_numba.py:52-95): repeated and
+    cancelling strengths, an exact zero, a scrambled panel list with a reversed panel, a 400-row decaying history
+    with zero rows."""
+    rng = np.random.default_rng(7)
+    for depth, width in ((2, 3), (3, 5), (12, 9), (41, 17)):
+        strengths = rng.normal(size=(depth - 1, width - 1))
+        strengths[1:, :] = strengths[:-1, :] * rng.choice([1.0, -1.0, 0.5], size=(depth - 2, width - 1))
+        strengths[0, 0] = 0.0
+        panels = np.column_stack([np.arange(width - 1), np.arange(1, width)])
+        yield f"random-{depth}x{width}", depth, width, panels, strengths
+    panels = np.array([[0, 1], [2, 1], [2, 3], [4, 5], [6, 5], [6, 7]])
+    yield "scrambled", 9, 8, panels, np.random.default_rng(3).normal(size=(8, len(panels)))
+    base = np.random.default_rng(11).normal(size=36)
+    strengths = base[None, :] * np.exp(-0.01 * np.arange(399))[:, None]
+    strengths[::7] = 0.0
+    yield "history-400x37", 400, 37, np.column_stack([np.arange(36), np.arange(1, 37)]), strengths
+
+
+def both_ring_forms(depth, width, panels, strengths):
+    e1, g1 = NP.pack_ring_edges_loop(depth, width, panels, strengths)
+    e2, g2 = NP.pack_ring_edges(depth, width, panels, strengths)
+    np.testing.assert_array_equal(e1, e2)
+    np.testing.assert_array_equal(g1.view(np.int64), g2.view(np.int64))
+    assert e2.dtype == np.int64 and g2.dtype == np.float64
+    return e2, g2
+
+
+def test_ring_edges_known_answer():
+    # 2 x 2 rings on 3 x 3 nodes (ids row * 3 + column); strengths a=1, b=2 (row 0), c=1, d=0.5 (row 1):
+    # the edge (3,4) between rows 0 and 1 of panel 0 gets -a + c = 0 and is dropped.
+    edges, gamma = both_ring_forms(3, 3, np.array([[0, 1], [1, 2]]), np.array([[1.0, 2.0], [1.0, 0.5]]))
+    assert edges.tolist() == [[0, 1], [1, 4], [0, 3], [1, 2], [2, 5], [4, 5], [4, 7], [6, 7], [3, 6], [5, 8],
+                              [7, 8]]
+    assert gamma.tolist() == [1.0, -1.0, -1.0, 2.0, 2.0, -1.5, 0.5, -1.0, -1.0, 0.5, -0.5]
+
+
+@pytest.mark.parametrize("case", list(ring_cases()), ids=lambda c: c[0])
+def test_ring_edges_forms_agree_and_rings_close(case):
+    _, depth, width, panels, strengths = case
+    edges, gamma = both_ring_forms(depth, width, panels, strengths)
+    assert np.all(edges[:, 0] < edges[:, 1]) and np.all(gamma != 0.0)
+    assert len(np.unique(edges, axis=0)) == len(edges)
+    # every ring is closed, so the circulation entering each node equals the circulation leaving it
+    balance = np.zeros(depth * width)
+    np.add.at(balance, edges[:, 0], gamma)
+    np.add.at(balance, edges[:, 1], -gamma)
+    assert np.abs(balance).max() <= 1e-12 * max(1.0, np.abs(strengths).max())
+    # a lone ring (one nonzero strength) gives exactly its four edges with +-strength
+    one = np.zeros_like(strengths)
+    one[-1, -1] = 1.25
+    e1, g1 = both_ring_forms(depth, width, panels, one)
+    assert len(e1) == 4 and sorted(np.abs(g1).tolist()) == [1.25] * 4
+
+
+def test_ring_edges_host_validation():
+    rows = np.zeros((3, 4, 3))
+    rows[..., 0] = np.arange(4)
+    good = np.ones((2, 3))
+    for args in ((rows[:1], good[:0]), (rows[..., :2], good), (rows, good[:, :2]), (rows, good, [[0, 4]]),
+                 (rows, good[:, :1], [[1, 1]]), (rows, good[:, :1], [[0.0, 1.0]]), (rows, np.full((2, 3), np.inf))):
+        with pytest.raises(ValueError):
+            NET.pack_rings(*args)
