@@ -46,6 +46,14 @@ enum ofw_precision {
   OFW_FP32 = 2   /* as MIXED with an FP32 accumulator */
 };
 
+enum ofw_stage {         /* RK stage arithmetic of ofw_stage_combine (wake.py:596-614), NumPy evaluation order */
+  OFW_STAGE_STEP = 0,    /* out = x + h*a               Euler step, Heun predictor, RK4 stage 4 */
+  OFW_STAGE_HALF = 1,    /* out = x + h*a/2             RK4 stages 2 and 3 */
+  OFW_STAGE_HEUN = 2,    /* out = x + h*(a+b)/2         Heun corrector */
+  OFW_STAGE_RK4 = 3,     /* out = x + h*(a+2b+2c+d)/6   RK4 combination */
+  OFW_STAGE_ADD = 4      /* out = x + a                 ambient plus induced velocity */
+};
+
 typedef struct ofw_ctx ofw_ctx;
 
 /* ABI version of the library (OFW_ABI_VERSION). */
@@ -196,6 +204,25 @@ int ofw_pack_ring_edges(ofw_ctx *ctx, int64_t depth, int64_t width, int64_t pane
  * Synchronises the stream once (to read the count). */
 int ofw_incident_pairs(ofw_ctx *ctx, int64_t n, const double *nodes, int64_t m, const int64_t *edges, int64_t first,
                        int64_t *pairs, int64_t capacity, int64_t *count_out);
+
+/* Segment preparation of segments_velocity (vortex.py:458-465) from a node/edge network: starts (m,3) =
+ * nodes[edges[:,0]], delta = nodes[edges[:,1]] - starts, lengths (m) = hypot(hypot(dx,dy),dz), directions (m,3) =
+ * delta / lengths.  OFW_ERR_ARG ("connectivity index out of range", "zero-length segment") as the reference.
+ * Synchronises the stream once (for the errors). */
+int ofw_network_segments(ofw_ctx *ctx, int64_t n, const double *nodes, int64_t m, const int64_t *edges,
+                         double *starts, double *directions, double *lengths);
+
+/* Exclusion CSR of a target range (the subsets of wake.py:440-487 followed by exclusion_csr, vortex_numba.py:180-243):
+ * from (point, segment) pairs (npairs,2) in any order, the pairs whose point lies in [lo, hi) with points shifted
+ * by -lo and, when remap (m) is given, segments mapped through it (pairs whose remap is negative are dropped).
+ * ptr_out (hi-lo+1) and seg_out (capacity npairs) receive per point its sorted unique segments.  OFW_ERR_ARG
+ * ("exclusion index out of range") when a kept pair names a segment outside [0, m). */
+int ofw_exclusion_subset(ofw_ctx *ctx, int64_t npairs, const int64_t *pairs, int64_t lo, int64_t hi, int64_t m,
+                         const int64_t *remap, int64_t *ptr_out, int64_t *seg_out);
+
+/* Stage arithmetic of convect_geometry over count doubles (enum ofw_stage; b, c, d may be NULL where unused). */
+int ofw_stage_combine(ofw_ctx *ctx, int mode, int64_t count, double h, const double *x, const double *a,
+                      const double *b, const double *c, const double *d, double *out);
 
 #ifdef __cplusplus
 }

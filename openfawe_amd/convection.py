@@ -1,0 +1,253 @@
+"""The free-wake geometry step on the GPU: ``_network_convection`` and the stage loop of ``convect_geometry``.
+
+``network_convection`` is the counterpart of ``wake._network_convection`` (wake.py:404-491): the velocity of the
+free points ``nodes[first:]`` of a frozen circulation graph is the ambient velocity plus the induced velocity of
+every segment, with the segments incident to a point excluded (``incident_pairs``), split as the reference splits
+the targets:
+
+* wing rows ``[0, rows_end)`` with singularity 'raise' and the edge cores ``radii``;
+* with ``free_convection_rows`` (WakeConfig), only the newest ``limit * first`` row points are targets; the
+  frozen rows keep the ambient velocity or, with ``frozen_rows_drift='ambient_rings'``, take the induction of the
+  ring edges alone (both nodes ring nodes; wake.py:453-462);
+* ring nodes ``[rows_end, ring_end)`` (the leading ``ring_free`` of them, all by default) with singularity 'zero'
+  and the cores ``ring_radii`` (default ``radii``).
+
+``convect_geometry`` is the stage loop of ``wake.convect_geometry`` (wake.py:559-616) on that graph: Euler, Heun
+or RK4, ``substeps`` substeps, the bound row moved linearly by ``displacement`` over the step.
+
+Everything stays in HBM: the exclusion pairs (``ofw_incident_pairs``), the segment geometry of each stage
+(``ofw_network_segments``), the exclusion CSR of each target range (``ofw_exclusion_subset``), the sums (the
+direct sum or the tree code of this package) and the stage arithmetic (``ofw_stage_combine``, NumPy's order of
+operations).  The host reads flags and error status only.  Not modelled: the surface guard
+(``surface_guard='off'`` only), the rings' opt-in sheet drift, callable inflows that need host arrays (an
+inflow is a 3-vector or a callable on device tensors).
+"""
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from . import network as NET
+from .vortex import (HipSettings, Statistics, TreeSettings, _ctx, _model, _ptr, _raise_for_flags, _torch,
+                     hip_sum_device)
+
+INTEGRATORS = ("euler", "heun", "rk4")
+
+
+@dataclass(frozen=True)
+class ConvectionSettings:
+    """The WakeConfig fields of the convection step (wake.py:67-133) and the engine of its sums.
+
+    backend 'hip' is the direct sum, 'hip_tree' the tree code (``tree`` required), both with ``hip`` settings
+    (the reference's kernel_options(config, large=True), wake.py:24-38)."""
+    model: int = 2
+    free_wake: bool = True
+    free_convection_rows: int | None = None
+    frozen_rows_drift: str = "ambient"
+    backend: str = "hip"
+    tree: TreeSettings | None = None
+    hip: HipSettings = HipSettings("fp64")
+
+    def __post_init__(self):
+        object.__setattr__(self, "model", int(_model(self.model)))
+        if self.backend not in ("hip", "hip_tree"):
+            raise ValueError("backend must be 'hip' or 'hip_tree'")
+        if (self.backend == "hip_tree") != isinstance(self.tree, TreeSettings):
+            raise ValueError("tree settings are required exactly for backend='hip_tree'")
+        if not isinstance(self.hip, HipSettings):
+            raise ValueError("hip must be HipSettings")
+        if self.frozen_rows_drift not in ("ambient", "ambient_rings"):
+            raise ValueError("frozen_rows_drift must be 'ambient' or 'ambient_rings'")
+        limit = self.free_convection_rows
+        if limit is not None and (isinstance(limit, bool) or int(limit) != limit or limit < 1):
+            raise ValueError("free_convection_rows must be a positive integer")
+
+
+# ------------------------------------------------------------------ device pieces
+def network_segments(nodes, edges):
+    """(starts (m,3), directions (m,3), lengths (m)) of the network's segments (vortex.py:458-465)."""
+    torch = _torch()
+    m = int(edges.shape[0])
+    starts = torch.empty((m, 3), dtype=torch.float64, device=nodes.device)
+    directions = torch.empty_like(starts)
+    lengths = torch.empty(m, dtype=torch.float64, device=nodes.device)
+    status = _lib.load().ofw_network_segments(_ctx(), int(nodes.shape[0]), _ptr(nodes), m, _ptr(edges),
+                                              _ptr(starts), _ptr(directions), _ptr(lengths))
+    _lib.check(status, "ofw_network_segments")
+    return starts, directions, lengths
+
+
+def exclusion_subset(pairs, lo, hi, m, remap=None):
+    """Exclusion CSR (ptr (hi-lo+1), seg) of the pairs whose point lies in [lo, hi), points shifted by -lo."""
+    torch = _torch()
+    count = int(hi) - int(lo)
+    ptr = torch.empty(count + 1, dtype=torch.int64, device=pairs.device)
+    seg = torch.empty(max(int(pairs.shape[0]), 1), dtype=torch.int64, device=pairs.device)
+    status = _lib.load().ofw_exclusion_subset(_ctx(), int(pairs.shape[0]), _ptr(pairs), int(lo), int(hi), int(m),
+                                              _ptr(remap), _ptr(ptr), _ptr(seg))
+    _lib.check(status, "ofw_exclusion_subset")
+    return ptr, seg
+
+
+def stage(mode, h, x, a, b=None, c=None, d=None, out=None):
+    """``out = `` the stage arithmetic ``mode`` (the _lib.STAGE_* values) over whole tensors."""
+    torch = _torch()
+    if out is None:
+        out = torch.empty_like(x)
+    status = _lib.load().ofw_stage_combine(_ctx(), int(mode), int(x.numel()), float(h), _ptr(x), _ptr(a), _ptr(b),
+                                           _ptr(c), _ptr(d), _ptr(out))
+    _lib.check(status, "ofw_stage_combine")
+    return out
+
+
+def _induced(points, geometry, gamma, radius, ptr, seg, singularity, settings):
+    """segments_velocity's sum and error rules (vortex.py:466-497) on device tensors."""
+    torch = _torch()
+    starts, directions, lengths = geometry
+    if settings.backend == "hip_tree":
+        from .tree import hip_tree_sum_device
+        out, flags, _, _ = hip_tree_sum_device(points, starts, directions, lengths, gamma, radius, settings.model,
+                                               ptr, seg, singularity, settings.tree, settings.hip)
+        flag_seg = None
+    else:
+        out, flags, flag_seg = hip_sum_device(points, starts, directions, lengths, gamma, radius, settings.model,
+                                              ptr, seg, singularity, settings.hip)
+    if bool(flags.any()):
+        stats = dict(flags=flags.cpu().numpy())
+        if flag_seg is not None:
+            stats["flag_seg"] = flag_seg.cpu().numpy()
+        Statistics.last = stats
+        _raise_for_flags(stats)
+    if not bool(torch.isfinite(out).all()):
+        raise ValueError("velocity exceeds finite float64 range")
+    return out
+
+
+def _per_edge(torch, value, m, device, name):
+    t = torch.as_tensor(value, dtype=torch.float64, device=device)
+    if t.dim() == 0:
+        t = t.expand(m)
+    if tuple(t.shape) != (m,):
+        raise ValueError(f"{name} must be a scalar or have one value per segment")
+    if not bool(torch.isfinite(t).all()) or bool((t < 0).any()):
+        raise ValueError(f"{name} must be finite and nonnegative")
+    return t.contiguous()
+
+
+def ambient_velocity(inflow, points, time):
+    """wake.ambient_velocity (wake.py:399-401) on device: a 3-vector, or a callable (points tensor, time)."""
+    torch = _torch()
+    if callable(inflow):
+        values = torch.as_tensor(inflow(points.clone(), float(time)), dtype=torch.float64, device=points.device)
+    else:
+        values = torch.as_tensor(np.asarray(inflow, dtype=np.float64), device=points.device)
+    if values.dim() == 1:
+        values = values.reshape(1, 3).expand(points.shape[0], 3)
+    if tuple(values.shape) != tuple(points.shape) or not bool(torch.isfinite(values).all()):
+        raise ValueError("ambient velocity must be finite with shape (3,) or (n, 3)")
+    return values.contiguous().clone()
+
+
+# ------------------------------------------------------------------ the step
+def network_convection(nodes, edges, gamma, first, radii, settings, ambient, ring_first=None, ring_free=None,
+                       ring_radii=None):
+    """wake._network_convection (wake.py:404-491) on device tensors: the velocity (n - first, 3) of nodes[first:].
+
+    nodes (n,3) float64, edges (m,2) int64, gamma (m,) float64, radii and ring_radii scalars or (m,) per edge,
+    ambient the (n - first, 3) ambient velocity of the points.
+    """
+    torch = _torch()
+    n, m, first = int(nodes.shape[0]), int(edges.shape[0]), int(first)
+    count = n - first
+    velocity = ambient.clone()
+    if not settings.free_wake or m == 0:
+        return velocity
+    nodes, edges, gamma = nodes.contiguous(), edges.contiguous(), gamma.contiguous()
+    points = nodes[first:]
+    pairs = NET.incident_pairs_device(nodes, edges, first)
+    geometry = network_segments(nodes, edges)
+    radii = _per_edge(torch, radii, m, nodes.device, "radii")
+    target_radii = radii if ring_radii is None else _per_edge(torch, ring_radii, m, nodes.device, "ring_radii")
+    rows_end = count if ring_first is None or ring_first >= n else int(ring_first) - first
+    ring_end = count if ring_free is None else min(count, rows_end + int(ring_free))
+
+    def add(lo, hi, radius, singularity):
+        ptr, seg = exclusion_subset(pairs, lo, hi, m)
+        out = _induced(points[lo:hi], geometry, gamma, radius, ptr, seg, singularity, settings)
+        part = velocity[lo:hi]
+        stage(_lib.STAGE_ADD, 0.0, part, out, out=part)
+
+    limit = settings.free_convection_rows
+    if limit is not None and limit * first < rows_end:
+        free = limit * first
+        add(0, free, radii, "raise")
+        if settings.frozen_rows_drift == "ambient_rings" and ring_first is not None and ring_first < n:
+            ring_edge = (edges >= int(ring_first)).all(dim=1)
+            sub = torch.nonzero(ring_edge).reshape(-1)
+            if sub.numel():
+                remap = torch.full((m,), -1, dtype=torch.int64, device=nodes.device)
+                remap[sub] = torch.arange(sub.numel(), dtype=torch.int64, device=nodes.device)
+                ptr, seg = exclusion_subset(pairs, free, rows_end, m, remap)
+                sub_geometry = tuple(g[sub].contiguous() for g in geometry)
+                out = _induced(points[free:rows_end], sub_geometry, gamma[sub].contiguous(), radii[sub].contiguous(),
+                               ptr, seg, "raise", settings)
+                part = velocity[free:rows_end]
+                stage(_lib.STAGE_ADD, 0.0, part, out, out=part)
+        if rows_end < ring_end:
+            add(rows_end, ring_end, target_radii, "zero")
+    elif ring_first is None or ring_first >= n:
+        add(0, count, radii, "raise")
+    else:
+        split = int(ring_first) - first
+        add(0, split, radii, "raise")
+        if split < ring_end:
+            add(split, ring_end, target_radii, "zero")
+    return velocity
+
+
+def convect_geometry(bound, points, edges, gamma, radii, settings, *, inflow, time, dt_s, substeps=1,
+                     integrator="euler", displacement=None, ring_first=None, ring_free=None, ring_radii=None):
+    """The stage loop of wake.convect_geometry (wake.py:586-616) on device tensors: the new free points.
+
+    bound (first,3) are the bound nodes at ``time`` (moved by ``displacement`` (first,3) over the step), points
+    (k,3) the free nodes (wing rows, then ring nodes from ``ring_first``), edges (m,2) index the network
+    concatenate(bound, points).
+    """
+    torch = _torch()
+    if integrator not in INTEGRATORS:
+        raise ValueError(f"convection_integrator must be one of {INTEGRATORS}")
+    if isinstance(substeps, bool) or int(substeps) != substeps or substeps < 1:
+        raise ValueError("substeps must be a positive integer")
+    first = int(bound.shape[0])
+    bound, points = bound.contiguous(), points.contiguous()
+    if displacement is not None:
+        displacement = displacement.contiguous()
+        if not bool(displacement.any()):
+            displacement = None
+    dt = dt_s / substeps
+
+    def rhs(current, t):
+        stage_bound = bound
+        if displacement is not None and t != time:
+            stage_bound = stage(_lib.STAGE_STEP, (t - time) / dt_s, bound, displacement)
+        nodes = torch.cat((stage_bound, current))
+        ambient = ambient_velocity(inflow, current, t)
+        return network_convection(nodes, edges, gamma, first, radii, settings, ambient, ring_first, ring_free,
+                                  ring_radii)
+
+    for substep in range(int(substeps)):
+        t = time + substep * dt
+        k1 = rhs(points, t)
+        if integrator == "euler":
+            candidate = stage(_lib.STAGE_STEP, dt, points, k1)
+        elif integrator == "heun":
+            predicted = stage(_lib.STAGE_STEP, dt, points, k1)
+            k2 = rhs(predicted, t + dt)
+            candidate = stage(_lib.STAGE_HEUN, dt, points, k1, k2)
+        else:
+            k2 = rhs(stage(_lib.STAGE_HALF, dt, points, k1), t + dt / 2)
+            k3 = rhs(stage(_lib.STAGE_HALF, dt, points, k2), t + dt / 2)
+            k4 = rhs(stage(_lib.STAGE_STEP, dt, points, k3), t + dt)
+            candidate = stage(_lib.STAGE_RK4, dt, points, k1, k2, k3, k4)
+        points = candidate
+    return points

@@ -214,9 +214,11 @@ __global__ void k_write_pairs(int total, int ends, const int32_t* __restrict__ e
   }
 }
 
+int radix_hist_size(int n) { return RADIX * ((n + SORT_TILE - 1) / SORT_TILE); }
+
 // Stable LSD radix sort of (key, val) by the low 8 * passes bits of key; returns the buffer index (0 or 1)
 // that holds the result.  hist and hist_off hold RADIX * tiles entries, part the scan's partials.
-static int radix_sort(Ctx& c, int n, int passes, int32_t* key[2], int32_t* val[2], int32_t* hist, int32_t* hist_off,
+int radix_sort(Ctx& c, int n, int passes, int32_t* key[2], int32_t* val[2], int32_t* hist, int32_t* hist_off,
                       int32_t* part) {
   const int tiles = (n + SORT_TILE - 1) / SORT_TILE;
   int cur = 0;
@@ -232,7 +234,7 @@ static int radix_sort(Ctx& c, int n, int passes, int32_t* key[2], int32_t* val[2
 }
 
 // Number of 8-bit passes for keys in [0, top].
-static int radix_passes(int64_t top) {
+int radix_passes(int64_t top) {
   int bits = 1;
   while ((1ll << bits) <= top) ++bits;
   return (bits + RADIX_BITS - 1) / RADIX_BITS;

@@ -69,6 +69,15 @@ int upload_i32(Ctx& c, const std::vector<int64_t>& values, int32_t* dst);
 // Exclusive scan of n int32 values (three kernels); the total goes to *total_out when it is not null.
 void scan_exclusive(Ctx& c, int n, const int32_t* in, int32_t* out, int32_t* part, int32_t* total_out);
 
+// Stable LSD radix sort of (key, val) int32 pairs of n items by the low 8 * passes bits of key (8-bit digits);
+// returns the buffer index (0 or 1) holding the result.  hist and hist_off hold radix_hist_size(n) entries, part
+// the scan partials of that many items.
+int radix_sort(Ctx& c, int n, int passes, int32_t* key[2], int32_t* val[2], int32_t* hist, int32_t* hist_off,
+               int32_t* part);
+// Number of 8-bit passes for keys in [0, top]; the histogram entries of a sort of n items.
+int radix_passes(int64_t top);
+int radix_hist_size(int n);
+
 // Reserve and lay out every array of a build of m segments in K trees with leaf size `leaf`.
 int tree_layout(Ctx& c, int buf, int64_t m, int K, int leaf, TreeArrays& t);
 // Enqueue the whole build (no host synchronisation): midpoints mx/my/mz (m), tree segment offsets on the
