@@ -207,6 +207,39 @@ def network_bench(case, steps, cpu=True):
     return result
 
 
+def convection_bench(case, rows, steps, tree, settings):
+    """One free-wake geometry step (wake.py:586-616) of the loop wake on the GPU: row 0 is the bound row, the newest
+    187 rows are free targets (free_convection_rows=187; 20,570 points), the older rows frozen, the rotor helices ring
+    nodes ('zero'); every stage runs the incident pairs, the segment geometry, the exclusion subsets and the tree sums
+    of wake._network_convection."""
+    import torch
+    from openfawe_amd import convection as CV
+    nodes, edges = case["nodes"], case["edges"]
+    width = 110
+    ring_first = rows * width          # the rotor helices follow the wake rows
+    dn = torch.from_numpy(np.ascontiguousarray(nodes)).cuda()
+    de = torch.from_numpy(np.ascontiguousarray(edges)).cuda()
+    dg = torch.from_numpy(np.ascontiguousarray(case["gamma"])).cuda()
+    dr = torch.from_numpy(np.ascontiguousarray(case["radius"])).cuda()
+    cs = CV.ConvectionSettings(model=case["model"], free_convection_rows=187, backend="hip_tree", tree=tree,
+                               hip=settings)
+    result = dict(workload=f"{case['name']}: bound row + {rows - 1} rows x {width} nodes + rotor helices",
+                  free_row_targets=187 * width, ring_targets=len(nodes) - ring_first, segments=len(edges),
+                  settings=dict(theta=tree.theta, precision=settings.precision, free_convection_rows=187))
+    for integrator in ("euler", "rk4"):
+        def run():
+            return CV.convect_geometry(dn[:width], dn[width:], de, dg, dr, cs, inflow=[10.0, 0.0, 0.0], time=0.0,
+                                       dt_s=0.05, integrator=integrator, ring_first=ring_first)
+        run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            run()
+        torch.cuda.synchronize()
+        result[f"{integrator}_step_ms"] = 1e3 * (time.perf_counter() - t0) / steps
+    return result
+
+
 def traffic(workload, kernel):
     """HBM bytes per launch of the kernel from the committed PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by
     the copy-kernel calibration), or None when none was measured for this workload."""
@@ -407,6 +440,7 @@ def main():
         result["batch"] = batch_bench(max(3, args.steps // 2))
         if "nodes" in case:
             result["network"] = network_bench(case, max(3, args.steps // 2), cpu=not args.no_cpu)
+            result["convection"] = convection_bench(case, spec["rows"], max(2, args.steps // 5), tree, settings)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(case, spec, usable_cpus())
     print(json.dumps(result), flush=True)

@@ -212,13 +212,16 @@ int ofw_incident_pairs(ofw_ctx *ctx, int64_t n, const double *nodes, int64_t m, 
 int ofw_network_segments(ofw_ctx *ctx, int64_t n, const double *nodes, int64_t m, const int64_t *edges,
                          double *starts, double *directions, double *lengths);
 
-/* Exclusion CSR of a target range (the subsets of wake.py:440-487 followed by exclusion_csr, vortex_numba.py:180-243):
- * from (point, segment) pairs (npairs,2) in any order, the pairs whose point lies in [lo, hi) with points shifted
- * by -lo and, when remap (m) is given, segments mapped through it (pairs whose remap is negative are dropped).
- * ptr_out (hi-lo+1) and seg_out (capacity npairs) receive per point its sorted unique segments.  OFW_ERR_ARG
- * ("exclusion index out of range") when a kept pair names a segment outside [0, m). */
-int ofw_exclusion_subset(ofw_ctx *ctx, int64_t npairs, const int64_t *pairs, int64_t lo, int64_t hi, int64_t m,
-                         const int64_t *remap, int64_t *ptr_out, int64_t *seg_out);
+/* Exclusion CSR of target ranges (the subsets of wake.py:440-487 followed by exclusion_csr, vortex_numba.py:
+ * 180-243): from (point, segment) pairs (npairs,2) in any order, the pairs whose point p lies in [lo, hi), as
+ * output point p - lo, or point_map[p - lo] when point_map (hi-lo) is given (negative: dropped; several target
+ * ranges packed into one sum), and, when remap (m) is given, with segments mapped through it (negative: dropped).
+ * ptr_out (count+1) and seg_out (capacity npairs) receive per output point its sorted unique segments; count must
+ * be hi - lo without a point map.  OFW_ERR_ARG ("exclusion index out of range") when a kept pair names a segment
+ * outside [0, m) or the map an output point at or beyond count. */
+int ofw_exclusion_subset(ofw_ctx *ctx, int64_t npairs, const int64_t *pairs, int64_t lo, int64_t hi, int64_t count,
+                         const int64_t *point_map, int64_t m, const int64_t *remap, int64_t *ptr_out,
+                         int64_t *seg_out);
 
 /* Stage arithmetic of convect_geometry over count doubles (enum ofw_stage; b, c, d may be NULL where unused). */
 int ofw_stage_combine(ofw_ctx *ctx, int mode, int64_t count, double h, const double *x, const double *a,

@@ -50,7 +50,7 @@ SIGNATURES = {
                                    ctypes.POINTER(_c_i64)]),
     "ofw_incident_pairs": (_c_i, [_c_p, _c_i64, _c_p, _c_i64, _c_p, _c_i64, _c_p, _c_i64, ctypes.POINTER(_c_i64)]),
     "ofw_network_segments": (_c_i, [_c_p, _c_i64, _c_p, _c_i64, _c_p, _c_p, _c_p, _c_p]),
-    "ofw_exclusion_subset": (_c_i, [_c_p, _c_i64, _c_p, _c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p]),
+    "ofw_exclusion_subset": (_c_i, [_c_p, _c_i64, _c_p, _c_i64, _c_i64, _c_i64, _c_p, _c_i64, _c_p, _c_p, _c_p]),
     "ofw_stage_combine": (_c_i, [_c_p, _c_i, _c_i64, _c_d, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p]),
 }
 STAGE_STEP, STAGE_HALF, STAGE_HEUN, STAGE_RK4, STAGE_ADD = 0, 1, 2, 3, 4

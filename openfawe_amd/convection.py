@@ -77,14 +77,15 @@ def network_segments(nodes, edges):
     return starts, directions, lengths
 
 
-def exclusion_subset(pairs, lo, hi, m, remap=None):
-    """Exclusion CSR (ptr (hi-lo+1), seg) of the pairs whose point lies in [lo, hi), points shifted by -lo."""
+def exclusion_subset(pairs, lo, hi, m, remap=None, point_map=None, count=None):
+    """Exclusion CSR (ptr (count+1), seg) of the pairs whose point lies in [lo, hi): points shifted by -lo, or
+    mapped through ``point_map`` (hi - lo entries, negative = dropped) onto ``count`` output points."""
     torch = _torch()
-    count = int(hi) - int(lo)
+    count = int(hi) - int(lo) if count is None else int(count)
     ptr = torch.empty(count + 1, dtype=torch.int64, device=pairs.device)
     seg = torch.empty(max(int(pairs.shape[0]), 1), dtype=torch.int64, device=pairs.device)
-    status = _lib.load().ofw_exclusion_subset(_ctx(), int(pairs.shape[0]), _ptr(pairs), int(lo), int(hi), int(m),
-                                              _ptr(remap), _ptr(ptr), _ptr(seg))
+    status = _lib.load().ofw_exclusion_subset(_ctx(), int(pairs.shape[0]), _ptr(pairs), int(lo), int(hi), count,
+                                              _ptr(point_map), int(m), _ptr(remap), _ptr(ptr), _ptr(seg))
     _lib.check(status, "ofw_exclusion_subset")
     return ptr, seg
 
@@ -100,26 +101,36 @@ def stage(mode, h, x, a, b=None, c=None, d=None, out=None):
     return out
 
 
-def _induced(points, geometry, gamma, radius, ptr, seg, singularity, settings):
-    """segments_velocity's sum and error rules (vortex.py:466-497) on device tensors."""
-    torch = _torch()
+def _sum(points, geometry, gamma, radius, ptr, seg, singularity, settings):
+    """The engine's sum on device tensors: (out, flags, flag_seg or None)."""
     starts, directions, lengths = geometry
     if settings.backend == "hip_tree":
         from .tree import hip_tree_sum_device
         out, flags, _, _ = hip_tree_sum_device(points, starts, directions, lengths, gamma, radius, settings.model,
                                                ptr, seg, singularity, settings.tree, settings.hip)
-        flag_seg = None
-    else:
-        out, flags, flag_seg = hip_sum_device(points, starts, directions, lengths, gamma, radius, settings.model,
-                                              ptr, seg, singularity, settings.hip)
+        return out, flags, None
+    return hip_sum_device(points, starts, directions, lengths, gamma, radius, settings.model, ptr, seg, singularity,
+                          settings.hip)
+
+
+def _check(out, flags, flag_seg):
+    """segments_velocity's error rules (vortex.py:488-497) for one sum; one host synchronisation."""
+    torch = _torch()
+    if not bool(torch.logical_or(flags.any(), torch.logical_not(torch.isfinite(out).all())).item()):
+        return
     if bool(flags.any()):
         stats = dict(flags=flags.cpu().numpy())
         if flag_seg is not None:
             stats["flag_seg"] = flag_seg.cpu().numpy()
         Statistics.last = stats
         _raise_for_flags(stats)
-    if not bool(torch.isfinite(out).all()):
-        raise ValueError("velocity exceeds finite float64 range")
+    raise ValueError("velocity exceeds finite float64 range")
+
+
+def _induced(points, geometry, gamma, radius, ptr, seg, singularity, settings):
+    """segments_velocity's sum and error rules (vortex.py:466-497) on device tensors."""
+    out, flags, flag_seg = _sum(points, geometry, gamma, radius, ptr, seg, singularity, settings)
+    _check(out, flags, flag_seg)
     return out
 
 
@@ -177,11 +188,45 @@ def network_convection(nodes, edges, gamma, first, radii, settings, ambient, rin
         part = velocity[lo:hi]
         stage(_lib.STAGE_ADD, 0.0, part, out, out=part)
 
+    def rows_and_rings(rows, ring_lo, ring_hi):
+        """Rows [0, rows) with 'raise' and ring nodes [ring_lo, ring_hi) with 'zero'.  With one set of cores both
+        go through one sum (one tree): a singular pair adds nothing in either mode, only its flag differs, so every
+        target's velocity is the one of its own call.  The rows' flags raise as in their own call; when any ring
+        target is flagged, the ring targets are summed again in their own 'zero' call."""
+        if ring_radii is not None or ring_lo >= ring_hi or rows == 0:
+            add(0, rows, radii, "raise")
+            if ring_lo < ring_hi:
+                add(ring_lo, ring_hi, target_radii, "zero")
+            return
+        rings = ring_hi - ring_lo
+        point_map = torch.full((ring_hi,), -1, dtype=torch.int64, device=nodes.device)
+        point_map[:rows] = torch.arange(rows, dtype=torch.int64, device=nodes.device)
+        point_map[ring_lo:] = torch.arange(rows, rows + rings, dtype=torch.int64, device=nodes.device)
+        ptr, seg = exclusion_subset(pairs, 0, ring_hi, m, point_map=point_map, count=rows + rings)
+        targets = torch.cat((points[:rows], points[ring_lo:ring_hi]))
+        out, flags, flag_seg = _sum(targets, geometry, gamma, radii, ptr, seg, "raise", settings)
+        _check(out[:rows], flags[:rows], None if flag_seg is None else flag_seg[:rows])
+        if bool(flags[rows:].any().item()):
+            ring_out = None
+            add(ring_lo, ring_hi, target_radii, "zero")
+        else:
+            ring_out = out[rows:]
+            _check(ring_out, flags[rows:], None)
+        part = velocity[:rows]
+        stage(_lib.STAGE_ADD, 0.0, part, out[:rows], out=part)
+        if ring_out is not None:
+            part = velocity[ring_lo:ring_hi]
+            stage(_lib.STAGE_ADD, 0.0, part, ring_out.contiguous(), out=part)
+
     limit = settings.free_convection_rows
     if limit is not None and limit * first < rows_end:
         free = limit * first
-        add(0, free, radii, "raise")
-        if settings.frozen_rows_drift == "ambient_rings" and ring_first is not None and ring_first < n:
+        frozen_rings = settings.frozen_rows_drift == "ambient_rings" and ring_first is not None and ring_first < n
+        if frozen_rings:                          # the reference's order: rows, frozen rows, ring nodes
+            add(0, free, radii, "raise")
+        else:
+            rows_and_rings(free, rows_end, ring_end)
+        if frozen_rings:
             ring_edge = (edges >= int(ring_first)).all(dim=1)
             sub = torch.nonzero(ring_edge).reshape(-1)
             if sub.numel():
@@ -193,15 +238,13 @@ def network_convection(nodes, edges, gamma, first, radii, settings, ambient, rin
                                ptr, seg, "raise", settings)
                 part = velocity[free:rows_end]
                 stage(_lib.STAGE_ADD, 0.0, part, out, out=part)
-        if rows_end < ring_end:
-            add(rows_end, ring_end, target_radii, "zero")
+            if rows_end < ring_end:
+                add(rows_end, ring_end, target_radii, "zero")
     elif ring_first is None or ring_first >= n:
         add(0, count, radii, "raise")
     else:
         split = int(ring_first) - first
-        add(0, split, radii, "raise")
-        if split < ring_end:
-            add(split, ring_end, target_radii, "zero")
+        rows_and_rings(split, split, ring_end)
     return velocity
 
 

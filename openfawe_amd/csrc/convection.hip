@@ -4,8 +4,9 @@
 //     network: starts = nodes[edges[:,0]], delta = nodes[edges[:,1]] - starts, L = hypot(hypot(dx,dy),dz),
 //     directions = delta / L, with the zero-length and index-range errors;
 //   * ofw_exclusion_subset -- the exclusion CSR (vortex_numba.py:180-243: per point, the sorted unique excluded
-//     segments) of the pairs whose point lies in [lo, hi), shifted by -lo, optionally through a segment remap
-//     (the ring-edge subset of wake.py:453-462), from an incident-pair list;
+//     segments) of the pairs whose point lies in [lo, hi), shifted by -lo or mapped through a point map (several
+//     target ranges evaluated in one sum), optionally through a segment remap (the ring-edge subset of
+//     wake.py:453-462), from an incident-pair list;
 //   * ofw_stage_combine -- the RK stage arithmetic of wake.py:604-614 and the stage bound of wake.py:596-598, in
 //     NumPy's evaluation order (left to right, every operation rounded on its own).
 #include <climits>
@@ -53,16 +54,22 @@ __global__ void k_network_segments(int m, int64_t n, const double* __restrict__ 
 // pkey = point - lo for a kept pair (point in [lo, hi) and, with a remap, remap[segment] >= 0), else hi - lo;
 // seg = the (remapped) segment, -1 when dropped; the first sort key is the segment.  status bit 0: a pair names a
 // segment outside [0, m).
-__global__ void k_exclusion_keys(int npairs, const int64_t* __restrict__ pairs, int64_t lo, int64_t hi, int64_t m,
-                                 const int64_t* __restrict__ remap, int32_t* __restrict__ key,
-                                 int32_t* __restrict__ val, int32_t* __restrict__ pkey, int32_t* __restrict__ seg,
-                                 int32_t* __restrict__ status) {
-  const int count = (int)(hi - lo);
+__global__ void k_exclusion_keys(int npairs, const int64_t* __restrict__ pairs, int64_t lo, int64_t hi, int count,
+                                 const int64_t* __restrict__ point_map, int64_t m, const int64_t* __restrict__ remap,
+                                 int32_t* __restrict__ key, int32_t* __restrict__ val, int32_t* __restrict__ pkey,
+                                 int32_t* __restrict__ seg, int32_t* __restrict__ status) {
   bool bad = false;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < npairs; i += gridDim.x * blockDim.x) {
     const int64_t p = pairs[2 * (size_t)i];
     int64_t s = pairs[2 * (size_t)i + 1];
-    const bool inside = p >= lo && p < hi;
+    int64_t q = p - lo;
+    bool inside = p >= lo && p < hi;
+    if (inside && point_map) q = point_map[q];
+    if (inside && q >= count) {
+      bad = true;
+      q = -1;
+    }
+    inside = inside && q >= 0;
     if (inside && (s < 0 || s >= m)) {
       bad = true;
       s = -1;
@@ -70,7 +77,7 @@ __global__ void k_exclusion_keys(int npairs, const int64_t* __restrict__ pairs, 
       s = remap[s];
     }
     const bool keep = inside && s >= 0;
-    pkey[i] = keep ? (int32_t)(p - lo) : count;
+    pkey[i] = keep ? (int32_t)q : count;
     seg[i] = keep ? (int32_t)s : -1;
     key[i] = keep ? (int32_t)s : 0;
     val[i] = i;
@@ -177,15 +184,17 @@ extern "C" int ofw_network_segments(ofw_ctx* x, int64_t n, const double* nodes, 
 }
 
 extern "C" int ofw_exclusion_subset(ofw_ctx* x, int64_t npairs, const int64_t* pairs, int64_t lo, int64_t hi,
-                                    int64_t m, const int64_t* remap, int64_t* ptr_out, int64_t* seg_out) {
+                                    int64_t count64, const int64_t* point_map, int64_t m, const int64_t* remap,
+                                    int64_t* ptr_out, int64_t* seg_out) {
   if (!x || npairs < 0 || npairs > ((int64_t)1 << 30) || lo < 0 || hi < lo || hi - lo > ((int64_t)1 << 30) ||
-      m < 0 || m > ((int64_t)1 << 30) || !ptr_out || (npairs > 0 && (!pairs || !seg_out))) {
+      count64 < 0 || count64 > ((int64_t)1 << 30) || (!point_map && count64 != hi - lo) || m < 0 ||
+      m > ((int64_t)1 << 30) || !ptr_out || (npairs > 0 && (!pairs || !seg_out))) {
     ofw::set_error("ofw_exclusion_subset: invalid arguments");
     return OFW_ERR_ARG;
   }
   ofw::Ctx& c = x->c;
   c.nev = 0;
-  const int P = (int)npairs, count = (int)(hi - lo);
+  const int P = (int)npairs, count = (int)count64;
   const int hist_n = std::max(ofw::radix_hist_size(P), 1);
   const int scan_max = std::max(std::max(P, count + 1), hist_n);
   const size_t spart = ((size_t)(scan_max + 4095) / 4096 + 8) * 4;
@@ -221,8 +230,8 @@ extern "C" int ofw_exclusion_subset(ofw_ctx* x, int64_t npairs, const int64_t* p
   hipLaunchKernelGGL(ofw::k_conv_fill, dim3(1), dim3(64), 0, c.stream, 1, total, 0);
   hipLaunchKernelGGL(ofw::k_conv_fill, dim3(1), dim3(64), 0, c.stream, 1, status, 0);
   if (P > 0) {
-    hipLaunchKernelGGL(ofw::k_exclusion_keys, blocks(P), dim3(256), 0, c.stream, P, pairs, lo, hi, m, remap, key[0],
-                       val[0], pkey, seg, status);
+    hipLaunchKernelGGL(ofw::k_exclusion_keys, blocks(P), dim3(256), 0, c.stream, P, pairs, lo, hi, count,
+                       point_map, m, remap, key[0], val[0], pkey, seg, status);
     // LSD: by segment, then (stably) by point
     int cur = ofw::radix_sort(c, P, ofw::radix_passes(std::max<int64_t>(m - 1, 0)), key, val, hist, hist_off, part);
     hipLaunchKernelGGL(ofw::k_gather_keys, blocks(P), dim3(256), 0, c.stream, P, val[cur], pkey, key[cur]);
@@ -243,7 +252,7 @@ extern "C" int ofw_exclusion_subset(ofw_ctx* x, int64_t npairs, const int64_t* p
     OFW_HIP(hipMemcpyAsync(&host, status, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
     OFW_HIP(hipStreamSynchronize(c.stream));
     if (host) {
-      ofw::set_error("exclusion index out of range");
+      ofw::set_error("exclusion index out of range");  // a segment outside [0, m) or a map beyond count
       return OFW_ERR_ARG;
     }
   }

@@ -39,8 +39,20 @@ def test_exclusion_subset_matches_the_host_csr():
     want_ptr, want_seg = V.exclusion_csr(190, sub)
     np.testing.assert_array_equal(ptr.cpu().numpy(), want_ptr)
     np.testing.assert_array_equal(seg.cpu().numpy()[:want_ptr[-1]], want_seg)
+    # two target ranges packed into one sum: [0, 100) -> [0, 100), [200, 250) -> [100, 150)
+    point_map = np.full(250, -1)
+    point_map[:100] = np.arange(100)
+    point_map[200:] = np.arange(100, 150)
+    ptr, seg = CV.exclusion_subset(dev(pairs), 0, 250, 700, point_map=dev(point_map), count=150)
+    keep = (pairs[:, 0] < 250) & (point_map[np.minimum(pairs[:, 0], 249)] >= 0)
+    sub = np.column_stack([point_map[pairs[keep, 0]], pairs[keep, 1]])
+    want_ptr, want_seg = V.exclusion_csr(150, sub)
+    np.testing.assert_array_equal(ptr.cpu().numpy(), want_ptr)
+    np.testing.assert_array_equal(seg.cpu().numpy()[:want_ptr[-1]], want_seg)
     with pytest.raises(ValueError):
         CV.exclusion_subset(dev(pairs), 0, 300, 699)                  # a segment id beyond m
+    with pytest.raises(ValueError):
+        CV.exclusion_subset(dev(pairs), 0, 250, 700, point_map=dev(point_map), count=149)   # map beyond count
 
 
 def test_stage_arithmetic_is_numpys_bit_for_bit():
