@@ -193,6 +193,17 @@ int ofw_direct_vjp(ofw_ctx *ctx, int64_t n, const double *points, int64_t m, con
 int ofw_pack_ring_edges(ofw_ctx *ctx, int64_t depth, int64_t width, int64_t panels, const int64_t *panel_edges,
                         const double *strengths, int64_t *edges, double *gamma, int64_t capacity, int64_t *count_out);
 
+/* Material core radius of every wake edge (replaces cores.edge_radii, cores.py:129-149).  Node ids (edges (m,2)
+ * plus `offset`) are row * width + column over the rows bound, trailing edge, free rows.  A spanwise edge of free
+ * row r >= 2 takes span[(r-2) * panels + j] for the (last) panel j of panel_edges (panels,2) joining its columns; a
+ * streamwise edge between rows r and r+1 at column c with r >= 1 takes stream[(r-1) * width + c]; every other
+ * bound or trailing-edge edge takes `initial`.  span (span_rows, panels) and stream (stream_rows, width) are the
+ * WakeCores arrays.  OFW_ERR_ARG for an edge that is neither spanwise nor streamwise, a spanwise edge without a
+ * panel, or a row beyond the core arrays.  Synchronises the stream once. */
+int ofw_edge_radii(ofw_ctx *ctx, int64_t m, const int64_t *edges, int64_t offset, int64_t width, int64_t panels,
+                   const int64_t *panel_edges, int64_t span_rows, const double *span, int64_t stream_rows,
+                   const double *stream, double initial, double *out);
+
 /* Incident-exclusion pairs of a wake network (replaces network_numba.incident_pairs, network_numba.py:242-253,
  * and the loop of wake.py:427-438): for the free points nodes[first:] (n,3), every (point, segment) pair in
  * which the point's coordinates equal an end of segment s (edges (m,2)) bit for bit, in segment order, the

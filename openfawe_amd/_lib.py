@@ -48,6 +48,8 @@ SIGNATURES = {
                               _c_p, _c_p, _c_p]),
     "ofw_pack_ring_edges": (_c_i, [_c_p, _c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_i64,
                                    ctypes.POINTER(_c_i64)]),
+    "ofw_edge_radii": (_c_i, [_c_p, _c_i64, _c_p, _c_i64, _c_i64, _c_i64, _c_p, _c_i64, _c_p, _c_i64, _c_p, _c_d,
+                              _c_p]),
     "ofw_incident_pairs": (_c_i, [_c_p, _c_i64, _c_p, _c_i64, _c_p, _c_i64, _c_p, _c_i64, ctypes.POINTER(_c_i64)]),
     "ofw_network_segments": (_c_i, [_c_p, _c_i64, _c_p, _c_i64, _c_p, _c_p, _c_p, _c_p]),
     "ofw_exclusion_subset": (_c_i, [_c_p, _c_i64, _c_p, _c_i64, _c_i64, _c_i64, _c_p, _c_i64, _c_p, _c_p, _c_p]),

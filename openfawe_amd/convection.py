@@ -294,3 +294,36 @@ def convect_geometry(bound, points, edges, gamma, radii, settings, *, inflow, ti
             candidate = stage(_lib.STAGE_RK4, dt, points, k1, k2, k3, k4)
         points = candidate
     return points
+
+
+# ------------------------------------------------------------------ rotor rings
+def ring_self_velocity(nodes, edges, gamma, radius, settings, singularity="zero"):
+    """RingWake.self_velocity (rotor/ring_wake.py:183-199) on device tensors: the velocity of every ring node from
+    all ring segments, the segments incident to a node excluded (incident_pairs with first = 0).  Identical rings
+    released from one rim stack onto each other (coincident nodes on each other's filament ends); with a core,
+    'zero' is the physical limit there, 'raise' reports such events."""
+    torch = _torch()
+    n, m = int(nodes.shape[0]), int(edges.shape[0])
+    if m == 0:
+        return torch.zeros((n, 3), dtype=torch.float64, device=nodes.device)
+    nodes, edges, gamma = nodes.contiguous(), edges.contiguous(), gamma.contiguous()
+    pairs = NET.incident_pairs_device(nodes, edges, 0)
+    geometry = network_segments(nodes, edges)
+    ptr, seg = exclusion_subset(pairs, 0, n, m)
+    radius = _per_edge(torch, radius, m, nodes.device, "core_radius")
+    return _induced(nodes, geometry, gamma, radius, ptr, seg, singularity, settings)
+
+
+def ring_induction(points, nodes, edges, gamma, radius, model):
+    """wake.ring_induction (wake.py:353-375) for rings without a refined sheet: the rings' velocity at
+    lifting-surface points, by the FP64 direct sum (float64 by contract near the surface core, wake.py:42-54)."""
+    torch = _torch()
+    k, m = int(points.shape[0]), int(edges.shape[0])
+    if m == 0 or k == 0:
+        return torch.zeros((k, 3), dtype=torch.float64, device=points.device)
+    geometry = network_segments(nodes.contiguous(), edges.contiguous())
+    radius = _per_edge(torch, radius, m, points.device, "core_radius")
+    ptr = torch.zeros(k + 1, dtype=torch.int64, device=points.device)
+    seg = torch.zeros(1, dtype=torch.int64, device=points.device)
+    settings = ConvectionSettings(model=model, hip=HipSettings("fp64"))
+    return _induced(points.contiguous(), geometry, gamma.contiguous(), radius, ptr, seg, "raise", settings)

@@ -496,7 +496,111 @@ int pack_ring_edges(Ctx& c, int64_t depth, int64_t width, int64_t panels, const 
   return OFW_OK;
 }
 
+// ---------------------------------------------------------------------------------------------------------------
+// Material core of every wake edge (cores.py:129-149, edge_radii).  Node id = row * width + column over the rows
+// (bound, trailing edge, free rows...).  A spanwise edge (same row r) of a free row (r >= 2) takes
+// span[r - 2, panel], the panel whose columns are the edge's (the last such panel, as the reference's dictionary
+// keeps the last); a streamwise edge (same column c, rows r and r + 1) with r >= 1 takes stream[r - 1, c]; the
+// bound and trailing-edge edges keep `initial`.  status bit 0: an edge that is neither; bit 1: a spanwise edge
+// without a panel; bit 2: a row beyond the core arrays.
+
+__global__ void k_panel_table(int panels, int64_t width, const int64_t* __restrict__ panel_edges,
+                              int32_t* __restrict__ table) {
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < panels; j += gridDim.x * blockDim.x) {
+    const int64_t a = panel_edges[2 * j], b = panel_edges[2 * j + 1];
+    if (a < 0 || b < 0 || a >= width || b >= width) continue;
+    atomicMax(&table[(a < b ? a : b) * width + (a < b ? b : a)], j);
+  }
+}
+
+__global__ void k_edge_radii(int m, const int64_t* __restrict__ edges, int64_t offset, int64_t width, int panels,
+                             const int32_t* __restrict__ table, int64_t span_rows, const double* __restrict__ span,
+                             int64_t stream_rows, const double* __restrict__ stream, double initial,
+                             double* __restrict__ out, int32_t* __restrict__ status) {
+  int bad = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+    const int64_t e0 = edges[2 * (size_t)i] + offset, e1 = edges[2 * (size_t)i + 1] + offset;
+    const int64_t ra = e0 / width, a = e0 % width, rb = e1 / width, b = e1 % width;
+    double r = initial;
+    if (e0 < 0 || e1 < 0) {
+      bad |= 1;
+    } else if (ra == rb) {
+      if (ra >= 2) {
+        const int j = table[(a < b ? a : b) * width + (a < b ? b : a)];
+        if (j < 0) {
+          bad |= 2;
+        } else if (ra - 2 >= span_rows) {
+          bad |= 4;
+        } else {
+          r = span[(ra - 2) * panels + j];
+        }
+      }
+    } else if (a == b && (rb - ra == 1 || ra - rb == 1)) {
+      const int64_t row = ra < rb ? ra : rb;
+      if (row >= 1) {
+        if (row - 1 >= stream_rows)
+          bad |= 4;
+        else
+          r = stream[(row - 1) * width + a];
+      }
+    } else {
+      bad |= 1;
+    }
+    out[i] = r;
+  }
+  if (__builtin_amdgcn_ballot_w64(bad != 0) != 0) {
+    for (int o = 32; o > 0; o >>= 1) bad |= __shfl_xor(bad, o);
+    if ((threadIdx.x & 63) == 0) atomicOr(status, bad);
+  }
+}
+
 }  // namespace ofw
+
+extern "C" int ofw_edge_radii(ofw_ctx* x, int64_t m, const int64_t* edges, int64_t offset, int64_t width,
+                              int64_t panels, const int64_t* panel_edges, int64_t span_rows, const double* span,
+                              int64_t stream_rows, const double* stream, double initial, double* out) {
+  if (!x || m < 0 || m > ((int64_t)1 << 30) || width < 1 || width > 46340 || panels < 0 || panels > INT_MAX ||
+      span_rows < 0 || stream_rows < 0 || (m > 0 && (!edges || !out)) || (panels > 0 && !panel_edges) ||
+      (span_rows > 0 && !span) || (stream_rows > 0 && !stream)) {
+    ofw::set_error("ofw_edge_radii: invalid arguments");
+    return OFW_ERR_ARG;
+  }
+  ofw::Ctx& c = x->c;
+  c.nev = 0;
+  if (m == 0) return OFW_OK;
+  const size_t cells = (size_t)(width * width);
+  OFW_HIP(c.tree[9].reserve(ofw::al(cells * 4) + ofw::al(4)));
+  int32_t* table = (int32_t*)c.tree[9].ptr;
+  int32_t* status = (int32_t*)((char*)c.tree[9].ptr + ofw::al(cells * 4));
+  const int grid = c.cu_count * 8;
+  auto blocks = [&](int64_t items) {
+    return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(grid, (items + 255) / 256)));
+  };
+  hipLaunchKernelGGL(ofw::k_net_fill, blocks((int64_t)cells), dim3(256), 0, c.stream, (int)cells, table, -1);
+  hipLaunchKernelGGL(ofw::k_net_fill, dim3(1), dim3(64), 0, c.stream, 1, status, 0);
+  if (panels > 0)
+    hipLaunchKernelGGL(ofw::k_panel_table, blocks(panels), dim3(256), 0, c.stream, (int)panels, width, panel_edges,
+                       table);
+  hipLaunchKernelGGL(ofw::k_edge_radii, blocks(m), dim3(256), 0, c.stream, (int)m, edges, offset, width,
+                     (int)panels, table, span_rows, span, stream_rows, stream, initial, out, status);
+  OFW_HIP(hipGetLastError());
+  int32_t host = 0;
+  OFW_HIP(hipMemcpyAsync(&host, status, 4, hipMemcpyDeviceToHost, c.stream));
+  OFW_HIP(hipStreamSynchronize(c.stream));
+  if (host & 1) {
+    ofw::set_error("edge is not a spanwise or streamwise wake segment");
+    return OFW_ERR_ARG;
+  }
+  if (host & 2) {
+    ofw::set_error("spanwise wake edge without a panel");
+    return OFW_ERR_ARG;
+  }
+  if (host & 4) {
+    ofw::set_error("wake edge beyond the core history");
+    return OFW_ERR_ARG;
+  }
+  return OFW_OK;
+}
 
 extern "C" int ofw_pack_ring_edges(ofw_ctx* x, int64_t depth, int64_t width, int64_t panels,
                                    const int64_t* panel_edges, const double* strengths, int64_t* edges,

@@ -161,3 +161,43 @@ def test_tree_engine_and_errors():
     out = CV.network_convection(dev(bad), dev(edges), dev(gamma), first, 0.15, fp64, dev(amb),
                                 ring_first=ring_first)
     assert bool(out.isfinite().all())
+
+
+def stacked_rings(copies=4, segments=24, seed=0):
+    """Rings released from one rim: ``copies`` identical rings (a starting-ring cluster, ring_wake.py:186-191),
+    then a few displaced ones; every ring closed."""
+    rng = np.random.default_rng(seed)
+    phi = np.linspace(0.0, 2 * np.pi, segments, endpoint=False)
+    rim = np.stack([np.zeros(segments), 1.2 * np.cos(phi), 1.2 * np.sin(phi)], axis=1)
+    nodes, edges, gamma = [], [], []
+    for r in range(copies + 3):
+        offset = 0.0 if r < copies else 0.3 * (r - copies + 1)
+        base = len(nodes) * segments
+        nodes.append(rim + [-offset, 0.0, 0.0] + (0 if r < copies else 0.01 * rng.normal(size=rim.shape)))
+        edges.append(np.stack([base + np.arange(segments), base + (np.arange(segments) + 1) % segments], axis=1))
+        gamma.append(np.full(segments, 0.7 + 0.1 * r))
+    return np.concatenate(nodes), np.concatenate(edges), np.concatenate(gamma)
+
+
+def test_ring_self_velocity_and_ring_induction():
+    from oracle import network_np as NP
+    nodes, edges, gamma = stacked_rings()
+    for model in (2, 1):
+        print(f"ring self velocity: model {model}", flush=True)
+        want = C.segments_velocity(nodes, nodes, edges, gamma, 0.3, model, NP.incident_pairs(nodes, edges, 0),
+                                   "zero")
+        got = CV.ring_self_velocity(dev(nodes), dev(edges), dev(gamma), 0.3, CV.ConvectionSettings(model=model))
+        np.testing.assert_allclose(got.cpu().numpy(), want, rtol=0, atol=1e-12 * np.abs(want).max())
+    # stacked rings meet only at segment ends, which the incident pairs exclude: 'raise' passes there
+    CV.ring_self_velocity(dev(nodes), dev(edges), dev(gamma), 0.3, CV.ConvectionSettings(model=2), "raise")
+    # a node on the inside of another ring's filament: 'zero' gives the regularised limit, 'raise' reports it
+    bad = nodes.copy()
+    bad[-1] = 0.5 * (nodes[edges[0, 0]] + nodes[edges[0, 1]])
+    out = CV.ring_self_velocity(dev(bad), dev(edges), dev(gamma), 0.3, CV.ConvectionSettings(model=2))
+    assert bool(out.isfinite().all())
+    with pytest.raises(V.FilamentSingularity):
+        CV.ring_self_velocity(dev(bad), dev(edges), dev(gamma), 0.3, CV.ConvectionSettings(model=2), "raise")
+    points = np.stack([np.full(30, 0.4), np.linspace(-2.0, 2.0, 30), np.zeros(30)], axis=1)
+    want = C.segments_velocity(points, nodes, edges, gamma, 0.1, 2, np.zeros((0, 2), np.int64), "raise")
+    got = CV.ring_induction(dev(points), dev(nodes), dev(edges), dev(gamma), 0.1, 2).cpu().numpy()
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * np.abs(want).max())
