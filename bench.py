@@ -207,7 +207,7 @@ def network_bench(case, steps, cpu=True):
     return result
 
 
-def convection_bench(case, rows, steps, tree, settings):
+def convection_bench(case, rows, steps, tree, settings, cpu_threads=None):
     """One free-wake geometry step (wake.py:586-616) of the loop wake on the GPU: row 0 is the bound row, the newest
     187 rows are free targets (free_convection_rows=187; 20,570 points), the older rows frozen, the rotor helices ring
     nodes ('zero'); every stage runs the incident pairs, the segment geometry, the exclusion subsets and the tree sums
@@ -237,6 +237,17 @@ def convection_bench(case, rows, steps, tree, settings):
             run()
         torch.cuda.synchronize()
         result[f"{integrator}_step_ms"] = 1e3 * (time.perf_counter() - t0) / steps
+    if cpu_threads:
+        # the same Euler step by the oracle (the reference's algorithm in C, FP64 tree code, OpenMP over targets)
+        from oracle import convection_np as CO
+        t0 = time.perf_counter()
+        CO.convect_geometry(nodes[:width], nodes[width:], edges, case["gamma"], case["radius"], case["model"],
+                            inflow=[10.0, 0.0, 0.0], time=0.0, dt_s=0.05, integrator="euler", ring_first=ring_first,
+                            free_convection_rows=187, engine=CO.tree_engine(tree.theta, tree.order, tree.leaf_size,
+                                                                             tree.core_clearance, cpu_threads))
+        result["cpu_euler_step_ms"] = 1e3 * (time.perf_counter() - t0)
+        result["cpu"] = (f"oracle/convection_np.py Euler step with the C oracle's tree code (FP64, theta "
+                         f"{tree.theta}), {cpu_threads} threads, {cpu_model()}")
     return result
 
 
@@ -440,7 +451,8 @@ def main():
         result["batch"] = batch_bench(max(3, args.steps // 2))
         if "nodes" in case:
             result["network"] = network_bench(case, max(3, args.steps // 2), cpu=not args.no_cpu)
-            result["convection"] = convection_bench(case, spec["rows"], max(2, args.steps // 5), tree, settings)
+            result["convection"] = convection_bench(case, spec["rows"], max(2, args.steps // 5), tree, settings,
+                                                    None if args.no_cpu else usable_cpus())
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(case, spec, usable_cpus())
     print(json.dumps(result), flush=True)

@@ -54,6 +54,17 @@ enum ofw_stage {         /* RK stage arithmetic of ofw_stage_combine (wake.py:59
   OFW_STAGE_ADD = 4      /* out = x + a                 ambient plus induced velocity */
 };
 
+enum ofw_step_status {          /* bits of a geometry step's status word (ofw_step_run) */
+  OFW_STEP_PAIR_OVERFLOW = 1,   /* the incident pairs exceeded the plan's capacity */
+  OFW_STEP_EDGE_INDEX = 2,      /* an edge names a node outside [0, n) */
+  OFW_STEP_ZERO_LENGTH = 4,     /* a zero-length segment */
+  OFW_STEP_EXCLUSION = 8,       /* an exclusion pair outside its ranges */
+  OFW_STEP_TREE_OVERFLOW = 16,  /* the tree code exceeded a capacity of its plan */
+  OFW_STEP_ROW_FLAG = 32,       /* a wing-row target has a flagged pair (the reference raises) */
+  OFW_STEP_RING_FLAG = 64,      /* a ring target has a flagged pair (its 'zero' call must be repeated) */
+  OFW_STEP_NONFINITE = 128      /* a non-finite velocity */
+};
+
 typedef struct ofw_ctx ofw_ctx;
 
 /* ABI version of the library (OFW_ABI_VERSION). */
@@ -228,15 +239,42 @@ int ofw_network_segments(ofw_ctx *ctx, int64_t n, const double *nodes, int64_t m
  * output point p - lo, or point_map[p - lo] when point_map (hi-lo) is given (negative: dropped; several target
  * ranges packed into one sum), and, when remap (m) is given, with segments mapped through it (negative: dropped).
  * ptr_out (count+1) and seg_out (capacity npairs) receive per output point its sorted unique segments; count must
- * be hi - lo without a point map.  OFW_ERR_ARG ("exclusion index out of range") when a kept pair names a segment
- * outside [0, m) or the map an output point at or beyond count. */
+ * be hi - lo without a point map.  segment_ordered != 0 promises that the kept pairs' (remapped) segments are
+ * non-decreasing in pair order (as ofw_incident_pairs writes them), which saves the sort by segment.  OFW_ERR_ARG
+ * ("exclusion index out of range") when a kept pair names a segment outside [0, m) or the map an output point at
+ * or beyond count. */
 int ofw_exclusion_subset(ofw_ctx *ctx, int64_t npairs, const int64_t *pairs, int64_t lo, int64_t hi, int64_t count,
-                         const int64_t *point_map, int64_t m, const int64_t *remap, int64_t *ptr_out,
-                         int64_t *seg_out);
+                         const int64_t *point_map, int64_t m, const int64_t *remap, int segment_ordered,
+                         int64_t *ptr_out, int64_t *seg_out);
 
 /* Stage arithmetic of convect_geometry over count doubles (enum ofw_stage; b, c, d may be NULL where unused). */
 int ofw_stage_combine(ofw_ctx *ctx, int mode, int64_t count, double h, const double *x, const double *a,
                       const double *b, const double *c, const double *d, double *out);
+
+/* A free-wake geometry step recorded as one graph (no counterpart in the reference, whose convect_geometry,
+ * wake.py:586-616, calls its engines stage by stage).  The plan advances the free points points_in (k,3) by
+ * `substeps` steps of the integrator (0 Euler, 1 Heun, 2 RK4; dt_s / substeps each) into points_out (k,3), with
+ * the stage velocity of _network_convection (wake.py:404-491): the uniform inflow (device, 3 doubles) plus, at the
+ * targets, the tree sum over the network concatenate(bound (first,3), free points) with edges (m,2), gamma, radius
+ * (m), the incident segments excluded; targets are the wing rows [0, rows) and the ring nodes [ring_lo, ring_hi)
+ * of the free points, every other free point keeps the inflow.  The bound row moves by time fraction times
+ * displacement (first,3; NULL: fixed).  Every pointer is kept: a run reads the arrays' current contents.
+ * Creating runs the step twice (measuring the pair and list capacities, then with them) and records it when
+ * use_graph != 0; stats_host (3 int64, may be NULL) receives the status of the last run and the two capacities. */
+typedef struct ofw_step ofw_step;
+int ofw_step_create(ofw_ctx *ctx, ofw_step **plan, int precision, int64_t first, const double *bound,
+                    const double *displacement, int64_t k, const double *points_in, double *points_out, int64_t m,
+                    const int64_t *edges, const double *gamma, const double *radius, int model, int64_t rows,
+                    int64_t ring_lo, int64_t ring_hi, double theta, int expansion_order, int64_t leaf_size,
+                    double clearance, const double *inflow, double time, double dt_s, int substeps, int integrator,
+                    int use_graph, int64_t *stats_host);
+/* Run the step (one graph launch) and read its status block (one synchronisation): status_host[0] the
+ * ofw_step_status bits, [1] the largest incident-pair count, [2] the largest list work-item count.  A capacity bit
+ * asks for a new plan; a flag, zero-length or non-finite bit for the stage-by-stage step, which raises or repeats a
+ * ring call as the reference. */
+int ofw_step_run(ofw_step *plan, int64_t *status_host);
+int ofw_step_timings(ofw_step *plan, float *ms_host);   /* the last run's time on the stream, ms */
+int ofw_step_destroy(ofw_step *plan);
 
 #ifdef __cplusplus
 }

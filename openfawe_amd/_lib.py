@@ -52,10 +52,19 @@ SIGNATURES = {
                               _c_p]),
     "ofw_incident_pairs": (_c_i, [_c_p, _c_i64, _c_p, _c_i64, _c_p, _c_i64, _c_p, _c_i64, ctypes.POINTER(_c_i64)]),
     "ofw_network_segments": (_c_i, [_c_p, _c_i64, _c_p, _c_i64, _c_p, _c_p, _c_p, _c_p]),
-    "ofw_exclusion_subset": (_c_i, [_c_p, _c_i64, _c_p, _c_i64, _c_i64, _c_i64, _c_p, _c_i64, _c_p, _c_p, _c_p]),
+    "ofw_exclusion_subset": (_c_i, [_c_p, _c_i64, _c_p, _c_i64, _c_i64, _c_i64, _c_p, _c_i64, _c_p, _c_i, _c_p,
+                                    _c_p]),
     "ofw_stage_combine": (_c_i, [_c_p, _c_i, _c_i64, _c_d, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p]),
+    "ofw_step_create": (_c_i, [_c_p, ctypes.POINTER(_c_p), _c_i, _c_i64, _c_p, _c_p, _c_i64, _c_p, _c_p, _c_i64, _c_p,
+                               _c_p, _c_p, _c_i, _c_i64, _c_i64, _c_i64, _c_d, _c_i, _c_i64, _c_d, _c_p, _c_d, _c_d,
+                               _c_i, _c_i, _c_i, ctypes.POINTER(_c_i64)]),
+    "ofw_step_run": (_c_i, [_c_p, ctypes.POINTER(_c_i64)]),
+    "ofw_step_timings": (_c_i, [_c_p, ctypes.POINTER(ctypes.c_float)]),
+    "ofw_step_destroy": (_c_i, [_c_p]),
 }
 STAGE_STEP, STAGE_HALF, STAGE_HEUN, STAGE_RK4, STAGE_ADD = 0, 1, 2, 3, 4
+(STEP_PAIR_OVERFLOW, STEP_EDGE_INDEX, STEP_ZERO_LENGTH, STEP_EXCLUSION, STEP_TREE_OVERFLOW, STEP_ROW_FLAG,
+ STEP_RING_FLAG, STEP_NONFINITE) = (1, 2, 4, 8, 16, 32, 64, 128)
 
 _lib = None
 

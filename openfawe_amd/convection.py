@@ -77,15 +77,17 @@ def network_segments(nodes, edges):
     return starts, directions, lengths
 
 
-def exclusion_subset(pairs, lo, hi, m, remap=None, point_map=None, count=None):
+def exclusion_subset(pairs, lo, hi, m, remap=None, point_map=None, count=None, segment_ordered=False):
     """Exclusion CSR (ptr (count+1), seg) of the pairs whose point lies in [lo, hi): points shifted by -lo, or
-    mapped through ``point_map`` (hi - lo entries, negative = dropped) onto ``count`` output points."""
+    mapped through ``point_map`` (hi - lo entries, negative = dropped) onto ``count`` output points.
+    ``segment_ordered``: the pairs' segments are non-decreasing (incident pairs), which saves a sort."""
     torch = _torch()
     count = int(hi) - int(lo) if count is None else int(count)
     ptr = torch.empty(count + 1, dtype=torch.int64, device=pairs.device)
     seg = torch.empty(max(int(pairs.shape[0]), 1), dtype=torch.int64, device=pairs.device)
     status = _lib.load().ofw_exclusion_subset(_ctx(), int(pairs.shape[0]), _ptr(pairs), int(lo), int(hi), count,
-                                              _ptr(point_map), int(m), _ptr(remap), _ptr(ptr), _ptr(seg))
+                                              _ptr(point_map), int(m), _ptr(remap), int(bool(segment_ordered)),
+                                              _ptr(ptr), _ptr(seg))
     _lib.check(status, "ofw_exclusion_subset")
     return ptr, seg
 
@@ -183,7 +185,7 @@ def network_convection(nodes, edges, gamma, first, radii, settings, ambient, rin
     ring_end = count if ring_free is None else min(count, rows_end + int(ring_free))
 
     def add(lo, hi, radius, singularity):
-        ptr, seg = exclusion_subset(pairs, lo, hi, m)
+        ptr, seg = exclusion_subset(pairs, lo, hi, m, segment_ordered=True)
         out = _induced(points[lo:hi], geometry, gamma, radius, ptr, seg, singularity, settings)
         part = velocity[lo:hi]
         stage(_lib.STAGE_ADD, 0.0, part, out, out=part)
@@ -202,7 +204,8 @@ def network_convection(nodes, edges, gamma, first, radii, settings, ambient, rin
         point_map = torch.full((ring_hi,), -1, dtype=torch.int64, device=nodes.device)
         point_map[:rows] = torch.arange(rows, dtype=torch.int64, device=nodes.device)
         point_map[ring_lo:] = torch.arange(rows, rows + rings, dtype=torch.int64, device=nodes.device)
-        ptr, seg = exclusion_subset(pairs, 0, ring_hi, m, point_map=point_map, count=rows + rings)
+        ptr, seg = exclusion_subset(pairs, 0, ring_hi, m, point_map=point_map, count=rows + rings,
+                                    segment_ordered=True)
         targets = torch.cat((points[:rows], points[ring_lo:ring_hi]))
         out, flags, flag_seg = _sum(targets, geometry, gamma, radii, ptr, seg, "raise", settings)
         _check(out[:rows], flags[:rows], None if flag_seg is None else flag_seg[:rows])
@@ -232,7 +235,7 @@ def network_convection(nodes, edges, gamma, first, radii, settings, ambient, rin
             if sub.numel():
                 remap = torch.full((m,), -1, dtype=torch.int64, device=nodes.device)
                 remap[sub] = torch.arange(sub.numel(), dtype=torch.int64, device=nodes.device)
-                ptr, seg = exclusion_subset(pairs, free, rows_end, m, remap)
+                ptr, seg = exclusion_subset(pairs, free, rows_end, m, remap, segment_ordered=True)
                 sub_geometry = tuple(g[sub].contiguous() for g in geometry)
                 out = _induced(points[free:rows_end], sub_geometry, gamma[sub].contiguous(), radii[sub].contiguous(),
                                ptr, seg, "raise", settings)
@@ -309,7 +312,7 @@ def ring_self_velocity(nodes, edges, gamma, radius, settings, singularity="zero"
     nodes, edges, gamma = nodes.contiguous(), edges.contiguous(), gamma.contiguous()
     pairs = NET.incident_pairs_device(nodes, edges, 0)
     geometry = network_segments(nodes, edges)
-    ptr, seg = exclusion_subset(pairs, 0, n, m)
+    ptr, seg = exclusion_subset(pairs, 0, n, m, segment_ordered=True)
     radius = _per_edge(torch, radius, m, nodes.device, "core_radius")
     return _induced(nodes, geometry, gamma, radius, ptr, seg, singularity, settings)
 
@@ -327,3 +330,116 @@ def ring_induction(points, nodes, edges, gamma, radius, model):
     seg = torch.zeros(1, dtype=torch.int64, device=points.device)
     settings = ConvectionSettings(model=model, hip=HipSettings("fp64"))
     return _induced(points.contiguous(), geometry, gamma.contiguous(), radius, ptr, seg, "raise", settings)
+
+
+# ------------------------------------------------------------------ the step as one graph
+class GeometryStepPlan:
+    """A geometry step recorded as one graph (ofw_step_*): the stage loop of wake.convect_geometry
+    (wake.py:586-616) over the target split of wake._network_convection for the tree code, one set of cores
+    (no ``ring_radii``), a uniform inflow and frozen rows that keep it (``frozen_rows_drift='ambient'``).
+
+    Every stage's incident pairs, segment geometry, exclusion CSR, tree sum and RK arithmetic are replayed by one
+    graph launch; the host reads one status block per step.  The plan keeps the tensors it was made with: a new
+    step with the same topology writes new contents into them (positions, circulation, cores, inflow) and calls
+    ``run`` again.  ``step`` returns the new free points, or takes the stage-by-stage ``convect_geometry`` when the
+    status asks for the reference's exact behaviour (a flagged pair raises there, a flagged ring target repeats its
+    'zero' call), and makes the plan again when a capacity was exceeded.
+    """
+
+    def __init__(self, bound, points, edges, gamma, radii, settings, *, inflow, time, dt_s, substeps=1,
+                 integrator="euler", displacement=None, ring_first=None, ring_free=None, use_graph=True):
+        torch = _torch()
+        if settings.backend != "hip_tree" or not settings.free_wake or settings.frozen_rows_drift != "ambient":
+            raise ValueError("a recorded step needs backend='hip_tree', a free wake and frozen_rows_drift='ambient'")
+        if integrator not in INTEGRATORS:
+            raise ValueError(f"convection_integrator must be one of {INTEGRATORS}")
+        if isinstance(substeps, bool) or int(substeps) != substeps or substeps < 1:
+            raise ValueError("substeps must be a positive integer")
+        tensors = dict(bound=bound, points=points, edges=edges, gamma=gamma)
+        for name, t in tensors.items():
+            if not (t.is_cuda and t.is_contiguous()):
+                raise ValueError(f"{name} must be a contiguous GPU tensor")
+        first, k, m = int(bound.shape[0]), int(points.shape[0]), int(edges.shape[0])
+        if first < 1 or k < 1 or m < 1:
+            raise ValueError("a recorded step needs bound nodes, free points and segments")
+        n = first + k
+        rows_end = k if ring_first is None or ring_first >= n else int(ring_first) - first
+        ring_end = k if ring_free is None else min(k, rows_end + int(ring_free))
+        limit = settings.free_convection_rows
+        rows = limit * first if limit is not None and limit * first < rows_end else rows_end
+        ring_lo, ring_hi = (rows_end, ring_end) if rows_end < ring_end else (rows_end, rows_end)
+        if rows + (ring_hi - ring_lo) < 1:
+            raise ValueError("a recorded step needs targets")
+        if displacement is not None:
+            displacement = displacement.contiguous()
+            if not bool(displacement.any()):
+                displacement = None
+        self.args = dict(bound=bound, points=points, edges=edges, gamma=gamma, radii=radii, settings=settings,
+                         time=time, dt_s=dt_s, substeps=int(substeps), integrator=integrator,
+                         displacement=displacement, ring_first=ring_first, ring_free=ring_free)
+        self.radii = _per_edge(torch, radii, m, points.device, "radii")
+        self.inflow = torch.as_tensor(np.asarray(inflow, dtype=np.float64).reshape(3), device=points.device).clone()
+        self.displacement = displacement
+        self.points_out = torch.empty_like(points)
+        self.shape = dict(first=first, k=k, m=m, rows=rows, ring_lo=ring_lo, ring_hi=ring_hi)
+        self.use_graph = bool(use_graph)
+        self.handle = None
+        self._create()
+
+    def _create(self):
+        import ctypes
+        a, s = self.args, self.shape
+        tree, settings = a["settings"].tree, a["settings"]
+        handle = ctypes.c_void_p()
+        stats = (ctypes.c_int64 * 3)()
+        status = _lib.load().ofw_step_create(
+            _ctx(), ctypes.byref(handle), _lib.PRECISIONS[settings.hip.precision], s["first"], _ptr(a["bound"]),
+            _ptr(self.displacement), s["k"], _ptr(a["points"]), _ptr(self.points_out), s["m"], _ptr(a["edges"]),
+            _ptr(a["gamma"]), _ptr(self.radii), settings.model, s["rows"], s["ring_lo"], s["ring_hi"],
+            float(tree.theta), int(tree.order), int(tree.leaf_size), float(tree.core_clearance), _ptr(self.inflow),
+            float(a["time"]), float(a["dt_s"]), a["substeps"], INTEGRATORS.index(a["integrator"]),
+            int(self.use_graph), stats)
+        _lib.check(status, "ofw_step_create")
+        self.handle = handle
+        self.capacities = dict(pairs=int(stats[1]), list_slots=int(stats[2]))
+
+    def run(self):
+        """Replay the step: (status bits, largest pair count, largest list count)."""
+        import ctypes
+        out = (ctypes.c_int64 * 3)()
+        _lib.check(_lib.load().ofw_step_run(self.handle, out), "ofw_step_run")
+        return int(out[0]), int(out[1]), int(out[2])
+
+    def last_ms(self):
+        import ctypes
+        ms = ctypes.c_float()
+        _lib.check(_lib.load().ofw_step_timings(self.handle, ctypes.byref(ms)), "ofw_step_timings")
+        return float(ms.value)
+
+    def step(self):
+        """The new free points (``points_out``), with the reference's behaviour in every case."""
+        bits, _, _ = self.run()
+        if bits & (_lib.STEP_PAIR_OVERFLOW | _lib.STEP_TREE_OVERFLOW):
+            self.close()
+            self._create()
+            bits, _, _ = self.run()
+        if bits:
+            a = self.args
+            out = convect_geometry(a["bound"], a["points"], a["edges"], a["gamma"], self.radii, a["settings"],
+                                   inflow=self.inflow.cpu().numpy(), time=a["time"], dt_s=a["dt_s"],
+                                   substeps=a["substeps"], integrator=a["integrator"],
+                                   displacement=self.displacement, ring_first=a["ring_first"],
+                                   ring_free=a["ring_free"])
+            self.points_out.copy_(out)
+        return self.points_out
+
+    def close(self):
+        if self.handle is not None:
+            _lib.load().ofw_step_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass

@@ -21,7 +21,8 @@ namespace {
 size_t al(size_t b) { return (b + 255) & ~(size_t)255; }
 }  // namespace
 
-// status[0] bit 0: an edge names a node outside [0, n); bit 1: a zero-length segment.
+// status: OFW_STEP_EDGE_INDEX when an edge names a node outside [0, n), OFW_STEP_ZERO_LENGTH for a zero-length
+// segment.
 __global__ void k_network_segments(int m, int64_t n, const double* __restrict__ nodes,
                                    const int64_t* __restrict__ edges, double* __restrict__ starts,
                                    double* __restrict__ dir, double* __restrict__ len, int32_t* __restrict__ status) {
@@ -30,13 +31,13 @@ __global__ void k_network_segments(int m, int64_t n, const double* __restrict__ 
   for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < m; s += gridDim.x * blockDim.x) {
     const int64_t ia = edges[2 * (size_t)s], ib = edges[2 * (size_t)s + 1];
     if (ia < 0 || ia >= n || ib < 0 || ib >= n) {
-      bad |= 1;
+      bad |= OFW_STEP_EDGE_INDEX;
       continue;
     }
     const double ax = nodes[3 * ia], ay = nodes[3 * ia + 1], az = nodes[3 * ia + 2];
     const double dx = nodes[3 * ib] - ax, dy = nodes[3 * ib + 1] - ay, dz = nodes[3 * ib + 2] - az;
     const double L = hypot(hypot(dx, dy), dz);
-    if (L == 0.0) bad |= 2;
+    if (L == 0.0) bad |= OFW_STEP_ZERO_LENGTH;
     starts[3 * (size_t)s] = ax;
     starts[3 * (size_t)s + 1] = ay;
     starts[3 * (size_t)s + 2] = az;
@@ -54,16 +55,21 @@ __global__ void k_network_segments(int m, int64_t n, const double* __restrict__ 
 // pkey = point - lo for a kept pair (point in [lo, hi) and, with a remap, remap[segment] >= 0), else hi - lo;
 // seg = the (remapped) segment, -1 when dropped; the first sort key is the segment.  status bit 0: a pair names a
 // segment outside [0, m).
-__global__ void k_exclusion_keys(int npairs, const int64_t* __restrict__ pairs, int64_t lo, int64_t hi, int count,
-                                 const int64_t* __restrict__ point_map, int64_t m, const int64_t* __restrict__ remap,
-                                 int32_t* __restrict__ key, int32_t* __restrict__ val, int32_t* __restrict__ pkey,
-                                 int32_t* __restrict__ seg, int32_t* __restrict__ status) {
+// With `live` (the pair count on the device, e.g. the incident-pair total), only pairs below it are read.
+__global__ void k_exclusion_keys(int npairs, const int64_t* __restrict__ pairs, const unsigned long long* live,
+                                 int64_t lo, int64_t hi, int count, const int64_t* __restrict__ point_map, int64_t m,
+                                 const int64_t* __restrict__ remap, int32_t* __restrict__ key,
+                                 int32_t* __restrict__ val, int32_t* __restrict__ pkey, int32_t* __restrict__ seg,
+                                 int32_t* __restrict__ status) {
   bool bad = false;
+  const int64_t valid = live ? (int64_t)(*live < (unsigned long long)npairs ? *live : (unsigned long long)npairs)
+                             : (int64_t)npairs;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < npairs; i += gridDim.x * blockDim.x) {
-    const int64_t p = pairs[2 * (size_t)i];
-    int64_t s = pairs[2 * (size_t)i + 1];
+    const bool real = i < valid;
+    const int64_t p = real ? pairs[2 * (size_t)i] : -1;
+    int64_t s = real ? pairs[2 * (size_t)i + 1] : -1;
     int64_t q = p - lo;
-    bool inside = p >= lo && p < hi;
+    bool inside = real && p >= lo && p < hi;
     if (inside && point_map) q = point_map[q];
     if (inside && q >= count) {
       bad = true;
@@ -82,7 +88,7 @@ __global__ void k_exclusion_keys(int npairs, const int64_t* __restrict__ pairs, 
     key[i] = keep ? (int32_t)s : 0;
     val[i] = i;
   }
-  if (__builtin_amdgcn_ballot_w64(bad) != 0 && (threadIdx.x & 63) == 0) atomicOr(status, 1);
+  if (__builtin_amdgcn_ballot_w64(bad) != 0 && (threadIdx.x & 63) == 0) atomicOr(status, OFW_STEP_EXCLUSION);
 }
 
 __global__ void k_gather_keys(int npairs, const int32_t* __restrict__ val, const int32_t* __restrict__ pkey,
@@ -152,55 +158,27 @@ __global__ void k_stage_combine(int mode, int64_t count, double h, const double*
 
 }  // namespace ofw
 
-extern "C" int ofw_network_segments(ofw_ctx* x, int64_t n, const double* nodes, int64_t m, const int64_t* edges,
-                                    double* starts, double* directions, double* lengths) {
-  if (!x || n < 0 || m < 0 || m > ((int64_t)1 << 28) || (m > 0 && (!nodes || !edges || !starts || !directions ||
-                                                                  !lengths))) {
-    ofw::set_error("ofw_network_segments: invalid arguments");
-    return OFW_ERR_ARG;
-  }
-  ofw::Ctx& c = x->c;
-  c.nev = 0;
+namespace ofw {
+
+int network_segments_enqueue(Ctx& c, int64_t n, const double* nodes, int64_t m, const int64_t* edges, double* starts,
+                             double* directions, double* lengths, int32_t* status) {
   if (m == 0) return OFW_OK;
-  OFW_HIP(c.pflag.reserve(sizeof(int32_t)));
-  int32_t* status = (int32_t*)c.pflag.ptr;
-  hipLaunchKernelGGL(ofw::k_conv_fill, dim3(1), dim3(64), 0, c.stream, 1, status, 0);
   const unsigned grid = (unsigned)std::min<int64_t>(c.cu_count * 8, (m + 255) / 256);
-  hipLaunchKernelGGL(ofw::k_network_segments, dim3(grid), dim3(256), 0, c.stream, (int)m, n, nodes, edges, starts,
+  hipLaunchKernelGGL(k_network_segments, dim3(grid), dim3(256), 0, c.stream, (int)m, n, nodes, edges, starts,
                      directions, lengths, status);
   OFW_HIP(hipGetLastError());
-  int32_t host = 0;
-  OFW_HIP(hipMemcpyAsync(&host, status, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
-  OFW_HIP(hipStreamSynchronize(c.stream));
-  if (host & 1) {
-    ofw::set_error("connectivity index out of range");
-    return OFW_ERR_ARG;
-  }
-  if (host & 2) {
-    ofw::set_error("zero-length segment");
-    return OFW_ERR_ARG;
-  }
   return OFW_OK;
 }
 
-extern "C" int ofw_exclusion_subset(ofw_ctx* x, int64_t npairs, const int64_t* pairs, int64_t lo, int64_t hi,
-                                    int64_t count64, const int64_t* point_map, int64_t m, const int64_t* remap,
-                                    int64_t* ptr_out, int64_t* seg_out) {
-  if (!x || npairs < 0 || npairs > ((int64_t)1 << 30) || lo < 0 || hi < lo || hi - lo > ((int64_t)1 << 30) ||
-      count64 < 0 || count64 > ((int64_t)1 << 30) || (!point_map && count64 != hi - lo) || m < 0 ||
-      m > ((int64_t)1 << 30) || !ptr_out || (npairs > 0 && (!pairs || !seg_out))) {
-    ofw::set_error("ofw_exclusion_subset: invalid arguments");
-    return OFW_ERR_ARG;
-  }
-  ofw::Ctx& c = x->c;
-  c.nev = 0;
+int exclusion_subset_enqueue(Ctx& c, int64_t npairs, const int64_t* pairs, const unsigned long long* live, int64_t lo,
+                             int64_t hi, int64_t count64, const int64_t* point_map, int64_t m, const int64_t* remap,
+                             int segment_ordered, int64_t* ptr_out, int64_t* seg_out, int32_t* status) {
   const int P = (int)npairs, count = (int)count64;
-  const int hist_n = std::max(ofw::radix_hist_size(P), 1);
+  const int hist_n = std::max(radix_hist_size(P), 1);
   const int scan_max = std::max(std::max(P, count + 1), hist_n);
   const size_t spart = ((size_t)(scan_max + 4095) / 4096 + 8) * 4;
-  using ofw::al;
   const size_t need = 8 * al((size_t)std::max(P, 1) * 4) + 2 * al((size_t)(count + 1) * 4) +
-                      2 * al((size_t)hist_n * 4) + al(spart) + 2 * al(4);
+                      2 * al((size_t)hist_n * 4) + al(spart) + al(4);
   OFW_HIP(c.tree[8].reserve(need));
   char* base = (char*)c.tree[8].ptr;
   auto carve = [&](size_t bytes) {
@@ -221,40 +199,105 @@ extern "C" int ofw_exclusion_subset(ofw_ctx* x, int64_t npairs, const int64_t* p
   int32_t* hist_off = carve((size_t)hist_n * 4);
   int32_t* part = carve(spart);
   int32_t* total = carve(4);
-  int32_t* status = carve(4);
   const int grid = c.cu_count * 8;
   auto blocks = [&](int64_t items) {
     return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(grid, (items + 255) / 256)));
   };
-  hipLaunchKernelGGL(ofw::k_conv_fill, blocks(count + 1), dim3(256), 0, c.stream, count + 1, per_point, 0);
-  hipLaunchKernelGGL(ofw::k_conv_fill, dim3(1), dim3(64), 0, c.stream, 1, total, 0);
-  hipLaunchKernelGGL(ofw::k_conv_fill, dim3(1), dim3(64), 0, c.stream, 1, status, 0);
+  hipLaunchKernelGGL(k_conv_fill, blocks(count + 1), dim3(256), 0, c.stream, count + 1, per_point, 0);
+  hipLaunchKernelGGL(k_conv_fill, dim3(1), dim3(64), 0, c.stream, 1, total, 0);
   if (P > 0) {
-    hipLaunchKernelGGL(ofw::k_exclusion_keys, blocks(P), dim3(256), 0, c.stream, P, pairs, lo, hi, count,
-                       point_map, m, remap, key[0], val[0], pkey, seg, status);
-    // LSD: by segment, then (stably) by point
-    int cur = ofw::radix_sort(c, P, ofw::radix_passes(std::max<int64_t>(m - 1, 0)), key, val, hist, hist_off, part);
-    hipLaunchKernelGGL(ofw::k_gather_keys, blocks(P), dim3(256), 0, c.stream, P, val[cur], pkey, key[cur]);
+    hipLaunchKernelGGL(k_exclusion_keys, blocks(P), dim3(256), 0, c.stream, P, pairs, live, lo, hi, count, point_map,
+                       m, remap, key[0], val[0], pkey, seg, status);
+    // LSD: by segment (unless the pairs already come in segment order), then (stably) by point
+    int cur = 0;
+    if (!segment_ordered) cur = radix_sort(c, P, radix_passes(std::max<int64_t>(m - 1, 0)), key, val, hist, hist_off, part);
+    hipLaunchKernelGGL(k_gather_keys, blocks(P), dim3(256), 0, c.stream, P, val[cur], pkey, key[cur]);
     int32_t* key2[2] = {key[cur], key[cur ^ 1]};
     int32_t* val2[2] = {val[cur], val[cur ^ 1]};
-    cur = ofw::radix_sort(c, P, ofw::radix_passes(count), key2, val2, hist, hist_off, part);
-    hipLaunchKernelGGL(ofw::k_exclusion_mark, blocks(P), dim3(256), 0, c.stream, P, count, key2[cur], val2[cur],
-                       seg, keep, per_point);
-    ofw::scan_exclusive(c, P, keep, pos, part, total);
-    hipLaunchKernelGGL(ofw::k_exclusion_emit, blocks(P), dim3(256), 0, c.stream, P, keep, pos, val2[cur], seg,
-                       seg_out);
+    cur = radix_sort(c, P, radix_passes(count), key2, val2, hist, hist_off, part);
+    hipLaunchKernelGGL(k_exclusion_mark, blocks(P), dim3(256), 0, c.stream, P, count, key2[cur], val2[cur], seg, keep,
+                       per_point);
+    scan_exclusive(c, P, keep, pos, part, total);
+    hipLaunchKernelGGL(k_exclusion_emit, blocks(P), dim3(256), 0, c.stream, P, keep, pos, val2[cur], seg, seg_out);
   }
-  ofw::scan_exclusive(c, count + 1, per_point, start, part, nullptr);
-  hipLaunchKernelGGL(ofw::k_ptr_widen, blocks(count + 1), dim3(256), 0, c.stream, count, start, total, ptr_out);
+  scan_exclusive(c, count + 1, per_point, start, part, nullptr);
+  hipLaunchKernelGGL(k_ptr_widen, blocks(count + 1), dim3(256), 0, c.stream, count, start, total, ptr_out);
   OFW_HIP(hipGetLastError());
-  if (P > 0) {
-    int32_t host = 0;
-    OFW_HIP(hipMemcpyAsync(&host, status, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
-    OFW_HIP(hipStreamSynchronize(c.stream));
-    if (host) {
-      ofw::set_error("exclusion index out of range");  // a segment outside [0, m) or a map beyond count
-      return OFW_ERR_ARG;
-    }
+  return OFW_OK;
+}
+
+// One status word in the context's flag buffer, cleared, for the synchronous entry points.
+static int32_t* sync_status(Ctx& c) {
+  if (c.pflag.reserve(sizeof(int32_t)) != hipSuccess) return nullptr;
+  int32_t* status = (int32_t*)c.pflag.ptr;
+  hipLaunchKernelGGL(k_conv_fill, dim3(1), dim3(64), 0, c.stream, 1, status, 0);
+  return status;
+}
+
+static int read_status(Ctx& c, const int32_t* status, int32_t* host) {
+  OFW_HIP(hipMemcpyAsync(host, status, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+  OFW_HIP(hipStreamSynchronize(c.stream));
+  return OFW_OK;
+}
+
+}  // namespace ofw
+
+extern "C" int ofw_network_segments(ofw_ctx* x, int64_t n, const double* nodes, int64_t m, const int64_t* edges,
+                                    double* starts, double* directions, double* lengths) {
+  if (!x || n < 0 || m < 0 || m > ((int64_t)1 << 28) || (m > 0 && (!nodes || !edges || !starts || !directions ||
+                                                                  !lengths))) {
+    ofw::set_error("ofw_network_segments: invalid arguments");
+    return OFW_ERR_ARG;
+  }
+  ofw::Ctx& c = x->c;
+  c.nev = 0;
+  if (m == 0) return OFW_OK;
+  int32_t* status = ofw::sync_status(c);
+  if (!status) {
+    ofw::set_error("ofw_network_segments: out of device memory");
+    return OFW_ERR_HIP;
+  }
+  int st = ofw::network_segments_enqueue(c, n, nodes, m, edges, starts, directions, lengths, status);
+  if (st) return st;
+  int32_t host = 0;
+  st = ofw::read_status(c, status, &host);
+  if (st) return st;
+  if (host & OFW_STEP_EDGE_INDEX) {
+    ofw::set_error("connectivity index out of range");
+    return OFW_ERR_ARG;
+  }
+  if (host & OFW_STEP_ZERO_LENGTH) {
+    ofw::set_error("zero-length segment");
+    return OFW_ERR_ARG;
+  }
+  return OFW_OK;
+}
+
+extern "C" int ofw_exclusion_subset(ofw_ctx* x, int64_t npairs, const int64_t* pairs, int64_t lo, int64_t hi,
+                                    int64_t count64, const int64_t* point_map, int64_t m, const int64_t* remap,
+                                    int segment_ordered, int64_t* ptr_out, int64_t* seg_out) {
+  if (!x || npairs < 0 || npairs > ((int64_t)1 << 30) || lo < 0 || hi < lo || hi - lo > ((int64_t)1 << 30) ||
+      count64 < 0 || count64 > ((int64_t)1 << 30) || (!point_map && count64 != hi - lo) || m < 0 ||
+      m > ((int64_t)1 << 30) || !ptr_out || (npairs > 0 && (!pairs || !seg_out))) {
+    ofw::set_error("ofw_exclusion_subset: invalid arguments");
+    return OFW_ERR_ARG;
+  }
+  ofw::Ctx& c = x->c;
+  c.nev = 0;
+  int32_t* status = ofw::sync_status(c);
+  if (!status) {
+    ofw::set_error("ofw_exclusion_subset: out of device memory");
+    return OFW_ERR_HIP;
+  }
+  int st = ofw::exclusion_subset_enqueue(c, npairs, pairs, nullptr, lo, hi, count64, point_map, m, remap,
+                                         segment_ordered, ptr_out, seg_out, status);
+  if (st) return st;
+  int32_t host = 0;
+  st = ofw::read_status(c, status, &host);
+  if (st) return st;
+  if (host) {
+    ofw::set_error("exclusion index out of range");  // a segment outside [0, m) or a map beyond count
+    return OFW_ERR_ARG;
   }
   return OFW_OK;
 }

@@ -10,7 +10,8 @@
 //   2. A stable LSD radix sort of the point indices by slot (8-bit digits, per-tile histograms, wavefront
 //      ballots for the stable rank inside a tile) leaves every group's members contiguous and in increasing
 //      order at start[slot] = the exclusive scan of count, whatever order the threads won the table in.
-//   3. k_end_groups: each segment end finds its group with one probe; the pair counts are scanned.
+//   3. every node's group: the free points' from step 1, the others' by one probe (k_fixed_groups); each segment end
+//      reads its node's group (k_end_groups) and the pair counts are scanned.
 //   4. k_write_pairs: one thread per output pair finds its segment end by binary search over the offsets.
 #include <climits>
 
@@ -71,13 +72,14 @@ __global__ void k_net_fill(int count, int32_t* __restrict__ a, int32_t v) {
 // slot, or H (sorted past every group) for a point with a NaN coordinate; val[i] = i.
 __global__ void k_group_points(int nfree, int first, const double* __restrict__ nodes, uint32_t H,
                                int32_t* __restrict__ table, int32_t* __restrict__ count, int32_t* __restrict__ key,
-                               int32_t* __restrict__ val) {
+                               int32_t* __restrict__ val, int32_t* __restrict__ node_group) {
   const uint32_t mask = H - 1;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nfree; i += gridDim.x * blockDim.x) {
     const int node = first + i;
     val[i] = i;
     if (has_nan(nodes, node)) {
       key[i] = (int32_t)H;
+      node_group[node] = -1;
       continue;
     }
     const NodeBits bits = node_bits(nodes, node);
@@ -94,7 +96,31 @@ __global__ void k_group_points(int nfree, int first, const double* __restrict__ 
       h = (h + 1) & mask;
     }
     key[i] = (int32_t)h;
+    node_group[node] = (int32_t)h;
     atomicAdd(&count[h], 1);
+  }
+}
+
+// The group of each node before `first` (not inserted): one probe; -1 when none or a NaN coordinate.
+__global__ void k_fixed_groups(int first, const double* __restrict__ nodes, uint32_t H,
+                               const int32_t* __restrict__ table, int32_t* __restrict__ node_group) {
+  const uint32_t mask = H - 1;
+  for (int node = blockIdx.x * blockDim.x + threadIdx.x; node < first; node += gridDim.x * blockDim.x) {
+    int g = -1;
+    if (!has_nan(nodes, node)) {
+      const NodeBits bits = node_bits(nodes, node);
+      uint32_t h = slot_of(bits, mask);
+      while (true) {
+        const int cur = table[h];
+        if (cur < 0) break;
+        if (same_bits(node_bits(nodes, cur), bits)) {
+          g = (int)h;
+          break;
+        }
+        h = (h + 1) & mask;
+      }
+    }
+    node_group[node] = g;
   }
 }
 
@@ -156,33 +182,20 @@ __global__ __launch_bounds__(SORT_THREADS) void k_radix_scatter(int n, int shift
   }
 }
 
-// Per segment end q = 2 s + e: its group (-1 when none) and the number of pairs it gives.  status[0] sums the
-// pairs over all ends; status[1] is set when an edge names a node outside [0, n).
-__global__ void k_end_groups(int m, int n, const int64_t* __restrict__ edges, const double* __restrict__ nodes,
-                             uint32_t H, const int32_t* __restrict__ table, const int32_t* __restrict__ count,
-                             int32_t* __restrict__ end_group, int32_t* __restrict__ end_count,
-                             unsigned long long* __restrict__ status) {
-  const uint32_t mask = H - 1;
+// Per segment end q = 2 s + e: its node's group (-1 when none) and the number of pairs it gives.  status[0] sums
+// the pairs over all ends; status[1] is set when an edge names a node outside [0, n).
+__global__ void k_end_groups(int m, int n, const int64_t* __restrict__ edges, const int32_t* __restrict__ node_group,
+                             const int32_t* __restrict__ count, int32_t* __restrict__ end_group,
+                             int32_t* __restrict__ end_count, unsigned long long* __restrict__ status) {
   unsigned long long mine = 0;
   bool bad = false;
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < 2 * m; q += gridDim.x * blockDim.x) {
     const int64_t e = edges[q];
     int g = -1;
-    if (e < 0 || e >= n) {
+    if (e < 0 || e >= n)
       bad = true;
-    } else if (!has_nan(nodes, (int)e)) {
-      const NodeBits bits = node_bits(nodes, (int)e);
-      uint32_t h = slot_of(bits, mask);
-      while (true) {
-        const int cur = table[h];
-        if (cur < 0) break;
-        if (same_bits(node_bits(nodes, cur), bits)) {
-          g = (int)h;
-          break;
-        }
-        h = (h + 1) & mask;
-      }
-    }
+    else
+      g = node_group[e];
     end_group[q] = g;
     const int c = g >= 0 ? count[g] : 0;
     end_count[q] = c;
@@ -194,12 +207,17 @@ __global__ void k_end_groups(int m, int n, const int64_t* __restrict__ edges, co
   if (__builtin_amdgcn_ballot_w64(bad) != 0 && lead) atomicOr(&status[1], 1ull);
 }
 
-// One thread per pair p: its end q is the last one with offset[q] <= p (the ends after q start beyond p, so
-// end_count[q] > 0 and p lies inside q's range).
-__global__ void k_write_pairs(int total, int ends, const int32_t* __restrict__ end_group,
-                              const int32_t* __restrict__ offset, const int32_t* __restrict__ start,
+// One thread per pair p < min(total, capacity): its end q is the last one with offset[q] <= p (the ends after q
+// start beyond p, so end_count[q] > 0 and p lies inside q's range).  Nothing is written after a bad edge or when the
+// total exceeds 2^31 - 1 (the int32 offsets would not hold it); the range checks keep every access in bounds.
+__global__ void k_write_pairs(int64_t capacity, int ends, const unsigned long long* __restrict__ status,
+                              const int32_t* __restrict__ end_group, const int32_t* __restrict__ offset,
+                              const int32_t* __restrict__ start, const int32_t* __restrict__ count,
                               const int32_t* __restrict__ members, int64_t* __restrict__ pairs) {
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < total; p += gridDim.x * blockDim.x) {
+  const unsigned long long all = status[0];
+  if (status[1] || all > (unsigned long long)INT_MAX) return;
+  const int64_t total = (int64_t)all < capacity ? (int64_t)all : capacity;
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < total; p += (int64_t)gridDim.x * blockDim.x) {
     int lo = 0, hi = ends - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
@@ -209,8 +227,10 @@ __global__ void k_write_pairs(int total, int ends, const int32_t* __restrict__ e
         hi = mid - 1;
     }
     const int g = end_group[lo];
-    pairs[2 * (int64_t)p] = members[start[g] + (p - offset[lo])];
-    pairs[2 * (int64_t)p + 1] = lo >> 1;
+    const int64_t r = p - offset[lo];
+    if (g < 0 || r < 0 || r >= count[g]) continue;
+    pairs[2 * p] = members[start[g] + r];
+    pairs[2 * p + 1] = lo >> 1;
   }
 }
 
@@ -240,59 +260,109 @@ int radix_passes(int64_t top) {
   return (bits + RADIX_BITS - 1) / RADIX_BITS;
 }
 
-int incident_pairs(Ctx& c, int64_t n, const double* nodes, int64_t m, const int64_t* edges, int64_t first,
-                   int64_t* pairs, int64_t capacity, int64_t* count_out) {
-  *count_out = 0;
+// Scratch of one incident-pair call (buffer 6 of the context).
+struct PairScratch {
+  uint32_t H = 0;
+  int F = 0, E = 0, passes = 0;
+  int32_t *table, *cnt, *start, *key[2], *val[2], *end_group, *end_count, *offset, *hist, *hist_off, *part;
+  int32_t* node_group;
+  unsigned long long* status;  // [0] the pair total, [1] a bad edge
+};
+
+static int pair_scratch(Ctx& c, int64_t n, int64_t m, int64_t first, PairScratch& s) {
   const int64_t nfree = n - first;
-  if (nfree <= 0 || m <= 0) return OFW_OK;
-  if (n > INT_MAX / 4 || m > INT_MAX / 4) {
-    set_error("ofw_incident_pairs: more than 2^29 nodes or segments");
-    return OFW_ERR_ARG;
-  }
-  uint32_t H = 2;
-  while ((int64_t)H < 2 * nfree) H <<= 1;
-  const int passes = radix_passes(H);  // keys run to H inclusive
-  const int F = (int)nfree, E = (int)(2 * m);
-  const int tiles = (F + SORT_TILE - 1) / SORT_TILE;
+  s.H = 2;
+  while ((int64_t)s.H < 2 * nfree) s.H <<= 1;
+  s.passes = radix_passes(s.H);  // keys run to H inclusive
+  s.F = (int)nfree;
+  s.E = (int)(2 * m);
+  const int tiles = (s.F + SORT_TILE - 1) / SORT_TILE;
   const int hist_n = RADIX * tiles;
-  const int scan_max = std::max<int>(std::max<int>((int)H, E), hist_n);
+  const int scan_max = std::max<int>(std::max<int>((int)s.H, s.E), hist_n);
   const size_t spart = ((size_t)(scan_max + 4095) / 4096 + 8) * 4;
-  const size_t need = 3 * al((size_t)H * 4) + 4 * al((size_t)F * 4) + 3 * al((size_t)E * 4) +
-                      2 * al((size_t)hist_n * 4) + al(spart) + al(16);
+  const size_t H4 = (size_t)s.H * 4, F4 = (size_t)s.F * 4, E4 = (size_t)s.E * 4;
+  const size_t need = 3 * al(H4) + 4 * al(F4) + 3 * al(E4) + 2 * al((size_t)hist_n * 4) + al(spart) + al(16) +
+                      al((size_t)n * 4);
   OFW_HIP(c.tree[6].reserve(need));
   char* base = (char*)c.tree[6].ptr;
   auto carve = [&](size_t bytes) {
     char* p = base;
     base += al(bytes);
-    return (void*)p;
+    return (int32_t*)p;
   };
-  int32_t* table = (int32_t*)carve((size_t)H * 4);
-  int32_t* cnt = (int32_t*)carve((size_t)H * 4);
-  int32_t* start = (int32_t*)carve((size_t)H * 4);
-  int32_t* key[2] = {(int32_t*)carve((size_t)F * 4), (int32_t*)carve((size_t)F * 4)};
-  int32_t* val[2] = {(int32_t*)carve((size_t)F * 4), (int32_t*)carve((size_t)F * 4)};
-  int32_t* end_group = (int32_t*)carve((size_t)E * 4);
-  int32_t* end_count = (int32_t*)carve((size_t)E * 4);
-  int32_t* offset = (int32_t*)carve((size_t)E * 4);
-  int32_t* hist = (int32_t*)carve((size_t)hist_n * 4);
-  int32_t* hist_off = (int32_t*)carve((size_t)hist_n * 4);
-  int32_t* part = (int32_t*)carve(spart);
-  unsigned long long* status = (unsigned long long*)carve(16);
+  s.table = carve(H4);
+  s.cnt = carve(H4);
+  s.start = carve(H4);
+  s.key[0] = carve(F4);
+  s.key[1] = carve(F4);
+  s.val[0] = carve(F4);
+  s.val[1] = carve(F4);
+  s.end_group = carve(E4);
+  s.end_count = carve(E4);
+  s.offset = carve(E4);
+  s.hist = carve((size_t)hist_n * 4);
+  s.hist_off = carve((size_t)hist_n * 4);
+  s.part = carve(spart);
+  s.status = (unsigned long long*)carve(16);
+  s.node_group = carve((size_t)n * 4);
+  return OFW_OK;
+}
 
+// Phase 1: the groups, each end's group and count, the 64-bit total (status[0]).  Returns the buffer index of the
+// sorted members.
+static int pairs_count(Ctx& c, int64_t n, const double* nodes, int64_t m, const int64_t* edges, int64_t first,
+                       PairScratch& s) {
   const int grid = c.cu_count * 8;
-  auto blocks = [&](int64_t items) { return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(grid, (items + 255) / 256))); };
-  hipLaunchKernelGGL(k_net_fill, blocks(H), dim3(256), 0, c.stream, (int)H, table, -1);
-  hipLaunchKernelGGL(k_net_fill, blocks(H), dim3(256), 0, c.stream, (int)H, cnt, 0);
-  hipLaunchKernelGGL(k_net_fill, dim3(1), dim3(64), 0, c.stream, 4, (int32_t*)status, 0);
-  hipLaunchKernelGGL(k_group_points, blocks(F), dim3(256), 0, c.stream, F, (int)first, nodes, H, table, cnt, key[0],
-                     val[0]);
-  scan_exclusive(c, (int)H, cnt, start, part, nullptr);
-  const int32_t* members = val[radix_sort(c, F, passes, key, val, hist, hist_off, part)];
-  hipLaunchKernelGGL(k_end_groups, blocks(E), dim3(256), 0, c.stream, (int)m, (int)n, edges, nodes, H, table, cnt,
-                     end_group, end_count, status);
+  auto blocks = [&](int64_t items) {
+    return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(grid, (items + 255) / 256)));
+  };
+  hipLaunchKernelGGL(k_net_fill, blocks(s.H), dim3(256), 0, c.stream, (int)s.H, s.table, -1);
+  hipLaunchKernelGGL(k_net_fill, blocks(s.H), dim3(256), 0, c.stream, (int)s.H, s.cnt, 0);
+  hipLaunchKernelGGL(k_net_fill, dim3(1), dim3(64), 0, c.stream, 4, (int32_t*)s.status, 0);
+  hipLaunchKernelGGL(k_group_points, blocks(s.F), dim3(256), 0, c.stream, s.F, (int)first, nodes, s.H, s.table, s.cnt,
+                     s.key[0], s.val[0], s.node_group);
+  if (first > 0)
+    hipLaunchKernelGGL(k_fixed_groups, blocks(first), dim3(256), 0, c.stream, (int)first, nodes, s.H, s.table,
+                       s.node_group);
+  scan_exclusive(c, (int)s.H, s.cnt, s.start, s.part, nullptr);
+  const int cur = radix_sort(c, s.F, s.passes, s.key, s.val, s.hist, s.hist_off, s.part);
+  hipLaunchKernelGGL(k_end_groups, blocks(s.E), dim3(256), 0, c.stream, (int)m, (int)n, edges, s.node_group, s.cnt,
+                     s.end_group, s.end_count, s.status);
   OFW_HIP(hipGetLastError());
+  return cur;
+}
+
+// Phase 2: the offsets and the pairs, at most `capacity` of them.
+static int pairs_write(Ctx& c, const PairScratch& s, int members_buf, int64_t capacity, int64_t* pairs) {
+  if (capacity <= 0) return OFW_OK;
+  scan_exclusive(c, s.E, s.end_count, s.offset, s.part, nullptr);
+  const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(c.cu_count * 8, (capacity + 255) / 256));
+  hipLaunchKernelGGL(k_write_pairs, dim3(grid), dim3(256), 0, c.stream, capacity, s.E, s.status, s.end_group,
+                     s.offset, s.start, s.cnt, s.val[members_buf], pairs);
+  OFW_HIP(hipGetLastError());
+  return OFW_OK;
+}
+
+static int pairs_args(int64_t n, int64_t m) {
+  if (n > INT_MAX / 4 || m > INT_MAX / 4) {
+    set_error("ofw_incident_pairs: more than 2^29 nodes or segments");
+    return OFW_ERR_ARG;
+  }
+  return OFW_OK;
+}
+
+int incident_pairs(Ctx& c, int64_t n, const double* nodes, int64_t m, const int64_t* edges, int64_t first,
+                   int64_t* pairs, int64_t capacity, int64_t* count_out) {
+  *count_out = 0;
+  if (n - first <= 0 || m <= 0) return OFW_OK;
+  int st = pairs_args(n, m);
+  if (st) return st;
+  PairScratch s;
+  st = pair_scratch(c, n, m, first, s);
+  if (st) return st;
+  const int cur = pairs_count(c, n, nodes, m, edges, first, s);
   unsigned long long host[2] = {0, 0};
-  OFW_HIP(hipMemcpyAsync(host, status, 16, hipMemcpyDeviceToHost, c.stream));
+  OFW_HIP(hipMemcpyAsync(host, s.status, 16, hipMemcpyDeviceToHost, c.stream));
   OFW_HIP(hipStreamSynchronize(c.stream));
   if (host[1]) {
     set_error("ofw_incident_pairs: an edge names a node outside [0, n)");
@@ -308,11 +378,21 @@ int incident_pairs(Ctx& c, int64_t n, const double* nodes, int64_t m, const int6
     return OFW_ERR_OVERFLOW;
   }
   if (host[0] == 0) return OFW_OK;
-  scan_exclusive(c, E, end_count, offset, part, nullptr);
-  hipLaunchKernelGGL(k_write_pairs, blocks((int64_t)host[0]), dim3(256), 0, c.stream, (int)host[0], E, end_group,
-                     offset, start, members, pairs);
-  OFW_HIP(hipGetLastError());
-  return OFW_OK;
+  return pairs_write(c, s, cur, (int64_t)host[0], pairs);
+}
+
+int incident_pairs_enqueue(Ctx& c, int64_t n, const double* nodes, int64_t m, const int64_t* edges, int64_t first,
+                           int64_t* pairs, int64_t capacity, const unsigned long long** status_out) {
+  *status_out = nullptr;
+  if (n - first <= 0 || m <= 0) return OFW_OK;
+  int st = pairs_args(n, m);
+  if (st) return st;
+  PairScratch s;
+  st = pair_scratch(c, n, m, first, s);
+  if (st) return st;
+  const int cur = pairs_count(c, n, nodes, m, edges, first, s);
+  *status_out = s.status;
+  return pairs_write(c, s, cur, capacity, pairs);
 }
 
 // ---------------------------------------------------------------------------------------------------------------

@@ -78,6 +78,19 @@ int radix_sort(Ctx& c, int n, int passes, int32_t* key[2], int32_t* val[2], int3
 int radix_passes(int64_t top);
 int radix_hist_size(int n);
 
+// Enqueue-only forms of the network bookkeeping (no host synchronisation, no allocation once the context's
+// buffers have the sizes of an earlier identical call), for recorded graphs:
+// incident pairs, at most `capacity` written; *status_out -> [0] the 64-bit pair total, [1] a bad edge.
+int incident_pairs_enqueue(Ctx& c, int64_t n, const double* nodes, int64_t m, const int64_t* edges, int64_t first,
+                           int64_t* pairs, int64_t capacity, const unsigned long long** status_out);
+// segment geometry; OFW_STEP_EDGE_INDEX / OFW_STEP_ZERO_LENGTH or-ed into *status.
+int network_segments_enqueue(Ctx& c, int64_t n, const double* nodes, int64_t m, const int64_t* edges, double* starts,
+                             double* directions, double* lengths, int32_t* status);
+// exclusion CSR of target ranges (only pairs below *live when live is given); OFW_STEP_EXCLUSION into *status.
+int exclusion_subset_enqueue(Ctx& c, int64_t npairs, const int64_t* pairs, const unsigned long long* live, int64_t lo,
+                             int64_t hi, int64_t count, const int64_t* point_map, int64_t m, const int64_t* remap,
+                             int segment_ordered, int64_t* ptr_out, int64_t* seg_out, int32_t* status);
+
 // Reserve and lay out every array of a build of m segments in K trees with leaf size `leaf`.
 int tree_layout(Ctx& c, int buf, int64_t m, int K, int leaf, TreeArrays& t);
 // Enqueue the whole build (no host synchronisation): midpoints mx/my/mz (m), tree segment offsets on the

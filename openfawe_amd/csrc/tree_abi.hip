@@ -55,32 +55,7 @@ static int midpoints(Ctx& c, int64_t m, const double* starts, const double* dire
   return tree_prep(c, m, starts, directions, lengths, *mx, *my, *mz);
 }
 
-// One tree sum over K independent snapshots (K = 1 for ofw_tree_sum).  With a fixed near-field capacity
-// (chunk_slots >= 0) it only enqueues: no host synchronisation, no memset, and no allocation once the
-// context's buffers have the sizes of an earlier identical call, so the whole sequence can be captured in a
-// graph.  With chunk_slots < 0 the capacity is read from the device once, after the walk.
-struct TreeCall {
-  int precision = OFW_MIXED;
-  int K = 1;
-  int64_t n = 0, m = 0;
-  std::vector<int64_t> toff, soff;
-  const double *points = nullptr, *starts = nullptr, *directions = nullptr, *lengths = nullptr;
-  const double *gamma = nullptr, *radius = nullptr;
-  int model = 0, raise_singular = 0;
-  const int64_t *ex_ptr = nullptr, *ex_seg = nullptr;
-  int64_t n_ex = 0;
-  double theta = 0.5;
-  int order = 2;
-  int leaf = 32;
-  double clearance = 0.0;
-  int split_depth = 0;
-  int64_t chunk_slots = -1;  // near-field work-item slots (< 0: sized from this call's count)
-  double* out = nullptr;
-  int8_t* flags = nullptr;
-  int64_t *far_counts = nullptr, *near_counts = nullptr;
-};
-
-static int tree_pipeline(Ctx& c, const TreeCall& tc, TreeArrays& t) {
+int tree_pipeline(Ctx& c, const TreeCall& tc, TreeArrays& t) {
   int32_t *p32 = nullptr, *s32 = nullptr;
   if (tc.ex_ptr && tc.n_ex > 0) {
     int st = csr_convert(c, tc.n, tc.n_ex, tc.ex_ptr, tc.ex_seg, &p32, &s32);

@@ -45,4 +45,32 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
 // Frontier depth that gives the traversal enough wavefronts: `groups` = target groups of 64 over all trees.
 int choose_split_depth(const Ctx& c, int64_t groups, int K);
 
+// One tree sum over K independent snapshots (K = 1 for ofw_tree_sum).  With a fixed near-field capacity
+// (chunk_slots >= 0) it only enqueues: no host synchronisation, no memset, and no allocation once the
+// context's buffers have the sizes of an earlier identical call, so the whole sequence can be captured in a
+// graph.  With chunk_slots < 0 the capacity is read from the device once, after the walk.
+struct TreeCall {
+  int precision = OFW_MIXED;
+  int K = 1;
+  int64_t n = 0, m = 0;
+  std::vector<int64_t> toff, soff;
+  const double *points = nullptr, *starts = nullptr, *directions = nullptr, *lengths = nullptr;
+  const double *gamma = nullptr, *radius = nullptr;
+  int model = 0, raise_singular = 0;
+  const int64_t *ex_ptr = nullptr, *ex_seg = nullptr;
+  int64_t n_ex = 0;
+  double theta = 0.5;
+  int order = 2;
+  int leaf = 32;
+  double clearance = 0.0;
+  int split_depth = 0;
+  int64_t chunk_slots = -1;  // near-field work-item slots (< 0: sized from this call's count)
+  double* out = nullptr;
+  int8_t* flags = nullptr;
+  int64_t *far_counts = nullptr, *near_counts = nullptr;
+};
+
+// Enqueue a whole tree sum (midpoints, build, node quantities, traversal).
+int tree_pipeline(Ctx& c, const TreeCall& tc, TreeArrays& t);
+
 }  // namespace ofw

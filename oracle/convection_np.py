@@ -1,8 +1,9 @@
 """NumPy restatement of the reference's free-wake geometry step.  TEST INFRASTRUCTURE ONLY.
 
 * ``segments_velocity`` -- vortex.py:458-497 for validated inputs: segment geometry from nodes and edges
-  (hypot norms, direction = delta / length, zero-length error), the exclusion CSR, the direct sum of the C oracle
-  (vortex_numba.direct_sum), and the flag and finiteness errors.
+  (hypot norms, direction = delta / length, zero-length error), the exclusion CSR, the sum of the C oracle (the
+  direct sum, vortex_numba.direct_sum, or the tree code, vortex_tree.tree_sum), and the flag and finiteness
+  errors.
 * ``network_convection`` -- wake._network_convection (wake.py:404-491): ambient plus induced velocity of the free
   points nodes[first:], incident segments excluded, targets split into wing rows ('raise'), frozen rows beyond
   ``free_convection_rows`` (ambient, or with 'ambient_rings' the ring edges alone) and ring nodes ('zero',
@@ -20,7 +21,21 @@ class Singular(ValueError):
     """A flagged pair (the reference raises FilamentSingularity or the float64 range error)."""
 
 
-def segments_velocity(points, nodes, edges, gamma, radius, model, exclusions, singularity):
+def direct_engine(points, starts, directions, lengths, gamma, radius, model, csr, singularity):
+    """The reference's direct sum (vortex_numba.direct_sum) by the C oracle: (out, flags)."""
+    return O.direct_sum(points, starts, directions, lengths, gamma, radius, model, csr, singularity)
+
+
+def tree_engine(theta, order=2, leaf_size=32, clearance=3.0, threads=None):
+    """The reference's tree code (vortex_tree.tree_sum) by the C oracle, as an engine: (out, flags)."""
+    def engine(points, starts, directions, lengths, gamma, radius, model, csr, singularity):
+        out, flags, _, _ = O.tree_sum(points, starts, directions, lengths, gamma, radius, model, csr, singularity,
+                                      theta, order, leaf_size, clearance, threads)
+        return out, flags
+    return engine
+
+
+def segments_velocity(points, nodes, edges, gamma, radius, model, exclusions, singularity, engine=direct_engine):
     """vortex.py:458-497: the induced velocity at points of the segments nodes[edges], pairs excluded."""
     nodes = np.asarray(nodes, dtype=np.float64)
     edges = np.asarray(edges, dtype=np.int64).reshape(-1, 2)
@@ -32,7 +47,7 @@ def segments_velocity(points, nodes, edges, gamma, radius, model, exclusions, si
     directions = delta / lengths[:, None]
     radius = np.broadcast_to(np.asarray(radius, dtype=np.float64), (len(edges),))
     csr = O.exclusion_csr(len(points), np.asarray(exclusions, dtype=np.int64).reshape(-1, 2))
-    out, flags = O.direct_sum(points, starts, directions, lengths, gamma, radius, model, csr, singularity)
+    out, flags = engine(points, starts, directions, lengths, gamma, radius, model, csr, singularity)
     if np.any(flags):
         raise Singular("flagged pair")
     if not np.isfinite(out).all():
@@ -42,7 +57,7 @@ def segments_velocity(points, nodes, edges, gamma, radius, model, exclusions, si
 
 def network_convection(nodes, edges, gamma, first, radii, model, ambient, *, free_wake=True,
                        free_convection_rows=None, frozen_rows_drift="ambient", ring_first=None, ring_free=None,
-                       ring_radii=None):
+                       ring_radii=None, engine=direct_engine):
     """wake._network_convection (wake.py:404-491): velocity (n - first, 3) of the points nodes[first:]."""
     nodes = np.asarray(nodes, dtype=np.float64)
     edges = np.asarray(edges, dtype=np.int64).reshape(-1, 2)
@@ -64,7 +79,7 @@ def network_convection(nodes, edges, gamma, first, radii, model, ambient, *, fre
     if limit is not None and limit * first < rows_end:
         free = limit * first
         velocity[:free] += segments_velocity(points[:free], nodes, edges, gamma, radii, model, within(0, free),
-                                             "raise")
+                                             "raise", engine)
         if frozen_rows_drift == "ambient_rings" and ring_first is not None and ring_first < len(nodes):
             ring_edge = np.all(edges >= ring_first, axis=1)
             if np.any(ring_edge):
@@ -76,19 +91,20 @@ def network_convection(nodes, edges, gamma, first, radii, model, ambient, *, fre
                 pairs = np.column_stack((pairs[:, 0], remap[pairs[:, 1]])).reshape(-1, 2)
                 cores = np.broadcast_to(radii, (len(edges),))[ring_edge]
                 velocity[free:rows_end] += segments_velocity(points[free:rows_end], nodes, edges[ring_edge],
-                                                             gamma[ring_edge], cores, model, pairs, "raise")
+                                                             gamma[ring_edge], cores, model, pairs, "raise", engine)
         if rows_end < ring_end:
             velocity[rows_end:ring_end] += segments_velocity(points[rows_end:ring_end], nodes, edges, gamma,
-                                                             target_radii, model, within(rows_end, ring_end), "zero")
+                                                             target_radii, model, within(rows_end, ring_end), "zero",
+                                                             engine)
     elif ring_first is None or ring_first >= len(nodes):
-        velocity += segments_velocity(points, nodes, edges, gamma, radii, model, exclusions, "raise")
+        velocity += segments_velocity(points, nodes, edges, gamma, radii, model, exclusions, "raise", engine)
     else:
         split = ring_first - first
         velocity[:split] += segments_velocity(points[:split], nodes, edges, gamma, radii, model, within(0, split),
-                                              "raise")
+                                              "raise", engine)
         if split < ring_end:
             velocity[split:ring_end] += segments_velocity(points[split:ring_end], nodes, edges, gamma, target_radii,
-                                                          model, within(split, ring_end), "zero")
+                                                          model, within(split, ring_end), "zero", engine)
     return velocity
 
 
