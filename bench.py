@@ -237,6 +237,18 @@ def convection_bench(case, rows, steps, tree, settings, cpu_threads=None):
             run()
         torch.cuda.synchronize()
         result[f"{integrator}_step_ms"] = 1e3 * (time.perf_counter() - t0) / steps
+        # the same step recorded as one graph (GeometryStepPlan): replay plus the status read-back
+        plan = CV.GeometryStepPlan(dn[:width], dn[width:], de, dg, dr, cs, inflow=[10.0, 0.0, 0.0], time=0.0,
+                                   dt_s=0.05, integrator=integrator, ring_first=ring_first)
+        plan.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            plan.run()
+        torch.cuda.synchronize()
+        result[f"{integrator}_graph_step_ms"] = 1e3 * (time.perf_counter() - t0) / steps
+        result[f"{integrator}_graph_capacities"] = plan.capacities
+        plan.close()
     if cpu_threads:
         # the same Euler step by the oracle (the reference's algorithm in C, FP64 tree code, OpenMP over targets)
         from oracle import convection_np as CO
