@@ -1,11 +1,11 @@
 // A free-wake geometry step as one recorded graph: the stage loop of convect_geometry (wake.py:586-616) over the
 // target split of _network_convection (wake.py:404-491), for the tree code with one set of cores, a uniform
 // inflow and frozen rows that keep it.  Every stage enqueues, with no host synchronisation:
-//   the stage nodes (bound row, moved linearly by the displacement, then the free points),
-//   the incident pairs (at most the plan's capacity), the segment geometry, the exclusion CSR of the targets
-//   (wing rows [0, rows) then ring nodes [ring_lo, ring_hi), packed by a point map), the targets,
-//   the tree sum ('raise' for all targets: a singular pair adds nothing in either mode, only its flag differs),
-//   the stage velocity (inflow plus the sums) and the RK arithmetic.
+//   the stage nodes (bound row, moved linearly by the displacement, then the free points), the segment geometry,
+//   on a side stream the incident pairs (at most the plan's capacity) and the exclusion CSR of the targets (wing
+//   rows [0, rows) then ring nodes [ring_lo, ring_hi), packed by a point map), meanwhile the tree build and node
+//   quantities, then (joined) the traversal ('raise' for all targets: a singular pair adds nothing in either mode,
+//   only its flag differs), the stage velocity (inflow plus the sums) and the RK arithmetic.
 // The host reads one status block per step: capacity overflows (the plan is made again with the measured sizes)
 // and the conditions the reference raises on or repeats a ring call for (row flags, ring flags, zero-length
 // segments, non-finite velocities); those send the caller back to the call-by-call step for the reference's exact
@@ -142,6 +142,8 @@ struct ofw_step {
   unsigned long long* block = nullptr;
   int64_t pair_cap = 0;
   hipStream_t user_stream = nullptr, capture_stream = nullptr;
+  hipStream_t side = nullptr;                  // the incident pairs and the exclusion CSR, beside the tree build
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -188,7 +190,20 @@ __global__ void k_point_map(int64_t rows, int64_t ring_lo, int64_t rings, int64_
     map[q] = q < rows ? q : (q >= ring_lo ? rows + (q - ring_lo) : -1);
 }
 
-// One stage: the velocity kv[slot] of the free points `pts` at time t.
+// The incident pairs and the exclusion CSR of a stage, on the side stream (they feed the traversal only).
+static int stage_exclusions(ofw_step& p, const unsigned long long** pstat) {
+  Ctx& c = p.c;
+  const int64_t n = p.first + p.k, T = p.rows + p.rings;
+  int32_t* status = (int32_t*)(p.block + SB_STATUS);
+  int st = incident_pairs_enqueue(c, n, p.nodes, p.m, p.edges, p.first, p.pairs, p.pair_cap, pstat);
+  if (st) return st;
+  if (!*pstat) *pstat = p.block + SB_ZERO;
+  return exclusion_subset_enqueue(c, p.pair_cap, p.pairs, *pstat, 0, p.ring_lo + p.rings, T, p.point_map, p.m,
+                                  nullptr, 1, p.ex_ptr, p.ex_seg, status);
+}
+
+// One stage: the velocity kv[slot] of the free points `pts` at time t.  The incident pairs and the exclusion CSR
+// run on a side stream while the tree is built on this one; they join before the traversal.
 static int step_stage(ofw_step& p, const double* pts, double t, int slot) {
   Ctx& c = p.c;
   const int64_t n = p.first + p.k, T = p.rows + p.rings;
@@ -196,16 +211,19 @@ static int step_stage(ofw_step& p, const double* pts, double t, int slot) {
   const double f = (t - p.time) / p.dt_s;
   hipLaunchKernelGGL(k_stage_nodes, grid_for(c, 3 * n), dim3(256), 0, c.stream, p.first, p.k, p.bound, p.disp, f,
                      (int)moved, pts, p.nodes);
-  const unsigned long long* pstat = nullptr;
-  int st = incident_pairs_enqueue(c, n, p.nodes, p.m, p.edges, p.first, p.pairs, p.pair_cap, &pstat);
-  if (st) return st;
-  if (!pstat) pstat = p.block + SB_ZERO;
   int32_t* status = (int32_t*)(p.block + SB_STATUS);
-  st = network_segments_enqueue(c, n, p.nodes, p.m, p.edges, p.starts, p.dirs, p.lens, status);
+  int st = network_segments_enqueue(c, n, p.nodes, p.m, p.edges, p.starts, p.dirs, p.lens, status);
   if (st) return st;
-  st = exclusion_subset_enqueue(c, p.pair_cap, p.pairs, pstat, 0, p.ring_lo + p.rings, T, p.point_map, p.m, nullptr,
-                                1, p.ex_ptr, p.ex_seg, status);
+  const hipStream_t main = c.stream;
+  OFW_HIP(hipEventRecord(p.fork_ev, main));
+  OFW_HIP(hipStreamWaitEvent(p.side, p.fork_ev, 0));
+  c.stream = p.side;
+  const unsigned long long* pstat = nullptr;
+  st = stage_exclusions(p, &pstat);
+  const hipError_t e_join = hipEventRecord(p.join_ev, p.side);
+  c.stream = main;
   if (st) return st;
+  OFW_HIP(e_join);
   hipLaunchKernelGGL(k_gather_targets, grid_for(c, 3 * T), dim3(256), 0, c.stream, p.rows, p.ring_lo, p.rings, pts,
                      p.targets);
   p.tc.points = p.targets;
@@ -217,7 +235,11 @@ static int step_stage(ofw_step& p, const double* pts, double t, int slot) {
   p.tc.n_ex = p.pair_cap;
   p.tc.out = p.out;
   p.tc.flags = p.flags;
-  st = tree_pipeline(c, p.tc, p.t);
+  TreeDevice td;
+  st = tree_pipeline_build(c, p.tc, p.t, td);
+  if (st) return st;
+  OFW_HIP(hipStreamWaitEvent(main, p.join_ev, 0));
+  st = tree_pipeline_eval(c, p.tc, p.t, td);
   if (st) return st;
   hipLaunchKernelGGL(k_step_collect, dim3(1), dim3(64), 0, c.stream, pstat, p.pair_cap, p.t.counters, p.block);
   hipLaunchKernelGGL(k_stage_velocity, grid_for(c, p.k), dim3(256), 0, c.stream, p.k, p.inflow, p.rows, p.ring_lo,
@@ -310,6 +332,12 @@ int ofw_step_create(ofw_ctx* x, ofw_step** plan, int precision, int64_t first, c
       set_error("ofw_step_create: hipEventCreate failed");
       return fail(OFW_ERR_HIP);
     }
+  if (hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&p->fork_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&p->join_ev, hipEventDisableTiming) != hipSuccess) {
+    set_error("ofw_step_create: side stream or events could not be created");
+    return fail(OFW_ERR_HIP);
+  }
   p->bound = bound;
   p->disp = displacement;
   p->points_in = points_in;
@@ -452,6 +480,12 @@ int ofw_step_destroy(ofw_step* p) {
     (void)hipStreamSynchronize(p->capture_stream);
     (void)hipStreamDestroy(p->capture_stream);
   }
+  if (p->side) {
+    (void)hipStreamSynchronize(p->side);
+    (void)hipStreamDestroy(p->side);
+  }
+  if (p->fork_ev) (void)hipEventDestroy(p->fork_ev);
+  if (p->join_ev) (void)hipEventDestroy(p->join_ev);
   p->store.release();
   c.rec64.release();
   c.rec32.release();

@@ -55,12 +55,8 @@ static int midpoints(Ctx& c, int64_t m, const double* starts, const double* dire
   return tree_prep(c, m, starts, directions, lengths, *mx, *my, *mz);
 }
 
-int tree_pipeline(Ctx& c, const TreeCall& tc, TreeArrays& t) {
-  int32_t *p32 = nullptr, *s32 = nullptr;
-  if (tc.ex_ptr && tc.n_ex > 0) {
-    int st = csr_convert(c, tc.n, tc.n_ex, tc.ex_ptr, tc.ex_seg, &p32, &s32);
-    if (st) return st;
-  }
+// The part of a tree sum that needs the segments only: midpoints, build and node quantities.
+int tree_pipeline_build(Ctx& c, const TreeCall& tc, TreeArrays& t, TreeDevice& td) {
   OFW_HIP(c.mark());
   double *mx, *my, *mz;
   int st = midpoints(c, tc.m, tc.starts, tc.directions, tc.lengths, &mx, &my, &mz);
@@ -72,11 +68,20 @@ int tree_pipeline(Ctx& c, const TreeCall& tc, TreeArrays& t) {
   st = tree_build(c, t, mx, my, mz, tc.soff);
   if (st) return st;
   OFW_HIP(c.mark());
-  TreeDevice td;
   st = tree_moments(c, 1, t, tc.soff, tc.starts, tc.directions, tc.lengths, tc.gamma, tc.radius, tc.model,
                     tc.clearance, td, false);
   if (st) return st;
   OFW_HIP(c.mark());
+  return OFW_OK;
+}
+
+// The part that needs the targets and their exclusions: the traversal.
+int tree_pipeline_eval(Ctx& c, const TreeCall& tc, TreeArrays& t, const TreeDevice& td) {
+  int32_t *p32 = nullptr, *s32 = nullptr;
+  if (tc.ex_ptr && tc.n_ex > 0) {
+    int st = csr_convert(c, tc.n, tc.n_ex, tc.ex_ptr, tc.ex_seg, &p32, &s32);
+    if (st) return st;
+  }
   EvalShape shape;
   shape.K = tc.K;
   shape.n = tc.n;
@@ -84,11 +89,18 @@ int tree_pipeline(Ctx& c, const TreeCall& tc, TreeArrays& t) {
   shape.seg_off = tc.soff;
   shape.split_depth = tc.split_depth;
   shape.chunk_slots = tc.chunk_slots;
-  st = tree_evaluate(c, 2, t, td, shape, tc.precision, tc.points, tc.model, tc.raise_singular, p32, s32,
-                     p32 ? tc.n_ex : 0, tc.theta, tc.order, tc.out, tc.flags, tc.far_counts, tc.near_counts);
+  int st = tree_evaluate(c, 2, t, td, shape, tc.precision, tc.points, tc.model, tc.raise_singular, p32, s32,
+                         p32 ? tc.n_ex : 0, tc.theta, tc.order, tc.out, tc.flags, tc.far_counts, tc.near_counts);
   if (st) return st;
   OFW_HIP(c.mark());
   return OFW_OK;
+}
+
+int tree_pipeline(Ctx& c, const TreeCall& tc, TreeArrays& t) {
+  TreeDevice td;
+  int st = tree_pipeline_build(c, tc, t, td);
+  if (st) return st;
+  return tree_pipeline_eval(c, tc, t, td);
 }
 
 // The counters of a finished call: an error when a capacity was exceeded, else the statistics.

@@ -70,7 +70,10 @@ struct TreeCall {
   int64_t *far_counts = nullptr, *near_counts = nullptr;
 };
 
-// Enqueue a whole tree sum (midpoints, build, node quantities, traversal).
+// Enqueue a whole tree sum (midpoints, build, node quantities, traversal), or its two halves: the build and node
+// quantities need the segments only, the traversal the targets and their exclusions as well.
 int tree_pipeline(Ctx& c, const TreeCall& tc, TreeArrays& t);
+int tree_pipeline_build(Ctx& c, const TreeCall& tc, TreeArrays& t, TreeDevice& td);
+int tree_pipeline_eval(Ctx& c, const TreeCall& tc, TreeArrays& t, const TreeDevice& td);
 
 }  // namespace ofw
