@@ -25,7 +25,7 @@ namespace ofw {
 namespace {
 
 constexpr int RADIX_BITS = 8, RADIX = 1 << RADIX_BITS;
-constexpr int SORT_THREADS = 256, SORT_PER = 16, SORT_TILE = SORT_THREADS * SORT_PER;
+constexpr int SORT_THREADS = 256, SORT_PER = 8, SORT_TILE = SORT_THREADS * SORT_PER;
 constexpr int SORT_WAVES = SORT_THREADS / 64;
 static_assert(SORT_THREADS == RADIX, "one histogram bin per thread");
 
@@ -201,15 +201,24 @@ __global__ void k_end_groups(int m, int n, const int64_t* __restrict__ edges, co
     end_count[q] = c;
     mine += (unsigned long long)c;
   }
+  // one atomic per workgroup on the shared total: wavefront sums, then the workgroup's in LDS
+  __shared__ unsigned long long s_sum;
+  if (threadIdx.x == 0) s_sum = 0;
+  __syncthreads();
   for (int o = 32; o > 0; o >>= 1) mine += __shfl_xor(mine, o);
   const bool lead = (threadIdx.x & 63) == 0;
-  if (lead && mine) atomicAdd(&status[0], mine);
+  if (lead && mine) atomicAdd(&s_sum, mine);
   if (__builtin_amdgcn_ballot_w64(bad) != 0 && lead) atomicOr(&status[1], 1ull);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_sum) atomicAdd(&status[0], s_sum);
 }
 
 // One thread per pair p < min(total, capacity): its end q is the last one with offset[q] <= p (the ends after q
 // start beyond p, so end_count[q] > 0 and p lies inside q's range).  Nothing is written after a bad edge or when the
 // total exceeds 2^31 - 1 (the int32 offsets would not hold it); the range checks keep every access in bounds.
+// Each thread writes PAIR_RUN consecutive pairs: one binary search for the first, then a walk along the ends.
+constexpr int PAIR_RUN = 8;
+
 __global__ void k_write_pairs(int64_t capacity, int ends, const unsigned long long* __restrict__ status,
                               const int32_t* __restrict__ end_group, const int32_t* __restrict__ offset,
                               const int32_t* __restrict__ start, const int32_t* __restrict__ count,
@@ -217,20 +226,43 @@ __global__ void k_write_pairs(int64_t capacity, int ends, const unsigned long lo
   const unsigned long long all = status[0];
   if (status[1] || all > (unsigned long long)INT_MAX) return;
   const int64_t total = (int64_t)all < capacity ? (int64_t)all : capacity;
-  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < total; p += (int64_t)gridDim.x * blockDim.x) {
-    int lo = 0, hi = ends - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (offset[mid] <= p)
-        lo = mid;
+  const int64_t runs = (total + PAIR_RUN - 1) / PAIR_RUN;
+  for (int64_t run = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; run < runs;
+       run += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p0 = run * PAIR_RUN, p1 = p0 + PAIR_RUN < total ? p0 + PAIR_RUN : total;
+    int q = 0, hi = ends - 1;
+    while (q < hi) {
+      const int mid = (q + hi + 1) >> 1;
+      if (offset[mid] <= p0)
+        q = mid;
       else
         hi = mid - 1;
     }
-    const int g = end_group[lo];
-    const int64_t r = p - offset[lo];
-    if (g < 0 || r < 0 || r >= count[g]) continue;
-    pairs[2 * p] = members[start[g] + r];
-    pairs[2 * p + 1] = lo >> 1;
+    for (int64_t p = p0; p < p1; ++p) {
+      // the last end starting at or before p: a few steps forward, then a binary search over the rest (runs of
+      // ends without pairs can be long)
+      int steps = 0;
+      while (q + 1 < ends && offset[q + 1] <= p && steps < 4) {
+        ++q;
+        ++steps;
+      }
+      if (q + 1 < ends && offset[q + 1] <= p) {
+        int lo = q + 1, top = ends - 1;
+        while (lo < top) {
+          const int mid = (lo + top + 1) >> 1;
+          if (offset[mid] <= p)
+            lo = mid;
+          else
+            top = mid - 1;
+        }
+        q = lo;
+      }
+      const int g = end_group[q];
+      const int64_t r = p - offset[q];
+      if (g < 0 || r < 0 || r >= count[g]) continue;
+      pairs[2 * p] = members[start[g] + r];
+      pairs[2 * p + 1] = q >> 1;
+    }
   }
 }
 
@@ -336,7 +368,8 @@ static int pairs_count(Ctx& c, int64_t n, const double* nodes, int64_t m, const 
 static int pairs_write(Ctx& c, const PairScratch& s, int members_buf, int64_t capacity, int64_t* pairs) {
   if (capacity <= 0) return OFW_OK;
   scan_exclusive(c, s.E, s.end_count, s.offset, s.part, nullptr);
-  const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(c.cu_count * 8, (capacity + 255) / 256));
+  const int64_t runs = (capacity + PAIR_RUN - 1) / PAIR_RUN;
+  const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(c.cu_count * 8, (runs + 255) / 256));
   hipLaunchKernelGGL(k_write_pairs, dim3(grid), dim3(256), 0, c.stream, capacity, s.E, s.status, s.end_group,
                      s.offset, s.start, s.cnt, s.val[members_buf], pairs);
   OFW_HIP(hipGetLastError());
