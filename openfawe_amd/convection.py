@@ -335,8 +335,9 @@ def ring_induction(points, nodes, edges, gamma, radius, model):
 # ------------------------------------------------------------------ the step as one graph
 class GeometryStepPlan:
     """A geometry step recorded as one graph (ofw_step_*): the stage loop of wake.convect_geometry
-    (wake.py:586-616) over the target split of wake._network_convection for the tree code, one set of cores
-    (no ``ring_radii``), a uniform inflow and frozen rows that keep it (``frozen_rows_drift='ambient'``).
+    (wake.py:586-616) over the target split of wake._network_convection for either engine (the tree code or the
+    direct sum), one set of cores (no ``ring_radii``), a uniform inflow and frozen rows that keep it
+    (``frozen_rows_drift='ambient'``).
 
     Every stage's incident pairs, segment geometry, exclusion CSR, tree sum and RK arithmetic are replayed by one
     graph launch; the host reads one status block per step.  The plan keeps the tensors it was made with: a new
@@ -349,8 +350,8 @@ class GeometryStepPlan:
     def __init__(self, bound, points, edges, gamma, radii, settings, *, inflow, time, dt_s, substeps=1,
                  integrator="euler", displacement=None, ring_first=None, ring_free=None, use_graph=True):
         torch = _torch()
-        if settings.backend != "hip_tree" or not settings.free_wake or settings.frozen_rows_drift != "ambient":
-            raise ValueError("a recorded step needs backend='hip_tree', a free wake and frozen_rows_drift='ambient'")
+        if not settings.free_wake or settings.frozen_rows_drift != "ambient":
+            raise ValueError("a recorded step needs a free wake and frozen_rows_drift='ambient'")
         if integrator not in INTEGRATORS:
             raise ValueError(f"convection_integrator must be one of {INTEGRATORS}")
         if isinstance(substeps, bool) or int(substeps) != substeps or substeps < 1:
@@ -389,7 +390,8 @@ class GeometryStepPlan:
     def _create(self):
         import ctypes
         a, s = self.args, self.shape
-        tree, settings = a["settings"].tree, a["settings"]
+        settings = a["settings"]
+        tree = settings.tree if settings.backend == "hip_tree" else TreeSettings()
         handle = ctypes.c_void_p()
         stats = (ctypes.c_int64 * 3)()
         status = _lib.load().ofw_step_create(
@@ -398,7 +400,7 @@ class GeometryStepPlan:
             _ptr(a["gamma"]), _ptr(self.radii), settings.model, s["rows"], s["ring_lo"], s["ring_hi"],
             float(tree.theta), int(tree.order), int(tree.leaf_size), float(tree.core_clearance), _ptr(self.inflow),
             float(a["time"]), float(a["dt_s"]), a["substeps"], INTEGRATORS.index(a["integrator"]),
-            int(self.use_graph), stats)
+            0 if settings.backend == "hip_tree" else 1, int(settings.hip.block), int(self.use_graph), stats)
         _lib.check(status, "ofw_step_create")
         self.handle = handle
         self.capacities = dict(pairs=int(stats[1]), list_slots=int(stats[2]))

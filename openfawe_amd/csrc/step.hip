@@ -89,8 +89,10 @@ __global__ void k_step_collect(const unsigned long long* __restrict__ pair_statu
   if (pair_status[1]) bits |= OFW_STEP_EDGE_INDEX;
   if (total > (unsigned long long)pair_cap || total > (unsigned long long)INT_MAX) bits |= OFW_STEP_PAIR_OVERFLOW;
   atomicMax(&block[SB_PAIRS], total);
-  if (counters[C_OVERFLOW]) bits |= OFW_STEP_TREE_OVERFLOW;
-  atomicMax(&block[SB_CHUNKS], (unsigned long long)(counters[C_CHUNKS] > 0 ? counters[C_CHUNKS] : 0));
+  if (counters) {  // the tree code's counters (none for the direct sum)
+    if (counters[C_OVERFLOW]) bits |= OFW_STEP_TREE_OVERFLOW;
+    atomicMax(&block[SB_CHUNKS], (unsigned long long)(counters[C_CHUNKS] > 0 ? counters[C_CHUNKS] : 0));
+  }
   if (bits) atomicOr(&block[SB_STATUS], bits);
 }
 
@@ -133,6 +135,9 @@ struct ofw_step {
   int64_t first = 0, k = 0, m = 0, rows = 0, ring_lo = 0, rings = 0;
   double time = 0.0, dt_s = 0.0;
   int substeps = 1, integrator = 0;
+  int engine = 0;                              // 0 the tree code, 1 the direct sum
+  int precision = OFW_MIXED, threads = 256, model = 0;  // threads: the direct sum's workgroup size
+  const double *gamma = nullptr, *radius = nullptr;
   // owned arrays
   ofw::DevBuf store;
   double *nodes = nullptr, *stage_pts = nullptr, *kv[4] = {}, *targets = nullptr, *out = nullptr;
@@ -235,13 +240,28 @@ static int step_stage(ofw_step& p, const double* pts, double t, int slot) {
   p.tc.n_ex = p.pair_cap;
   p.tc.out = p.out;
   p.tc.flags = p.flags;
-  TreeDevice td;
-  st = tree_pipeline_build(c, p.tc, p.t, td);
-  if (st) return st;
-  OFW_HIP(hipStreamWaitEvent(main, p.join_ev, 0));
-  st = tree_pipeline_eval(c, p.tc, p.t, td);
-  if (st) return st;
-  hipLaunchKernelGGL(k_step_collect, dim3(1), dim3(64), 0, c.stream, pstat, p.pair_cap, p.t.counters, p.block);
+  if (p.engine == 0) {
+    TreeDevice td;
+    st = tree_pipeline_build(c, p.tc, p.t, td);
+    if (st) return st;
+    OFW_HIP(hipStreamWaitEvent(main, p.join_ev, 0));
+    st = tree_pipeline_eval(c, p.tc, p.t, td);
+    if (st) return st;
+  } else {
+    // the direct sum: the segment records beside the exclusions, then the sum over every target
+    st = pack_segments(c, p.m, p.starts, p.dirs, p.lens, p.gamma, p.radius, p.model, (double*)c.rec64.ptr,
+                       (float*)c.rec32.ptr);
+    if (st) return st;
+    OFW_HIP(hipStreamWaitEvent(main, p.join_ev, 0));
+    int32_t *p32 = nullptr, *s32 = nullptr;
+    st = csr_convert(c, T, p.pair_cap, p.ex_ptr, p.ex_seg, &p32, &s32);
+    if (st) return st;
+    st = direct_sum(c, p.precision, p.threads, T, p.targets, p.m, (const double*)c.rec64.ptr,
+                    (const float*)c.rec32.ptr, p.model, 1, p32, s32, p.out, p.flags, nullptr);
+    if (st) return st;
+  }
+  hipLaunchKernelGGL(k_step_collect, dim3(1), dim3(64), 0, c.stream, pstat, p.pair_cap,
+                     p.engine == 0 ? (const int32_t*)p.t.counters : nullptr, p.block);
   hipLaunchKernelGGL(k_stage_velocity, grid_for(c, p.k), dim3(256), 0, c.stream, p.k, p.inflow, p.rows, p.ring_lo,
                      p.rings, p.out, p.flags, p.kv[slot], p.block);
   OFW_HIP(hipGetLastError());
@@ -306,11 +326,13 @@ int ofw_step_create(ofw_ctx* x, ofw_step** plan, int precision, int64_t first, c
                     const int64_t* edges, const double* gamma, const double* radius, int model, int64_t rows,
                     int64_t ring_lo, int64_t ring_hi, double theta, int expansion_order, int64_t leaf_size,
                     double clearance, const double* inflow, double time, double dt_s, int substeps, int integrator,
-                    int use_graph, int64_t* stats_host) {
+                    int engine, int block, int use_graph, int64_t* stats_host) {
+  const int max_model = engine == 0 ? 2 : 5;
   if (!x || !plan || first < 1 || k < 1 || m < 1 || !bound || !points_in || !points_out || !edges || !gamma ||
+      engine < 0 || engine > 1 || !(block == 32 || block == 64 || block == 128 || block == 256) || model > max_model ||
       !radius || !inflow || rows < 0 || rows > k || ring_lo < rows || ring_hi < ring_lo || ring_hi > k ||
       rows + (ring_hi - ring_lo) < 1 || substeps < 1 || integrator < 0 || integrator > 2 || !(dt_s > 0.0) ||
-      precision < 0 || precision > 2 || model < 0 || model > 2 || leaf_size < 1 || leaf_size > (1 << 20) ||
+      precision < 0 || precision > 2 || model < 0 || leaf_size < 1 || leaf_size > (1 << 20) ||
       expansion_order < 0 || expansion_order > 2 || !(theta > 0.0 && theta < 1.0) || !(clearance >= 0.0) ||
       first + k > (int64_t)1 << 28 || m > (int64_t)1 << 28) {
     set_error("ofw_step_create: invalid arguments");
@@ -354,6 +376,19 @@ int ofw_step_create(ofw_ctx* x, ofw_step** plan, int precision, int64_t first, c
   p->dt_s = dt_s;
   p->substeps = substeps;
   p->integrator = integrator;
+  p->engine = engine;
+  p->precision = precision;
+  p->threads = block;
+  p->model = model;
+  p->gamma = gamma;
+  p->radius = radius;
+  if (engine == 1) {
+    if (c.rec64.reserve((size_t)m * REC64 * sizeof(double)) != hipSuccess ||
+        c.rec32.reserve((size_t)m * REC32 * sizeof(float)) != hipSuccess) {
+      set_error("ofw_step_create: out of device memory");
+      return fail(OFW_ERR_HIP);
+    }
+  }
   const int64_t T = rows + p->rings;
   ofw::TreeCall& tc = p->tc;
   tc.precision = precision;

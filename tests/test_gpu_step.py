@@ -108,3 +108,19 @@ def test_fallbacks_and_capacity_overflow():
     want = eager(crowd, edges, gamma, first, TREE, **kw)
     assert np.array_equal(got.view(np.int64), want.view(np.int64))
     plan.close()
+
+
+@pytest.mark.parametrize("precision,model", [("fp64", 2), ("mixed", 2), ("fp64", 3)])
+@pytest.mark.parametrize("integrator", ["euler", "rk4"])
+def test_direct_engine_replays(precision, model, integrator):
+    print(f"step plan, direct engine: {precision} model {model} {integrator}", flush=True)
+    nodes, edges, gamma, first, ring_first, disp = case(depth=8, width=11, rings=2)
+    settings = CV.ConvectionSettings(model=model, hip=V.HipSettings(precision, 128), free_convection_rows=5)
+    kw = dict(inflow=[-3.0, 0.2, 0.0], time=0.25, dt_s=0.04, integrator=integrator, displacement=dev(disp),
+              ring_first=ring_first)
+    want = eager(nodes, edges, gamma, first, settings, **kw)
+    plan = CV.GeometryStepPlan(dev(nodes[:first]), dev(nodes[first:]), dev(edges), dev(gamma), 0.15, settings, **kw)
+    for _ in range(2):
+        assert plan.run()[0] == 0
+        assert np.array_equal(plan.points_out.cpu().numpy().view(np.int64), want.view(np.int64))
+    plan.close()
