@@ -10,7 +10,10 @@
 * ``edge_radii`` -- cores.edge_radii (cores.py:129-149) on the GPU (``ofw_edge_radii``).
 * ``history_induction`` -- history_vsm_induction (vsm_wake.py:255-278) for one surface class: the wake history's
   rings (``pack_rings`` on the GPU) with their material cores summed at the closure and the load points (FP64
-  direct sum).  The two-class cross-kernel split (history_vsm_parts, OF-0010 P2) is not modelled.
+  direct sum).
+* ``history_induction_classes`` -- the same with surface classes (history_vsm_parts, vsm_wake.py:281-300): own-class
+  edges with the material kernel, other-class edges with the Rosenhead-Moore kernel and the larger of the AVL and
+  material cores.
 """
 import numpy as np
 
@@ -138,5 +141,70 @@ def history_induction(closure_points, load_points, rows, strengths, panel_edges,
     out, flags, flag_seg = hip_sum_device(points, s, d, L, gamma, radii, int(_model(model)), ptr, seg, "raise",
                                           settings)
     _check(out, flags, flag_seg)
+    out = out.cpu().numpy()
+    return out[:len(closure)], out[len(closure):]
+
+
+CROSS_MODEL = 5  # Rosenhead-Moore: the finite core between surface classes (surface_cores.py:31)
+
+
+def history_induction_classes(closure_points, load_points, rows, strengths, panel_edges, model, core_radius,
+                              panel_class, node_class, panel_radius, node_radius, span=None, stream=None,
+                              settings=HipSettings("fp64")):
+    """history_vsm_induction with surface classes (vsm_wake.py:268-300, history_vsm_parts): the points of each class
+    take the history edges of their own class with the material kernel and cores, and the edges of the other
+    classes with the Rosenhead-Moore kernel and max(AVL radius, material core) (surface_cores.py:112-131: a spanwise
+    edge its strip's radius, a streamwise edge its node's).  Closure and load points are one per panel, of the
+    panel's class.  Returns (at closure points, at load points), the sum of the two parts."""
+    torch = _torch()
+    closure = np.asarray(closure_points, dtype=np.float64).reshape(-1, 3)
+    load = np.asarray(load_points, dtype=np.float64).reshape(-1, 3)
+    panel_class = np.asarray(panel_class, dtype=np.int64)
+    node_class = np.asarray(node_class, dtype=np.int64)
+    if len(closure) != len(panel_class) or len(load) != len(panel_class):
+        raise ValueError("one closure and one load point per panel")
+    if len(rows) <= 1:
+        return np.zeros_like(closure), np.zeros_like(load)
+    rows, strengths, panels = NET._check_rings(np.asarray(rows, dtype=np.float64), strengths, panel_edges)
+    depth, width = rows.shape[:2]
+    if len(node_class) != width or len(panel_radius) != len(panels) or len(node_radius) != width:
+        raise ValueError("one class per node and panel, one radius per panel and node")
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda")  # noqa: E731
+    edges, gamma = NET.pack_ring_edges_device(depth, width, dev(panels), dev(strengths))
+    nodes = dev(rows.reshape(-1, 3))
+    radii = edge_radii(edges, width, panels, core_radius, span, stream, offset=2 * width)
+    # the AVL radius of every edge (rows from two on): the strips' radius per row for spanwise edges, the nodes'
+    # for streamwise edges
+    avl = edge_radii(edges, width, panels, 0.0, np.tile(np.asarray(panel_radius, np.float64), (depth + 2, 1)),
+                     np.tile(np.asarray(node_radius, np.float64), (depth + 2, 1)), offset=2 * width)
+    cross_radii = torch.maximum(radii, avl)
+    source = dev(node_class)[edges[:, 0] % width]
+    points = np.concatenate([closure, load])
+    point_class = np.concatenate([panel_class, panel_class])
+    out = torch.zeros((len(points), 3), dtype=torch.float64, device="cuda")
+    geometry = network_segments(nodes, edges)
+
+    def part(target, keep, radius, kernel):
+        sub = torch.nonzero(keep).reshape(-1)
+        if sub.numel() == 0:
+            return None
+        pts = dev(points[target])
+        ptr = torch.zeros(len(target) + 1, dtype=torch.int64, device="cuda")
+        seg = torch.zeros(1, dtype=torch.int64, device="cuda")
+        s, d, L = (g[sub].contiguous() for g in geometry)
+        res, flags, flag_seg = hip_sum_device(pts, s, d, L, gamma[sub].contiguous(), radius[sub].contiguous(),
+                                              kernel, ptr, seg, "raise", settings)
+        _check(res, flags, flag_seg)
+        return res
+
+    model = int(_model(model))
+    for c in np.unique(panel_class):
+        target = np.flatnonzero(point_class == c)
+        same = source == int(c)
+        index = torch.from_numpy(target).to("cuda")
+        for keep, radius, kernel in ((same, radii, model), (~same, cross_radii, CROSS_MODEL)):
+            res = part(target, keep, radius, kernel)
+            if res is not None:
+                out[index] += res
     out = out.cpu().numpy()
     return out[:len(closure)], out[len(closure):]

@@ -84,3 +84,39 @@ def history_induction(closure, load, rows, strengths, panel_edges, model, core_r
     out, flags = O.direct_sum(points, *_geometry(rows.reshape(-1, 3), edges), gamma, radii, model)
     assert not flags.any()
     return out[:len(closure)], out[len(closure):]
+
+
+def history_induction_classes(closure, load, rows, strengths, panel_edges, model, core_radius, panel_class, node_class,
+                              panel_radius, node_radius, span=None, stream=None):
+    """history_vsm_parts (vsm_wake.py:281-300) with the AVL radii of surface_cores.py:112-131."""
+    rows = np.asarray(rows, dtype=np.float64)
+    depth, width = rows.shape[:2]
+    panel_edges = np.asarray(panel_edges, dtype=np.int64)
+    edges, gamma = network_np.pack_ring_edges_loop(depth, width, panel_edges, strengths)
+    glob = edges + 2 * width
+    radii = edge_radii(glob, width, panel_edges, core_radius, span, stream)
+    lookup = {}
+    for j, (a, b) in enumerate(panel_edges):
+        lookup[(a, b)] = lookup[(b, a)] = j
+    avl = np.empty(len(edges))
+    for i, (e0, e1) in enumerate(glob):
+        ra, a = divmod(int(e0), width)
+        rb, b = divmod(int(e1), width)
+        avl[i] = panel_radius[lookup[(a, b)]] if ra == rb else node_radius[a]
+    cross = np.maximum(radii, avl)
+    source = np.asarray(node_class)[edges[:, 0] % width]
+    points = np.concatenate([closure, load])
+    point_class = np.concatenate([panel_class, panel_class])
+    nodes = rows.reshape(-1, 3)
+    same_part, cross_part = np.zeros_like(points), np.zeros_like(points)
+    for c in np.unique(panel_class):
+        target = np.flatnonzero(point_class == c)
+        same = source == c
+        for keep, part, kernel, rad in ((same, same_part, model, radii), (~same, cross_part, 5, cross)):
+            if np.any(keep):
+                out, flags = O.direct_sum(points[target], *_geometry(nodes, edges[keep]), gamma[keep], rad[keep],
+                                          kernel)
+                assert not flags.any()
+                part[target] = out
+    total = same_part + cross_part
+    return total[:len(closure)], total[len(closure):]

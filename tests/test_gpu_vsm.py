@@ -82,3 +82,36 @@ def test_history_induction(with_cores):
         np.testing.assert_allclose(g, w, rtol=0, atol=1e-11 * np.abs(w).max())
     zeros = vsm.history_induction(closure, load, history[:1], strengths[:0], panels, 2, 0.15)
     assert not np.any(zeros[0]) and not np.any(zeros[1])
+
+
+def test_history_induction_with_surface_classes():
+    depth = 24
+    rows, _, closure_all, load_all = wing_rows(depth + 2, width=11)
+    # class 0: nodes 0-6 (panels 0-5); class 1: nodes 7-10 (panels 7-9); no panel joins the two
+    panels = np.array([[0, 1], [1, 2], [2, 3], [3, 4], [4, 5], [5, 6], [7, 8], [8, 9], [9, 10]])
+    keep = [0, 1, 2, 3, 4, 5, 7, 8, 9]
+    closure, load = closure_all[keep], load_all[keep]
+    panel_class = np.array([0] * 6 + [1] * 3)
+    node_class = np.array([0] * 7 + [1] * 4)
+    panel_radius = 0.2 + 0.05 * np.arange(9)
+    node_radius = np.zeros(11)
+    for j, (a, b) in enumerate(panels):
+        node_radius[a] = max(node_radius[a], panel_radius[j])
+        node_radius[b] = max(node_radius[b], panel_radius[j])
+    strengths = 1.0 + 0.2 * np.random.default_rng(4).normal(size=(depth - 1, len(panels)))
+    history = rows[2:]
+    span, stream = cores(depth + 2, 11)
+    span = span[:, :len(panels)]
+    for s_, st_ in ((None, None), (span, stream)):
+        got = vsm.history_induction_classes(closure, load, history, strengths, panels, 2, 0.15, panel_class,
+                                            node_class, panel_radius, node_radius, s_, st_)
+        want = VO.history_induction_classes(closure, load, history, strengths, panels, 2, 0.15, panel_class,
+                                            node_class, panel_radius, node_radius, s_, st_)
+        for g, w in zip(got, want):
+            np.testing.assert_allclose(g, w, rtol=0, atol=1e-11 * np.abs(w).max())
+    # one class: the single-class history induction
+    one = vsm.history_induction_classes(closure, load, history, strengths, panels, 2, 0.15, np.zeros(9, int),
+                                        np.zeros(11, int), panel_radius, node_radius)
+    plain = vsm.history_induction(closure, load, history, strengths, panels, 2, 0.15)
+    for g, w in zip(one, plain):
+        np.testing.assert_allclose(g, w, rtol=0, atol=1e-12 * np.abs(w).max())
