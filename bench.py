@@ -364,7 +364,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this build "
-              f"(targets shard across GPUs without communication; see DESIGN.md)", file=sys.stderr)
+              f"(targets shard across GPUs without communication; this measures one GPU's share)", file=sys.stderr)
         sys.exit(2)
 
     if args.copy_calibration:

@@ -4,7 +4,7 @@
  * This file restates, in plain C, the reference algorithm of OpenFAWE's
  * Biot-Savart segment sum (direct and tree).  It exists to check the HIP
  * product: only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg
- * may load it, and only as the checker / the timed CPU baseline.  The product
+ * may load it, and only to check results or as the timed CPU baseline.  The product
  * library (openfawe_amd/libofw_hip.so) never links or calls it.
  *
  * Every function cites the reference file:line it follows.  Arithmetic is

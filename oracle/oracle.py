@@ -2,7 +2,7 @@
 
 The oracle restates the reference algorithm of OpenFAWE's Biot-Savart segment
 sum on the CPU (see ofw_oracle.c for the file:line each function follows).  It
-is the checker for the HIP product and the CPU baseline of bench.py; the
+checks the HIP product's results and is the CPU baseline of bench.py; the
 product package ``openfawe_amd`` never imports it.
 
 Signatures follow the reference's engine convention: float64 C-contiguous
