@@ -388,6 +388,10 @@ static constexpr int MERGE_THREADS = 512;
 static constexpr int MERGE_E = 8;                                // outputs per thread
 static constexpr int MERGE_OUT = MERGE_THREADS * MERGE_E;        // outputs per work item
 static constexpr int MERGE_SLICES = TILE / MERGE_OUT;            // work items per sort tile
+// every merge pass must write its whole output (the host swaps the buffers after each pass), and a slice must
+// not straddle a pair of runs: whole slices per tile
+static_assert(TILE >= MERGE_OUT && TILE % MERGE_OUT == 0, "a sort tile holds a whole number of merge slices");
+static_assert(TILE % TILE_THREADS == 0 && (TILE_E & (TILE_E - 1)) == 0, "a power-of-two tile sort per thread");
 
 // number of elements taken from run A among the first `diag` outputs of merging A (la) and B (lb)
 __device__ __forceinline__ int merge_split(const uint64_t* __restrict__ ka, const uint32_t* __restrict__ pa, int la,
