@@ -10,7 +10,7 @@
 // 64-bit image of the double and -0.0 folded into +0.0 (np.argsort compares with <).
 //
 //  * Top phase, nodes of more than S_BOTTOM segments, level by level: box reduction over tiles, sort of
-//    8,192-element tiles (registers, then in-place LDS merge rounds), merge-path passes between tiles.  A
+//    4,096-element tiles (registers, then in-place LDS merge rounds), merge-path passes between tiles.  A
 //    one-workgroup plan kernel lists each level's nodes, tiles and deferred subtrees; a one-workgroup kernel
 //    creates the children.
 //  * Bottom phase: each node of at most S_BOTTOM segments is finished by one workgroup entirely in LDS,
@@ -29,7 +29,7 @@
 
 namespace ofw {
 
-static constexpr int TILE = 8192;           // elements per LDS tile of the top phase
+static constexpr int TILE = 4096;           // elements per LDS tile of the top phase
 static constexpr int TILE_THREADS = 1024;
 static constexpr int TILE_E = TILE / TILE_THREADS;  // elements per thread of the tile sort
 static constexpr int S_BOTTOM = 2048;       // largest subtree finished in LDS
