@@ -11,8 +11,8 @@ namespace ofw {
 // Device counters of a build (one int32 array; graph-capturable: nothing is read back during the build).
 enum {
   C_NODES = 0,      // nodes created (global ids are dense from 0; some heap slots may stay unused)
-  C_LEVEL,          // nodes in the current top level list
-  C_NEXT,           // nodes in the next top level list
+  C_LEVEL,          // roots of the forest (the first top level)
+  C_NEXT,           // unused
   C_TILES,          // tiles of the current level
   C_BIG,            // nodes of the current level sorted by tiles
   C_BOTTOM,         // subtree roots of the bottom phase
@@ -25,6 +25,7 @@ enum {
   C_DEPTH,          // largest node depth
   C_CHUNKS,         // near-field work items of the traversal
   C_SUBTREES,       // subtrees of the moment pass
+  C_STRAND,         // top-phase nodes whose segments stay in the second order buffer
   C_COUNT = 16
 };
 
@@ -47,12 +48,12 @@ struct TreeArrays {
   int32_t *lo, *hi, *left, *right, *parent, *depth, *tree;
   int8_t *axis, *split;
   int32_t* counters;
-  int32_t *level_a, *level_b, *big, *big_t0, *p_flag;
+  int32_t *level_a, *big, *big_alt, *strand, *p_flag;
   int32_t *tile_node, *tile_p0, *tile_p1;
   double* box_partial;
   uint64_t *keyA, *keyB;
   uint32_t *posA, *posB;
-  int32_t* tmp;
+  int32_t* tmp;                  // second order buffer of the top phase (odd depths)
   int32_t *bottom, *heap_base;
   // finish: pre-order layout
   int32_t *lo_cnt, *lo_mind, *lo_base, *scan_part;

@@ -107,8 +107,12 @@ int upload_i32(Ctx& c, const std::vector<int64_t>& values, int32_t* dst) {
 }
 
 // ------------------------------------------------------------------ top phase
-// One workgroup: nodes of the level with more than S_BOTTOM (and `leaf`) segments get tiles; nodes with
-// leaf < size <= S_BOTTOM are deferred to the bottom phase.
+// Classes of a top-phase node: a big node (more than S_BOTTOM and `leaf` segments) is sorted by tiles, a node of
+// leaf < size <= S_BOTTOM is deferred to the bottom phase, any other node is a leaf.
+__host__ __device__ __forceinline__ bool is_big(int size, int leaf) { return size > S_BOTTOM && size > leaf; }
+__host__ __device__ __forceinline__ bool is_small(int size, int leaf) { return size > leaf && size <= S_BOTTOM; }
+
+// One workgroup: the plan of the first level (the roots): big roots get tiles, small ones go to the bottom.
 __global__ __launch_bounds__(PLAN_THREADS) void k_level_plan(TreeArrays t, const int32_t* __restrict__ level) {
   const int count = live_count(t.counters, C_LEVEL);
   int nbig = 0, ntiles = 0, nsmall = 0;
@@ -120,8 +124,8 @@ __global__ __launch_bounds__(PLAN_THREADS) void k_level_plan(TreeArrays t, const
       node = level[i];
       size = t.hi[node] - t.lo[node];
     }
-    const bool big = node >= 0 && size > S_BOTTOM && size > t.leaf;
-    const bool small = node >= 0 && size > t.leaf && size <= S_BOTTOM;
+    const bool big = node >= 0 && is_big(size, t.leaf);
+    const bool small = node >= 0 && is_small(size, t.leaf);
     const int tiles = big ? (size + TILE - 1) / TILE : 0;
     int tb, tt, ts;
     const int rb = block_exclusive_scan(big ? 1 : 0, &tb);
@@ -132,7 +136,6 @@ __global__ __launch_bounds__(PLAN_THREADS) void k_level_plan(TreeArrays t, const
       const int t0 = ntiles + rt;
       if (bi < t.cap && t0 + tiles <= t.tile_cap) {
         t.big[bi] = node;
-        t.big_t0[bi] = t0;  // first tile of this big node
         for (int q = 0; q < tiles; ++q) {
           t.tile_node[t0 + q] = node;
           t.tile_p0[t0 + q] = t.lo[node] + q * TILE;
@@ -193,8 +196,9 @@ __device__ __forceinline__ void sub_range(int p0, int p1, int sub, int& a, int& 
   b = min(p1, a + chunk);
 }
 
-__global__ __launch_bounds__(256) void k_box_tiles(TreeArrays t, const double* __restrict__ mx,
-                                                   const double* __restrict__ my, const double* __restrict__ mz) {
+__global__ __launch_bounds__(256) void k_box_tiles(TreeArrays t, const int32_t* __restrict__ order_in,
+                                                   const double* __restrict__ mx, const double* __restrict__ my,
+                                                   const double* __restrict__ mz) {
   __shared__ double red[4][6];
   const int ntiles = live_count(t.counters, C_TILES);
   for (int wu = blockIdx.x; wu < ntiles * SUBS; wu += gridDim.x) {
@@ -203,7 +207,7 @@ __global__ __launch_bounds__(256) void k_box_tiles(TreeArrays t, const double* _
     sub_range(t.tile_p0[tile], t.tile_p1[tile], wu % SUBS, p0, p1);
     double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
     for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-      const int s = t.order[p];
+      const int s = order_in[p];
       const double c[3] = {mx[s], my[s], mz[s]};
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
@@ -225,26 +229,6 @@ __global__ __launch_bounds__(256) void k_box_tiles(TreeArrays t, const double* _
       for (int q = 0; q < 6; ++q) t.box_partial[6 * (size_t)wu + q] = v[q];
     }
     __syncthreads();
-  }
-}
-
-// One wavefront per big node: its box from the partial boxes of its tiles (exact min / max, in any order).
-__global__ __launch_bounds__(256) void k_box_combine(TreeArrays t) {
-  const int nbig = live_count(t.counters, C_BIG);
-  const int lane = threadIdx.x & 63;
-  for (int i = blockIdx.x * 4 + (int)(threadIdx.x >> 6); i < nbig; i += gridDim.x * 4) {
-    const int node = t.big[i];
-    const int t0 = t.big_t0[i];
-    const int tiles = (t.hi[node] - t.lo[node] + TILE - 1) / TILE;
-    double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-    for (int q = t0 * SUBS + lane; q < (t0 + tiles) * SUBS; q += 64)
-      for (int c = 0; c < 3; ++c) {
-        const double lo = t.box_partial[6 * (size_t)q + c], hi = t.box_partial[6 * (size_t)q + 3 + c];
-        v[c] = lo < v[c] ? lo : v[c];
-        v[3 + c] = hi > v[3 + c] ? hi : v[3 + c];
-      }
-    box_wave_reduce(v);
-    if (lane == 0) choose_axis(v, &t.axis[node], &t.split[node]);
   }
 }
 
@@ -345,7 +329,49 @@ struct PairLess {
   }
 };
 
-__global__ __launch_bounds__(TILE_THREADS) void k_sort_tiles(TreeArrays t, const double* __restrict__ mx,
+// Box of a big node from the partial boxes of all its tiles, by every workgroup that sorts one of them (exact
+// min / max, so every tile finds the same box); the reference's axis choice and split flag follow.
+__device__ void node_axis(const TreeArrays& t, int tile, int node, int* axis_out, bool* split_out) {
+  __shared__ double red[TILE_THREADS / 64][6];
+  __shared__ int8_t s_axis, s_split;
+  const int lo = t.lo[node], hi = t.hi[node];
+  const int t0 = tile - (t.tile_p0[tile] - lo) / TILE;  // first tile of the node
+  const int w1 = (t0 + (hi - lo + TILE - 1) / TILE) * SUBS;
+  double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  for (int w = t0 * SUBS + (int)threadIdx.x; w < w1; w += blockDim.x)
+    for (int c = 0; c < 3; ++c) {
+      const double l = t.box_partial[6 * (size_t)w + c], h = t.box_partial[6 * (size_t)w + 3 + c];
+      v[c] = l < v[c] ? l : v[c];
+      v[3 + c] = h > v[3 + c] ? h : v[3 + c];
+    }
+  box_wave_reduce(v);
+  if ((threadIdx.x & 63) == 0)
+    for (int q = 0; q < 6; ++q) red[threadIdx.x / 64][q] = v[q];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < (int)(blockDim.x / 64); ++k)
+      for (int q = 0; q < 3; ++q) {
+        v[q] = red[k][q] < v[q] ? red[k][q] : v[q];
+        v[3 + q] = red[k][3 + q] > v[3 + q] ? red[k][3 + q] : v[3 + q];
+      }
+    choose_axis(v, &s_axis, &s_split);
+    if (t0 == tile) {  // the node's first tile records it for the merge passes and the next plan
+      t.axis[node] = s_axis;
+      t.split[node] = s_split;
+    }
+  }
+  __syncthreads();
+  *axis_out = s_axis;
+  *split_out = s_split != 0;
+  __syncthreads();  // the next tile reuses red and the flags
+}
+
+// Sort of each tile of the level's big nodes by (key, position).  With no merge pass after it (every node fits
+// one tile) the sorted positions are applied at once: order_out[p] = order_in[sorted position], else the sorted
+// pairs go to keyA / posA.
+__global__ __launch_bounds__(TILE_THREADS) void k_sort_tiles(TreeArrays t, const int32_t* __restrict__ order_in,
+                                                             int32_t* __restrict__ order_out,
+                                                             const double* __restrict__ mx,
                                                              const double* __restrict__ my,
                                                              const double* __restrict__ mz) {
   __shared__ uint64_t key[TILE];
@@ -353,16 +379,18 @@ __global__ __launch_bounds__(TILE_THREADS) void k_sort_tiles(TreeArrays t, const
   const int ntiles = live_count(t.counters, C_TILES);
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int node = t.tile_node[tile];
-    if (!t.split[node]) continue;  // uniform over the workgroup
+    int axis;
+    bool split;
+    node_axis(t, tile, node, &axis, &split);
+    if (!split) continue;  // uniform over the workgroup
     const int p0 = t.tile_p0[tile], p1 = t.tile_p1[tile];
     const int len = p1 - p0;
     int width = TILE_E;
     while (width < len) width <<= 1;
-    const int axis = t.axis[node];
     const double* coord = axis == 0 ? mx : (axis == 1 ? my : mz);
     for (int i = threadIdx.x; i < width; i += blockDim.x) {
       if (i < len) {
-        key[i] = ordered_key(coord[t.order[p0 + i]]);
+        key[i] = ordered_key(coord[order_in[p0 + i]]);
         pos[i] = (uint32_t)(p0 + i);
       } else {
         key[i] = ~0ull;
@@ -371,9 +399,13 @@ __global__ __launch_bounds__(TILE_THREADS) void k_sort_tiles(TreeArrays t, const
     }
     __syncthreads();
     block_merge_sort<TILE_E>(key, pos, width, PairLess());
-    for (int i = threadIdx.x; i < len; i += blockDim.x) {
-      t.keyA[p0 + i] = key[i];
-      t.posA[p0 + i] = pos[i];
+    if (order_out) {
+      for (int i = threadIdx.x; i < len; i += blockDim.x) order_out[p0 + i] = order_in[pos[i]];
+    } else {
+      for (int i = threadIdx.x; i < len; i += blockDim.x) {
+        t.keyA[p0 + i] = key[i];
+        t.posA[p0 + i] = pos[i];
+      }
     }
     __syncthreads();
   }
@@ -442,11 +474,15 @@ __device__ int wave_merge_split(const uint64_t* __restrict__ ka, const uint32_t*
   }
 }
 
+// The level's last pass (order_out not null) applies the sorted positions, order_out[q] = order_in[position],
+// instead of storing the pairs.
 __global__ __launch_bounds__(MERGE_THREADS) void k_merge_pass(TreeArrays t, int width,
                                                               const uint64_t* __restrict__ in_key,
                                                               const uint32_t* __restrict__ in_pos,
                                                               uint64_t* __restrict__ out_key,
-                                                              uint32_t* __restrict__ out_pos) {
+                                                              uint32_t* __restrict__ out_pos,
+                                                              const int32_t* __restrict__ order_in,
+                                                              int32_t* __restrict__ order_out) {
   __shared__ uint64_t s_key[MERGE_OUT];
   __shared__ uint32_t s_pos[MERGE_OUT];
   __shared__ int s_split[2];
@@ -514,79 +550,134 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_merge_pass(TreeArrays t, int 
         s_pos[o_begin + e] = p[e];
       }
     __syncthreads();
-    for (int e = threadIdx.x; e < na + nb; e += blockDim.x) {
-      out_key[q0 + e] = s_key[e];
-      out_pos[q0 + e] = s_pos[e];
+    if (order_out) {
+      for (int e = threadIdx.x; e < na + nb; e += blockDim.x) order_out[q0 + e] = order_in[s_pos[e]];
+    } else {
+      for (int e = threadIdx.x; e < na + nb; e += blockDim.x) {
+        out_key[q0 + e] = s_key[e];
+        out_pos[q0 + e] = s_pos[e];
+      }
     }
     __syncthreads();  // the next work item reuses the LDS
   }
 }
 
-__global__ __launch_bounds__(256) void k_apply_perm(TreeArrays t, const uint32_t* __restrict__ sorted_pos) {
-  const int ntiles = live_count(t.counters, C_TILES);
-  for (int wu = blockIdx.x; wu < ntiles * SUBS; wu += gridDim.x) {
-    const int tile = wu / SUBS;
-    if (!t.split[t.tile_node[tile]]) continue;
-    int p0, p1;
-    sub_range(t.tile_p0[tile], t.tile_p1[tile], wu % SUBS, p0, p1);
-    for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) t.tmp[p] = t.order[sorted_pos[p]];
-  }
-}
-
-__global__ __launch_bounds__(256) void k_copy_tiles(TreeArrays t) {
-  const int ntiles = live_count(t.counters, C_TILES);
-  for (int wu = blockIdx.x; wu < ntiles * SUBS; wu += gridDim.x) {
-    const int tile = wu / SUBS;
-    if (!t.split[t.tile_node[tile]]) continue;
-    int p0, p1;
-    sub_range(t.tile_p0[tile], t.tile_p1[tile], wu % SUBS, p0, p1);
-    for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) t.order[p] = t.tmp[p];
-  }
-}
-
-// One workgroup: children of the big nodes that split (vortex_tree.py:244-268); they form the next level.
-__global__ __launch_bounds__(PLAN_THREADS) void k_level_children(TreeArrays t, int32_t* __restrict__ next) {
+// One workgroup after the sort of level `lvl`: the children of its big nodes that split (vortex_tree.py:244-268)
+// form level lvl + 1, which is planned at once (big children get tiles, small ones go to the bottom phase).
+// A top-phase node at depth d holds its segments in the order buffer of d's parity (even: order, odd: tmp);
+// the nodes that stop at an odd depth without going to the bottom phase (leaves, and big nodes that do not
+// split) are listed for the final copy into `order`.
+__global__ __launch_bounds__(PLAN_THREADS) void k_level_next(TreeArrays t, int lvl, const int32_t* __restrict__ big_in,
+                                                             int32_t* __restrict__ big_out) {
   const int nbig = live_count(t.counters, C_BIG);
   const int nodes0 = t.counters[C_NODES];
-  int made = 0;
+  const int bottom0 = t.counters[C_BOTTOM];
+  const int strand0 = t.counters[C_STRAND];
+  const bool odd = (lvl & 1) != 0;
+  int made = 0, nnext = 0, ntiles = 0, nsmall = 0, nstrand = 0;
   for (int base = 0; base < nbig; base += blockDim.x) {
     const int i = base + threadIdx.x;
-    const int node = i < nbig ? t.big[i] : -1;
+    const int node = i < nbig ? big_in[i] : -1;
     const bool sp = node >= 0 && t.split[node];
-    int total;
-    const int r = block_exclusive_scan(sp ? 1 : 0, &total);
+    int a = 0, b = 0, middle = 0;
     if (sp) {
-      const int first = nodes0 + 2 * (made + r);
+      a = t.lo[node];
+      b = t.hi[node];
+      middle = (int)(((int64_t)a + b) / 2);
+    }
+    const int size[2] = {middle - a, b - middle};
+    bool big[2], small[2], leaf[2];
+    int tiles[2];
+    for (int q = 0; q < 2; ++q) {
+      big[q] = sp && is_big(size[q], t.leaf);
+      small[q] = sp && is_small(size[q], t.leaf);
+      leaf[q] = sp && !big[q] && !small[q];
+      tiles[q] = big[q] ? (size[q] + TILE - 1) / TILE : 0;
+    }
+    const bool strand_self = node >= 0 && !sp && odd;           // stays where level lvl left it
+    const int strands = (int)strand_self + (odd ? 0 : (int)leaf[0] + (int)leaf[1]);  // children: depth lvl + 1
+    int tm, tb, tt, ts, tst;
+    const int rm = block_exclusive_scan(sp ? 1 : 0, &tm);
+    const int rb = block_exclusive_scan((int)big[0] + (int)big[1], &tb);
+    const int rt = block_exclusive_scan(tiles[0] + tiles[1], &tt);
+    const int rs = block_exclusive_scan((int)small[0] + (int)small[1], &ts);
+    const int rst = block_exclusive_scan(strands, &tst);
+    int st = strand0 + nstrand + rst;
+    if (strand_self) {
+      if (st < t.cap) t.strand[st] = node;
+      else t.counters[C_OVERFLOW] = 1;
+      ++st;
+    }
+    if (sp) {
+      const int first = nodes0 + 2 * (made + rm);
       if (first + 1 < t.cap) {
-        const int a = t.lo[node], b = t.hi[node], middle = (int)(((int64_t)a + b) / 2);
         const int d = t.depth[node] + 1, tr = t.tree[node];
-        t.lo[first] = a;
-        t.hi[first] = middle;
-        t.lo[first + 1] = middle;
-        t.hi[first + 1] = b;
+        int bi = nnext + rb, t0 = ntiles + rt, sb = bottom0 + nsmall + rs;
         for (int q = 0; q < 2; ++q) {
-          t.left[first + q] = -1;
-          t.right[first + q] = -1;
-          t.parent[first + q] = node;
-          t.depth[first + q] = d;
-          t.tree[first + q] = tr;
+          const int child = first + q;
+          const int clo = q ? middle : a, chi = q ? b : middle;
+          t.lo[child] = clo;
+          t.hi[child] = chi;
+          t.left[child] = -1;
+          t.right[child] = -1;
+          t.parent[child] = node;
+          t.depth[child] = d;
+          t.tree[child] = tr;
+          if (big[q]) {
+            if (bi < t.cap && t0 + tiles[q] <= t.tile_cap) {
+              big_out[bi] = child;
+              for (int k = 0; k < tiles[q]; ++k) {
+                t.tile_node[t0 + k] = child;
+                t.tile_p0[t0 + k] = clo + k * TILE;
+                t.tile_p1[t0 + k] = min(chi, clo + (k + 1) * TILE);
+              }
+            } else {
+              t.counters[C_OVERFLOW] = 1;
+            }
+            ++bi;
+            t0 += tiles[q];
+          }
+          if (small[q]) {
+            if (sb < t.bottom_cap) t.bottom[sb] = child;
+            else t.counters[C_OVERFLOW] = 1;
+            ++sb;
+          }
+          if (leaf[q] && !odd) {  // a leaf at odd depth lvl + 1
+            if (st < t.cap) t.strand[st] = child;
+            else t.counters[C_OVERFLOW] = 1;
+            ++st;
+          }
         }
         t.left[node] = first;
         t.right[node] = first + 1;
-        next[2 * (made + r)] = first;
-        next[2 * (made + r) + 1] = first + 1;
       } else {
         t.counters[C_OVERFLOW] = 1;
       }
     }
-    made += total;
+    made += tm;
+    nnext += tb;
+    ntiles += tt;
+    nsmall += ts;
+    nstrand += tst;
   }
+  __syncthreads();  // every thread has read the counters of this level
   if (threadIdx.x == 0) {
     t.counters[C_NODES] = nodes0 + 2 * made;
-    t.counters[C_LEVEL] = 2 * made;
+    t.counters[C_BIG] = nnext;
+    t.counters[C_TILES] = ntiles;
+    t.counters[C_BOTTOM] = bottom0 + nsmall;
+    t.counters[C_STRAND] = strand0 + nstrand;
   }
 }
 
+// The listed nodes' segments, left in tmp (odd depth), move to `order`.
+__global__ __launch_bounds__(256) void k_copy_strands(TreeArrays t) {
+  const int count = live_count(t.counters, C_STRAND);
+  for (int k = blockIdx.x; k < count; k += gridDim.x) {
+    const int node = t.strand[k];
+    for (int p = t.lo[node] + (int)threadIdx.x; p < t.hi[node]; p += blockDim.x) t.order[p] = t.tmp[p];
+  }
+}
 // ------------------------------------------------------------------ bottom phase
 __global__ __launch_bounds__(PLAN_THREADS) void k_bottom_bases(TreeArrays t) {
   const int count = live_count(t.counters, C_BOTTOM);
@@ -646,8 +737,9 @@ __global__ __launch_bounds__(BOTTOM_THREADS) void k_bottom(TreeArrays t, const d
     const int a = t.lo[root];
     const int s = t.hi[root] - a;
     const int slots = heap_slots(s, t.leaf);
+    const int32_t* src = (t.depth[root] & 1) ? t.tmp : t.order;  // the buffer of the root's depth parity
     for (int i = threadIdx.x; i < s; i += blockDim.x) {
-      const int g = t.order[a + i];
+      const int g = src[a + i];
       seg[i] = g;
       cx[i] = mx[g];
       cy[i] = my[g];
@@ -895,7 +987,7 @@ int tree_layout(Ctx& c, int buf, int64_t m64, int K, int leaf, TreeArrays& t) {
   size_t need = 0;
   need += al((K + 1) * 4) + al(M * 4);                                    // roots_lo, order
   need += 7 * al(C * 4) + 2 * al(C);                                      // node table, axis, split
-  need += al(C_COUNT * 4) + 5 * al(C * 4);                                // counters, levels, big, big_t0, flags
+  need += al(C_COUNT * 4) + 5 * al(C * 4);                                // counters, roots, big lists, strands, flags
   need += 3 * al((size_t)t.tile_cap * 4) + al((size_t)t.tile_cap * SUBS * 48);   // tiles, box partials
   need += 2 * al(M * 8) + 2 * al(M * 4) + al(std::max(M, C) * 4);         // sort buffers, tmp
   need += 2 * al((size_t)t.bottom_cap * 4);                               // bottom, heap base
@@ -921,9 +1013,9 @@ int tree_layout(Ctx& c, int buf, int64_t m64, int K, int leaf, TreeArrays& t) {
   t.split = (int8_t*)carve(C);
   t.counters = (int32_t*)carve(C_COUNT * 4);
   t.level_a = (int32_t*)carve(C * 4);
-  t.level_b = (int32_t*)carve(C * 4);
   t.big = (int32_t*)carve(C * 4);
-  t.big_t0 = (int32_t*)carve(C * 4);
+  t.big_alt = (int32_t*)carve(C * 4);
+  t.strand = (int32_t*)carve(C * 4);
   t.p_flag = (int32_t*)carve(C * 4);
   t.tile_node = (int32_t*)carve((size_t)t.tile_cap * 4);
   t.tile_p0 = (int32_t*)carve((size_t)t.tile_cap * 4);
@@ -962,32 +1054,35 @@ int tree_build(Ctx& c, TreeArrays& t, const double* mx, const double* my, const 
   if (m > 0) hipLaunchKernelGGL(k_iota, dim3((m + 255) / 256), dim3(256), 0, c.stream, m, t.order);
   hipLaunchKernelGGL(k_init_roots, dim3(1), dim3(256), 0, c.stream, t);
   OFW_HIP(hipGetLastError());
-  // top levels: while the largest node of a level can hold more than S_BOTTOM segments
-  int32_t* level = t.level_a;
-  int32_t* next = t.level_b;
+  // top levels: while the largest node of a level can hold more than S_BOTTOM segments.  Level lvl reads the
+  // order buffer of its parity and leaves its sorted nodes in the other one.
+  hipLaunchKernelGGL(k_level_plan, dim3(1), dim3(PLAN_THREADS), 0, c.stream, t, t.level_a);
+  int32_t* const bigs[2] = {t.big, t.big_alt};
+  int32_t* const orders[2] = {t.order, t.tmp};
   for (int lvl = 0;; ++lvl) {
     const int64_t biggest = (max_root + ((int64_t)1 << lvl) - 1) >> lvl;  // ceil(max_root / 2^lvl)
-    hipLaunchKernelGGL(k_level_plan, dim3(1), dim3(PLAN_THREADS), 0, c.stream, t, level);
-    if (biggest <= S_BOTTOM || biggest <= t.leaf) break;  // the plan above sent every node to the bottom
-    hipLaunchKernelGGL(k_box_tiles, dim3(grid), dim3(256), 0, c.stream, t, mx, my, mz);
-    const int64_t nodes_bound = std::min<int64_t>((int64_t)t.K << std::min(lvl, 30), t.cap);
-    hipLaunchKernelGGL(k_box_combine, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(grid, (nodes_bound + 3) / 4))),
-                       dim3(256), 0, c.stream, t);
-    hipLaunchKernelGGL(k_sort_tiles, dim3(grid), dim3(TILE_THREADS), 0, c.stream, t, mx, my, mz);
+    if (!is_big((int)std::min<int64_t>(biggest, INT_MAX), t.leaf)) break;  // planned: all to the bottom
+    const int32_t* oin = orders[lvl & 1];
+    int32_t* oout = orders[(lvl + 1) & 1];
+    int passes = 0;
+    for (int64_t width = TILE; width < biggest; width <<= 1) ++passes;
+    hipLaunchKernelGGL(k_box_tiles, dim3(grid), dim3(256), 0, c.stream, t, oin, mx, my, mz);
+    hipLaunchKernelGGL(k_sort_tiles, dim3(grid), dim3(TILE_THREADS), 0, c.stream, t, oin,
+                       passes == 0 ? oout : nullptr, mx, my, mz);
     uint64_t *kin = t.keyA, *kout = t.keyB;
     uint32_t *pin = t.posA, *pout = t.posB;
-    for (int64_t width = TILE; width < biggest; width <<= 1) {
-      hipLaunchKernelGGL(k_merge_pass, dim3(grid * 2), dim3(MERGE_THREADS), 0, c.stream, t, (int)width, kin, pin,
-                         kout, pout);
+    for (int p = 0; p < passes; ++p) {
+      const bool last = p == passes - 1;
+      hipLaunchKernelGGL(k_merge_pass, dim3(grid * 2), dim3(MERGE_THREADS), 0, c.stream, t, (int)(TILE << p), kin,
+                         pin, kout, pout, last ? oin : nullptr, last ? oout : nullptr);
       std::swap(kin, kout);
       std::swap(pin, pout);
     }
-    hipLaunchKernelGGL(k_apply_perm, dim3(grid), dim3(256), 0, c.stream, t, pin);
-    hipLaunchKernelGGL(k_copy_tiles, dim3(grid), dim3(256), 0, c.stream, t);
-    hipLaunchKernelGGL(k_level_children, dim3(1), dim3(PLAN_THREADS), 0, c.stream, t, next);
+    hipLaunchKernelGGL(k_level_next, dim3(1), dim3(PLAN_THREADS), 0, c.stream, t, lvl, bigs[lvl & 1],
+                       bigs[(lvl + 1) & 1]);
     OFW_HIP(hipGetLastError());
-    std::swap(level, next);
   }
+  hipLaunchKernelGGL(k_copy_strands, dim3(grid), dim3(256), 0, c.stream, t);
   // bottom phase
   hipLaunchKernelGGL(k_bottom_bases, dim3(1), dim3(PLAN_THREADS), 0, c.stream, t);
   hipLaunchKernelGGL(k_bottom, dim3(grid), dim3(BOTTOM_THREADS), 0, c.stream, t, mx, my, mz);

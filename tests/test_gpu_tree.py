@@ -34,6 +34,14 @@ def tree_cases():
     rng = np.random.default_rng(3)
     ties = np.round(rng.random((9000, 3)) * 4) / 4   # many equal coordinates across big (merged) nodes
     cases.append(("ties-9000", ties, 4))
+    # big nodes that cannot split (5,000 equal midpoints) and leaves larger than an LDS subtree, each ending at
+    # an odd and at an even depth: the segments of such nodes stay in the order buffer of their depth
+    same = np.zeros((5000, 3))
+    spread = np.column_stack((1 + rng.random(15000), 0.5 * rng.random(15000), 0.5 * rng.random(15000)))
+    cases.append(("unsplittable-depth-1", np.concatenate((spread[:5000], same)), 16))
+    cases.append(("unsplittable-depth-2", np.concatenate((spread, same)), 16))
+    cases.append(("leaf-3000-depth-1", rng.random((5000, 3)), 3000))
+    cases.append(("leaf-3000-depth-2", rng.random((9000, 3)), 3000))
     return cases
 
 
