@@ -416,8 +416,8 @@ __global__ __launch_bounds__(TILE_THREADS) void k_sort_tiles(TreeArrays t, const
 // wavefront each) give the slice's input ranges of both runs, which are staged in LDS; each thread finds its
 // own start in LDS, merges MERGE_E outputs into registers, and the slice leaves through LDS with coalesced
 // stores.
-static constexpr int MERGE_THREADS = 512;
-static constexpr int MERGE_E = 8;                                // outputs per thread
+static constexpr int MERGE_THREADS = 1024;
+static constexpr int MERGE_E = 4;                                // outputs per thread
 static constexpr int MERGE_OUT = MERGE_THREADS * MERGE_E;        // outputs per work item
 static constexpr int MERGE_SLICES = TILE / MERGE_OUT;            // work items per sort tile
 // every merge pass must write its whole output (the host swaps the buffers after each pass), and a slice must
