@@ -136,10 +136,11 @@ __global__ __launch_bounds__(PLAN_THREADS) void k_level_plan(TreeArrays t, const
       const int t0 = ntiles + rt;
       if (bi < t.cap && t0 + tiles <= t.tile_cap) {
         t.big[bi] = node;
+        const int lo = t.lo[node], hi = t.hi[node];  // read once: the stores below may alias them
         for (int q = 0; q < tiles; ++q) {
           t.tile_node[t0 + q] = node;
-          t.tile_p0[t0 + q] = t.lo[node] + q * TILE;
-          t.tile_p1[t0 + q] = min(t.hi[node], t.lo[node] + (q + 1) * TILE);
+          t.tile_p0[t0 + q] = lo + q * TILE;
+          t.tile_p1[t0 + q] = min(hi, lo + (q + 1) * TILE);
         }
       } else {
         t.counters[C_OVERFLOW] = 1;
@@ -933,22 +934,30 @@ void scan_exclusive(Ctx& c, int n, const int32_t* in, int32_t* out, int32_t* par
 
 __global__ void k_node_pre(TreeArrays t) {
   const int count = live_count(t.counters, C_NODES);
+  int deepest = 0;
   for (int id = blockIdx.x * blockDim.x + threadIdx.x; id < count; id += gridDim.x * blockDim.x) {
     const int lo = t.lo[id];
     if (lo < 0) {
       t.pre[id] = -1;
       continue;
     }
-    const int p = t.lo_base[lo] + t.depth[id] - t.lo_mind[lo];
+    // read before the stores, which the compiler cannot tell apart from these arrays
+    const int hi = t.hi[id], d = t.depth[id], tr = t.tree[id];
+    const int p = t.lo_base[lo] + d - t.lo_mind[lo];
+    const int skip = t.lo_base[hi];
     t.pre[id] = p;
     t.p_lo[p] = lo;
-    t.p_hi[p] = t.hi[id];
-    t.p_skip[p] = t.lo_base[t.hi[id]];
-    t.p_depth[p] = t.depth[id];
-    t.p_tree[p] = t.tree[id];
+    t.p_hi[p] = hi;
+    t.p_skip[p] = skip;
+    t.p_depth[p] = d;
+    t.p_tree[p] = tr;
     t.p_node[p] = id;
-    atomicMax(&t.counters[C_DEPTH], t.depth[id]);
+    deepest = max(deepest, d);
   }
+  // the largest depth: one atomic per wavefront
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) deepest = max(deepest, __shfl_xor(deepest, off, 64));
+  if ((threadIdx.x & 63) == 0 && deepest > 0) atomicMax(&t.counters[C_DEPTH], deepest);
 }
 
 __global__ void k_node_pre_links(TreeArrays t) {
@@ -1066,7 +1075,7 @@ int tree_build(Ctx& c, TreeArrays& t, const double* mx, const double* my, const 
     int32_t* oout = orders[(lvl + 1) & 1];
     int passes = 0;
     for (int64_t width = TILE; width < biggest; width <<= 1) ++passes;
-    hipLaunchKernelGGL(k_box_tiles, dim3(grid), dim3(256), 0, c.stream, t, oin, mx, my, mz);
+    hipLaunchKernelGGL(k_box_tiles, dim3(grid * 4), dim3(256), 0, c.stream, t, oin, mx, my, mz);
     hipLaunchKernelGGL(k_sort_tiles, dim3(grid), dim3(TILE_THREADS), 0, c.stream, t, oin,
                        passes == 0 ? oout : nullptr, mx, my, mz);
     uint64_t *kin = t.keyA, *kout = t.keyB;

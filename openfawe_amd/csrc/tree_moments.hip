@@ -317,7 +317,16 @@ __global__ __launch_bounds__(256) void k_moment_lists(TreeArrays t, MomentForest
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < count; p += gridDim.x * blockDim.x) {
     const int k = t.p_tree[p], d = t.p_depth[p], cut = mf.cut[k];
     const bool leaf = is_leaf(t, p);
-    if (d == cut || (d < cut && leaf)) mf.roots[atomicAdd(t.counters + C_SUBTREES, 1)] = p;
+    // subtree roots appended with one atomic per wavefront
+    const bool root = d == cut || (d < cut && leaf);
+    const unsigned long long ballot = __builtin_amdgcn_ballot_w64(root);
+    if (ballot) {
+      const int lane = threadIdx.x & 63, leader = __builtin_ctzll(ballot);
+      int base = 0;
+      if (lane == leader) base = atomicAdd(t.counters + C_SUBTREES, __popcll(ballot));
+      base = __shfl(base, leader, 64);
+      if (root) mf.roots[base + __popcll(ballot & ((1ull << lane) - 1ull))] = p;
+    }
     if (d < min(cut, MB_MAX_CUT) && !leaf) {
       const int slot = atomicAdd(mf.top_count + (size_t)k * MB_MAX_CUT + d, 1);
       mf.top_list[((size_t)k << MB_MAX_CUT) + (1 << d) - 1 + slot] = p;
