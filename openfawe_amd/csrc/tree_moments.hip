@@ -481,17 +481,22 @@ __device__ __forceinline__ void max_bits(unsigned long long* a, double v) {
 
 // Exact squared bounds of the internal nodes above the cut depth in one pass over the segments (four per
 // lane): each segment's end points against the centre of each such node on its root path; the maximum over
-// a wavefront's segments that share the node, then an integer atomic maximum on the bits.
+// a wavefront's segments that share the node, over the workgroup's wavefronts when they all share it, then
+// an integer atomic maximum on the bits (exact in any order: non-negative doubles order as their bits).
 __global__ __launch_bounds__(256) void k_bound_top(TreeArrays t, MomentForest mf, MomentInputs mi,
                                                    const double* __restrict__ centre, double* __restrict__ r2out) {
 #pragma clang fp contract(off)
-  constexpr int J = 4;
+  constexpr int J = 4, WAVES = 4;
+  __shared__ int s_node[WAVES];  // per wavefront: its one node, -1 nothing live, -2 several nodes
+  __shared__ unsigned long long s_bits[WAVES];
   if (t.counters[C_OVERFLOW]) return;
   const int m = (int)t.m;
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   unsigned long long* bits = reinterpret_cast<unsigned long long*>(r2out);
-  const int stride = gridDim.x * 4 * 64 * J;
-  for (int base = (blockIdx.x * 4 + (int)(threadIdx.x >> 6)) * 64 * J; base < m; base += stride) {
+  const int stride = gridDim.x * WAVES * 64 * J;
+  // the wavefronts of a workgroup run the same iterations and levels (they combine their maxima)
+  for (int bb = blockIdx.x * WAVES * 64 * J; bb < m; bb += stride) {
+    const int base = bb + wv * 64 * J;
     int p[J], kk[J], D[J];
     bool live[J];
     double s[J][3], e[J][3];
@@ -499,7 +504,7 @@ __global__ __launch_bounds__(256) void k_bound_top(TreeArrays t, MomentForest mf
     for (int j = 0; j < J; ++j) {
       kk[j] = base + 64 * j + lane;
       const bool valid = kk[j] < m;
-      const int tr = last_le(t.roots_lo, t.K, valid ? kk[j] : base);
+      const int tr = last_le(t.roots_lo, t.K, valid ? kk[j] : min(base, m - 1));
       D[j] = valid ? mf.cut[tr] : 0;
       p[j] = valid ? t.tree_first[tr] : -1;
       live[j] = valid && D[j] > 0;
@@ -516,9 +521,10 @@ __global__ __launch_bounds__(256) void k_bound_top(TreeArrays t, MomentForest mf
         if (live[j] && (lev >= D[j] || is_leaf(t, p[j]))) live[j] = false;
         any = any || live[j];
       }
-      if (!wave_any(any)) break;
+      if (!__syncthreads_or(any ? 1 : 0)) break;
       double v[J];
       int id[J];
+      bool mine = false;
 #pragma unroll
       for (int j = 0; j < J; ++j) {
         v[j] = 0.0;
@@ -532,23 +538,48 @@ __global__ __launch_bounds__(256) void k_bound_top(TreeArrays t, MomentForest mf
           const double qe = e0 * e0 + e1 * e1 + e2 * e2;
           v[j] = qs > qe ? qs : qe;
           id[j] = p[j];
+          mine = true;
         }
       }
       const int first = __shfl(id[0], 0, 64);
       bool same = first >= 0;
 #pragma unroll
       for (int j = 0; j < J; ++j) same = same && (id[j] == first || id[j] < 0);
-      if (wave_all(same)) {
-        double vm = v[0];
+      const bool wsame = wave_all(same);
+      const bool wlive = wave_any(mine);
+      double vm = v[0];
 #pragma unroll
-        for (int j = 1; j < J; ++j) vm = v[j] > vm ? v[j] : vm;
-        vm = wave_max(vm);
+      for (int j = 1; j < J; ++j) vm = v[j] > vm ? v[j] : vm;
+      vm = wave_max(vm);
+      if (lane == 0) {
+        s_node[wv] = wsame ? first : (wlive ? -2 : -1);
+        s_bits[wv] = dbits(vm);
+      }
+      __syncthreads();
+      int node = -1;
+      bool one = true;
+      unsigned long long mb = 0ull;
+      for (int w = 0; w < WAVES; ++w) {
+        const int st = s_node[w];
+        if (st == -1) continue;
+        if (st == -2 || (node >= 0 && st != node)) {
+          one = false;
+        } else {
+          node = st;
+          mb = s_bits[w] > mb ? s_bits[w] : mb;
+        }
+      }
+      if (one && node >= 0) {
+        if (threadIdx.x == 0 && mb > *reinterpret_cast<volatile unsigned long long*>(bits + node))
+          atomicMax(bits + node, mb);
+      } else if (wsame) {
         if (lane == 0) max_bits(bits + first, vm);
       } else {
 #pragma unroll
         for (int j = 0; j < J; ++j)
           if (id[j] >= 0) max_bits(bits + id[j], v[j]);
       }
+      __syncthreads();  // s_node and s_bits are written again at the next level
 #pragma unroll
       for (int j = 0; j < J; ++j)
         if (live[j]) {
