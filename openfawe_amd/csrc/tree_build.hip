@@ -703,21 +703,103 @@ __global__ __launch_bounds__(PLAN_THREADS) void k_bottom_bases(TreeArrays t) {
   }
 }
 
-__device__ __forceinline__ bool key3_less(uint32_t ta, uint64_t ka, uint32_t tb, uint64_t kb) {
-  // tags are (start position of the element's node << 11) | position: nodes of every depth partition the
-  // range in position order, so ordering by node start, then key, then position sorts each splitting
-  // node's range and leaves every other node's range as it is
-  const uint32_t na = ta >> 11, nb = tb >> 11;
-  if (na != nb) return na < nb;
-  if (ka != kb) return ka < kb;
-  return (ta & 2047u) < (tb & 2047u);
-}
-
-struct Key3Less {
-  __device__ bool operator()(uint64_t ka, uint32_t ta, uint64_t kb, uint32_t tb) const {
-    return key3_less(ta, ka, tb, kb);
+// Sort of every splitting node of heap level k (slots [first, last)) by (key, position), in place in LDS: node
+// first + g on group g of blockDim >> k threads, E elements per thread (a sorting network in registers), then
+// merge rounds of runs E, 2E, ... within the node (a merge-path search per thread) while a run is shorter than
+// `maxlen`, the largest node of the level.  Other nodes' ranges are left as they are.  Every thread calls it.
+template <int E>
+__device__ void sort_heap_level(uint64_t* key, uint32_t* tag, const uint16_t* hlo, const uint16_t* hhi,
+                                const uint8_t* hstate, int k, int first, int last, int maxlen) {
+  const int T = max(1, (int)blockDim.x >> k);  // a splitting node holds at most E * T elements
+  const int g = threadIdx.x / T, r = threadIdx.x - g * T;
+  const int j = first + g;
+  int lo = 0, len = 0;
+  if (j < last && hstate[j] == 2) {
+    lo = hlo[j];
+    len = hhi[j] - lo;
   }
-};
+  uint64_t* kn = key + lo;
+  uint32_t* tn = tag + lo;
+  const int o = r * E;
+  const int cnt = max(0, min(E, len - o));
+  uint64_t kv[E];
+  uint32_t tv[E];
+  if (cnt > 0) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      kv[e] = e < cnt ? kn[o + e] : ~0ull;
+      tv[e] = e < cnt ? tn[o + e] : 0xffffffffu;
+    }
+#pragma unroll
+    for (int size = 2; size <= E; size <<= 1)
+#pragma unroll
+      for (int stride = size >> 1; stride > 0; stride >>= 1)
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+          const int q = i ^ stride;
+          if (q > i) {
+            const bool up = (i & size) == 0;
+            const bool swap = up ? pair_less(kv[q], tv[q], kv[i], tv[i]) : pair_less(kv[i], tv[i], kv[q], tv[q]);
+            if (swap) {
+              const uint64_t a = kv[i];
+              kv[i] = kv[q];
+              kv[q] = a;
+              const uint32_t b = tv[i];
+              tv[i] = tv[q];
+              tv[q] = b;
+            }
+          }
+        }
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+      if (e < cnt) {
+        kn[o + e] = kv[e];
+        tn[o + e] = tv[e];
+      }
+  }
+  __syncthreads();
+  for (int w = E; w < maxlen; w <<= 1) {
+    if (cnt > 0) {
+      const int pair0 = o & ~(2 * w - 1);
+      const int a1 = min(pair0 + w, len), b1 = min(pair0 + 2 * w, len);
+      const int la = a1 - pair0, lb = b1 - a1, diag = o - pair0;
+      const uint64_t* ka = kn + pair0;
+      const uint32_t* ta = tn + pair0;
+      const uint64_t* kb = kn + a1;
+      const uint32_t* tb = tn + a1;
+      int lo_s = max(0, diag - lb), hi_s = min(diag, la);
+      while (lo_s < hi_s) {
+        const int mid = (lo_s + hi_s) >> 1;
+        const int q = diag - mid - 1;
+        if (pair_less(kb[q], tb[q], ka[mid], ta[mid])) hi_s = mid;
+        else lo_s = mid + 1;
+      }
+      int ia = lo_s, ib = diag - lo_s;
+#pragma unroll
+      for (int e = 0; e < E; ++e)
+        if (e < cnt) {
+          const bool take_a = ib >= lb || (ia < la && !pair_less(kb[ib], tb[ib], ka[ia], ta[ia]));
+          if (take_a) {
+            kv[e] = ka[ia];
+            tv[e] = ta[ia];
+            ++ia;
+          } else {
+            kv[e] = kb[ib];
+            tv[e] = tb[ib];
+            ++ib;
+          }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+      if (e < cnt) {
+        kn[o + e] = kv[e];
+        tn[o + e] = tv[e];
+      }
+    __syncthreads();
+  }
+}
 
 // One workgroup per subtree of at most S_BOTTOM segments: every remaining level in LDS.
 __global__ __launch_bounds__(BOTTOM_THREADS) void k_bottom(TreeArrays t, const double* __restrict__ mx,
@@ -787,36 +869,31 @@ __global__ __launch_bounds__(BOTTOM_THREADS) void k_bottom(TreeArrays t, const d
       }
       __syncthreads();
       if (!any_split) break;
-      // sort every splitting node's range by (node, key, position); other nodes keep their order
-      int width = BOTTOM_E;
-      while (width < s) width <<= 1;
-      for (int i = threadIdx.x; i < width; i += blockDim.x) {
-        if (i < s) {
-          const int j = nodeof[i];
-          uint64_t kk = 0;
-          if (hstate[j] == 2) {
-            const int ax = haxis[j];
-            kk = ordered_key(ax == 0 ? cx[i] : (ax == 1 ? cy[i] : cz[i]));
-          }
-          key[i] = kk;
-          tag[i] = ((uint32_t)hlo[j] << 11) | (uint32_t)i;
-        } else {
-          key[i] = ~0ull;
-          tag[i] = 0xffffffffu;
+      // sort every splitting node's range by (key, position); other nodes keep their order
+      for (int i = threadIdx.x; i < s; i += blockDim.x) {
+        const int j = nodeof[i];
+        uint64_t kk = 0;
+        if (hstate[j] == 2) {
+          const int ax = haxis[j];
+          kk = ordered_key(ax == 0 ? cx[i] : (ax == 1 ? cy[i] : cz[i]));
         }
+        key[i] = kk;
+        tag[i] = (uint32_t)i;
       }
       __syncthreads();
-      block_merge_sort<BOTTOM_E>(key, tag, width, Key3Less());
+      int maxlen = s;  // the largest node of heap level k
+      for (int q = 0; q < k; ++q) maxlen = (maxlen + 1) / 2;
+      sort_heap_level<BOTTOM_E>(key, tag, hlo, hhi, hstate, k, first, last, maxlen);
       // permute segments and midpoints by the sorted positions (one array at a time through a scratch)
       double* dtmp = reinterpret_cast<double*>(key);
-      for (int i = threadIdx.x; i < s; i += blockDim.x) itmp[i] = seg[tag[i] & 2047u];
+      for (int i = threadIdx.x; i < s; i += blockDim.x) itmp[i] = seg[tag[i]];
       __syncthreads();
       for (int i = threadIdx.x; i < s; i += blockDim.x) seg[i] = itmp[i];
       __syncthreads();
       double* arrays[3] = {cx, cy, cz};
       for (int q = 0; q < 3; ++q) {
         double* arr = arrays[q];
-        for (int i = threadIdx.x; i < s; i += blockDim.x) dtmp[i] = arr[tag[i] & 2047u];
+        for (int i = threadIdx.x; i < s; i += blockDim.x) dtmp[i] = arr[tag[i]];
         __syncthreads();
         for (int i = threadIdx.x; i < s; i += blockDim.x) arr[i] = dtmp[i];
         __syncthreads();

@@ -217,6 +217,7 @@ __device__ __forceinline__ int find_tree(const int32_t* __restrict__ table, int 
 // pair sums the list kernel evaluates spread over the whole GPU.  Segments of the near field are staged
 // one per lane in a window of 64 consecutive tree positions and broadcast with readlane.
 static constexpr int WIN = 32;
+static_assert(WIN * ACC_REC % 128 == 0, "the node window is loaded in 16-byte chunks by 64 lanes");
 static constexpr int EVAL_WAVES = 4;
 
 template <int MODEL, int PREC>
@@ -352,9 +353,10 @@ __global__ __launch_bounds__(256) void k_tree_eval(ForestTables ft, const double
     // the window is one contiguous block of each record array: each lane copies 16 bytes per chunk,
     // stored with the element type the window is read with
     const double* src = accr + (size_t)base * ACC_REC;
-    double v[4];
+    constexpr int CHUNKS = WIN * ACC_REC / 128;  // 16-byte chunks per lane
+    double v[2 * CHUNKS];
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
+    for (int c = 0; c < CHUNKS; ++c) {
       const int e = 2 * lane + 128 * c;  // first double of this lane's chunk
       const bool in = base + e / ACC_REC < ft.node_cap;
       const double2 d = in ? *reinterpret_cast<const double2*>(src + e) : make_double2(0.0, 0.0);
@@ -362,7 +364,7 @@ __global__ __launch_bounds__(256) void k_tree_eval(ForestTables ft, const double
       v[2 * c + 1] = d.y;
     }
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
+    for (int c = 0; c < CHUNKS; ++c) {
       s_acc[wv][2 * lane + 128 * c] = v[2 * c];
       s_acc[wv][2 * lane + 128 * c + 1] = v[2 * c + 1];
     }
