@@ -685,8 +685,11 @@ __global__ void k_chunk_count(ForestTables ft, int wave_cap, const int32_t* __re
     nch[w] = w < nwaves ? (list_count[w] + CHUNK - 1) / CHUNK : 0;
 }
 
-// Per target: its tree's subtree partials added in pre-order of the subtrees (those of wavefronts that
-// added nothing are skipped: they are zero).
+// Per target: its tree's subtree partials and their near-field partials, in pre-order of the subtrees (those
+// of wavefronts that added nothing are skipped: they are zero).  A workgroup serves 64 targets with four
+// wavefronts: wavefront p adds the p-th quarter of every subtree's near-field partials in order (and p = 0 the
+// subtree partials), and the four sums are added in that order, a fixed association.  The flag kept is the
+// first in the sequential order.
 template <bool WIDE>
 __global__ __launch_bounds__(256) void k_tree_reduce(ForestTables ft, int n, const double* __restrict__ part,
                                                      const int8_t* __restrict__ pflag,
@@ -700,47 +703,85 @@ __global__ __launch_bounds__(256) void k_tree_reduce(ForestTables ft, int n, con
                                                      int8_t* __restrict__ flags, int64_t* __restrict__ far_counts,
                                                      int64_t* __restrict__ near_counts) {
   typedef typename std::conditional<WIDE, double, float>::type acc_t;
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  const int k = find_tree(ft.toff, ft.K, j);
-  const int t0 = ft.toff[k], nt = ft.toff[k + 1] - t0;
-  const int ntasks = ft.counters[C_OVERFLOW] ? 0 : ft.task_first[k + 1] - ft.task_first[k];
-  const int groups = (nt + 63) >> 6;
-  const int w0 = ft.wave_off[k] + ((j - t0) >> 6);
-  const size_t base = (size_t)ft.part_off[k] + (j - t0);
+  constexpr int PARTS = 4;
+  __shared__ acc_t s_u[PARTS][64][3];
+  __shared__ long long s_key[PARTS][64], s_far[PARTS][64], s_near[PARTS][64];
+  __shared__ int s_flag[PARTS][64];
+  const int lane = threadIdx.x & 63, pid = (int)(threadIdx.x >> 6);
+  const int j = blockIdx.x * 64 + lane;
   acc_t ux = 0, uy = 0, uz = 0;
   int flag = 0;
-  int64_t far = 0, near = 0;
-  for (int q = 0; q < ntasks; ++q) {
-    const int w = w0 + q * groups;
-    if (pnz[w]) {
-      const size_t o = base + (size_t)q * nt;
-      ux += (acc_t)part[3 * o];
-      uy += (acc_t)part[3 * o + 1];
-      uz += (acc_t)part[3 * o + 2];
-      if (flag == 0) flag = pflag[o];
-      if (pfar) {
-        far += pfar[o];
-        near += pnear[o];
+  long long fkey = LLONG_MAX;  // (task << 32) | (0 for the subtree partial, 2 i + 1 for near-field item i)
+  long long far = 0, near = 0;
+  if (j < n) {
+    const int k = find_tree(ft.toff, ft.K, j);
+    const int t0 = ft.toff[k], nt = ft.toff[k + 1] - t0;
+    const int ntasks = ft.counters[C_OVERFLOW] ? 0 : ft.task_first[k + 1] - ft.task_first[k];
+    const int groups = (nt + 63) >> 6;
+    const int w0 = ft.wave_off[k] + ((j - t0) >> 6);
+    const size_t base = (size_t)ft.part_off[k] + (j - t0);
+    for (int q = 0; q < ntasks; ++q) {
+      const int w = w0 + q * groups;
+      if (pid == 0 && pnz[w]) {
+        const size_t o = base + (size_t)q * nt;
+        ux += (acc_t)part[3 * o];
+        uy += (acc_t)part[3 * o + 1];
+        uz += (acc_t)part[3 * o + 2];
+        if (flag == 0 && pflag[o] != 0) {
+          flag = pflag[o];
+          fkey = (long long)q << 32;
+        }
+        if (pfar) {
+          far += pfar[o];
+          near += pnear[o];
+        }
       }
-    }
-    if (chunk_off) {
-      // the subtree's near field, in the order of its walk
-      for (int ch = chunk_off[w]; ch < chunk_off[w + 1]; ++ch) {
-        const size_t o = (size_t)ch * 64 + ((j - t0) & 63);
-        ux += (acc_t)cpart[3 * o];
-        uy += (acc_t)cpart[3 * o + 1];
-        uz += (acc_t)cpart[3 * o + 2];
-        if (flag == 0) flag = cflag[o];
-        if (cnear) near += cnear[o];
+      if (chunk_off) {
+        // this wavefront's quarter of the subtree's near field, in the order of its walk
+        const int c0 = chunk_off[w], len = chunk_off[w + 1] - c0;
+        const int cs = c0 + (int)(((long long)len * pid) / PARTS), ce = c0 + (int)(((long long)len * (pid + 1)) / PARTS);
+        for (int ch = cs; ch < ce; ++ch) {
+          const size_t o = (size_t)ch * 64 + ((j - t0) & 63);
+          ux += (acc_t)cpart[3 * o];
+          uy += (acc_t)cpart[3 * o + 1];
+          uz += (acc_t)cpart[3 * o + 2];
+          if (flag == 0 && cflag[o] != 0) {
+            flag = cflag[o];
+            fkey = ((long long)q << 32) | (long long)(2 * (ch - c0) + 1);
+          }
+          if (cnear) near += cnear[o];
+        }
       }
     }
   }
-  if (!(isfinite((double)ux) && isfinite((double)uy) && isfinite((double)uz)) && flag == 0) flag = OFW_FLAG_NONFINITE;
-  out[3 * (size_t)j] = (double)ux;
-  out[3 * (size_t)j + 1] = (double)uy;
-  out[3 * (size_t)j + 2] = (double)uz;
-  flags[j] = (int8_t)flag;
+  s_u[pid][lane][0] = ux;
+  s_u[pid][lane][1] = uy;
+  s_u[pid][lane][2] = uz;
+  s_key[pid][lane] = fkey;
+  s_flag[pid][lane] = flag;
+  s_far[pid][lane] = far;
+  s_near[pid][lane] = near;
+  __syncthreads();
+  if (pid != 0 || j >= n) return;
+  acc_t vx = s_u[0][lane][0], vy = s_u[0][lane][1], vz = s_u[0][lane][2];
+  int fl = s_flag[0][lane];
+  long long best = fl != 0 ? s_key[0][lane] : LLONG_MAX;
+  for (int q = 1; q < PARTS; ++q) {
+    vx += s_u[q][lane][0];
+    vy += s_u[q][lane][1];
+    vz += s_u[q][lane][2];
+    far += s_far[q][lane];
+    near += s_near[q][lane];
+    if (s_flag[q][lane] != 0 && s_key[q][lane] < best) {
+      best = s_key[q][lane];
+      fl = s_flag[q][lane];
+    }
+  }
+  if (!(isfinite((double)vx) && isfinite((double)vy) && isfinite((double)vz)) && fl == 0) fl = OFW_FLAG_NONFINITE;
+  out[3 * (size_t)j] = (double)vx;
+  out[3 * (size_t)j + 1] = (double)vy;
+  out[3 * (size_t)j + 2] = (double)vz;
+  flags[j] = (int8_t)fl;
   if (far_counts) far_counts[j] = far;
   if (near_counts) near_counts[j] = near;
 }
@@ -1057,11 +1098,11 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
   OFW_HIP(c.mark());
   if (n > 0) {
     if (precision == OFW_FP32)
-      hipLaunchKernelGGL(k_tree_reduce<false>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, ft, (int)n,
+      hipLaunchKernelGGL(k_tree_reduce<false>, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, c.stream, ft, (int)n,
                          part, pflag, pfar, pnear, pnz, (const int32_t*)chunk_off, cpart, cflag, cnear, out, flags,
                          far_counts, near_counts);
     else
-      hipLaunchKernelGGL(k_tree_reduce<true>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, ft, (int)n,
+      hipLaunchKernelGGL(k_tree_reduce<true>, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, c.stream, ft, (int)n,
                          part, pflag, pfar, pnear, pnz, (const int32_t*)chunk_off, cpart, cflag, cnear, out, flags,
                          far_counts, near_counts);
   }
