@@ -686,12 +686,12 @@ __global__ void k_chunk_count(ForestTables ft, int wave_cap, const int32_t* __re
 }
 
 // Per target: its tree's subtree partials and their near-field partials, in pre-order of the subtrees (those
-// of wavefronts that added nothing are skipped: they are zero).  A workgroup serves 64 targets with four
-// wavefronts: wavefront p adds the p-th quarter of every subtree's near-field partials in order (and p = 0 the
-// subtree partials), and the four sums are added in that order, a fixed association.  The flag kept is the
+// of wavefronts that added nothing are skipped: they are zero).  A workgroup serves 64 targets with eight
+// wavefronts: wavefront p adds the p-th eighth of every subtree's near-field partials in order (and p = 0 the
+// subtree partials), and the eight sums are added in that order, a fixed association.  The flag kept is the
 // first in the sequential order.
 template <bool WIDE>
-__global__ __launch_bounds__(256) void k_tree_reduce(ForestTables ft, int n, const double* __restrict__ part,
+__global__ __launch_bounds__(512) void k_tree_reduce(ForestTables ft, int n, const double* __restrict__ part,
                                                      const int8_t* __restrict__ pflag,
                                                      const int32_t* __restrict__ pfar,
                                                      const int32_t* __restrict__ pnear,
@@ -703,7 +703,7 @@ __global__ __launch_bounds__(256) void k_tree_reduce(ForestTables ft, int n, con
                                                      int8_t* __restrict__ flags, int64_t* __restrict__ far_counts,
                                                      int64_t* __restrict__ near_counts) {
   typedef typename std::conditional<WIDE, double, float>::type acc_t;
-  constexpr int PARTS = 4;
+  constexpr int PARTS = 8;
   __shared__ acc_t s_u[PARTS][64][3];
   __shared__ long long s_key[PARTS][64], s_far[PARTS][64], s_near[PARTS][64];
   __shared__ int s_flag[PARTS][64];
@@ -737,7 +737,7 @@ __global__ __launch_bounds__(256) void k_tree_reduce(ForestTables ft, int n, con
         }
       }
       if (chunk_off) {
-        // this wavefront's quarter of the subtree's near field, in the order of its walk
+        // this wavefront's part of the subtree's near field, in the order of its walk
         const int c0 = chunk_off[w], len = chunk_off[w + 1] - c0;
         const int cs = c0 + (int)(((long long)len * pid) / PARTS), ce = c0 + (int)(((long long)len * (pid + 1)) / PARTS);
         for (int ch = cs; ch < ce; ++ch) {
@@ -1098,11 +1098,11 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
   OFW_HIP(c.mark());
   if (n > 0) {
     if (precision == OFW_FP32)
-      hipLaunchKernelGGL(k_tree_reduce<false>, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, c.stream, ft, (int)n,
+      hipLaunchKernelGGL(k_tree_reduce<false>, dim3((unsigned)((n + 63) / 64)), dim3(512), 0, c.stream, ft, (int)n,
                          part, pflag, pfar, pnear, pnz, (const int32_t*)chunk_off, cpart, cflag, cnear, out, flags,
                          far_counts, near_counts);
     else
-      hipLaunchKernelGGL(k_tree_reduce<true>, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, c.stream, ft, (int)n,
+      hipLaunchKernelGGL(k_tree_reduce<true>, dim3((unsigned)((n + 63) / 64)), dim3(512), 0, c.stream, ft, (int)n,
                          part, pflag, pfar, pnear, pnz, (const int32_t*)chunk_off, cpart, cflag, cnear, out, flags,
                          far_counts, near_counts);
   }
