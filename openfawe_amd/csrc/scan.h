@@ -34,6 +34,46 @@ __device__ inline int block_exclusive_scan(int v, int* total) {
   return result;
 }
 
+// Exclusive scans of N ints per thread over the workgroup together, for the barriers of one; every thread of
+// the workgroup must call it.  total[q] receives the workgroup's sum of value q.
+template <int N>
+__device__ inline void block_exclusive_scan_n(const int (&v)[N], int (&r)[N], int (&total)[N]) {
+  __shared__ int wave_sums[N][17];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+  int x[N];
+#pragma unroll
+  for (int q = 0; q < N; ++q) x[q] = v[q];
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1)
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+      const int y = __shfl_up(x[q], off, 64);
+      if (lane >= off) x[q] += y;
+    }
+  if (lane == 63)
+#pragma unroll
+    for (int q = 0; q < N; ++q) wave_sums[q][w] = x[q];
+  __syncthreads();
+  if ((int)threadIdx.x < N) {  // thread q scans the wavefront sums of value q
+    const int q = threadIdx.x;
+    int acc = 0;
+    for (int k = 0; k < nw; ++k) {
+      const int s = wave_sums[q][k];
+      wave_sums[q][k] = acc;
+      acc += s;
+    }
+    wave_sums[q][16] = acc;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < N; ++q) {
+    r[q] = x[q] - v[q] + wave_sums[q][w];
+    total[q] = wave_sums[q][16];
+  }
+  __syncthreads();  // the next call may overwrite wave_sums
+}
+
 // true when the predicate holds on any / every active lane of the wavefront
 __device__ __forceinline__ bool wave_any(bool v) { return __builtin_amdgcn_ballot_w64(v) != 0; }
 __device__ __forceinline__ bool wave_all(bool v) {

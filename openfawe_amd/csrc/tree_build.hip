@@ -127,10 +127,11 @@ __global__ __launch_bounds__(PLAN_THREADS) void k_level_plan(TreeArrays t, const
     const bool big = node >= 0 && is_big(size, t.leaf);
     const bool small = node >= 0 && is_small(size, t.leaf);
     const int tiles = big ? (size + TILE - 1) / TILE : 0;
-    int tb, tt, ts;
-    const int rb = block_exclusive_scan(big ? 1 : 0, &tb);
-    const int rt = block_exclusive_scan(tiles, &tt);
-    const int rs = block_exclusive_scan(small ? 1 : 0, &ts);
+    const int sv[3] = {big ? 1 : 0, tiles, small ? 1 : 0};
+    int sr[3], stot[3];
+    block_exclusive_scan_n<3>(sv, sr, stot);
+    const int rb = sr[0], rt = sr[1], rs = sr[2];
+    const int tb = stot[0], tt = stot[1], ts = stot[2];
     if (big) {
       const int bi = nbig + rb;
       const int t0 = ntiles + rt;
@@ -597,12 +598,12 @@ __global__ __launch_bounds__(PLAN_THREADS) void k_level_next(TreeArrays t, int l
     }
     const bool strand_self = node >= 0 && !sp && odd;           // stays where level lvl left it
     const int strands = (int)strand_self + (odd ? 0 : (int)leaf[0] + (int)leaf[1]);  // children: depth lvl + 1
-    int tm, tb, tt, ts, tst;
-    const int rm = block_exclusive_scan(sp ? 1 : 0, &tm);
-    const int rb = block_exclusive_scan((int)big[0] + (int)big[1], &tb);
-    const int rt = block_exclusive_scan(tiles[0] + tiles[1], &tt);
-    const int rs = block_exclusive_scan((int)small[0] + (int)small[1], &ts);
-    const int rst = block_exclusive_scan(strands, &tst);
+    const int sv[5] = {sp ? 1 : 0, (int)big[0] + (int)big[1], tiles[0] + tiles[1], (int)small[0] + (int)small[1],
+                       strands};
+    int sr[5], stot[5];
+    block_exclusive_scan_n<5>(sv, sr, stot);
+    const int rm = sr[0], rb = sr[1], rt = sr[2], rs = sr[3], rst = sr[4];
+    const int tm = stot[0], tb = stot[1], tt = stot[2], ts = stot[3], tst = stot[4];
     int st = strand0 + nstrand + rst;
     if (strand_self) {
       if (st < t.cap) t.strand[st] = node;

@@ -870,9 +870,10 @@ __global__ __launch_bounds__(1024) void k_work_plan(TreeArrays t, const int32_t*
     } else if (k == t.K) {
       task_first[k] = index[t.tree_first[k]];
     }
-    int tw, tp;
-    const int rw = block_exclusive_scan(w, &tw);
-    const int rp = block_exclusive_scan(pp, &tp);
+    const int sv[2] = {w, pp};
+    int sr[2], stot[2];
+    block_exclusive_scan_n<2>(sv, sr, stot);
+    const int rw = sr[0], rp = sr[1], tw = stot[0], tp = stot[1];
     if (k <= t.K) {
       wave_off[k] = waves + rw;
       part_off[k] = parts + rp;
