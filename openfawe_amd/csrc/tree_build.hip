@@ -9,12 +9,13 @@
 // sort by key equals an ordinary sort by the unique pair (key, position), with the key the order-preserving
 // 64-bit image of the double and -0.0 folded into +0.0 (np.argsort compares with <).
 //
-//  * Top phase, nodes of more than S_BOTTOM segments, level by level: box reduction over tiles, sort of
-//    4,096-element tiles (registers, then in-place LDS merge rounds), merge-path passes between tiles.  A
-//    one-workgroup plan kernel lists each level's nodes, tiles and deferred subtrees; a one-workgroup kernel
-//    creates the children.
+//  * Top phase, nodes of more than S_BOTTOM segments, level by level: partial boxes over tiles, the node's
+//    box and sort of each 4,096-element tile (registers, then in-place LDS merge rounds), merge-path passes
+//    between tiles, the last of which applies the sorted order into the other of two order buffers; a
+//    one-workgroup kernel creates the children and plans the next level (tiles and deferred subtrees).
 //  * Bottom phase: each node of at most S_BOTTOM segments is finished by one workgroup entirely in LDS,
-//    level after level (box, composite (node, key, position) merge sort, children), into a heap block.
+//    level after level (box, each splitting node's (key, position) merge sort on its own thread group,
+//    children), into a heap block.
 //  * Finish: in a median-split tree the pre-order (the visit order of the reference's stack walk) is the
 //    order of the nodes by (lo, depth), so pre(N) = #nodes with lo < N.lo + depth(N) - min depth at N.lo and
 //    the escape index is #nodes with lo < N.hi: a histogram over lo and one scan give the whole layout.
