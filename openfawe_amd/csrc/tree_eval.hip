@@ -999,12 +999,13 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
   int32_t* flag = (int32_t*)carve(((size_t)t.cap + 1) * 4);
   int32_t* index = (int32_t*)carve(((size_t)t.cap + 1) * 4);
   EvalTask* tasks = (EvalTask*)carve((size_t)std::max<int64_t>(task_cap, 1) * sizeof(EvalTask));
-  int32_t* toff = (int32_t*)carve((size_t)(K + 1) * 4);
+  // the host tables (target offsets, list bases, list capacities) side by side: one upload
+  int32_t* toff = (int32_t*)carve((size_t)(K + 1) * 3 * 4);
+  int32_t* d_list_base = toff + (K + 1);
+  int32_t* d_list_cap = toff + 2 * (K + 1);
   int32_t* task_first = (int32_t*)carve((size_t)(K + 1) * 4);
   int32_t* wave_off = (int32_t*)carve((size_t)(K + 1) * 4);
   int32_t* part_off = (int32_t*)carve((size_t)(K + 1) * 4);
-  int32_t* d_list_base = (int32_t*)carve((size_t)(K + 1) * 4);
-  int32_t* d_list_cap = (int32_t*)carve((size_t)(K + 1) * 4);
   double* part = (double*)carve(P * 24);
   int8_t* pflag = (int8_t*)carve(P);
   int32_t* pfar = counts ? (int32_t*)carve(P * 4) : nullptr;
@@ -1023,14 +1024,13 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
     chunk_part = (int32_t*)carve(((W1 + 4095) / 4096 + 8) * 4);
   }
 
-  int st = upload_i32(c, shape.target_off, toff);
-  if (st) return st;
+  std::vector<int64_t> tables(shape.target_off);
   if (lists) {
-    st = upload_i32(c, list_base, d_list_base);
-    if (st) return st;
-    st = upload_i32(c, list_cap, d_list_cap);
-    if (st) return st;
+    tables.insert(tables.end(), list_base.begin(), list_base.end());
+    tables.insert(tables.end(), list_cap.begin(), list_cap.end());
   }
+  int st = upload_i32(c, tables, toff);
+  if (st) return st;
   const int grid = c.cu_count * 2;
   if (n_ex > 0 && ex_ptr) {
     hipLaunchKernelGGL(k_inverse, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, c.stream, (int)m, t.order,
