@@ -281,8 +281,12 @@ __device__ void block_merge_sort(uint64_t* s_key, uint32_t* s_tag, int width, Le
       s_tag[tid * E + e] = p[e];
     }
   }
-  __syncthreads();
   for (int run = E; run < width; run <<= 1) {
+    // a merge of two runs within one wavefront's 64 E elements reads and writes that wavefront's elements
+    // only: its LDS order needs no workgroup barrier
+    const bool local = 2 * run <= 64 * E;
+    if (local) wave_lds_sync();
+    else __syncthreads();  // the previous stores are visible
     if (active) {
       const int out0 = tid * E;
       const int pair0 = out0 & ~(2 * run - 1);
@@ -314,7 +318,8 @@ __device__ void block_merge_sort(uint64_t* s_key, uint32_t* s_tag, int width, Le
         }
       }
     }
-    __syncthreads();
+    if (local) wave_lds_sync();
+    else __syncthreads();  // every read of this round is done
     if (active) {
 #pragma unroll
       for (int e = 0; e < E; ++e) {
@@ -322,8 +327,8 @@ __device__ void block_merge_sort(uint64_t* s_key, uint32_t* s_tag, int width, Le
         s_tag[tid * E + e] = p[e];
       }
     }
-    __syncthreads();
   }
+  __syncthreads();
 }
 
 struct PairLess {
@@ -759,8 +764,12 @@ __device__ void sort_heap_level(uint64_t* key, uint32_t* tag, const uint16_t* hl
         tn[o + e] = tv[e];
       }
   }
-  __syncthreads();
   for (int w = E; w < maxlen; w <<= 1) {
+    // the threads of a merge (a node's group, or the 2w / E threads of two runs) inside one wavefront read
+    // and write that wavefront's elements only: their LDS order needs no workgroup barrier
+    const bool local = T <= 64 || 2 * w <= 64 * E;
+    if (local) wave_lds_sync();
+    else __syncthreads();
     if (cnt > 0) {
       const int pair0 = o & ~(2 * w - 1);
       const int a1 = min(pair0 + w, len), b1 = min(pair0 + 2 * w, len);
@@ -792,15 +801,16 @@ __device__ void sort_heap_level(uint64_t* key, uint32_t* tag, const uint16_t* hl
           }
         }
     }
-    __syncthreads();
+    if (local) wave_lds_sync();
+    else __syncthreads();
 #pragma unroll
     for (int e = 0; e < E; ++e)
       if (e < cnt) {
         kn[o + e] = kv[e];
         tn[o + e] = tv[e];
       }
-    __syncthreads();
   }
+  __syncthreads();
 }
 
 // One workgroup per subtree of at most S_BOTTOM segments: every remaining level in LDS.
