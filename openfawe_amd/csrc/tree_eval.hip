@@ -665,24 +665,33 @@ __global__ __launch_bounds__(256, 4) void k_tree_near(ForestTables ft, const dou
 }
 
 // the walk wavefront of every work item of the list kernel (within the partial slots)
-__global__ void k_item_waves(const int32_t* __restrict__ counters, const int32_t* __restrict__ chunk_off, int slots,
-                             int32_t* __restrict__ item_wave) {
+// The walk wavefront of every work item (within the `slots` partial slots); more work items than the fixed
+// number `limit` of slots is reported as an overflow.
+__global__ void k_item_waves(int32_t* __restrict__ counters, const int32_t* __restrict__ chunk_off, int slots,
+                             int limit, int32_t* __restrict__ item_wave) {
+  if (blockIdx.x == 0 && threadIdx.x == 0 && counters[C_CHUNKS] > limit) counters[C_OVERFLOW] = 1;
   const int nwaves = live_count(counters, C_WORK);
   for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < nwaves; w += gridDim.x * blockDim.x)
     for (int it = chunk_off[w]; it < min(chunk_off[w + 1], slots); ++it) item_wave[it] = w;
 }
 
-// a fixed number of near-field partial slots that the work items exceed is reported as an overflow
-__global__ void k_check_slots(int32_t* __restrict__ counters, int slots) {
-  if (threadIdx.x == 0 && counters[C_CHUNKS] > slots) counters[C_OVERFLOW] = 1;
-}
-
-// near-field work items per walk wavefront: ceil(list entries / CHUNK), zero past the planned waves
-__global__ void k_chunk_count(ForestTables ft, int wave_cap, const int32_t* __restrict__ list_count,
-                              int32_t* __restrict__ nch) {
+// One workgroup: the near-field work items of each walk wavefront, ceil(list entries / CHUNK) (zero past the
+// planned waves), scanned into chunk_off (W1 entries, each thread a run of consecutive ones); the total goes
+// to counters[C_CHUNKS].
+__global__ __launch_bounds__(1024) void k_chunk_plan(ForestTables ft, int W1, const int32_t* __restrict__ list_count,
+                                                     int32_t* __restrict__ chunk_off, int32_t* __restrict__ counters) {
   const int nwaves = live_count(ft.counters, C_WORK);
-  for (int w = blockIdx.x * blockDim.x + threadIdx.x; w <= wave_cap; w += gridDim.x * blockDim.x)
-    nch[w] = w < nwaves ? (list_count[w] + CHUNK - 1) / CHUNK : 0;
+  const int per = (W1 + (int)blockDim.x - 1) / (int)blockDim.x;
+  const int a = min(W1, (int)threadIdx.x * per), b = min(W1, a + per);
+  int sum = 0;
+  for (int w = a; w < b; ++w) sum += w < nwaves ? (list_count[w] + CHUNK - 1) / CHUNK : 0;
+  int total;
+  int run = block_exclusive_scan(sum, &total);
+  for (int w = a; w < b; ++w) {
+    chunk_off[w] = run;
+    run += w < nwaves ? (list_count[w] + CHUNK - 1) / CHUNK : 0;
+  }
+  if (threadIdx.x == 0) counters[C_CHUNKS] = total;
 }
 
 // Per target: its tree's subtree partials and their near-field partials, in pre-order of the subtrees (those
@@ -986,7 +995,7 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
   need += 2 * al(((size_t)t.cap + 1) * 4) + al((size_t)std::max<int64_t>(task_cap, 1) * sizeof(EvalTask));
   need += 6 * al((size_t)(K + 1) * 4) + al(P * 24) + al(P) + (counts ? 2 * al(P * 4) : 0);
   need += al(W1);
-  if (lists) need += al(E * sizeof(NearEntry)) + 3 * al(W1 * 4) + al(((W1 + 4095) / 4096 + 8) * 4);
+  if (lists) need += al(E * sizeof(NearEntry)) + 2 * al(W1 * 4);
   OFW_HIP(c.tree[buf].reserve(need));
   char* base = (char*)c.tree[buf].ptr;
   auto carve = [&](size_t bytes) {
@@ -1012,16 +1021,14 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
   int32_t* pnear = counts ? (int32_t*)carve(P * 4) : nullptr;
   int8_t* pnz = (int8_t*)carve(W1);
   NearEntry* list = nullptr;
-  int32_t *list_count = nullptr, *nch = nullptr, *chunk_off = nullptr, *chunk_part = nullptr;
+  int32_t *list_count = nullptr, *chunk_off = nullptr;
   double* cpart = nullptr;
   int8_t* cflag = nullptr;
   int32_t* cnear = nullptr;
   if (lists) {
     list = (NearEntry*)carve(E * sizeof(NearEntry));
     list_count = (int32_t*)carve(W1 * 4);
-    nch = (int32_t*)carve(W1 * 4);
     chunk_off = (int32_t*)carve(W1 * 4);
-    chunk_part = (int32_t*)carve(((W1 + 4095) / 4096 + 8) * 4);
   }
 
   std::vector<int64_t> tables(shape.target_off);
@@ -1059,9 +1066,7 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
   OFW_HIP(c.mark());
   if (lists && wave_cap > 0) {
     // near-field work items: CHUNK list entries each, numbered in walk-wavefront order
-    hipLaunchKernelGGL(k_chunk_count, dim3((unsigned)((W1 + 255) / 256)), dim3(256), 0, c.stream, ft, (int)wave_cap,
-                       list_count, nch);
-    scan_exclusive(c, (int)W1, nch, chunk_off, chunk_part, t.counters + C_CHUNKS);
+    hipLaunchKernelGGL(k_chunk_plan, dim3(1), dim3(1024), 0, c.stream, ft, (int)W1, list_count, chunk_off, t.counters);
     // partial sums of the near-field work items (scratch buffer 5 of the context)
     int64_t slots = shape.chunk_slots;
     if (slots < 0) {
@@ -1084,9 +1089,8 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
       cb += al(CH * 4);
       cnear = counts ? (int32_t*)cb : nullptr;
     }
-    hipLaunchKernelGGL(k_check_slots, dim3(1), dim3(64), 0, c.stream, t.counters, (int)std::min<int64_t>(slots, INT_MAX));
     hipLaunchKernelGGL(k_item_waves, dim3((unsigned)((W1 + 255) / 256)), dim3(256), 0, c.stream, t.counters, chunk_off,
-                       (int)CH, item_wave);
+                       (int)CH, (int)std::min<int64_t>(slots, INT_MAX), item_wave);
     OFW_HIP(c.mark());
     const dim3 ngrid((unsigned)std::max<int64_t>(1, std::min<int64_t>((int64_t)c.cu_count * 16, (CH + 3) / 4)));
     switch (model) {
