@@ -38,7 +38,7 @@ static constexpr int BOTTOM_THREADS = 1024;
 static constexpr int BOTTOM_E = S_BOTTOM / BOTTOM_THREADS;  // elements per thread of the subtree sort
 static constexpr int HCAP = 2 * S_BOTTOM - 1;
 static constexpr int PLAN_THREADS = 1024;
-static constexpr int SUBS = 8;              // workgroups per tile of the elementwise top-phase kernels
+static constexpr int SUBS = 4;              // workgroups per tile of the elementwise top-phase kernels
 
 __device__ __forceinline__ uint64_t ordered_key(double x) {
   uint64_t u = (x == 0.0) ? 0ull : (uint64_t)__double_as_longlong(x);
