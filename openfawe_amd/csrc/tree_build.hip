@@ -199,7 +199,11 @@ __device__ __forceinline__ void sub_range(int p0, int p1, int sub, int& a, int& 
   b = min(p1, a + chunk);
 }
 
-__global__ __launch_bounds__(256) void k_box_tiles(TreeArrays t, const int32_t* __restrict__ order_in,
+static constexpr int BOX_THREADS = 256;
+static constexpr int BOX_PER = TILE / SUBS / BOX_THREADS;  // elements per thread of a work unit
+static_assert(TILE % (SUBS * BOX_THREADS) == 0, "a box work unit holds a whole number of elements per thread");
+
+__global__ __launch_bounds__(BOX_THREADS) void k_box_tiles(TreeArrays t, const int32_t* __restrict__ order_in,
                                                    const double* __restrict__ mx, const double* __restrict__ my,
                                                    const double* __restrict__ mz) {
   __shared__ double red[4][6];
@@ -209,15 +213,25 @@ __global__ __launch_bounds__(256) void k_box_tiles(TreeArrays t, const int32_t* 
     int p0, p1;
     sub_range(t.tile_p0[tile], t.tile_p1[tile], wu % SUBS, p0, p1);
     double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-    for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-      const int s = order_in[p];
-      const double c[3] = {mx[s], my[s], mz[s]};
+    // a work unit holds at most BOX_PER elements per thread: every order entry is read first, then the
+    // midpoints, so that no load waits for another
+    int sidx[BOX_PER];
 #pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        v[q] = c[q] < v[q] ? c[q] : v[q];
-        v[3 + q] = c[q] > v[3 + q] ? c[q] : v[3 + q];
-      }
+    for (int e = 0; e < BOX_PER; ++e) {
+      const int p = p0 + (int)threadIdx.x + e * BOX_THREADS;
+      sidx[e] = p < p1 ? order_in[p] : -1;
     }
+#pragma unroll
+    for (int e = 0; e < BOX_PER; ++e)
+      if (sidx[e] >= 0) {
+        const int s = sidx[e];
+        const double c[3] = {mx[s], my[s], mz[s]};
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          v[q] = c[q] < v[q] ? c[q] : v[q];
+          v[3 + q] = c[q] > v[3 + q] ? c[q] : v[3 + q];
+        }
+      }
     box_wave_reduce(v);
     const int wv = threadIdx.x / 64;
     if ((threadIdx.x & 63) == 0)
@@ -396,11 +410,20 @@ __global__ __launch_bounds__(TILE_THREADS) void k_sort_tiles(TreeArrays t, const
     int width = TILE_E;
     while (width < len) width <<= 1;
     const double* coord = axis == 0 ? mx : (axis == 1 ? my : mz);
-    for (int i = threadIdx.x; i < width; i += blockDim.x) {
+    // TILE_E elements per thread: the order entries first, then the coordinates
+    int gidx[TILE_E];
+#pragma unroll
+    for (int e = 0; e < TILE_E; ++e) {
+      const int i = (int)threadIdx.x + e * TILE_THREADS;
+      gidx[e] = i < len ? order_in[p0 + i] : 0;
+    }
+#pragma unroll
+    for (int e = 0; e < TILE_E; ++e) {
+      const int i = (int)threadIdx.x + e * TILE_THREADS;
       if (i < len) {
-        key[i] = ordered_key(coord[order_in[p0 + i]]);
+        key[i] = ordered_key(coord[gidx[e]]);
         pos[i] = (uint32_t)(p0 + i);
-      } else {
+      } else if (i < width) {
         key[i] = ~0ull;
         pos[i] = 0xffffffffu;
       }
@@ -408,7 +431,17 @@ __global__ __launch_bounds__(TILE_THREADS) void k_sort_tiles(TreeArrays t, const
     __syncthreads();
     block_merge_sort<TILE_E>(key, pos, width, PairLess());
     if (order_out) {
-      for (int i = threadIdx.x; i < len; i += blockDim.x) order_out[p0 + i] = order_in[pos[i]];
+      int g[TILE_E];
+#pragma unroll
+      for (int e = 0; e < TILE_E; ++e) {
+        const int i = (int)threadIdx.x + e * TILE_THREADS;
+        g[e] = i < len ? order_in[pos[i]] : 0;
+      }
+#pragma unroll
+      for (int e = 0; e < TILE_E; ++e) {
+        const int i = (int)threadIdx.x + e * TILE_THREADS;
+        if (i < len) order_out[p0 + i] = g[e];
+      }
     } else {
       for (int i = threadIdx.x; i < len; i += blockDim.x) {
         t.keyA[p0 + i] = key[i];
@@ -519,10 +552,27 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_merge_pass(TreeArrays t, int 
     const int i0 = s_split[0], i1 = s_split[1];
     const int j0 = d0 - i0, j1 = d1 - i1;
     const int na = i1 - i0, nb = j1 - j0;  // na + nb = q1 - q0 <= MERGE_OUT
-    for (int e = threadIdx.x; e < na + nb; e += blockDim.x) {
-      const int src = e < na ? a0 + i0 + e : b0 + j0 + (e - na);
-      s_key[e] = in_key[src];
-      s_pos[e] = in_pos[src];
+    {
+      // MERGE_E inputs per thread, loaded together
+      uint64_t kk[MERGE_E];
+      uint32_t pp[MERGE_E];
+#pragma unroll
+      for (int r = 0; r < MERGE_E; ++r) {
+        const int e = (int)threadIdx.x + r * MERGE_THREADS;
+        if (e < na + nb) {
+          const int src = e < na ? a0 + i0 + e : b0 + j0 + (e - na);
+          kk[r] = in_key[src];
+          pp[r] = in_pos[src];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < MERGE_E; ++r) {
+        const int e = (int)threadIdx.x + r * MERGE_THREADS;
+        if (e < na + nb) {
+          s_key[e] = kk[r];
+          s_pos[e] = pp[r];
+        }
+      }
     }
     __syncthreads();
     // each thread merges MERGE_E consecutive outputs into registers
@@ -559,7 +609,17 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_merge_pass(TreeArrays t, int 
       }
     __syncthreads();
     if (order_out) {
-      for (int e = threadIdx.x; e < na + nb; e += blockDim.x) order_out[q0 + e] = order_in[s_pos[e]];
+      int g[MERGE_E];
+#pragma unroll
+      for (int r = 0; r < MERGE_E; ++r) {
+        const int e = (int)threadIdx.x + r * MERGE_THREADS;
+        g[r] = e < na + nb ? order_in[s_pos[e]] : 0;
+      }
+#pragma unroll
+      for (int r = 0; r < MERGE_E; ++r) {
+        const int e = (int)threadIdx.x + r * MERGE_THREADS;
+        if (e < na + nb) order_out[q0 + e] = g[r];
+      }
     } else {
       for (int e = threadIdx.x; e < na + nb; e += blockDim.x) {
         out_key[q0 + e] = s_key[e];
@@ -1164,7 +1224,7 @@ int tree_build(Ctx& c, TreeArrays& t, const double* mx, const double* my, const 
     int32_t* oout = orders[(lvl + 1) & 1];
     int passes = 0;
     for (int64_t width = TILE; width < biggest; width <<= 1) ++passes;
-    hipLaunchKernelGGL(k_box_tiles, dim3(grid * 4), dim3(256), 0, c.stream, t, oin, mx, my, mz);
+    hipLaunchKernelGGL(k_box_tiles, dim3(grid * 4), dim3(BOX_THREADS), 0, c.stream, t, oin, mx, my, mz);
     hipLaunchKernelGGL(k_sort_tiles, dim3(grid), dim3(TILE_THREADS), 0, c.stream, t, oin,
                        passes == 0 ? oout : nullptr, mx, my, mz);
     uint64_t *kin = t.keyA, *kout = t.keyB;
