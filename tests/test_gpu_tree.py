@@ -298,3 +298,37 @@ def test_batch_with_exclusions_and_default_depth():
             assert np.array_equal(out[a:b].view(np.uint64), ref.view(np.uint64)), k
     finally:
         batch.close()
+
+
+def coincident_snapshot(n_same, n_spread, seed):
+    """Segments of which n_same share one midpoint (an unsplittable node of more than an LDS subtree) beside
+    n_spread scattered further along x, so that the equal ones fill a whole child of the top splits."""
+    rng = np.random.default_rng(seed)
+    m = n_same + n_spread
+    starts = np.zeros((m, 3))
+    starts[n_same:] = np.column_stack((1 + rng.random(n_spread), 0.5 * rng.random(n_spread),
+                                       0.5 * rng.random(n_spread)))
+    directions = np.zeros((m, 3))
+    directions[:, 1] = 1.0
+    return dict(points=rng.random((200, 3)) * [2.0, 1.0, 1.0] + [0.0, 0.0, 0.3], starts=starts,
+                directions=directions, lengths=np.full(m, 0.05), gamma=rng.uniform(0.5, 1.5, m),
+                radius=np.full(m, 0.01))
+
+
+@pytest.mark.parametrize("use_graph", [False, True], ids=["eager", "graph"])
+def test_batch_with_unsplittable_nodes(use_graph):
+    """A forest whose trees hold unsplittable nodes of 5,000 segments at depth 1 and at depth 2 (their segments
+    stay in the order buffer of that depth until the final copy): every snapshot equals its single call."""
+    snaps = [coincident_snapshot(5000, 5000, 1), batch_snapshots([(300, 6000)])[0], coincident_snapshot(5000, 15000, 2)]
+    settings = V.TreeSettings(theta=0.5, order=2, leaf_size=32, core_clearance=3.0)
+    batch = T.TreeBatch(snaps, settings, model=2, precision="mixed", split_depth=3, use_graph=use_graph)
+    try:
+        for replay in range(2):
+            out = batch.run().cpu().numpy()
+            for k, snap in enumerate(snaps):
+                a, b = batch.target_offsets[k], batch.target_offsets[k + 1]
+                ref, _, _ = single_call(snap, settings, "mixed", 3)
+                assert np.array_equal(out[a:b].view(np.uint64), ref.view(np.uint64)), (replay, k)
+                assert np.isfinite(out[a:b]).all()
+    finally:
+        batch.close()
