@@ -104,7 +104,8 @@ int ofw_direct_sum(ofw_ctx *ctx, int precision, int block, int64_t n, const doub
 
 /* Counters of the context's last call (host buffer of `count` int64): [0] direct-sum segment chunks,
  * [1] segments per chunk, [2] traversal subtrees, [3] traversal split depth, [4] tree nodes, [5] tree depth,
- * [6] near-field work items of the traversal.
+ * [6] near-field work items of the traversal, [7] runs of the tree pipeline the last ofw_tree_sum took (2 when the
+ * work-item slots kept from the call before were too few).
  * Returns the number written. */
 int ofw_ctx_stats(ofw_ctx *ctx, int64_t *stats_host, int count);
 

@@ -346,6 +346,7 @@ int ofw_tree_sum(ofw_ctx* x, int precision, int64_t n, const double* points, int
     const int64_t used = counters[ofw::C_CHUNKS];
     const bool short_slots = counters[ofw::C_OVERFLOW] && tc.chunk_slots >= 0 && used > tc.chunk_slots;
     c.list_slots = used + used / 4 + 1024;
+    c.stats_i[7] = attempt + 1;  // runs of the pipeline this call took
     if (!short_slots || attempt > 0) break;
   }
   c.stats_i[6] = counters[ofw::C_CHUNKS];

@@ -104,7 +104,8 @@ def hip_tree_sum(points, starts, directions, lengths, gamma, radius, model, csr,
     Statistics.last = dict(points=n, segments=m, precision=settings.precision, flags=host_flags, flag_seg=None,
                            far_counts=far, near_counts=near.cpu().numpy(), mean_far_nodes=float(far.mean()),
                            nodes=int(stats[4]), depth=int(stats[5]), subtrees=int(stats[2]),
-                           split_depth=int(stats[3]), phases_ms=_lib.timings(_ctx()))
+                           split_depth=int(stats[3]), work_items=int(stats[6]), attempts=int(stats[7]),
+                           phases_ms=_lib.timings(_ctx()))
     return out.cpu().numpy(), bool(np.any(host_flags))
 
 

@@ -332,3 +332,15 @@ def test_batch_with_unsplittable_nodes(use_graph):
                 assert np.isfinite(out[a:b]).all()
     finally:
         batch.close()
+
+
+def test_list_slots_grow_between_calls():
+    """A call whose near-field work items exceed the slots kept from the previous, smaller call reports the
+    overflow on the device, runs again with its own count, and returns what its first call returned."""
+    small, big = workloads.sheet(8), workloads.sheet(60)
+    first, _, st_first = run_tree(big, 0.3)
+    run_tree(small, 0.3)
+    again, _, st_again = run_tree(big, 0.3)
+    assert st_again["attempts"] == 2 and st_again["work_items"] == st_first["work_items"]
+    assert np.array_equal(first.view(np.uint64), again.view(np.uint64))
+    assert np.array_equal(st_first["near_counts"], st_again["near_counts"])
