@@ -1224,14 +1224,19 @@ int tree_build(Ctx& c, TreeArrays& t, const double* mx, const double* my, const 
     int32_t* oout = orders[(lvl + 1) & 1];
     int passes = 0;
     for (int64_t width = TILE; width < biggest; width <<= 1) ++passes;
-    hipLaunchKernelGGL(k_box_tiles, dim3(grid * 4), dim3(BOX_THREADS), 0, c.stream, t, oin, mx, my, mz);
-    hipLaunchKernelGGL(k_sort_tiles, dim3(grid), dim3(TILE_THREADS), 0, c.stream, t, oin,
+    // no more tiles than m / TILE plus one partial tile per node of the level: grids beyond that only idle
+    const int64_t tile_bound = (int64_t)m / TILE + ((int64_t)t.K << std::min(lvl, 30)) + 1;
+    const unsigned g1 = (unsigned)std::min<int64_t>(grid, tile_bound);
+    const unsigned g2 = (unsigned)std::min<int64_t>(2 * (int64_t)grid, tile_bound);
+    const unsigned g4 = (unsigned)std::min<int64_t>(4 * (int64_t)grid, tile_bound * SUBS);
+    hipLaunchKernelGGL(k_box_tiles, dim3(g4), dim3(BOX_THREADS), 0, c.stream, t, oin, mx, my, mz);
+    hipLaunchKernelGGL(k_sort_tiles, dim3(g1), dim3(TILE_THREADS), 0, c.stream, t, oin,
                        passes == 0 ? oout : nullptr, mx, my, mz);
     uint64_t *kin = t.keyA, *kout = t.keyB;
     uint32_t *pin = t.posA, *pout = t.posB;
     for (int p = 0; p < passes; ++p) {
       const bool last = p == passes - 1;
-      hipLaunchKernelGGL(k_merge_pass, dim3(grid * 2), dim3(MERGE_THREADS), 0, c.stream, t, (int)(TILE << p), kin,
+      hipLaunchKernelGGL(k_merge_pass, dim3(g2), dim3(MERGE_THREADS), 0, c.stream, t, (int)(TILE << p), kin,
                          pin, kout, pout, last ? oin : nullptr, last ? oout : nullptr);
       std::swap(kin, kout);
       std::swap(pin, pout);
