@@ -127,38 +127,41 @@ int ofw_device_info(char* name_host, int name_len, int* cu_count_host) {
   return OFW_OK;
 }
 
-int ofw_direct_sum(ofw_ctx* x, int precision, int block, int64_t n, const double* points, int64_t m,
-                   const double* starts, const double* directions, const double* lengths, const double* gamma,
-                   const double* radius, int model, int raise_singular, const int64_t* ex_ptr,
-                   const int64_t* ex_seg, double* out, int8_t* flags, int32_t* flag_seg) {
+static int direct_args(ofw_ctx* x, const char* what, int precision, int block, int64_t n, int64_t m, int model) {
   if (!x) {
     set_error("null context");
     return OFW_ERR_ARG;
   }
   if (precision < 0 || precision > 2 || model < 0 || model > 5 || n < 0 || m < 0 ||
       !(block == 32 || block == 64 || block == 128 || block == 256)) {
-    set_error("ofw_direct_sum: invalid precision, model, block or size");
+    set_error(std::string(what) + ": invalid precision, model, block or size");
     return OFW_ERR_ARG;
   }
   if (n > (int64_t)1 << 30 || m > (int64_t)1 << 28) {
-    set_error("ofw_direct_sum: too many targets or segments for one call");
+    set_error(std::string(what) + ": too many targets or segments for one call");
     return OFW_ERR_ARG;
   }
+  return OFW_OK;
+}
+
+int ofw_direct_sum(ofw_ctx* x, int precision, int block, int64_t n, const double* points, int64_t m,
+                   const double* starts, const double* directions, const double* lengths, const double* gamma,
+                   const double* radius, int model, int raise_singular, const int64_t* ex_ptr,
+                   const int64_t* ex_seg, double* out, int8_t* flags, int32_t* flag_seg) {
+  int st = direct_args(x, "ofw_direct_sum", precision, block, n, m, model);
+  if (st) return st;
   ofw::Ctx& c = x->c;
   c.nev = 0;
   if (n == 0) return OFW_OK;
   OFW_HIP(c.mark());
   OFW_HIP(c.rec64.reserve((size_t)(m > 0 ? m : 1) * REC64 * sizeof(double)));
   OFW_HIP(c.rec32.reserve((size_t)(m > 0 ? m : 1) * REC32 * sizeof(float)));
-  int st = ofw::pack_segments(c, m, starts, directions, lengths, gamma, radius, model, (double*)c.rec64.ptr,
-                              (float*)c.rec32.ptr);
-  if (st) return st;
-  int32_t *p32 = nullptr, *s32 = nullptr;
-  st = ofw::csr_to_i32(c, n, ex_ptr, ex_seg, &p32, &s32);
+  st = ofw::pack_segments(c, m, starts, directions, lengths, gamma, radius, model, (double*)c.rec64.ptr,
+                          (float*)c.rec32.ptr);
   if (st) return st;
   OFW_HIP(c.mark());
   st = ofw::direct_sum(c, precision, block, n, points, m, (const double*)c.rec64.ptr, (const float*)c.rec32.ptr,
-                       model, raise_singular, p32, s32, out, flags, flag_seg);
+                       model, raise_singular, ex_ptr, ex_seg, out, flags, flag_seg);
   if (st) return st;
   OFW_HIP(hipStreamSynchronize(c.stream));
   return OFW_OK;

@@ -87,8 +87,8 @@ __device__ __forceinline__ int flag_code(const FlagState& st) { return st.flag ?
 template <int MODEL>
 __global__ __launch_bounds__(256) void k_direct64(int n, const double* __restrict__ pts, int m,
                                                   const double* __restrict__ rec, int chunk,
-                                                  const int32_t* __restrict__ ex_ptr,
-                                                  const int32_t* __restrict__ ex_seg, int raise_singular,
+                                                  const int64_t* __restrict__ ex_ptr,
+                                                  const int64_t* __restrict__ ex_seg, int raise_singular,
                                                   double* __restrict__ part, int32_t* __restrict__ pflag) {
   const int pa = blockIdx.x * (2 * blockDim.x) + threadIdx.x;
   const int pb = pa + blockDim.x;
@@ -97,7 +97,8 @@ __global__ __launch_bounds__(256) void k_direct64(int n, const double* __restric
   const int s1 = min(m, s0 + chunk);
   const bool act_a = pa < n, act_b = pb < n;
   double ax = 0.0, ay = 0.0, az = 0.0, bx = 0.0, by = 0.0, bz = 0.0;
-  int ka = 0, stop_a = 0, next_a = INT_MAX, kb = 0, stop_b = 0, next_b = INT_MAX;
+  int64_t ka = 0, stop_a = 0, kb = 0, stop_b = 0;
+  int next_a = INT_MAX, next_b = INT_MAX;
   if (act_a) {
     ax = pts[3 * (size_t)pa];
     ay = pts[3 * (size_t)pa + 1];
@@ -105,7 +106,7 @@ __global__ __launch_bounds__(256) void k_direct64(int n, const double* __restric
     ka = ex_ptr[pa];
     stop_a = ex_ptr[pa + 1];
     while (ka < stop_a && ex_seg[ka] < s0) ka++;
-    next_a = ka < stop_a ? ex_seg[ka] : INT_MAX;
+    next_a = ka < stop_a ? (int)ex_seg[ka] : INT_MAX;
   }
   if (act_b) {
     bx = pts[3 * (size_t)pb];
@@ -114,7 +115,7 @@ __global__ __launch_bounds__(256) void k_direct64(int n, const double* __restric
     kb = ex_ptr[pb];
     stop_b = ex_ptr[pb + 1];
     while (kb < stop_b && ex_seg[kb] < s0) kb++;
-    next_b = kb < stop_b ? ex_seg[kb] : INT_MAX;
+    next_b = kb < stop_b ? (int)ex_seg[kb] : INT_MAX;
   }
   double uax = 0.0, uay = 0.0, uaz = 0.0, ubx = 0.0, uby = 0.0, ubz = 0.0;
   FlagState sta{OFW_FLAG_OK, INT_MAX}, stb{OFW_FLAG_OK, INT_MAX};
@@ -124,11 +125,11 @@ __global__ __launch_bounds__(256) void k_direct64(int n, const double* __restric
     if (__builtin_amdgcn_ballot_w64(hit_a || hit_b) != 0) {
       if (hit_a) {
         ++ka;
-        next_a = ka < stop_a ? ex_seg[ka] : INT_MAX;
+        next_a = ka < stop_a ? (int)ex_seg[ka] : INT_MAX;
       }
       if (hit_b) {
         ++kb;
-        next_b = kb < stop_b ? ex_seg[kb] : INT_MAX;
+        next_b = kb < stop_b ? (int)ex_seg[kb] : INT_MAX;
       }
     }
     const int fa = pair_fast64<MODEL>(ax, ay, az, r, raise_singular, act_a && !hit_a, uax, uay, uaz);
@@ -159,8 +160,8 @@ template <int MODEL, bool WIDE>
 __global__ __launch_bounds__(256) void k_direct_mixed(int n, const double* __restrict__ pts, int m,
                                                       const double* __restrict__ rec,
                                                       const float* __restrict__ rec32, int chunk,
-                                                      const int32_t* __restrict__ ex_ptr,
-                                                      const int32_t* __restrict__ ex_seg, int raise_singular,
+                                                      const int64_t* __restrict__ ex_ptr,
+                                                      const int64_t* __restrict__ ex_seg, int raise_singular,
                                                       double* __restrict__ part, int32_t* __restrict__ pflag) {
   typedef typename std::conditional<WIDE, double, float>::type acc_t;
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -169,7 +170,8 @@ __global__ __launch_bounds__(256) void k_direct_mixed(int n, const double* __res
   const int s1 = min(m, s0 + chunk);
   const bool active = p < n;
   double px = 0.0, py = 0.0, pz = 0.0;
-  int k = 0, stop = 0, next = INT_MAX;
+  int64_t k = 0, stop = 0;
+  int next = INT_MAX;
   if (active) {
     px = pts[3 * (size_t)p];
     py = pts[3 * (size_t)p + 1];
@@ -177,7 +179,7 @@ __global__ __launch_bounds__(256) void k_direct_mixed(int n, const double* __res
     k = ex_ptr[p];
     stop = ex_ptr[p + 1];
     while (k < stop && ex_seg[k] < s0) k++;
-    next = k < stop ? ex_seg[k] : INT_MAX;
+    next = k < stop ? (int)ex_seg[k] : INT_MAX;
   }
   acc_t ux = 0, uy = 0, uz = 0;
   FlagState st{OFW_FLAG_OK, INT_MAX};
@@ -186,7 +188,7 @@ __global__ __launch_bounds__(256) void k_direct_mixed(int n, const double* __res
     const float* f = rec32 + (size_t)s * REC32;
     if (s == next) {
       ++k;
-      next = k < stop ? ex_seg[k] : INT_MAX;
+      next = k < stop ? (int)ex_seg[k] : INT_MAX;
       continue;
     }
     const float rx = (float)(px - r[R_SX]), ry = (float)(py - r[R_SY]), rz = (float)(pz - r[R_SZ]);
@@ -213,26 +215,43 @@ __global__ __launch_bounds__(256) void k_direct_mixed(int n, const double* __res
   }
 }
 
-// Sum of the chunk partials in chunk order; first flag in segment order; the reference's final
-// non-finite check (vortex_numba.py:142-143).
-template <bool WIDE>
+// Sum of the chunk partials of every target; first flag in segment order; the reference's final non-finite
+// check (vortex_numba.py:142-143).  LANES consecutive lanes share a target: lane j adds the chunks j, j + LANES,
+// j + 2 LANES, ... in order, then the lanes' sums are combined by a butterfly over the group (lane j adds lane
+// j ^ o for o = LANES/2 ... 1).  Both partners of a butterfly step form the same sum, so every lane of the group
+// ends with the same bits, and the association is fixed by (chunks, LANES): the result is deterministic.  The
+// first flag is the smallest flag code (seg * 4 + flag), i.e. the flagged pair with the lowest segment index.
+template <bool WIDE, int LANES>
 __global__ __launch_bounds__(256) void k_direct_reduce(int n, int chunks, const double* __restrict__ part,
                                                        const int32_t* __restrict__ pflag, double* __restrict__ out,
                                                        int8_t* __restrict__ flags, int32_t* __restrict__ flag_seg) {
+  static_assert(LANES >= 1 && LANES <= 64 && (64 % LANES) == 0, "a target's lanes share one wavefront");
   typedef typename std::conditional<WIDE, double, float>::type acc_t;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = g / LANES;
+  const int j = g % LANES;
   acc_t ux = 0, uy = 0, uz = 0;
-  int code = -1;
-  for (int c = 0; c < chunks; ++c) {
-    const size_t o = (size_t)c * n + p;
-    ux += (acc_t)part[3 * o];
-    uy += (acc_t)part[3 * o + 1];
-    uz += (acc_t)part[3 * o + 2];
-    if (code < 0) code = pflag[o];
+  int code = INT_MAX;
+  if (p < n) {
+    for (int c = j; c < chunks; c += LANES) {
+      const size_t o = (size_t)c * n + p;
+      ux += (acc_t)part[3 * o];
+      uy += (acc_t)part[3 * o + 1];
+      uz += (acc_t)part[3 * o + 2];
+      const int pc = pflag[o];
+      code = (pc >= 0 && pc < code) ? pc : code;
+    }
   }
-  int flag = code < 0 ? OFW_FLAG_OK : (code & 3);
-  int seg = code < 0 ? -1 : (code >> 2);
+#pragma unroll
+  for (int o = LANES / 2; o > 0; o >>= 1) {
+    ux += __shfl_xor(ux, o);
+    uy += __shfl_xor(uy, o);
+    uz += __shfl_xor(uz, o);
+    code = min(code, __shfl_xor(code, o));
+  }
+  if (p >= n || j != 0) return;
+  int flag = code == INT_MAX ? OFW_FLAG_OK : (code & 3);
+  int seg = code == INT_MAX ? -1 : (code >> 2);
   if (!(isfinite((double)ux) && isfinite((double)uy) && isfinite((double)uz)) && flag == OFW_FLAG_OK)
     flag = OFW_FLAG_NONFINITE;
   out[3 * (size_t)p] = (double)ux;
@@ -244,8 +263,8 @@ __global__ __launch_bounds__(256) void k_direct_reduce(int n, int chunks, const 
 
 template <int MODEL>
 static void launch_direct(int precision, dim3 grid, dim3 block, hipStream_t st, int n, const double* pts, int m,
-                          const double* rec64, const float* rec32, int chunk, const int32_t* ex_ptr,
-                          const int32_t* ex_seg, int raise, double* part, int32_t* pflag) {
+                          const double* rec64, const float* rec32, int chunk, const int64_t* ex_ptr,
+                          const int64_t* ex_seg, int raise, double* part, int32_t* pflag) {
   if (precision == OFW_FP64)
     hipLaunchKernelGGL(k_direct64<MODEL>, grid, block, 0, st, n, pts, m, rec64, chunk, ex_ptr, ex_seg, raise, part,
                        pflag);
@@ -257,56 +276,86 @@ static void launch_direct(int precision, dim3 grid, dim3 block, hipStream_t st, 
                        ex_seg, raise, part, pflag);
 }
 
-int direct_sum(Ctx& c, int precision, int block, int64_t n, const double* points, int64_t m, const double* rec64,
-               const float* rec32, int model, int raise_singular, const int32_t* ex_ptr, const int32_t* ex_seg,
-               double* out, int8_t* flags, int32_t* flag_seg) {
-  if (n == 0) return OFW_OK;
-  // the FP64 kernel gives each lane two targets
-  const int per_block = precision == OFW_FP64 ? 2 * block : block;
-  const int64_t bx = (n + per_block - 1) / per_block;
-  const int64_t waves_per_block = (block + 63) / 64;
+template <bool WIDE>
+static void launch_reduce(int lanes, hipStream_t st, int n, int chunks, const double* part, const int32_t* pflag,
+                          double* out, int8_t* flags, int32_t* flag_seg) {
+  const dim3 grid((unsigned)(((int64_t)n * lanes + 255) / 256));
+  switch (lanes) {
+    case 1: hipLaunchKernelGGL((k_direct_reduce<WIDE, 1>), grid, dim3(256), 0, st, n, chunks, part, pflag, out, flags, flag_seg); break;
+    case 4: hipLaunchKernelGGL((k_direct_reduce<WIDE, 4>), grid, dim3(256), 0, st, n, chunks, part, pflag, out, flags, flag_seg); break;
+    case 16: hipLaunchKernelGGL((k_direct_reduce<WIDE, 16>), grid, dim3(256), 0, st, n, chunks, part, pflag, out, flags, flag_seg); break;
+    default: hipLaunchKernelGGL((k_direct_reduce<WIDE, 64>), grid, dim3(256), 0, st, n, chunks, part, pflag, out, flags, flag_seg); break;
+  }
+}
+
+DirectPlan direct_plan(int cu_count, int precision, int block, int64_t n, int64_t m, int64_t max_chunks) {
+  DirectPlan pl{};
+  // A workgroup wider than the targets only adds idle lanes: halve it (down to one wavefront) while the
+  // targets still fit, so a call with few targets spends its lanes on more segment chunks instead.
+  const int per_lane = precision == OFW_FP64 ? 2 : 1;
+  int eff = block;
+  while (eff > 64 && (int64_t)per_lane * (eff / 2) >= n) eff /= 2;
+  pl.block = eff;
+  const int64_t per_block = (int64_t)per_lane * eff;
+  pl.bx = (n + per_block - 1) / per_block;
+  const int64_t waves_per_block = (eff + 63) / 64;
   // enough wavefronts for every SIMD of every CU to hold several (MI355X: 256 CUs x 4 SIMDs)
-  const int64_t want_waves = (int64_t)c.cu_count * 4 * 8;
-  int64_t chunks = (want_waves + bx * waves_per_block - 1) / (bx * waves_per_block);
+  const int64_t want_waves = (int64_t)cu_count * 4 * 8;
+  int64_t chunks = (want_waves + pl.bx * waves_per_block - 1) / (pl.bx * waves_per_block);
   const int64_t min_chunk = 64;
-  int64_t max_chunks = (m + min_chunk - 1) / min_chunk;
+  const int64_t most = (m + min_chunk - 1) / min_chunk;
+  if (chunks > most) chunks = most;
   if (chunks > max_chunks) chunks = max_chunks;
   if (chunks < 1) chunks = 1;
   if (chunks > 65535) chunks = 65535;
   int64_t chunk = (m + chunks - 1) / chunks;
   if (chunk < 1) chunk = 1;
-  chunks = m > 0 ? (m + chunk - 1) / chunk : 1;
-  OFW_HIP(c.part.reserve((size_t)chunks * n * 3 * sizeof(double)));
-  OFW_HIP(c.pflag.reserve((size_t)chunks * n * sizeof(int32_t)));
-  double* part = (double*)c.part.ptr;
-  int32_t* pflag = (int32_t*)c.pflag.ptr;
-  if (m > 0) {
-    dim3 grid((unsigned)bx, (unsigned)chunks);
-    dim3 blk((unsigned)block);
+  pl.chunk = chunk;
+  pl.chunks = m > 0 ? (m + chunk - 1) / chunk : 0;
+  pl.lanes = pl.chunks <= 8 ? 1 : pl.chunks <= 32 ? 4 : pl.chunks <= 128 ? 16 : 64;
+  return pl;
+}
+
+int direct_run(Ctx& c, const DirectPlan& pl, int precision, int64_t n, const double* points, int64_t m,
+               const double* rec64, const float* rec32, int model, int raise_singular, const int64_t* ex_ptr,
+               const int64_t* ex_seg, double* part, int32_t* pflag, double* out, int8_t* flags, int32_t* flag_seg) {
+  if (n == 0) return OFW_OK;
+  if (pl.chunks > 0) {
+    dim3 grid((unsigned)pl.bx, (unsigned)pl.chunks);
+    dim3 blk((unsigned)pl.block);
+    const int chunk = (int)pl.chunk;
     switch (model) {
-      case 0: launch_direct<0>(precision, grid, blk, c.stream, (int)n, points, (int)m, rec64, rec32, (int)chunk, ex_ptr, ex_seg, raise_singular, part, pflag); break;
-      case 1: launch_direct<1>(precision, grid, blk, c.stream, (int)n, points, (int)m, rec64, rec32, (int)chunk, ex_ptr, ex_seg, raise_singular, part, pflag); break;
-      case 2: launch_direct<2>(precision, grid, blk, c.stream, (int)n, points, (int)m, rec64, rec32, (int)chunk, ex_ptr, ex_seg, raise_singular, part, pflag); break;
-      case 3: launch_direct<3>(precision, grid, blk, c.stream, (int)n, points, (int)m, rec64, rec32, (int)chunk, ex_ptr, ex_seg, raise_singular, part, pflag); break;
-      case 4: launch_direct<4>(precision, grid, blk, c.stream, (int)n, points, (int)m, rec64, rec32, (int)chunk, ex_ptr, ex_seg, raise_singular, part, pflag); break;
-      default: launch_direct<5>(precision, grid, blk, c.stream, (int)n, points, (int)m, rec64, rec32, (int)chunk, ex_ptr, ex_seg, raise_singular, part, pflag); break;
+      case 0: launch_direct<0>(precision, grid, blk, c.stream, (int)n, points, (int)m, rec64, rec32, chunk, ex_ptr, ex_seg, raise_singular, part, pflag); break;
+      case 1: launch_direct<1>(precision, grid, blk, c.stream, (int)n, points, (int)m, rec64, rec32, chunk, ex_ptr, ex_seg, raise_singular, part, pflag); break;
+      case 2: launch_direct<2>(precision, grid, blk, c.stream, (int)n, points, (int)m, rec64, rec32, chunk, ex_ptr, ex_seg, raise_singular, part, pflag); break;
+      case 3: launch_direct<3>(precision, grid, blk, c.stream, (int)n, points, (int)m, rec64, rec32, chunk, ex_ptr, ex_seg, raise_singular, part, pflag); break;
+      case 4: launch_direct<4>(precision, grid, blk, c.stream, (int)n, points, (int)m, rec64, rec32, chunk, ex_ptr, ex_seg, raise_singular, part, pflag); break;
+      default: launch_direct<5>(precision, grid, blk, c.stream, (int)n, points, (int)m, rec64, rec32, chunk, ex_ptr, ex_seg, raise_singular, part, pflag); break;
     }
     OFW_HIP(hipGetLastError());
-  } else {
-    chunks = 0;
   }
   OFW_HIP(c.mark());
   if (precision == OFW_FP32)
-    hipLaunchKernelGGL(k_direct_reduce<false>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, (int)n,
-                       (int)chunks, part, pflag, out, flags, flag_seg);
+    launch_reduce<false>(pl.lanes, c.stream, (int)n, (int)pl.chunks, part, pflag, out, flags, flag_seg);
   else
-    hipLaunchKernelGGL(k_direct_reduce<true>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, (int)n,
-                       (int)chunks, part, pflag, out, flags, flag_seg);
+    launch_reduce<true>(pl.lanes, c.stream, (int)n, (int)pl.chunks, part, pflag, out, flags, flag_seg);
   OFW_HIP(hipGetLastError());
   OFW_HIP(c.mark());
-  c.stats_i[0] = chunks;
-  c.stats_i[1] = chunk;
+  c.stats_i[0] = pl.chunks;
+  c.stats_i[1] = pl.chunk;
   return OFW_OK;
+}
+
+int direct_sum(Ctx& c, int precision, int block, int64_t n, const double* points, int64_t m, const double* rec64,
+               const float* rec32, int model, int raise_singular, const int64_t* ex_ptr, const int64_t* ex_seg,
+               double* out, int8_t* flags, int32_t* flag_seg) {
+  if (n == 0) return OFW_OK;
+  const DirectPlan pl = direct_plan(c.cu_count, precision, block, n, m, INT64_MAX);
+  const int64_t slots = pl.chunks > 0 ? pl.chunks : 1;
+  OFW_HIP(c.part.reserve((size_t)slots * n * 3 * sizeof(double)));
+  OFW_HIP(c.pflag.reserve((size_t)slots * n * sizeof(int32_t)));
+  return direct_run(c, pl, precision, n, points, m, rec64, rec32, model, raise_singular, ex_ptr, ex_seg,
+                    (double*)c.part.ptr, (int32_t*)c.pflag.ptr, out, flags, flag_seg);
 }
 
 }  // namespace ofw

@@ -46,10 +46,24 @@ struct Ctx {
   } while (0)
 
 // direct.hip
+// Launch shape of a direct sum: workgroup width, target blocks, segment chunks (0 when there are no segments),
+// segments per chunk and lanes per target of the chunk reduction.  A function of (CU count, precision, block,
+// n, m, chunk cap) only, so a call's association is fixed by its configuration.
+struct DirectPlan {
+  int block;
+  int64_t bx, chunks, chunk;
+  int lanes;
+};
+DirectPlan direct_plan(int cu_count, int precision, int block, int64_t n, int64_t m, int64_t max_chunks);
+// the sum with caller-provided partial slots (max(pl.chunks, 1) * n of each); the exclusion CSR is the caller's
+// int64 one (ptr (n+1), per-target sorted unique segment ids)
+int direct_run(Ctx& c, const DirectPlan& pl, int precision, int64_t n, const double* points, int64_t m,
+               const double* rec64, const float* rec32, int model, int raise_singular, const int64_t* ex_ptr,
+               const int64_t* ex_seg, double* part, int32_t* pflag, double* out, int8_t* flags, int32_t* flag_seg);
 int pack_segments(Ctx& c, int64_t m, const double* starts, const double* directions, const double* lengths,
                   const double* gamma, const double* radius, int model, double* rec64, float* rec32);
 int direct_sum(Ctx& c, int precision, int block, int64_t n, const double* points, int64_t m, const double* rec64,
-               const float* rec32, int model, int raise_singular, const int32_t* ex_ptr, const int32_t* ex_seg,
+               const float* rec32, int model, int raise_singular, const int64_t* ex_ptr, const int64_t* ex_seg,
                double* out, int8_t* flags, int32_t* flag_seg);
 int csr_to_i32(Ctx& c, int64_t n, const int64_t* ex_ptr, const int64_t* ex_seg, int32_t** ptr32, int32_t** seg32);
 // the two halves of csr_to_i32: the entry count (read back, synchronising) and the enqueued conversion

@@ -253,11 +253,8 @@ static int step_stage(ofw_step& p, const double* pts, double t, int slot) {
                        (float*)c.rec32.ptr);
     if (st) return st;
     OFW_HIP(hipStreamWaitEvent(main, p.join_ev, 0));
-    int32_t *p32 = nullptr, *s32 = nullptr;
-    st = csr_convert(c, T, p.pair_cap, p.ex_ptr, p.ex_seg, &p32, &s32);
-    if (st) return st;
     st = direct_sum(c, p.precision, p.threads, T, p.targets, p.m, (const double*)c.rec64.ptr,
-                    (const float*)c.rec32.ptr, p.model, 1, p32, s32, p.out, p.flags, nullptr);
+                    (const float*)c.rec32.ptr, p.model, 1, p.ex_ptr, p.ex_seg, p.out, p.flags, nullptr);
     if (st) return st;
   }
   hipLaunchKernelGGL(k_step_collect, dim3(1), dim3(64), 0, c.stream, pstat, p.pair_cap,
