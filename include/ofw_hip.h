@@ -102,6 +102,26 @@ int ofw_direct_sum(ofw_ctx *ctx, int precision, int block, int64_t n, const doub
                    const double *radius, int model, int raise_singular, const int64_t *ex_ptr,
                    const int64_t *ex_seg, double *out, int8_t *flags, int32_t *flag_seg);
 
+/* Bytes of the static scratch of ofw_direct_sum_static: one device array of this size per GPU, held from the
+ * first context created on that GPU until the process ends. */
+#define OFW_STATIC_SCRATCH_BYTES (128LL << 20)
+
+/* ofw_direct_sum without allocating device memory.  It has no counterpart in the reference (whose engines
+ * allocate on every call); it is for callers that cannot allocate: tight serving loops and harnesses that
+ * forbid hipMalloc.  Same arguments, results and flags as ofw_direct_sum, and the same bits whenever the call
+ * fits the scratch with ofw_direct_sum's own chunk count.  Its scratch (the packed segment records, 160 B per
+ * segment, and the chunk partials, 28 B per target and segment chunk) is carved from a static device array of
+ * OFW_STATIC_SCRATCH_BYTES on the context's GPU instead of the context's buffers; the chunk count is lowered
+ * until the partials fit, and a call that does not fit with one chunk returns OFW_ERR_ARG (at most about
+ * 800,000 segments).  It calls no hipMalloc or hipFree.  Every call on one GPU shares that array, which keeps the
+ * last call's contents: calls that overlap on one GPU (several host threads, or several contexts and streams)
+ * race.  Synchronous on return, like ofw_direct_sum. */
+int ofw_direct_sum_static(ofw_ctx *ctx, int precision, int block, int64_t n, const double *points, int64_t m,
+                          const double *starts, const double *directions, const double *lengths,
+                          const double *gamma, const double *radius, int model, int raise_singular,
+                          const int64_t *ex_ptr, const int64_t *ex_seg, double *out, int8_t *flags,
+                          int32_t *flag_seg);
+
 /* Counters of the context's last call (host buffer of `count` int64): [0] direct-sum segment chunks,
  * [1] segments per chunk, [2] traversal subtrees, [3] traversal split depth, [4] tree nodes, [5] tree depth,
  * [6] near-field work items of the traversal, [7] runs of the tree pipeline the last ofw_tree_sum took (2 when the

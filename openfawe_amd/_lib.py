@@ -28,6 +28,8 @@ SIGNATURES = {
     "ofw_device_info": (_c_i, [ctypes.c_char_p, _c_i, ctypes.POINTER(_c_i)]),
     "ofw_direct_sum": (_c_i, [_c_p, _c_i, _c_i, _c_i64, _c_p, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i, _c_i,
                               _c_p, _c_p, _c_p, _c_p, _c_p]),
+    "ofw_direct_sum_static": (_c_i, [_c_p, _c_i, _c_i, _c_i64, _c_p, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i,
+                                     _c_i, _c_p, _c_p, _c_p, _c_p, _c_p]),
     "ofw_ctx_stats": (_c_i, [_c_p, ctypes.POINTER(_c_i64), _c_i]),
     "ofw_build_tree": (_c_i, [_c_p, _c_i64, _c_p, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_p, ctypes.POINTER(_c_i64)]),
     "ofw_tree_nodes": (_c_i, [_c_p, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i64] + [_c_p] * 9

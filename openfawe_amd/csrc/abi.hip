@@ -1,5 +1,6 @@
 // C ABI of libofw_hip.so: contexts, errors and the entry points declared in include/ofw_hip.h.
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "common.h"
@@ -36,6 +37,31 @@ void DevBuf::release() {
   bytes = 0;
 }
 
+// Scratch of ofw_direct_sum_static: a fixed device array, one copy per GPU, that lives until the process ends.
+// In host code `scratch` names a host-side stand-in, not device memory: its device address is resolved once per
+// device (with that device current) by hipGetSymbolAddress, and only that address is handed to kernels.
+static __device__ __align__(256) unsigned char scratch[OFW_STATIC_SCRATCH_BYTES];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mutex;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device`, which must be the current device.
+static int static_scratch(int device, void** ptr_out) {
+  if (device < 0 || device >= kMaxDevices) {
+    set_error("static scratch: device index out of range");
+    return OFW_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    OFW_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = ptr;
+  }
+  *ptr_out = g_scratch_ptr[device];
+  return OFW_OK;
+}
+
 }  // namespace ofw
 
 using ofw::set_error;
@@ -58,6 +84,9 @@ int ofw_ctx_create(ofw_ctx** out, void* stream) {
   OFW_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, x->c.device));
   x->c.cu_count = cus > 0 ? cus : 256;
   for (auto& e : x->c.ev) OFW_HIP(hipEventCreate(&e));
+  // set-up of the no-allocation entry point: this device's copy of the static scratch
+  int st = ofw::static_scratch(x->c.device, &x->c.static_scratch);
+  if (st) return st;
   *out = x;
   return OFW_OK;
 }
@@ -162,6 +191,57 @@ int ofw_direct_sum(ofw_ctx* x, int precision, int block, int64_t n, const double
   OFW_HIP(c.mark());
   st = ofw::direct_sum(c, precision, block, n, points, m, (const double*)c.rec64.ptr, (const float*)c.rec32.ptr,
                        model, raise_singular, ex_ptr, ex_seg, out, flags, flag_seg);
+  if (st) return st;
+  OFW_HIP(hipStreamSynchronize(c.stream));
+  return OFW_OK;
+}
+
+// ofw_direct_sum on the static scratch: the same kernels and launch plan, with the records and the chunk
+// partials carved from this device's copy of `scratch` (its resolved address) instead of the context's buffers.
+int ofw_direct_sum_static(ofw_ctx* x, int precision, int block, int64_t n, const double* points, int64_t m,
+                          const double* starts, const double* directions, const double* lengths,
+                          const double* gamma, const double* radius, int model, int raise_singular,
+                          const int64_t* ex_ptr, const int64_t* ex_seg, double* out, int8_t* flags,
+                          int32_t* flag_seg) {
+  int st = direct_args(x, "ofw_direct_sum_static", precision, block, n, m, model);
+  if (st) return st;
+  ofw::Ctx& c = x->c;
+  c.nev = 0;
+  if (n == 0) return OFW_OK;
+  if (!c.static_scratch) {
+    set_error("ofw_direct_sum_static: no static scratch on this device");
+    return OFW_ERR_HIP;
+  }
+  auto up = [](int64_t bytes) { return (bytes + 255) / 256 * 256; };
+  const int64_t records = m > 0 ? m : 1;
+  const int64_t rec64_bytes = up(records * REC64 * (int64_t)sizeof(double));
+  const int64_t rec32_bytes = up(records * REC32 * (int64_t)sizeof(float));
+  const int64_t slot_bytes = n * (3 * (int64_t)sizeof(double) + (int64_t)sizeof(int32_t));  // one chunk's partials
+  const int64_t fixed = rec64_bytes + rec32_bytes + 256;
+  const int64_t room = OFW_STATIC_SCRATCH_BYTES - fixed;
+  if (room < slot_bytes) {
+    set_error("ofw_direct_sum_static: " + std::to_string(n) + " targets x " + std::to_string(m) + " segments need " +
+              std::to_string(fixed + slot_bytes) + " bytes of scratch, more than the " +
+              std::to_string((int64_t)OFW_STATIC_SCRATCH_BYTES) + " of the static scratch");
+    return OFW_ERR_ARG;
+  }
+  const ofw::DirectPlan pl = ofw::direct_plan(c.cu_count, precision, block, n, m, room / slot_bytes);
+  char* base = (char*)c.static_scratch;
+  double* rec64 = (double*)base;
+  float* rec32 = (float*)(base + rec64_bytes);
+  double* part = (double*)(base + rec64_bytes + rec32_bytes);
+  const int64_t slots = pl.chunks > 0 ? pl.chunks : 1;
+  int32_t* pflag = (int32_t*)((char*)part + up(slots * n * 3 * (int64_t)sizeof(double)));
+  if ((char*)(pflag + slots * n) > base + OFW_STATIC_SCRATCH_BYTES) {
+    set_error("ofw_direct_sum_static: scratch layout exceeds the static scratch");
+    return OFW_ERR_ARG;
+  }
+  OFW_HIP(c.mark());
+  st = ofw::pack_segments(c, m, starts, directions, lengths, gamma, radius, model, rec64, rec32);
+  if (st) return st;
+  OFW_HIP(c.mark());
+  st = ofw::direct_run(c, pl, precision, n, points, m, rec64, rec32, model, raise_singular, ex_ptr, ex_seg, part,
+                       pflag, out, flags, flag_seg);
   if (st) return st;
   OFW_HIP(hipStreamSynchronize(c.stream));
   return OFW_OK;

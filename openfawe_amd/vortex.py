@@ -206,10 +206,13 @@ class Statistics:
 
 
 def hip_sum_device(points, starts, directions, lengths, gamma, radius, model, ex_ptr, ex_seg, singularity,
-                   settings, out=None, flags=None, flag_seg=None):
+                   settings, out=None, flags=None, flag_seg=None, static_scratch=False):
     """Direct sum on device tensors (float64 / int64, contiguous, on the current GPU).
 
-    Returns (out (n,3) float64, flags (n,) int8, flag_seg (n,) int32) as device tensors.
+    Returns (out (n,3) float64, flags (n,) int8, flag_seg (n,) int32) as device tensors.  With
+    ``static_scratch`` the call goes through ``ofw_direct_sum_static``, which allocates no device memory
+    (its scratch is the library's fixed per-GPU array; with out / flags / flag_seg given, nothing is
+    allocated at all).
     """
     torch = _torch()
     n, m = int(points.shape[0]), int(starts.shape[0])
@@ -222,11 +225,12 @@ def hip_sum_device(points, starts, directions, lengths, gamma, radius, model, ex
     if n == 0:
         return out, flags, flag_seg
     lib = _lib.load()
-    status = lib.ofw_direct_sum(_ctx(), _lib.PRECISIONS[settings.precision], int(settings.block), n, _ptr(points), m,
+    name = "ofw_direct_sum_static" if static_scratch else "ofw_direct_sum"
+    status = getattr(lib, name)(_ctx(), _lib.PRECISIONS[settings.precision], int(settings.block), n, _ptr(points), m,
                                 _ptr(starts), _ptr(directions), _ptr(lengths), _ptr(gamma), _ptr(radius), int(model),
                                 int(singularity == "raise"), _ptr(ex_ptr), _ptr(ex_seg), _ptr(out), _ptr(flags),
                                 _ptr(flag_seg))
-    _lib.check(status, "ofw_direct_sum")
+    _lib.check(status, name)
     return out, flags, flag_seg
 
 
