@@ -33,6 +33,15 @@ enum {
 
 #define OFW_AXIS_EPS (8.0 * 2.220446049250313e-16)   // 8 * np.finfo(np.float64).eps, vortex_numba.py:28
 #define OFW_LAMB 1.25643
+
+// Threshold bands of the fast FP64 pair.  The axis guard (h <= 8 eps max(1, |a|, |b|), vortex_numba.py:59) and the
+// Lamb-Oseen cutoff (h / rc <= sqrt(10 / 1.25643), vortex_numba.py:95-97) are decided on squares only where the
+// squares are more than 1e-12 (relative) from the threshold; the squares and the reference's hypot norms differ
+// by a few ulps (below 1e-15), so the decision there is the reference's.  Inside the band the decision is taken
+// with the reference's operations (hypot norms, h / rc) on the wavefront's ballot.
+#define OFW_BAND_HI (1.0 + 1e-12)
+#define OFW_BAND_LO (1.0 - 1e-12)
+#define OFW_LAMB_Q (10.0 / OFW_LAMB)                  // (h / rc)^2 at the cutoff
 #define OFW_FOUR_PI (4.0 * 3.141592653589793)        // 4 * np.pi, vortex_numba.py:30
 
 // Fast FP64 path limits: outside them the pair is evaluated operation by operation as the reference
@@ -126,6 +135,22 @@ __device__ __noinline__ PairOut pair_exact64(double px, double py, double pz, co
   return PairOut{coefficient * cx, coefficient * cy, coefficient * cz, OFW_FLAG_OK};
 }
 
+// The reference's axis guard with its own operations: h <= 8 eps max(1, max(|a|, |a - d|)) with hypot norms.
+__device__ __noinline__ bool axis_reference(double ax, double ay, double az, double bx, double by, double bz,
+                                           double cx, double cy, double cz) {
+#pragma clang fp contract(off)
+  const double h = ofw_norm3(cx, cy, cz), na = ofw_norm3(ax, ay, az), nb = ofw_norm3(bx, by, bz);
+  const double big = na > nb ? na : nb;
+  return h <= OFW_AXIS_EPS * (1.0 > big ? 1.0 : big);
+}
+
+// The reference's Lamb-Oseen cutoff test with its own operations: ratio = h / rc <= sqrt(10 / 1.25643).
+__device__ __noinline__ bool lamb_inside_reference(double cx, double cy, double cz, double rc) {
+#pragma clang fp contract(off)
+  const double ratio = ofw_norm3(cx, cy, cz) / rc;
+  return ratio <= sqrt(10.0 / OFW_LAMB);
+}
+
 // Reciprocal and square root for normal, positive FP64 operands: the hardware estimates (v_rcp_f64,
 // v_rsq_f64) refined by Newton-Raphson steps to within an ulp or two, instead of the IEEE-exact division and
 // square-root sequences with their range scaling.  Every operation is written out (no contraction), so the
@@ -163,7 +188,8 @@ __device__ __forceinline__ double ofw_rsqrt(double x) {
 
 // Fast FP64 pair: the reference formula (vortex_numba.py:46-109) reorganised for the VALU: a per-segment
 // reciprocal length and circulation factor, squared norms instead of hypot, the axis test and the core
-// ratios on squares, one reciprocal per pair.  The same decisions and flags, taken with selects so that a
+// ratios on squares, one reciprocal per pair.  The same decisions and flags (the thresholds where a decision jumps
+// are decided within their bands by the reference's operations), taken with selects so that a
 // wavefront evaluating one segment for 64 targets does not diverge; only the rare paths (the Lamb-Oseen core
 // inside its cutoff, extreme ranges) branch, on the wavefront's ballot.  A lane with use == false evaluates
 // the pair but adds nothing (an excluded pair) and reports no flag.  a = (p - s)/L and the cross product
@@ -195,7 +221,11 @@ __device__ __forceinline__ int pair_fast64(double px, double py, double pz, cons
   const double t = __builtin_fma(az, dz, __builtin_fma(ay, dy, ax * dx));
   const double na = ofw_sqrt(na2), nb = ofw_sqrt(nb2);
   const double mx = fmax(1.0, fmax(na, nb));
-  const bool axis = h2 <= (OFW_AXIS_EPS * OFW_AXIS_EPS) * (mx * mx);
+  // axis guard: certain outside the band (one multiply more than a plain test), exact inside it
+  bool axis = h2 <= (OFW_AXIS_EPS * OFW_AXIS_EPS * OFW_BAND_HI) * (mx * mx);
+  if (__builtin_amdgcn_ballot_w64(use && !slow && axis) != 0 && axis &&
+      !(h2 < (OFW_AXIS_EPS * OFW_AXIS_EPS * OFW_BAND_LO) * (mx * mx)))
+    axis = axis_reference(ax, ay, az, bx, by, bz, cx, cy, cz);
   const double g4 = r[R_G4];
   double sa = na, sb = nb;
   double product = na * nb;
@@ -227,9 +257,13 @@ __device__ __forceinline__ int pair_fast64(double px, double py, double pz, cons
       if (MODEL == OFW_MODEL_RANKINE) {
         factor = cored ? fmin(q, 1.0) : 1.0;
       } else if (MODEL == OFW_MODEL_LAMB) {
-        // inside the cutoff only (a few pairs per target at most): a branch on the wavefront's ballot
-        const bool inside = use && cored && q <= 10.0 / OFW_LAMB;
-        if (__builtin_amdgcn_ballot_w64(inside) != 0 && inside) factor = -expm1(-OFW_LAMB * q);
+        // inside the cutoff only (a few pairs per target at most): a branch on the wavefront's ballot, where a
+        // q within the band of the cutoff is decided by the reference's test
+        bool inside = use && cored && q <= OFW_LAMB_Q * OFW_BAND_HI;
+        if (__builtin_amdgcn_ballot_w64(inside) != 0 && inside) {
+          if (!(q < OFW_LAMB_Q * OFW_BAND_LO)) inside = lamb_inside_reference(cx, cy, cz, r[R_RC]);
+          if (inside) factor = -expm1(-OFW_LAMB * q);
+        }
       } else {  // Vatistas: q / sqrt(1 + q^2) below one core radius, 1 / sqrt(1 + q^-2) beyond
         const double iq = q < 1.0 ? q : ofw_rcp(q);
         const double rs = ofw_rsqrt(__builtin_fma(iq, iq, 1.0));
