@@ -119,9 +119,9 @@ def cpu_baseline(case, spec, threads):
 
 
 def adjoint_bench(steps):
-    """Differentiation case: 100-panel VSM wing in a ~205k-segment wake.  Times the FP64 forward direct sum at
-    the 3/4-chord points, the fused forward + adjoint (dV -> dGamma, d(end points), d(points)), and the
-    100 x 100 attached-ring influence Jacobian (pair matrix + ordered column sums)."""
+    """Differentiation case: 100-panel VSM wing in a ~205k-segment wake.  Times the forward direct sum at the
+    3/4-chord points, the fused forward + adjoint (dV -> dGamma, d(end points), d(points)), each in FP64 and in
+    mixed precision, and the 100 x 100 attached-ring influence Jacobian (pair matrix + ordered column sums)."""
     import torch
     from openfawe_amd import adjoint as AD
     from openfawe_amd import vortex as V
@@ -134,7 +134,6 @@ def adjoint_bench(steps):
     lens, gam, rad = dev(case["lengths"]), dev(case["gamma"]), dev(case["radius"])
     ends = starts + dirs * lens[:, None]
     ptr, seg = dev(np.zeros(n + 1, np.int64), torch.int64), dev(np.zeros(1, np.int64), torch.int64)
-    settings = V.HipSettings("fp64", 128)
     grad = dev(np.random.default_rng(0).normal(size=(n, 3)))
     ws, wd, wl, wr = (dev(case[k]) for k in ("wing_starts", "wing_directions", "wing_lengths", "wing_radius"))
     cptr, cseg, csign = dev(case["column_ptr"], torch.int64), dev(case["column_segment"], torch.int64), \
@@ -150,15 +149,22 @@ def adjoint_bench(steps):
         torch.cuda.synchronize()
         return 1e3 * (time.perf_counter() - t0) / steps
 
-    fwd = timed(lambda: V.hip_sum_device(pts, starts, dirs, lens, gam, rad, 2, ptr, seg, "zero", settings))
-    fused = timed(lambda: AD.direct_vjp(pts, starts, ends, gam, rad, 2, grad, ptr, seg, fused_forward=True))
+    res = {}
+    for precision in ("fp64", "mixed"):
+        hs = V.HipSettings(precision, 128)
+        fwd = timed(lambda: V.hip_sum_device(pts, starts, dirs, lens, gam, rad, 2, ptr, seg, "zero", hs))
+        fused = timed(lambda: AD.direct_vjp(pts, starts, ends, gam, rad, 2, grad, ptr, seg, fused_forward=True,
+                                            precision=precision))
+        res[precision] = dict(forward_ms=fwd, fused_forward_adjoint_ms=fused, adjoint_fwd_ratio=fused / fwd)
 
     def jac():
         pairs, _ = AD.pair_matrix_device(pts, ws, wd, wl, wr, 2)
         return AD.assemble_columns(pairs, cptr, cseg, csign)
     jms = timed(jac)
-    return dict(workload=case["name"], targets=n, segments=m, forward_ms=fwd, fused_forward_adjoint_ms=fused,
-                adjoint_fwd_ratio=fused / fwd, influence_jacobian_ms=jms, jacobian_shape=[n, n, 3])
+    return dict(workload=case["name"], targets=n, segments=m, **res["fp64"], mixed=res["mixed"],
+                influence_jacobian_ms=jms, jacobian_shape=[n, n, 3],
+                note="fused = forward velocities + dL/dGamma + dL/d(end points) + dL/d(targets) in one call; "
+                     "mixed: far pairs in FP32, near pairs and accumulation in FP64, against the mixed forward")
 
 
 def network_bench(case, steps, cpu=True):

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define OFW_ABI_VERSION 1
+#define OFW_ABI_VERSION 2
 
 enum ofw_status {
   OFW_OK = 0,
@@ -207,9 +207,12 @@ int ofw_segment_geometry(ofw_ctx *ctx, int64_t m, const double *starts, const do
  * VSM Jacobian vsm.py:333-349 consumes dV/dGamma built from pair_matrix).  Given grad_out = dL/dV (n,3):
  * grad_gamma (m) = dL/dGamma, grad_starts / grad_ends (m,3) = dL/d(segment end points), and optionally
  * grad_points (n,3) = dL/d(targets) and the fused forward velocities out (n,3) (either may be NULL).
- * Segment s runs from starts[s] to ends[s]; models 0-4 (the Rosenhead-Moore core is refused). */
-int ofw_direct_vjp(ofw_ctx *ctx, int64_t n, const double *points, int64_t m, const double *starts, const double *ends,
-                   const double *gamma, const double *radius, int model, const int64_t *ex_ptr,
+ * Segment s runs from starts[s] to ends[s]; models 0-5 (vortex.py:21-27), the Rosenhead-Moore core included.
+ * precision: OFW_FP64, or OFW_MIXED (pairs more than 4 core radii from the segment and away from its line in
+ * FP32, near pairs and every accumulation in FP64; the criteria of the mixed direct sum).  The exclusion CSR is
+ * that of ofw_direct_sum (sorted unique segment ids per target). */
+int ofw_direct_vjp(ofw_ctx *ctx, int precision, int64_t n, const double *points, int64_t m, const double *starts,
+                   const double *ends, const double *gamma, const double *radius, int model, const int64_t *ex_ptr,
                    const int64_t *ex_seg, const double *grad_out, double *grad_gamma, double *grad_starts,
                    double *grad_ends, double *grad_points, double *out);
 

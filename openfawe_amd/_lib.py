@@ -46,7 +46,7 @@ SIGNATURES = {
     "ofw_pair_matrix": (_c_i, [_c_p, _c_i64, _c_p, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_i, _c_i, _c_p, _c_p]),
     "ofw_assemble_columns": (_c_i, [_c_p, _c_i64, _c_i64, _c_p, _c_i64, _c_p, _c_p, _c_p, _c_p]),
     "ofw_segment_geometry": (_c_i, [_c_p, _c_i64, _c_p, _c_p, _c_p, _c_p]),
-    "ofw_direct_vjp": (_c_i, [_c_p, _c_i64, _c_p, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_i, _c_p, _c_p, _c_p, _c_p, _c_p,
+    "ofw_direct_vjp": (_c_i, [_c_p, _c_i, _c_i64, _c_p, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_i, _c_p, _c_p, _c_p, _c_p, _c_p,
                               _c_p, _c_p, _c_p]),
     "ofw_pack_ring_edges": (_c_i, [_c_p, _c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_p, _c_i64,
                                    ctypes.POINTER(_c_i64)]),

@@ -83,11 +83,14 @@ def segment_geometry(starts, ends):
 
 
 def direct_vjp(points, starts, ends, gamma, radius, model, grad_out, ex_ptr=None, ex_seg=None, fused_forward=False,
-               want_points=True):
+               want_points=True, precision="fp64"):
     """Adjoint of the direct sum on device tensors.
 
-    Returns (grad_gamma (m,), grad_starts (m,3), grad_ends (m,3), grad_points (n,3) or None, out (n,3) or None).
+    ``precision`` 'fp64', or 'mixed' (far pairs in FP32, near pairs and accumulation in FP64).  Returns
+    (grad_gamma (m,), grad_starts (m,3), grad_ends (m,3), grad_points (n,3) or None, out (n,3) or None).
     """
+    if precision not in ("fp64", "mixed"):
+        raise ValueError("the adjoint runs in precision 'fp64' or 'mixed'")
     torch = _torch()
     pts, a, b, g, r, go = (_f64(torch, v) for v in (points, starts, ends, gamma, radius, grad_out))
     n, m = pts.shape[0], a.shape[0]
@@ -99,9 +102,10 @@ def direct_vjp(points, starts, ends, gamma, radius, model, grad_out, ex_ptr=None
     gb = torch.zeros((m, 3), dtype=torch.float64, device="cuda")
     gx = torch.zeros((n, 3), dtype=torch.float64, device="cuda") if want_points else None
     out = torch.zeros((n, 3), dtype=torch.float64, device="cuda") if fused_forward else None
-    status = _lib.load().ofw_direct_vjp(_ctx(), n, _ptr(pts), m, _ptr(a), _ptr(b), _ptr(g), _ptr(r), int(model),
-                                        _ptr(ex_ptr), _ptr(ex_seg), _ptr(go), _ptr(gg), _ptr(ga), _ptr(gb),
-                                        _ptr(gx) if gx is not None else None, _ptr(out) if out is not None else None)
+    status = _lib.load().ofw_direct_vjp(_ctx(), _lib.PRECISIONS[precision], n, _ptr(pts), m, _ptr(a), _ptr(b), _ptr(g),
+                                        _ptr(r), int(model), _ptr(ex_ptr), _ptr(ex_seg), _ptr(go), _ptr(gg), _ptr(ga),
+                                        _ptr(gb), _ptr(gx) if gx is not None else None,
+                                        _ptr(out) if out is not None else None)
     _lib.check(status, "ofw_direct_vjp")
     return gg, ga, gb, gx, out
 
@@ -137,13 +141,16 @@ def _autograd_function():
                            ex_ptr, ex_seg, singularity, precision)
             ctx.save_for_backward(points, starts, ends, gamma, radius, ex_ptr, ex_seg)
             ctx.model = model
+            ctx.precision = precision
             return out
 
         @staticmethod
         def backward(ctx, grad_out):
             points, starts, ends, gamma, radius, ex_ptr, ex_seg = ctx.saved_tensors
+            # the adjoint of the forward's own arithmetic: FP64, or the mixed split of far and near pairs
             gg, ga, gb, gx, _ = direct_vjp(points, starts, ends, gamma, radius, ctx.model, grad_out.contiguous(),
-                                           ex_ptr, ex_seg, want_points=ctx.needs_input_grad[0])
+                                           ex_ptr, ex_seg, want_points=ctx.needs_input_grad[0],
+                                           precision=ctx.precision)
             return gx, ga, gb, gg, None, None, None, None, None, None
 
     return InducedVelocity
@@ -154,7 +161,11 @@ _FN = []
 
 def induced_velocity(points, starts, ends, gamma, radius, model=0, exclusions=None, singularity="raise",
                      precision="fp64"):
-    """Differentiable induced velocity (n,3) of straight segments starts -> ends at points (torch tensors)."""
+    """Differentiable induced velocity (n,3) of straight segments starts -> ends at points (torch tensors).
+
+    ``precision`` 'fp64' or 'mixed': the forward is the direct sum in that precision and the backward the
+    adjoint of the same split (far pairs in FP32, near pairs in FP64 for 'mixed').
+    """
     torch = _torch()
     if not _FN:
         _FN.append(_autograd_function())
@@ -166,5 +177,6 @@ def induced_velocity(points, starts, ends, gamma, radius, model=0, exclusions=No
     ptr, seg = exclusion_csr(n, exclusions if exclusions is not None else np.zeros((0, 2), np.int64))
     ex_ptr = torch.from_numpy(ptr).to("cuda")
     ex_seg = torch.from_numpy(seg if len(seg) else np.zeros(1, np.int64)).to("cuda")
-    HipSettings(precision)
+    if HipSettings(precision).precision not in ("fp64", "mixed"):
+        raise ValueError("induced_velocity is differentiable in precision 'fp64' or 'mixed'")
     return _FN[0].apply(points, starts, ends, gamma, radius, int(model), ex_ptr, ex_seg, singularity, precision)

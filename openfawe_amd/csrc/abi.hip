@@ -103,6 +103,7 @@ int ofw_ctx_destroy(ofw_ctx* x) {
   c.expos.release();
   c.exptr32.release();
   c.exseg32.release();
+  c.vjp.release();
   for (auto& b : c.tree) b.release();
   for (auto& e : c.ev)
     if (e) (void)hipEventDestroy(e);

@@ -25,7 +25,7 @@ struct Ctx {
   hipStream_t stream = nullptr;
   int device = 0;
   int cu_count = 256;
-  DevBuf rec64, rec32, part, pflag, expos, exptr32, exseg32;
+  DevBuf rec64, rec32, part, pflag, expos, exptr32, exseg32, vjp;
   DevBuf tree[24];           // tree build / moments / evaluation scratch
   std::vector<char> host;    // host staging
   hipEvent_t ev[12] = {};

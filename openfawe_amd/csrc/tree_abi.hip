@@ -536,6 +536,7 @@ int ofw_tree_batch_destroy(ofw_tree_batch* b) {
   c.expos.release();
   c.exptr32.release();
   c.exseg32.release();
+  c.vjp.release();
   for (auto& buf : c.tree) buf.release();
   for (auto& e : c.ev)
     if (e) (void)hipEventDestroy(e);

@@ -4,9 +4,11 @@ Bars: pair matrix within 1e-13 relative per pair of the oracle's pair_matrix and
 assembly bit for bit the oracle's on the same pairs and bit for bit the GPU direct sum of each column's
 signed segments (the reference's bitwise check, tests/test_vsm_wake_batched.py:50-70); the adjoint's
 dL/dGamma equal to pair_matrix^T dL/dV (linearity) to 1e-12; dL/d(end points) and dL/d(targets) against
-central finite differences (h = 1e-6) of the forward, rtol 2e-6; the VSM Newton Jacobian built from the GPU
-influence operator against central differences of the closure (rtol 2e-7, atol 2e-9, the reference's
-tests/test_vsm_jacobian.py:67-88).
+central finite differences (h = 1e-6) of the forward, rtol 2e-6, for every core model (the Rosenhead-Moore core
+included); the mixed-precision adjoint (far pairs in FP32) within 1e-5 relative L2 of the FP64 adjoint; the VSM
+Newton Jacobian built from the GPU influence operator against central differences of the closure (rtol 2e-7,
+atol 2e-9, the reference's tests/test_vsm_jacobian.py:67-88); the differentiation case at its full 207k-segment
+wake against the oracle.
 """
 import numpy as np
 import pytest
@@ -92,7 +94,7 @@ def forward(points, a, b, gamma, radius, model):
     return out
 
 
-@pytest.mark.parametrize("model", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("model", range(6))
 def test_vjp_against_linearity_and_finite_differences(model):
     nodes, edges, gamma, radius, points, s, d, L = network(4)
     points = points[:40]
@@ -223,3 +225,98 @@ def test_vsm_wing_in_wake_influence_and_adjoint():
     lin = np.einsum("psc,pc->s", pm, grad)
     bound = 1e-12 * np.einsum("psc->s", np.abs(pm) * np.abs(grad)[:, None, :]) + 1e-300
     assert np.all(np.abs(gg.cpu().numpy() - lin) <= bound)
+
+
+@pytest.mark.parametrize("model", [0, 2, 5])
+def test_mixed_adjoint_against_fp64(model):
+    """Far pairs in FP32: every gradient within 1e-5 relative L2 of the FP64 adjoint (and the fused forward within
+    the mixed direct sum's bound of the FP64 oracle), on a wake where most pairs are far."""
+    case = workloads.loop(rows=60, target_rows=4, seed=2)
+    pts = case["points"][::3]
+    s, d, L = case["starts"], case["directions"], case["lengths"]
+    ends = s + d * L[:, None]
+    g, r = case["gamma"], case["radius"]
+    grad = np.random.default_rng(5).normal(size=pts.shape)
+    ptr, seg = V.exclusion_csr(len(case["points"]), case["pairs"])
+    keep = np.arange(0, len(case["points"]), 3)
+    pairs = case["pairs"][np.isin(case["pairs"][:, 0], keep)].copy()
+    pairs[:, 0] //= 3
+    import torch
+    ptr, seg = V.exclusion_csr(len(pts), pairs)
+    ptr_d = torch.from_numpy(ptr).cuda()
+    seg_d = torch.from_numpy(seg if len(seg) else np.zeros(1, np.int64)).cuda()
+    res = {}
+    for precision in ("fp64", "mixed"):
+        res[precision] = [v.cpu().numpy() for v in AD.direct_vjp(pts, s, ends, g, r, model, grad, ptr_d, seg_d,
+                                                                  fused_forward=True, precision=precision)]
+    for k, name in enumerate(("gamma", "starts", "ends", "points", "forward")):
+        a, b = res["mixed"][k], res["fp64"][k]
+        rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+        assert rel <= 1e-5, (name, rel)
+        assert rel > 0.0 or name == "gamma"
+    ref, _, _ = O.direct_sum(pts, s, d, L, g, r, model, (ptr, seg), "raise", with_scale=True)
+    out = res["mixed"][4]
+    assert np.linalg.norm(out - ref) / np.linalg.norm(ref) <= 1e-6
+
+
+def test_rosenhead_moore_vjp_inside_the_core():
+    """One segment with a core of 0.3 lengths, targets inside the core beside the segment and close to its line
+    beyond an end, where the regularised norms dominate: gradients against central differences (h = 1e-6)."""
+    a = np.array([[0.0, 0.0, 0.0]])
+    b = np.array([[1.0, 0.0, 0.0]])
+    pts = np.array([[0.5, 0.2, 0.1], [1.5, 0.05, 0.02], [-0.1, 0.01, -0.03]])
+    g = np.random.default_rng(1).normal(size=pts.shape)
+    gg, ga, gb, gx, out = (v.cpu().numpy() for v in AD.direct_vjp(pts, a, b, np.array([1.3]), np.array([0.3]), 5, g,
+                                                                  fused_forward=True))
+    h = 1e-6
+    for base_name, grad in (("b", gb), ("x", gx)):
+        for k in range(len(grad)):
+            for c in range(3):
+                args = dict(a=a.copy(), b=b.copy(), x=pts.copy())
+                args[base_name][k, c] += h
+                vp = forward(args["x"], args["a"], args["b"], np.array([1.3]), np.array([0.3]), 5)
+                args[base_name][k, c] -= 2 * h
+                vm = forward(args["x"], args["a"], args["b"], np.array([1.3]), np.array([0.3]), 5)
+                fd = np.sum(g * (vp - vm)) / (2 * h)
+                assert abs(fd - grad[k, c]) <= 2e-6 * max(abs(fd), abs(grad[k, c])) + 1e-9, (base_name, k, c)
+
+
+def test_differentiation_case_full_wake():
+    """Config 5 at its own size: a 100-panel VSM wing in a 207k-segment wake.  The fused forward within A1 of the
+    oracle's direct sum, dL/dGamma equal to the oracle's unit-circulation pair velocities contracted with dL/dV
+    (seeded subset of 4,000 segments, 1e-12), end-point gradients against central differences (h = 1e-6) of
+    the GPU forward at sampled segments, and the mixed adjoint within 1e-5 of the FP64 one."""
+    case = workloads.vsm_wing()
+    pts, s, d, L, g, r = (case[k] for k in ("points", "starts", "directions", "lengths", "gamma", "radius"))
+    assert len(pts) == 100 and len(s) > 200_000
+    ends = s + d * L[:, None]
+    grad = np.random.default_rng(11).normal(size=pts.shape)
+    gg, ga, gb, gx, out = (v.cpu().numpy() for v in AD.direct_vjp(pts, s, ends, g, r, 2, grad, fused_forward=True))
+    ref, flags, scale = O.direct_sum(pts, s, d, L, g, r, 2, None, "raise", with_scale=True)
+    assert not flags.any()
+    assert np.all(np.linalg.norm(out - ref, axis=1) <= 1e-12 * scale)
+    rng = np.random.default_rng(12)
+    sub = np.sort(rng.choice(len(s), 4000, replace=False))
+    pm, _ = O.pair_matrix(pts, s[sub], d[sub], L[sub], r[sub], 2, "raise")
+    lin = np.einsum("psc,pc->s", pm, grad)
+    bound = 1e-12 * np.einsum("psc->s", np.abs(pm) * np.abs(grad)[:, None, :]) + 1e-300
+    assert np.all(np.abs(gg[sub] - lin) <= bound)
+    # the segments nearest the collocation points carry the largest end-point gradients: test those and a few
+    near = np.argsort(-np.linalg.norm(ga, axis=1))[:4]
+    h = 1e-6
+    for k in np.concatenate([near, rng.choice(len(s), 2, replace=False)]):
+        for which, grad_k in (("a", ga[k]), ("b", gb[k])):
+            for c in range(3):
+                sa, sb = s.copy(), ends.copy()
+                arr = sa if which == "a" else sb
+                arr[k, c] += h
+                vp = forward(pts, sa, sb, g, r, 2)
+                arr[k, c] -= 2 * h
+                vm = forward(pts, sa, sb, g, r, 2)
+                fd = np.sum(grad * (vp - vm)) / (2 * h)
+                assert abs(fd - grad_k[c]) <= 2e-5 * max(abs(fd), abs(grad_k[c])) + 1e-9, (which, k, c, fd,
+                                                                                            grad_k[c])
+    mixed = [v.cpu().numpy() for v in AD.direct_vjp(pts, s, ends, g, r, 2, grad, fused_forward=True,
+                                                     precision="mixed")]
+    for a_, b_ in zip(mixed, (gg, ga, gb, gx, out)):
+        assert np.linalg.norm(a_ - b_) / np.linalg.norm(b_) <= 1e-5
