@@ -170,7 +170,10 @@ int ofw_tree_sum(ofw_ctx *ctx, int precision, int64_t n, const double *points, i
  * bit.  Creation runs the call once (sizing all scratch and checking every capacity) and, when use_graph is
  * set, records its whole launch sequence (build, node quantities, traversal and reduction of every snapshot)
  * in a HIP graph.  The pointers are fixed at creation; ofw_tree_batch_run replays the call on the current
- * contents of the same arrays, on the context's stream, synchronously. */
+ * contents of the same arrays, on the stream the context had at creation, synchronously.  Writes to the arrays
+ * made on another stream must be ordered before the run by the caller (an event the creation stream waits on).
+ * A replay whose inputs need more near-field work-item slots than the plan recorded returns OFW_ERR_OVERFLOW
+ * (the velocities are then not valid); a new plan sizes them again. */
 typedef struct ofw_tree_batch ofw_tree_batch;
 int ofw_tree_batch_create(ofw_ctx *ctx, ofw_tree_batch **plan, int precision, int64_t K,
                           const int64_t *target_offsets_host, const int64_t *segment_offsets_host,

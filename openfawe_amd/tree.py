@@ -154,6 +154,8 @@ class TreeBatch:
         self.near_counts = torch.zeros(n, dtype=torch.int64, device="cuda") if counts else None
         self._lib = _lib.load()
         self._plan = ctypes.c_void_p()
+        # the plan replays on the stream that is current now; run() orders it after the caller's current stream
+        self._stream = torch.cuda.current_stream()
         toff = (ctypes.c_int64 * (self.K + 1))(*self.target_offsets.tolist())
         soff = (ctypes.c_int64 * (self.K + 1))(*self.segment_offsets.tolist())
         status = self._lib.ofw_tree_batch_create(
@@ -165,8 +167,19 @@ class TreeBatch:
         _lib.check(status, "ofw_tree_batch_create")
 
     def run(self):
-        """Replay the call; returns the packed device velocities (n, 3)."""
+        """Replay the call; returns the packed device velocities (n, 3).
+
+        The replay runs on the stream that was current when the batch was made.  When another stream is current
+        now (where the caller may have changed ``arrays``), the replay waits for it, and it waits for the replay.
+        """
+        torch = _torch()
+        current = torch.cuda.current_stream()
+        other = current != self._stream
+        if other:
+            self._stream.wait_stream(current)
         _lib.check(self._lib.ofw_tree_batch_run(self._plan), "ofw_tree_batch_run")
+        if other:
+            current.wait_stream(self._stream)
         return self.out
 
     def split(self, array):
