@@ -91,6 +91,8 @@ int ofw_device_info(char *name_host, int name_len, int *cu_count_host);
 
 /* Direct sum over all segments for every target, in segment order per target, skipping the pairs of the
  * exclusion CSR (ex_ptr (n+1), ex_seg (e), per-target sorted unique segment ids; vortex_numba.py:228-243).
+ * The ids of a target must be sorted and unique, as exclusion_csr makes them: the kernels walk them with a cursor
+ * that advances in segment order, so an unsorted row would exclude other pairs than the reference does.
  * Replaces vortex_numba.direct_sum (vortex_numba.py:246-260) and vortex_cuda.cuda_sum (vortex_cuda.py:381-467).
  *   precision  ofw_precision
  *   block      threads per workgroup (32, 64, 128 or 256; CudaSettings.block, vortex.py:103)

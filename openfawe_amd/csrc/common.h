@@ -70,7 +70,8 @@ struct PairOut {
 
 __device__ __forceinline__ double ofw_norm3(double x, double y, double z) { return hypot(hypot(x, y), z); }
 
-// Operation-by-operation restatement of vortex_numba.py:42-109 (pair_velocity); the FP64 fallback for
+// Operation-by-operation restatement of vortex_numba.py:42-109 (pair_velocity; OpenFAWE, Apache-2.0, see NOTICE);
+// the FP64 fallback for
 // extreme ranges (huge |a|, tiny cores).  Returns the pair velocity and its flag by value (nothing passes
 // through memory on the call).
 __device__ __noinline__ PairOut pair_exact64(double px, double py, double pz, const double* __restrict__ r,

@@ -77,8 +77,14 @@ def hip_tree_sum_device(points, starts, directions, lengths, gamma, radius, mode
 
 
 def hip_tree_sum(points, starts, directions, lengths, gamma, radius, model, csr, singularity, tree, settings, *,
-                 split_depth=-1):
-    """Return (velocity, any_flag) like vortex_tree.tree_sum for validated inputs."""
+                 split_depth=-1, targets_per_launch=None, stack_size=None, block=None):
+    """Return (velocity, any_flag) like vortex_tree.tree_sum for validated inputs.
+
+    ``targets_per_launch``, ``stack_size`` and ``block`` are accepted for the signature of
+    vortex_cuda_tree.cuda_tree_sum (vortex_cuda_tree.py:439-440) and have no effect: all targets go in one launch,
+    the walk is stackless (no traversal stack can overflow) and the workgroup shapes are fixed per kernel.
+    ``split_depth`` is this engine's own knob (the depth of the traversal's subtree frontier, -1: chosen from the
+    target count)."""
     if not isinstance(tree, TreeSettings):
         raise ValueError("tree must be TreeSettings")
     if settings is None:

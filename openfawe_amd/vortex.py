@@ -1,3 +1,7 @@
+# SPDX-License-Identifier: Apache-2.0
+# The settings classes, the input checks of segments_velocity (their order, exception types and messages) and the
+# segment_velocity / ring_velocity helpers follow OpenFAWE's src/openfawe/aero/vortex.py (Copyright 2026 Yinong
+# Tian, Apache License 2.0, itself adapted from OpenFAST/OLAF; see NOTICE): the drop-in boundary keeps them.
 """Directed straight vortex segments on MI355X: the reference's batch API and engine signatures.
 
 Public API (mirrors src/openfawe/aero/vortex.py of the reference):
@@ -244,9 +248,12 @@ def _upload(points, starts, directions, lengths, gamma, radius, csr):
             _dev(torch, segments if len(segments) else np.zeros(1, i64), i64))
 
 
-def hip_sum(points, starts, directions, lengths, gamma, radius, model, csr, singularity, settings):
+def hip_sum(points, starts, directions, lengths, gamma, radius, model, csr, singularity, settings, *,
+            pairs_per_launch=None):
     """Return (velocity, any_flag) for validated inputs, like vortex_cuda.cuda_sum (vortex_cuda.py:381).
 
+    ``pairs_per_launch`` is accepted for the reference's signature (vortex_cuda.py:381-382) and has no effect:
+    one call is one launch of the pair kernel here, whatever the pair count.
     ``csr`` is the (pointer, segments) exclusion structure from ``exclusion_csr``.  Per-target flags
     and the first flagged segment of each target are kept in ``Statistics.last`` for the caller's
     exception precedence.

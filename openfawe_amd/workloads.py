@@ -1,3 +1,7 @@
+# SPDX-License-Identifier: Apache-2.0
+# ``sheet`` reproduces the synthetic sheet of OpenFAWE's apps/studio/openfawe_studio/selftest.py:243-272
+# (Copyright 2026 Yinong Tian, Apache License 2.0; see NOTICE), whose random stream fixes the reference speed
+# table's sizes; the other generators are this package's own.
 """Seeded synthetic wakes: the inputs of the parity tests and of bench.py.
 
 * ``sheet`` -- the reference's synthetic sheet (apps/studio/openfawe_studio/selftest.py:243-272), the

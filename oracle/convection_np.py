@@ -1,3 +1,6 @@
+# SPDX-License-Identifier: Apache-2.0
+# Test oracle: a restatement of OpenFAWE's src/openfawe/aero/wake.py geometry step (Copyright 2026 Yinong Tian, Apache License 2.0, itself
+# adapted from OpenFAST/OLAF; see NOTICE).
 """NumPy restatement of the reference's free-wake geometry step.  TEST INFRASTRUCTURE ONLY.
 
 * ``segments_velocity`` -- vortex.py:458-497 for validated inputs: segment geometry from nodes and edges

@@ -1,3 +1,6 @@
+# SPDX-License-Identifier: Apache-2.0
+# Test oracle: a restatement of OpenFAWE's src/openfawe/aero/network_numba.py and wake.pack_rings (Copyright 2026 Yinong Tian, Apache License 2.0, itself
+# adapted from OpenFAST/OLAF; see NOTICE).
 """NumPy restatement of the reference's incident-exclusion pairs.  TEST INFRASTRUCTURE ONLY.
 
 ``incident_pairs`` (network_numba.py:242-253, with ``_incident_pairs_sorted`` :177-239 and the dictionary

@@ -1,3 +1,6 @@
+/* SPDX-License-Identifier: Apache-2.0
+ * Test oracle: a restatement in C of OpenFAWE's src/openfawe/aero/vortex_numba.py and vortex_tree.py (Copyright
+ * 2026 Yinong Tian, Apache License 2.0, itself adapted from OpenFAST/OLAF; see NOTICE). */
 /*
  * CPU oracle for the free-wake induced-velocity path.  TEST INFRASTRUCTURE ONLY.
  *

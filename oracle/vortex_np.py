@@ -1,3 +1,6 @@
+# SPDX-License-Identifier: Apache-2.0
+# Test oracle: a restatement of OpenFAWE's src/openfawe/aero/vortex.py pair and summation code (Copyright 2026 Yinong Tian, Apache License 2.0, itself
+# adapted from OpenFAST/OLAF; see NOTICE).
 """Vectorised NumPy restatement of the reference's pair kernel.  TEST INFRASTRUCTURE ONLY.
 
 A second, independent CPU formulation of the segment velocity that follows the

@@ -1,3 +1,6 @@
+# SPDX-License-Identifier: Apache-2.0
+# Test oracle: a restatement of OpenFAWE's src/openfawe/aero/vsm_wake.py and cores.py operators (Copyright 2026 Yinong Tian, Apache License 2.0, itself
+# adapted from OpenFAST/OLAF; see NOTICE).
 """NumPy restatement of the reference's VSM wake operators.  TEST INFRASTRUCTURE ONLY.
 
 * ``ring_columns`` -- the definition of vsm_wake._ring_layout (vsm_wake.py:177-185) and of the per-panel path it

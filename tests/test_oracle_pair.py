@@ -10,7 +10,6 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from oracle import vortex_np
 from openfawe_amd import workloads
 from tests.helpers import oracle_direct, prepare, seg1
 
@@ -144,6 +143,7 @@ def test_k08_singular_flags():
 
 def a1_check(points, nodes, edges, gamma, model, radius, exclusions=None):
     """A1: |u_oracle - u_numpy| <= 1e-12 * sum_s |u_s| per point."""
+    vortex_np = pytest.importorskip("oracle.vortex_np")   # CPU-only check: the module stays off the GPU box
     starts, directions, lengths = prepare(nodes, edges)
     m = len(starts)
     gamma = np.broadcast_to(np.asarray(gamma, float), (m,))
