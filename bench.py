@@ -18,6 +18,15 @@ Workloads (--workload):
   direct-fp64-102k  FP64 direct sum over the reference's 468-row sheet (20,570 x 102,382)
   direct-mixed-102k mixed-precision direct sum, same wake
 The headline metric is pair-interactions/s = n * m / (time per call), with ms per call.
+
+Beside the headline workload the default line carries (skip them with --no-extra):
+  legs          tree-mixed-410k (the metric's other size) and tree-fp64-102k (the FP64 tree of 20,570 x 104,430)
+  sheet_table   the reference's speed table (README.md:83-91): the synthetic sheet at 468 / 936 / 1,872 / 3,744 rows,
+                theta 0.2, FP64 and mixed, resident and as the drop-in call on host arrays, the oracle tree on the
+                host's cores, and the accuracy against the CPU tree
+  drop_in_host  the headline call on host NumPy arrays (upload / evaluation / download split)
+  adjoint, batch, network, convection: the differentiation case, the 64-snapshot batch, the wake bookkeeping and the
+                geometry step
 """
 import argparse
 import json
@@ -355,6 +364,152 @@ def batch_bench(steps, split_depth=-1, autotune=True):
                               precision="mixed"), autotune=tuned)
 
 
+def upload_case(case):
+    """The engine inputs of a case as device tensors (float64 / int64) and preallocated outputs."""
+    import torch
+    from openfawe_amd import vortex as V
+    dev = lambda a, dt=torch.float64: torch.from_numpy(np.ascontiguousarray(a)).to("cuda", dt)  # noqa: E731
+    n = len(case["points"])
+    ptr, seg = V.exclusion_csr(n, case["pairs"])
+    return dict(pts=dev(case["points"]), starts=dev(case["starts"]), dirs=dev(case["directions"]),
+                lens=dev(case["lengths"]), gam=dev(case["gamma"]), rad=dev(case["radius"]), ptr=dev(ptr, torch.int64),
+                seg=dev(seg if len(seg) else np.zeros(1, np.int64), torch.int64),
+                out=torch.zeros((n, 3), dtype=torch.float64, device="cuda"),
+                flags=torch.zeros(n, dtype=torch.int8, device="cuda"),
+                far=torch.zeros(n, dtype=torch.int64, device="cuda"),
+                near=torch.zeros(n, dtype=torch.int64, device="cuda"),
+                fseg=torch.zeros(n, dtype=torch.int32, device="cuda"))
+
+
+def timed_calls(step, steps, warmup):
+    """Mean ms per call over `steps` calls after `warmup`, bracketed by device synchronisations, and the mean of
+    the library's phase timings (HIP events on its stream) over the timed calls."""
+    import torch
+    from openfawe_amd import _lib
+    from openfawe_amd import vortex as V
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    phases = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+        phases.append(_lib.timings(V._ctx()))
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    return ms, np.mean(np.array(phases), axis=0).tolist()
+
+
+def tree_step(case, d, spec, split_depth=-1):
+    from openfawe_amd import tree as T
+    from openfawe_amd import vortex as V
+    settings = V.HipSettings(spec["precision"], 256)
+    tree = V.TreeSettings(spec.get("theta", 0.5), 2, 32, 3.0)
+
+    def step():
+        T.hip_tree_sum_device(d["pts"], d["starts"], d["dirs"], d["lens"], d["gam"], d["rad"], case["model"], d["ptr"],
+                              d["seg"], "raise", tree, settings, d["out"], d["flags"], d["far"], d["near"],
+                              split_depth=split_depth)
+    return step
+
+
+def tree_leg(name, steps, warmup):
+    """One more tree workload on the line: time per call (resident inputs), rate, phases and accuracy."""
+    spec = WORKLOADS[name]
+    case = make_case(spec)
+    d = upload_case(case)
+    n, m = len(case["points"]), len(case["starts"])
+    ms, phase_ms = timed_calls(tree_step(case, d, spec), steps, warmup)
+    near_n, far_n = int(d["near"].sum().item()), int(d["far"].sum().item())
+    peak = FP64_PEAK_TFLOPS if spec["precision"] == "fp64" else FP32_PEAK_TFLOPS
+    kernel_tflops = (near_n * OPS_PER_PAIR[case["model"]] + far_n * OPS_PER_FAR) / (phase_ms[5] * 1e-3) / 1e12
+    return dict(workload=name, targets=n, segments=m, precision=spec["precision"], theta=spec["theta"],
+                ms_per_call=ms, value=n * m / (ms * 1e-3), unit="pair-interactions/s",
+                phases_ms=dict(zip(PHASES, phase_ms)), k_tree_near_tflops=kernel_tflops,
+                k_tree_near_frac=kernel_tflops / peak, accuracy=accuracy(case, d["out"], spec))
+
+
+def drop_in_host(case, spec, calls=5):
+    """The drop-in call as a maintainer binds it (host NumPy arrays in and out, hip_tree_sum): median of `calls`
+    calls, with the upload / evaluation / download split (vortex_cuda_tree.py:581-594)."""
+    from openfawe_amd import tree as T
+    from openfawe_amd import vortex as V
+    args = (case["points"], case["starts"], case["directions"], case["lengths"], case["gamma"], case["radius"],
+            case["model"])
+    csr = V.exclusion_csr(len(case["points"]), case["pairs"])
+    tree = V.TreeSettings(spec.get("theta", 0.5), 2, 32, 3.0)
+    settings = V.HipSettings(spec["precision"], 256)
+    T.hip_tree_sum(*args, csr, "raise", tree, settings)
+    rows = []
+    for _ in range(calls):
+        t0 = time.perf_counter()
+        T.hip_tree_sum(*args, csr, "raise", tree, settings)
+        st = V.Statistics.last
+        rows.append((1e3 * (time.perf_counter() - t0), st["upload_ms"], st["evaluate_ms"], st["download_ms"]))
+    med = np.median(np.array(rows), axis=0)
+    mb = sum(np.asarray(case[k]).nbytes for k in ("points", "starts", "directions", "lengths", "gamma", "radius"))
+    return dict(call="hip_tree_sum on host NumPy arrays (exclusion CSR built on the host beforehand)",
+                ms_per_call=float(med[0]), upload_ms=float(med[1]), evaluate_ms=float(med[2]),
+                download_ms=float(med[3]), input_mb=mb / 1e6, calls=calls, statistic="median")
+
+
+def sheet_table(steps, threads, cpu=True):
+    """The reference's speed table (README.md:83-91): the synthetic sheet (selftest.py:243-272) at 468, 936,
+    1,872 and 3,744 rows, 20,570 targets, theta 0.2, order 2, leaf 32, Lamb-Oseen cores; FP64 and mixed on the
+    GPU (resident inputs, and the drop-in call on host arrays), the oracle tree on the host's cores, and the
+    accuracy statements of README.md:93-94 (FP64 against the CPU tree, mixed within a tenth of the tree's own
+    error against the direct sum)."""
+    from oracle import oracle as O
+    from openfawe_amd import autotune as AT
+    from openfawe_amd import vortex as V
+    from openfawe_amd import workloads
+    table = []
+    for rows in (468, 936, 1872, 3744):
+        case = workloads.sheet(rows)
+        n, m = len(case["points"]), len(case["starts"])
+        args = workloads.engine_args(case)
+        row = dict(rows=rows, targets=n, segments=m)
+        csr_o = O.exclusion_csr(n, case["pairs"])
+        if cpu:
+            times = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                ref_tree, _, _, _ = O.tree_sum(*args, csr_o, "raise", 0.2, 2, 32, 3.0, threads=threads)
+                times.append(time.perf_counter() - t0)
+            row["cpu_oracle_tree_ms"] = 1e3 * float(np.median(times))
+            row["cpu_threads"] = threads
+        else:
+            ref_tree, _, _, _ = O.tree_sum(*args, csr_o, "raise", 0.2, 2, 32, 3.0, threads=threads)
+        idx = np.sort(np.random.default_rng(0).choice(n, size=1024, replace=False))
+        direct, _ = V.hip_sum(case["points"][idx], *args[1:], AT._sample_csr(case, idx), "raise",
+                              V.HipSettings("fp64", 256))
+        tree_err = np.linalg.norm(ref_tree[idx] - direct, axis=1).max()
+        d = upload_case(case)
+        for precision in ("fp64", "mixed"):
+            spec = dict(kind="tree", precision=precision, theta=0.2)
+            ms, _ = timed_calls(tree_step(case, d, spec), steps, 2)
+            out = d["out"].cpu().numpy()
+            host = drop_in_host(case, spec)
+            entry = dict(ms_per_call=ms, value=n * m / (ms * 1e-3), drop_in_ms=host["ms_per_call"],
+                         drop_in_split_ms=[host["upload_ms"], host["evaluate_ms"], host["download_ms"]])
+            if precision == "fp64":
+                entry["max_abs_vs_cpu_tree_m_s"] = float(np.abs(out - ref_tree).max())
+            else:
+                entry["max_err_vs_cpu_tree_over_tree_err"] = float(
+                    np.linalg.norm(out[idx] - ref_tree[idx], axis=1).max() / tree_err)
+            row[precision] = entry
+        row["tree_err_vs_direct_max_m_s"] = float(tree_err)
+        table.append(row)
+    return dict(reference="README.md:83-94 (RTX 4090 D, 22 CPU threads: GPU double 0.14-0.53 s, mixed "
+                          "0.07-0.52 s per call, host arrays in and out)", settings=dict(theta=0.2, order=2,
+                                                                                        leaf_size=32),
+                rows=table)
+
+
+PHASES = ["build (midpoints + tree)", "node moments", "traversal plan (exclusions, tasks)", "walk (k_tree_eval)",
+          "list plan", "interaction lists (k_tree_near)", "reduce"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -378,8 +533,6 @@ def main():
         return
 
     import torch
-    from openfawe_amd import _lib
-    from openfawe_amd import tree as T
     from openfawe_amd import vortex as V
 
     spec = dict(WORKLOADS[args.workload])
@@ -390,55 +543,28 @@ def main():
     n, m = len(case["points"]), len(case["starts"])
     settings = V.HipSettings(spec["precision"], 256)
     tree = V.TreeSettings(spec.get("theta", 0.5), 2, 32, 3.0)
-    dev = lambda a, dt=torch.float64: torch.from_numpy(np.ascontiguousarray(a)).to("cuda", dt)  # noqa: E731
-    pts, starts, dirs = dev(case["points"]), dev(case["starts"]), dev(case["directions"])
-    lens, gam, rad = dev(case["lengths"]), dev(case["gamma"]), dev(case["radius"])
-    ptr, seg = V.exclusion_csr(n, case["pairs"])
-    ptr = dev(ptr, torch.int64)
-    seg = dev(seg if len(seg) else np.zeros(1, np.int64), torch.int64)
-    out = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
-    flags = torch.zeros(n, dtype=torch.int8, device="cuda")
-    far = torch.zeros(n, dtype=torch.int64, device="cuda")
-    near = torch.zeros(n, dtype=torch.int64, device="cuda")
-    fseg = torch.zeros(n, dtype=torch.int32, device="cuda")
-
+    d = upload_case(case)
     if spec["kind"] == "tree":
-        def step():
-            T.hip_tree_sum_device(pts, starts, dirs, lens, gam, rad, case["model"], ptr, seg, "raise", tree, settings,
-                                  out, flags, far, near, split_depth=args.split_depth)
+        step = tree_step(case, d, spec, args.split_depth)
     else:
         def step():
-            V.hip_sum_device(pts, starts, dirs, lens, gam, rad, case["model"], ptr, seg, "raise", settings,
-                             out, flags, fseg)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    phases = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-        phases.append(_lib.timings(V._ctx()))
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms = 1e3 * elapsed / args.steps
+            V.hip_sum_device(d["pts"], d["starts"], d["dirs"], d["lens"], d["gam"], d["rad"], case["model"], d["ptr"],
+                             d["seg"], "raise", settings, d["out"], d["flags"], d["fseg"])
+    ms, phase_ms = timed_calls(step, args.steps, args.warmup)
     value = n * m / (ms * 1e-3)
-    phase_ms = np.mean(np.array(phases), axis=0).tolist()
     fp64 = spec["precision"] == "fp64"
     peak = FP64_PEAK_TFLOPS if fp64 else FP32_PEAK_TFLOPS
     if spec["kind"] == "tree":
-        far_n = int(far.sum().item())
-        near_n = int(near.sum().item())
+        far_n = int(d["far"].sum().item())
+        near_n = int(d["near"].sum().item())
         # the roofline kernel: the interaction-list kernel, which evaluates every near pair and every accepted
         # node's far field of the tree code; the walk (acceptance tests only) is reported beside it
         flops = near_n * OPS_PER_PAIR[case["model"]] + far_n * OPS_PER_FAR
         kms = phase_ms[5]
-        names = ["build (midpoints + tree)", "node moments", "traversal plan (exclusions, tasks)",
-                 "walk (k_tree_eval)", "list plan", "interaction lists (k_tree_near)", "reduce"]
         metric = "pair-interactions/s (tree code, direct-sum equivalent n*m/t)"
         kernel = "k_tree_near"
         extra = dict(far_nodes_per_target=far_n / n, near_pairs_per_target=near_n / n,
-                     phases_ms=dict(zip(names, phase_ms)),
+                     phases_ms=dict(zip(PHASES, phase_ms)),
                      walk=dict(kernel="k_tree_eval", kernel_ms=phase_ms[3],
                                note="latency-bound pre-order walk: acceptance tests, list appends"))
     else:
@@ -463,8 +589,12 @@ def main():
         **extra,
     )
     if spec["kind"] == "tree":
-        result["accuracy"] = accuracy(case, out, spec)
+        result["accuracy"] = accuracy(case, d["out"], spec)
+        result["drop_in_host"] = drop_in_host(case, spec)
     if not args.no_extra and spec["kind"] == "tree":
+        # the metric's other size and config 2, then the reference's own speed-table shape
+        result["legs"] = [tree_leg(name, args.steps, args.warmup) for name in ("tree-mixed-410k", "tree-fp64-102k")]
+        result["sheet_table"] = sheet_table(max(3, args.steps // 2), usable_cpus(), cpu=not args.no_cpu)
         result["adjoint"] = adjoint_bench(max(3, args.steps // 2))
         result["batch"] = batch_bench(max(3, args.steps // 2))
         if "nodes" in case:

@@ -6,6 +6,7 @@ node quantities and traversal all run in libofw_hip.so on the GPU.  ``build_tree
 expose the tree and its node quantities for the bit-exactness checks.
 """
 import ctypes
+import time
 
 import numpy as np
 
@@ -100,10 +101,17 @@ def hip_tree_sum(points, starts, directions, lengths, gamma, radius, model, csr,
     if n == 0 or m == 0:
         Statistics.last = dict(points=n, segments=m)
         return np.zeros((n, 3)), False
+    # the upload / evaluation / download split of vortex_cuda_tree.py:581-594 (host clock, device synchronised)
+    t0 = time.perf_counter()
     torch, *dev = _upload(points, starts, directions, lengths, gamma, radius, csr)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
     out, flags, far, near = hip_tree_sum_device(*dev[:6], model, dev[6], dev[7], singularity, tree, settings,
                                                 split_depth=split_depth)
+    t2 = time.perf_counter()
+    host_out = out.cpu().numpy()
     host_flags = flags.cpu().numpy()
+    t3 = time.perf_counter()
     far = far.cpu().numpy()
     stats = (ctypes.c_int64 * 8)()
     _lib.load().ofw_ctx_stats(_ctx(), stats, 8)
@@ -111,8 +119,9 @@ def hip_tree_sum(points, starts, directions, lengths, gamma, radius, model, csr,
                            far_counts=far, near_counts=near.cpu().numpy(), mean_far_nodes=float(far.mean()),
                            nodes=int(stats[4]), depth=int(stats[5]), subtrees=int(stats[2]),
                            split_depth=int(stats[3]), work_items=int(stats[6]), attempts=int(stats[7]),
-                           phases_ms=_lib.timings(_ctx()))
-    return out.cpu().numpy(), bool(np.any(host_flags))
+                           phases_ms=_lib.timings(_ctx()), upload_ms=1e3 * (t1 - t0), evaluate_ms=1e3 * (t2 - t1),
+                           download_ms=1e3 * (t3 - t2))
+    return host_out, bool(np.any(host_flags))
 
 
 class TreeBatch:
