@@ -314,11 +314,21 @@ __device__ __forceinline__ int pair_fast64(double px, double py, double pz, cons
 // pair velocity (zero for a segment without circulation).  The arithmetic runs unconditionally, so that a
 // wavefront evaluating one segment against 64 targets takes no divergent branch except the rare core factor
 // of a target close to the line beyond an end of the segment.
+struct Rec32 {
+  float dx, dy, dz, invL, g4, rc2, exact, pad;
+};
+
+// The 32-byte FP32 record in registers (two 16-byte loads).
+__device__ __forceinline__ Rec32 load_rec32(const float* __restrict__ f) {
+  const float4 a = reinterpret_cast<const float4*>(f)[0], b = reinterpret_cast<const float4*>(f)[1];
+  return Rec32{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+}
+
 template <int MODEL>
-__device__ __forceinline__ bool pair_f32(float rx, float ry, float rz, const float* __restrict__ f,
-                                         float& vx, float& vy, float& vz) {
-  const float invL = f[F_INVL];
-  const float dx = f[F_DX], dy = f[F_DY], dz = f[F_DZ];
+__device__ __forceinline__ bool pair_f32(float rx, float ry, float rz, const Rec32& f, float& vx, float& vy,
+                                         float& vz) {
+  const float invL = f.invL;
+  const float dx = f.dx, dy = f.dy, dz = f.dz;
   const float ax = rx * invL, ay = ry * invL, az = rz * invL;
   const float cx = dy * az - dz * ay, cy = dz * ax - dx * az, cz = dx * ay - dy * ax;
   const float na2 = ax * ax + ay * ay + az * az;
@@ -326,48 +336,60 @@ __device__ __forceinline__ bool pair_f32(float rx, float ry, float rz, const flo
   const float nb2 = bx * bx + by * by + bz * bz;
   const float h2 = cx * cx + cy * cy + cz * cz;
   const float m2 = fmaxf(1.0f, fmaxf(na2, nb2));
-  const float rc2 = f[F_RC2];
+  const float rc2 = f.rc2;
   const float t = (ax * dx + ay * dy) + az * dz;
-  const bool beside = t >= 0.0f && t <= 1.0f;
-  bool near = !(m2 <= OFW_F32_A2_MAX) || h2 <= OFW_NEAR_AXIS2 * m2 || f[F_EXACT] != 0.0f ||
-              (beside && h2 <= OFW_NEAR_BESIDE2 * m2);
+  // every test evaluated (bitwise combination, no short-circuit): no branch per pair
+  const bool beside = (t >= 0.0f) & (t <= 1.0f);
+  bool near = (!(m2 <= OFW_F32_A2_MAX)) | (h2 <= OFW_NEAR_AXIS2 * m2) | (f.exact != 0.0f) |
+              (beside & (h2 <= OFW_NEAR_BESIDE2 * m2));
   if (MODEL != OFW_MODEL_NONE) {
     // squared distance to the segment itself (in segment lengths)
     const float dseg2 = beside ? h2 : (t < 0.0f ? na2 : nb2);
-    near = near || dseg2 < OFW_NEAR_CORES2 * rc2;
+    near = near | (dseg2 < OFW_NEAR_CORES2 * rc2);
   }
-  const float g4 = f[F_G4];
+  const float g4 = f.g4;
   float sa = __builtin_amdgcn_sqrtf(na2), sb = __builtin_amdgcn_sqrtf(nb2);
   float product = sa * sb;
   float dot = h2 + t * (t - 1.0f);
-  float plus = dot < 0.0f ? h2 * __builtin_amdgcn_rcpf(product - dot) : product + dot;
-  float den = product * plus;
   float factor = 1.0f;
-  if (MODEL == OFW_MODEL_RANKINE || MODEL == OFW_MODEL_LAMB) {
-    // core factors of q = (h / rc)^2 (vortex_numba.py:79-101): an FP32 pair is more than 4 core radii from
-    // the segment but may be closer than that to its line, beyond one of its ends
-    const bool core = !near && rc2 > 0.0f && h2 < OFW_NEAR_CORES2 * rc2;
-    if (__builtin_amdgcn_ballot_w64(core) != 0 && core) {
-      const float q = h2 * __builtin_amdgcn_rcpf(rc2);
-      if (MODEL == OFW_MODEL_RANKINE) factor = fminf(q, 1.0f);
-      else if (q <= (float)(10.0 / OFW_LAMB)) factor = -expm1f(-(float)OFW_LAMB * q);
+  float coef;
+  if (MODEL == OFW_MODEL_OFFSET || MODEL == OFW_MODEL_RM) {
+    float plus = dot < 0.0f ? h2 * __builtin_amdgcn_rcpf(product - dot) : product + dot;
+    float den = product * plus;
+    if (MODEL == OFW_MODEL_OFFSET) {
+      den = den + rc2;
+    } else if (rc2 > 0.0f) {
+      sa = __builtin_amdgcn_sqrtf(na2 + rc2);
+      sb = __builtin_amdgcn_sqrtf(nb2 + rc2);
+      product = sa * sb;
+      const float tr2 = h2 + rc2;
+      dot = tr2 + t * (t - 1.0f);
+      plus = dot < 0.0f ? tr2 * __builtin_amdgcn_rcpf(product - dot) : product + dot;
+      den = product * plus;
     }
-  } else if (MODEL == OFW_MODEL_VATISTAS && rc2 > 0.0f) {
-    // q / sqrt(1 + q^2) = 1 / sqrt(1 + q^-2) for every q > 0
-    const float iq = rc2 * __builtin_amdgcn_rcpf(h2);
-    factor = __builtin_amdgcn_rsqf(1.0f + iq * iq);
-  } else if (MODEL == OFW_MODEL_OFFSET) {
-    den = den + rc2;
-  } else if (MODEL == OFW_MODEL_RM && rc2 > 0.0f) {
-    sa = __builtin_amdgcn_sqrtf(na2 + rc2);
-    sb = __builtin_amdgcn_sqrtf(nb2 + rc2);
-    product = sa * sb;
-    const float tr2 = h2 + rc2;
-    dot = tr2 + t * (t - 1.0f);
-    plus = dot < 0.0f ? tr2 * __builtin_amdgcn_rcpf(product - dot) : product + dot;
-    den = product * plus;
+    coef = g4 * (sa + sb) * __builtin_amdgcn_rcpf(den);
+  } else {
+    if (MODEL == OFW_MODEL_RANKINE || MODEL == OFW_MODEL_LAMB) {
+      // core factors of q = (h / rc)^2 (vortex_numba.py:79-101): an FP32 pair is more than 4 core radii from
+      // the segment but may be closer than that to its line, beyond one of its ends
+      const bool core = !near && rc2 > 0.0f && h2 < OFW_NEAR_CORES2 * rc2;
+      if (__builtin_amdgcn_ballot_w64(core) != 0 && core) {
+        const float q = h2 * __builtin_amdgcn_rcpf(rc2);
+        if (MODEL == OFW_MODEL_RANKINE) factor = fminf(q, 1.0f);
+        else if (q <= (float)(10.0 / OFW_LAMB)) factor = -expm1f(-(float)OFW_LAMB * q);
+      }
+    } else if (MODEL == OFW_MODEL_VATISTAS && rc2 > 0.0f) {
+      // q / sqrt(1 + q^2) = 1 / sqrt(1 + q^-2) for every q > 0
+      const float iq = rc2 * __builtin_amdgcn_rcpf(h2);
+      factor = __builtin_amdgcn_rsqf(1.0f + iq * iq);
+    }
+    // den = product * plus, plus = h^2 / (product - dot) where dot < 0 (the rationalised form): one reciprocal,
+    // of product * h^2 there and of product * (product + dot) elsewhere
+    const bool rat = dot < 0.0f;
+    const float dd = rat ? product * h2 : product * (product + dot);
+    const float xm = rat ? product - dot : 1.0f;
+    coef = g4 * factor * (sa + sb) * xm * __builtin_amdgcn_rcpf(dd);
   }
-  const float coef = g4 * factor * (sa + sb) * __builtin_amdgcn_rcpf(den);
   vx = coef * cx;
   vy = coef * cy;
   vz = coef * cz;

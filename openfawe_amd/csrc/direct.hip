@@ -193,7 +193,7 @@ __global__ __launch_bounds__(256) void k_direct_mixed(int n, const double* __res
     }
     const float rx = (float)(px - r[R_SX]), ry = (float)(py - r[R_SY]), rz = (float)(pz - r[R_SZ]);
     float vx = 0.0f, vy = 0.0f, vz = 0.0f;
-    if (pair_f32<MODEL>(rx, ry, rz, f, vx, vy, vz)) {
+    if (pair_f32<MODEL>(rx, ry, rz, load_rec32(f), vx, vy, vz)) {
       double wx = 0.0, wy = 0.0, wz = 0.0;
       const int fl = pair_fast64<MODEL>(px, py, pz, r, raise_singular, wx, wy, wz);
       note_flag(st, fl, s);

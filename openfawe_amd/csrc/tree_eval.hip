@@ -616,25 +616,34 @@ __global__ __launch_bounds__(256, 4) void k_tree_near(ForestTables ft, const dou
           wave_lds_sync();
         }
         const int stop = min(hi, sbase + 64);
+        // the window's pairs summed in FP32 and added to the target's sum once; excluded segments are looked
+        // for pair by pair only in a window where an open lane has one
+        const bool any_ex = wave_any(open && nextex < stop);
+        float lx = 0.0f, ly = 0.0f, lz = 0.0f;
+        int hits = 0;
         for (int s = base; s < stop; ++s) {
+          // the pair's record and start: the same LDS address on every lane (a broadcast read)
           const int tl = s - sbase;
+          const Rec32 fc = load_rec32(&s_seg[wv][tl][0]);
+          const double cx0 = s_start[wv][tl][0], cy0 = s_start[wv][tl][1], cz0 = s_start[wv][tl][2];
           // every lane evaluates the FP32 pair; the lanes that use it select the result, and only the rare
           // excluded segment and the FP64 pairs branch (on the wavefront's ballot)
-          const bool hit = open && nextex == s;
-          if (wave_any(hit)) {
-            if (hit) advance_to(s + 1);
+          bool use = open;
+          if (any_ex) {
+            const bool hit = open && nextex == s;
+            if (wave_any(hit)) {
+              if (hit) advance_to(s + 1);
+            }
+            use = open && !hit;
+            hits += hit ? 1 : 0;
           }
-          const bool use = open && !hit;
-          // the segment's record and start: the same LDS address on every lane (a broadcast read)
-          const float* f = &s_seg[wv][tl][0];
-          const double sx = x0 - s_start[wv][tl][0], sy = x1 - s_start[wv][tl][1], sz = x2 - s_start[wv][tl][2];
+          const double sx = x0 - cx0, sy = x1 - cy0, sz = x2 - cz0;
           float vx, vy, vz;
-          const bool need64 = pair_f32<MODEL>((float)sx, (float)sy, (float)sz, f, vx, vy, vz);
+          const bool need64 = pair_f32<MODEL>((float)sx, (float)sy, (float)sz, fc, vx, vy, vz);
           const bool add = use && !need64;
-          ux += (acc_t)(add ? vx : 0.0f);
-          uy += (acc_t)(add ? vy : 0.0f);
-          uz += (acc_t)(add ? vz : 0.0f);
-          near += use ? 1 : 0;
+          lx += add ? vx : 0.0f;
+          ly += add ? vy : 0.0f;
+          lz += add ? vz : 0.0f;
           const bool slow = use && need64;
           if (wave_any(slow)) {
             if (slow) {
@@ -650,6 +659,10 @@ __global__ __launch_bounds__(256, 4) void k_tree_near(ForestTables ft, const dou
             }
           }
         }
+        ux += (acc_t)lx;
+        uy += (acc_t)ly;
+        uz += (acc_t)lz;
+        near += open ? (stop - base) - hits : 0;
         base = stop;
       }
     }
