@@ -219,9 +219,13 @@ __device__ __forceinline__ int find_tree(const int32_t* __restrict__ table, int 
 static constexpr int WIN = 32;
 static_assert(WIN * ACC_REC % 128 == 0, "the node window is loaded in 16-byte chunks by 64 lanes");
 static constexpr int EVAL_WAVES = 4;
+// The walk and the list kernel run one wavefront per workgroup: their work items differ widely in cost, and a
+// workgroup keeps its slot (and its LDS) until its slowest wavefront ends, so single-wavefront workgroups let the
+// dispatcher start the next item wherever one has finished.
+static constexpr int NEAR_WAVES = 1;
 
 template <int MODEL, int PREC>
-__global__ __launch_bounds__(256) void k_tree_eval(ForestTables ft, const double* __restrict__ pts,
+__global__ __launch_bounds__(64 * NEAR_WAVES) void k_tree_eval(ForestTables ft, const double* __restrict__ pts,
                                                    const int32_t* __restrict__ ex_ptr,
                                                    const int32_t* __restrict__ ex_pos,
                                                    const EvalTask* __restrict__ tasks,
@@ -233,9 +237,9 @@ __global__ __launch_bounds__(256) void k_tree_eval(ForestTables ft, const double
                                                    int32_t* __restrict__ pnear, int8_t* __restrict__ pnz,
                                                    NearEntry* __restrict__ list, int32_t* __restrict__ list_count) {
   typedef typename std::conditional<PREC == OFW_FP32, float, double>::type acc_t;
-  __shared__ double s_acc[EVAL_WAVES][WIN * ACC_REC];
+  __shared__ double s_acc[NEAR_WAVES][WIN * ACC_REC];
   const int wv = threadIdx.x >> 6;
-  const int wave = uniform(xcd_block(blockIdx.x, gridDim.x) * EVAL_WAVES + wv);
+  const int wave = uniform(xcd_block(blockIdx.x, gridDim.x) * NEAR_WAVES + wv);
   // whole wavefront beyond the work, or a plan that overflowed its capacity (reported by the host)
   if (wave >= ft.counters[C_WORK] || ft.counters[C_OVERFLOW]) return;
   const int lane = threadIdx.x & 63;
@@ -455,7 +459,7 @@ __global__ __launch_bounds__(256) void k_tree_eval(ForestTables ft, const double
 // (vortex_tree.py:538-557 for each of them).  A grid-stride
 // loop covers the work items, whose count is known on the device only.
 template <int MODEL, int PREC>
-__global__ __launch_bounds__(256, 4) void k_tree_near(ForestTables ft, const double* __restrict__ pts,
+__global__ __launch_bounds__(64 * NEAR_WAVES) void k_tree_near(ForestTables ft, const double* __restrict__ pts,
                                                    const int32_t* __restrict__ ex_ptr,
                                                    const int32_t* __restrict__ ex_pos,
                                                    const double* __restrict__ rec64, const float* __restrict__ rec32,
@@ -470,19 +474,19 @@ __global__ __launch_bounds__(256, 4) void k_tree_near(ForestTables ft, const dou
   typedef typename std::conditional<PREC == OFW_FP32, float, double>::type acc_t;
   typedef typename std::conditional<PREC == OFW_FP64, double, float>::type far_t;
   // far-field records (and centres) of the accepted nodes of one work item, per wavefront
-  __shared__ far_t s_far[EVAL_WAVES][CHUNK][FAR_REC];
-  __shared__ double s_cen[EVAL_WAVES][CHUNK][4];
+  __shared__ far_t s_far[NEAR_WAVES][CHUNK][FAR_REC];
+  __shared__ double s_cen[NEAR_WAVES][CHUNK][4];
   // below FP64, a window of 64 consecutive segments: FP32 records and FP64 starts
   constexpr int SEGW = PREC == OFW_FP64 ? 1 : 64;
-  __shared__ float s_seg[EVAL_WAVES][SEGW][REC32];
-  __shared__ double s_start[EVAL_WAVES][SEGW][4];
+  __shared__ float s_seg[NEAR_WAVES][SEGW][REC32];
+  __shared__ double s_start[NEAR_WAVES][SEGW][4];
   const far_t* __restrict__ frec = PREC == OFW_FP64 ? (const far_t*)fr64 : (const far_t*)fr32;
   const int wv = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int works = min(live_count(ft.counters, C_CHUNKS), chunk_slots);
   const int nwaves = live_count(ft.counters, C_WORK);
-  const int stride = gridDim.x * EVAL_WAVES;
-  for (int item = uniform(blockIdx.x * EVAL_WAVES + (int)(threadIdx.x >> 6)); item < works; item += stride) {
+  const int stride = gridDim.x * NEAR_WAVES;
+  for (int item = uniform(blockIdx.x * NEAR_WAVES + (int)(threadIdx.x >> 6)); item < works; item += stride) {
     // the walk wavefront of this item
     const int w = uniform(item_wave[item]);
     const int c = item - uniform(chunk_off[w]);
@@ -920,15 +924,15 @@ static void launch_eval(int precision, dim3 grid, hipStream_t st, const ForestTa
                         double theta, int order, int raise, double* part, int8_t* pflag, int32_t* pfar,
                         int32_t* pnear, int8_t* pnz, NearEntry* list, int32_t* list_count) {
   if (precision == OFW_FP64)
-    hipLaunchKernelGGL((k_tree_eval<MODEL, OFW_FP64>), grid, dim3(256), 0, st, ft, pts, ex_ptr, ex_pos, tasks, td.acc,
+    hipLaunchKernelGGL((k_tree_eval<MODEL, OFW_FP64>), grid, dim3(64 * NEAR_WAVES), 0, st, ft, pts, ex_ptr, ex_pos, tasks, td.acc,
                        td.far64, td.far32, td.rec64, td.rec32, theta, order, raise, part, pflag, pfar, pnear, pnz,
                        list, list_count);
   else if (precision == OFW_MIXED)
-    hipLaunchKernelGGL((k_tree_eval<MODEL, OFW_MIXED>), grid, dim3(256), 0, st, ft, pts, ex_ptr, ex_pos, tasks,
+    hipLaunchKernelGGL((k_tree_eval<MODEL, OFW_MIXED>), grid, dim3(64 * NEAR_WAVES), 0, st, ft, pts, ex_ptr, ex_pos, tasks,
                        td.acc, td.far64, td.far32, td.rec64, td.rec32, theta, order, raise, part, pflag, pfar, pnear,
                        pnz, list, list_count);
   else
-    hipLaunchKernelGGL((k_tree_eval<MODEL, OFW_FP32>), grid, dim3(256), 0, st, ft, pts, ex_ptr, ex_pos, tasks, td.acc,
+    hipLaunchKernelGGL((k_tree_eval<MODEL, OFW_FP32>), grid, dim3(64 * NEAR_WAVES), 0, st, ft, pts, ex_ptr, ex_pos, tasks, td.acc,
                        td.far64, td.far32, td.rec64, td.rec32, theta, order, raise, part, pflag, pfar, pnear, pnz,
                        list, list_count);
 }
@@ -939,15 +943,15 @@ static void launch_near(int precision, dim3 grid, hipStream_t st, const ForestTa
                         const NearEntry* list, const int32_t* list_count, const int32_t* chunk_off,
                         const int32_t* item_wave, int slots, double* cpart, int8_t* cflag, int32_t* cnear) {
   if (precision == OFW_FP64)
-    hipLaunchKernelGGL((k_tree_near<MODEL, OFW_FP64>), grid, dim3(256), 0, st, ft, pts, ex_ptr, ex_pos, td.rec64,
+    hipLaunchKernelGGL((k_tree_near<MODEL, OFW_FP64>), grid, dim3(64 * NEAR_WAVES), 0, st, ft, pts, ex_ptr, ex_pos, td.rec64,
                        td.rec32, td.acc, td.far64, td.far32, order, raise, list, list_count, chunk_off, item_wave, slots, cpart,
                        cflag, cnear);
   else if (precision == OFW_MIXED)
-    hipLaunchKernelGGL((k_tree_near<MODEL, OFW_MIXED>), grid, dim3(256), 0, st, ft, pts, ex_ptr, ex_pos, td.rec64,
+    hipLaunchKernelGGL((k_tree_near<MODEL, OFW_MIXED>), grid, dim3(64 * NEAR_WAVES), 0, st, ft, pts, ex_ptr, ex_pos, td.rec64,
                        td.rec32, td.acc, td.far64, td.far32, order, raise, list, list_count, chunk_off, item_wave, slots, cpart,
                        cflag, cnear);
   else
-    hipLaunchKernelGGL((k_tree_near<MODEL, OFW_FP32>), grid, dim3(256), 0, st, ft, pts, ex_ptr, ex_pos, td.rec64,
+    hipLaunchKernelGGL((k_tree_near<MODEL, OFW_FP32>), grid, dim3(64 * NEAR_WAVES), 0, st, ft, pts, ex_ptr, ex_pos, td.rec64,
                        td.rec32, td.acc, td.far64, td.far32, order, raise, list, list_count, chunk_off, item_wave, slots, cpart,
                        cflag, cnear);
 }
@@ -1068,7 +1072,7 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
   OFW_HIP(c.mark());
   ForestTables ft{K, td.count, (int)m, toff, task_first, wave_off, part_off, d_list_base, d_list_cap, t.counters};
   if (wave_cap > 0) {
-    dim3 egrid((unsigned)((wave_cap + 3) / 4));
+    dim3 egrid((unsigned)((wave_cap + NEAR_WAVES - 1) / NEAR_WAVES));
     switch (model) {
       case 0: launch_eval<0>(precision, egrid, c.stream, ft, points, ex_ptr_used, ex_pos, tasks, td, theta, expansion_order, raise_singular, part, pflag, pfar, pnear, pnz, list, list_count); break;
       case 1: launch_eval<1>(precision, egrid, c.stream, ft, points, ex_ptr_used, ex_pos, tasks, td, theta, expansion_order, raise_singular, part, pflag, pfar, pnear, pnz, list, list_count); break;
@@ -1105,7 +1109,8 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
     hipLaunchKernelGGL(k_item_waves, dim3((unsigned)((W1 + 255) / 256)), dim3(256), 0, c.stream, t.counters, chunk_off,
                        (int)CH, (int)std::min<int64_t>(slots, INT_MAX), item_wave);
     OFW_HIP(c.mark());
-    const dim3 ngrid((unsigned)std::max<int64_t>(1, std::min<int64_t>((int64_t)c.cu_count * 16, (CH + 3) / 4)));
+    // one wavefront per work item: the dispatcher starts the next item wherever a wavefront has finished
+    const dim3 ngrid((unsigned)std::max<int64_t>(1, ((int64_t)CH + NEAR_WAVES - 1) / NEAR_WAVES));
     switch (model) {
       case 0: launch_near<0>(precision, ngrid, c.stream, ft, points, ex_ptr_used, ex_pos, td, expansion_order, raise_singular, list, list_count, chunk_off, item_wave, (int)CH, cpart, cflag, cnear); break;
       case 1: launch_near<1>(precision, ngrid, c.stream, ft, points, ex_ptr_used, ex_pos, td, expansion_order, raise_singular, list, list_count, chunk_off, item_wave, (int)CH, cpart, cflag, cnear); break;
