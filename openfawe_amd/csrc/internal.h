@@ -33,6 +33,7 @@ struct Ctx {
   int64_t stats_i[8] = {};
   int64_t list_slots = 0;    // interaction-list work-item slots of the next tree call (0: measure)
   bool capturing = false;    // enqueuing into a graph capture: no events are recorded
+  bool sort_build = false;   // build trees with the per-level sorts (set when the list build met equal coordinates)
   void* static_scratch = nullptr;  // this device's copy of the no-allocation scratch (resolved address)
   hipError_t mark() { return (!capturing && nev < 12) ? hipEventRecord(ev[nev++], stream) : hipSuccess; }
 };

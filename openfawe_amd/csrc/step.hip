@@ -414,6 +414,11 @@ int ofw_step_create(ofw_ctx* x, ofw_step** plan, int precision, int64_t first, c
     if (st) return fail(st);
     st = ofw::read_block(*p, host);
     if (st) return fail(st);
+    if ((host[0] & OFW_STEP_TREE_OVERFLOW) && !c.sort_build && attempt < 3) {
+      // the list build met equal midpoint coordinates (the only tree overflow of a measuring run): per-level sorts
+      c.sort_build = true;
+      continue;
+    }
     if (!(host[0] & OFW_STEP_PAIR_OVERFLOW) || attempt >= 3) break;
     p->pair_cap = (int64_t)host[1] + (int64_t)host[1] / 4 + 1024;
   }

@@ -12,7 +12,7 @@ namespace ofw {
 enum {
   C_NODES = 0,      // nodes created (global ids are dense from 0; some heap slots may stay unused)
   C_LEVEL,          // roots of the forest (the first top level)
-  C_NEXT,           // unused
+  C_DUP,            // the list build found a run of equal coordinates too long to re-order: sort build instead
   C_TILES,          // tiles of the current level
   C_BIG,            // nodes of the current level sorted by tiles
   C_BOTTOM,         // subtree roots of the bottom phase
@@ -26,7 +26,9 @@ enum {
   C_CHUNKS,         // near-field work items of the traversal
   C_SUBTREES,       // subtrees of the moment pass
   C_STRAND,         // top-phase nodes whose segments stay in the second order buffer
-  C_COUNT = 16
+  C_RUNS,           // runs of equal coordinates in the roots' lists (list build)
+  C_RUNMEM,         // their members
+  C_COUNT = 18
 };
 
 // A device count, or 0 once a capacity has overflowed: every kernel after an overflow does no work, and the
@@ -61,6 +63,23 @@ struct TreeArrays {
   int32_t *p_lo, *p_hi, *p_skip, *p_parent, *p_depth, *p_tree, *p_node, *p_irank;
   int32_t* tree_first;           // (K+1) first pre-order index of each tree
   int32_t* order = nullptr;      // segment permutation (tree order -> segment)
+  // list build: per-axis segment lists presorted once, split by stable partitions level after level
+  int lists = 0;                 // 1: the top phase uses the lists (every node's segments stay in `order`)
+  int rs_passes = 0;             // radix passes of the presort (8 key digits + the tree-id digits)
+  int rs_tiles = 0;              // radix tiles per axis
+  uint64_t* rkey[2] = {};        // 3 x m keys per buffer (axis-major)
+  int32_t* rval[2] = {};         // 3 x m segment ids per buffer; the lists live here after the presort
+  uint8_t* side = nullptr;       // per segment: 1 when it goes to the left child of its node
+  int32_t* tcount = nullptr;     // tile_cap x 3 left counts of the partition
+  int32_t* rcount = nullptr;     // 3 x 256 x rs_tiles digit counts of one presort pass, then their scans
+  int32_t* rtotal = nullptr;     // 3 x 256 digit totals of the pass
+  // runs of equal coordinates (at most 3m members in 3m/2 runs)
+  int32_t* run_seg = nullptr;    // members of every run, run after run
+  int32_t* run_off = nullptr;    // first member of each run
+  int32_t* run_len = nullptr;    // length << 2 | axis
+  uint8_t* tie = nullptr;        // per segment: 1 when it is in a run of any axis
+  int32_t* inv_o = nullptr;      // per run segment: its position in `order`
+  int32_t* invb = nullptr;       // 3 x m, per run segment: its position in each list
 };
 
 // Copy host int64 values (each below 2^31) to a device int32 array through kernel arguments: no host

@@ -105,6 +105,10 @@ int tree_pipeline(Ctx& c, const TreeCall& tc, TreeArrays& t) {
 
 // The counters of a finished call: an error when a capacity was exceeded, else the statistics.
 static int check_counters(Ctx& c, const int32_t* counters) {
+  if (counters[C_DUP]) {
+    set_error("tree code: the recorded list build met equal midpoint coordinates (make the plan again)");
+    return OFW_ERR_OVERFLOW;
+  }
   if (counters[C_OVERFLOW]) {
     set_error("tree code exceeded a capacity of its build or traversal plan");
     return OFW_ERR_OVERFLOW;
@@ -120,6 +124,28 @@ static int read_counters(Ctx& c, const TreeArrays& t, int32_t* counters) {
   OFW_HIP(hipMemcpyAsync(counters, t.counters, C_COUNT * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
   OFW_HIP(hipStreamSynchronize(c.stream));
   return check_counters(c, counters);
+}
+
+// A build for the bit-exactness entry points: the list build, made again with the per-level sorts when the
+// midpoints have equal coordinates (one read-back of the counters).
+static int build_once(Ctx& c, TreeArrays& t, int64_t m, int leaf, const double* mx, const double* my,
+                      const double* mz, const std::vector<int64_t>& offsets) {
+  const bool saved = c.sort_build;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    int st = tree_layout(c, 0, m, 1, leaf, t);
+    if (st) return st;
+    st = upload_i32(c, offsets, t.roots_lo);
+    if (st) return st;
+    st = tree_build(c, t, mx, my, mz, offsets);
+    if (st) return st;
+    int32_t dup = 0;
+    OFW_HIP(hipMemcpyAsync(&dup, t.counters + C_DUP, sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+    OFW_HIP(hipStreamSynchronize(c.stream));
+    if (!dup) break;
+    c.sort_build = true;
+  }
+  c.sort_build = saved;
+  return OFW_OK;
 }
 
 static int check_tree_args(const char* what, int precision, int64_t n, int64_t m, int64_t leaf_size, double theta,
@@ -189,11 +215,7 @@ int ofw_build_tree(ofw_ctx* x, int64_t m, const double* mid, int64_t leaf_size, 
                      mz);
   ofw::TreeArrays t;
   const std::vector<int64_t> offsets = {0, m};
-  int st = ofw::tree_layout(c, 0, m, 1, (int)leaf_size, t);
-  if (st) return st;
-  st = ofw::upload_i32(c, offsets, t.roots_lo);
-  if (st) return st;
-  st = ofw::tree_build(c, t, mx, my, mz, offsets);
+  int st = ofw::build_once(c, t, m, (int)leaf_size, mx, my, mz, offsets);
   if (st) return st;
   hipLaunchKernelGGL(ofw::k_i32_to_i64, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, c.stream, (int)m, t.order,
                      order);
@@ -227,11 +249,7 @@ int ofw_tree_nodes(ofw_ctx* x, int64_t m, const double* starts, const double* di
   if (st) return st;
   ofw::TreeArrays t;
   const std::vector<int64_t> offsets = {0, m};
-  st = ofw::tree_layout(c, 0, m, 1, (int)leaf_size, t);
-  if (st) return st;
-  st = ofw::upload_i32(c, offsets, t.roots_lo);
-  if (st) return st;
-  st = ofw::tree_build(c, t, mx, my, mz, offsets);
+  st = ofw::build_once(c, t, m, (int)leaf_size, mx, my, mz, offsets);
   if (st) return st;
   std::vector<int64_t> vlo, vhi, vleft, vright;
   std::vector<int32_t> ref_of_pre;
@@ -334,21 +352,33 @@ int ofw_tree_sum(ofw_ctx* x, int precision, int64_t n, const double* points, int
   tc.near_counts = near_counts;
   // the interaction-list slots of the previous call, with margin, spare this call's read-back of the count;
   // a call that needs more runs again with the count it measured
+  // a call whose midpoints have equal coordinates runs again with the per-level sort build
   ofw::TreeArrays t;
   int32_t counters[ofw::C_COUNT];
+  const bool saved_sort = c.sort_build;
+  bool retried_slots = false;
   for (int attempt = 0;; ++attempt) {
     c.nev = 0;  // the phase timings describe the last attempt
     tc.chunk_slots = c.list_slots > 0 ? c.list_slots : -1;
     st = ofw::tree_pipeline(c, tc, t);
-    if (st) return st;
+    if (st) {
+      c.sort_build = saved_sort;
+      return st;
+    }
     OFW_HIP(hipMemcpyAsync(counters, t.counters, sizeof(counters), hipMemcpyDeviceToHost, c.stream));
     OFW_HIP(hipStreamSynchronize(c.stream));
+    c.stats_i[7] = attempt + 1;  // runs of the pipeline this call took
+    if (counters[ofw::C_DUP] && !c.sort_build) {
+      c.sort_build = true;
+      continue;
+    }
     const int64_t used = counters[ofw::C_CHUNKS];
     const bool short_slots = counters[ofw::C_OVERFLOW] && tc.chunk_slots >= 0 && used > tc.chunk_slots;
     c.list_slots = used + used / 4 + 1024;
-    c.stats_i[7] = attempt + 1;  // runs of the pipeline this call took
-    if (!short_slots || attempt > 0) break;
+    if (!short_slots || retried_slots) break;
+    retried_slots = true;
   }
+  c.sort_build = saved_sort;
   c.stats_i[6] = counters[ofw::C_CHUNKS];
   return ofw::check_counters(c, counters);
 }
@@ -436,6 +466,13 @@ int ofw_tree_batch_create(ofw_ctx* x, ofw_tree_batch** plan, int precision, int6
   if (st) return fail(st);
   int32_t counters[ofw::C_COUNT];
   st = ofw::read_counters(c, b->t, counters);
+  if (counters[ofw::C_DUP]) {  // equal midpoint coordinates: the plan builds with the per-level sorts
+    c.sort_build = true;
+    c.nev = 0;
+    st = ofw::tree_pipeline(c, tc, b->t);
+    if (st) return fail(st);
+    st = ofw::read_counters(c, b->t, counters);
+  }
   if (st) return fail(st);
   const int64_t measured = c.stats_i[6];
   tc.chunk_slots = measured + measured / 2 + 1024;

@@ -81,6 +81,12 @@ __global__ void k_init_roots(TreeArrays t) {
   }
 }
 
+template <typename T>
+__global__ void k_fill_n(int64_t count, T* __restrict__ a, T v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) a[i] = v;
+}
+
 __global__ void k_fill_i32(int count, int32_t* __restrict__ a, int32_t v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) a[i] = v;
@@ -746,6 +752,435 @@ __global__ __launch_bounds__(256) void k_copy_strands(TreeArrays t) {
     for (int p = t.lo[node] + (int)threadIdx.x; p < t.hi[node]; p += blockDim.x) t.order[p] = t.tmp[p];
   }
 }
+// ------------------------------------------------------------------ list build (top phase)
+// A stable sort of a node by one axis (vortex_tree.py:95-101) orders its segments by (coordinate, position in
+// the node's order).  The top phase keeps, for every big node and axis, the node's segments in that order (the
+// node's list of the axis), so that no level sorts anything:
+//  * the node's box is the first and last entry of its lists (exact min / max), which give the axis and flag;
+//  * its sorted order is its list of that axis: the order entries and each segment's side ((a+b)//2 cut) are
+//    written;
+//  * each list is partitioned, stably, into the children's ranges [a, mid) and [mid, b);
+//  * a partitioned list is ordered by (coordinate, position in the parent's order), while the child's order is
+//    the parent's list of the split axis: in a run of equal coordinates the entries are put in the order of
+//    their new positions in `order`.  The segments of such runs carry TIE_FLAG in the lists, and their positions
+//    in `order` (inv_o) and in each list (invb) are kept per segment.
+// The roots' lists are each tree's segments sorted by (coordinate, index), its order at the root: an LSD radix
+// sort of (tree id, 64-bit key) by 8-bit digits, three kernels per digit (the digit counts of every tile, their
+// scan, the stable scatter of every tile).  Integer counts only: the result is the same for every schedule.  A
+// run of equal coordinates longer than RUN_MAX sets C_DUP (and the overflow flag, so that nothing runs after it):
+// the call is then made again with the per-level sorts.
+static constexpr int RS_THREADS = 256;
+static constexpr int RS_E = 8;                       // elements per thread
+static constexpr int RS_TILE = RS_THREADS * RS_E;    // elements per radix tile
+static constexpr int RS_WAVE = RS_TILE / (RS_THREADS / 64);  // elements per wavefront of a tile
+static constexpr int RS_BINS = 256;
+static constexpr int RS_KEY_PASSES = 8;
+static constexpr int RS_MAX_PASSES = RS_KEY_PASSES + 3;
+static constexpr int RUN_MAX = 1024;                 // longest run of equal coordinates the lists re-order
+static constexpr int FIX_WAVES = 4;                  // wavefronts (one run each) per workgroup of the re-order
+static constexpr int32_t TIE_FLAG = INT32_MIN;       // list entry of a segment in a run of equal coordinates
+static constexpr int32_t SEG_MASK = INT32_MAX;
+static_assert(RS_WAVE % 64 == 0, "a wavefront ranks whole rounds of 64 elements");
+static_assert(RS_THREADS == RS_BINS, "one thread per digit");
+
+// tree of segment position s: the last k with roots_lo[k] <= s
+__device__ __forceinline__ int tree_of(const int32_t* __restrict__ roots_lo, int K, int s) {
+  int lo = 0, hi = K;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (roots_lo[mid] <= s) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ int rs_digit(uint64_t key, int tree, int pass) {
+  return pass < RS_KEY_PASSES ? (int)((key >> (8 * pass)) & 0xff) : ((tree >> (8 * (pass - RS_KEY_PASSES))) & 0xff);
+}
+
+// Keys and segment ids of the three axes (buffer 0); no segment is in a run yet.
+__global__ void k_rs_init(TreeArrays t, const double* __restrict__ mx, const double* __restrict__ my,
+                          const double* __restrict__ mz) {
+  const int m = (int)t.m, A = blockIdx.y;
+  const double* __restrict__ coord = A == 0 ? mx : (A == 1 ? my : mz);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+    const size_t g = (size_t)A * m + i;
+    t.rkey[0][g] = ordered_key(coord[i]);
+    t.rval[0][g] = i;
+    if (A == 0) t.tie[i] = 0;
+  }
+}
+
+// Digit counts of each tile of buffer src for digit `pass`, digit-major per axis: rcount[A][d][tile].
+__global__ __launch_bounds__(RS_THREADS) void k_rs_count(TreeArrays t, int pass, int src) {
+  __shared__ int h[RS_BINS];
+  const int m = (int)t.m, A = blockIdx.y, tile = blockIdx.x;
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t* __restrict__ kin = t.rkey[src] + (size_t)A * m;
+  const int32_t* __restrict__ vin = t.rval[src] + (size_t)A * m;
+  const int i0 = tile * RS_TILE;
+#pragma unroll 4
+  for (int e = 0; e < RS_E; ++e) {
+    const int i = i0 + e * RS_THREADS + (int)threadIdx.x;
+    if (i < m) {
+      const int tr = pass >= RS_KEY_PASSES ? tree_of(t.roots_lo, t.K, vin[i]) : 0;
+      atomicAdd(&h[rs_digit(kin[i], tr, pass)], 1);
+    }
+  }
+  __syncthreads();
+  t.rcount[((size_t)A * RS_BINS + threadIdx.x) * t.rs_tiles + tile] = h[threadIdx.x];
+}
+
+// Per axis and digit (one wavefront each): the exclusive scan of the digit's tile counts, and its total.
+__global__ __launch_bounds__(256) void k_rs_scan(TreeArrays t) {
+  const int gw = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int A = gw / RS_BINS, d = gw % RS_BINS;  // the grid is 3 * RS_BINS / 4 workgroups
+  const int T = t.rs_tiles;
+  int32_t* __restrict__ x = t.rcount + ((size_t)A * RS_BINS + d) * T;
+  int run = 0;
+  for (int t0 = 0; t0 < T; t0 += 256) {
+    int v[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int q = t0 + c * 64 + lane;
+      v[c] = q < T ? x[q] : 0;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      int incl = v[c];
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += y;
+      }
+      const int q = t0 + c * 64 + lane;
+      if (q < T) x[q] = run + incl - v[c];
+      run += __shfl(incl, 63, 64);
+    }
+  }
+  if (lane == 0) t.rtotal[A * RS_BINS + d] = run;
+}
+
+// One stable pass of the presort on digit `pass`, from buffer src to 1 - src: each tile ranks its elements
+// (per-wavefront running counts, peers found by ballots on the digit's bits), orders them by digit in LDS and
+// writes each digit's run to consecutive addresses from its base (the axis's block, the digits before it, the
+// tiles before this one).
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(TreeArrays t, int pass, int src) {
+  __shared__ int s_cnt[RS_THREADS / 64][RS_BINS];  // running counts of each wavefront, then its digit bases
+  __shared__ int s_base[RS_BINS];                  // destination of the digit's first element of the tile
+  __shared__ int s_start[RS_BINS];                 // the digit's first place in the tile's digit order
+  __shared__ uint64_t s_key[RS_TILE];
+  __shared__ int32_t s_val[RS_TILE];
+  __shared__ uint8_t s_dig[RS_TILE];
+  const int m = (int)t.m, A = blockIdx.y, tile = blockIdx.x;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int q = threadIdx.x; q < (RS_THREADS / 64) * RS_BINS; q += blockDim.x) (&s_cnt[0][0])[q] = 0;
+  {
+    const int d = threadIdx.x;
+    int total;
+    const int below = block_exclusive_scan(t.rtotal[A * RS_BINS + d], &total);
+    s_base[d] = (int)((int64_t)A * m) + below + t.rcount[((size_t)A * RS_BINS + d) * t.rs_tiles + tile];
+  }
+  __syncthreads();
+  const uint64_t* __restrict__ kin = t.rkey[src] + (size_t)A * m;
+  const int32_t* __restrict__ vin = t.rval[src] + (size_t)A * m;
+  uint64_t key[RS_E];
+  int32_t val[RS_E];
+  int dig[RS_E], rank[RS_E];
+  const int w0 = tile * RS_TILE + w * RS_WAVE;
+  const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int r = 0; r < RS_E; ++r) {
+    const int i = w0 + r * 64 + lane;
+    const bool act = i < m;
+    key[r] = act ? kin[i] : 0ull;
+    val[r] = act ? vin[i] : 0;
+  }
+#pragma unroll
+  for (int r = 0; r < RS_E; ++r) {
+    const int i = w0 + r * 64 + lane;
+    const bool act = i < m;
+    const int tr = (pass >= RS_KEY_PASSES && act) ? tree_of(t.roots_lo, t.K, val[r]) : 0;
+    const int d = rs_digit(key[r], tr, pass);
+    // the lanes holding the same digit: AND over the digit's bits of the matching ballots
+    unsigned long long peers = __builtin_amdgcn_ballot_w64(act);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const unsigned long long on = __builtin_amdgcn_ballot_w64(((d >> b) & 1) != 0);
+      peers &= ((d >> b) & 1) ? on : ~on;
+    }
+    const int before = s_cnt[w][d];
+    wave_lds_sync();
+    dig[r] = d;
+    rank[r] = before + __popcll(peers & below);
+    const bool leader = act && (peers & below) == 0ull;
+    if (leader) s_cnt[w][d] = before + __popcll(peers);
+    wave_lds_sync();
+  }
+  __syncthreads();
+  // digit d: the wavefronts' bases within the digit, and the digit's start in the tile's digit order
+  {
+    const int d = threadIdx.x;
+    int tot = 0;
+#pragma unroll
+    for (int v = 0; v < RS_THREADS / 64; ++v) {
+      const int c = s_cnt[v][d];
+      s_cnt[v][d] = tot;
+      tot += c;
+    }
+    int total;
+    s_start[d] = block_exclusive_scan(tot, &total);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < RS_E; ++r) {
+    const int i = w0 + r * 64 + lane;
+    if (i < m) {
+      const int lp = s_start[dig[r]] + s_cnt[w][dig[r]] + rank[r];
+      if ((unsigned)lp < (unsigned)RS_TILE) {
+        s_key[lp] = key[r];
+        s_val[lp] = val[r];
+        s_dig[lp] = (uint8_t)dig[r];
+      }
+    }
+  }
+  __syncthreads();
+  uint64_t* __restrict__ kout = t.rkey[1 - src];
+  int32_t* __restrict__ vout = t.rval[1 - src];
+  const int cnt = min(RS_TILE, m - tile * RS_TILE);
+  for (int j = threadIdx.x; j < cnt; j += RS_THREADS) {
+    const int d = s_dig[j];
+    const int dest = s_base[d] + (j - s_start[d]);
+    if ((unsigned)(dest - A * m) < (unsigned)m) {
+      kout[dest] = s_key[j];
+      vout[dest] = s_val[j];
+    } else {
+      t.counters[C_OVERFLOW] = 1;  // cannot happen for a consistent count; reported, never written past
+    }
+  }
+}
+
+// Positions i < j of a sorted list hold the same key of the same tree.
+__device__ __forceinline__ bool same_run(const TreeArrays& t, const uint64_t* __restrict__ key, int i, int j) {
+  return key[i] == key[j] && (t.K == 1 || tree_of(t.roots_lo, t.K, i) == tree_of(t.roots_lo, t.K, j));
+}
+
+// The runs of equal keys of the roots' sorted lists (buffer buf): each run's members, axis and length are
+// recorded (in any order: runs are disjoint, so the re-order does not depend on it), its segments marked, and
+// their positions in `order` (the identity at the roots) and in the axis's list set.  A run longer than RUN_MAX
+// stops the list build.
+__global__ void k_tie_scan(TreeArrays t, int buf) {
+  const int m = (int)t.m, A = blockIdx.y;
+  const uint64_t* __restrict__ key = t.rkey[buf] + (size_t)A * m;
+  const int32_t* __restrict__ val = t.rval[buf] + (size_t)A * m;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i + 1 < m; i += gridDim.x * blockDim.x) {
+    if (!same_run(t, key, i, i + 1) || (i > 0 && same_run(t, key, i - 1, i))) continue;  // first entries only
+    int len = 2;
+    while (i + len < m && len <= RUN_MAX && same_run(t, key, i, i + len)) ++len;
+    if (len > RUN_MAX) {
+      t.counters[C_DUP] = 1;
+      t.counters[C_OVERFLOW] = 1;
+      continue;
+    }
+    const int r = atomicAdd(&t.counters[C_RUNS], 1);
+    const int off = atomicAdd(&t.counters[C_RUNMEM], len);
+    t.run_off[r] = off;
+    t.run_len[r] = (len << 2) | A;
+    for (int k = 0; k < len; ++k) {
+      const int s = val[i + k];
+      t.run_seg[off + k] = s;
+      t.tie[s] = 1;
+      t.inv_o[s] = s;
+      t.invb[(size_t)A * m + s] = i + k;
+    }
+  }
+}
+
+// The flag of every list entry whose segment is in a run of any axis.
+__global__ void k_tie_mark(TreeArrays t, int buf) {
+  if (live_count(t.counters, C_RUNS) == 0) return;
+  const int m = (int)t.m, A = blockIdx.y;
+  int32_t* __restrict__ val = t.rval[buf] + (size_t)A * m;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+    const int s = val[i];
+    if (t.tie[s]) val[i] = s | TIE_FLAG;
+  }
+}
+
+// After a level's partition (lists in lst): each run's members, in every node, in the order of their positions
+// in `order`.  A run's members in one node's list are consecutive (equal keys), and nodes are ordered alike in
+// `order` and in the lists, so the member with the k-th smallest position in `order` takes the k-th smallest of
+// the run's list positions.  One wavefront per run; positions are unique, so ranks are counts of smaller ones.
+__global__ __launch_bounds__(64 * FIX_WAVES) void k_tie_fix(TreeArrays t, int32_t* __restrict__ lst) {
+  __shared__ int s_o[FIX_WAVES][RUN_MAX], s_p[FIX_WAVES][RUN_MAX], s_q[FIX_WAVES][RUN_MAX];
+  const int runs = live_count(t.counters, C_RUNS);
+  const int m = (int)t.m;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int r = blockIdx.x * FIX_WAVES + wv; r < runs; r += gridDim.x * FIX_WAVES) {
+    const int off = t.run_off[r], len = t.run_len[r] >> 2, A = t.run_len[r] & 3;
+    if (len > RUN_MAX || off < 0 || off + len > 3 * m) {  // cannot happen for a consistent scan; reported
+      t.counters[C_OVERFLOW] = 1;
+      continue;
+    }
+    const int32_t* __restrict__ seg = t.run_seg + off;
+    int32_t* __restrict__ inv = t.invb + (size_t)A * m;
+    for (int k = lane; k < len; k += 64) {
+      const int s = seg[k];
+      s_o[wv][k] = t.inv_o[s];
+      s_p[wv][k] = inv[s];
+    }
+    wave_lds_sync();
+    for (int k = lane; k < len; k += 64) {
+      const int p = s_p[wv][k];
+      int rank = 0;
+      for (int j = 0; j < len; ++j) rank += s_p[wv][j] < p;
+      s_q[wv][rank] = p;
+    }
+    wave_lds_sync();
+    for (int k = lane; k < len; k += 64) {
+      const int o = s_o[wv][k];
+      int rank = 0;
+      for (int j = 0; j < len; ++j) rank += s_o[wv][j] < o;
+      const int dest = s_q[wv][rank];
+      const int s = seg[k];
+      if ((unsigned)dest < (unsigned)m) {
+        lst[(size_t)A * m + dest] = s | TIE_FLAG;
+        inv[s] = dest;
+      } else {
+        t.counters[C_OVERFLOW] = 1;
+      }
+    }
+    wave_lds_sync();
+  }
+}
+
+// Box, axis and split flag of a big node from its lists' first and last entries.
+__device__ __forceinline__ void list_axis(const TreeArrays& t, const int32_t* __restrict__ lst, int lo, int hi,
+                                          const double* __restrict__ mx, const double* __restrict__ my,
+                                          const double* __restrict__ mz, int8_t* axis, int8_t* split) {
+  const int m = (int)t.m;
+  double v[6];
+  const double* cs[3] = {mx, my, mz};
+#pragma unroll
+  for (int A = 0; A < 3; ++A) {
+    const int a = lst[(size_t)A * m + lo] & SEG_MASK, b = lst[(size_t)A * m + hi - 1] & SEG_MASK;
+    v[A] = cs[A][(unsigned)a < (unsigned)m ? a : 0];
+    v[3 + A] = cs[A][(unsigned)b < (unsigned)m ? b : 0];
+  }
+  choose_axis(v, axis, split);
+}
+
+// Each tile of the level's big nodes: the node's sorted order (its list of the split axis) into `order`, and the
+// side of each of its segments.
+__global__ __launch_bounds__(256) void k_list_apply(TreeArrays t, const int32_t* __restrict__ lst,
+                                                    const double* __restrict__ mx, const double* __restrict__ my,
+                                                    const double* __restrict__ mz) {
+  const int ntiles = live_count(t.counters, C_TILES);
+  const int m = (int)t.m;
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int node = t.tile_node[tile];
+    const int lo = t.lo[node], hi = t.hi[node];
+    const int p0 = t.tile_p0[tile], p1 = t.tile_p1[tile];
+    int8_t axis, split;
+    list_axis(t, lst, lo, hi, mx, my, mz, &axis, &split);
+    if (p0 == lo && threadIdx.x == 0) {  // the node's first tile records it for the next plan
+      t.axis[node] = axis;
+      t.split[node] = split;
+    }
+    if (!split) continue;
+    const int mid = (int)(((int64_t)lo + hi) / 2);
+    const int32_t* __restrict__ src = lst + (size_t)axis * m;
+    for (int p = p0 + (int)threadIdx.x; p < p1; p += blockDim.x) {
+      const int v = src[p];
+      const int s = v & SEG_MASK;
+      if (s < m) {
+        t.order[p] = s;
+        t.side[s] = p < mid ? 1 : 0;
+        if (v < 0) t.inv_o[s] = p;
+      } else {
+        t.counters[C_OVERFLOW] = 1;
+      }
+    }
+  }
+}
+
+// Per tile and axis: how many of the tile's list entries go to the left child.
+__global__ __launch_bounds__(256) void k_list_count(TreeArrays t, const int32_t* __restrict__ lst) {
+  const int ntiles = live_count(t.counters, C_TILES);
+  const int m = (int)t.m, A = blockIdx.y;
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int node = t.tile_node[tile];
+    if (!t.split[node]) continue;
+    const int p0 = t.tile_p0[tile], p1 = t.tile_p1[tile];
+    const int32_t* __restrict__ src = lst + (size_t)A * m;
+    int c = 0;
+    for (int p = p0 + (int)threadIdx.x; p < p1; p += blockDim.x) {
+      const int q = src[p] & SEG_MASK;
+      c += q < m ? t.side[q] : 0;
+    }
+    int total;
+    block_exclusive_scan(c, &total);
+    if (threadIdx.x == 0) t.tcount[3 * tile + A] = total;
+  }
+}
+
+// Per tile and axis: the stable partition of the node's list into its children's ranges.
+static constexpr int LS_THREADS = 1024;
+static constexpr int LS_E = TILE / LS_THREADS;
+__global__ __launch_bounds__(LS_THREADS) void k_list_scatter(TreeArrays t, const int32_t* __restrict__ lst_in,
+                                                             int32_t* __restrict__ lst_out) {
+  const int ntiles = live_count(t.counters, C_TILES);
+  const int m = (int)t.m, A = blockIdx.y;
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int node = t.tile_node[tile];
+    if (!t.split[node]) continue;
+    const int lo = t.lo[node], hi = t.hi[node];
+    const int p0 = t.tile_p0[tile], p1 = t.tile_p1[tile];
+    const int mid = (int)(((int64_t)lo + hi) / 2);
+    // left entries of the node's tiles before this one (the node's tiles are consecutive)
+    const int first = tile - (p0 - lo) / TILE;
+    int before = 0;
+    for (int q = first + (int)threadIdx.x; q < tile; q += blockDim.x) before += t.tcount[3 * q + A];
+    int prefix;
+    block_exclusive_scan(before, &prefix);
+    const int32_t* __restrict__ src = lst_in + (size_t)A * m;
+    int s[LS_E], f[LS_E];
+    int mine = 0;
+    const int e0 = p0 + (int)threadIdx.x * LS_E;
+#pragma unroll
+    for (int e = 0; e < LS_E; ++e) {
+      const int p = e0 + e;
+      s[e] = p < p1 ? src[p] : 0;
+    }
+#pragma unroll
+    for (int e = 0; e < LS_E; ++e) {
+      const int p = e0 + e;
+      const int q = s[e] & SEG_MASK;
+      f[e] = (p < p1 && q < m) ? t.side[q] : 0;
+      mine += f[e];
+    }
+    int total;
+    int left = prefix + block_exclusive_scan(mine, &total);
+    int32_t* __restrict__ dst = lst_out + (size_t)A * m;
+    int32_t* __restrict__ inv = t.invb + (size_t)A * m;
+#pragma unroll
+    for (int e = 0; e < LS_E; ++e) {
+      const int p = e0 + e;
+      if (p < p1) {
+        const int dest = f[e] ? lo + left : mid + (p - lo - left);
+        if (dest >= lo && dest < hi) {
+          dst[dest] = s[e];
+          if (s[e] < 0 && (s[e] & SEG_MASK) < m) inv[s[e] & SEG_MASK] = dest;
+        } else {
+          t.counters[C_OVERFLOW] = 1;
+        }
+        left += f[e];
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------------ bottom phase
 __global__ __launch_bounds__(PLAN_THREADS) void k_bottom_bases(TreeArrays t) {
   const int count = live_count(t.counters, C_BOTTOM);
@@ -892,7 +1327,8 @@ __global__ __launch_bounds__(BOTTOM_THREADS) void k_bottom(TreeArrays t, const d
     const int a = t.lo[root];
     const int s = t.hi[root] - a;
     const int slots = heap_slots(s, t.leaf);
-    const int32_t* src = (t.depth[root] & 1) ? t.tmp : t.order;  // the buffer of the root's depth parity
+    // the buffer of the root's depth parity (the list build keeps every node's segments in order)
+    const int32_t* src = (!t.lists && (t.depth[root] & 1)) ? t.tmp : t.order;
     for (int i = threadIdx.x; i < s; i += blockDim.x) {
       const int g = src[a + i];
       seg[i] = g;
@@ -1151,6 +1587,13 @@ int tree_layout(Ctx& c, int buf, int64_t m64, int K, int leaf, TreeArrays& t) {
   need += 2 * al((size_t)t.bottom_cap * 4);                               // bottom, heap base
   need += 3 * al((M + 1 + C) * 4) + al(((M + C) / 4096 + 8) * 4);         // lo_cnt, lo_mind, lo_base, parts
   need += al(C * 4) + 8 * al(C * 4) + al((K + 1) * 4);                    // pre, p_*, tree_first
+  // list build: keys and ids of the three axes (two buffers), sides, partition counts, presort bookkeeping
+  t.rs_tiles = (int)((M + RS_TILE - 1) / RS_TILE);
+  t.rs_passes = RS_KEY_PASSES + (K <= 1 ? 0 : (K <= 256 ? 1 : (K <= 65536 ? 2 : 3)));
+  const size_t RUNS = 3 * M / 2 + 1;
+  need += 2 * al(3 * M * 8) + 2 * al(3 * M * 4) + al(M) + al((size_t)t.tile_cap * 3 * 4);
+  need += al(3 * (size_t)t.rs_tiles * RS_BINS * 4) + al(3 * RS_BINS * 4);
+  need += al(3 * M * 4) + 2 * al(RUNS * 4) + al(M) + al(M * 4) + al(3 * M * 4);  // runs
   OFW_HIP(c.tree[buf].reserve(need));
   char* base = (char*)c.tree[buf].ptr;
   auto carve = [&](size_t bytes) {
@@ -1200,6 +1643,19 @@ int tree_layout(Ctx& c, int buf, int64_t m64, int K, int leaf, TreeArrays& t) {
   t.p_node = (int32_t*)carve(C * 4);
   t.p_irank = (int32_t*)carve(C * 4);
   t.tree_first = (int32_t*)carve((K + 1) * 4);
+  for (int b = 0; b < 2; ++b) t.rkey[b] = (uint64_t*)carve(3 * M * 8);
+  for (int b = 0; b < 2; ++b) t.rval[b] = (int32_t*)carve(3 * M * 4);
+  t.side = (uint8_t*)carve(M);
+  t.tcount = (int32_t*)carve((size_t)t.tile_cap * 3 * 4);
+  t.rcount = (int32_t*)carve(3 * (size_t)t.rs_tiles * RS_BINS * 4);
+  t.rtotal = (int32_t*)carve(3 * RS_BINS * 4);
+  t.run_seg = (int32_t*)carve(3 * M * 4);
+  t.run_off = (int32_t*)carve(RUNS * 4);
+  t.run_len = (int32_t*)carve(RUNS * 4);
+  t.tie = (uint8_t*)carve(M);
+  t.inv_o = (int32_t*)carve(M * 4);
+  t.invb = (int32_t*)carve(3 * M * 4);
+  t.lists = c.sort_build ? 0 : 1;
   return OFW_OK;
 }
 
@@ -1214,38 +1670,63 @@ int tree_build(Ctx& c, TreeArrays& t, const double* mx, const double* my, const 
   OFW_HIP(hipGetLastError());
   // top levels: while the largest node of a level can hold more than S_BOTTOM segments.  Level lvl reads the
   // order buffer of its parity and leaves its sorted nodes in the other one.
+  int lbuf = 0;  // the list buffer of the current level
+  if (t.lists && m > 0) {
+    // presort of the three axes' lists and the runs of equal coordinates in them
+    const unsigned ge = (unsigned)std::min<int64_t>((m + 255) / 256, 4 * (int64_t)grid);
+    hipLaunchKernelGGL(k_rs_init, dim3(ge, 3), dim3(256), 0, c.stream, t, mx, my, mz);
+    for (int p = 0; p < t.rs_passes; ++p) {
+      hipLaunchKernelGGL(k_rs_count, dim3((unsigned)t.rs_tiles, 3), dim3(RS_THREADS), 0, c.stream, t, p, lbuf);
+      hipLaunchKernelGGL(k_rs_scan, dim3(3 * RS_BINS / 4), dim3(256), 0, c.stream, t);
+      hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)t.rs_tiles, 3), dim3(RS_THREADS), 0, c.stream, t, p, lbuf);
+      lbuf ^= 1;
+    }
+    hipLaunchKernelGGL(k_tie_scan, dim3(ge, 3), dim3(256), 0, c.stream, t, lbuf);
+    hipLaunchKernelGGL(k_tie_mark, dim3(ge, 3), dim3(256), 0, c.stream, t, lbuf);
+    OFW_HIP(hipGetLastError());
+  }
   hipLaunchKernelGGL(k_level_plan, dim3(1), dim3(PLAN_THREADS), 0, c.stream, t, t.level_a);
   int32_t* const bigs[2] = {t.big, t.big_alt};
   int32_t* const orders[2] = {t.order, t.tmp};
   for (int lvl = 0;; ++lvl) {
     const int64_t biggest = (max_root + ((int64_t)1 << lvl) - 1) >> lvl;  // ceil(max_root / 2^lvl)
     if (!is_big((int)std::min<int64_t>(biggest, INT_MAX), t.leaf)) break;  // planned: all to the bottom
-    const int32_t* oin = orders[lvl & 1];
-    int32_t* oout = orders[(lvl + 1) & 1];
-    int passes = 0;
-    for (int64_t width = TILE; width < biggest; width <<= 1) ++passes;
     // no more tiles than m / TILE plus one partial tile per node of the level: grids beyond that only idle
     const int64_t tile_bound = (int64_t)m / TILE + ((int64_t)t.K << std::min(lvl, 30)) + 1;
-    const unsigned g1 = (unsigned)std::min<int64_t>(grid, tile_bound);
-    const unsigned g2 = (unsigned)std::min<int64_t>(2 * (int64_t)grid, tile_bound);
-    const unsigned g4 = (unsigned)std::min<int64_t>(4 * (int64_t)grid, tile_bound * SUBS);
-    hipLaunchKernelGGL(k_box_tiles, dim3(g4), dim3(BOX_THREADS), 0, c.stream, t, oin, mx, my, mz);
-    hipLaunchKernelGGL(k_sort_tiles, dim3(g1), dim3(TILE_THREADS), 0, c.stream, t, oin,
-                       passes == 0 ? oout : nullptr, mx, my, mz);
-    uint64_t *kin = t.keyA, *kout = t.keyB;
-    uint32_t *pin = t.posA, *pout = t.posB;
-    for (int p = 0; p < passes; ++p) {
-      const bool last = p == passes - 1;
-      hipLaunchKernelGGL(k_merge_pass, dim3(g2), dim3(MERGE_THREADS), 0, c.stream, t, (int)(TILE << p), kin,
-                         pin, kout, pout, last ? oin : nullptr, last ? oout : nullptr);
-      std::swap(kin, kout);
-      std::swap(pin, pout);
+    if (t.lists) {
+      const unsigned gt = (unsigned)std::min<int64_t>(2 * (int64_t)grid, tile_bound);
+      hipLaunchKernelGGL(k_list_apply, dim3(gt), dim3(256), 0, c.stream, t, t.rval[lbuf], mx, my, mz);
+      hipLaunchKernelGGL(k_list_count, dim3(gt, 3), dim3(256), 0, c.stream, t, t.rval[lbuf]);
+      hipLaunchKernelGGL(k_list_scatter, dim3(gt, 3), dim3(LS_THREADS), 0, c.stream, t, t.rval[lbuf],
+                         t.rval[lbuf ^ 1]);
+      lbuf ^= 1;
+      hipLaunchKernelGGL(k_tie_fix, dim3(grid), dim3(64 * FIX_WAVES), 0, c.stream, t, t.rval[lbuf]);
+    } else {
+      const int32_t* oin = orders[lvl & 1];
+      int32_t* oout = orders[(lvl + 1) & 1];
+      int passes = 0;
+      for (int64_t width = TILE; width < biggest; width <<= 1) ++passes;
+      const unsigned g1 = (unsigned)std::min<int64_t>(grid, tile_bound);
+      const unsigned g2 = (unsigned)std::min<int64_t>(2 * (int64_t)grid, tile_bound);
+      const unsigned g4 = (unsigned)std::min<int64_t>(4 * (int64_t)grid, tile_bound * SUBS);
+      hipLaunchKernelGGL(k_box_tiles, dim3(g4), dim3(BOX_THREADS), 0, c.stream, t, oin, mx, my, mz);
+      hipLaunchKernelGGL(k_sort_tiles, dim3(g1), dim3(TILE_THREADS), 0, c.stream, t, oin,
+                         passes == 0 ? oout : nullptr, mx, my, mz);
+      uint64_t *kin = t.keyA, *kout = t.keyB;
+      uint32_t *pin = t.posA, *pout = t.posB;
+      for (int p = 0; p < passes; ++p) {
+        const bool last = p == passes - 1;
+        hipLaunchKernelGGL(k_merge_pass, dim3(g2), dim3(MERGE_THREADS), 0, c.stream, t, (int)(TILE << p), kin,
+                           pin, kout, pout, last ? oin : nullptr, last ? oout : nullptr);
+        std::swap(kin, kout);
+        std::swap(pin, pout);
+      }
     }
     hipLaunchKernelGGL(k_level_next, dim3(1), dim3(PLAN_THREADS), 0, c.stream, t, lvl, bigs[lvl & 1],
                        bigs[(lvl + 1) & 1]);
     OFW_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_copy_strands, dim3(grid), dim3(256), 0, c.stream, t);
+  if (!t.lists) hipLaunchKernelGGL(k_copy_strands, dim3(grid), dim3(256), 0, c.stream, t);
   // bottom phase
   hipLaunchKernelGGL(k_bottom_bases, dim3(1), dim3(PLAN_THREADS), 0, c.stream, t);
   hipLaunchKernelGGL(k_bottom, dim3(grid), dim3(BOTTOM_THREADS), 0, c.stream, t, mx, my, mz);
