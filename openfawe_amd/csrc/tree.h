@@ -80,6 +80,7 @@ struct TreeArrays {
   uint8_t* tie = nullptr;        // per segment: 1 when it is in a run of any axis
   int32_t* inv_o = nullptr;      // per run segment: its position in `order`
   int32_t* invb = nullptr;       // 3 x m, per run segment: its position in each list
+  int32_t* ord_inv = nullptr;    // per segment of a bottom subtree: its position in `order`
 };
 
 // Copy host int64 values (each below 2^31) to a device int32 array through kernel arguments: no host

@@ -39,6 +39,12 @@ static constexpr int BOTTOM_E = S_BOTTOM / BOTTOM_THREADS;  // elements per thre
 static constexpr int HCAP = 2 * S_BOTTOM - 1;
 static constexpr int PLAN_THREADS = 1024;
 static constexpr int SUBS = 4;              // workgroups per tile of the elementwise top-phase kernels
+// list build
+static constexpr int RUN_MAX = 256;                  // longest run of equal coordinates the lists re-order
+static constexpr int FIX_WAVES = PLAN_THREADS / 64;  // wavefronts (one run each) per workgroup of the re-order
+static constexpr int FIX_BLOCKS = 64;                // workgroups of the re-order
+static constexpr int32_t TIE_FLAG = INT32_MIN;       // list entry of a segment in a run of equal coordinates
+static constexpr int32_t SEG_MASK = INT32_MAX;       // the segment id of a list entry
 
 __device__ __forceinline__ uint64_t ordered_key(double x) {
   uint64_t u = (x == 0.0) ? 0ull : (uint64_t)__double_as_longlong(x);
@@ -636,13 +642,68 @@ __global__ __launch_bounds__(MERGE_THREADS) void k_merge_pass(TreeArrays t, int 
   }
 }
 
+// After a level's partition (lists in lst): each run's members, in every node, in the order of their positions
+// in `order`.  A run's members in one node's list are consecutive (equal keys), and nodes are ordered alike in
+// `order` and in the lists, so the member with the k-th smallest position in `order` takes the k-th smallest of
+// the run's list positions.  One wavefront per run (run `wave`, then every nwaves-th); positions are unique,
+// so ranks are counts of smaller ones.
+__device__ void tie_fix_runs(TreeArrays& t, int32_t* __restrict__ lst, int wave, int nwaves) {
+  __shared__ int s_o[FIX_WAVES][RUN_MAX], s_p[FIX_WAVES][RUN_MAX], s_q[FIX_WAVES][RUN_MAX];
+  const int runs = live_count(t.counters, C_RUNS);
+  const int m = (int)t.m;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int r = wave; r < runs; r += nwaves) {
+    const int off = t.run_off[r], len = t.run_len[r] >> 2, A = t.run_len[r] & 3;
+    if (len > RUN_MAX || off < 0 || off + len > 3 * m) {  // cannot happen for a consistent scan; reported
+      t.counters[C_OVERFLOW] = 1;
+      continue;
+    }
+    const int32_t* __restrict__ seg = t.run_seg + off;
+    int32_t* __restrict__ inv = t.invb + (size_t)A * m;
+    for (int k = lane; k < len; k += 64) {
+      const int s = seg[k];
+      s_o[wv][k] = t.inv_o[s];
+      s_p[wv][k] = inv[s];
+    }
+    wave_lds_sync();
+    for (int k = lane; k < len; k += 64) {
+      const int p = s_p[wv][k];
+      int rank = 0;
+      for (int j = 0; j < len; ++j) rank += s_p[wv][j] < p;
+      s_q[wv][rank] = p;
+    }
+    wave_lds_sync();
+    for (int k = lane; k < len; k += 64) {
+      const int o = s_o[wv][k];
+      int rank = 0;
+      for (int j = 0; j < len; ++j) rank += s_o[wv][j] < o;
+      const int dest = s_q[wv][rank];
+      const int s = seg[k];
+      if ((unsigned)dest < (unsigned)m) {
+        lst[(size_t)A * m + dest] = s | TIE_FLAG;
+        inv[s] = dest;
+      } else {
+        t.counters[C_OVERFLOW] = 1;
+      }
+    }
+    wave_lds_sync();
+  }
+}
+
 // One workgroup after the sort of level `lvl`: the children of its big nodes that split (vortex_tree.py:244-268)
 // form level lvl + 1, which is planned at once (big children get tiles, small ones go to the bottom phase).
 // A top-phase node at depth d holds its segments in the order buffer of d's parity (even: order, odd: tmp);
 // the nodes that stop at an odd depth without going to the bottom phase (leaves, and big nodes that do not
 // split) are listed for the final copy into `order`.
+// With the list build (fix_lst: the lists just partitioned), workgroups 1, 2, ... put the runs of equal
+// coordinates back in order (tie_fix_runs) while workgroup 0 plans.
 __global__ __launch_bounds__(PLAN_THREADS) void k_level_next(TreeArrays t, int lvl, const int32_t* __restrict__ big_in,
-                                                             int32_t* __restrict__ big_out) {
+                                                             int32_t* __restrict__ big_out,
+                                                             int32_t* __restrict__ fix_lst) {
+  if (blockIdx.x > 0) {
+    if (fix_lst) tie_fix_runs(t, fix_lst, (blockIdx.x - 1) * FIX_WAVES + (threadIdx.x >> 6), (gridDim.x - 1) * FIX_WAVES);
+    return;
+  }
   const int nbig = live_count(t.counters, C_BIG);
   const int nodes0 = t.counters[C_NODES];
   const int bottom0 = t.counters[C_BOTTOM];
@@ -776,10 +837,6 @@ static constexpr int RS_WAVE = RS_TILE / (RS_THREADS / 64);  // elements per wav
 static constexpr int RS_BINS = 256;
 static constexpr int RS_KEY_PASSES = 8;
 static constexpr int RS_MAX_PASSES = RS_KEY_PASSES + 3;
-static constexpr int RUN_MAX = 1024;                 // longest run of equal coordinates the lists re-order
-static constexpr int FIX_WAVES = 4;                  // wavefronts (one run each) per workgroup of the re-order
-static constexpr int32_t TIE_FLAG = INT32_MIN;       // list entry of a segment in a run of equal coordinates
-static constexpr int32_t SEG_MASK = INT32_MAX;
 static_assert(RS_WAVE % 64 == 0, "a wavefront ranks whole rounds of 64 elements");
 static_assert(RS_THREADS == RS_BINS, "one thread per digit");
 
@@ -807,7 +864,10 @@ __global__ void k_rs_init(TreeArrays t, const double* __restrict__ mx, const dou
     const size_t g = (size_t)A * m + i;
     t.rkey[0][g] = ordered_key(coord[i]);
     t.rval[0][g] = i;
-    if (A == 0) t.tie[i] = 0;
+    if (A == 0) {
+      t.tie[i] = 0;
+      t.ord_inv[i] = i;
+    }
   }
 }
 
@@ -1008,53 +1068,6 @@ __global__ void k_tie_mark(TreeArrays t, int buf) {
   }
 }
 
-// After a level's partition (lists in lst): each run's members, in every node, in the order of their positions
-// in `order`.  A run's members in one node's list are consecutive (equal keys), and nodes are ordered alike in
-// `order` and in the lists, so the member with the k-th smallest position in `order` takes the k-th smallest of
-// the run's list positions.  One wavefront per run; positions are unique, so ranks are counts of smaller ones.
-__global__ __launch_bounds__(64 * FIX_WAVES) void k_tie_fix(TreeArrays t, int32_t* __restrict__ lst) {
-  __shared__ int s_o[FIX_WAVES][RUN_MAX], s_p[FIX_WAVES][RUN_MAX], s_q[FIX_WAVES][RUN_MAX];
-  const int runs = live_count(t.counters, C_RUNS);
-  const int m = (int)t.m;
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int r = blockIdx.x * FIX_WAVES + wv; r < runs; r += gridDim.x * FIX_WAVES) {
-    const int off = t.run_off[r], len = t.run_len[r] >> 2, A = t.run_len[r] & 3;
-    if (len > RUN_MAX || off < 0 || off + len > 3 * m) {  // cannot happen for a consistent scan; reported
-      t.counters[C_OVERFLOW] = 1;
-      continue;
-    }
-    const int32_t* __restrict__ seg = t.run_seg + off;
-    int32_t* __restrict__ inv = t.invb + (size_t)A * m;
-    for (int k = lane; k < len; k += 64) {
-      const int s = seg[k];
-      s_o[wv][k] = t.inv_o[s];
-      s_p[wv][k] = inv[s];
-    }
-    wave_lds_sync();
-    for (int k = lane; k < len; k += 64) {
-      const int p = s_p[wv][k];
-      int rank = 0;
-      for (int j = 0; j < len; ++j) rank += s_p[wv][j] < p;
-      s_q[wv][rank] = p;
-    }
-    wave_lds_sync();
-    for (int k = lane; k < len; k += 64) {
-      const int o = s_o[wv][k];
-      int rank = 0;
-      for (int j = 0; j < len; ++j) rank += s_o[wv][j] < o;
-      const int dest = s_q[wv][rank];
-      const int s = seg[k];
-      if ((unsigned)dest < (unsigned)m) {
-        lst[(size_t)A * m + dest] = s | TIE_FLAG;
-        inv[s] = dest;
-      } else {
-        t.counters[C_OVERFLOW] = 1;
-      }
-    }
-    wave_lds_sync();
-  }
-}
-
 // Box, axis and split flag of a big node from its lists' first and last entries.
 __device__ __forceinline__ void list_axis(const TreeArrays& t, const int32_t* __restrict__ lst, int lo, int hi,
                                           const double* __restrict__ mx, const double* __restrict__ my,
@@ -1090,6 +1103,7 @@ __global__ __launch_bounds__(256) void k_list_apply(TreeArrays t, const int32_t*
     }
     if (!split) continue;
     const int mid = (int)(((int64_t)lo + hi) / 2);
+    const bool small_l = mid - lo <= S_BOTTOM, small_r = hi - mid <= S_BOTTOM;  // children that leave the lists
     const int32_t* __restrict__ src = lst + (size_t)axis * m;
     for (int p = p0 + (int)threadIdx.x; p < p1; p += blockDim.x) {
       const int v = src[p];
@@ -1098,6 +1112,7 @@ __global__ __launch_bounds__(256) void k_list_apply(TreeArrays t, const int32_t*
         t.order[p] = s;
         t.side[s] = p < mid ? 1 : 0;
         if (v < 0) t.inv_o[s] = p;
+        if (p < mid ? small_l : small_r) t.ord_inv[s] = p;
       } else {
         t.counters[C_OVERFLOW] = 1;
       }
@@ -1308,7 +1323,25 @@ __device__ void sort_heap_level(uint64_t* key, uint32_t* tag, const uint16_t* hl
   __syncthreads();
 }
 
-// One workgroup per subtree of at most S_BOTTOM segments: every remaining level in LDS.
+// Node-table entries of heap slot j of bottom subtree b (root `root` at segment a, heap nodes from hb on).
+__device__ __forceinline__ void heap_node_out(TreeArrays& t, int root, int hb, int a, int d0, int tr, int j, bool exists,
+                                              bool sp, int lo_j, int hi_j) {
+  const int gid = j == 0 ? root : hb + j - 1;
+  if (j > 0) {
+    t.lo[gid] = exists ? a + lo_j : -1;
+    t.hi[gid] = exists ? a + hi_j : -1;
+    const int pj = (j - 1) / 2;
+    t.parent[gid] = pj == 0 ? root : hb + pj - 1;
+    t.tree[gid] = tr;
+    int d = 0;
+    for (int q = j; q > 0; q = (q - 1) / 2) ++d;
+    t.depth[gid] = d0 + d;
+  }
+  t.left[gid] = sp ? hb + 2 * j : -1;       // heap slot 2j+1 -> hb + 2j
+  t.right[gid] = sp ? hb + 2 * j + 1 : -1;  // heap slot 2j+2 -> hb + 2j+1
+}
+
+// One workgroup per subtree of at most S_BOTTOM segments: every remaining level in LDS (the sort build).
 __global__ __launch_bounds__(BOTTOM_THREADS) void k_bottom(TreeArrays t, const double* __restrict__ mx,
                                                            const double* __restrict__ my,
                                                            const double* __restrict__ mz) {
@@ -1327,8 +1360,8 @@ __global__ __launch_bounds__(BOTTOM_THREADS) void k_bottom(TreeArrays t, const d
     const int a = t.lo[root];
     const int s = t.hi[root] - a;
     const int slots = heap_slots(s, t.leaf);
-    // the buffer of the root's depth parity (the list build keeps every node's segments in order)
-    const int32_t* src = (!t.lists && (t.depth[root] & 1)) ? t.tmp : t.order;
+    // the buffer of the root's depth parity
+    const int32_t* src = (t.depth[root] & 1) ? t.tmp : t.order;
     for (int i = threadIdx.x; i < s; i += blockDim.x) {
       const int g = src[a + i];
       seg[i] = g;
@@ -1428,24 +1461,243 @@ __global__ __launch_bounds__(BOTTOM_THREADS) void k_bottom(TreeArrays t, const d
     for (int i = threadIdx.x; i < s; i += blockDim.x) t.order[a + i] = seg[i];
     const int hb = t.heap_base[b];
     const int d0 = t.depth[root], tr = t.tree[root];
-    for (int j = threadIdx.x; j < slots; j += blockDim.x) {
-      const int gid = j == 0 ? root : hb + j - 1;
-      const bool exists = hstate[j] != 0;
-      const bool sp = hstate[j] == 2;
-      if (j > 0) {
-        t.lo[gid] = exists ? a + hlo[j] : -1;
-        t.hi[gid] = exists ? a + hhi[j] : -1;
-        const int pj = (j - 1) / 2;
-        t.parent[gid] = pj == 0 ? root : hb + pj - 1;
-        t.tree[gid] = tr;
-        int d = 0;
-        for (int q = j; q > 0; q = (q - 1) / 2) ++d;
-        t.depth[gid] = d0 + d;
+    for (int j = threadIdx.x; j < slots; j += blockDim.x)
+      heap_node_out(t, root, hb, a, d0, tr, j, hstate[j] != 0, hstate[j] == 2, hlo[j], hhi[j]);
+    __syncthreads();
+  }
+}
+
+// The list build's bottom phase: one workgroup per subtree of at most S_BOTTOM segments.  Local ids are the
+// positions in the root's order; every level in LDS splits each node's three lists by a stable partition (no
+// sorting): the node's box is the first and last entry of its lists, its sides and its children's order come from
+// its list of the split axis, and the order of a leaf is its parent's list of that axis restricted to the leaf.
+// When the subtree holds segments of runs of equal coordinates, each partitioned run is put back in the order of
+// the new positions (ranks counted within the run; runs are at most RUN_MAX long).
+static constexpr int BL_THREADS = 512;
+static constexpr int BL_E = S_BOTTOM / BL_THREADS;   // consecutive positions per thread
+static constexpr int BL_LEVEL_NODES = S_BOTTOM / 2;  // splitting nodes of one heap level (two segments or more)
+static constexpr int RID_NONE = 2 * S_BOTTOM;
+__global__ __launch_bounds__(BL_THREADS) void k_bottom_lists(TreeArrays t, int lbuf0, const double* __restrict__ mx,
+                                                             const double* __restrict__ my,
+                                                             const double* __restrict__ mz) {
+  __shared__ uint16_t L[2][3][S_BOTTOM];  // the lists (local ids), two buffers
+  __shared__ uint16_t rid[3][S_BOTTOM];   // local id -> first position of its run of equal coordinates in the
+                                          // root's list, or RID_NONE + position when it is in no run of that axis
+  __shared__ uint16_t pos_new[S_BOTTOM];  // local id -> its position in the order after this level's split
+  __shared__ uint8_t side[S_BOTTOM];      // local id -> 1 when it goes left at this level
+  __shared__ uint16_t nodeof[S_BOTTOM];
+  __shared__ uint16_t hlo[HCAP], hhi[HCAP];
+  __shared__ uint8_t hinfo[HCAP];         // state (0 absent, 1 leaf so far, 2 split) | axis << 2 | parent's axis << 4
+  __shared__ uint16_t nbase[BL_LEVEL_NODES][3];
+  __shared__ int any_split, any_tie;
+  const int count = live_count(t.counters, C_BOTTOM);
+  const int m = (int)t.m;
+  const double* cs[3] = {mx, my, mz};
+  for (int b = blockIdx.x; b < count; b += gridDim.x) {
+    const int root = t.bottom[b];
+    const int a = t.lo[root];
+    const int s = t.hi[root] - a;
+    const int slots = heap_slots(s, t.leaf);
+    const int32_t* __restrict__ lst = t.rval[(lbuf0 + t.depth[root]) & 1];
+    const int32_t* __restrict__ seg = t.order + a;  // local id -> segment
+    __syncthreads();  // every thread has read the previous subtree's flags
+    if (threadIdx.x == 0) any_tie = 0;
+    __syncthreads();
+    for (int A = 0; A < 3; ++A)
+      for (int i = threadIdx.x; i < s; i += blockDim.x) {
+        const int v = lst[(size_t)A * m + a + i];
+        const int q = v & SEG_MASK;
+        const int loc = q < m ? t.ord_inv[q] - a : -1;
+        if (v < 0) any_tie = 1;
+        if ((unsigned)loc < (unsigned)s) {
+          L[0][A][i] = (uint16_t)loc;
+          if (A == 0) side[loc] = v < 0 ? 1 : 0;  // the segment is in a run of some axis
+        } else {
+          L[0][A][i] = 0;
+          t.counters[C_OVERFLOW] = 1;  // cannot happen for a consistent build; reported
+        }
       }
-      t.left[gid] = sp ? hb + 2 * j : -1;       // heap slot 2j+1 -> hb + 2j
-      t.right[gid] = sp ? hb + 2 * j + 1 : -1;  // heap slot 2j+2 -> hb + 2j+1
+    for (int i = threadIdx.x; i < s; i += blockDim.x) nodeof[i] = 0;
+    for (int j = threadIdx.x; j < slots; j += blockDim.x) {
+      hinfo[j] = j == 0 ? (uint8_t)(1 | (3 << 4)) : 0;
+      hlo[j] = 0;
+      hhi[j] = j == 0 ? (uint16_t)s : 0;
     }
     __syncthreads();
+    const bool ties = any_tie != 0;
+    if (ties) {  // run ids from the root's lists (coordinates compared for flagged neighbours only)
+      for (int A = 0; A < 3; ++A) {
+        for (int i = threadIdx.x; i < s; i += blockDim.x) {
+          const int x = L[0][A][i];
+          bool eq = false;
+          if (i > 0 && side[x]) {
+            const int y = L[0][A][i - 1];
+            eq = side[y] && cs[A][seg[x]] == cs[A][seg[y]];
+          }
+          pos_new[i] = eq ? 1 : 0;  // equal to the entry before it
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < s; i += blockDim.x) {
+          const bool member = pos_new[i] || (i + 1 < s && pos_new[i + 1]);
+          int r = i;
+          while (r > 0 && pos_new[r]) --r;
+          rid[A][L[0][A][i]] = (uint16_t)(member ? r : RID_NONE + i);
+        }
+        __syncthreads();
+      }
+    }
+    int cur = 0;
+    for (int k = 0;; ++k) {
+      const int first = (1 << k) - 1;
+      if (first >= slots) break;
+      const int last = min(slots, (2 << k) - 1);
+      if (threadIdx.x == 0) any_split = 0;
+      __syncthreads();
+      // axis and split flag of each node of this level from its lists' ends
+      for (int j = first + (int)threadIdx.x; j < last; j += blockDim.x) {
+        if ((hinfo[j] & 3) == 0) continue;
+        const int lo = hlo[j], hi = hhi[j];
+        if (hi - lo <= t.leaf) continue;
+        double v[6];
+#pragma unroll
+        for (int A = 0; A < 3; ++A) {
+          v[A] = cs[A][seg[L[cur][A][lo]]];
+          v[3 + A] = cs[A][seg[L[cur][A][hi - 1]]];
+        }
+        int8_t ax, sp;
+        choose_axis(v, &ax, &sp);
+        if (sp) {
+          hinfo[j] = (uint8_t)((hinfo[j] & 0xF0) | 2 | (ax << 2));
+          any_split = 1;
+        }
+      }
+      __syncthreads();
+      if (!any_split) break;
+      // sides and new positions from each splitting node's list of its axis
+      for (int i = threadIdx.x; i < s; i += blockDim.x) {
+        const int j = nodeof[i];
+        if ((hinfo[j] & 3) == 2) {
+          const int mid = (hlo[j] + hhi[j]) / 2;
+          const int x = L[cur][(hinfo[j] >> 2) & 3][i];
+          side[x] = i < mid ? 1 : 0;
+          pos_new[x] = (uint16_t)i;
+        }
+      }
+      __syncthreads();
+      // stable partition of the three lists within every splitting node (one scan for the three)
+      const int i0 = (int)threadIdx.x * BL_E;
+      int f[3][BL_E], cnt[3], excl[3], tot[3];
+#pragma unroll
+      for (int A = 0; A < 3; ++A) {
+        cnt[A] = 0;
+#pragma unroll
+        for (int e = 0; e < BL_E; ++e) {
+          const int i = i0 + e;
+          f[A][e] = (i < s && (hinfo[nodeof[i]] & 3) == 2) ? side[L[cur][A][i]] : 0;
+          cnt[A] += f[A][e];
+        }
+      }
+      block_exclusive_scan_n<3>(cnt, excl, tot);
+#pragma unroll
+      for (int A = 0; A < 3; ++A) {
+        int run = excl[A];
+#pragma unroll
+        for (int e = 0; e < BL_E; ++e) {
+          const int i = i0 + e;
+          if (i < s) {
+            const int j = nodeof[i];
+            if ((hinfo[j] & 3) == 2 && hlo[j] == i) nbase[j - first][A] = (uint16_t)run;
+          }
+          run += f[A][e];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int A = 0; A < 3; ++A) {
+        int run = excl[A];
+#pragma unroll
+        for (int e = 0; e < BL_E; ++e) {
+          const int i = i0 + e;
+          if (i < s) {
+            const int j = nodeof[i];
+            int dest = i;
+            if ((hinfo[j] & 3) == 2) {
+              const int lo = hlo[j], hi = hhi[j], mid = (lo + hi) / 2;
+              const int left = run - nbase[j - first][A];
+              dest = f[A][e] ? lo + left : mid + (i - lo - left);
+            }
+            if ((unsigned)dest < (unsigned)s) L[cur ^ 1][A][dest] = L[cur][A][i];
+            else t.counters[C_OVERFLOW] = 1;
+          }
+          run += f[A][e];
+        }
+      }
+      __syncthreads();
+      cur ^= 1;
+      if (ties) {
+        // each run of equal coordinates within a child, in the order of the new positions
+        for (int A = 0; A < 3; ++A)
+          for (int i = threadIdx.x; i < s; i += blockDim.x) {
+            const int j = nodeof[i];
+            const int x = L[cur][A][i];
+            int dest = i;
+            const int r = rid[A][x];
+            if ((hinfo[j] & 3) == 2 && r < RID_NONE) {
+              const int lo = hlo[j], hi = hhi[j], mid = (lo + hi) / 2;
+              const int cl = i < mid ? lo : mid, ch = i < mid ? mid : hi;
+              int rs = i, re = i + 1;
+              while (rs > cl && rid[A][L[cur][A][rs - 1]] == r) --rs;
+              while (re < ch && rid[A][L[cur][A][re]] == r) ++re;
+              if (re - rs > 1) {
+                const int p = pos_new[x];
+                int rank = 0;
+                for (int q = rs; q < re; ++q) rank += pos_new[L[cur][A][q]] < p;
+                dest = rs + rank;
+              }
+            }
+            L[cur ^ 1][A][dest] = (uint16_t)x;
+          }
+        __syncthreads();
+        cur ^= 1;
+      }
+      // children of the splitting nodes, then every position's node for the next level
+      for (int j = first + (int)threadIdx.x; j < last; j += blockDim.x) {
+        if ((hinfo[j] & 3) != 2) continue;
+        const int ax = (hinfo[j] >> 2) & 3;
+        const int lo = hlo[j], hi = hhi[j], middle = (lo + hi) / 2;
+        hinfo[2 * j + 1] = (uint8_t)(1 | (ax << 4));
+        hlo[2 * j + 1] = (uint16_t)lo;
+        hhi[2 * j + 1] = (uint16_t)middle;
+        hinfo[2 * j + 2] = (uint8_t)(1 | (ax << 4));
+        hlo[2 * j + 2] = (uint16_t)middle;
+        hhi[2 * j + 2] = (uint16_t)hi;
+      }
+      __syncthreads();
+      for (int i = threadIdx.x; i < s; i += blockDim.x) {
+        const int j = nodeof[i];
+        if ((hinfo[j] & 3) == 2) nodeof[i] = (uint16_t)(i < hhi[2 * j + 1] ? 2 * j + 1 : 2 * j + 2);
+      }
+      __syncthreads();
+    }
+    // results: each leaf's order is its parent's list of the split axis (a root that did not split keeps its own)
+    int out[BL_E];
+    for (int e = 0; e < BL_E; ++e) {
+      const int i = (int)threadIdx.x + e * BL_THREADS;
+      out[e] = -1;
+      if (i < s) {
+        const int pa = (hinfo[nodeof[i]] >> 4) & 3;
+        if (pa != 3) out[e] = seg[L[cur][pa][i]];
+      }
+    }
+    __syncthreads();  // every read of the root's order is done before it is overwritten
+    for (int e = 0; e < BL_E; ++e) {
+      const int i = (int)threadIdx.x + e * BL_THREADS;
+      if (out[e] >= 0) t.order[a + i] = out[e];
+    }
+    const int hb = t.heap_base[b];
+    const int d0 = t.depth[root], tr = t.tree[root];
+    for (int j = threadIdx.x; j < slots; j += blockDim.x)
+      heap_node_out(t, root, hb, a, d0, tr, j, (hinfo[j] & 3) != 0, (hinfo[j] & 3) == 2, hlo[j], hhi[j]);
   }
 }
 
@@ -1594,6 +1846,7 @@ int tree_layout(Ctx& c, int buf, int64_t m64, int K, int leaf, TreeArrays& t) {
   need += 2 * al(3 * M * 8) + 2 * al(3 * M * 4) + al(M) + al((size_t)t.tile_cap * 3 * 4);
   need += al(3 * (size_t)t.rs_tiles * RS_BINS * 4) + al(3 * RS_BINS * 4);
   need += al(3 * M * 4) + 2 * al(RUNS * 4) + al(M) + al(M * 4) + al(3 * M * 4);  // runs
+  need += al(M * 4);                                                            // bottom phase on the lists
   OFW_HIP(c.tree[buf].reserve(need));
   char* base = (char*)c.tree[buf].ptr;
   auto carve = [&](size_t bytes) {
@@ -1654,6 +1907,7 @@ int tree_layout(Ctx& c, int buf, int64_t m64, int K, int leaf, TreeArrays& t) {
   t.run_len = (int32_t*)carve(RUNS * 4);
   t.tie = (uint8_t*)carve(M);
   t.inv_o = (int32_t*)carve(M * 4);
+  t.ord_inv = (int32_t*)carve(M * 4);
   t.invb = (int32_t*)carve(3 * M * 4);
   t.lists = c.sort_build ? 0 : 1;
   return OFW_OK;
@@ -1671,6 +1925,7 @@ int tree_build(Ctx& c, TreeArrays& t, const double* mx, const double* my, const 
   // top levels: while the largest node of a level can hold more than S_BOTTOM segments.  Level lvl reads the
   // order buffer of its parity and leaves its sorted nodes in the other one.
   int lbuf = 0;  // the list buffer of the current level
+  int lbuf0 = 0;  // the buffer of the roots' lists (a node at depth d holds its lists in lbuf0 ^ (d & 1))
   if (t.lists && m > 0) {
     // presort of the three axes' lists and the runs of equal coordinates in them
     const unsigned ge = (unsigned)std::min<int64_t>((m + 255) / 256, 4 * (int64_t)grid);
@@ -1681,6 +1936,7 @@ int tree_build(Ctx& c, TreeArrays& t, const double* mx, const double* my, const 
       hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)t.rs_tiles, 3), dim3(RS_THREADS), 0, c.stream, t, p, lbuf);
       lbuf ^= 1;
     }
+    lbuf0 = lbuf;
     hipLaunchKernelGGL(k_tie_scan, dim3(ge, 3), dim3(256), 0, c.stream, t, lbuf);
     hipLaunchKernelGGL(k_tie_mark, dim3(ge, 3), dim3(256), 0, c.stream, t, lbuf);
     OFW_HIP(hipGetLastError());
@@ -1700,7 +1956,6 @@ int tree_build(Ctx& c, TreeArrays& t, const double* mx, const double* my, const 
       hipLaunchKernelGGL(k_list_scatter, dim3(gt, 3), dim3(LS_THREADS), 0, c.stream, t, t.rval[lbuf],
                          t.rval[lbuf ^ 1]);
       lbuf ^= 1;
-      hipLaunchKernelGGL(k_tie_fix, dim3(grid), dim3(64 * FIX_WAVES), 0, c.stream, t, t.rval[lbuf]);
     } else {
       const int32_t* oin = orders[lvl & 1];
       int32_t* oout = orders[(lvl + 1) & 1];
@@ -1722,14 +1977,15 @@ int tree_build(Ctx& c, TreeArrays& t, const double* mx, const double* my, const 
         std::swap(pin, pout);
       }
     }
-    hipLaunchKernelGGL(k_level_next, dim3(1), dim3(PLAN_THREADS), 0, c.stream, t, lvl, bigs[lvl & 1],
-                       bigs[(lvl + 1) & 1]);
+    hipLaunchKernelGGL(k_level_next, dim3(t.lists ? 1 + FIX_BLOCKS : 1), dim3(PLAN_THREADS), 0, c.stream, t, lvl,
+                       bigs[lvl & 1], bigs[(lvl + 1) & 1], t.lists ? t.rval[lbuf] : nullptr);
     OFW_HIP(hipGetLastError());
   }
   if (!t.lists) hipLaunchKernelGGL(k_copy_strands, dim3(grid), dim3(256), 0, c.stream, t);
   // bottom phase
   hipLaunchKernelGGL(k_bottom_bases, dim3(1), dim3(PLAN_THREADS), 0, c.stream, t);
-  hipLaunchKernelGGL(k_bottom, dim3(grid), dim3(BOTTOM_THREADS), 0, c.stream, t, mx, my, mz);
+  if (t.lists) hipLaunchKernelGGL(k_bottom_lists, dim3(grid), dim3(BL_THREADS), 0, c.stream, t, lbuf0, mx, my, mz);
+  else hipLaunchKernelGGL(k_bottom, dim3(grid), dim3(BOTTOM_THREADS), 0, c.stream, t, mx, my, mz);
   OFW_HIP(hipGetLastError());
   // finish: pre-order layout from (lo, depth)
   const int n1 = m + 1;

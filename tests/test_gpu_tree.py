@@ -42,6 +42,18 @@ def tree_cases():
     cases.append(("unsplittable-depth-2", np.concatenate((spread, same)), 16))
     cases.append(("leaf-3000-depth-1", rng.random((5000, 3)), 3000))
     cases.append(("leaf-3000-depth-2", rng.random((9000, 3)), 3000))
+    # runs of equal coordinates on every axis (at most about 250 long) through the levels that split the
+    # presorted lists: each partition puts the runs back in the order of their new positions
+    cases.append(("runs-20000", np.round(rng.random((20000, 3)) * 128) / 128, 8))
+    cases.append(("runs-200000", np.round(rng.random((200000, 3)) * 1024) / 1024, 32))
+    # a run longer than the lists re-order (every z equal): the build is made with the per-level sorts
+    planar = rng.random((20000, 3))
+    planar[:, 2] = 0.0
+    cases.append(("planar-20000", planar, 32))
+    # the 821,874-segment loop wake: 32 rotor segments share the midpoint z = 1.5
+    loop = workloads.loop(rows=3744)
+    cases.append(("loop-3744", O.segment_arrays(loop["starts"], loop["directions"], loop["lengths"],
+                                                loop["gamma"])[3], 32))
     return cases
 
 
