@@ -65,11 +65,11 @@ struct TreeArrays {
   int32_t* order = nullptr;      // segment permutation (tree order -> segment)
   // list build: per-axis segment lists presorted once, split by stable partitions level after level
   int lists = 0;                 // 1: the top phase uses the lists (every node's segments stay in `order`)
-  int rs_passes = 0;             // radix passes of the presort (8 key digits + the tree-id digits)
+  int rs_passes = 0;             // radix passes of the presort (4 key digits + the tree-id digits)
   int rs_tiles = 0;              // radix tiles per axis
-  uint64_t* rkey[2] = {};        // 3 x m keys per buffer (axis-major)
+  uint32_t* rkey[2] = {};        // 3 x m keys per buffer (axis-major): coordinates rounded to float32
   int32_t* rval[2] = {};         // 3 x m segment ids per buffer; the lists live here after the presort
-  uint8_t* side = nullptr;       // per segment: 1 when it goes to the left child of its node
+  uint8_t* lside = nullptr;      // 3 x m: 1 where a list entry goes to the left child of its node
   int32_t* tcount = nullptr;     // tile_cap x 3 left counts of the partition
   int32_t* rcount = nullptr;     // 3 x 256 x rs_tiles digit counts of one presort pass, then their scans
   int32_t* rtotal = nullptr;     // 3 x 256 digit totals of the pass

@@ -46,6 +46,14 @@ static constexpr int FIX_BLOCKS = 64;                // workgroups of the re-ord
 static constexpr int32_t TIE_FLAG = INT32_MIN;       // list entry of a segment in a run of equal coordinates
 static constexpr int32_t SEG_MASK = INT32_MAX;       // the segment id of a list entry
 
+// Order-preserving 32-bit key of x rounded to float32 (round to nearest: a < b gives key(a) <= key(b)); -0.0 and
+// +0.0 share the key of +0.0.
+__device__ __forceinline__ uint32_t ordered_key32(double x) {
+  const float f = (float)x;
+  const uint32_t u = f == 0.0f ? 0u : __float_as_uint(f);
+  return (u >> 31) ? ~u : (u | 0x80000000u);
+}
+
 __device__ __forceinline__ uint64_t ordered_key(double x) {
   uint64_t u = (x == 0.0) ? 0ull : (uint64_t)__double_as_longlong(x);
   return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
@@ -835,8 +843,9 @@ static constexpr int RS_E = 8;                       // elements per thread
 static constexpr int RS_TILE = RS_THREADS * RS_E;    // elements per radix tile
 static constexpr int RS_WAVE = RS_TILE / (RS_THREADS / 64);  // elements per wavefront of a tile
 static constexpr int RS_BINS = 256;
-static constexpr int RS_KEY_PASSES = 8;
+static constexpr int RS_KEY_PASSES = 4;
 static constexpr int RS_MAX_PASSES = RS_KEY_PASSES + 3;
+static constexpr int FIX_RUN = 64;  // longest run of equal 32-bit keys the presort orders by its doubles
 static_assert(RS_WAVE % 64 == 0, "a wavefront ranks whole rounds of 64 elements");
 static_assert(RS_THREADS == RS_BINS, "one thread per digit");
 
@@ -851,7 +860,7 @@ __device__ __forceinline__ int tree_of(const int32_t* __restrict__ roots_lo, int
   return lo;
 }
 
-__device__ __forceinline__ int rs_digit(uint64_t key, int tree, int pass) {
+__device__ __forceinline__ int rs_digit(uint32_t key, int tree, int pass) {
   return pass < RS_KEY_PASSES ? (int)((key >> (8 * pass)) & 0xff) : ((tree >> (8 * (pass - RS_KEY_PASSES))) & 0xff);
 }
 
@@ -862,7 +871,7 @@ __global__ void k_rs_init(TreeArrays t, const double* __restrict__ mx, const dou
   const double* __restrict__ coord = A == 0 ? mx : (A == 1 ? my : mz);
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
     const size_t g = (size_t)A * m + i;
-    t.rkey[0][g] = ordered_key(coord[i]);
+    t.rkey[0][g] = ordered_key32(coord[i]);
     t.rval[0][g] = i;
     if (A == 0) {
       t.tie[i] = 0;
@@ -877,7 +886,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_count(TreeArrays t, int pass,
   const int m = (int)t.m, A = blockIdx.y, tile = blockIdx.x;
   h[threadIdx.x] = 0;
   __syncthreads();
-  const uint64_t* __restrict__ kin = t.rkey[src] + (size_t)A * m;
+  const uint32_t* __restrict__ kin = t.rkey[src] + (size_t)A * m;
   const int32_t* __restrict__ vin = t.rval[src] + (size_t)A * m;
   const int i0 = tile * RS_TILE;
 #pragma unroll 4
@@ -930,7 +939,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(TreeArrays t, int pas
   __shared__ int s_cnt[RS_THREADS / 64][RS_BINS];  // running counts of each wavefront, then its digit bases
   __shared__ int s_base[RS_BINS];                  // destination of the digit's first element of the tile
   __shared__ int s_start[RS_BINS];                 // the digit's first place in the tile's digit order
-  __shared__ uint64_t s_key[RS_TILE];
+  __shared__ uint32_t s_key[RS_TILE];
   __shared__ int32_t s_val[RS_TILE];
   __shared__ uint8_t s_dig[RS_TILE];
   const int m = (int)t.m, A = blockIdx.y, tile = blockIdx.x;
@@ -943,9 +952,9 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(TreeArrays t, int pas
     s_base[d] = (int)((int64_t)A * m) + below + t.rcount[((size_t)A * RS_BINS + d) * t.rs_tiles + tile];
   }
   __syncthreads();
-  const uint64_t* __restrict__ kin = t.rkey[src] + (size_t)A * m;
+  const uint32_t* __restrict__ kin = t.rkey[src] + (size_t)A * m;
   const int32_t* __restrict__ vin = t.rval[src] + (size_t)A * m;
-  uint64_t key[RS_E];
+  uint32_t key[RS_E];
   int32_t val[RS_E];
   int dig[RS_E], rank[RS_E];
   const int w0 = tile * RS_TILE + w * RS_WAVE;
@@ -954,7 +963,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(TreeArrays t, int pas
   for (int r = 0; r < RS_E; ++r) {
     const int i = w0 + r * 64 + lane;
     const bool act = i < m;
-    key[r] = act ? kin[i] : 0ull;
+    key[r] = act ? kin[i] : 0u;
     val[r] = act ? vin[i] : 0;
   }
 #pragma unroll
@@ -1006,7 +1015,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(TreeArrays t, int pas
     }
   }
   __syncthreads();
-  uint64_t* __restrict__ kout = t.rkey[1 - src];
+  uint32_t* __restrict__ kout = t.rkey[1 - src];
   int32_t* __restrict__ vout = t.rval[1 - src];
   const int cnt = min(RS_TILE, m - tile * RS_TILE);
   for (int j = threadIdx.x; j < cnt; j += RS_THREADS) {
@@ -1022,22 +1031,71 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(TreeArrays t, int pas
 }
 
 // Positions i < j of a sorted list hold the same key of the same tree.
-__device__ __forceinline__ bool same_run(const TreeArrays& t, const uint64_t* __restrict__ key, int i, int j) {
-  return key[i] == key[j] && (t.K == 1 || tree_of(t.roots_lo, t.K, i) == tree_of(t.roots_lo, t.K, j));
+__device__ __forceinline__ bool same_tree(const TreeArrays& t, int i, int j) {
+  return t.K == 1 || tree_of(t.roots_lo, t.K, i) == tree_of(t.roots_lo, t.K, j);
 }
 
-// The runs of equal keys of the roots' sorted lists (buffer buf): each run's members, axis and length are
+// The presort orders by the 32-bit keys of the coordinates rounded to float32; within a run of equal keys (one
+// tree) the entries are put in the order of (coordinate, segment id), the order of a stable sort by the
+// coordinate.  Values from buffer buf to buf ^ 1 (the keys stay: the run's keys are equal).  A run longer than
+// FIX_RUN stops the list build.
+__global__ void k_rs_fix(TreeArrays t, int buf, const double* __restrict__ mx, const double* __restrict__ my,
+                         const double* __restrict__ mz) {
+  const int m = (int)t.m, A = blockIdx.y;
+  const double* __restrict__ coord = A == 0 ? mx : (A == 1 ? my : mz);
+  const uint32_t* __restrict__ key = t.rkey[buf] + (size_t)A * m;
+  const int32_t* __restrict__ in = t.rval[buf] + (size_t)A * m;
+  int32_t* __restrict__ out = t.rval[buf ^ 1] + (size_t)A * m;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+    const uint32_t k = key[i];
+    const int s = in[i];
+    int rs = i, re = i + 1;
+    while (rs > 0 && i - rs <= FIX_RUN && key[rs - 1] == k && same_tree(t, rs - 1, i)) --rs;
+    while (re < m && re - i <= FIX_RUN && key[re] == k && same_tree(t, re, i)) ++re;
+    if (re - rs == 1) {
+      out[i] = s;
+      continue;
+    }
+    if (re - rs > FIX_RUN) {
+      out[i] = s;
+      t.counters[C_DUP] = 1;
+      t.counters[C_OVERFLOW] = 1;
+      continue;
+    }
+    const double c = coord[s];
+    int rank = 0;
+    for (int j = rs; j < re; ++j) {
+      const int sj = in[j];
+      const double cj = coord[sj];
+      rank += (cj < c || (cj == c && sj < s)) ? 1 : 0;
+    }
+    out[rs + rank] = s;
+  }
+}
+
+// Positions i < j of a sorted list hold the same coordinate of the same tree.
+__device__ __forceinline__ bool same_run(const TreeArrays& t, const uint32_t* __restrict__ key,
+                                         const int32_t* __restrict__ val, const double* __restrict__ coord, int i,
+                                         int j) {
+  return key[i] == key[j] && coord[val[i]] == coord[val[j]] && same_tree(t, i, j);
+}
+
+// The runs of equal coordinates of the roots' sorted lists (keys in kbuf, lists in buf): each run's members, axis
+// and length are
 // recorded (in any order: runs are disjoint, so the re-order does not depend on it), its segments marked, and
 // their positions in `order` (the identity at the roots) and in the axis's list set.  A run longer than RUN_MAX
 // stops the list build.
-__global__ void k_tie_scan(TreeArrays t, int buf) {
+__global__ void k_tie_scan(TreeArrays t, int kbuf, int buf, const double* __restrict__ mx,
+                           const double* __restrict__ my, const double* __restrict__ mz) {
   const int m = (int)t.m, A = blockIdx.y;
-  const uint64_t* __restrict__ key = t.rkey[buf] + (size_t)A * m;
+  const double* __restrict__ coord = A == 0 ? mx : (A == 1 ? my : mz);
+  const uint32_t* __restrict__ key = t.rkey[kbuf] + (size_t)A * m;
   const int32_t* __restrict__ val = t.rval[buf] + (size_t)A * m;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i + 1 < m; i += gridDim.x * blockDim.x) {
-    if (!same_run(t, key, i, i + 1) || (i > 0 && same_run(t, key, i - 1, i))) continue;  // first entries only
+    if (!same_run(t, key, val, coord, i, i + 1) || (i > 0 && same_run(t, key, val, coord, i - 1, i)))
+      continue;  // first entries only
     int len = 2;
-    while (i + len < m && len <= RUN_MAX && same_run(t, key, i, i + len)) ++len;
+    while (i + len < m && len <= RUN_MAX && same_run(t, key, val, coord, i, i + len)) ++len;
     if (len > RUN_MAX) {
       t.counters[C_DUP] = 1;
       t.counters[C_OVERFLOW] = 1;
@@ -1084,55 +1142,69 @@ __device__ __forceinline__ void list_axis(const TreeArrays& t, const int32_t* __
   choose_axis(v, axis, split);
 }
 
-// Each tile of the level's big nodes: the node's sorted order (its list of the split axis) into `order`, and the
-// side of each of its segments.
-__global__ __launch_bounds__(256) void k_list_apply(TreeArrays t, const int32_t* __restrict__ lst,
+// Per tile and axis: the side of each of the tile's list entries (lside) and how many go to the left child.  The
+// node's sorted order (vortex_tree.py:95-101) is its list of the split axis, cut at (a+b)//2: an entry goes left
+// when its coordinate on that axis is below the median entry's, or equal to it and before it in that list (then
+// it is in a run of equal coordinates, whose list positions are kept).  The node's first unit records its axis
+// and split flag for the scatter and the next plan.
+static constexpr int LC_E = TILE / 256;  // entries per thread of the count kernel
+__global__ __launch_bounds__(256) void k_list_count(TreeArrays t, const int32_t* __restrict__ lst,
                                                     const double* __restrict__ mx, const double* __restrict__ my,
                                                     const double* __restrict__ mz) {
   const int ntiles = live_count(t.counters, C_TILES);
-  const int m = (int)t.m;
+  const int m = (int)t.m, A = blockIdx.y;
+  const double* cs[3] = {mx, my, mz};
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int node = t.tile_node[tile];
-    const int lo = t.lo[node], hi = t.hi[node];
     const int p0 = t.tile_p0[tile], p1 = t.tile_p1[tile];
+    const int32_t* __restrict__ src = lst + (size_t)A * m;
+    int q[LC_E];  // the tile's entries (loaded beside the node's)
+#pragma unroll
+    for (int e = 0; e < LC_E; ++e) {
+      const int p = p0 + (int)threadIdx.x + e * 256;
+      q[e] = p < p1 ? (src[p] & SEG_MASK) : m;
+    }
+    const int lo = t.lo[node], hi = t.hi[node];
+    const int mid = (int)(((int64_t)lo + hi) / 2);
+    int med[3];
+#pragma unroll
+    for (int B = 0; B < 3; ++B) med[B] = lst[(size_t)B * m + mid] & SEG_MASK;
     int8_t axis, split;
     list_axis(t, lst, lo, hi, mx, my, mz, &axis, &split);
-    if (p0 == lo && threadIdx.x == 0) {  // the node's first tile records it for the next plan
+    if (p0 == lo && A == 0 && threadIdx.x == 0) {
       t.axis[node] = axis;
       t.split[node] = split;
     }
     if (!split) continue;
-    const int mid = (int)(((int64_t)lo + hi) / 2);
-    const bool small_l = mid - lo <= S_BOTTOM, small_r = hi - mid <= S_BOTTOM;  // children that leave the lists
-    const int32_t* __restrict__ src = lst + (size_t)axis * m;
-    for (int p = p0 + (int)threadIdx.x; p < p1; p += blockDim.x) {
-      const int v = src[p];
-      const int s = v & SEG_MASK;
-      if (s < m) {
-        t.order[p] = s;
-        t.side[s] = p < mid ? 1 : 0;
-        if (v < 0) t.inv_o[s] = p;
-        if (p < mid ? small_l : small_r) t.ord_inv[s] = p;
-      } else {
-        t.counters[C_OVERFLOW] = 1;
-      }
-    }
-  }
-}
-
-// Per tile and axis: how many of the tile's list entries go to the left child.
-__global__ __launch_bounds__(256) void k_list_count(TreeArrays t, const int32_t* __restrict__ lst) {
-  const int ntiles = live_count(t.counters, C_TILES);
-  const int m = (int)t.m, A = blockIdx.y;
-  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int node = t.tile_node[tile];
-    if (!t.split[node]) continue;
-    const int p0 = t.tile_p0[tile], p1 = t.tile_p1[tile];
-    const int32_t* __restrict__ src = lst + (size_t)A * m;
+    const double* __restrict__ ca = cs[axis];
+    const int s_med = med[axis];
+    const double c_med = ca[s_med < m ? s_med : 0];
+    const int32_t* __restrict__ run_pos = t.invb + (size_t)axis * m;
+    uint8_t* __restrict__ side = t.lside + (size_t)A * m;
     int c = 0;
-    for (int p = p0 + (int)threadIdx.x; p < p1; p += blockDim.x) {
-      const int q = src[p] & SEG_MASK;
-      c += q < m ? t.side[q] : 0;
+    if (A == axis) {
+#pragma unroll
+      for (int e = 0; e < LC_E; ++e) {
+        const int p = p0 + (int)threadIdx.x + e * 256;
+        if (p < p1) {
+          const int f = p < mid ? 1 : 0;
+          side[p] = (uint8_t)f;
+          c += f;
+        }
+      }
+    } else {
+      double x[LC_E];
+#pragma unroll
+      for (int e = 0; e < LC_E; ++e) x[e] = q[e] < m ? ca[q[e]] : 0.0;
+#pragma unroll
+      for (int e = 0; e < LC_E; ++e) {
+        const int p = p0 + (int)threadIdx.x + e * 256;
+        if (p < p1 && q[e] < m) {
+          const int f = (x[e] < c_med || (x[e] == c_med && q[e] != s_med && run_pos[q[e]] < mid)) ? 1 : 0;
+          side[p] = (uint8_t)f;
+          c += f;
+        }
+      }
     }
     int total;
     block_exclusive_scan(c, &total);
@@ -1140,7 +1212,8 @@ __global__ __launch_bounds__(256) void k_list_count(TreeArrays t, const int32_t*
   }
 }
 
-// Per tile and axis: the stable partition of the node's list into its children's ranges.
+// Per tile and axis: the stable partition of the node's list into its children's ranges; the units of the split
+// axis write the node's sorted order.
 static constexpr int LS_THREADS = 1024;
 static constexpr int LS_E = TILE / LS_THREADS;
 __global__ __launch_bounds__(LS_THREADS) void k_list_scatter(TreeArrays t, const int32_t* __restrict__ lst_in,
@@ -1150,16 +1223,13 @@ __global__ __launch_bounds__(LS_THREADS) void k_list_scatter(TreeArrays t, const
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int node = t.tile_node[tile];
     if (!t.split[node]) continue;
+    const int axis = t.axis[node];
     const int lo = t.lo[node], hi = t.hi[node];
     const int p0 = t.tile_p0[tile], p1 = t.tile_p1[tile];
     const int mid = (int)(((int64_t)lo + hi) / 2);
-    // left entries of the node's tiles before this one (the node's tiles are consecutive)
-    const int first = tile - (p0 - lo) / TILE;
-    int before = 0;
-    for (int q = first + (int)threadIdx.x; q < tile; q += blockDim.x) before += t.tcount[3 * q + A];
-    int prefix;
-    block_exclusive_scan(before, &prefix);
+    const bool small_l = mid - lo <= S_BOTTOM, small_r = hi - mid <= S_BOTTOM;  // children that leave the lists
     const int32_t* __restrict__ src = lst_in + (size_t)A * m;
+    const uint8_t* __restrict__ side = t.lside + (size_t)A * m;
     int s[LS_E], f[LS_E];
     int mine = 0;
     const int e0 = p0 + (int)threadIdx.x * LS_E;
@@ -1167,14 +1237,15 @@ __global__ __launch_bounds__(LS_THREADS) void k_list_scatter(TreeArrays t, const
     for (int e = 0; e < LS_E; ++e) {
       const int p = e0 + e;
       s[e] = p < p1 ? src[p] : 0;
-    }
-#pragma unroll
-    for (int e = 0; e < LS_E; ++e) {
-      const int p = e0 + e;
-      const int q = s[e] & SEG_MASK;
-      f[e] = (p < p1 && q < m) ? t.side[q] : 0;
+      f[e] = p < p1 ? side[p] : 0;
       mine += f[e];
     }
+    // left entries of the node's tiles before this one (the node's tiles are consecutive)
+    const int first = tile - (p0 - lo) / TILE;
+    int before = 0;
+    for (int q = first + (int)threadIdx.x; q < tile; q += blockDim.x) before += t.tcount[3 * q + A];
+    int prefix;
+    block_exclusive_scan(before, &prefix);
     int total;
     int left = prefix + block_exclusive_scan(mine, &total);
     int32_t* __restrict__ dst = lst_out + (size_t)A * m;
@@ -1184,9 +1255,15 @@ __global__ __launch_bounds__(LS_THREADS) void k_list_scatter(TreeArrays t, const
       const int p = e0 + e;
       if (p < p1) {
         const int dest = f[e] ? lo + left : mid + (p - lo - left);
-        if (dest >= lo && dest < hi) {
+        const int q = s[e] & SEG_MASK;
+        if (dest >= lo && dest < hi && q < m) {
           dst[dest] = s[e];
-          if (s[e] < 0 && (s[e] & SEG_MASK) < m) inv[s[e] & SEG_MASK] = dest;
+          if (s[e] < 0) inv[q] = dest;
+          if (A == axis) {  // the node's sorted order
+            t.order[p] = q;
+            if (s[e] < 0) t.inv_o[q] = p;
+            if (p < mid ? small_l : small_r) t.ord_inv[q] = p;
+          }
         } else {
           t.counters[C_OVERFLOW] = 1;
         }
@@ -1843,7 +1920,7 @@ int tree_layout(Ctx& c, int buf, int64_t m64, int K, int leaf, TreeArrays& t) {
   t.rs_tiles = (int)((M + RS_TILE - 1) / RS_TILE);
   t.rs_passes = RS_KEY_PASSES + (K <= 1 ? 0 : (K <= 256 ? 1 : (K <= 65536 ? 2 : 3)));
   const size_t RUNS = 3 * M / 2 + 1;
-  need += 2 * al(3 * M * 8) + 2 * al(3 * M * 4) + al(M) + al((size_t)t.tile_cap * 3 * 4);
+  need += 2 * al(3 * M * 4) + 2 * al(3 * M * 4) + al(3 * M) + al((size_t)t.tile_cap * 3 * 4);
   need += al(3 * (size_t)t.rs_tiles * RS_BINS * 4) + al(3 * RS_BINS * 4);
   need += al(3 * M * 4) + 2 * al(RUNS * 4) + al(M) + al(M * 4) + al(3 * M * 4);  // runs
   need += al(M * 4);                                                            // bottom phase on the lists
@@ -1896,9 +1973,9 @@ int tree_layout(Ctx& c, int buf, int64_t m64, int K, int leaf, TreeArrays& t) {
   t.p_node = (int32_t*)carve(C * 4);
   t.p_irank = (int32_t*)carve(C * 4);
   t.tree_first = (int32_t*)carve((K + 1) * 4);
-  for (int b = 0; b < 2; ++b) t.rkey[b] = (uint64_t*)carve(3 * M * 8);
+  for (int b = 0; b < 2; ++b) t.rkey[b] = (uint32_t*)carve(3 * M * 4);
   for (int b = 0; b < 2; ++b) t.rval[b] = (int32_t*)carve(3 * M * 4);
-  t.side = (uint8_t*)carve(M);
+  t.lside = (uint8_t*)carve(3 * M);
   t.tcount = (int32_t*)carve((size_t)t.tile_cap * 3 * 4);
   t.rcount = (int32_t*)carve(3 * (size_t)t.rs_tiles * RS_BINS * 4);
   t.rtotal = (int32_t*)carve(3 * RS_BINS * 4);
@@ -1936,8 +2013,10 @@ int tree_build(Ctx& c, TreeArrays& t, const double* mx, const double* my, const 
       hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)t.rs_tiles, 3), dim3(RS_THREADS), 0, c.stream, t, p, lbuf);
       lbuf ^= 1;
     }
+    hipLaunchKernelGGL(k_rs_fix, dim3(ge, 3), dim3(256), 0, c.stream, t, lbuf, mx, my, mz);
+    hipLaunchKernelGGL(k_tie_scan, dim3(ge, 3), dim3(256), 0, c.stream, t, lbuf, lbuf ^ 1, mx, my, mz);
+    lbuf ^= 1;
     lbuf0 = lbuf;
-    hipLaunchKernelGGL(k_tie_scan, dim3(ge, 3), dim3(256), 0, c.stream, t, lbuf);
     hipLaunchKernelGGL(k_tie_mark, dim3(ge, 3), dim3(256), 0, c.stream, t, lbuf);
     OFW_HIP(hipGetLastError());
   }
@@ -1951,8 +2030,7 @@ int tree_build(Ctx& c, TreeArrays& t, const double* mx, const double* my, const 
     const int64_t tile_bound = (int64_t)m / TILE + ((int64_t)t.K << std::min(lvl, 30)) + 1;
     if (t.lists) {
       const unsigned gt = (unsigned)std::min<int64_t>(2 * (int64_t)grid, tile_bound);
-      hipLaunchKernelGGL(k_list_apply, dim3(gt), dim3(256), 0, c.stream, t, t.rval[lbuf], mx, my, mz);
-      hipLaunchKernelGGL(k_list_count, dim3(gt, 3), dim3(256), 0, c.stream, t, t.rval[lbuf]);
+      hipLaunchKernelGGL(k_list_count, dim3(gt, 3), dim3(256), 0, c.stream, t, t.rval[lbuf], mx, my, mz);
       hipLaunchKernelGGL(k_list_scatter, dim3(gt, 3), dim3(LS_THREADS), 0, c.stream, t, t.rval[lbuf],
                          t.rval[lbuf ^ 1]);
       lbuf ^= 1;
