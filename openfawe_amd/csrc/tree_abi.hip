@@ -69,7 +69,7 @@ int tree_pipeline_build(Ctx& c, const TreeCall& tc, TreeArrays& t, TreeDevice& t
   if (st) return st;
   OFW_HIP(c.mark());
   st = tree_moments(c, 1, t, tc.soff, tc.starts, tc.directions, tc.lengths, tc.gamma, tc.radius, tc.model,
-                    tc.clearance, td, false);
+                    tc.clearance, td, false, tc.precision == OFW_FP64);
   if (st) return st;
   OFW_HIP(c.mark());
   return OFW_OK;
@@ -278,7 +278,8 @@ int ofw_tree_nodes(ofw_ctx* x, int64_t m, const double* starts, const double* di
   td.out.w1 = pre[6];
   td.out.t2 = pre[7];
   td.out.w2 = pre[8];
-  st = ofw::tree_moments(c, 1, t, offsets, starts, directions, lengths, gamma, radius, OFW_MODEL_NONE, 0.0, td, true);
+  st = ofw::tree_moments(c, 1, t, offsets, starts, directions, lengths, gamma, radius, OFW_MODEL_NONE, 0.0, td, true,
+                         false);
   if (st) return st;
   OFW_HIP(hipMemcpyAsync(pre[0], td.centre, (size_t)count * 3 * 8, hipMemcpyDeviceToDevice, c.stream));
   OFW_HIP(hipMemcpyAsync(pre[2], td.core, (size_t)count * 8, hipMemcpyDeviceToDevice, c.stream));

@@ -37,7 +37,7 @@ int tree_prep(Ctx& c, int64_t m, const double* starts, const double* directions,
 // Node quantities and traversal records of a built forest; `offsets` are the trees' segment offsets (K+1).
 int tree_moments(Ctx& c, int buf, const TreeArrays& t, const std::vector<int64_t>& offsets, const double* starts,
                  const double* directions, const double* lengths, const double* gamma, const double* radius, int model,
-                 double clearance, TreeDevice& td, bool want_outputs);
+                 double clearance, TreeDevice& td, bool want_outputs, bool want_rec64);
 int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, const EvalShape& shape, int precision,
                   const double* points, int model, int raise_singular, const int32_t* ex_ptr, const int32_t* ex_seg,
                   int64_t n_ex, double theta, int expansion_order, double* out, int8_t* flags, int64_t* far_counts,

@@ -458,11 +458,19 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_tree_eval(ForestTables ft, 
 // with scalar loads.  Every lane evaluates the pair and the lanes that opened the leaf select the result
 // (vortex_tree.py:538-557 for each of them).  A grid-stride
 // loop covers the work items, whose count is known on the device only.
+// The near field's segments in tree order: FP64 records (FP64 evaluations), or the starts and, for the few
+// pairs that take the FP64 path, the engine inputs their records are packed from.
+struct NearSegs {
+  const double* rec64;
+  const double* start[3];
+  SegSource src;
+};
+
 template <int MODEL, int PREC>
 __global__ __launch_bounds__(64 * NEAR_WAVES) void k_tree_near(ForestTables ft, const double* __restrict__ pts,
                                                    const int32_t* __restrict__ ex_ptr,
                                                    const int32_t* __restrict__ ex_pos,
-                                                   const double* __restrict__ rec64, const float* __restrict__ rec32,
+                                                   NearSegs segs, const float* __restrict__ rec32,
                                                    const double* __restrict__ accr, const double* __restrict__ fr64,
                                                    const float* __restrict__ fr32, int order,
                                                    int raise_singular, const NearEntry* __restrict__ list,
@@ -480,6 +488,8 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_tree_near(ForestTables ft, 
   constexpr int SEGW = PREC == OFW_FP64 ? 1 : 64;
   __shared__ float s_seg[NEAR_WAVES][SEGW][REC32];
   __shared__ double s_start[NEAR_WAVES][SEGW][4];
+  // below FP64, the FP64 record of the segment whose pairs take the FP64 path (the same for every lane)
+  __shared__ double s_rec[NEAR_WAVES][REC64];
   const far_t* __restrict__ frec = PREC == OFW_FP64 ? (const far_t*)fr64 : (const far_t*)fr32;
   const int wv = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
@@ -591,7 +601,8 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_tree_near(ForestTables ft, 
           }
           const bool use = open && !hit;
           double vx = 0.0, vy = 0.0, vz = 0.0;
-          const int fl = pair_fast64<MODEL>(x0, x1, x2, rec64 + (size_t)s * REC64, raise_singular, use, vx, vy, vz);
+          const int fl = pair_fast64<MODEL>(x0, x1, x2, segs.rec64 + (size_t)s * REC64, raise_singular, use, vx, vy,
+                                            vz);
           ux = use ? ux + (acc_t)vx : ux;
           uy = use ? uy + (acc_t)vy : uy;
           uz = use ? uz + (acc_t)vz : uz;
@@ -607,9 +618,8 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_tree_near(ForestTables ft, 
           wave_lds_sync();  // the previous window's reads come first
           if (s < ft.seg_count) {
             const float4* f4 = reinterpret_cast<const float4*>(rec32 + (size_t)s * REC32);
-            const double* r = rec64 + (size_t)s * REC64;
             const float4 fa = f4[0], fb = f4[1];
-            const double qx = r[R_SX], qy = r[R_SY], qz = r[R_SZ];
+            const double qx = segs.start[0][s], qy = segs.start[1][s], qz = segs.start[2][s];
             float4* d4 = reinterpret_cast<float4*>(&s_seg[wv][lane][0]);
             d4[0] = fa;
             d4[1] = fb;
@@ -649,10 +659,18 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_tree_near(ForestTables ft, 
           ly += add ? vy : 0.0f;
           lz += add ? vz : 0.0f;
           const bool slow = use && need64;
-          if (wave_any(slow)) {
+          const unsigned long long slow_lanes = __builtin_amdgcn_ballot_w64(slow);
+          if (slow_lanes) {
+            // the segment's FP64 record, packed from the engine inputs by one lane
+            double* r = &s_rec[wv][0];
+            if (lane == (int)__builtin_ctzll(slow_lanes)) {
+              const SegSource& q = segs.src;
+              pack_record(q.order[s], q.starts, q.directions, q.lengths, q.gamma, q.radius, q.model, r, nullptr);
+            }
+            wave_lds_sync();
             if (slow) {
               double wx = 0.0, wy = 0.0, wz = 0.0;
-              const int fl = pair_fast64<MODEL>(x0, x1, x2, rec64 + (size_t)s * REC64, raise_singular, wx, wy, wz);
+              const int fl = pair_fast64<MODEL>(x0, x1, x2, r, raise_singular, wx, wy, wz);
               if (fl != OFW_FLAG_OK) {
                 if (flag == OFW_FLAG_OK) flag = fl;
               } else {
@@ -661,6 +679,7 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_tree_near(ForestTables ft, 
                 uz += (acc_t)wz;
               }
             }
+            wave_lds_sync();  // every lane has read the record before the next one is packed
           }
         }
         ux += (acc_t)lx;
@@ -942,16 +961,17 @@ static void launch_near(int precision, dim3 grid, hipStream_t st, const ForestTa
                         const int32_t* ex_ptr, const int32_t* ex_pos, const TreeDevice& td, int order, int raise,
                         const NearEntry* list, const int32_t* list_count, const int32_t* chunk_off,
                         const int32_t* item_wave, int slots, double* cpart, int8_t* cflag, int32_t* cnear) {
+  const NearSegs segs{td.rec64, {td.start[0], td.start[1], td.start[2]}, td.src};
   if (precision == OFW_FP64)
-    hipLaunchKernelGGL((k_tree_near<MODEL, OFW_FP64>), grid, dim3(64 * NEAR_WAVES), 0, st, ft, pts, ex_ptr, ex_pos, td.rec64,
+    hipLaunchKernelGGL((k_tree_near<MODEL, OFW_FP64>), grid, dim3(64 * NEAR_WAVES), 0, st, ft, pts, ex_ptr, ex_pos, segs,
                        td.rec32, td.acc, td.far64, td.far32, order, raise, list, list_count, chunk_off, item_wave, slots, cpart,
                        cflag, cnear);
   else if (precision == OFW_MIXED)
-    hipLaunchKernelGGL((k_tree_near<MODEL, OFW_MIXED>), grid, dim3(64 * NEAR_WAVES), 0, st, ft, pts, ex_ptr, ex_pos, td.rec64,
+    hipLaunchKernelGGL((k_tree_near<MODEL, OFW_MIXED>), grid, dim3(64 * NEAR_WAVES), 0, st, ft, pts, ex_ptr, ex_pos, segs,
                        td.rec32, td.acc, td.far64, td.far32, order, raise, list, list_count, chunk_off, item_wave, slots, cpart,
                        cflag, cnear);
   else
-    hipLaunchKernelGGL((k_tree_near<MODEL, OFW_FP32>), grid, dim3(64 * NEAR_WAVES), 0, st, ft, pts, ex_ptr, ex_pos, td.rec64,
+    hipLaunchKernelGGL((k_tree_near<MODEL, OFW_FP32>), grid, dim3(64 * NEAR_WAVES), 0, st, ft, pts, ex_ptr, ex_pos, segs,
                        td.rec32, td.acc, td.far64, td.far32, order, raise, list, list_count, chunk_off, item_wave, slots, cpart,
                        cflag, cnear);
 }

@@ -46,8 +46,9 @@ __global__ __launch_bounds__(256) void k_tree_prep(int m, const double* __restri
   mz[s] = mid[2];
 }
 
-// Tree-order copies: the moment inputs (start, end, alpha, length vector, radius, gamma) and the pair
-// records of the traversal's near field.
+// Tree-order copies: the moment inputs (start, length vector, radius, gamma) and the pair records of the
+// traversal's near field (the FP64 records only when rec64 is given: the FP32 near field rebuilds the few it
+// needs from the engine inputs).
 __global__ __launch_bounds__(256) void k_tree_permute(int m, const int32_t* __restrict__ order,
                                                       const double* __restrict__ starts,
                                                       const double* __restrict__ directions,
@@ -66,13 +67,11 @@ __global__ __launch_bounds__(256) void k_tree_permute(int m, const int32_t* __re
     const double st = starts[3 * (size_t)s + c];
     const double lv = directions[3 * (size_t)s + c] * L;
     mi.s[c][k] = st;
-    mi.e[c][k] = st + lv;
-    mi.a[c][k] = g * lv;
     mi.l[c][k] = lv;
   }
   mi.rad[k] = radius[s];
   mi.gam[k] = g;
-  pack_record(s, starts, directions, lengths, gamma, radius, model, rec64 + (size_t)k * REC64,
+  pack_record(s, starts, directions, lengths, gamma, radius, model, rec64 ? rec64 + (size_t)k * REC64 : nullptr,
               rec32 + (size_t)k * REC32);
 }
 
@@ -215,7 +214,7 @@ __device__ void leaf_job(const MomentInputs& mi, NodeRef v, unsigned long long* 
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         sh[c][hl] = mi.s[c][base + hl];
-        sh[3 + c][hl] = mi.e[c][base + hl];
+        sh[3 + c][hl] = mi.end(c, base + hl);
       }
     }
     wave_lds_sync();
@@ -254,7 +253,7 @@ __device__ void leaf_job(const MomentInputs& mi, NodeRef v, unsigned long long* 
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
         sv[q] = mi.s[q][k];
-        ev[q] = mi.e[q][k];
+        ev[q] = mi.end(q, k);
       }
       const double s0 = sv[0] - c[0], s1 = sv[1] - c[1], s2 = sv[2] - c[2];
       const double qs = s0 * s0 + s1 * s1 + s2 * s2;
@@ -267,7 +266,7 @@ __device__ void leaf_job(const MomentInputs& mi, NodeRef v, unsigned long long* 
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
         sh[q][hl] = 0.5 * (sv[q] + ev[q]) - c[q];
-        sh[3 + q][hl] = mi.a[q][k];
+        sh[3 + q][hl] = mi.alpha(q, k);
         sh[6 + q][hl] = mi.l[q][k];
       }
       sh[9][hl] = mi.gam[k];
@@ -390,7 +389,7 @@ __global__ __launch_bounds__(BOTTOM_MT) void k_moments_bottom(TreeArrays t, Mome
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
           sv[q] = mi.s[q][kk];
-          ev[q] = mi.e[q][kk];
+          ev[q] = mi.end(q, kk);
         }
       int i = 0;
       while (wave_any(live)) {
@@ -511,7 +510,7 @@ __global__ __launch_bounds__(256) void k_bound_top(TreeArrays t, MomentForest mf
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
         s[j][q] = valid ? mi.s[q][kk[j]] : 0.0;
-        e[j][q] = valid ? mi.e[q][kk[j]] : 0.0;
+        e[j][q] = valid ? mi.end(q, kk[j]) : 0.0;
       }
     }
     for (int lev = 0;; ++lev) {
@@ -674,7 +673,7 @@ __global__ __launch_bounds__(256) void k_moments_final(TreeArrays t, NodeScratch
 
 int tree_moments(Ctx& c, int buf, const TreeArrays& t, const std::vector<int64_t>& offsets, const double* starts,
                  const double* directions, const double* lengths, const double* gamma, const double* radius, int model,
-                 double clearance, TreeDevice& td, bool want_outputs) {
+                 double clearance, TreeDevice& td, bool want_outputs, bool want_rec64) {
   const int m = (int)t.m, cap = t.cap, K = t.K;
   // per tree: the cut depth and the subtrees below it (at most one per path of that depth); the table also
   // clears the per-level counters of the top lists
@@ -699,7 +698,7 @@ int tree_moments(Ctx& c, int buf, const TreeArrays& t, const std::vector<int64_t
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const size_t M = (size_t)std::max(m, 1);
   const size_t C = (size_t)cap;
-  size_t need = 14 * al(M * 8) + al(M * REC64 * 8) + al(M * REC32 * 4) + al(table.size() * 4);
+  size_t need = 8 * al(M * 8) + (want_rec64 ? al(M * REC64 * 8) : 0) + al(M * REC32 * 4) + al(table.size() * 4);
   need += al(((size_t)K << MB_MAX_CUT) * 4) + al((size_t)std::max<int64_t>(wgs, 1) * 4);
   need += al(C * 3 * 8) + al(C * 8);
   need += al(C * ACC_REC * 8) + al(C * FAR_REC * 8) + al(C * FAR_REC * 4);
@@ -713,12 +712,11 @@ int tree_moments(Ctx& c, int buf, const TreeArrays& t, const std::vector<int64_t
   };
   MomentInputs mi;
   for (int q = 0; q < 3; ++q) mi.s[q] = (double*)carve(M * 8);
-  for (int q = 0; q < 3; ++q) mi.e[q] = (double*)carve(M * 8);
-  for (int q = 0; q < 3; ++q) mi.a[q] = (double*)carve(M * 8);
   for (int q = 0; q < 3; ++q) mi.l[q] = (double*)carve(M * 8);
   mi.rad = (double*)carve(M * 8);
   mi.gam = (double*)carve(M * 8);
-  td.rec64 = (double*)carve(M * REC64 * 8);
+  td.rec64 = want_rec64 ? (double*)carve(M * REC64 * 8) : nullptr;
+  for (int q = 0; q < 3; ++q) td.start[q] = mi.s[q];
   td.rec32 = (float*)carve(M * REC32 * 4);
   int32_t* d_table = (int32_t*)carve(table.size() * 4);
   int32_t* d_top = (int32_t*)carve(((size_t)K << MB_MAX_CUT) * 4);
@@ -733,6 +731,7 @@ int tree_moments(Ctx& c, int buf, const TreeArrays& t, const std::vector<int64_t
   ns.mom = (double*)carve(C * 30 * 8);
   ns.r2 = (double*)carve(C * 8);
   td.count = cap;
+  td.src = SegSource{t.order, starts, directions, lengths, gamma, radius, model};
   int st = upload_i32(c, table, d_table);
   if (st) return st;
   MomentForest mf{K, d_table, d_table + K, d_top, d_roots};

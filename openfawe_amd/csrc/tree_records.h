@@ -8,7 +8,8 @@
 
 namespace ofw {
 
-// Pack one segment's FP64 and FP32 pair records (slots in common.h) from the reference's engine inputs.
+// Pack one segment's FP64 and FP32 pair records (slots in common.h) from the reference's engine inputs; either
+// output may be null.
 __device__ __forceinline__ void pack_record(int s, const double* __restrict__ starts,
                                             const double* __restrict__ directions,
                                             const double* __restrict__ lengths, const double* __restrict__ gamma,
@@ -22,6 +23,7 @@ __device__ __forceinline__ void pack_record(int s, const double* __restrict__ st
   const bool cored = rad > 0.0 && model != OFW_MODEL_NONE;
   const bool ratio_model = model == OFW_MODEL_RANKINE || model == OFW_MODEL_LAMB || model == OFW_MODEL_VATISTAS;
   const double exact = (cored && ratio_model && !(rc >= OFW_FAST_RC_MIN)) ? 1.0 : 0.0;
+  if (r) {
   r[R_SX] = starts[3 * (size_t)s];
   r[R_SY] = starts[3 * (size_t)s + 1];
   r[R_SZ] = starts[3 * (size_t)s + 2];
@@ -38,13 +40,15 @@ __device__ __forceinline__ void pack_record(int s, const double* __restrict__ st
   r[R_EXACT] = exact;
   r[R_IRC2] = cored ? 1.0 / rc2 : 0.0;
   r[R_PAD1] = 0.0;
+  }
+  if (!f) return;
   // a segment whose factors leave the normal FP32 range always takes the FP64 path
   const double ag = fabs(g4);
   const bool f32_ok = exact == 0.0 && invL < 1e30 && invL > 1e-30 && ag < 1e30 && (ag == 0.0 || ag > 1e-30) &&
                       (!cored || (rc2 < 1e30 && rc2 > 1e-30));
-  f[F_DX] = (float)r[R_DX];
-  f[F_DY] = (float)r[R_DY];
-  f[F_DZ] = (float)r[R_DZ];
+  f[F_DX] = (float)directions[3 * (size_t)s];
+  f[F_DY] = (float)directions[3 * (size_t)s + 1];
+  f[F_DZ] = (float)directions[3 * (size_t)s + 2];
   f[F_INVL] = (float)invL;
   f[F_G4] = (float)g4;
   f[F_RC2] = cored ? (float)rc2 : 0.0f;
@@ -52,14 +56,28 @@ __device__ __forceinline__ void pack_record(int s, const double* __restrict__ st
   f[F_PAD0] = 0.0f;
 }
 
-// Moment inputs in tree order (SoA): start, end, alpha = gamma * length vector, length vector, radius, gamma
+// Moment inputs in tree order (SoA): start, length vector, radius, gamma.  The end point and alpha = gamma x
+// length vector are formed where they are read, by the same FP64 operations as vortex_tree.py:608-611.
 struct MomentInputs {
   double* s[3];
-  double* e[3];
-  double* a[3];
   double* l[3];
   double* rad;
   double* gam;
+  __device__ __forceinline__ double end(int c, int k) const {
+#pragma clang fp contract(off)
+    return s[c][k] + l[c][k];
+  }
+  __device__ __forceinline__ double alpha(int c, int k) const {
+#pragma clang fp contract(off)
+    return gam[k] * l[c][k];
+  }
+};
+
+// Where a tree-order pair record comes from: the segment of tree position k is order[k] in the engine inputs.
+struct SegSource {
+  const int32_t* order = nullptr;
+  const double *starts = nullptr, *directions = nullptr, *lengths = nullptr, *gamma = nullptr, *radius = nullptr;
+  int model = 0;
 };
 
 // Optional full per-node outputs (pre-order node index), as node_moments / node_contractions return them
@@ -84,8 +102,10 @@ enum { FR_M0 = 0, FR_M1 = 3, FR_W1 = 12, FR_M2 = 15, FR_T2 = 33, FR_W2 = 36, FR_
 
 struct TreeDevice {
   int count = 0;
-  double* rec64 = nullptr;  // pair records in tree order
-  float* rec32 = nullptr;
+  double* rec64 = nullptr;  // FP64 pair records in tree order (FP64 evaluations only)
+  float* rec32 = nullptr;   // FP32 pair records in tree order
+  const double* start[3] = {};  // segment starts in tree order
+  SegSource src;            // the engine inputs, for the FP64 record of one pair
   double* centre = nullptr;  // pre-order
   double* core = nullptr;
   double* acc = nullptr;
