@@ -51,16 +51,18 @@ enum {
 #define OFW_FAST_A2_MIN 1e-200
 #define OFW_FAST_RC_MIN 1e-50
 
-// Mixed precision: a pair goes to FP64 when the target is within NEAR_CORES core radii of the segment
-// (its closest point), beside the segment within NEAR_BESIDE of its line, or within NEAR_AXIS of its line
-// anywhere (distances relative to max(L, |p - s|, |p - e|), as the axis guard scales them).  NEAR_AXIS is
-// 1e4 times the FP32 rounding of the cross product, so every pair the reference's FP64 axis guard catches
-// (8 eps64 relative) reaches the FP64 path and takes the same branch there.  A target on the line's
-// extension beyond either end needs no FP64: there the denominator does not cancel and the pair velocity
-// vanishes with the distance to the line.
+// Mixed precision: a pair goes to FP64 when the target is beside the segment within NEAR_BESIDE of its line,
+// or within NEAR_AXIS of its line anywhere (distances relative to max(L, |p - s|, |p - e|), as the axis guard
+// scales them).  NEAR_AXIS is 1e4 times the FP32 rounding of the cross product, so every pair the reference's
+// FP64 axis guard catches (8 eps64 relative) reaches the FP64 path and takes the same branch there.  A target
+// on the line's extension beyond either end needs no FP64: there the denominator does not cancel and the pair
+// velocity vanishes with the distance to the line.  Core pairs: where the cross product is formed in FP32
+// (the direct sum), a target within NEAR_CORES core radii of the segment takes FP64; where it is formed in
+// FP64 (the tree's near field), only a target within LAMB_BAND32 of the Lamb-Oseen cutoff does.
 #define OFW_NEAR_CORES2 16.0f          // (4 core radii)^2
 #define OFW_NEAR_BESIDE2 1e-6f         // (1e-3)^2
 #define OFW_NEAR_AXIS2 1e-8f           // (1e-4)^2
+#define OFW_LAMB_BAND32 1e-4f          // relative band of q about the Lamb-Oseen cutoff that takes FP64
 #define OFW_F32_A2_MAX 1e30f
 
 struct PairOut {
@@ -324,13 +326,19 @@ __device__ __forceinline__ Rec32 load_rec32(const float* __restrict__ f) {
   return Rec32{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 }
 
-template <int MODEL>
+// The FP32 pair (vortex_numba.py:42-109 in FP32, the rationalised denominator where r1.r2 < 0); it returns true
+// when the pair must be evaluated in FP64 instead.  r = target - start (unscaled).  With XC the caller gives the
+// cross product c = d x (r / L) formed in FP64 and rounded (cin): h is then exact to FP32 rounding however close
+// the target is to the line, so core pairs stay in FP32 and only a thin band about the Lamb-Oseen cutoff takes
+// FP64 (the decision there is the reference's).  Without it, c is formed in FP32 from rounded r / L and d, with
+// an error of eps32 |r| / L in h, and the pairs within 4 core radii of the segment take FP64.
+template <int MODEL, bool XC = false>
 __device__ __forceinline__ bool pair_f32(float rx, float ry, float rz, const Rec32& f, float& vx, float& vy,
-                                         float& vz) {
+                                         float& vz, float cxi = 0.0f, float cyi = 0.0f, float czi = 0.0f) {
   const float invL = f.invL;
   const float dx = f.dx, dy = f.dy, dz = f.dz;
   const float ax = rx * invL, ay = ry * invL, az = rz * invL;
-  const float cx = dy * az - dz * ay, cy = dz * ax - dx * az, cz = dx * ay - dy * ax;
+  const float cx = XC ? cxi : dy * az - dz * ay, cy = XC ? cyi : dz * ax - dx * az, cz = XC ? czi : dx * ay - dy * ax;
   const float na2 = ax * ax + ay * ay + az * az;
   const float bx = ax - dx, by = ay - dy, bz = az - dz;
   const float nb2 = bx * bx + by * by + bz * bz;
@@ -340,12 +348,22 @@ __device__ __forceinline__ bool pair_f32(float rx, float ry, float rz, const Rec
   const float t = (ax * dx + ay * dy) + az * dz;
   // every test evaluated (bitwise combination, no short-circuit): no branch per pair
   const bool beside = (t >= 0.0f) & (t <= 1.0f);
+  // FP64 for the pairs the reference's single-precision kernels mark suspect (within 1e-4 of the line,
+  // vortex_cuda.py:110-124), beside the segment within 1e-3 of its line, or outside the FP32 range
   bool near = (!(m2 <= OFW_F32_A2_MAX)) | (h2 <= OFW_NEAR_AXIS2 * m2) | (f.exact != 0.0f) |
               (beside & (h2 <= OFW_NEAR_BESIDE2 * m2));
   if (MODEL != OFW_MODEL_NONE) {
-    // squared distance to the segment itself (in segment lengths)
-    const float dseg2 = beside ? h2 : (t < 0.0f ? na2 : nb2);
-    near = near | (dseg2 < OFW_NEAR_CORES2 * rc2);
+    if (XC) {
+      if (MODEL == OFW_MODEL_LAMB) {
+        // the cutoff decision q <= 10/1.25643 within 1e-4 of it (q exact to a few FP32 roundings)
+        const float qc = (float)OFW_LAMB_Q;
+        near = near | ((rc2 > 0.0f) & (fabsf(h2 - qc * rc2) <= OFW_LAMB_BAND32 * qc * rc2));
+      }
+    } else {
+      // squared distance to the segment itself (in segment lengths)
+      const float dseg2 = beside ? h2 : (t < 0.0f ? na2 : nb2);
+      near = near | (dseg2 < OFW_NEAR_CORES2 * rc2);
+    }
   }
   const float g4 = f.g4;
   float sa = __builtin_amdgcn_sqrtf(na2), sb = __builtin_amdgcn_sqrtf(nb2);
@@ -370,8 +388,8 @@ __device__ __forceinline__ bool pair_f32(float rx, float ry, float rz, const Rec
     coef = g4 * (sa + sb) * __builtin_amdgcn_rcpf(den);
   } else {
     if (MODEL == OFW_MODEL_RANKINE || MODEL == OFW_MODEL_LAMB) {
-      // core factors of q = (h / rc)^2 (vortex_numba.py:79-101): an FP32 pair is more than 4 core radii from
-      // the segment but may be closer than that to its line, beyond one of its ends
+      // core factors of q = (h / rc)^2 (vortex_numba.py:79-101) within 4 core radii of the line (beyond them
+      // both factors are 1)
       const bool core = !near && rc2 > 0.0f && h2 < OFW_NEAR_CORES2 * rc2;
       if (__builtin_amdgcn_ballot_w64(core) != 0 && core) {
         const float q = h2 * __builtin_amdgcn_rcpf(rc2);

@@ -463,6 +463,7 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_tree_eval(ForestTables ft, 
 struct NearSegs {
   const double* rec64;
   const double* start[3];
+  const double* lenv[3];
   SegSource src;
 };
 
@@ -488,8 +489,11 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_tree_near(ForestTables ft, 
   constexpr int SEGW = PREC == OFW_FP64 ? 1 : 64;
   __shared__ float s_seg[NEAR_WAVES][SEGW][REC32];
   __shared__ double s_start[NEAR_WAVES][SEGW][4];
-  // below FP64, the FP64 record of the segment whose pairs take the FP64 path (the same for every lane)
+  __shared__ double s_lenv[NEAR_WAVES][SEGW][4];   // length vectors: the pair's cross product in FP64
+  // below FP64, the FP64 record of the segment whose pairs take the FP64 path (the same for every lane), and the
+  // engine index of each window segment
   __shared__ double s_rec[NEAR_WAVES][REC64];
+  __shared__ int s_orig[NEAR_WAVES][SEGW];
   const far_t* __restrict__ frec = PREC == OFW_FP64 ? (const far_t*)fr64 : (const far_t*)fr32;
   const int wv = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
@@ -620,6 +624,11 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_tree_near(ForestTables ft, 
             const float4* f4 = reinterpret_cast<const float4*>(rec32 + (size_t)s * REC32);
             const float4 fa = f4[0], fb = f4[1];
             const double qx = segs.start[0][s], qy = segs.start[1][s], qz = segs.start[2][s];
+            const double lx = segs.lenv[0][s], ly = segs.lenv[1][s], lz = segs.lenv[2][s];
+            s_orig[wv][lane] = segs.src.order[s];
+            s_lenv[wv][lane][0] = lx;
+            s_lenv[wv][lane][1] = ly;
+            s_lenv[wv][lane][2] = lz;
             float4* d4 = reinterpret_cast<float4*>(&s_seg[wv][lane][0]);
             d4[0] = fa;
             d4[1] = fb;
@@ -652,8 +661,13 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_tree_near(ForestTables ft, 
             hits += hit ? 1 : 0;
           }
           const double sx = x0 - cx0, sy = x1 - cy0, sz = x2 - cz0;
+          // c = d x (r / L) = (l x r) / L^2 formed in FP64: h keeps its FP32 accuracy next to the line
+          const double gx = s_lenv[wv][tl][0], gy = s_lenv[wv][tl][1], gz = s_lenv[wv][tl][2];
+          const float il2 = fc.invL * fc.invL;
+          const float ccx = (float)(gy * sz - gz * sy) * il2, ccy = (float)(gz * sx - gx * sz) * il2,
+                      ccz = (float)(gx * sy - gy * sx) * il2;
           float vx, vy, vz;
-          const bool need64 = pair_f32<MODEL>((float)sx, (float)sy, (float)sz, fc, vx, vy, vz);
+          const bool need64 = pair_f32<MODEL, true>((float)sx, (float)sy, (float)sz, fc, vx, vy, vz, ccx, ccy, ccz);
           const bool add = use && !need64;
           lx += add ? vx : 0.0f;
           ly += add ? vy : 0.0f;
@@ -665,7 +679,7 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_tree_near(ForestTables ft, 
             double* r = &s_rec[wv][0];
             if (lane == (int)__builtin_ctzll(slow_lanes)) {
               const SegSource& q = segs.src;
-              pack_record(q.order[s], q.starts, q.directions, q.lengths, q.gamma, q.radius, q.model, r, nullptr);
+              pack_record(s_orig[wv][tl], q.starts, q.directions, q.lengths, q.gamma, q.radius, q.model, r, nullptr);
             }
             wave_lds_sync();
             if (slow) {
@@ -961,7 +975,7 @@ static void launch_near(int precision, dim3 grid, hipStream_t st, const ForestTa
                         const int32_t* ex_ptr, const int32_t* ex_pos, const TreeDevice& td, int order, int raise,
                         const NearEntry* list, const int32_t* list_count, const int32_t* chunk_off,
                         const int32_t* item_wave, int slots, double* cpart, int8_t* cflag, int32_t* cnear) {
-  const NearSegs segs{td.rec64, {td.start[0], td.start[1], td.start[2]}, td.src};
+  const NearSegs segs{td.rec64, {td.start[0], td.start[1], td.start[2]}, {td.lenv[0], td.lenv[1], td.lenv[2]}, td.src};
   if (precision == OFW_FP64)
     hipLaunchKernelGGL((k_tree_near<MODEL, OFW_FP64>), grid, dim3(64 * NEAR_WAVES), 0, st, ft, pts, ex_ptr, ex_pos, segs,
                        td.rec32, td.acc, td.far64, td.far32, order, raise, list, list_count, chunk_off, item_wave, slots, cpart,

@@ -93,7 +93,7 @@ __device__ __forceinline__ bool is_leaf(const TreeArrays& t, int p) { return t.p
 // internal nodes between those and the cut (trees deeper than MB_MAX_CUT + MB_DEPTH).
 static constexpr int MB_DEPTH = 5;
 static constexpr int SUB_CAP = 64;
-static constexpr int MB_MAX_CUT = 12;
+static constexpr int MB_MAX_CUT = 8;
 static constexpr int LEAF_LANES = 32;                  // lanes of one leaf job (half a wavefront)
 static constexpr int BOTTOM_MT = 256;                  // threads of k_moments_bottom
 static constexpr int HALVES = BOTTOM_MT / LEAF_LANES;  // leaf jobs at a time
@@ -124,6 +124,7 @@ struct NodeScratch {
   double* box;
   double* mom;
   double* r2;
+  int32_t* slot;  // top nodes: the index in their level's list
 };
 
 // Node quantities addressed by a node index, in global memory (pre-order index) or in LDS (index within a
@@ -153,14 +154,22 @@ __device__ __forceinline__ void shift_add(const double* __restrict__ cm, const d
   }
 }
 
+// One node's quantities (box 6, centre 3, moments 30, core 1), in global memory or in LDS.
+struct NodeP {
+  double *box, *centre, *mom, *core;
+};
+__device__ __forceinline__ NodeP node_at(NodeRef v, int i) {
+  return NodeP{v.box + 6 * (size_t)i, v.centre + 3 * (size_t)i, v.mom + 30 * (size_t)i, v.core + i};
+}
+
 // An internal node from its two children: box and core exactly by min / max, the centre from the box,
-// moments by shift_add in the fixed order left then right.
-__device__ __forceinline__ void combine_children(NodeRef v, int par, int left, int right) {
+// moments by shift_add in the fixed order left then right; written to out and, when copy.box is set, to copy.
+__device__ __forceinline__ void combine_nodes(NodeP L, NodeP R, NodeP out, NodeP copy) {
 #pragma clang fp contract(off)
   double pl[3], ph[3], pc[3];
   for (int q = 0; q < 3; ++q) {
-    const double l0 = v.box[6 * (size_t)left + q], r0 = v.box[6 * (size_t)right + q];
-    const double l1 = v.box[6 * (size_t)left + 3 + q], r1 = v.box[6 * (size_t)right + 3 + q];
+    const double l0 = L.box[q], r0 = R.box[q];
+    const double l1 = L.box[3 + q], r1 = R.box[3 + q];
     // the left child's segments come first: on equal values its bound is kept, as a sequential scan keeps it
     pl[q] = r0 < l0 ? r0 : l0;
     ph[q] = r1 > l1 ? r1 : l1;
@@ -169,18 +178,32 @@ __device__ __forceinline__ void combine_children(NodeRef v, int par, int left, i
   double pm[30];
   for (int q = 0; q < 30; ++q) pm[q] = 0.0;
   double dl[3];
-  for (int q = 0; q < 3; ++q) dl[q] = v.centre[3 * (size_t)left + q] - pc[q];
-  shift_add(v.mom + 30 * (size_t)left, dl, pm);
-  for (int q = 0; q < 3; ++q) dl[q] = v.centre[3 * (size_t)right + q] - pc[q];
-  shift_add(v.mom + 30 * (size_t)right, dl, pm);
-  const double cl = v.core[left], cr = v.core[right];
+  for (int q = 0; q < 3; ++q) dl[q] = L.centre[q] - pc[q];
+  shift_add(L.mom, dl, pm);
+  for (int q = 0; q < 3; ++q) dl[q] = R.centre[q] - pc[q];
+  shift_add(R.mom, dl, pm);
+  const double cl = L.core[0], cr = R.core[0];
+  const double pcore = cr > cl ? cr : cl;
   for (int q = 0; q < 3; ++q) {
-    v.box[6 * (size_t)par + q] = pl[q];
-    v.box[6 * (size_t)par + 3 + q] = ph[q];
-    v.centre[3 * (size_t)par + q] = pc[q];
+    out.box[q] = pl[q];
+    out.box[3 + q] = ph[q];
+    out.centre[q] = pc[q];
   }
-  for (int q = 0; q < 30; ++q) v.mom[30 * (size_t)par + q] = pm[q];
-  v.core[par] = cr > cl ? cr : cl;
+  for (int q = 0; q < 30; ++q) out.mom[q] = pm[q];
+  out.core[0] = pcore;
+  if (copy.box) {
+    for (int q = 0; q < 3; ++q) {
+      copy.box[q] = pl[q];
+      copy.box[3 + q] = ph[q];
+      copy.centre[q] = pc[q];
+    }
+    for (int q = 0; q < 30; ++q) copy.mom[q] = pm[q];
+    copy.core[0] = pcore;
+  }
+}
+
+__device__ __forceinline__ void combine_children(NodeRef v, int par, int left, int right) {
+  combine_nodes(node_at(v, left), node_at(v, right), node_at(v, par), NodeP{});
 }
 
 __device__ __forceinline__ double half_max(double v) {
@@ -443,28 +466,85 @@ __global__ __launch_bounds__(256) void k_internal_level(TreeArrays t, MomentFore
 // One workgroup per tree: the internal nodes of the first min(cut, MB_MAX_CUT) levels (listed by
 // k_moment_lists), level by level from the deepest; their children were finished by k_moments_bottom or
 // k_internal_level.  Their squared bounds are cleared for k_bound_top.
+// The levels of a tree's top at most TOP_LDS_LEVEL deep (level d holds at most 2^d nodes) keep their quantities
+// in LDS for the level above, besides writing them to global memory: the even levels in the slots from 0, the
+// odd ones after the deepest even level's 2^TOP_EVEN slots.  (Wider levels, whose children's quantities come
+// from global memory, are spread over the device by k_internal_level: one workgroup reads them at about one
+// cache line per clock.)
+static constexpr int TOP_LDS_LEVEL = MB_MAX_CUT - 1;
+static constexpr int TOP_EVEN = TOP_LDS_LEVEL & ~1;                                      // deepest even kept level
+static constexpr int TOP_ODD = (TOP_LDS_LEVEL & 1) ? TOP_LDS_LEVEL : TOP_LDS_LEVEL - 1;  // deepest odd kept level
+static constexpr int TOP_QN = (1 << TOP_EVEN) + (1 << TOP_ODD);                          // slots of both parities
 __global__ __launch_bounds__(512) void k_moments_top(TreeArrays t, MomentForest mf, NodeScratch ns,
                                                      double* __restrict__ centre, double* __restrict__ core) {
   __shared__ int s_node[1 << MB_MAX_CUT], s_right[1 << MB_MAX_CUT];
+  __shared__ int16_t s_lq[1 << MB_MAX_CUT], s_rq[1 << MB_MAX_CUT];  // a child's LDS slot, or -1: global memory
+  __shared__ int s_first[MB_MAX_CUT + 1];                          // listed nodes of the levels before each
+  __shared__ double q_box[TOP_QN][6], q_cen[TOP_QN][3], q_mom[TOP_QN][30], q_core[TOP_QN];
   if (t.counters[C_OVERFLOW]) return;
   const int k = blockIdx.x;
   const int D = min(mf.cut[k], MB_MAX_CUT);
   if (D == 0 || t.tree_first[k] == t.tree_first[k + 1]) return;
   const int32_t* list = mf.top_list + ((size_t)k << MB_MAX_CUT);
   const int32_t* cnt = mf.top_count + (size_t)k * MB_MAX_CUT;
-  // every listed node and its right child, loaded together
-  for (int d = 0; d < D; ++d)
-    for (int i = (int)threadIdx.x; i < cnt[d]; i += blockDim.x) {
-      const int p = list[(1 << d) - 1 + i];
-      s_node[(1 << d) - 1 + i] = p;
-      s_right[(1 << d) - 1 + i] = t.p_skip[p + 1];
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int d = 0; d < D; ++d) {
+      s_first[d] = acc;
+      acc += cnt[d];
     }
+    s_first[D] = acc;
+  }
+  __syncthreads();
+  // every listed node (all levels at once) and its right child; a node's slot in its level's list
+  auto level_of = [&](int f) {
+    int d = 0;
+    while (s_first[d + 1] <= f) ++d;
+    return d;
+  };
+  const int total = s_first[D];
+  for (int f = (int)threadIdx.x; f < total; f += blockDim.x) {
+    const int d = level_of(f), i = f - s_first[d];
+    const int p = list[(1 << d) - 1 + i];
+    s_node[(1 << d) - 1 + i] = p;
+    s_right[(1 << d) - 1 + i] = t.p_skip[p + 1];
+    ns.slot[p] = i;
+  }
+  __syncthreads();
+  auto qbase = [](int d) { return (d & 1) ? (1 << TOP_EVEN) : 0; };
+  // where each node's children are: in LDS when this kernel makes them at a level kept there, else global
+  for (int f = (int)threadIdx.x; f < total; f += blockDim.x) {
+    const int d = level_of(f), i = f - s_first[d], dc = d + 1;
+    const int j = (1 << d) - 1 + i;
+    const int c[2] = {s_node[j] + 1, s_right[j]};
+    int16_t q[2];
+    for (int h = 0; h < 2; ++h) {
+      int sl = -1;
+      if (dc <= TOP_LDS_LEVEL && dc < D && !is_leaf(t, c[h])) {
+        sl = ns.slot[c[h]] + qbase(dc);
+        if ((unsigned)sl >= (unsigned)TOP_QN) {  // cannot happen (level dc holds at most 2^dc nodes); reported
+          t.counters[C_OVERFLOW] = 1;
+          sl = -1;
+        }
+      }
+      q[h] = (int16_t)sl;
+    }
+    s_lq[j] = q[0];
+    s_rq[j] = q[1];
+  }
   __syncthreads();
   const NodeRef g{ns.box, centre, ns.mom, core};
+  auto child = [&](int c, int sl) -> NodeP {
+    return sl >= 0 ? NodeP{q_box[sl], q_cen[sl], q_mom[sl], &q_core[sl]} : node_at(g, c);
+  };
   for (int d = D - 1; d >= 0; --d) {
-    for (int i = (int)threadIdx.x; i < cnt[d]; i += blockDim.x) {
-      const int p = s_node[(1 << d) - 1 + i];
-      combine_children(g, p, p + 1, s_right[(1 << d) - 1 + i]);
+    for (int i = (int)threadIdx.x; i < s_first[d + 1] - s_first[d]; i += blockDim.x) {
+      const int j = (1 << d) - 1 + i;
+      const int p = s_node[j];
+      const int sl = i + qbase(d);
+      const NodeP copy = (d <= TOP_LDS_LEVEL && sl < TOP_QN) ? NodeP{q_box[sl], q_cen[sl], q_mom[sl], &q_core[sl]}
+                                                             : NodeP{};
+      combine_nodes(child(p + 1, s_lq[j]), child(s_right[j], s_rq[j]), node_at(g, p), copy);
       ns.r2[p] = 0.0;
     }
     __syncthreads();
@@ -702,7 +782,7 @@ int tree_moments(Ctx& c, int buf, const TreeArrays& t, const std::vector<int64_t
   need += al(((size_t)K << MB_MAX_CUT) * 4) + al((size_t)std::max<int64_t>(wgs, 1) * 4);
   need += al(C * 3 * 8) + al(C * 8);
   need += al(C * ACC_REC * 8) + al(C * FAR_REC * 8) + al(C * FAR_REC * 4);
-  need += al(C * 6 * 8) + al(C * 30 * 8) + al(C * 8);
+  need += al(C * 6 * 8) + al(C * 30 * 8) + al(C * 8) + al(C * 4);
   OFW_HIP(c.tree[buf].reserve(need));
   char* base = (char*)c.tree[buf].ptr;
   auto carve = [&](size_t bytes) {
@@ -716,7 +796,10 @@ int tree_moments(Ctx& c, int buf, const TreeArrays& t, const std::vector<int64_t
   mi.rad = (double*)carve(M * 8);
   mi.gam = (double*)carve(M * 8);
   td.rec64 = want_rec64 ? (double*)carve(M * REC64 * 8) : nullptr;
-  for (int q = 0; q < 3; ++q) td.start[q] = mi.s[q];
+  for (int q = 0; q < 3; ++q) {
+    td.start[q] = mi.s[q];
+    td.lenv[q] = mi.l[q];
+  }
   td.rec32 = (float*)carve(M * REC32 * 4);
   int32_t* d_table = (int32_t*)carve(table.size() * 4);
   int32_t* d_top = (int32_t*)carve(((size_t)K << MB_MAX_CUT) * 4);
@@ -730,6 +813,7 @@ int tree_moments(Ctx& c, int buf, const TreeArrays& t, const std::vector<int64_t
   ns.box = (double*)carve(C * 6 * 8);
   ns.mom = (double*)carve(C * 30 * 8);
   ns.r2 = (double*)carve(C * 8);
+  ns.slot = (int32_t*)carve(C * 4);
   td.count = cap;
   td.src = SegSource{t.order, starts, directions, lengths, gamma, radius, model};
   int st = upload_i32(c, table, d_table);

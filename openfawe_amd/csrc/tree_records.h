@@ -105,6 +105,7 @@ struct TreeDevice {
   double* rec64 = nullptr;  // FP64 pair records in tree order (FP64 evaluations only)
   float* rec32 = nullptr;   // FP32 pair records in tree order
   const double* start[3] = {};  // segment starts in tree order
+  const double* lenv[3] = {};   // segment length vectors in tree order
   SegSource src;            // the engine inputs, for the FP64 record of one pair
   double* centre = nullptr;  // pre-order
   double* core = nullptr;
