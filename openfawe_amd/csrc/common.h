@@ -178,6 +178,24 @@ __device__ __forceinline__ double ofw_sqrt(double x) {
   return __builtin_fma(d, h, g);
 }
 
+// One Newton-Raphson step from the hardware estimates (relative error 2^-24 or less): 2^-48 or less, a few
+// ulps, for the pair's reciprocal and norms (within the direct sum's 1e-12 bound by three orders of magnitude,
+// and far inside the 1e-12 bands of its decisions).
+__device__ __forceinline__ double ofw_rcp1(double x) {
+#pragma clang fp contract(off)
+  const double r = __builtin_amdgcn_rcp(x);
+  const double e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+
+__device__ __forceinline__ double ofw_sqrt1(double x) {
+#pragma clang fp contract(off)
+  const double y = __builtin_amdgcn_rsq(x);
+  const double g = x * y;
+  const double d = __builtin_fma(-g, g, x);
+  return __builtin_fma(d, 0.5 * y, g);
+}
+
 __device__ __forceinline__ double ofw_rsqrt(double x) {
 #pragma clang fp contract(off)
   double y = __builtin_amdgcn_rsq(x);
@@ -197,9 +215,10 @@ __device__ __forceinline__ double ofw_rsqrt(double x) {
 // inside its cutoff, extreme ranges) branch, on the wavefront's ballot.  A lane with use == false evaluates
 // the pair but adds nothing (an excluded pair) and reports no flag.  a = (p - s)/L and the cross product
 // d x a are rounded exactly as the reference rounds them (they decide h near a segment's line, where the
-// cross product cancels); the other quantities agree with it to an ulp or two.  The arithmetic is written
-// out operation by operation (no contraction), so every kernel that inlines it (direct sum, tree near field,
-// pair matrix) gives the same bits for the same pair.  Pairs outside the fast range take pair_exact64.
+// cross product cancels); the other quantities agree with it to a few ulps (one Newton step from the hardware
+// reciprocal and square-root estimates).  The arithmetic is written out operation by operation (no
+// contraction), so every kernel that inlines it (direct sum, tree near field, pair matrix) gives the same bits
+// for the same pair.  Pairs outside the fast range take pair_exact64.
 template <int MODEL>
 __device__ __forceinline__ int pair_fast64(double px, double py, double pz, const double* __restrict__ r,
                                            int raise_singular, bool use, double& ux, double& uy, double& uz) {
@@ -217,12 +236,12 @@ __device__ __forceinline__ int pair_fast64(double px, double py, double pz, cons
   const double na2 = __builtin_fma(az, az, __builtin_fma(ay, ay, ax * ax));
   const double bx = ax - dx, by = ay - dy, bz = az - dz;
   const double nb2 = __builtin_fma(bz, bz, __builtin_fma(by, by, bx * bx));
+  const double h2 = __builtin_fma(cz, cz, __builtin_fma(cy, cy, cx * cx));
+  const double t = __builtin_fma(az, dz, __builtin_fma(ay, dy, ax * dx));
   const bool exact_seg = (MODEL == OFW_MODEL_RANKINE || MODEL == OFW_MODEL_LAMB || MODEL == OFW_MODEL_VATISTAS)
                              ? r[R_EXACT] != 0.0 : false;
   const bool slow = use && (!(fmax(na2, nb2) <= OFW_FAST_A2_MAX) || !(fmin(na2, nb2) >= OFW_FAST_A2_MIN) || exact_seg);
-  const double h2 = __builtin_fma(cz, cz, __builtin_fma(cy, cy, cx * cx));
-  const double t = __builtin_fma(az, dz, __builtin_fma(ay, dy, ax * dx));
-  const double na = ofw_sqrt(na2), nb = ofw_sqrt(nb2);
+  const double na = ofw_sqrt1(na2), nb = ofw_sqrt1(nb2);
   const double mx = fmax(1.0, fmax(na, nb));
   // axis guard: certain outside the band (one multiply more than a plain test), exact inside it
   bool axis = h2 <= (OFW_AXIS_EPS * OFW_AXIS_EPS * OFW_BAND_HI) * (mx * mx);
@@ -236,23 +255,23 @@ __device__ __forceinline__ int pair_fast64(double px, double py, double pz, cons
   double factor = 1.0;
   double coef;
   if (MODEL == OFW_MODEL_OFFSET || MODEL == OFW_MODEL_RM) {
-    double plus = dot < 0.0 ? h2 * ofw_rcp(product - dot) : product + dot;
+    double plus = dot < 0.0 ? h2 * ofw_rcp1(product - dot) : product + dot;
     double den = product * plus;
     const double rc2 = r[R_RC2];
     if (r[R_RADIUS] > 0.0) {
       if (MODEL == OFW_MODEL_OFFSET) {
         den = den + rc2;
       } else {
-        sa = ofw_sqrt(na2 + rc2);
-        sb = ofw_sqrt(nb2 + rc2);
+        sa = ofw_sqrt1(na2 + rc2);
+        sb = ofw_sqrt1(nb2 + rc2);
         product = sa * sb;
         const double tr2 = h2 + rc2;
         dot = __builtin_fma(t, t - 1.0, tr2);
-        plus = dot < 0.0 ? tr2 * ofw_rcp(product - dot) : product + dot;
+        plus = dot < 0.0 ? tr2 * ofw_rcp1(product - dot) : product + dot;
         den = product * plus;
       }
     }
-    coef = ((g4 * factor) * (sa + sb)) * ofw_rcp(den);
+    coef = ((g4 * factor) * (sa + sb)) * ofw_rcp1(den);
   } else {
     if (MODEL != OFW_MODEL_NONE) {
       const bool cored = r[R_RADIUS] > 0.0;
@@ -268,7 +287,7 @@ __device__ __forceinline__ int pair_fast64(double px, double py, double pz, cons
           if (inside) factor = -expm1(-OFW_LAMB * q);
         }
       } else {  // Vatistas: q / sqrt(1 + q^2) below one core radius, 1 / sqrt(1 + q^-2) beyond
-        const double iq = q < 1.0 ? q : ofw_rcp(q);
+        const double iq = q < 1.0 ? q : ofw_rcp1(q);
         const double rs = ofw_rsqrt(__builtin_fma(iq, iq, 1.0));
         factor = cored ? (q < 1.0 ? q * rs : rs) : 1.0;
       }
@@ -278,7 +297,7 @@ __device__ __forceinline__ int pair_fast64(double px, double py, double pz, cons
     const bool rat = dot < 0.0;
     const double dd = rat ? product * h2 : product * (product + dot);
     const double xm = rat ? product - dot : 1.0;
-    coef = (((g4 * factor) * (sa + sb)) * xm) * ofw_rcp(dd);
+    coef = (((g4 * factor) * (sa + sb)) * xm) * ofw_rcp1(dd);
   }
   bool add = use && !slow && !axis && g4 != 0.0;
   const bool finite = isfinite(coef);
@@ -289,10 +308,12 @@ __device__ __forceinline__ int pair_fast64(double px, double py, double pz, cons
     if (add && !finite) flag = OFW_FLAG_NONFINITE;
   }
   add = add && finite;
-  // the pair velocity is rounded before it is added, as in the reference's ordered sum
-  ux = add ? ux + coef * cx : ux;
-  uy = add ? uy + coef * cy : uy;
-  uz = add ? uz + coef * cz : uz;
+  // the pair velocity is rounded before it is added, as in the reference's ordered sum; a pair not added adds
+  // +0.0 (the sums start at +0.0, so this leaves them as they are)
+  const double cf = add ? coef : 0.0;
+  ux = ux + cf * cx;
+  uy = uy + cf * cy;
+  uz = uz + cf * cz;
   if (__builtin_amdgcn_ballot_w64(slow) != 0 && slow) {
     const PairOut e = pair_exact64(px, py, pz, r, MODEL, raise_singular);
     if (e.flag == OFW_FLAG_OK) {
