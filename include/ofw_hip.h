@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define OFW_ABI_VERSION 2
+#define OFW_ABI_VERSION 3
 
 enum ofw_status {
   OFW_OK = 0,
@@ -286,8 +286,10 @@ int ofw_stage_combine(ofw_ctx *ctx, int mode, int64_t count, double h, const dou
  * the stage velocity of _network_convection (wake.py:404-491): the uniform inflow (device, 3 doubles) plus, at the
  * targets, the tree sum over the network concatenate(bound (first,3), free points) with edges (m,2), gamma, radius
  * (m), the incident segments excluded; targets are the wing rows [0, rows) and the ring nodes [ring_lo, ring_hi)
- * of the free points, every other free point keeps the inflow.  The bound row moves by time fraction times
- * displacement (first,3; NULL: fixed).  Every pointer is kept: a run reads the arrays' current contents.
+ * of the free points, every other free point keeps the inflow.  drift (k,3; NULL: none) is added to every free
+ * point's stage velocity after that sum: the rings' sheet drift (wake.py:582-588), zero outside the drifting ring
+ * nodes.  The bound row moves by time fraction times displacement (first,3; NULL: fixed).  Every pointer is
+ * kept: a run reads the arrays' current contents.
  * engine 0 is the tree code (theta, expansion_order, leaf_size, clearance; models 0-2), engine 1 the direct sum
  * (block as ofw_direct_sum; models 0-5); both in `precision`.  Creating runs the step twice (measuring the pair and
  * list capacities, then with them) and records it when use_graph != 0; stats_host (3 int64, may be NULL) receives
@@ -297,8 +299,8 @@ int ofw_step_create(ofw_ctx *ctx, ofw_step **plan, int precision, int64_t first,
                     const double *displacement, int64_t k, const double *points_in, double *points_out, int64_t m,
                     const int64_t *edges, const double *gamma, const double *radius, int model, int64_t rows,
                     int64_t ring_lo, int64_t ring_hi, double theta, int expansion_order, int64_t leaf_size,
-                    double clearance, const double *inflow, double time, double dt_s, int substeps, int integrator,
-                    int engine, int block, int use_graph, int64_t *stats_host);
+                    double clearance, const double *inflow, const double *drift, double time, double dt_s,
+                    int substeps, int integrator, int engine, int block, int use_graph, int64_t *stats_host);
 /* Run the step (one graph launch) and read its status block (one synchronisation): status_host[0] the
  * ofw_step_status bits, [1] the largest incident-pair count, [2] the largest list work-item count.  A capacity bit
  * asks for a new plan; a flag, zero-length or non-finite bit for the stage-by-stage step, which raises or repeats a

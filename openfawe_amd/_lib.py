@@ -58,8 +58,8 @@ SIGNATURES = {
                                     _c_p]),
     "ofw_stage_combine": (_c_i, [_c_p, _c_i, _c_i64, _c_d, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p]),
     "ofw_step_create": (_c_i, [_c_p, ctypes.POINTER(_c_p), _c_i, _c_i64, _c_p, _c_p, _c_i64, _c_p, _c_p, _c_i64, _c_p,
-                               _c_p, _c_p, _c_i, _c_i64, _c_i64, _c_i64, _c_d, _c_i, _c_i64, _c_d, _c_p, _c_d, _c_d,
-                               _c_i, _c_i, _c_i, _c_i, _c_i, ctypes.POINTER(_c_i64)]),
+                               _c_p, _c_p, _c_i, _c_i64, _c_i64, _c_i64, _c_d, _c_i, _c_i64, _c_d, _c_p, _c_p, _c_d,
+                               _c_d, _c_i, _c_i, _c_i, _c_i, _c_i, ctypes.POINTER(_c_i64)]),
     "ofw_step_run": (_c_i, [_c_p, ctypes.POINTER(_c_i64)]),
     "ofw_step_timings": (_c_i, [_c_p, ctypes.POINTER(ctypes.c_float)]),
     "ofw_step_destroy": (_c_i, [_c_p]),

@@ -18,9 +18,9 @@ or RK4, ``substeps`` substeps, the bound row moved linearly by ``displacement`` 
 Everything stays in HBM: the exclusion pairs (``ofw_incident_pairs``), the segment geometry of each stage
 (``ofw_network_segments``), the exclusion CSR of each target range (``ofw_exclusion_subset``), the sums (the
 direct sum or the tree code of this package) and the stage arithmetic (``ofw_stage_combine``, NumPy's order of
-operations).  The host reads flags and error status only.  Not modelled: the surface guard
-(``surface_guard='off'`` only), the rings' opt-in sheet drift, callable inflows that need host arrays (an
-inflow is a 3-vector or a callable on device tensors).
+operations).  The host reads flags and error status only, except for the reference's surface guard, which is
+given the rows of each stage on the host.  Not modelled: callable inflows that need host arrays (an inflow is a
+3-vector or a callable on device tensors).
 """
 from dataclasses import dataclass
 
@@ -252,12 +252,21 @@ def network_convection(nodes, edges, gamma, first, radii, settings, ambient, rin
 
 
 def convect_geometry(bound, points, edges, gamma, radii, settings, *, inflow, time, dt_s, substeps=1,
-                     integrator="euler", displacement=None, ring_first=None, ring_free=None, ring_radii=None):
-    """The stage loop of wake.convect_geometry (wake.py:586-616) on device tensors: the new free points.
+                     integrator="euler", displacement=None, ring_first=None, ring_free=None, ring_radii=None,
+                     ring_drift=None, guard=None, wing=None):
+    """The stage loop of wake.convect_geometry (wake.py:572-616) on device tensors: the new free points.
 
     bound (first,3) are the bound nodes at ``time`` (moved by ``displacement`` (first,3) over the step), points
     (k,3) the free nodes (wing rows, then ring nodes from ``ring_first``), edges (m,2) index the network
     concatenate(bound, points).
+
+    * ``ring_drift`` (per ring node, as RingWake.drift_m_s): the rings' opt-in sheet drift (wake.py:584-590), a
+      constant velocity of the ring nodes beyond the leading ``ring_free``, added to every stage's velocity.
+    * ``guard``: the reference's surface guard object (``make_surface_guard(wing, config, next_wing)``,
+      wake.py:619-631), or anything with its ``check_rows(old_rows, new_rows, wing, stage=, time_s=)``.  It is
+      called where wake.convect_geometry calls it (wake.py:576-582, 601-615): on the wing rows of every Heun / RK4
+      stage point and of the accepted geometry, before they are used, with host copies of the (depth, first, 3)
+      rows.  Its exception stops the step.
     """
     torch = _torch()
     if integrator not in INTEGRATORS:
@@ -271,6 +280,21 @@ def convect_geometry(bound, points, edges, gamma, radii, settings, *, inflow, ti
         if not bool(displacement.any()):
             displacement = None
     dt = dt_s / substeps
+    rows = int(points.shape[0]) if ring_first is None else int(ring_first) - first  # row points precede rings
+
+    def checked(candidate, previous, label, t):
+        if guard is not None:
+            old = previous[:rows].cpu().numpy().reshape(-1, first, 3)
+            new = candidate[:rows].cpu().numpy().reshape(-1, first, 3)
+            guard.check_rows(old, new, wing, stage=label, time_s=t)
+        return candidate
+
+    drift = None
+    if ring_drift is not None and ring_first is not None and ring_free is not None:
+        node_drift = torch.as_tensor(ring_drift, dtype=torch.float64, device=points.device)
+        if int(node_drift.shape[0]) > int(ring_free):
+            drift = torch.zeros_like(points)
+            drift[rows + int(ring_free):] = node_drift[int(ring_free):]
 
     def rhs(current, t):
         stage_bound = bound
@@ -278,8 +302,9 @@ def convect_geometry(bound, points, edges, gamma, radii, settings, *, inflow, ti
             stage_bound = stage(_lib.STAGE_STEP, (t - time) / dt_s, bound, displacement)
         nodes = torch.cat((stage_bound, current))
         ambient = ambient_velocity(inflow, current, t)
-        return network_convection(nodes, edges, gamma, first, radii, settings, ambient, ring_first, ring_free,
-                                  ring_radii)
+        velocity = network_convection(nodes, edges, gamma, first, radii, settings, ambient, ring_first, ring_free,
+                                      ring_radii)
+        return velocity if drift is None else velocity + drift
 
     for substep in range(int(substeps)):
         t = time + substep * dt
@@ -287,15 +312,15 @@ def convect_geometry(bound, points, edges, gamma, radii, settings, *, inflow, ti
         if integrator == "euler":
             candidate = stage(_lib.STAGE_STEP, dt, points, k1)
         elif integrator == "heun":
-            predicted = stage(_lib.STAGE_STEP, dt, points, k1)
+            predicted = checked(stage(_lib.STAGE_STEP, dt, points, k1), points, "heun_predictor", t + dt)
             k2 = rhs(predicted, t + dt)
             candidate = stage(_lib.STAGE_HEUN, dt, points, k1, k2)
         else:
-            k2 = rhs(stage(_lib.STAGE_HALF, dt, points, k1), t + dt / 2)
-            k3 = rhs(stage(_lib.STAGE_HALF, dt, points, k2), t + dt / 2)
-            k4 = rhs(stage(_lib.STAGE_STEP, dt, points, k3), t + dt)
+            k2 = rhs(checked(stage(_lib.STAGE_HALF, dt, points, k1), points, "rk4_stage2", t + dt / 2), t + dt / 2)
+            k3 = rhs(checked(stage(_lib.STAGE_HALF, dt, points, k2), points, "rk4_stage3", t + dt / 2), t + dt / 2)
+            k4 = rhs(checked(stage(_lib.STAGE_STEP, dt, points, k3), points, "rk4_stage4", t + dt), t + dt)
             candidate = stage(_lib.STAGE_RK4, dt, points, k1, k2, k3, k4)
-        points = candidate
+        points = checked(candidate, points, "accepted_geometry", t + dt)
     return points
 
 
@@ -345,10 +370,16 @@ class GeometryStepPlan:
     ``run`` again.  ``step`` returns the new free points, or takes the stage-by-stage ``convect_geometry`` when the
     status asks for the reference's exact behaviour (a flagged pair raises there, a flagged ring target repeats its
     'zero' call), and makes the plan again when a capacity was exceeded.
+
+    ``ring_drift`` (per ring node, as RingWake.drift_m_s) is the rings' sheet drift (wake.py:584-590), recorded into
+    every stage's velocity.  A ``guard`` (wake.py:576-582) sees the wing rows of every stage point: with one Euler
+    substep its only call is on the accepted geometry, made by ``step`` on host copies after the replay; any other
+    integrator calls it between stages, so ``step`` then takes the stage-by-stage path.
     """
 
     def __init__(self, bound, points, edges, gamma, radii, settings, *, inflow, time, dt_s, substeps=1,
-                 integrator="euler", displacement=None, ring_first=None, ring_free=None, use_graph=True):
+                 integrator="euler", displacement=None, ring_first=None, ring_free=None, ring_drift=None,
+                 guard=None, wing=None, use_graph=True):
         torch = _torch()
         if not settings.free_wake or settings.frozen_rows_drift != "ambient":
             raise ValueError("a recorded step needs a free wake and frozen_rows_drift='ambient'")
@@ -377,12 +408,19 @@ class GeometryStepPlan:
                 displacement = None
         self.args = dict(bound=bound, points=points, edges=edges, gamma=gamma, radii=radii, settings=settings,
                          time=time, dt_s=dt_s, substeps=int(substeps), integrator=integrator,
-                         displacement=displacement, ring_first=ring_first, ring_free=ring_free)
+                         displacement=displacement, ring_first=ring_first, ring_free=ring_free,
+                         ring_drift=ring_drift, guard=guard, wing=wing)
         self.radii = _per_edge(torch, radii, m, points.device, "radii")
+        self.drift = None
+        if ring_drift is not None and ring_first is not None and ring_free is not None:
+            node_drift = torch.as_tensor(ring_drift, dtype=torch.float64, device=points.device).reshape(-1, 3)
+            if int(node_drift.shape[0]) > int(ring_free):
+                self.drift = torch.zeros_like(points)
+                self.drift[rows_end + int(ring_free):] = node_drift[int(ring_free):]
         self.inflow = torch.as_tensor(np.asarray(inflow, dtype=np.float64).reshape(3), device=points.device).clone()
         self.displacement = displacement
         self.points_out = torch.empty_like(points)
-        self.shape = dict(first=first, k=k, m=m, rows=rows, ring_lo=ring_lo, ring_hi=ring_hi)
+        self.shape = dict(first=first, k=k, m=m, rows=rows, ring_lo=ring_lo, ring_hi=ring_hi, split=rows_end)
         self.use_graph = bool(use_graph)
         self.handle = None
         self._create()
@@ -399,7 +437,7 @@ class GeometryStepPlan:
             _ptr(self.displacement), s["k"], _ptr(a["points"]), _ptr(self.points_out), s["m"], _ptr(a["edges"]),
             _ptr(a["gamma"]), _ptr(self.radii), settings.model, s["rows"], s["ring_lo"], s["ring_hi"],
             float(tree.theta), int(tree.order), int(tree.leaf_size), float(tree.core_clearance), _ptr(self.inflow),
-            float(a["time"]), float(a["dt_s"]), a["substeps"], INTEGRATORS.index(a["integrator"]),
+            _ptr(self.drift), float(a["time"]), float(a["dt_s"]), a["substeps"], INTEGRATORS.index(a["integrator"]),
             0 if settings.backend == "hip_tree" else 1, int(settings.hip.block), int(self.use_graph), stats)
         _lib.check(status, "ofw_step_create")
         self.handle = handle
@@ -420,19 +458,28 @@ class GeometryStepPlan:
 
     def step(self):
         """The new free points (``points_out``), with the reference's behaviour in every case."""
-        bits, _, _ = self.run()
-        if bits & (_lib.STEP_PAIR_OVERFLOW | _lib.STEP_TREE_OVERFLOW):
-            self.close()
-            self._create()
+        a = self.args
+        guard = a["guard"]
+        staged = guard is not None and not (a["integrator"] == "euler" and a["substeps"] == 1)
+        bits = 0
+        if not staged:
             bits, _, _ = self.run()
-        if bits:
-            a = self.args
+            if bits & (_lib.STEP_PAIR_OVERFLOW | _lib.STEP_TREE_OVERFLOW):
+                self.close()
+                self._create()
+                bits, _, _ = self.run()
+        if bits or staged:
             out = convect_geometry(a["bound"], a["points"], a["edges"], a["gamma"], self.radii, a["settings"],
                                    inflow=self.inflow.cpu().numpy(), time=a["time"], dt_s=a["dt_s"],
                                    substeps=a["substeps"], integrator=a["integrator"],
                                    displacement=self.displacement, ring_first=a["ring_first"],
-                                   ring_free=a["ring_free"])
+                                   ring_free=a["ring_free"], ring_drift=a["ring_drift"], guard=guard, wing=a["wing"])
             self.points_out.copy_(out)
+        elif guard is not None:
+            split, first = self.shape["split"], self.shape["first"]
+            old = a["points"][:split].cpu().numpy().reshape(-1, first, 3)
+            new = self.points_out[:split].cpu().numpy().reshape(-1, first, 3)
+            guard.check_rows(old, new, a["wing"], stage="accepted_geometry", time_s=a["time"] + a["dt_s"])
         return self.points_out
 
     def close(self):

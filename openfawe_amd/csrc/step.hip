@@ -54,11 +54,12 @@ __global__ void k_gather_targets(int64_t rows, int64_t ring_lo, int64_t rings, c
   }
 }
 
-// stage velocity of every free point: the inflow, plus the sum at the targets; flags and non-finite sums into
-// the status bits
-__global__ void k_stage_velocity(int64_t k, const double* __restrict__ inflow, int64_t rows, int64_t ring_lo,
-                                 int64_t rings, const double* __restrict__ out, const int8_t* __restrict__ flags,
-                                 double* __restrict__ vel, unsigned long long* __restrict__ block) {
+// stage velocity of every free point: the inflow, plus the sum at the targets, plus the drift when there is one
+// (wake.py:598); flags and non-finite sums into the status bits
+__global__ void k_stage_velocity(int64_t k, const double* __restrict__ inflow, const double* __restrict__ drift,
+                                 int64_t rows, int64_t ring_lo, int64_t rings, const double* __restrict__ out,
+                                 const int8_t* __restrict__ flags, double* __restrict__ vel,
+                                 unsigned long long* __restrict__ block) {
 #pragma clang fp contract(off)
   unsigned long long bits = 0;
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < k; q += (int64_t)gridDim.x * blockDim.x) {
@@ -71,6 +72,11 @@ __global__ void k_stage_velocity(int64_t k, const double* __restrict__ inflow, i
       vx = vx + ox;
       vy = vy + oy;
       vz = vz + oz;
+    }
+    if (drift) {
+      vx = vx + drift[3 * q];
+      vy = vy + drift[3 * q + 1];
+      vz = vz + drift[3 * q + 2];
     }
     vel[3 * q] = vx;
     vel[3 * q + 1] = vy;
@@ -129,7 +135,7 @@ struct ofw_step {
   ofw::TreeCall tc;
   ofw::TreeArrays t;
   // caller's arrays
-  const double *bound = nullptr, *disp = nullptr, *points_in = nullptr, *inflow = nullptr;
+  const double *bound = nullptr, *disp = nullptr, *points_in = nullptr, *inflow = nullptr, *drift = nullptr;
   const int64_t* edges = nullptr;
   double* points_out = nullptr;
   int64_t first = 0, k = 0, m = 0, rows = 0, ring_lo = 0, rings = 0;
@@ -259,7 +265,8 @@ static int step_stage(ofw_step& p, const double* pts, double t, int slot) {
   }
   hipLaunchKernelGGL(k_step_collect, dim3(1), dim3(64), 0, c.stream, pstat, p.pair_cap,
                      p.engine == 0 ? (const int32_t*)p.t.counters : nullptr, p.block);
-  hipLaunchKernelGGL(k_stage_velocity, grid_for(c, p.k), dim3(256), 0, c.stream, p.k, p.inflow, p.rows, p.ring_lo,
+  hipLaunchKernelGGL(k_stage_velocity, grid_for(c, p.k), dim3(256), 0, c.stream, p.k, p.inflow, p.drift, p.rows,
+                     p.ring_lo,
                      p.rings, p.out, p.flags, p.kv[slot], p.block);
   OFW_HIP(hipGetLastError());
   return OFW_OK;
@@ -322,8 +329,8 @@ int ofw_step_create(ofw_ctx* x, ofw_step** plan, int precision, int64_t first, c
                     const double* displacement, int64_t k, const double* points_in, double* points_out, int64_t m,
                     const int64_t* edges, const double* gamma, const double* radius, int model, int64_t rows,
                     int64_t ring_lo, int64_t ring_hi, double theta, int expansion_order, int64_t leaf_size,
-                    double clearance, const double* inflow, double time, double dt_s, int substeps, int integrator,
-                    int engine, int block, int use_graph, int64_t* stats_host) {
+                    double clearance, const double* inflow, const double* drift, double time, double dt_s,
+                    int substeps, int integrator, int engine, int block, int use_graph, int64_t* stats_host) {
   const int max_model = engine == 0 ? 2 : 5;
   if (!x || !plan || first < 1 || k < 1 || m < 1 || !bound || !points_in || !points_out || !edges || !gamma ||
       engine < 0 || engine > 1 || !(block == 32 || block == 64 || block == 128 || block == 256) || model > max_model ||
@@ -362,6 +369,7 @@ int ofw_step_create(ofw_ctx* x, ofw_step** plan, int precision, int64_t first, c
   p->points_in = points_in;
   p->points_out = points_out;
   p->inflow = inflow;
+  p->drift = drift;
   p->edges = edges;
   p->first = first;
   p->k = k;

@@ -112,8 +112,14 @@ def network_convection(nodes, edges, gamma, first, radii, model, ambient, *, fre
 
 
 def convect_geometry(bound, points, edges, gamma, radii, model, *, inflow, time, dt_s, substeps=1,
-                     integrator="euler", displacement=None, **split):
-    """The stage loop of wake.convect_geometry (wake.py:586-616) with a uniform inflow: the new free points."""
+                     integrator="euler", displacement=None, ring_drift=None, guard=None, wing=None, **split):
+    """The stage loop of wake.convect_geometry (wake.py:572-616) with a uniform inflow: the new free points.
+
+    ring_drift (per ring node, RingWake.drift_m_s): the rings' sheet drift (wake.py:584-590), a constant velocity
+    of the ring nodes beyond the leading ring_free, added to every stage's velocity.  guard: an object with the
+    reference's check_rows(old_rows, new_rows, wing, stage=, time_s=) (wake.py:576-582), called on the rows
+    (the points before ring_first) of every stage point and of the accepted geometry, before they are used.
+    """
     bound = np.asarray(bound, dtype=np.float64)
     points = np.asarray(points, dtype=np.float64).copy()
     first = len(bound)
@@ -121,14 +127,30 @@ def convect_geometry(bound, points, edges, gamma, radii, model, *, inflow, time,
     if displacement is not None and not np.any(displacement):
         displacement = None
     dt = dt_s / substeps
+    ring_first, ring_free = split.get("ring_first"), split.get("ring_free")
+    rows = len(points) if ring_first is None else ring_first - first  # row points precede ring points
+
+    def checked(candidate, previous, label, t):
+        if guard is not None:
+            guard.check_rows(previous[:rows].reshape(-1, first, 3), candidate[:rows].reshape(-1, first, 3), wing,
+                             stage=label, time_s=t)
+        return candidate
+
+    drift = None
+    if ring_drift is not None and ring_first is not None and ring_free is not None:
+        node_drift = np.asarray(ring_drift, dtype=np.float64)
+        if len(node_drift) > ring_free:
+            drift = np.zeros_like(points)
+            drift[rows + ring_free:] = node_drift[ring_free:]
 
     def rhs(current, t):
         stage_bound = bound
         if displacement is not None and t != time:
             stage_bound = bound + ((t - time) / dt_s) * displacement
         ambient = np.broadcast_to(inflow, current.shape)
-        return network_convection(np.concatenate((stage_bound, current)), edges, gamma, first, radii, model,
-                                  ambient, **split)
+        velocity = network_convection(np.concatenate((stage_bound, current)), edges, gamma, first, radii, model,
+                                      ambient, **split)
+        return velocity if drift is None else velocity + drift
 
     for substep in range(substeps):
         t = time + substep * dt
@@ -136,12 +158,12 @@ def convect_geometry(bound, points, edges, gamma, radii, model, *, inflow, time,
         if integrator == "euler":
             candidate = points + dt * k1
         elif integrator == "heun":
-            k2 = rhs(points + dt * k1, t + dt)
+            k2 = rhs(checked(points + dt * k1, points, "heun_predictor", t + dt), t + dt)
             candidate = points + dt * (k1 + k2) / 2
         else:
-            k2 = rhs(points + dt * k1 / 2, t + dt / 2)
-            k3 = rhs(points + dt * k2 / 2, t + dt / 2)
-            k4 = rhs(points + dt * k3, t + dt)
+            k2 = rhs(checked(points + dt * k1 / 2, points, "rk4_stage2", t + dt / 2), t + dt / 2)
+            k3 = rhs(checked(points + dt * k2 / 2, points, "rk4_stage3", t + dt / 2), t + dt / 2)
+            k4 = rhs(checked(points + dt * k3, points, "rk4_stage4", t + dt), t + dt)
             candidate = points + dt * (k1 + 2 * k2 + 2 * k3 + k4) / 6
-        points = candidate
+        points = checked(candidate, points, "accepted_geometry", t + dt)
     return points

@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
     for name in names:
         assert hasattr(lib, name), name
     assert set(names) == set(_lib.SIGNATURES), "the ctypes signature table must cover the header exactly"
-    assert _lib.load().ofw_abi_version() == 2
+    assert _lib.load().ofw_abi_version() == 3
 
 
 def test_settings_validation():

@@ -9,7 +9,7 @@ from oracle import convection_np as C
 from openfawe_amd import _lib
 from openfawe_amd import convection as CV
 from openfawe_amd import vortex as V
-from tests.test_oracle_convection import wing_wake
+from tests.test_oracle_convection import RecordingGuard, wing_wake
 
 pytestmark = pytest.mark.gpu
 
@@ -128,14 +128,30 @@ def test_convect_geometry_matches_the_oracle(integrator):
     nodes, edges, gamma, first, ring_first = wing_wake(coincide=False)
     disp = np.zeros((first, 3))
     disp[:, 2] = 0.03
-    kw = dict(inflow=[-4.0, 0.1, 0.2], time=0.5, dt_s=0.05, substeps=2, integrator=integrator)
+    drift = np.random.default_rng(3).normal(size=(len(nodes) - ring_first, 3))      # the rings' sheet drift
+    kw = dict(inflow=[-4.0, 0.1, 0.2], time=0.5, dt_s=0.05, substeps=2, integrator=integrator, ring_drift=drift,
+              wing="wing")
+    host_guard, dev_guard = RecordingGuard(), RecordingGuard()
     want = C.convect_geometry(nodes[:first], nodes[first:], edges, gamma, 0.15, 2, displacement=disp,
-                              ring_first=ring_first, ring_free=7, free_convection_rows=3, **kw)
+                              ring_first=ring_first, ring_free=7, free_convection_rows=3, guard=host_guard, **kw)
     settings = CV.ConvectionSettings(model=2, free_convection_rows=3)
     got = CV.convect_geometry(dev(nodes[:first]), dev(nodes[first:]), dev(edges), dev(gamma), 0.15, settings,
-                              displacement=dev(disp), ring_first=ring_first, ring_free=7, **kw).cpu().numpy()
+                              displacement=dev(disp), ring_first=ring_first, ring_free=7, guard=dev_guard,
+                              **kw).cpu().numpy()
     step = np.abs(want - nodes[first:]).max()
-    np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * step + 4 * np.spacing(np.abs(want).max()))
+    tol = 1e-12 * step + 4 * np.spacing(np.abs(want).max())
+    np.testing.assert_allclose(got, want, rtol=0, atol=tol)
+    # the guard's calls: the same stages, times and wing, rows within the same bar
+    assert [c[:2] for c in dev_guard.calls] == [c[:2] for c in host_guard.calls] and dev_guard.calls
+    for (_, _, old_d, new_d, wing_d), (_, _, old_h, new_h, _) in zip(dev_guard.calls, host_guard.calls):
+        assert wing_d == "wing" and old_d.shape == old_h.shape == new_d.shape
+        np.testing.assert_allclose(old_d, old_h, rtol=0, atol=tol)
+        np.testing.assert_allclose(new_d, new_h, rtol=0, atol=tol)
+    # the guard's exception stops the device step too
+    with pytest.raises(RuntimeError, match="surface crossed"):
+        CV.convect_geometry(dev(nodes[:first]), dev(nodes[first:]), dev(edges), dev(gamma), 0.15, settings,
+                            displacement=dev(disp), ring_first=ring_first, ring_free=7,
+                            guard=RecordingGuard(fail_at=1), **kw)
 
 
 def test_tree_engine_and_errors():

@@ -1,14 +1,15 @@
 """The geometry step recorded as one graph (convection.GeometryStepPlan, ofw_step_*) against the stage-by-stage
 convect_geometry: bit for bit on every replay (with eager work and synchronisations between replays), after the
 inputs change, without the graph, and through the fallbacks (a flagged ring target, a flagged row, a capacity
-overflow that makes the plan again)."""
+overflow that makes the plan again); the rings' sheet drift recorded into the graph, and the surface guard's calls
+as the stage-by-stage step makes them."""
 import numpy as np
 import pytest
 
 from openfawe_amd import _lib
 from openfawe_amd import convection as CV
 from openfawe_amd import vortex as V
-from tests.test_oracle_convection import wing_wake
+from tests.test_oracle_convection import RecordingGuard, wing_wake
 
 pytestmark = pytest.mark.gpu
 
@@ -123,4 +124,37 @@ def test_direct_engine_replays(precision, model, integrator):
     for _ in range(2):
         assert plan.run()[0] == 0
         assert np.array_equal(plan.points_out.cpu().numpy().view(np.int64), want.view(np.int64))
+    plan.close()
+
+
+@pytest.mark.parametrize("integrator,substeps", [("euler", 1), ("heun", 2)])
+def test_sheet_drift_and_surface_guard(integrator, substeps):
+    print(f"step plan: sheet drift and guard, {integrator} x{substeps}", flush=True)
+    nodes, edges, gamma, first, ring_first, disp = case()
+    drift = np.random.default_rng(8).normal(size=(len(nodes) - ring_first, 3))
+    settings = CV.ConvectionSettings(model=2, backend="hip_tree", tree=TREE.tree, hip=TREE.hip,
+                                     free_convection_rows=4)
+    kw = dict(inflow=[-4.0, 0.1, 0.2], time=0.5, dt_s=0.05, substeps=substeps, integrator=integrator,
+              displacement=dev(disp), ring_first=ring_first, ring_free=9, ring_drift=drift)
+    want = eager(nodes, edges, gamma, first, settings, **kw)
+    assert not np.array_equal(want, eager(nodes, edges, gamma, first, settings, **dict(kw, ring_drift=None)))
+    bound, pts, e, g = dev(nodes[:first]), dev(nodes[first:]), dev(edges), dev(gamma)
+    plan = CV.GeometryStepPlan(bound, pts, e, g, 0.15, settings, **kw)
+    for _ in range(2):                                   # the drift is recorded into every stage's velocity
+        assert plan.run()[0] == 0
+        assert np.array_equal(plan.points_out.cpu().numpy().view(np.int64), want.view(np.int64))
+    plan.close()
+    # a guard: the plan's step calls it as the stage-by-stage step does, on the same rows
+    eager_guard, plan_guard = RecordingGuard(), RecordingGuard()
+    eager(nodes, edges, gamma, first, settings, guard=eager_guard, wing="wing", **kw)
+    plan = CV.GeometryStepPlan(bound, pts, e, g, 0.15, settings, guard=plan_guard, wing="wing", **kw)
+    got = plan.step().cpu().numpy()
+    assert np.array_equal(got.view(np.int64), want.view(np.int64))
+    assert [c[:2] for c in plan_guard.calls] == [c[:2] for c in eager_guard.calls] and plan_guard.calls
+    for a, b in zip(plan_guard.calls, eager_guard.calls):
+        assert np.array_equal(a[2], b[2]) and np.array_equal(a[3], b[3]) and a[4] == b[4] == "wing"
+    plan.close()
+    plan = CV.GeometryStepPlan(bound, pts, e, g, 0.15, settings, guard=RecordingGuard(fail_at=1), wing="wing", **kw)
+    with pytest.raises(RuntimeError, match="surface crossed"):
+        plan.step()
     plan.close()

@@ -158,3 +158,83 @@ def test_convection_settings_validation():
         with pytest.raises(ValueError):
             CV.ConvectionSettings(**bad)
     assert CV.ConvectionSettings(backend="hip_tree", tree=TreeSettings()).model == 2
+
+
+class RecordingGuard:
+    """Stands in for the reference's StaticSurfaceGuard: records every check_rows call, raises on ``fail_at``."""
+
+    def __init__(self, fail_at=None):
+        self.calls, self.fail_at = [], fail_at
+
+    def check_rows(self, old_rows, new_rows, wing, *, stage, time_s):
+        self.calls.append((stage, time_s, np.array(old_rows), np.array(new_rows), wing))
+        if self.fail_at is not None and len(self.calls) == self.fail_at:
+            raise RuntimeError(f"surface crossed at {stage}")
+
+
+@pytest.mark.parametrize("integrator", ["euler", "heun", "rk4"])
+def test_sheet_drift_moves_the_frozen_rings_by_dt_times_drift(integrator):
+    # without a free wake every stage's velocity is U (+ the drift of the rings beyond ring_free): closed form
+    nodes, edges, gamma, first, ring_first = wing_wake(rings=3)
+    rows, rings_nodes = ring_first - first, len(nodes) - ring_first
+    drift = np.random.default_rng(4).normal(size=(rings_nodes, 3))
+    U = np.array([-6.0, 0.25, -0.5])
+    kw = dict(inflow=U, time=0.2, dt_s=0.08, substeps=2, integrator=integrator, free_wake=False,
+              ring_first=ring_first, ring_free=8)
+    out = C.convect_geometry(nodes[:first], nodes[first:], edges, gamma, 0.15, 2, ring_drift=drift, **kw)
+    want = nodes[first:] + 0.08 * U
+    want[rows + 8:] = nodes[ring_first + 8:] + 0.08 * (U + drift[8:])
+    np.testing.assert_allclose(out, want, rtol=0, atol=1e-14)
+    # no ring_free (every ring convects freely) or a drift array no longer than ring_free: no drift at all
+    plain = C.convect_geometry(nodes[:first], nodes[first:], edges, gamma, 0.15, 2, **kw)
+    for other in (dict(kw, ring_free=None), dict(kw, ring_free=rings_nodes)):
+        assert np.array_equal(C.convect_geometry(nodes[:first], nodes[first:], edges, gamma, 0.15, 2,
+                                                 ring_drift=drift, **other),
+                              C.convect_geometry(nodes[:first], nodes[first:], edges, gamma, 0.15, 2, **other))
+    assert not np.array_equal(out, plain)
+
+
+def test_sheet_drift_with_a_free_wake_is_linear_for_euler():
+    # Euler evaluates the velocity once, at the start points: the drift adds dt * drift to the frozen rings only
+    nodes, edges, gamma, first, ring_first = wing_wake(rings=3, coincide=False)
+    drift = np.random.default_rng(5).normal(size=(len(nodes) - ring_first, 3))
+    kw = dict(inflow=[-3.0, 0.0, 0.1], time=0.0, dt_s=0.05, integrator="euler", ring_first=ring_first,
+              ring_free=11)
+    args = (nodes[:first], nodes[first:], edges, gamma, 0.15, 2)
+    plain = C.convect_geometry(*args, **kw)
+    drifted = C.convect_geometry(*args, ring_drift=drift, **kw)
+    split = ring_first - first + 11
+    assert np.array_equal(drifted[:split], plain[:split])
+    np.testing.assert_allclose(drifted[split:] - plain[split:], 0.05 * drift[11:], rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize("integrator,labels", [
+    ("euler", ["accepted_geometry"]),
+    ("heun", ["heun_predictor", "accepted_geometry"]),
+    ("rk4", ["rk4_stage2", "rk4_stage3", "rk4_stage4", "accepted_geometry"])])
+def test_guard_sees_every_stage_point_of_the_wing_rows(integrator, labels):
+    nodes, edges, gamma, first, ring_first = wing_wake(coincide=False)
+    rows = ring_first - first
+    guard, wing = RecordingGuard(), object()
+    kw = dict(inflow=[-3.0, 0.0, 0.0], time=1.0, dt_s=0.1, substeps=2, integrator=integrator, ring_first=ring_first)
+    args = (nodes[:first], nodes[first:], edges, gamma, 0.15, 2)
+    out = C.convect_geometry(*args, guard=guard, wing=wing, **kw)
+    assert np.array_equal(out, C.convect_geometry(*args, **kw))        # the guard changes nothing it accepts
+    assert [c[0] for c in guard.calls] == labels * 2
+    offsets = {"heun_predictor": 1.0, "rk4_stage2": 0.5, "rk4_stage3": 0.5, "rk4_stage4": 1.0,
+               "accepted_geometry": 1.0}
+    half = C.convect_geometry(*args, **dict(kw, dt_s=0.05, substeps=1))   # the state after the first substep
+    for i, (label, t, old, new, seen) in enumerate(guard.calls):
+        sub = i // len(labels)
+        assert seen is wing
+        assert t == pytest.approx(1.0 + 0.05 * (sub + offsets[label]), abs=1e-15)
+        assert old.shape == new.shape == (rows // first, first, 3)
+        start = nodes[first:ring_first] if sub == 0 else half[:rows]
+        assert np.array_equal(old.reshape(-1, 3), start)
+    # the accepted rows of each substep are the rows that substep hands on, the last ones the result
+    accepted = [c for c in guard.calls if c[0] == "accepted_geometry"]
+    assert np.array_equal(accepted[0][3].reshape(-1, 3), half[:rows])
+    assert np.array_equal(accepted[1][3].reshape(-1, 3), out[:rows])
+    # an exception from the guard stops the step
+    with pytest.raises(RuntimeError, match="surface crossed"):
+        C.convect_geometry(*args, guard=RecordingGuard(fail_at=len(labels)), wing=wing, **kw)
