@@ -63,6 +63,29 @@ __device__ __forceinline__ int accept_sure(double x0, double x1, double x2, cons
   return !sane ? -1 : (yes ? 1 : (no ? 0 : -1));
 }
 
+// accept_sure split in two for a node tested by many lanes: the node's thresholds once (above t_yes the test
+// accepts, below t_no it rejects; infinite when the node's b2 or c2 is out of range), then per lane its d2 against
+// them.  Both answers equal accept_sure's: d2 > b2(1 + 1e-9) && d2 > c2(1 + 1e-9) is d2 > their maximum, and
+// d2 < b2(1 - 1e-9) || d2 < c2(1 - 1e-9) is d2 < theirs.
+__device__ __forceinline__ void accept_thresholds(double bound, double clear, double inv_theta2, double& t_yes,
+                                                  double& t_no) {
+#pragma clang fp contract(off)
+  const double b2 = (bound * bound) * inv_theta2;
+  const double sum = bound + clear;
+  const double c2 = sum * sum;
+  const bool sane = b2 <= 1e200 && c2 <= 1e200;
+  t_yes = sane ? fmax(b2 * (1.0 + 1e-9), c2 * (1.0 + 1e-9)) : __builtin_inf();
+  t_no = sane ? fmax(b2 * (1.0 - 1e-9), c2 * (1.0 - 1e-9)) : -__builtin_inf();
+}
+__device__ __forceinline__ int accept_sure_t(double x0, double x1, double x2, double cx, double cy, double cz,
+                                             double t_yes, double t_no) {
+#pragma clang fp contract(off)
+  const double rx = x0 - cx, ry = x1 - cy, rz = x2 - cz;
+  const double d2 = rx * rx + ry * ry + rz * rz;
+  const bool sane = d2 >= 1e-200 && d2 <= 1e200;
+  return sane && d2 > t_yes ? 1 : (sane && d2 < t_no ? 0 : -1);
+}
+
 // The same test on a record given as values (centre, bound, clear).
 __device__ __forceinline__ bool accept_test5(double x0, double x1, double x2, const double* rec, double theta,
                                              double& rx, double& ry, double& rz, double& dist) {
@@ -218,6 +241,13 @@ __device__ __forceinline__ int find_tree(const int32_t* __restrict__ table, int 
 // one per lane in a window of 64 consecutive tree positions and broadcast with readlane.
 static constexpr int WIN = 32;
 static_assert(WIN * ACC_REC % 128 == 0, "the node window is loaded in 16-byte chunks by 64 lanes");
+// The breadth-first walk: a FIFO of nodes in LDS (children queued in pairs that share one lane mask), the
+// records of 64 queued nodes loaded together, and a bitmap of the list entries over the subtree's pre-order
+// indices (two bits per node) that ranks every entry into pre-order.  Subtrees of more than BFS_NODES nodes,
+// and walks whose queue or list outgrows its bound, take the stackless walk.
+static constexpr int QCAP = 512;
+static constexpr int BFS_NODES = 8192;
+static_assert((QCAP & (QCAP - 1)) == 0, "the queue is a power-of-two ring");
 static constexpr int EVAL_WAVES = 4;
 // The walk and the list kernel run one wavefront per workgroup: their work items differ widely in cost, and a
 // workgroup keeps its slot (and its LDS) until its slowest wavefront ends, so single-wavefront workgroups let the
@@ -237,9 +267,16 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_tree_eval(ForestTables ft, 
                                                    int32_t* __restrict__ pnear, int8_t* __restrict__ pnz,
                                                    NearEntry* __restrict__ list, int32_t* __restrict__ list_count) {
   typedef typename std::conditional<PREC == OFW_FP32, float, double>::type acc_t;
-  __shared__ double s_acc[NEAR_WAVES][WIN * ACC_REC];
+  // the stackless walk's node window, or the breadth-first walk's test records of one batch (8 doubles per node)
+  __shared__ double s_acc[NEAR_WAVES][64 * 8];
+  __shared__ int s_q[NEAR_WAVES][QCAP];
+  __shared__ unsigned long long s_qm[NEAR_WAVES][QCAP / 2];
+  __shared__ uint32_t s_bits[NEAR_WAVES][BFS_NODES / 16];
+  __shared__ int s_cp[NEAR_WAVES][64];
   const int wv = threadIdx.x >> 6;
-  const int wave = uniform(xcd_block(blockIdx.x, gridDim.x) * NEAR_WAVES + wv);
+  // Workgroups go to the XCDs round-robin: the long walks, which belong to neighbouring groups of one task, are
+  // spread over the XCDs and their compute units instead of sharing SIMDs.
+  const int wave = uniform((int)blockIdx.x * NEAR_WAVES + wv);
   // whole wavefront beyond the work, or a plan that overflowed its capacity (reported by the host)
   if (wave >= ft.counters[C_WORK] || ft.counters[C_OVERFLOW]) return;
   const int lane = threadIdx.x & 63;
@@ -256,7 +293,6 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_tree_eval(ForestTables ft, 
   const bool valid = jl < nt;
   const int j = t0 + jl;
   double x0 = 0.0, x1 = 0.0, x2 = 0.0;
-  // exclusion cursor: the lane's next excluded tree position (INT_MAX when none is left)
   int cur = 0, cend = 0;
   if (valid) {
     x0 = pts[3 * (size_t)j];
@@ -267,25 +303,30 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_tree_eval(ForestTables ft, 
       cend = ex_ptr[j + 1];
     }
   }
-  // the next four excluded positions are held in registers (nextex, e1, e2, e3), so that passing one costs
-  // no memory access; they are reloaded four at a time when a lane has more
-  int nextex = INT_MAX, e1 = INT_MAX, e2 = INT_MAX, e3 = INT_MAX;
-  auto refill = [&]() {
-    nextex = cur < cend ? ex_pos[cur] : INT_MAX;
-    e1 = cur + 1 < cend ? ex_pos[cur + 1] : INT_MAX;
-    e2 = cur + 2 < cend ? ex_pos[cur + 2] : INT_MAX;
-    e3 = cur + 3 < cend ? ex_pos[cur + 3] : INT_MAX;
-  };
-  refill();
-  auto advance_to = [&](int pos) {
-    while (nextex < pos) {
-      ++cur;
-      nextex = e1;
-      e1 = e2;
-      e2 = e3;
-      e3 = INT_MAX;
-      if (nextex == INT_MAX && cur < cend) refill();
+  // The lane's excluded tree positions (sorted): the first eight in registers and their range, so that the
+  // test of a node's segment range [lo, hi) needs no memory access (and no wait on the wavefront's stores) unless
+  // the lane has more than eight and the range reaches past the eighth.
+  int xs[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) xs[u] = cur + u < cend ? ex_pos[cur + u] : INT_MAX;
+  const int xmin = xs[0];
+  const int xmax = cend > cur ? ex_pos[cend - 1] : INT_MIN;
+  auto has_ex = [&](int lo, int hi) -> bool {
+    if (hi <= xmin || lo > xmax) return false;
+    bool h = false;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) h = h || (xs[u] >= lo && xs[u] < hi);
+    if (!h && cend - cur > 8 && xs[7] < hi) {
+      for (int c = cur + 8; c < cend; ++c) {
+        const int v = ex_pos[c];
+        if (v >= hi) break;
+        if (v >= lo) {
+          h = true;
+          break;
+        }
+      }
     }
+    return h;
   };
   acc_t ux = 0, uy = 0, uz = 0;
   int flag = OFW_FLAG_OK, far = 0, near = 0;
@@ -334,10 +375,7 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_tree_eval(ForestTables ft, 
       const int alo = __builtin_amdgcn_readlane(alo_l, a_i), ahi = __builtin_amdgcn_readlane(ahi_l, a_i);
       double rx, ry, rz, dist;
       bool acc = alive && accept_test5(x0, x1, x2, rec, theta, rx, ry, rz, dist);
-      if (acc) {
-        advance_to(alo);
-        acc = !(nextex < ahi);
-      }
+      if (acc) acc = !has_ex(alo, ahi);
       if (acc) {
         alive = false;
         // the subtree that holds the ancestor's leftmost segment adds it, where the reference's walk does
@@ -349,7 +387,6 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_tree_eval(ForestTables ft, 
     }
   }
 
-  // ---- stackless walk of the frontier subtree through the node window
   const int end = uniform(reinterpret_cast<const int32_t*>(accr + (size_t)F * ACC_REC + A_INTS)[2]);
   int wbase = INT_MIN / 2;
   auto load_window = [&](int base) {
@@ -377,60 +414,262 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_tree_eval(ForestTables ft, 
   const int lcap = uniform(ft.list_cap[k]);
   const size_t lbase = (size_t)uniform(ft.list_base[k]) + (size_t)local * lcap;
   int nlist = 0;
-  int i = F;
-  int skip_until = F;
-  while (i < end) {
-    const bool act = alive && i >= skip_until;
-    if (!wave_any(act)) break;
-    if (i >= wbase + WIN) load_window(i);
-    const double* a = &s_acc[wv][(i - wbase) * ACC_REC];
-    // (lo, hi) and (skip, tree) are int32 pairs in two record slots
-    const long long ints0 = __double_as_longlong(a[A_INTS]), ints1 = __double_as_longlong(a[A_INTS2]);
-    const int lo = uniform((int)(ints0 & 0xffffffffll)), hi = uniform((int)(ints0 >> 32));
-    const int skip = uniform((int)(ints1 & 0xffffffffll));
-    int sure = 0;
-    if (act) sure = accept_sure(x0, x1, x2, a, inv_theta2);
-    bool acc = sure > 0;
-    if (wave_any(act && sure < 0)) {
-      if (act && sure < 0) {
-        double rx, ry, rz, dist;
-        acc = accept_test(x0, x1, x2, a, theta, rx, ry, rz, dist);
+  bool walked = false;
+  if (end - F <= BFS_NODES) {
+    // ---- breadth-first walk of the frontier subtree.  A batch of up to 64 queued nodes is taken in two
+    // phases: (A) every node's acceptance test for every lane, four nodes at a time so that their arithmetic
+    // overlaps, the node's accept and open lane masks written into the lane that holds the node; (B) every lane's
+    // node emits its list entries and queues its children at once, at offsets from one scan over the lanes.
+    const int far0 = far;
+    const int words = (2 * (end - F) + 31) >> 5;
+    for (int w = lane; w < words; w += 64) s_bits[wv][w] = 0u;
+    const unsigned long long alive_mask = __builtin_amdgcn_ballot_w64(alive);
+    int head = 0, tail = 0;
+    if (alive_mask) {
+      if (lane == 0) {
+        s_q[wv][0] = F;
+        s_q[wv][1] = -1;
+        s_qm[wv][0] = alive_mask;
       }
+      tail = 2;
     }
-    if (act) {
-      advance_to(lo);
-      if (acc) acc = !(nextex < hi);
-    }
-    if (wave_any(acc)) {
-      // the accepted node goes to the wavefront's interaction list with the lanes that accepted it; the list
-      // kernel adds its far field (vortex_tree.py:526-537 for each of them)
-      const unsigned long long amask = __builtin_amdgcn_ballot_w64(acc);
-      if (nlist < lcap) {
-        if (lane == 0) list[lbase + nlist] = NearEntry{-1 - i, 0, (uint32_t)amask, (uint32_t)(amask >> 32)};
-        ++nlist;
-      } else if (lane == 0) {
-        atomicOr(const_cast<int32_t*>(ft.counters) + C_OVERFLOW, 1);
+    wave_lds_sync();
+    int nb = 0;
+    bool ok = true;
+    while (head < tail) {
+      const int nbatch = min(64, tail - head);
+      int my_node = -1;
+      unsigned long long my_mask = 0ull;
+      if (lane < nbatch) {
+        const int idx = (head + lane) & (QCAP - 1);
+        my_node = s_q[wv][idx];
+        my_mask = s_qm[wv][idx >> 1];
       }
-      if (acc) ++far;
+      head += nbatch;
+      // The batch's records, one per lane (queued nodes lie in [F, end): children have larger pre-order indices
+      // within the frontier subtree, so the walk ends).  The lane that holds a node writes its test record to LDS
+      // for broadcast reads: the centre, the two squared-distance thresholds of accept_sure (above t_yes it
+      // accepts, below t_no it rejects, in between the reference's operations decide), (lo, hi), the lane mask
+      // and the node index.
+      const bool mine = my_node >= F && my_node < end;
+      int r_skip = 0, r_right = -1;
+      {
+        double cx = 0.0, cy = 0.0, cz = 0.0, t_yes = 0.0, t_no = 0.0;
+        long long lohi = 0;
+        if (mine) {
+          const double2* src = reinterpret_cast<const double2*>(accr + (size_t)my_node * ACC_REC);
+          const double2 v0 = src[0], v1 = src[1], v2 = src[2], v3 = src[3];
+          cx = v0.x;
+          cy = v0.y;
+          cz = v1.x;
+          const double bound = v1.y, clear = v2.x;
+          lohi = __double_as_longlong(v2.y);
+          const long long i1 = __double_as_longlong(v3.x), i2 = __double_as_longlong(v3.y);
+          r_skip = (int)(i1 & 0xffffffffll);
+          r_right = (int)(i2 & 0xffffffffll);
+          accept_thresholds(bound, clear, inv_theta2, t_yes, t_no);
+        }
+        double* t = &s_acc[wv][lane * 8];
+        t[0] = cx;
+        t[1] = cy;
+        t[2] = cz;
+        t[3] = t_yes;
+        t[4] = t_no;
+        t[5] = __longlong_as_double(lohi);
+        t[6] = __longlong_as_double(mine ? (long long)my_mask : 0ll);
+        t[7] = __longlong_as_double((long long)my_node);
+      }
+      // Wait here, once per batch, for the loads above: a wait on a load inside the loops below would also wait
+      // for every list store issued before it (loads and stores share one counter).
+      __builtin_amdgcn_s_waitcnt(0);
+      wave_lds_sync();
+      // (A) lane e collects node e's masks
+      int am_lo = 0, am_hi = 0, om_lo = 0, om_hi = 0;
+      for (int e0 = 0; e0 < nbatch; e0 += 4) {
+        bool act[4], acc[4], unc[4];
+        int lo[4], hi[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const double* t = &s_acc[wv][min(e0 + u, 63) * 8];
+          const double cx = t[0], cy = t[1], cz = t[2], t_yes = t[3], t_no = t[4];
+          const long long lohi = __double_as_longlong(t[5]);
+          const unsigned long long am = (unsigned long long)__double_as_longlong(t[6]);
+          lo[u] = (int)(lohi & 0xffffffffll);
+          hi[u] = (int)(lohi >> 32);
+          act[u] = e0 + u < nbatch && ((am >> lane) & 1ull);
+          const int sure = accept_sure_t(x0, x1, x2, cx, cy, cz, t_yes, t_no);
+          acc[u] = act[u] && sure > 0;
+          unc[u] = act[u] && sure < 0;
+        }
+        if (wave_any(unc[0] || unc[1] || unc[2] || unc[3])) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (unc[u]) {
+              const double* t = &s_acc[wv][min(e0 + u, 63) * 8];
+              const int node = (int)__double_as_longlong(t[7]);
+              const double* g = accr + (size_t)node * ACC_REC;
+              double rx, ry, rz, dist;
+              acc[u] = accept_test(x0, x1, x2, g, theta, rx, ry, rz, dist);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (acc[u]) acc[u] = !has_ex(lo[u], hi[u]);
+          far += acc[u] ? 1 : 0;
+          const unsigned long long amask = __builtin_amdgcn_ballot_w64(acc[u]);
+          const unsigned long long omask = __builtin_amdgcn_ballot_w64(act[u] && !acc[u]);
+          if (lane == e0 + u) {
+            am_lo = (int)(uint32_t)amask;
+            am_hi = (int)(uint32_t)(amask >> 32);
+            om_lo = (int)(uint32_t)omask;
+            om_hi = (int)(uint32_t)(omask >> 32);
+          }
+        }
+      }
+      // (B) list entries (an accepted node, then an opened leaf) staged from the top of the wavefront's list,
+      // their bits marking their pre-order places, and both children of an opened node queued with the lanes
+      // that opened it (vortex_tree.py:526-561 for each lane)
+      const unsigned long long my_am = ((unsigned long long)(uint32_t)am_hi << 32) | (uint32_t)am_lo;
+      const unsigned long long my_om = ((unsigned long long)(uint32_t)om_hi << 32) | (uint32_t)om_lo;
+      const bool leaf = mine && r_skip == my_node + 1;
+      const int n_ent = mine ? (my_am ? 1 : 0) + (leaf && my_om ? 1 : 0) : 0;
+      const int n_push = mine && !leaf && my_om ? 2 : 0;
+      const int packed = n_ent | (n_push << 16);  // at most 2 each per lane: 16-bit fields suffice
+      int incl = packed;
+      for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += v;
+      }
+      const int total = __builtin_amdgcn_readlane(incl, 63);
+      const int tot_ent = total & 0xffff, tot_push = total >> 16;
+      if (2 * (nb + tot_ent) > lcap || tail + tot_push - head > QCAP) {
+        ok = false;
+        break;
+      }
+      if (mine) {
+        const int before = incl - packed;
+        const int code = 2 * (my_node - F);
+        int b = nb + (before & 0xffff);
+        if (my_am) {
+          list[lbase + lcap - 1 - b] = NearEntry{code, 0, (uint32_t)my_am, (uint32_t)(my_am >> 32)};
+          atomicOr(&s_bits[wv][code >> 5], 1u << (code & 31));
+          ++b;
+        }
+        if (leaf && my_om) {
+          list[lbase + lcap - 1 - b] = NearEntry{code + 1, 0, (uint32_t)my_om, (uint32_t)(my_om >> 32)};
+          atomicOr(&s_bits[wv][(code + 1) >> 5], 1u << ((code + 1) & 31));
+        }
+        if (n_push) {
+          const int t = tail + (before >> 16);
+          s_q[wv][t & (QCAP - 1)] = my_node + 1;
+          s_q[wv][(t + 1) & (QCAP - 1)] = r_right;
+          s_qm[wv][(t & (QCAP - 1)) >> 1] = my_om;
+        }
+      }
+      nb += tot_ent;
+      tail += tot_push;
+      wave_lds_sync();  // the queue and bitmap writes come before the next batch's reads
     }
-    const bool open = act && !acc;
-    if (skip == i + 1) {
-      // leaf: the near field of the lanes that did not accept it (vortex_tree.py:538-557)
-      if (wave_any(open)) {
-        // the leaf goes to the wavefront's near-field list with the lanes that opened it; the near-field
-        // kernel evaluates the lists in short chunks spread over the whole GPU
-        const unsigned long long mask = __builtin_amdgcn_ballot_w64(open);
+    if (ok) {
+      // rank of each entry: its bit's place among the set bits (a scan of per-lane runs of words)
+      const int per = (words + 63) >> 6;
+      int cnt = 0;
+      for (int w = lane * per; w < min(words, lane * per + per); ++w) cnt += __popc(s_bits[wv][w]);
+      int incl = cnt;
+      for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += v;
+      }
+      s_cp[wv][lane] = incl - cnt;
+      wave_lds_sync();
+      // the staged entries were written by other lanes: their stores complete first, and they are read from L2
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_s_waitcnt(0);
+      for (int b0 = 0; b0 < nb; b0 += 64) {
+        const int b = b0 + lane;
+        if (b < nb) {
+          unsigned long long* sp = reinterpret_cast<unsigned long long*>(list + lbase + lcap - 1 - b);
+          const unsigned long long w0 = __hip_atomic_load(sp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          const unsigned long long w1 = __hip_atomic_load(sp + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          const NearEntry st{(int32_t)(uint32_t)w0, (int32_t)(w0 >> 32), (uint32_t)w1, (uint32_t)(w1 >> 32)};
+          const int code = st.lo;
+          if (code >= 0 && code < 2 * (end - F)) {
+            const int w = code >> 5;
+            const int c = w / per;
+            int r = s_cp[wv][c];
+            for (int u = c * per; u < w; ++u) r += __popc(s_bits[wv][u]);
+            r += __popc(s_bits[wv][w] & ((1u << (code & 31)) - 1u));
+            const int node = F + (code >> 1);
+            NearEntry out{-1 - node, 0, st.mask_lo, st.mask_hi};
+            if (code & 1) {
+              const int32_t* ai = reinterpret_cast<const int32_t*>(accr + (size_t)node * ACC_REC + A_INTS);
+              out.lo = ai[0];
+              out.hi = ai[1];
+            }
+            if (r < nb) list[lbase + r] = out;
+          }
+        }
+      }
+      nlist = nb;
+      walked = true;
+    } else {
+      far = far0;
+    }
+  }
+  if (!walked) {
+    // ---- stackless walk of the frontier subtree through the node window
+    int i = F;
+    int skip_until = F;
+    while (i < end) {
+      const bool act = alive && i >= skip_until;
+      if (!wave_any(act)) break;
+      if (i >= wbase + WIN) load_window(i);
+      const double* a = &s_acc[wv][(i - wbase) * ACC_REC];
+      // (lo, hi) and (skip, tree) are int32 pairs in two record slots
+      const long long ints0 = __double_as_longlong(a[A_INTS]), ints1 = __double_as_longlong(a[A_INTS2]);
+      const int lo = uniform((int)(ints0 & 0xffffffffll)), hi = uniform((int)(ints0 >> 32));
+      const int skip = uniform((int)(ints1 & 0xffffffffll));
+      int sure = 0;
+      if (act) sure = accept_sure(x0, x1, x2, a, inv_theta2);
+      bool acc = sure > 0;
+      if (wave_any(act && sure < 0)) {
+        if (act && sure < 0) {
+          double rx, ry, rz, dist;
+          acc = accept_test(x0, x1, x2, a, theta, rx, ry, rz, dist);
+        }
+      }
+      if (acc) acc = !has_ex(lo, hi);
+      if (wave_any(acc)) {
+        // the accepted node goes to the wavefront's interaction list with the lanes that accepted it; the list
+        // kernel adds its far field (vortex_tree.py:526-537 for each of them)
+        const unsigned long long amask = __builtin_amdgcn_ballot_w64(acc);
         if (nlist < lcap) {
-          if (lane == 0) list[lbase + nlist] = NearEntry{lo, hi, (uint32_t)mask, (uint32_t)(mask >> 32)};
+          if (lane == 0) list[lbase + nlist] = NearEntry{-1 - i, 0, (uint32_t)amask, (uint32_t)(amask >> 32)};
           ++nlist;
         } else if (lane == 0) {
           atomicOr(const_cast<int32_t*>(ft.counters) + C_OVERFLOW, 1);
         }
+        if (acc) ++far;
       }
-      i = i + 1;
-    } else {
-      if (acc) skip_until = skip;
-      i = wave_any(open) ? i + 1 : skip;
+      const bool open = act && !acc;
+      if (skip == i + 1) {
+        // leaf: the near field of the lanes that did not accept it (vortex_tree.py:538-557)
+        if (wave_any(open)) {
+          // the leaf goes to the wavefront's near-field list with the lanes that opened it; the near-field
+          // kernel evaluates the lists in short chunks spread over the whole GPU
+          const unsigned long long mask = __builtin_amdgcn_ballot_w64(open);
+          if (nlist < lcap) {
+            if (lane == 0) list[lbase + nlist] = NearEntry{lo, hi, (uint32_t)mask, (uint32_t)(mask >> 32)};
+            ++nlist;
+          } else if (lane == 0) {
+            atomicOr(const_cast<int32_t*>(ft.counters) + C_OVERFLOW, 1);
+          }
+        }
+        i = i + 1;
+      } else {
+        if (acc) skip_until = skip;
+        i = wave_any(open) ? i + 1 : skip;
+      }
     }
   }
   // partial sums only from wavefronts that added anything (the reduction reads the flag)

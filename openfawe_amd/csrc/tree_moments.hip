@@ -728,7 +728,10 @@ __global__ __launch_bounds__(256) void k_moments_final(TreeArrays t, NodeScratch
     ai[1] = t.p_hi[node];
     ai[2] = t.p_skip[node];
     ai[3] = t.p_tree[node];
-    a[A_PAD] = 0.0;
+    // the right child (the left one is node + 1), -1 for a leaf: a breadth-first walk queues both children
+    const int skip = t.p_skip[node];
+    ai[4] = skip != node + 1 ? t.p_skip[node + 1] : -1;
+    ai[5] = 0;
     double* f = far64 + (size_t)node * FAR_REC;
     float* g = far32 + (size_t)node * FAR_REC;
     double rec[FAR_REC];

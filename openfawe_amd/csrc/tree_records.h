@@ -91,9 +91,9 @@ struct NodeOutputs {
   double* w2 = nullptr;
 };
 
-// Acceptance record of a node (8 doubles, one 64-byte scalar load): centre, bounding radius,
-// clearance * largest core, and (lo, hi, skip, 0) as int32 in the last two slots but one.
-enum { A_CX = 0, A_CY, A_CZ, A_BOUND, A_CLEAR, A_INTS, A_INTS2, A_PAD, ACC_REC = 8 };
+// Acceptance record of a node (8 doubles, 64 bytes): centre, bounding radius, clearance * largest core, and
+// (lo, hi), (skip, tree), (right child or -1, 0) as int32 pairs in the last three slots.
+enum { A_CX = 0, A_CY, A_CZ, A_BOUND, A_CLEAR, A_INTS, A_INTS2, A_INTS3, ACC_REC = 8 };
 
 // Far-field record (48 values): m0 (3), m1 (9, row-major), w1 (3), symmetric m2 (3 x 6: 00 01 02 11 12 22,
 // the off-diagonal entries doubled), t2 (3), w2 (9, row-major), pad
