@@ -435,6 +435,73 @@ __device__ __forceinline__ bool pair_f32(float rx, float ry, float rz, const Rec
   return near;
 }
 
+// FP32 record of the tree's near field (8 floats, 32 B): the length vector l, L^2, Gamma / (4 pi), Q (L rc)^2 with
+// rc the core radius in metres and Q = 10/1.25643 the Lamb-Oseen cutoff of (h / rc)^2 (0 without a core), the FP64
+// flag, and the half-width of the band about the cutoff that takes FP64 (-1 without a core).  The near field evaluates the pair in
+// metres: u = Gamma / (4 pi) K(r1, r2) (l x r1), K = (|r1| + |r2|) / (|r1| |r2| (|r1| |r2| + r1.r2)), which is the
+// formula of vortex_numba.py:46-109 in segment lengths (a = r1 / L, c = d x a = (l x r1) / L^2,
+// g4 = Gamma / (4 pi L), K(a, b) = L^3 K(r1, r2)) with the powers of L cancelled; its thresholds carry the same
+// powers of L (h^2 <= k max(1, |a|^2, |b|^2) is |l x r1|^2 <= k L^2 max(L^2, |r1|^2, |r2|^2)).
+enum { N_LX = 0, N_LY, N_LZ, N_L2, N_GP, N_QLR2, N_EXACT, N_BAND, NEAR32 = 8 };
+struct Near32 {
+  float lx, ly, lz, l2, gp, qlr2, exact, band;
+};
+// Segments of length in [1e-6, 1e6] (every threshold product stays a normal float) with |Gamma| / (4 pi) and
+// (L rc)^2 inside [1e-30, 1e30] take the FP32 evaluation; targets beyond 1e9 of a segment's ends take FP64.
+#define OFW_NEAR_L2_MIN 1e-12
+#define OFW_NEAR_L2_MAX 1e12
+#define OFW_NEAR_R2_MAX 1e18f
+
+__device__ __forceinline__ Near32 load_near32(const float* __restrict__ f) {
+  const float4 a = reinterpret_cast<const float4*>(f)[0], b = reinterpret_cast<const float4*>(f)[1];
+  return Near32{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+}
+
+// The tree near field's FP32 pair in metres: r1 = target - start, C = l x r1 (formed in FP64 by the caller and
+// rounded, so h keeps its FP32 accuracy however close the target is to the line).  coef is the factor of C in the
+// velocity; the return value is true when the pair must be evaluated in FP64 instead: within NEAR_AXIS of the
+// line anywhere, within NEAR_BESIDE of it beside the segment, within LAMB_BAND32 of the Lamb-Oseen cutoff, or
+// outside the FP32 range (the criteria of pair_f32 with the cross product given).  Models NONE, RANKINE and LAMB
+// (the tree code's).
+template <int MODEL>
+__device__ __forceinline__ bool pair_near32(float rx, float ry, float rz, float cx, float cy, float cz,
+                                            const Near32& f, float& coef) {
+  const float r1 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
+  const float bx = rx - f.lx, by = ry - f.ly, bz = rz - f.lz;
+  const float r2 = __builtin_fmaf(bz, bz, __builtin_fmaf(by, by, bx * bx));
+  const float h2 = __builtin_fmaf(cz, cz, __builtin_fmaf(cy, cy, cx * cx));
+  const float t = __builtin_fmaf(rz, f.lz, __builtin_fmaf(ry, f.ly, rx * f.lx));  // r1 . l
+  const float m2 = fmaxf(f.l2, fmaxf(r1, r2));
+  const float lm = f.l2 * m2;
+  // every test evaluated (bitwise combination, no short-circuit): no branch per pair
+  const bool beside = (t >= 0.0f) & (t <= f.l2);
+  bool near = (!(m2 <= OFW_NEAR_R2_MAX)) | (h2 <= OFW_NEAR_AXIS2 * lm) | (f.exact != 0.0f) |
+              (beside & (h2 <= OFW_NEAR_BESIDE2 * lm));
+  // the cutoff decision q = h2 / (L rc)^2 <= 10/1.25643 within 1e-4 of it (no core: band -1, never)
+  if (MODEL == OFW_MODEL_LAMB) near = near | (fabsf(h2 - f.qlr2) <= f.band);
+  float factor = 1.0f;
+  if (MODEL == OFW_MODEL_RANKINE || MODEL == OFW_MODEL_LAMB) {
+    // core factors of q = (h / rc)^2 (vortex_numba.py:79-101) within 4 core radii of the line (beyond them both
+    // factors are 1; no core: qlr2 = 0, never)
+    const float qc = (float)OFW_LAMB_Q;
+    const bool core = !near && h2 < (float)(16.0 / OFW_LAMB_Q) * f.qlr2;
+    if (__builtin_amdgcn_ballot_w64(core) != 0 && core) {
+      const float q = qc * (h2 * __builtin_amdgcn_rcpf(f.qlr2));
+      if (MODEL == OFW_MODEL_RANKINE) factor = fminf(q, 1.0f);
+      else if (q <= (float)(10.0 / OFW_LAMB)) factor = -expm1f(-(float)OFW_LAMB * q);
+    }
+  }
+  const float sa = __builtin_amdgcn_sqrtf(r1), sb = __builtin_amdgcn_sqrtf(r2);
+  const float product = sa * sb;
+  const float dot = r1 - t;  // r1 . r2
+  // |r1| |r2| + r1.r2 = |l x r1|^2 / (|r1| |r2| - r1.r2) where r1.r2 < 0 (the rationalised form): one reciprocal
+  const bool rat = dot < 0.0f;
+  const float dd = rat ? product * h2 : product * (product + dot);
+  const float xm = rat ? product - dot : 1.0f;
+  coef = f.gp * factor * (sa + sb) * xm * __builtin_amdgcn_rcpf(dd);
+  return near;
+}
+
 // Deterministic first-flag bookkeeping: a lane keeps the first flagged segment in its own order.
 struct FlagState {
   int flag;

@@ -88,6 +88,59 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// ---- single-pass prefix over ordered work units (decoupled look-back)
+// Each unit publishes a 64-bit status word: its aggregate first, then its inclusive prefix.  Word layout: bits
+// 63..33 the caller's tag (words of another tag read as not ready, so the status array is cleared once and
+// reused with a new tag), bit 32 the inclusive flag, bits 31..0 the value (non-negative).  Units take their
+// indices from an atomic ticket in launch order, so every unit a look-back waits for was claimed earlier by a
+// running workgroup and finishes without waiting on a later one.
+__device__ __forceinline__ void lb_publish(unsigned long long* word, unsigned tag, bool inclusive, int value) {
+  const unsigned long long w = ((unsigned long long)tag << 33) | ((unsigned long long)(inclusive ? 1 : 0) << 32) |
+                               (unsigned long long)(uint32_t)value;
+  __hip_atomic_store(word, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+static constexpr int LB_SPIN_LIMIT = 1 << 20;  // polls of one window before the look-back gives up
+
+// Exclusive prefix of unit i over the units [first, i) whose status words are status[stride * u]; called by a
+// whole wavefront, the result on every lane.  Windows of 64 predecessors are read together; a window is used
+// once every unit back to its nearest inclusive word is ready.  *ok turns false (and the prefix is not the
+// sum) only when a window was polled LB_SPIN_LIMIT times: every wait ends.
+__device__ inline int lb_lookback(const unsigned long long* status, int stride, int first, int i, unsigned tag,
+                                  bool* ok) {
+  const int lane = threadIdx.x & 63;
+  int prefix = 0;
+  int j = i - 1;
+  int spins = 0;
+  while (j >= first) {
+    const int u = j - lane;
+    const bool in = u >= first;
+    unsigned long long w = 0;
+    if (in) w = __hip_atomic_load(status + (size_t)stride * u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const bool ready = in && (unsigned)(w >> 33) == tag;
+    const unsigned long long incl = __builtin_amdgcn_ballot_w64(ready && ((w >> 32) & 1ull));
+    const unsigned long long inmask = __builtin_amdgcn_ballot_w64(in);
+    // the lanes needed: up to the nearest inclusive word, or the whole window
+    const unsigned long long need =
+        incl ? (__builtin_ctzll(incl) == 63 ? ~0ull : ((2ull << __builtin_ctzll(incl)) - 1ull)) & inmask : inmask;
+    if ((__builtin_amdgcn_ballot_w64(ready) & need) != need) {
+      if (++spins >= LB_SPIN_LIMIT) {
+        *ok = false;
+        return prefix;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    int v = ((need >> lane) & 1ull) ? (int)(uint32_t)w : 0;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    prefix += v;
+    if (incl) break;
+    j -= 64;
+  }
+  return prefix;
+}
+
 // Workgroup index remapped so that each XCD (workgroups are dispatched to the 8 XCDs round-robin) receives a
 // contiguous range of logical indices: neighbouring work shares one XCD's L2.  A bijection on [0, nb).
 __device__ __forceinline__ int xcd_block(int b, int nb) {

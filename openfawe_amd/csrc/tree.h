@@ -30,6 +30,7 @@ enum {
   C_RUNMEM,         // their members
   C_COUNT = 18
 };
+static constexpr int LEVEL_MAX = 64;  // top levels of the list build (a tree of 2^28 segments has at most 18)
 
 // A device count, or 0 once a capacity has overflowed: every kernel after an overflow does no work, and the
 // host reports the overflow when the call has finished.
@@ -69,8 +70,8 @@ struct TreeArrays {
   int rs_tiles = 0;              // radix tiles per axis
   uint32_t* rkey[2] = {};        // 3 x m keys per buffer (axis-major): coordinates rounded to float32
   int32_t* rval[2] = {};         // 3 x m segment ids per buffer; the lists live here after the presort
-  uint8_t* lside = nullptr;      // 3 x m: 1 where a list entry goes to the left child of its node
-  int32_t* tcount = nullptr;     // tile_cap x 3 left counts of the partition
+  unsigned long long* lstat = nullptr;  // tile_cap x 3 look-back status words of the partition (tagged by level)
+  int32_t* tickets = nullptr;    // LEVEL_MAX work-unit tickets, one per top level
   int32_t* rcount = nullptr;     // 3 x 256 x rs_tiles digit counts of one presort pass, then their scans
   int32_t* rtotal = nullptr;     // 3 x 256 digit totals of the pass
   // runs of equal coordinates (at most 3m members in 3m/2 runs)

@@ -88,6 +88,11 @@ __global__ void k_init_roots(TreeArrays t) {
     t.level_a[k] = k;
   }
   if (threadIdx.x < C_COUNT) t.counters[threadIdx.x] = 0;
+  // the list build's work-unit tickets and look-back status words start empty in every call
+  if (t.lists) {
+    for (int i = threadIdx.x; i < LEVEL_MAX; i += blockDim.x) t.tickets[i] = 0;
+    for (int i = threadIdx.x; i < 3 * t.tile_cap; i += blockDim.x) t.lstat[i] = 0ull;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     t.counters[C_NODES] = t.K;
@@ -1142,134 +1147,148 @@ __device__ __forceinline__ void list_axis(const TreeArrays& t, const int32_t* __
   choose_axis(v, axis, split);
 }
 
-// Per tile and axis: the side of each of the tile's list entries (lside) and how many go to the left child.  The
-// node's sorted order (vortex_tree.py:95-101) is its list of the split axis, cut at (a+b)//2: an entry goes left
-// when its coordinate on that axis is below the median entry's, or equal to it and before it in that list (then
-// it is in a run of equal coordinates, whose list positions are kept).  The node's first unit records its axis
-// and split flag for the scatter and the next plan.
-static constexpr int LC_E = TILE / 256;  // entries per thread of the count kernel
-__global__ __launch_bounds__(256) void k_list_count(TreeArrays t, const int32_t* __restrict__ lst,
-                                                    const double* __restrict__ mx, const double* __restrict__ my,
-                                                    const double* __restrict__ mz) {
+// One top level of the list build in one launch: per work unit (a 4,096-entry tile of one node and one axis),
+// the side of each of the tile's list entries, its rank among the tile's left entries, the left entries of the
+// node's earlier tiles (single-pass look-back over the node's units of that axis), and the stable partition of
+// the list into the children's ranges [lo, mid) and [mid, hi); the units of the split axis write the node's
+// sorted order.  The node's sorted order (vortex_tree.py:95-101) is its list of the split axis, cut at
+// (a+b)//2: an entry goes left when its coordinate on that axis is below the median entry's, or equal to it
+// and before it in that list (then it is in a run of equal coordinates, whose list positions are kept).  The
+// node's first unit records its axis and split flag for the next plan.
+static constexpr int LL_THREADS = 256;
+static constexpr int LL_E = TILE / LL_THREADS;  // entries per thread (strided by LL_THREADS)
+static_assert(LL_E * (LL_THREADS / 64) == 64, "the per-(row, wavefront) left counts are scanned by one wavefront");
+__global__ __launch_bounds__(LL_THREADS) void k_list_level(TreeArrays t, int lvl, const int32_t* __restrict__ lst_in,
+                                                           int32_t* __restrict__ lst_out,
+                                                           const double* __restrict__ mx,
+                                                           const double* __restrict__ my,
+                                                           const double* __restrict__ mz) {
+  __shared__ int s_unit;
+  __shared__ int s_cell[64];  // left entries of each (row, wavefront), then their exclusive scan
+  __shared__ int s_prefix;
   const int ntiles = live_count(t.counters, C_TILES);
-  const int m = (int)t.m, A = blockIdx.y;
+  const int units = 3 * ntiles;
+  const int m = (int)t.m;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const unsigned tag = (unsigned)lvl + 1u;
   const double* cs[3] = {mx, my, mz};
-  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  for (;;) {
+    if (threadIdx.x == 0) s_unit = atomicAdd(&t.tickets[lvl], 1);
+    __syncthreads();
+    const int u = s_unit;
+    __syncthreads();  // every thread has read the ticket before the next one is drawn
+    if (u >= units) break;
+    const int tile = u / 3, A = u - 3 * tile;
     const int node = t.tile_node[tile];
     const int p0 = t.tile_p0[tile], p1 = t.tile_p1[tile];
-    const int32_t* __restrict__ src = lst + (size_t)A * m;
-    int q[LC_E];  // the tile's entries (loaded beside the node's)
+    const int32_t* __restrict__ src = lst_in + (size_t)A * m;
+    int sv[LL_E];  // the tile's entries (loaded beside the node's)
 #pragma unroll
-    for (int e = 0; e < LC_E; ++e) {
-      const int p = p0 + (int)threadIdx.x + e * 256;
-      q[e] = p < p1 ? (src[p] & SEG_MASK) : m;
+    for (int e = 0; e < LL_E; ++e) {
+      const int p = p0 + (int)threadIdx.x + e * LL_THREADS;
+      sv[e] = p < p1 ? src[p] : 0;
     }
     const int lo = t.lo[node], hi = t.hi[node];
     const int mid = (int)(((int64_t)lo + hi) / 2);
-    int med[3];
-#pragma unroll
-    for (int B = 0; B < 3; ++B) med[B] = lst[(size_t)B * m + mid] & SEG_MASK;
     int8_t axis, split;
-    list_axis(t, lst, lo, hi, mx, my, mz, &axis, &split);
+    list_axis(t, lst_in, lo, hi, mx, my, mz, &axis, &split);
     if (p0 == lo && A == 0 && threadIdx.x == 0) {
       t.axis[node] = axis;
       t.split[node] = split;
     }
-    if (!split) continue;
+    if (!split) continue;  // uniform over the workgroup
+    const int s_med = lst_in[(size_t)axis * m + mid] & SEG_MASK;
     const double* __restrict__ ca = cs[axis];
-    const int s_med = med[axis];
     const double c_med = ca[s_med < m ? s_med : 0];
     const int32_t* __restrict__ run_pos = t.invb + (size_t)axis * m;
-    uint8_t* __restrict__ side = t.lside + (size_t)A * m;
-    int c = 0;
+    int f[LL_E];
     if (A == axis) {
 #pragma unroll
-      for (int e = 0; e < LC_E; ++e) {
-        const int p = p0 + (int)threadIdx.x + e * 256;
-        if (p < p1) {
-          const int f = p < mid ? 1 : 0;
-          side[p] = (uint8_t)f;
-          c += f;
-        }
+      for (int e = 0; e < LL_E; ++e) {
+        const int p = p0 + (int)threadIdx.x + e * LL_THREADS;
+        f[e] = p < p1 && p < mid ? 1 : 0;
       }
     } else {
-      double x[LC_E];
+      double x[LL_E];
 #pragma unroll
-      for (int e = 0; e < LC_E; ++e) x[e] = q[e] < m ? ca[q[e]] : 0.0;
+      for (int e = 0; e < LL_E; ++e) {
+        const int q = sv[e] & SEG_MASK;
+        const int p = p0 + (int)threadIdx.x + e * LL_THREADS;
+        x[e] = p < p1 && q < m ? ca[q] : 0.0;
+      }
 #pragma unroll
-      for (int e = 0; e < LC_E; ++e) {
-        const int p = p0 + (int)threadIdx.x + e * 256;
-        if (p < p1 && q[e] < m) {
-          const int f = (x[e] < c_med || (x[e] == c_med && q[e] != s_med && run_pos[q[e]] < mid)) ? 1 : 0;
-          side[p] = (uint8_t)f;
-          c += f;
-        }
+      for (int e = 0; e < LL_E; ++e) {
+        const int q = sv[e] & SEG_MASK;
+        const int p = p0 + (int)threadIdx.x + e * LL_THREADS;
+        f[e] = (p < p1 && q < m &&
+                (x[e] < c_med || (x[e] == c_med && q != s_med && run_pos[q] < mid))) ? 1 : 0;
       }
     }
-    int total;
-    block_exclusive_scan(c, &total);
-    if (threadIdx.x == 0) t.tcount[3 * tile + A] = total;
-  }
-}
-
-// Per tile and axis: the stable partition of the node's list into its children's ranges; the units of the split
-// axis write the node's sorted order.
-static constexpr int LS_THREADS = 1024;
-static constexpr int LS_E = TILE / LS_THREADS;
-__global__ __launch_bounds__(LS_THREADS) void k_list_scatter(TreeArrays t, const int32_t* __restrict__ lst_in,
-                                                             int32_t* __restrict__ lst_out) {
-  const int ntiles = live_count(t.counters, C_TILES);
-  const int m = (int)t.m, A = blockIdx.y;
-  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int node = t.tile_node[tile];
-    if (!t.split[node]) continue;
-    const int axis = t.axis[node];
-    const int lo = t.lo[node], hi = t.hi[node];
-    const int p0 = t.tile_p0[tile], p1 = t.tile_p1[tile];
-    const int mid = (int)(((int64_t)lo + hi) / 2);
-    const bool small_l = mid - lo <= S_BOTTOM, small_r = hi - mid <= S_BOTTOM;  // children that leave the lists
-    const int32_t* __restrict__ src = lst_in + (size_t)A * m;
-    const uint8_t* __restrict__ side = t.lside + (size_t)A * m;
-    int s[LS_E], f[LS_E];
-    int mine = 0;
-    const int e0 = p0 + (int)threadIdx.x * LS_E;
+    // ranks among the tile's left entries: row e of wavefront w covers positions p0 + 256 e + 64 w + lane, so
+    // the (row, wavefront) cells in row-major order are in position order
+    int rank[LL_E];
 #pragma unroll
-    for (int e = 0; e < LS_E; ++e) {
-      const int p = e0 + e;
-      s[e] = p < p1 ? src[p] : 0;
-      f[e] = p < p1 ? side[p] : 0;
-      mine += f[e];
+    for (int e = 0; e < LL_E; ++e) {
+      const unsigned long long mask = __builtin_amdgcn_ballot_w64(f[e] != 0);
+      rank[e] = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+      if (lane == 0) s_cell[e * (LL_THREADS / 64) + wv] = __popcll(mask);
     }
-    // left entries of the node's tiles before this one (the node's tiles are consecutive)
-    const int first = tile - (p0 - lo) / TILE;
-    int before = 0;
-    for (int q = first + (int)threadIdx.x; q < tile; q += blockDim.x) before += t.tcount[3 * q + A];
-    int prefix;
-    block_exclusive_scan(before, &prefix);
-    int total;
-    int left = prefix + block_exclusive_scan(mine, &total);
+    __syncthreads();
+    if (wv == 0) {
+      const int v = s_cell[lane];
+      int incl = v;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+      }
+      s_cell[lane] = incl - v;
+      const int total = __builtin_amdgcn_readlane(incl, 63);
+      // the node's tiles are consecutive; its first publishes its inclusive count at once
+      const int first = tile - (p0 - lo) / TILE;
+      unsigned long long* st = t.lstat + A;
+      int prefix = 0;
+      if (tile == first) {
+        if (lane == 0) lb_publish(st + 3 * (size_t)tile, tag, true, total);
+      } else {
+        if (lane == 0) lb_publish(st + 3 * (size_t)tile, tag, false, total);
+        bool ok = true;
+        prefix = lb_lookback(st, 3, first, tile, tag, &ok);
+        if (lane == 0) {
+          if (!ok) t.counters[C_OVERFLOW] = 1;
+          lb_publish(st + 3 * (size_t)tile, tag, true, prefix + total);
+        }
+      }
+      if (lane == 0) s_prefix = prefix;
+    }
+    __syncthreads();
+    const int prefix = s_prefix;
+    const bool small_l = mid - lo <= S_BOTTOM, small_r = hi - mid <= S_BOTTOM;  // children that leave the lists
     int32_t* __restrict__ dst = lst_out + (size_t)A * m;
     int32_t* __restrict__ inv = t.invb + (size_t)A * m;
 #pragma unroll
-    for (int e = 0; e < LS_E; ++e) {
-      const int p = e0 + e;
+    for (int e = 0; e < LL_E; ++e) {
+      const int p = p0 + (int)threadIdx.x + e * LL_THREADS;
       if (p < p1) {
+        const int left = prefix + s_cell[e * (LL_THREADS / 64) + wv] + rank[e];
         const int dest = f[e] ? lo + left : mid + (p - lo - left);
-        const int q = s[e] & SEG_MASK;
+        const int q = sv[e] & SEG_MASK;
         if (dest >= lo && dest < hi && q < m) {
-          dst[dest] = s[e];
-          if (s[e] < 0) inv[q] = dest;
+          dst[dest] = sv[e];
+          // the split axis's list keeps every position (dest == p), so its run positions, which the other
+          // axes' units read for their sides, are not written
+          if (sv[e] < 0 && A != axis) inv[q] = dest;
           if (A == axis) {  // the node's sorted order
             t.order[p] = q;
-            if (s[e] < 0) t.inv_o[q] = p;
+            if (sv[e] < 0) t.inv_o[q] = p;
             if (p < mid ? small_l : small_r) t.ord_inv[q] = p;
           }
         } else {
           t.counters[C_OVERFLOW] = 1;
         }
-        left += f[e];
       }
     }
+    __syncthreads();  // s_cell and s_prefix are read before the next unit writes them
   }
 }
 
@@ -1920,7 +1939,7 @@ int tree_layout(Ctx& c, int buf, int64_t m64, int K, int leaf, TreeArrays& t) {
   t.rs_tiles = (int)((M + RS_TILE - 1) / RS_TILE);
   t.rs_passes = RS_KEY_PASSES + (K <= 1 ? 0 : (K <= 256 ? 1 : (K <= 65536 ? 2 : 3)));
   const size_t RUNS = 3 * M / 2 + 1;
-  need += 2 * al(3 * M * 4) + 2 * al(3 * M * 4) + al(3 * M) + al((size_t)t.tile_cap * 3 * 4);
+  need += 2 * al(3 * M * 4) + 2 * al(3 * M * 4) + al((size_t)t.tile_cap * 3 * 8) + al(LEVEL_MAX * 4);
   need += al(3 * (size_t)t.rs_tiles * RS_BINS * 4) + al(3 * RS_BINS * 4);
   need += al(3 * M * 4) + 2 * al(RUNS * 4) + al(M) + al(M * 4) + al(3 * M * 4);  // runs
   need += al(M * 4);                                                            // bottom phase on the lists
@@ -1975,8 +1994,8 @@ int tree_layout(Ctx& c, int buf, int64_t m64, int K, int leaf, TreeArrays& t) {
   t.tree_first = (int32_t*)carve((K + 1) * 4);
   for (int b = 0; b < 2; ++b) t.rkey[b] = (uint32_t*)carve(3 * M * 4);
   for (int b = 0; b < 2; ++b) t.rval[b] = (int32_t*)carve(3 * M * 4);
-  t.lside = (uint8_t*)carve(3 * M);
-  t.tcount = (int32_t*)carve((size_t)t.tile_cap * 3 * 4);
+  t.lstat = (unsigned long long*)carve((size_t)t.tile_cap * 3 * 8);
+  t.tickets = (int32_t*)carve(LEVEL_MAX * 4);
   t.rcount = (int32_t*)carve(3 * (size_t)t.rs_tiles * RS_BINS * 4);
   t.rtotal = (int32_t*)carve(3 * RS_BINS * 4);
   t.run_seg = (int32_t*)carve(3 * M * 4);
@@ -2029,10 +2048,13 @@ int tree_build(Ctx& c, TreeArrays& t, const double* mx, const double* my, const 
     // no more tiles than m / TILE plus one partial tile per node of the level: grids beyond that only idle
     const int64_t tile_bound = (int64_t)m / TILE + ((int64_t)t.K << std::min(lvl, 30)) + 1;
     if (t.lists) {
-      const unsigned gt = (unsigned)std::min<int64_t>(2 * (int64_t)grid, tile_bound);
-      hipLaunchKernelGGL(k_list_count, dim3(gt, 3), dim3(256), 0, c.stream, t, t.rval[lbuf], mx, my, mz);
-      hipLaunchKernelGGL(k_list_scatter, dim3(gt, 3), dim3(LS_THREADS), 0, c.stream, t, t.rval[lbuf],
-                         t.rval[lbuf ^ 1]);
+      if (lvl >= LEVEL_MAX) {
+        set_error("tree too deep for the list build");
+        return OFW_ERR_ARG;
+      }
+      const unsigned gl = (unsigned)std::min<int64_t>(4 * (int64_t)c.cu_count, 3 * tile_bound);
+      hipLaunchKernelGGL(k_list_level, dim3(gl), dim3(LL_THREADS), 0, c.stream, t, lvl, t.rval[lbuf],
+                         t.rval[lbuf ^ 1], mx, my, mz);
       lbuf ^= 1;
     } else {
       const int32_t* oin = orders[lvl & 1];

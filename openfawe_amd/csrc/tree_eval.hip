@@ -726,7 +726,7 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_tree_near(ForestTables ft, 
   __shared__ double s_cen[NEAR_WAVES][CHUNK][4];
   // below FP64, a window of 64 consecutive segments: FP32 records and FP64 starts
   constexpr int SEGW = PREC == OFW_FP64 ? 1 : 64;
-  __shared__ float s_seg[NEAR_WAVES][SEGW][REC32];
+  __shared__ float s_seg[NEAR_WAVES][SEGW][NEAR32];
   __shared__ double s_start[NEAR_WAVES][SEGW][4];
   __shared__ double s_lenv[NEAR_WAVES][SEGW][4];   // length vectors: the pair's cross product in FP64
   // below FP64, the FP64 record of the segment whose pairs take the FP64 path (the same for every lane), and the
@@ -860,7 +860,7 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_tree_near(ForestTables ft, 
           const int s = base + lane;
           wave_lds_sync();  // the previous window's reads come first
           if (s < ft.seg_count) {
-            const float4* f4 = reinterpret_cast<const float4*>(rec32 + (size_t)s * REC32);
+            const float4* f4 = reinterpret_cast<const float4*>(rec32 + (size_t)s * NEAR32);
             const float4 fa = f4[0], fb = f4[1];
             const double qx = segs.start[0][s], qy = segs.start[1][s], qz = segs.start[2][s];
             const double lx = segs.lenv[0][s], ly = segs.lenv[1][s], lz = segs.lenv[2][s];
@@ -886,7 +886,7 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_tree_near(ForestTables ft, 
         for (int s = base; s < stop; ++s) {
           // the pair's record and start: the same LDS address on every lane (a broadcast read)
           const int tl = s - sbase;
-          const Rec32 fc = load_rec32(&s_seg[wv][tl][0]);
+          const Near32 fc = load_near32(&s_seg[wv][tl][0]);
           const double cx0 = s_start[wv][tl][0], cy0 = s_start[wv][tl][1], cz0 = s_start[wv][tl][2];
           // every lane evaluates the FP32 pair; the lanes that use it select the result, and only the rare
           // excluded segment and the FP64 pairs branch (on the wavefront's ballot)
@@ -900,17 +900,17 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_tree_near(ForestTables ft, 
             hits += hit ? 1 : 0;
           }
           const double sx = x0 - cx0, sy = x1 - cy0, sz = x2 - cz0;
-          // c = d x (r / L) = (l x r) / L^2 formed in FP64: h keeps its FP32 accuracy next to the line
+          // C = l x r formed in FP64: h keeps its FP32 accuracy next to the line
           const double gx = s_lenv[wv][tl][0], gy = s_lenv[wv][tl][1], gz = s_lenv[wv][tl][2];
-          const float il2 = fc.invL * fc.invL;
-          const float ccx = (float)(gy * sz - gz * sy) * il2, ccy = (float)(gz * sx - gx * sz) * il2,
-                      ccz = (float)(gx * sy - gy * sx) * il2;
-          float vx, vy, vz;
-          const bool need64 = pair_f32<MODEL, true>((float)sx, (float)sy, (float)sz, fc, vx, vy, vz, ccx, ccy, ccz);
+          const float ccx = (float)(gy * sz - gz * sy), ccy = (float)(gz * sx - gx * sz),
+                      ccz = (float)(gx * sy - gy * sx);
+          float coef;
+          const bool need64 = pair_near32<MODEL>((float)sx, (float)sy, (float)sz, ccx, ccy, ccz, fc, coef);
           const bool add = use && !need64;
-          lx += add ? vx : 0.0f;
-          ly += add ? vy : 0.0f;
-          lz += add ? vz : 0.0f;
+          const float cw = add ? coef : 0.0f;
+          lx = __builtin_fmaf(cw, ccx, lx);
+          ly = __builtin_fmaf(cw, ccy, ly);
+          lz = __builtin_fmaf(cw, ccz, lz);
           const bool slow = use && need64;
           const unsigned long long slow_lanes = __builtin_amdgcn_ballot_w64(slow);
           if (slow_lanes) {

@@ -47,8 +47,8 @@ __global__ __launch_bounds__(256) void k_tree_prep(int m, const double* __restri
 }
 
 // Tree-order copies: the moment inputs (start, length vector, radius, gamma) and the pair records of the
-// traversal's near field (the FP64 records only when rec64 is given: the FP32 near field rebuilds the few it
-// needs from the engine inputs).
+// traversal's near field: the FP32 near-field records, and the FP64 records only when rec64 is given (the FP32
+// near field rebuilds the few it needs from the engine inputs).
 __global__ __launch_bounds__(256) void k_tree_permute(int m, const int32_t* __restrict__ order,
                                                       const double* __restrict__ starts,
                                                       const double* __restrict__ directions,
@@ -71,8 +71,8 @@ __global__ __launch_bounds__(256) void k_tree_permute(int m, const int32_t* __re
   }
   mi.rad[k] = radius[s];
   mi.gam[k] = g;
-  pack_record(s, starts, directions, lengths, gamma, radius, model, rec64 ? rec64 + (size_t)k * REC64 : nullptr,
-              rec32 + (size_t)k * REC32);
+  if (rec64) pack_record(s, starts, directions, lengths, gamma, radius, model, rec64 + (size_t)k * REC64, nullptr);
+  pack_near32(s, directions, lengths, gamma, radius, model, rec32 + (size_t)k * NEAR32);
 }
 
 __device__ __forceinline__ double wave_max(double v) {
@@ -781,7 +781,7 @@ int tree_moments(Ctx& c, int buf, const TreeArrays& t, const std::vector<int64_t
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const size_t M = (size_t)std::max(m, 1);
   const size_t C = (size_t)cap;
-  size_t need = 8 * al(M * 8) + (want_rec64 ? al(M * REC64 * 8) : 0) + al(M * REC32 * 4) + al(table.size() * 4);
+  size_t need = 8 * al(M * 8) + (want_rec64 ? al(M * REC64 * 8) : 0) + al(M * NEAR32 * 4) + al(table.size() * 4);
   need += al(((size_t)K << MB_MAX_CUT) * 4) + al((size_t)std::max<int64_t>(wgs, 1) * 4);
   need += al(C * 3 * 8) + al(C * 8);
   need += al(C * ACC_REC * 8) + al(C * FAR_REC * 8) + al(C * FAR_REC * 4);
@@ -803,7 +803,7 @@ int tree_moments(Ctx& c, int buf, const TreeArrays& t, const std::vector<int64_t
     td.start[q] = mi.s[q];
     td.lenv[q] = mi.l[q];
   }
-  td.rec32 = (float*)carve(M * REC32 * 4);
+  td.rec32 = (float*)carve(M * NEAR32 * 4);
   int32_t* d_table = (int32_t*)carve(table.size() * 4);
   int32_t* d_top = (int32_t*)carve(((size_t)K << MB_MAX_CUT) * 4);
   int32_t* d_roots = (int32_t*)carve((size_t)std::max<int64_t>(wgs, 1) * 4);

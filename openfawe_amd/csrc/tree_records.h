@@ -56,6 +56,33 @@ __device__ __forceinline__ void pack_record(int s, const double* __restrict__ st
   f[F_PAD0] = 0.0f;
 }
 
+// The tree near field's FP32 record (Near32, common.h) of segment s from the engine inputs; the length vector is
+// the FP64 product direction * length, rounded.
+__device__ __forceinline__ void pack_near32(int s, const double* __restrict__ directions,
+                                            const double* __restrict__ lengths, const double* __restrict__ gamma,
+                                            const double* __restrict__ radius, int model, float* __restrict__ f) {
+#pragma clang fp contract(off)
+  const double L = lengths[s], g = gamma[s], rad = radius[s];
+  const double gp = g / OFW_FOUR_PI;
+  const bool cored = rad > 0.0 && model != OFW_MODEL_NONE;
+  const double lr = L * rad;
+  const double lr2 = cored ? lr * lr : 0.0;
+  const double L2 = L * L;
+  const bool ratio_model = model == OFW_MODEL_RANKINE || model == OFW_MODEL_LAMB || model == OFW_MODEL_VATISTAS;
+  const bool tiny_core = cored && ratio_model && !(rad / L >= OFW_FAST_RC_MIN);  // the FP64 pair's exact path
+  const double ag = fabs(gp);
+  const bool f32_ok = !tiny_core && L2 >= OFW_NEAR_L2_MIN && L2 <= OFW_NEAR_L2_MAX &&
+                      (ag == 0.0 || (ag > 1e-30 && ag < 1e30)) && (!cored || (lr2 > 1e-30 && lr2 < 1e30));
+  f[N_LX] = (float)(directions[3 * (size_t)s] * L);
+  f[N_LY] = (float)(directions[3 * (size_t)s + 1] * L);
+  f[N_LZ] = (float)(directions[3 * (size_t)s + 2] * L);
+  f[N_L2] = (float)L2;
+  f[N_GP] = (float)gp;
+  f[N_QLR2] = (float)(OFW_LAMB_Q * lr2);
+  f[N_EXACT] = f32_ok ? 0.0f : 1.0f;
+  f[N_BAND] = cored ? (float)((double)OFW_LAMB_BAND32 * OFW_LAMB_Q * lr2) : -1.0f;
+}
+
 // Moment inputs in tree order (SoA): start, length vector, radius, gamma.  The end point and alpha = gamma x
 // length vector are formed where they are read, by the same FP64 operations as vortex_tree.py:608-611.
 struct MomentInputs {
@@ -103,7 +130,7 @@ enum { FR_M0 = 0, FR_M1 = 3, FR_W1 = 12, FR_M2 = 15, FR_T2 = 33, FR_W2 = 36, FR_
 struct TreeDevice {
   int count = 0;
   double* rec64 = nullptr;  // FP64 pair records in tree order (FP64 evaluations only)
-  float* rec32 = nullptr;   // FP32 pair records in tree order
+  float* rec32 = nullptr;   // FP32 near-field records (Near32) in tree order
   const double* start[3] = {};  // segment starts in tree order
   const double* lenv[3] = {};   // segment length vectors in tree order
   SegSource src;            // the engine inputs, for the FP64 record of one pair
