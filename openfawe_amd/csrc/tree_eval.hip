@@ -222,6 +222,40 @@ struct NearEntry {
   uint32_t mask_lo, mask_hi;
 };
 static constexpr int CHUNK = 8;  // list entries per work item of the list kernel
+static_assert(CHUNK <= 64, "a work item's entries are loaded one per lane");
+// cost classes of the work items (dispatched heaviest first; see k_item_waves)
+static constexpr int ITEM_CLASSES = 8;
+static constexpr int ITEM_CLASS_W = 32;  // segments per class step
+static constexpr int ITEM_FAR = 2;       // cost of an accepted node's far field, in segments
+static_assert(CHUNK == 8, "a work item's cost is summed over eight consecutive lanes");
+
+// The cost class of the work item of entry b0 + lane of a list of `count` entries (b0 a multiple of 64): every
+// lane of the item's eight holds it, and *leader is set on the first lane of each item that has entries.  With
+// ATOMIC the entries are read past the vector cache (the calling wavefront has just written them).
+template <bool ATOMIC>
+__device__ __forceinline__ int item_class(const NearEntry* __restrict__ list, size_t lbase, int count, int b0,
+                                          int lane, bool* leader) {
+  const int e = b0 + lane;
+  int cost = 0;
+  if (e < count) {
+    int lo, hi;
+    if (ATOMIC) {
+      const unsigned long long w0 = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(list + lbase + e),
+                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      lo = (int32_t)(uint32_t)w0;
+      hi = (int32_t)(w0 >> 32);
+    } else {
+      lo = list[lbase + e].lo;
+      hi = list[lbase + e].hi;
+    }
+    cost = lo < 0 ? ITEM_FAR : hi - lo;
+  }
+  cost += __shfl_xor(cost, 1, 64);
+  cost += __shfl_xor(cost, 2, 64);
+  cost += __shfl_xor(cost, 4, 64);
+  *leader = (lane & 7) == 0 && e < count;
+  return min(cost / ITEM_CLASS_W, ITEM_CLASSES - 1);
+}
 
 // the tree k of a wavefront / target: the last k with table[k] <= v (table[0] = 0 <= v < table[K])
 __device__ __forceinline__ int find_tree(const int32_t* __restrict__ table, int K, int v) {
@@ -265,7 +299,8 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_tree_eval(ForestTables ft, 
                                                    int raise_singular, double* __restrict__ part,
                                                    int8_t* __restrict__ pflag, int32_t* __restrict__ pfar,
                                                    int32_t* __restrict__ pnear, int8_t* __restrict__ pnz,
-                                                   NearEntry* __restrict__ list, int32_t* __restrict__ list_count) {
+                                                   NearEntry* __restrict__ list, int32_t* __restrict__ list_count,
+                                                   int32_t* __restrict__ cls_wcnt) {
   typedef typename std::conditional<PREC == OFW_FP32, float, double>::type acc_t;
   // the stackless walk's node window, or the breadth-first walk's test records of one batch (8 doubles per node)
   __shared__ double s_acc[NEAR_WAVES][64 * 8];
@@ -678,6 +713,23 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_tree_eval(ForestTables ft, 
     pnz[wave] = any ? 1 : 0;
     list_count[wave] = nlist;
   }
+  // the list's work items per cost class (lane c counts class c), for the heaviest-first dispatch of the list
+  // kernel; the entries were written by other lanes, whose stores complete first
+  {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0);
+    int cnt = 0;
+    for (int b0 = 0; b0 < nlist; b0 += 64) {
+      bool lead;
+      const int cls = item_class<true>(list, lbase, nlist, b0, lane, &lead);
+#pragma unroll
+      for (int cc = 0; cc < ITEM_CLASSES; ++cc) {
+        const int nc = __popcll(__builtin_amdgcn_ballot_w64(lead && cls == cc));
+        if (lane == cc) cnt += nc;
+      }
+    }
+    if (lane < ITEM_CLASSES) cls_wcnt[(size_t)wave * ITEM_CLASSES + lane] = cnt;
+  }
   if (any && valid) {
     const size_t o = (size_t)uniform(ft.part_off[k]) + (size_t)q * nt + jl;
     part[3 * o] = (double)ux;
@@ -717,6 +769,8 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_tree_near(ForestTables ft, 
                                                    const int32_t* __restrict__ list_count,
                                                    const int32_t* __restrict__ chunk_off,
                                                    const int32_t* __restrict__ item_wave, int chunk_slots,
+                                                   const int32_t* __restrict__ cls_count,
+                                                   const int32_t* __restrict__ cls_list,
                                                    double* __restrict__ cpart, int8_t* __restrict__ cflag,
                                                    int32_t* __restrict__ cnear) {
   typedef typename std::conditional<PREC == OFW_FP32, float, double>::type acc_t;
@@ -739,7 +793,21 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_tree_near(ForestTables ft, 
   const int works = min(live_count(ft.counters, C_CHUNKS), chunk_slots);
   const int nwaves = live_count(ft.counters, C_WORK);
   const int stride = gridDim.x * NEAR_WAVES;
-  for (int item = uniform(blockIdx.x * NEAR_WAVES + (int)(threadIdx.x >> 6)); item < works; item += stride) {
+  for (int slot = uniform(blockIdx.x * NEAR_WAVES + (int)(threadIdx.x >> 6)); slot < works; slot += stride) {
+    // the item of this dispatch slot: the classes heaviest first
+    int item = -1;
+    {
+      int before = 0;
+      for (int cc = ITEM_CLASSES - 1; cc >= 0; --cc) {
+        const int cnt = uniform(cls_count[cc]);
+        if (slot < before + cnt) {
+          item = uniform(cls_list[(size_t)cc * chunk_slots + (slot - before)]);
+          break;
+        }
+        before += cnt;
+      }
+    }
+    if (item < 0) continue;  // cannot happen: the classes list every item below the slots
     // the walk wavefront of this item
     const int w = uniform(item_wave[item]);
     const int c = item - uniform(chunk_off[w]);
@@ -789,12 +857,13 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_tree_near(ForestTables ft, 
     const int e0 = c * CHUNK;
     const int ne = min(uniform(list_count[w]), e0 + CHUNK) - e0;
     // the item's entries, one per lane, and the records of its accepted nodes staged in LDS together (lane
-    // 8 q + u copies part u of entry q's record), so that no entry waits for its own loads
+    // 8 q' + u copies part u of the record of entry q' + 8 r in round r), so that no entry waits for its own loads
     NearEntry mine{0, 0, 0u, 0u};
     if (lane < ne) mine = list[lbase + e0 + lane];
     wave_lds_sync();  // the previous item's reads of the stage come first
-    {
-      const int q = lane >> 3, u = lane & 7;
+#pragma unroll
+    for (int r = 0; r < (CHUNK + 7) / 8; ++r) {
+      const int q = (lane >> 3) + 8 * r, u = lane & 7;
       const int lo_q = __shfl(mine.lo, q, 64);
       if (q < ne && lo_q < 0) {
         const int node = -1 - lo_q;
@@ -953,34 +1022,113 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_tree_near(ForestTables ft, 
   }
 }
 
-// the walk wavefront of every work item of the list kernel (within the partial slots)
-// The walk wavefront of every work item (within the `slots` partial slots); more work items than the fixed
-// number `limit` of slots is reported as an overflow.
-__global__ void k_item_waves(int32_t* __restrict__ counters, const int32_t* __restrict__ chunk_off, int slots,
-                             int limit, int32_t* __restrict__ item_wave) {
+// The work items of the list kernel are dispatched heaviest first: the kernel's end is then set by the light
+// items, not by a heavy one started late.  An item's cost is its leaf entries' segments plus ITEM_FAR per accepted
+// node; class c = min(cost / ITEM_CLASS_W, ITEM_CLASSES - 1) lists the item in cls_list[c] (order within a class
+// from atomics: the dispatch order does not change any result, every item writes its own partial slot).
+
+// One wavefront per walk wavefront: the walk wavefront of each of its work items (within the `slots` partial
+// slots), and the items listed by cost class (cls_list[c], `slots` entries per class) at the wavefront's offsets
+// within each class (cls_woff), in item order; more work items than the fixed number `limit` of slots is reported
+// as an overflow.
+__global__ __launch_bounds__(256) void k_item_waves(ForestTables ft, const int32_t* __restrict__ chunk_off, int slots,
+                                                    int limit, const NearEntry* __restrict__ list,
+                                                    const int32_t* __restrict__ list_count,
+                                                    const int32_t* __restrict__ cls_woff,
+                                                    int32_t* __restrict__ item_wave, int32_t* __restrict__ cls_list) {
+  int32_t* counters = const_cast<int32_t*>(ft.counters);
   if (blockIdx.x == 0 && threadIdx.x == 0 && counters[C_CHUNKS] > limit) counters[C_OVERFLOW] = 1;
-  const int nwaves = live_count(counters, C_WORK);
-  for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < nwaves; w += gridDim.x * blockDim.x)
-    for (int it = chunk_off[w]; it < min(chunk_off[w + 1], slots); ++it) item_wave[it] = w;
+  const int nwaves = live_count(ft.counters, C_WORK);
+  const int w = uniform((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+  if (w >= nwaves) return;
+  const int lane = threadIdx.x & 63;
+  const int k = uniform(find_tree(ft.wave_off, ft.K, w));
+  const int local = w - uniform(ft.wave_off[k]);
+  const size_t lbase = (size_t)uniform(ft.list_base[k]) + (size_t)local * uniform(ft.list_cap[k]);
+  const int count = uniform(list_count[w]);
+  const int it0 = uniform(chunk_off[w]), it1 = min(uniform(chunk_off[w + 1]), slots);
+  const int woff = lane < ITEM_CLASSES ? cls_woff[(size_t)w * ITEM_CLASSES + lane] : 0;
+  int run = 0;  // lane c: items of class c placed so far
+  for (int b0 = 0; b0 < count; b0 += 64) {
+    bool lead;
+    const int cls = item_class<false>(list, lbase, count, b0, lane, &lead);
+    const int it = it0 + ((b0 + lane) >> 3);
+    if (lead && it < it1) item_wave[it] = w;
+#pragma unroll
+    for (int cc = 0; cc < ITEM_CLASSES; ++cc) {
+      const bool mine = lead && cls == cc;
+      const unsigned long long mask = __builtin_amdgcn_ballot_w64(mine);
+      if (mask == 0ull) continue;
+      const int base = __builtin_amdgcn_readlane(woff + run, cc);
+      const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+      const int pos = base + rank;
+      if (mine && it < it1 && pos < slots) cls_list[(size_t)cc * slots + pos] = it;
+      if (lane == cc) run += __popcll(mask);
+    }
+  }
 }
 
-// One workgroup: the near-field work items of each walk wavefront, ceil(list entries / CHUNK) (zero past the
-// planned waves), scanned into chunk_off (W1 entries, each thread a run of consecutive ones); the total goes
-// to counters[C_CHUNKS].
-__global__ __launch_bounds__(1024) void k_chunk_plan(ForestTables ft, int W1, const int32_t* __restrict__ list_count,
-                                                     int32_t* __restrict__ chunk_off, int32_t* __restrict__ counters) {
+// The near-field work items of each walk wavefront, ceil(list entries / CHUNK) (zero past the planned waves),
+// scanned into chunk_off (W1 entries), and the wavefronts' items per cost class (cls_wcnt, counted by the walk)
+// scanned into each wavefront's offset within the class's list (cls_woff): 256 wavefronts per workgroup, a
+// single-pass look-back across the workgroups (tiles taken by ticket; the status words and the ticket were
+// cleared by k_work_plan).  The last tile writes the totals: counters[C_CHUNKS] and cls_count.
+static constexpr int PLAN_VALUES = 1 + ITEM_CLASSES;
+__global__ __launch_bounds__(256) void k_chunk_plan(ForestTables ft, int W1, const int32_t* __restrict__ list_count,
+                                                    const int32_t* __restrict__ cls_wcnt,
+                                                    int32_t* __restrict__ chunk_off, int32_t* __restrict__ cls_woff,
+                                                    int32_t* __restrict__ counters, int32_t* __restrict__ cls_count,
+                                                    int32_t* __restrict__ ticket, unsigned long long* __restrict__ status) {
+  __shared__ int s_tile;
+  __shared__ int s_prefix[PLAN_VALUES];
   const int nwaves = live_count(ft.counters, C_WORK);
-  const int per = (W1 + (int)blockDim.x - 1) / (int)blockDim.x;
-  const int a = min(W1, (int)threadIdx.x * per), b = min(W1, a + per);
-  int sum = 0;
-  for (int w = a; w < b; ++w) sum += w < nwaves ? (list_count[w] + CHUNK - 1) / CHUNK : 0;
-  int total;
-  int run = block_exclusive_scan(sum, &total);
-  for (int w = a; w < b; ++w) {
-    chunk_off[w] = run;
-    run += w < nwaves ? (list_count[w] + CHUNK - 1) / CHUNK : 0;
+  const int ntiles = (W1 + 255) / 256;
+  if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1);
+  __syncthreads();
+  const int tile = s_tile;
+  if (tile >= ntiles) return;
+  const int w = tile * 256 + (int)threadIdx.x;
+  const bool live = w < nwaves;
+  int v[PLAN_VALUES];
+  v[0] = live ? (list_count[w] + CHUNK - 1) / CHUNK : 0;
+#pragma unroll
+  for (int cc = 0; cc < ITEM_CLASSES; ++cc) v[1 + cc] = live ? cls_wcnt[(size_t)w * ITEM_CLASSES + cc] : 0;
+  int r[PLAN_VALUES], tot[PLAN_VALUES];
+  block_exclusive_scan_n<PLAN_VALUES>(v, r, tot);
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    if (lane < PLAN_VALUES) {
+      int mine = 0;
+#pragma unroll
+      for (int q = 0; q < PLAN_VALUES; ++q) mine = lane == q ? tot[q] : mine;
+      lb_publish(status + (size_t)lane * ntiles + tile, 1u, tile == 0, mine);
+    }
+    int pre[PLAN_VALUES];
+    bool ok = true;
+#pragma unroll
+    for (int q = 0; q < PLAN_VALUES; ++q) pre[q] = tile == 0 ? 0 : lb_lookback(status + (size_t)q * ntiles, 1, 0, tile, 1u, &ok);
+    if (lane < PLAN_VALUES) {
+      int p = 0, t = 0;
+#pragma unroll
+      for (int q = 0; q < PLAN_VALUES; ++q) {
+        p = lane == q ? pre[q] : p;
+        t = lane == q ? tot[q] : t;
+      }
+      if (tile > 0) lb_publish(status + (size_t)lane * ntiles + tile, 1u, true, p + t);
+      s_prefix[lane] = p;
+      if (tile == ntiles - 1) {
+        if (lane == 0) counters[C_CHUNKS] = p + t;
+        else cls_count[lane - 1] = p + t;
+      }
+    }
+    if (!ok && lane == 0) counters[C_OVERFLOW] = 1;
   }
-  if (threadIdx.x == 0) counters[C_CHUNKS] = total;
+  __syncthreads();
+  if (w < W1) {
+    chunk_off[w] = s_prefix[0] + r[0];
+#pragma unroll
+    for (int cc = 0; cc < ITEM_CLASSES; ++cc) cls_woff[(size_t)w * ITEM_CLASSES + cc] = s_prefix[1 + cc] + r[1 + cc];
+  }
 }
 
 // Per target: its tree's subtree partials and their near-field partials, in pre-order of the subtrees (those
@@ -1153,7 +1301,13 @@ __global__ void k_task_fill(TreeArrays t, const int32_t* __restrict__ flag, cons
 __global__ __launch_bounds__(1024) void k_work_plan(TreeArrays t, const int32_t* __restrict__ index,
                                                     const int32_t* __restrict__ toff, int32_t* __restrict__ task_first,
                                                     int32_t* __restrict__ wave_off, int32_t* __restrict__ part_off,
-                                                    int wave_cap, int part_cap) {
+                                                    int wave_cap, int part_cap, int32_t* __restrict__ plan_ticket,
+                                                    unsigned long long* __restrict__ plan_status, int plan_words) {
+  // the list plan's tile ticket and look-back status words start empty in every call
+  if (plan_ticket) {
+    if (threadIdx.x == 0) *plan_ticket = 0;
+    for (int i = threadIdx.x; i < plan_words; i += blockDim.x) plan_status[i] = 0ull;
+  }
   int waves = 0, parts = 0;
   for (int base = 0; base <= t.K; base += blockDim.x) {
     const int k = base + threadIdx.x;
@@ -1194,39 +1348,40 @@ template <int MODEL>
 static void launch_eval(int precision, dim3 grid, hipStream_t st, const ForestTables& ft, const double* pts,
                         const int32_t* ex_ptr, const int32_t* ex_pos, const EvalTask* tasks, const TreeDevice& td,
                         double theta, int order, int raise, double* part, int8_t* pflag, int32_t* pfar,
-                        int32_t* pnear, int8_t* pnz, NearEntry* list, int32_t* list_count) {
+                        int32_t* pnear, int8_t* pnz, NearEntry* list, int32_t* list_count, int32_t* cls_wcnt) {
   if (precision == OFW_FP64)
     hipLaunchKernelGGL((k_tree_eval<MODEL, OFW_FP64>), grid, dim3(64 * NEAR_WAVES), 0, st, ft, pts, ex_ptr, ex_pos, tasks, td.acc,
                        td.far64, td.far32, td.rec64, td.rec32, theta, order, raise, part, pflag, pfar, pnear, pnz,
-                       list, list_count);
+                       list, list_count, cls_wcnt);
   else if (precision == OFW_MIXED)
     hipLaunchKernelGGL((k_tree_eval<MODEL, OFW_MIXED>), grid, dim3(64 * NEAR_WAVES), 0, st, ft, pts, ex_ptr, ex_pos, tasks,
                        td.acc, td.far64, td.far32, td.rec64, td.rec32, theta, order, raise, part, pflag, pfar, pnear,
-                       pnz, list, list_count);
+                       pnz, list, list_count, cls_wcnt);
   else
     hipLaunchKernelGGL((k_tree_eval<MODEL, OFW_FP32>), grid, dim3(64 * NEAR_WAVES), 0, st, ft, pts, ex_ptr, ex_pos, tasks, td.acc,
                        td.far64, td.far32, td.rec64, td.rec32, theta, order, raise, part, pflag, pfar, pnear, pnz,
-                       list, list_count);
+                       list, list_count, cls_wcnt);
 }
 
 template <int MODEL>
 static void launch_near(int precision, dim3 grid, hipStream_t st, const ForestTables& ft, const double* pts,
                         const int32_t* ex_ptr, const int32_t* ex_pos, const TreeDevice& td, int order, int raise,
                         const NearEntry* list, const int32_t* list_count, const int32_t* chunk_off,
-                        const int32_t* item_wave, int slots, double* cpart, int8_t* cflag, int32_t* cnear) {
+                        const int32_t* item_wave, int slots, const int32_t* cls_count, const int32_t* cls_list,
+                        double* cpart, int8_t* cflag, int32_t* cnear) {
   const NearSegs segs{td.rec64, {td.start[0], td.start[1], td.start[2]}, {td.lenv[0], td.lenv[1], td.lenv[2]}, td.src};
   if (precision == OFW_FP64)
     hipLaunchKernelGGL((k_tree_near<MODEL, OFW_FP64>), grid, dim3(64 * NEAR_WAVES), 0, st, ft, pts, ex_ptr, ex_pos, segs,
-                       td.rec32, td.acc, td.far64, td.far32, order, raise, list, list_count, chunk_off, item_wave, slots, cpart,
-                       cflag, cnear);
+                       td.rec32, td.acc, td.far64, td.far32, order, raise, list, list_count, chunk_off, item_wave, slots, cls_count,
+                       cls_list, cpart, cflag, cnear);
   else if (precision == OFW_MIXED)
     hipLaunchKernelGGL((k_tree_near<MODEL, OFW_MIXED>), grid, dim3(64 * NEAR_WAVES), 0, st, ft, pts, ex_ptr, ex_pos, segs,
-                       td.rec32, td.acc, td.far64, td.far32, order, raise, list, list_count, chunk_off, item_wave, slots, cpart,
-                       cflag, cnear);
+                       td.rec32, td.acc, td.far64, td.far32, order, raise, list, list_count, chunk_off, item_wave, slots, cls_count,
+                       cls_list, cpart, cflag, cnear);
   else
     hipLaunchKernelGGL((k_tree_near<MODEL, OFW_FP32>), grid, dim3(64 * NEAR_WAVES), 0, st, ft, pts, ex_ptr, ex_pos, segs,
-                       td.rec32, td.acc, td.far64, td.far32, order, raise, list, list_count, chunk_off, item_wave, slots, cpart,
-                       cflag, cnear);
+                       td.rec32, td.acc, td.far64, td.far32, order, raise, list, list_count, chunk_off, item_wave, slots, cls_count,
+                       cls_list, cpart, cflag, cnear);
 }
 
 int choose_split_depth(const Ctx& c, int64_t groups, int K) {
@@ -1285,7 +1440,9 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
   need += 2 * al(((size_t)t.cap + 1) * 4) + al((size_t)std::max<int64_t>(task_cap, 1) * sizeof(EvalTask));
   need += 6 * al((size_t)(K + 1) * 4) + al(P * 24) + al(P) + (counts ? 2 * al(P * 4) : 0);
   need += al(W1);
-  if (lists) need += al(E * sizeof(NearEntry)) + 2 * al(W1 * 4);
+  const int plan_words = PLAN_VALUES * (int)((W1 + 255) / 256);
+  if (lists) need += al(E * sizeof(NearEntry)) + 2 * al(W1 * 4) + al(ITEM_CLASSES * 4) + 2 * al(W1 * ITEM_CLASSES * 4) +
+                     al(4) + al((size_t)plan_words * 8);
   OFW_HIP(c.tree[buf].reserve(need));
   char* base = (char*)c.tree[buf].ptr;
   auto carve = [&](size_t bytes) {
@@ -1311,7 +1468,9 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
   int32_t* pnear = counts ? (int32_t*)carve(P * 4) : nullptr;
   int8_t* pnz = (int8_t*)carve(W1);
   NearEntry* list = nullptr;
-  int32_t *list_count = nullptr, *chunk_off = nullptr;
+  int32_t *list_count = nullptr, *chunk_off = nullptr, *cls_count = nullptr, *cls_wcnt = nullptr, *cls_woff = nullptr;
+  int32_t* plan_ticket = nullptr;
+  unsigned long long* plan_status = nullptr;
   double* cpart = nullptr;
   int8_t* cflag = nullptr;
   int32_t* cnear = nullptr;
@@ -1319,6 +1478,11 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
     list = (NearEntry*)carve(E * sizeof(NearEntry));
     list_count = (int32_t*)carve(W1 * 4);
     chunk_off = (int32_t*)carve(W1 * 4);
+    cls_count = (int32_t*)carve(ITEM_CLASSES * 4);
+    cls_wcnt = (int32_t*)carve(W1 * ITEM_CLASSES * 4);
+    cls_woff = (int32_t*)carve(W1 * ITEM_CLASSES * 4);
+    plan_ticket = (int32_t*)carve(4);
+    plan_status = (unsigned long long*)carve((size_t)plan_words * 8);
   }
 
   std::vector<int64_t> tables(shape.target_off);
@@ -1340,23 +1504,24 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
   scan_exclusive(c, t.cap + 1, flag, index, t.scan_part, nullptr);
   hipLaunchKernelGGL(k_task_fill, dim3(grid), dim3(256), 0, c.stream, t, flag, index, (int)task_cap, tasks);
   hipLaunchKernelGGL(k_work_plan, dim3(1), dim3(1024), 0, c.stream, t, index, toff, task_first, wave_off, part_off,
-                     (int)wave_cap, (int)part_cap);
+                     (int)wave_cap, (int)part_cap, plan_ticket, plan_status, plan_words);
   OFW_HIP(hipGetLastError());
   OFW_HIP(c.mark());
   ForestTables ft{K, td.count, (int)m, toff, task_first, wave_off, part_off, d_list_base, d_list_cap, t.counters};
   if (wave_cap > 0) {
     dim3 egrid((unsigned)((wave_cap + NEAR_WAVES - 1) / NEAR_WAVES));
     switch (model) {
-      case 0: launch_eval<0>(precision, egrid, c.stream, ft, points, ex_ptr_used, ex_pos, tasks, td, theta, expansion_order, raise_singular, part, pflag, pfar, pnear, pnz, list, list_count); break;
-      case 1: launch_eval<1>(precision, egrid, c.stream, ft, points, ex_ptr_used, ex_pos, tasks, td, theta, expansion_order, raise_singular, part, pflag, pfar, pnear, pnz, list, list_count); break;
-      default: launch_eval<2>(precision, egrid, c.stream, ft, points, ex_ptr_used, ex_pos, tasks, td, theta, expansion_order, raise_singular, part, pflag, pfar, pnear, pnz, list, list_count); break;
+      case 0: launch_eval<0>(precision, egrid, c.stream, ft, points, ex_ptr_used, ex_pos, tasks, td, theta, expansion_order, raise_singular, part, pflag, pfar, pnear, pnz, list, list_count, cls_wcnt); break;
+      case 1: launch_eval<1>(precision, egrid, c.stream, ft, points, ex_ptr_used, ex_pos, tasks, td, theta, expansion_order, raise_singular, part, pflag, pfar, pnear, pnz, list, list_count, cls_wcnt); break;
+      default: launch_eval<2>(precision, egrid, c.stream, ft, points, ex_ptr_used, ex_pos, tasks, td, theta, expansion_order, raise_singular, part, pflag, pfar, pnear, pnz, list, list_count, cls_wcnt); break;
     }
   }
   OFW_HIP(hipGetLastError());
   OFW_HIP(c.mark());
   if (lists && wave_cap > 0) {
     // near-field work items: CHUNK list entries each, numbered in walk-wavefront order
-    hipLaunchKernelGGL(k_chunk_plan, dim3(1), dim3(1024), 0, c.stream, ft, (int)W1, list_count, chunk_off, t.counters);
+    hipLaunchKernelGGL(k_chunk_plan, dim3((unsigned)((W1 + 255) / 256)), dim3(256), 0, c.stream, ft, (int)W1, list_count,
+                       cls_wcnt, chunk_off, cls_woff, t.counters, cls_count, plan_ticket, plan_status);
     // partial sums of the near-field work items (scratch buffer 5 of the context)
     int64_t slots = shape.chunk_slots;
     if (slots < 0) {
@@ -1367,8 +1532,9 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
       c.stats_i[6] = count;
     }
     const size_t CH = (size_t)std::max<int64_t>(slots, 1);
-    OFW_HIP(c.tree[5].reserve(al(CH * 64 * 24) + al(CH * 64) + al(CH * 4) + (counts ? al(CH * 64 * 4) : 0)));
-    int32_t* item_wave;
+    OFW_HIP(c.tree[5].reserve(al(CH * 64 * 24) + al(CH * 64) + al(CH * 4) + al(CH * ITEM_CLASSES * 4) +
+                              (counts ? al(CH * 64 * 4) : 0)));
+    int32_t *item_wave, *cls_list;
     {
       char* cb = (char*)c.tree[5].ptr;
       cpart = (double*)cb;
@@ -1377,17 +1543,19 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
       cb += al(CH * 64);
       item_wave = (int32_t*)cb;
       cb += al(CH * 4);
+      cls_list = (int32_t*)cb;
+      cb += al(CH * ITEM_CLASSES * 4);
       cnear = counts ? (int32_t*)cb : nullptr;
     }
-    hipLaunchKernelGGL(k_item_waves, dim3((unsigned)((W1 + 255) / 256)), dim3(256), 0, c.stream, t.counters, chunk_off,
-                       (int)CH, (int)std::min<int64_t>(slots, INT_MAX), item_wave);
+    hipLaunchKernelGGL(k_item_waves, dim3((unsigned)((W1 + 3) / 4)), dim3(256), 0, c.stream, ft, chunk_off, (int)CH,
+                       (int)std::min<int64_t>(slots, INT_MAX), list, list_count, cls_woff, item_wave, cls_list);
     OFW_HIP(c.mark());
     // one wavefront per work item: the dispatcher starts the next item wherever a wavefront has finished
     const dim3 ngrid((unsigned)std::max<int64_t>(1, ((int64_t)CH + NEAR_WAVES - 1) / NEAR_WAVES));
     switch (model) {
-      case 0: launch_near<0>(precision, ngrid, c.stream, ft, points, ex_ptr_used, ex_pos, td, expansion_order, raise_singular, list, list_count, chunk_off, item_wave, (int)CH, cpart, cflag, cnear); break;
-      case 1: launch_near<1>(precision, ngrid, c.stream, ft, points, ex_ptr_used, ex_pos, td, expansion_order, raise_singular, list, list_count, chunk_off, item_wave, (int)CH, cpart, cflag, cnear); break;
-      default: launch_near<2>(precision, ngrid, c.stream, ft, points, ex_ptr_used, ex_pos, td, expansion_order, raise_singular, list, list_count, chunk_off, item_wave, (int)CH, cpart, cflag, cnear); break;
+      case 0: launch_near<0>(precision, ngrid, c.stream, ft, points, ex_ptr_used, ex_pos, td, expansion_order, raise_singular, list, list_count, chunk_off, item_wave, (int)CH, cls_count, cls_list, cpart, cflag, cnear); break;
+      case 1: launch_near<1>(precision, ngrid, c.stream, ft, points, ex_ptr_used, ex_pos, td, expansion_order, raise_singular, list, list_count, chunk_off, item_wave, (int)CH, cls_count, cls_list, cpart, cflag, cnear); break;
+      default: launch_near<2>(precision, ngrid, c.stream, ft, points, ex_ptr_used, ex_pos, td, expansion_order, raise_singular, list, list_count, chunk_off, item_wave, (int)CH, cls_count, cls_list, cpart, cflag, cnear); break;
     }
     OFW_HIP(hipGetLastError());
   }
