@@ -97,6 +97,7 @@ static constexpr int MB_MAX_CUT = 8;
 static constexpr int LEAF_LANES = 32;                  // lanes of one leaf job (half a wavefront)
 static constexpr int BOTTOM_MT = 256;                  // threads of k_moments_bottom
 static constexpr int HALVES = BOTTOM_MT / LEAF_LANES;  // leaf jobs at a time
+static constexpr int STAGE_SEGS = 16;                  // segments of a leaf staged in LDS at a time
 
 // Per-tree tables and lists of the moment pass (device, int32).
 struct MomentForest {
@@ -226,13 +227,13 @@ __device__ __forceinline__ unsigned long long dbits(double v) { return (unsigned
 //  * moments: lane q < 30 accumulates m[q] (m0 3, m1 9, m2 as 3 x 6).
 // The results go to node i of v; the squared bound to r2b[i] as its bits.
 __device__ void leaf_job(const MomentInputs& mi, NodeRef v, unsigned long long* r2b, int i, int a, int b,
-                         double (*sh)[LEAF_LANES], int hl) {
+                         double (*sh)[STAGE_SEGS], int hl) {
 #pragma clang fp contract(off)
   const int axis = hl < 3 ? hl : (hl < 6 ? hl - 3 : 0);
   const bool low = hl < 3;
   double bnd = low ? INFINITY : -INFINITY;
-  for (int base = a; base < b; base += LEAF_LANES) {
-    const int n = min(LEAF_LANES, b - base);
+  for (int base = a; base < b; base += STAGE_SEGS) {
+    const int n = min(STAGE_SEGS, b - base);
     if (hl < n) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -268,8 +269,8 @@ __device__ void leaf_job(const MomentInputs& mi, NodeRef v, unsigned long long* 
   const int al = kind == 2 ? (u == 0 ? 0 : (u == 1 || u == 3) ? 1 : 2) : 0;
   double acc = 0.0, r2 = 0.0, largest = 0.0;
   // staged per segment: d (0-2), alpha (3-5), length vector (6-8), gamma (9)
-  for (int base = a; base < b; base += LEAF_LANES) {
-    const int n = min(LEAF_LANES, b - base);
+  for (int base = a; base < b; base += STAGE_SEGS) {
+    const int n = min(STAGE_SEGS, b - base);
     if (hl < n) {
       const int k = base + hl;
       double sv[3], ev[3];
@@ -361,14 +362,14 @@ __global__ __launch_bounds__(256) void k_moment_lists(TreeArrays t, MomentForest
 // sweep over the subtree's segments (each segment against the centres of the internal nodes on its path;
 // the maximum over a wavefront's lanes that share a node, then an LDS atomic maximum on the bits, which is
 // exact and independent of the order: non-negative doubles order as their bit patterns).
-__global__ __launch_bounds__(BOTTOM_MT) void k_moments_bottom(TreeArrays t, MomentForest mf, MomentInputs mi,
+__global__ __launch_bounds__(BOTTOM_MT, 4) void k_moments_bottom(TreeArrays t, MomentForest mf, MomentInputs mi,
                                                               NodeScratch ns, double* __restrict__ centre,
                                                               double* __restrict__ core) {
 #pragma clang fp contract(off)
   __shared__ double s_box[SUB_CAP * 6], s_cen[SUB_CAP * 3], s_mom[SUB_CAP * 30], s_core[SUB_CAP];
   __shared__ unsigned long long s_r2[SUB_CAP];
   __shared__ int s_lo[SUB_CAP], s_hi[SUB_CAP], s_skip[SUB_CAP], s_depth[SUB_CAP], s_leaves[SUB_CAP];
-  __shared__ double stage[HALVES][10][LEAF_LANES];
+  __shared__ double stage[HALVES][10][STAGE_SEGS];
   if (t.counters[C_OVERFLOW] || (int)blockIdx.x >= t.counters[C_SUBTREES]) return;
   const int r = mf.roots[blockIdx.x];
   const int nn = t.p_skip[r] - r;  // at most 2^(MB_DEPTH+1) - 1 <= SUB_CAP
