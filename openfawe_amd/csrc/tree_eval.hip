@@ -759,7 +759,7 @@ struct NearSegs {
 };
 
 template <int MODEL, int PREC>
-__global__ __launch_bounds__(64 * NEAR_WAVES) void k_tree_near(ForestTables ft, const double* __restrict__ pts,
+__global__ __launch_bounds__(64 * NEAR_WAVES) __attribute__((amdgpu_waves_per_eu(4))) void k_tree_near(ForestTables ft, const double* __restrict__ pts,
                                                    const int32_t* __restrict__ ex_ptr,
                                                    const int32_t* __restrict__ ex_pos,
                                                    NearSegs segs, const float* __restrict__ rec32,
