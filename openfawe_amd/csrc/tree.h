@@ -72,6 +72,10 @@ struct TreeArrays {
   int32_t* rval[2] = {};         // 3 x m segment ids per buffer; the lists live here after the presort
   unsigned long long* lstat = nullptr;  // tile_cap x 3 look-back status words of the partition (tagged by level)
   int32_t* tickets = nullptr;    // LEVEL_MAX work-unit tickets, one per top level
+  // the finish's two single-pass scans (over lo and over the split flags): tickets and look-back status words
+  int32_t* scan_ticket = nullptr;
+  unsigned long long* scan_status = nullptr;
+  int scan_words = 0;
   int32_t* rcount = nullptr;     // 3 x 256 x rs_tiles digit counts of one presort pass, then their scans
   int32_t* rtotal = nullptr;     // 3 x 256 digit totals of the pass
   // runs of equal coordinates (at most 3m members in 3m/2 runs)
@@ -90,6 +94,11 @@ int upload_i32(Ctx& c, const std::vector<int64_t>& values, int32_t* dst);
 
 // Exclusive scan of n int32 values (three kernels); the total goes to *total_out when it is not null.
 void scan_exclusive(Ctx& c, int n, const int32_t* in, int32_t* out, int32_t* part, int32_t* total_out);
+// The same in one launch (single-pass look-back): *ticket and scan_lb_words(n) status words must be zero when it
+// starts; a look-back that gives up sets counters[C_OVERFLOW].
+int scan_lb_words(int64_t n);
+void scan_exclusive_lb(Ctx& c, int n, const int32_t* in, int32_t* out, int32_t* ticket, unsigned long long* status,
+                       int32_t* total_out, int32_t* counters);
 
 // Stable LSD radix sort of (key, val) int32 pairs of n items by the low 8 * passes bits of key (8-bit digits);
 // returns the buffer index (0 or 1) holding the result.  hist and hist_off hold radix_hist_size(n) entries, part

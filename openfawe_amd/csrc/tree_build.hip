@@ -88,6 +88,9 @@ __global__ void k_init_roots(TreeArrays t) {
     t.level_a[k] = k;
   }
   if (threadIdx.x < C_COUNT) t.counters[threadIdx.x] = 0;
+  // the finish's single-pass scans: tickets and status words start empty in every call
+  if (threadIdx.x < 2) t.scan_ticket[threadIdx.x] = 0;
+  for (int i = threadIdx.x; i < t.scan_words; i += blockDim.x) t.scan_status[i] = 0ull;
   // the list build's work-unit tickets and look-back status words start empty in every call
   if (t.lists) {
     for (int i = threadIdx.x; i < LEVEL_MAX; i += blockDim.x) t.tickets[i] = 0;
@@ -881,6 +884,13 @@ __global__ void k_rs_init(TreeArrays t, const double* __restrict__ mx, const dou
     if (A == 0) {
       t.tie[i] = 0;
       t.ord_inv[i] = i;
+      // the finish's histogram over lo (m + 1 entries)
+      t.lo_cnt[i] = 0;
+      t.lo_mind[i] = INT_MAX;
+      if (i == 0) {
+        t.lo_cnt[m] = 0;
+        t.lo_mind[m] = INT_MAX;
+      }
     }
   }
 }
@@ -1865,6 +1875,69 @@ void scan_exclusive(Ctx& c, int n, const int32_t* in, int32_t* out, int32_t* par
   hipLaunchKernelGGL(k_scan_apply, dim3(blocks), dim3(SCAN_THREADS), 0, c.stream, n, in, part, out);
 }
 
+// Exclusive scan of n int32 values in one launch: tiles of SCAN_ITEMS taken by ticket, each tile's prefix by a
+// single-pass look-back over the tiles before it.  The ticket and the status words (scan_lb_words(n) of them)
+// must be zero when the kernel starts; the last tile writes the total to *total_out when it is not null.
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_lb(int n, const int32_t* __restrict__ in,
+                                                          int32_t* __restrict__ out, int32_t* __restrict__ ticket,
+                                                          unsigned long long* __restrict__ status,
+                                                          int32_t* __restrict__ total_out,
+                                                          int32_t* __restrict__ counters) {
+  __shared__ int s_tile, s_prefix;
+  const int tiles = (n + SCAN_ITEMS - 1) / SCAN_ITEMS;
+  if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1);
+  __syncthreads();
+  const int tile = s_tile;
+  if (tile >= tiles) return;
+  const int base = tile * SCAN_ITEMS;
+  int v[SCAN_PER];
+  int sum = 0;
+#pragma unroll
+  for (int q = 0; q < SCAN_PER; ++q) {
+    const int i = base + (int)threadIdx.x * SCAN_PER + q;
+    v[q] = i < n ? in[i] : 0;
+    sum += v[q];
+  }
+  int total;
+  int r = block_exclusive_scan(sum, &total);
+  if (threadIdx.x < 64) {
+    int prefix = 0;
+    if (tile == 0) {
+      if (threadIdx.x == 0) lb_publish(status, 1u, true, total);
+    } else {
+      if (threadIdx.x == 0) lb_publish(status + tile, 1u, false, total);
+      bool ok = true;
+      prefix = lb_lookback(status, 1, 0, tile, 1u, &ok);
+      if (threadIdx.x == 0) {
+        lb_publish(status + tile, 1u, true, prefix + total);
+        if (!ok) counters[C_OVERFLOW] = 1;
+      }
+    }
+    if (threadIdx.x == 0) {
+      s_prefix = prefix;
+      if (tile == tiles - 1 && total_out) *total_out = prefix + total;
+    }
+  }
+  __syncthreads();
+  r += s_prefix;
+#pragma unroll
+  for (int q = 0; q < SCAN_PER; ++q) {
+    const int i = base + (int)threadIdx.x * SCAN_PER + q;
+    if (i < n) out[i] = r;
+    r += v[q];
+  }
+}
+
+int scan_lb_words(int64_t n) { return (int)((n + SCAN_ITEMS - 1) / SCAN_ITEMS) + 1; }
+
+void scan_exclusive_lb(Ctx& c, int n, const int32_t* in, int32_t* out, int32_t* ticket, unsigned long long* status,
+                       int32_t* total_out, int32_t* counters) {
+  const int tiles = (n + SCAN_ITEMS - 1) / SCAN_ITEMS;
+  if (tiles == 0) return;
+  hipLaunchKernelGGL(k_scan_lb, dim3(tiles), dim3(SCAN_THREADS), 0, c.stream, n, in, out, ticket, status, total_out,
+                     counters);
+}
+
 __global__ void k_node_pre(TreeArrays t) {
   const int count = live_count(t.counters, C_NODES);
   int deepest = 0;
@@ -1940,6 +2013,7 @@ int tree_layout(Ctx& c, int buf, int64_t m64, int K, int leaf, TreeArrays& t) {
   t.rs_passes = RS_KEY_PASSES + (K <= 1 ? 0 : (K <= 256 ? 1 : (K <= 65536 ? 2 : 3)));
   const size_t RUNS = 3 * M / 2 + 1;
   need += 2 * al(3 * M * 4) + 2 * al(3 * M * 4) + al((size_t)t.tile_cap * 3 * 8) + al(LEVEL_MAX * 4);
+  need += al(2 * 4) + al(((size_t)scan_lb_words(m + 1) + scan_lb_words(t.cap)) * 8);  // single-pass scans
   need += al(3 * (size_t)t.rs_tiles * RS_BINS * 4) + al(3 * RS_BINS * 4);
   need += al(3 * M * 4) + 2 * al(RUNS * 4) + al(M) + al(M * 4) + al(3 * M * 4);  // runs
   need += al(M * 4);                                                            // bottom phase on the lists
@@ -1995,6 +2069,9 @@ int tree_layout(Ctx& c, int buf, int64_t m64, int K, int leaf, TreeArrays& t) {
   for (int b = 0; b < 2; ++b) t.rkey[b] = (uint32_t*)carve(3 * M * 4);
   for (int b = 0; b < 2; ++b) t.rval[b] = (int32_t*)carve(3 * M * 4);
   t.lstat = (unsigned long long*)carve((size_t)t.tile_cap * 3 * 8);
+  t.scan_ticket = (int32_t*)carve(2 * 4);
+  t.scan_words = scan_lb_words(m + 1) + scan_lb_words(t.cap);
+  t.scan_status = (unsigned long long*)carve((size_t)t.scan_words * 8);
   t.tickets = (int32_t*)carve(LEVEL_MAX * 4);
   t.rcount = (int32_t*)carve(3 * (size_t)t.rs_tiles * RS_BINS * 4);
   t.rtotal = (int32_t*)carve(3 * RS_BINS * 4);
@@ -2089,14 +2166,17 @@ int tree_build(Ctx& c, TreeArrays& t, const double* mx, const double* my, const 
   OFW_HIP(hipGetLastError());
   // finish: pre-order layout from (lo, depth)
   const int n1 = m + 1;
-  hipLaunchKernelGGL(k_fill_i32, dim3((n1 + 255) / 256), dim3(256), 0, c.stream, n1, t.lo_cnt, 0);
-  hipLaunchKernelGGL(k_fill_i32, dim3((n1 + 255) / 256), dim3(256), 0, c.stream, n1, t.lo_mind, INT_MAX);
+  if (!(t.lists && m > 0)) {  // the list build's presort clears them
+    hipLaunchKernelGGL(k_fill_i32, dim3((n1 + 255) / 256), dim3(256), 0, c.stream, n1, t.lo_cnt, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3((n1 + 255) / 256), dim3(256), 0, c.stream, n1, t.lo_mind, INT_MAX);
+  }
   hipLaunchKernelGGL(k_node_hist, dim3(grid), dim3(256), 0, c.stream, t);
-  scan_exclusive(c, n1, t.lo_cnt, t.lo_base, t.scan_part, t.counters + C_FINAL);
+  scan_exclusive_lb(c, n1, t.lo_cnt, t.lo_base, t.scan_ticket, t.scan_status, t.counters + C_FINAL, t.counters);
   hipLaunchKernelGGL(k_node_pre, dim3(grid), dim3(256), 0, c.stream, t);
   hipLaunchKernelGGL(k_node_pre_links, dim3(grid), dim3(256), 0, c.stream, t);
   // internal ranks in pre-order (for the reference numbering)
-  scan_exclusive(c, t.cap, t.p_flag, t.p_irank, t.scan_part, nullptr);
+  scan_exclusive_lb(c, t.cap, t.p_flag, t.p_irank, t.scan_ticket + 1, t.scan_status + scan_lb_words(t.m + 1), nullptr,
+                    t.counters);
   OFW_HIP(hipGetLastError());
   return OFW_OK;
 }

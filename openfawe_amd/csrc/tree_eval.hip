@@ -1261,9 +1261,14 @@ __global__ void k_ex_positions(int n, int m, const int32_t* __restrict__ ex_ptr,
 
 // ------------------------------------------------------------------ traversal plan on the device
 // frontier nodes: depth == D, or leaves shallower than D (pre-order flags; zero past the node count)
-__global__ void k_task_flags(TreeArrays t, int D, int32_t* __restrict__ flag) {
+// The single-pass scan of the flags that follows: its ticket and status words are cleared here.
+__global__ void k_task_flags(TreeArrays t, int D, int32_t* __restrict__ flag, int32_t* __restrict__ scan_ticket,
+                             unsigned long long* __restrict__ scan_status, int scan_words) {
   const int count = live_count(t.counters, C_FINAL);
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p <= t.cap; p += gridDim.x * blockDim.x) {
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gt == 0) *scan_ticket = 0;
+  for (int i = gt; i < scan_words; i += gridDim.x * blockDim.x) scan_status[i] = 0ull;
+  for (int p = gt; p <= t.cap; p += gridDim.x * blockDim.x) {
     int f = 0;
     if (p < count) {
       const int d = t.p_depth[p];
@@ -1437,7 +1442,9 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
   const size_t E = (size_t)std::max<int64_t>(entry_cap, 1);
   (void)chunk_cap;
   size_t need = al((size_t)std::max<int64_t>(m, 1) * 4) + al((size_t)std::max<int64_t>(n_ex, 1) * 4);
+  const int task_words = scan_lb_words((int64_t)t.cap + 1);
   need += 2 * al(((size_t)t.cap + 1) * 4) + al((size_t)std::max<int64_t>(task_cap, 1) * sizeof(EvalTask));
+  need += al(4) + al((size_t)task_words * 8);
   need += 6 * al((size_t)(K + 1) * 4) + al(P * 24) + al(P) + (counts ? 2 * al(P * 4) : 0);
   need += al(W1);
   const int plan_words = PLAN_VALUES * (int)((W1 + 255) / 256);
@@ -1454,6 +1461,8 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
   int32_t* ex_pos = (int32_t*)carve((size_t)std::max<int64_t>(n_ex, 1) * 4);
   int32_t* flag = (int32_t*)carve(((size_t)t.cap + 1) * 4);
   int32_t* index = (int32_t*)carve(((size_t)t.cap + 1) * 4);
+  int32_t* task_ticket = (int32_t*)carve(4);
+  unsigned long long* task_status = (unsigned long long*)carve((size_t)task_words * 8);
   EvalTask* tasks = (EvalTask*)carve((size_t)std::max<int64_t>(task_cap, 1) * sizeof(EvalTask));
   // the host tables (target offsets, list bases, list capacities) side by side: one upload
   int32_t* toff = (int32_t*)carve((size_t)(K + 1) * 3 * 4);
@@ -1500,8 +1509,9 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
                        ex_ptr, ex_seg, inverse, ex_pos);
   }
   const int32_t* ex_ptr_used = (n_ex > 0) ? ex_ptr : nullptr;
-  hipLaunchKernelGGL(k_task_flags, dim3(grid), dim3(256), 0, c.stream, t, D, flag);
-  scan_exclusive(c, t.cap + 1, flag, index, t.scan_part, nullptr);
+  hipLaunchKernelGGL(k_task_flags, dim3(grid), dim3(256), 0, c.stream, t, D, flag, task_ticket, task_status,
+                     task_words);
+  scan_exclusive_lb(c, t.cap + 1, flag, index, task_ticket, task_status, nullptr, t.counters);
   hipLaunchKernelGGL(k_task_fill, dim3(grid), dim3(256), 0, c.stream, t, flag, index, (int)task_cap, tasks);
   hipLaunchKernelGGL(k_work_plan, dim3(1), dim3(1024), 0, c.stream, t, index, toff, task_first, wave_off, part_off,
                      (int)wave_cap, (int)part_cap, plan_ticket, plan_status, plan_words);
