@@ -104,6 +104,7 @@ int ofw_ctx_destroy(ofw_ctx* x) {
   c.exptr32.release();
   c.exseg32.release();
   c.vjp.release();
+  c.release_side();
   for (auto& b : c.tree) b.release();
   for (auto& e : c.ev)
     if (e) (void)hipEventDestroy(e);

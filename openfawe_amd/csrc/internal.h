@@ -35,7 +35,24 @@ struct Ctx {
   bool capturing = false;    // enqueuing into a graph capture: no events are recorded
   bool sort_build = false;   // build trees with the per-level sorts (set when the list build met equal coordinates)
   void* static_scratch = nullptr;  // this device's copy of the no-allocation scratch (resolved address)
+  // A tree call's second stream: the traversal plan (which needs the tree, not its node quantities) runs on it
+  // beside the node quantities; created on first use, default priority.
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  bool fork_pending = false;          // the node quantities record fork_ev once their subtree kernel is enqueued
+  bool plan_forked = false;           // the plan is being enqueued on `side`
+  hipStream_t plan_main = nullptr;    // the call's stream meanwhile (the null stream is a valid one)
   hipError_t mark() { return (!capturing && nev < 12) ? hipEventRecord(ev[nev++], stream) : hipSuccess; }
+  void release_side() {
+    if (side) {
+      (void)hipStreamSynchronize(side);
+      (void)hipStreamDestroy(side);
+    }
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+    if (join_ev) (void)hipEventDestroy(join_ev);
+    side = nullptr;
+    fork_ev = join_ev = nullptr;
+  }
 };
 
 #define OFW_HIP(expr)                                                                              \

@@ -540,6 +540,7 @@ int ofw_step_destroy(ofw_step* p) {
   c.exptr32.release();
   c.exseg32.release();
   c.vjp.release();
+  c.release_side();
   for (auto& b : c.tree) b.release();
   for (auto& e : c.ev)
     if (e) (void)hipEventDestroy(e);

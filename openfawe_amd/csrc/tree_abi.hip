@@ -68,15 +68,32 @@ int tree_pipeline_build(Ctx& c, const TreeCall& tc, TreeArrays& t, TreeDevice& t
   st = tree_build(c, t, mx, my, mz, tc.soff);
   if (st) return st;
   OFW_HIP(c.mark());
+  // the traversal plan needs the tree only: outside a capture it runs on the second stream beside the node
+  // quantities' last kernels (the subtree kernel fills the device; the top levels are one workgroup)
+  if (!c.capturing) {
+    if (!c.side) {
+      if (hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking) != hipSuccess ||
+          hipEventCreateWithFlags(&c.fork_ev, hipEventDisableTiming) != hipSuccess ||
+          hipEventCreateWithFlags(&c.join_ev, hipEventDisableTiming) != hipSuccess) {
+        c.release_side();
+        (void)hipGetLastError();
+      }
+    }
+    c.fork_pending = c.side != nullptr;
+  }
   st = tree_moments(c, 1, t, tc.soff, tc.starts, tc.directions, tc.lengths, tc.gamma, tc.radius, tc.model,
                     tc.clearance, td, false, tc.precision == OFW_FP64);
+  if (c.fork_pending) {  // not recorded by the node quantities (no segments, or an error)
+    c.fork_pending = false;
+    OFW_HIP(hipEventRecord(c.fork_ev, c.stream));
+  }
   if (st) return st;
   OFW_HIP(c.mark());
   return OFW_OK;
 }
 
 // The part that needs the targets and their exclusions: the traversal.
-int tree_pipeline_eval(Ctx& c, const TreeCall& tc, TreeArrays& t, const TreeDevice& td) {
+static int traversal(Ctx& c, const TreeCall& tc, TreeArrays& t, const TreeDevice& td) {
   int32_t *p32 = nullptr, *s32 = nullptr;
   if (tc.ex_ptr && tc.n_ex > 0) {
     int st = csr_convert(c, tc.n, tc.n_ex, tc.ex_ptr, tc.ex_seg, &p32, &s32);
@@ -89,8 +106,29 @@ int tree_pipeline_eval(Ctx& c, const TreeCall& tc, TreeArrays& t, const TreeDevi
   shape.seg_off = tc.soff;
   shape.split_depth = tc.split_depth;
   shape.chunk_slots = tc.chunk_slots;
-  int st = tree_evaluate(c, 2, t, td, shape, tc.precision, tc.points, tc.model, tc.raise_singular, p32, s32,
-                         p32 ? tc.n_ex : 0, tc.theta, tc.order, tc.out, tc.flags, tc.far_counts, tc.near_counts);
+  return tree_evaluate(c, 2, t, td, shape, tc.precision, tc.points, tc.model, tc.raise_singular, p32, s32,
+                       p32 ? tc.n_ex : 0, tc.theta, tc.order, tc.out, tc.flags, tc.far_counts, tc.near_counts);
+}
+
+int tree_pipeline_eval(Ctx& c, const TreeCall& tc, TreeArrays& t, const TreeDevice& td) {
+  // the plan's kernels go to the second stream, after the build; tree_evaluate joins it before the walk
+  if (!c.capturing && c.side) {
+    OFW_HIP(hipStreamWaitEvent(c.side, c.fork_ev, 0));
+    c.plan_main = c.stream;
+    c.plan_forked = true;
+    c.stream = c.side;
+  }
+  int st = traversal(c, tc, t, td);
+  if (c.plan_forked) {  // returned before tree_evaluate's join (an error): join here, the call's stream comes back
+    const hipError_t e1 = hipEventRecord(c.join_ev, c.stream);
+    c.stream = c.plan_main;
+    c.plan_forked = false;
+    const hipError_t e2 = hipStreamWaitEvent(c.stream, c.join_ev, 0);
+    if (!st && (e1 != hipSuccess || e2 != hipSuccess)) {
+      set_error("tree call: joining the plan stream failed");
+      return OFW_ERR_HIP;
+    }
+  }
   if (st) return st;
   OFW_HIP(c.mark());
   return OFW_OK;
@@ -575,6 +613,7 @@ int ofw_tree_batch_destroy(ofw_tree_batch* b) {
   c.exptr32.release();
   c.exseg32.release();
   c.vjp.release();
+  c.release_side();
   for (auto& buf : c.tree) buf.release();
   for (auto& e : c.ev)
     if (e) (void)hipEventDestroy(e);

@@ -1516,6 +1516,13 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
   hipLaunchKernelGGL(k_work_plan, dim3(1), dim3(1024), 0, c.stream, t, index, toff, task_first, wave_off, part_off,
                      (int)wave_cap, (int)part_cap, plan_ticket, plan_status, plan_words);
   OFW_HIP(hipGetLastError());
+  if (c.plan_forked) {
+    // the plan ran on the second stream beside the node quantities: the call's stream waits for it here
+    OFW_HIP(hipEventRecord(c.join_ev, c.stream));
+    c.stream = c.plan_main;
+    c.plan_forked = false;
+    OFW_HIP(hipStreamWaitEvent(c.stream, c.join_ev, 0));
+  }
   OFW_HIP(c.mark());
   ForestTables ft{K, td.count, (int)m, toff, task_first, wave_off, part_off, d_list_base, d_list_cap, t.counters};
   if (wave_cap > 0) {
