@@ -64,6 +64,7 @@ struct TreeArrays {
   int32_t *p_lo, *p_hi, *p_skip, *p_parent, *p_depth, *p_tree, *p_node, *p_irank;
   int32_t* tree_first;           // (K+1) first pre-order index of each tree
   int32_t* order = nullptr;      // segment permutation (tree order -> segment)
+  const double* mid[3] = {};     // the midpoints the tree is built on (segment order), for the traversal plan
   // list build: per-axis segment lists presorted once, split by stable partitions level after level
   int lists = 0;                 // 1: the top phase uses the lists (every node's segments stay in `order`)
   int rs_passes = 0;             // radix passes of the presort (4 key digits + the tree-id digits)

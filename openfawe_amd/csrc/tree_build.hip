@@ -2089,6 +2089,9 @@ int tree_layout(Ctx& c, int buf, int64_t m64, int K, int leaf, TreeArrays& t) {
 int tree_build(Ctx& c, TreeArrays& t, const double* mx, const double* my, const double* mz,
                const std::vector<int64_t>& offsets) {
   const int m = (int)t.m;
+  t.mid[0] = mx;
+  t.mid[1] = my;
+  t.mid[2] = mz;
   int64_t max_root = 0;
   for (int k = 0; k < t.K; ++k) max_root = std::max(max_root, offsets[k + 1] - offsets[k]);
   const int grid = c.cu_count * 2;

@@ -12,12 +12,18 @@ namespace ofw {
 
 static constexpr int MAX_ANC = 14;
 
-// One frontier subtree of the traversal: its root (pre-order index), the tree it belongs to, and the
-// root's ancestors, root of the tree first.
+// Frontier subtrees near their tree's targets (whose walks open most of their nodes) are split EXTRA_DEPTH levels
+// deeper than the others, so that no single walk sets the walk kernel's end.
+static constexpr int EXTRA_DEPTH = 3;
+
+// One frontier subtree of the traversal: its root (pre-order index), the root's ancestors (root of the tree first),
+// and its interaction lists: one of list_cap entries per group of 64 targets, from list_off.
 struct EvalTask {
   int32_t node;
   int32_t nanc;
   int32_t anc[MAX_ANC];
+  int32_t list_off;
+  int32_t list_cap;
 };
 
 // Per-call shape of the targets of a forest: targets of tree k are [target_off[k], target_off[k+1]).

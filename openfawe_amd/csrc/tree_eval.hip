@@ -209,8 +209,7 @@ struct ForestTables {
   const int32_t* task_first;  // first traversal task of each tree
   const int32_t* wave_off;    // first work item (wavefront) of each tree
   const int32_t* part_off;    // first partial-sum slot of each tree
-  const int32_t* list_base;   // first near-list entry of each tree
-  const int32_t* list_cap;    // near-list entries per wavefront of each tree
+  const EvalTask* tasks;      // the traversal tasks (frontier subtrees) of all trees, tree after tree
   const int32_t* counters;
 };
 
@@ -266,6 +265,15 @@ __device__ __forceinline__ int find_tree(const int32_t* __restrict__ table, int 
     else hi = mid;
   }
   return lo;
+}
+
+// The interaction list of walk wavefront `local` of tree k (groups of 64 targets per task): its first entry, and
+// its capacity in *cap.
+__device__ __forceinline__ size_t wave_list(const ForestTables& ft, int k, int local, int groups, int* cap) {
+  const int q = local / groups, g = local - q * groups;
+  const EvalTask* T = ft.tasks + ft.task_first[k] + q;
+  *cap = uniform(T->list_cap);
+  return (size_t)uniform(T->list_off) + (size_t)g * (size_t)*cap;
 }
 
 // Acceptance records are read through a per-wavefront window of WIN consecutive pre-order nodes in LDS: the
@@ -446,8 +454,8 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) void k_tree_eval(ForestTables ft, 
     }
   };
   // near-field list of this wavefront
-  const int lcap = uniform(ft.list_cap[k]);
-  const size_t lbase = (size_t)uniform(ft.list_base[k]) + (size_t)local * lcap;
+  const int lcap = uniform(T.list_cap);
+  const size_t lbase = (size_t)uniform(T.list_off) + (size_t)g * lcap;
   int nlist = 0;
   bool walked = false;
   if (end - F <= BFS_NODES) {
@@ -853,7 +861,8 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) __attribute__((amdgpu_waves_per_eu
     acc_t ux = 0, uy = 0, uz = 0;
     int flag = OFW_FLAG_OK, near = 0;
     int sbase = INT_MIN / 2;
-    const size_t lbase = (size_t)uniform(ft.list_base[k]) + (size_t)local * uniform(ft.list_cap[k]);
+    int lcap_unused;
+    const size_t lbase = wave_list(ft, k, local, groups, &lcap_unused);
     const int e0 = c * CHUNK;
     const int ne = min(uniform(list_count[w]), e0 + CHUNK) - e0;
     // the item's entries, one per lane, and the records of its accepted nodes staged in LDS together (lane
@@ -1044,7 +1053,9 @@ __global__ __launch_bounds__(256) void k_item_waves(ForestTables ft, const int32
   const int lane = threadIdx.x & 63;
   const int k = uniform(find_tree(ft.wave_off, ft.K, w));
   const int local = w - uniform(ft.wave_off[k]);
-  const size_t lbase = (size_t)uniform(ft.list_base[k]) + (size_t)local * uniform(ft.list_cap[k]);
+  const int nt = uniform(ft.toff[k + 1]) - uniform(ft.toff[k]);
+  int lcap_unused;
+  const size_t lbase = wave_list(ft, k, local, (nt + 63) >> 6, &lcap_unused);
   const int count = uniform(list_count[w]);
   const int it0 = uniform(chunk_off[w]), it1 = min(uniform(chunk_off[w + 1]), slots);
   const int woff = lane < ITEM_CLASSES ? cls_woff[(size_t)w * ITEM_CLASSES + lane] : 0;
@@ -1131,72 +1142,110 @@ __global__ __launch_bounds__(256) void k_chunk_plan(ForestTables ft, int W1, con
   }
 }
 
-// Per target: its tree's subtree partials and their near-field partials, in pre-order of the subtrees (those
-// of wavefronts that added nothing are skipped: they are zero).  A workgroup serves 64 targets with eight
-// wavefronts: wavefront p adds the p-th eighth of every subtree's near-field partials in order (and p = 0 the
-// subtree partials), and the eight sums are added in that order, a fixed association.  The flag kept is the
-// first in the sequential order.
+// Per target: its tree's subtree partials and their near-field partials.  A target's items are, subtree after
+// subtree in pre-order, the subtree partial (when the subtree's wavefront added anything) and the near-field
+// partials of that wavefront's list, in list order.  A workgroup serves one group of 64 targets of one tree
+// (groups numbered tree after tree from goff) with eight wavefronts: 64 subtrees at a time, the batch's items are
+// cut into eight equal ranges, wavefront p adds the p-th range in order (four items' loads in flight together),
+// and the eight sums are added in that order, a fixed association.  The flag kept is the first in item order.
+static constexpr int REDUCE_PARTS = 8;
+static constexpr int REDUCE_UNROLL = 4;
 template <bool WIDE>
-__global__ __launch_bounds__(512) void k_tree_reduce(ForestTables ft, int n, const double* __restrict__ part,
-                                                     const int8_t* __restrict__ pflag,
-                                                     const int32_t* __restrict__ pfar,
-                                                     const int32_t* __restrict__ pnear,
-                                                     const int8_t* __restrict__ pnz,
-                                                     const int32_t* __restrict__ chunk_off,
-                                                     const double* __restrict__ cpart,
-                                                     const int8_t* __restrict__ cflag,
-                                                     const int32_t* __restrict__ cnear, double* __restrict__ out,
-                                                     int8_t* __restrict__ flags, int64_t* __restrict__ far_counts,
-                                                     int64_t* __restrict__ near_counts) {
+__global__ __launch_bounds__(64 * REDUCE_PARTS) void k_tree_reduce(
+    ForestTables ft, const int32_t* __restrict__ goff, const double* __restrict__ part,
+    const int8_t* __restrict__ pflag, const int32_t* __restrict__ pfar, const int32_t* __restrict__ pnear,
+    const int8_t* __restrict__ pnz, const int32_t* __restrict__ chunk_off, const double* __restrict__ cpart,
+    const int8_t* __restrict__ cflag, const int32_t* __restrict__ cnear, double* __restrict__ out,
+    int8_t* __restrict__ flags, int64_t* __restrict__ far_counts, int64_t* __restrict__ near_counts) {
   typedef typename std::conditional<WIDE, double, float>::type acc_t;
-  constexpr int PARTS = 8;
+  constexpr int PARTS = REDUCE_PARTS;
   __shared__ acc_t s_u[PARTS][64][3];
   __shared__ long long s_key[PARTS][64], s_far[PARTS][64], s_near[PARTS][64];
   __shared__ int s_flag[PARTS][64];
   const int lane = threadIdx.x & 63, pid = (int)(threadIdx.x >> 6);
-  const int j = blockIdx.x * 64 + lane;
+  const int k = find_tree(goff, ft.K, (int)blockIdx.x);
+  const int g = (int)blockIdx.x - goff[k];
+  const int t0 = ft.toff[k], nt = ft.toff[k + 1] - t0;
+  const int tl = g * 64 + lane;  // the target's index in its tree
+  const bool valid = tl < nt;
+  const int ntasks = ft.counters[C_OVERFLOW] ? 0 : ft.task_first[k + 1] - ft.task_first[k];
+  const int groups = (nt + 63) >> 6;
+  const int w0 = ft.wave_off[k] + g;
+  const size_t base = (size_t)ft.part_off[k] + tl;
   acc_t ux = 0, uy = 0, uz = 0;
   int flag = 0;
-  long long fkey = LLONG_MAX;  // (task << 32) | (0 for the subtree partial, 2 i + 1 for near-field item i)
+  long long fkey = LLONG_MAX;  // (task << 32) | (0 for the subtree partial, 2 i + 1 for near-field partial i)
   long long far = 0, near = 0;
-  if (j < n) {
-    const int k = find_tree(ft.toff, ft.K, j);
-    const int t0 = ft.toff[k], nt = ft.toff[k + 1] - t0;
-    const int ntasks = ft.counters[C_OVERFLOW] ? 0 : ft.task_first[k + 1] - ft.task_first[k];
-    const int groups = (nt + 63) >> 6;
-    const int w0 = ft.wave_off[k] + ((j - t0) >> 6);
-    const size_t base = (size_t)ft.part_off[k] + (j - t0);
-    for (int q = 0; q < ntasks; ++q) {
-      const int w = w0 + q * groups;
-      if (pid == 0 && pnz[w]) {
-        const size_t o = base + (size_t)q * nt;
-        ux += (acc_t)part[3 * o];
-        uy += (acc_t)part[3 * o + 1];
-        uz += (acc_t)part[3 * o + 2];
-        if (flag == 0 && pflag[o] != 0) {
-          flag = pflag[o];
-          fkey = (long long)q << 32;
-        }
-        if (pfar) {
-          far += pfar[o];
-          near += pnear[o];
+  for (int qb = 0; qb < ntasks; qb += 64) {
+    // lane l describes subtree qb + l: its first chunk, its items, and whether its partial is there
+    const int qn = min(64, ntasks - qb);
+    int c0 = 0, cnt = 0, nz = 0;
+    if (lane < qn) {
+      const int w = w0 + (qb + lane) * groups;
+      nz = pnz[w] ? 1 : 0;
+      c0 = chunk_off ? chunk_off[w] : 0;
+      cnt = (chunk_off ? chunk_off[w + 1] - c0 : 0) + nz;
+    }
+    int incl = cnt;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += y;
+    }
+    const int total = __builtin_amdgcn_readlane(incl, 63);
+    const int i0 = (int)((long long)total * pid / PARTS), i1 = (int)((long long)total * (pid + 1) / PARTS);
+    for (int i = i0; i < i1; i += REDUCE_UNROLL) {
+      double vx[REDUCE_UNROLL], vy[REDUCE_UNROLL], vz[REDUCE_UNROLL];
+      int vf[REDUCE_UNROLL], vfar[REDUCE_UNROLL], vnear[REDUCE_UNROLL];
+      long long vkey[REDUCE_UNROLL];
+#pragma unroll
+      for (int u = 0; u < REDUCE_UNROLL; ++u) {
+        vx[u] = vy[u] = vz[u] = 0.0;
+        vf[u] = vfar[u] = vnear[u] = 0;
+        vkey[u] = LLONG_MAX;
+        const int ii = i + u;
+        if (ii >= i1) continue;
+        // the subtree holding item ii: the lanes whose items end at or before it
+        const int sl = __popcll(__ballot(incl <= ii));
+        const int first = __builtin_amdgcn_readlane(incl - cnt, sl);
+        const int fz = __builtin_amdgcn_readlane(nz, sl), fc = __builtin_amdgcn_readlane(c0, sl);
+        const int r = ii - first;
+        const long long q = qb + sl;
+        if (!valid) continue;
+        if (fz && r == 0) {
+          const size_t o = base + (size_t)q * nt;
+          vx[u] = part[3 * o];
+          vy[u] = part[3 * o + 1];
+          vz[u] = part[3 * o + 2];
+          vf[u] = pflag[o];
+          vkey[u] = q << 32;
+          if (pfar) {
+            vfar[u] = pfar[o];
+            vnear[u] = pnear[o];
+          }
+        } else {
+          const int ci = r - fz;
+          const size_t o = (size_t)(fc + ci) * 64 + (tl & 63);
+          vx[u] = cpart[3 * o];
+          vy[u] = cpart[3 * o + 1];
+          vz[u] = cpart[3 * o + 2];
+          vf[u] = cflag[o];
+          vkey[u] = (q << 32) | (long long)(2 * ci + 1);
+          if (cnear) vnear[u] = cnear[o];
         }
       }
-      if (chunk_off) {
-        // this wavefront's part of the subtree's near field, in the order of its walk
-        const int c0 = chunk_off[w], len = chunk_off[w + 1] - c0;
-        const int cs = c0 + (int)(((long long)len * pid) / PARTS), ce = c0 + (int)(((long long)len * (pid + 1)) / PARTS);
-        for (int ch = cs; ch < ce; ++ch) {
-          const size_t o = (size_t)ch * 64 + ((j - t0) & 63);
-          ux += (acc_t)cpart[3 * o];
-          uy += (acc_t)cpart[3 * o + 1];
-          uz += (acc_t)cpart[3 * o + 2];
-          if (flag == 0 && cflag[o] != 0) {
-            flag = cflag[o];
-            fkey = ((long long)q << 32) | (long long)(2 * (ch - c0) + 1);
-          }
-          if (cnear) near += cnear[o];
+#pragma unroll
+      for (int u = 0; u < REDUCE_UNROLL; ++u) {
+        if (i + u >= i1) break;
+        ux += (acc_t)vx[u];
+        uy += (acc_t)vy[u];
+        uz += (acc_t)vz[u];
+        if (flag == 0 && vf[u] != 0) {
+          flag = vf[u];
+          fkey = vkey[u];
         }
+        far += vfar[u];
+        near += vnear[u];
       }
     }
   }
@@ -1208,7 +1257,7 @@ __global__ __launch_bounds__(512) void k_tree_reduce(ForestTables ft, int n, con
   s_far[pid][lane] = far;
   s_near[pid][lane] = near;
   __syncthreads();
-  if (pid != 0 || j >= n) return;
+  if (pid != 0 || !valid) return;
   acc_t vx = s_u[0][lane][0], vy = s_u[0][lane][1], vz = s_u[0][lane][2];
   int fl = s_flag[0][lane];
   long long best = fl != 0 ? s_key[0][lane] : LLONG_MAX;
@@ -1224,9 +1273,10 @@ __global__ __launch_bounds__(512) void k_tree_reduce(ForestTables ft, int n, con
     }
   }
   if (!(isfinite((double)vx) && isfinite((double)vy) && isfinite((double)vz)) && fl == 0) fl = OFW_FLAG_NONFINITE;
-  out[3 * (size_t)j] = (double)vx;
-  out[3 * (size_t)j + 1] = (double)vy;
-  out[3 * (size_t)j + 2] = (double)vz;
+  const size_t j = (size_t)t0 + tl;
+  out[3 * j] = (double)vx;
+  out[3 * j + 1] = (double)vy;
+  out[3 * j + 2] = (double)vz;
   flags[j] = (int8_t)fl;
   if (far_counts) far_counts[j] = far;
   if (near_counts) near_counts[j] = near;
@@ -1260,10 +1310,106 @@ __global__ void k_ex_positions(int n, int m, const int32_t* __restrict__ ex_ptr,
 }
 
 // ------------------------------------------------------------------ traversal plan on the device
-// frontier nodes: depth == D, or leaves shallower than D (pre-order flags; zero past the node count)
-// The single-pass scan of the flags that follows: its ticket and status words are cleared here.
-__global__ void k_task_flags(TreeArrays t, int D, int32_t* __restrict__ flag, int32_t* __restrict__ scan_ticket,
-                             unsigned long long* __restrict__ scan_status, int scan_words) {
+// Boxes for the frontier, B workgroups per box, each over a B-th of its range (partial boxes at box_part[(j B + b)
+// 6 ...]): slot j < K 2^D is the j-th node at depth D of tree j >> D (the bits of j, first bit first, choose left or
+// right from the root; a slot whose path meets a leaf has no node), boxed over its segments' midpoints, its slot
+// recorded at its pre-order index; slot K 2^D + k is tree k's targets.
+__global__ __launch_bounds__(256) void k_task_boxes(TreeArrays t, int D, int B, const double* __restrict__ pts,
+                                                    const int32_t* __restrict__ toff, int32_t* __restrict__ node_slot,
+                                                    double* __restrict__ box_part) {
+  __shared__ double s_v[6][4];
+  const int slots = t.K << D;
+  const int j = blockIdx.x / B, b = blockIdx.x - j * B;
+  double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  if (t.counters[C_OVERFLOW]) return;
+  if (j < slots) {
+    const int k = j >> D;
+    int p = t.tree_first[k];
+    if (p >= t.tree_first[k + 1]) return;  // an empty tree
+    for (int d = D - 1; d >= 0 && p >= 0; --d) {
+      if (t.p_skip[p] == p + 1) p = -1;     // a leaf above depth D
+      else p = ((j >> d) & 1) ? t.p_skip[p + 1] : p + 1;
+    }
+    if (p < 0) return;
+    if (b == 0 && threadIdx.x == 0) node_slot[p] = j;
+    const int64_t lo = t.p_lo[p], len = t.p_hi[p] - lo;
+    const int q0 = (int)(lo + len * b / B), q1 = (int)(lo + len * (b + 1) / B);
+    for (int q = q0 + (int)threadIdx.x; q < q1; q += blockDim.x) {
+      const int s = t.order[q];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double x = t.mid[c][s];
+        v[c] = fmin(v[c], x);
+        v[3 + c] = fmax(v[3 + c], x);
+      }
+    }
+  } else {
+    const int k = j - slots;
+    const int64_t lo = toff[k], len = toff[k + 1] - lo;
+    const int i0 = (int)(lo + len * b / B), i1 = (int)(lo + len * (b + 1) / B);
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double x = pts[3 * (size_t)i + c];
+        v[c] = fmin(v[c], x);
+        v[3 + c] = fmax(v[3 + c], x);
+      }
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      v[c] = fmin(v[c], __shfl_xor(v[c], off, 64));
+      v[3 + c] = fmax(v[3 + c], __shfl_xor(v[3 + c], off, 64));
+    }
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0)
+#pragma unroll
+    for (int c = 0; c < 6; ++c) s_v[c][wv] = v[c];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int c = threadIdx.x;
+    double r = s_v[c][0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) r = c < 3 ? fmin(r, s_v[c][w]) : fmax(r, s_v[c][w]);
+    box_part[6 * ((size_t)j * B + b) + c] = r;
+  }
+}
+
+// the box of slot j from its B partial boxes
+__device__ __forceinline__ void slot_box(const double* __restrict__ box_part, int j, int B, double* box) {
+  for (int c = 0; c < 6; ++c) box[c] = c < 3 ? INFINITY : -INFINITY;
+  for (int b = 0; b < B; ++b) {
+    const double* q = box_part + 6 * ((size_t)j * B + b);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      box[c] = fmin(box[c], q[c]);
+      box[3 + c] = fmax(box[3 + c], q[3 + c]);
+    }
+  }
+}
+
+// A frontier subtree at depth D is split deeper when its tree's targets come within one half-diagonal of its
+// midpoints' box: the walks of the groups near it open most of its nodes.  (A heuristic of the schedule only:
+// every target's accepted nodes and near pairs are the reference's whatever the frontier.)
+__device__ __forceinline__ bool near_targets(const double* __restrict__ b, const double* __restrict__ tb) {
+  double gap2 = 0.0, diag2 = 0.0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double g = fmax(0.0, fmax(tb[c] - b[3 + c], b[c] - tb[3 + c]));
+    gap2 += g * g;
+    const double e = b[3 + c] - b[c];
+    diag2 += e * e;
+  }
+  return gap2 <= 0.25 * diag2;
+}
+
+// Frontier nodes (pre-order flags; zero past the node count): depth D, or a leaf shallower than D; in a subtree of
+// depth D near its tree's targets, depth D + extra instead (or a leaf shallower than that).  The single-pass scan
+// of the flags that follows: its ticket and status words are cleared here.
+__global__ void k_task_flags(TreeArrays t, int D, int extra, int B, const int32_t* __restrict__ node_slot,
+                             const double* __restrict__ box_part, int32_t* __restrict__ flag,
+                             int32_t* __restrict__ scan_ticket, unsigned long long* __restrict__ scan_status,
+                             int scan_words) {
   const int count = live_count(t.counters, C_FINAL);
   const int gt = blockIdx.x * blockDim.x + threadIdx.x;
   if (gt == 0) *scan_ticket = 0;
@@ -1272,7 +1418,20 @@ __global__ void k_task_flags(TreeArrays t, int D, int32_t* __restrict__ flag, in
     int f = 0;
     if (p < count) {
       const int d = t.p_depth[p];
-      f = (d == D || (d < D && t.p_skip[p] == p + 1)) ? 1 : 0;
+      const bool leaf = t.p_skip[p] == p + 1;
+      if (d < D) {
+        f = leaf ? 1 : 0;
+      } else if (d == D && (leaf || extra == 0)) {
+        f = 1;
+      } else if (d <= D + extra) {
+        int a = p;
+        for (int q = d; q > D; --q) a = t.p_parent[a];  // the ancestor at depth D
+        double nb[6], tb[6];
+        slot_box(box_part, node_slot[a], B, nb);
+        slot_box(box_part, (t.K << D) + t.p_tree[p], B, tb);
+        const bool split = near_targets(nb, tb);
+        f = split ? ((d == D + extra || leaf) ? 1 : 0) : (d == D ? 1 : 0);
+      }
     }
     flag[p] = f;
   }
@@ -1307,7 +1466,8 @@ __global__ __launch_bounds__(1024) void k_work_plan(TreeArrays t, const int32_t*
                                                     const int32_t* __restrict__ toff, int32_t* __restrict__ task_first,
                                                     int32_t* __restrict__ wave_off, int32_t* __restrict__ part_off,
                                                     int wave_cap, int part_cap, int32_t* __restrict__ plan_ticket,
-                                                    unsigned long long* __restrict__ plan_status, int plan_words) {
+                                                    unsigned long long* __restrict__ plan_status, int plan_words,
+                                                    EvalTask* __restrict__ tasks, int min_leaf, int64_t entry_cap) {
   // the list plan's tile ticket and look-back status words start empty in every call
   if (plan_ticket) {
     if (threadIdx.x == 0) *plan_ticket = 0;
@@ -1338,13 +1498,35 @@ __global__ __launch_bounds__(1024) void k_work_plan(TreeArrays t, const int32_t*
     waves += tw;
     parts += tp;
   }
+  // each task's interaction lists: groups x (3 entries per leaf its subtree can hold, plus 6), in task order
+  const int ntasks = index[t.tree_first[t.K]];
+  int64_t entries = 0;
+  for (int base = 0; base < ntasks; base += blockDim.x) {
+    const int i = base + threadIdx.x;
+    int cap = 0, sz = 0;
+    if (i < ntasks) {
+      const int p = tasks[i].node;
+      const int k = t.p_tree[p];
+      const int groups = (toff[k + 1] - toff[k] + 63) >> 6;
+      cap = 3 * ((t.p_hi[p] - t.p_lo[p]) / min_leaf + 2);
+      sz = groups * cap;
+    }
+    int excl, tot;
+    excl = block_exclusive_scan(sz, &tot);
+    if (i < ntasks) {
+      const int64_t off = entries + excl;
+      tasks[i].list_off = off <= (int64_t)INT_MAX ? (int32_t)off : 0;
+      tasks[i].list_cap = cap;
+    }
+    entries += tot;
+  }
   if (threadIdx.x == 0) {
-    if (waves > wave_cap || parts > part_cap) {
+    if (waves > wave_cap || parts > part_cap || entries > entry_cap) {
       t.counters[C_OVERFLOW] = 1;
       waves = 0;
     }
     t.counters[C_WORK] = waves;
-    t.counters[C_TASKS] = index[t.tree_first[t.K]];
+    t.counters[C_TASKS] = ntasks;
   }
 }
 
@@ -1404,29 +1586,29 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
   const int K = shape.K;
   const int64_t n = shape.n, m = t.m;
   const int D = std::max(0, std::min(shape.split_depth, MAX_ANC));
+  // deeper subtrees near the targets only where the frontier depth was raised to fill the GPU with walks: a call of
+  // many target groups (D = 0) has walks enough
+  const int64_t slots64 = ((int64_t)K << D) + K;
+  const int extra = D > 0 && slots64 <= (1 << 20) ? std::max(0, std::min(EXTRA_DEPTH, MAX_ANC - D)) : 0;
+  // the frontier boxes (K 2^D subtrees, K target sets): B partial boxes each, about two workgroups per CU in all
+  const int box_slots = extra > 0 ? (int)slots64 : 0;
+  const int B = extra > 0 ? (int)std::min<int64_t>(16, (2 * (int64_t)c.cu_count + slots64 - 1) / slots64) : 1;
   const bool lists = true;
-  // host bounds: a tree has at most min(2^D, m_k) frontier subtrees of at most ceil(m_k / 2^D) segments; a
-  // subtree of S segments has at most S / floor((leaf + 1) / 2) + 1 leaves
+  // host bounds: a tree's frontier is an antichain of at most min(2^(D + extra), m_k) subtrees; a task's lists
+  // hold 3 entries per leaf its subtree can have (per visited node one entry, accepted or opened; a leaf two)
+  // plus 6, and a subtree of S segments has at most S / floor((leaf + 1) / 2) + 1 leaves
   const int64_t min_leaf = std::max(1, (t.leaf + 1) / 2);
-  int64_t task_cap = 0, wave_cap = 0, part_cap = 0, entry_cap = 0, chunk_cap = 0;
-  std::vector<int64_t> list_base(K + 1, 0), list_cap(K + 1, 0);
+  int64_t task_cap = 0, wave_cap = 0, part_cap = 0, entry_cap = 0;
   for (int k = 0; k < K; ++k) {
     const int64_t mk = shape.seg_off[k + 1] - shape.seg_off[k];
     const int64_t nk = shape.target_off[k + 1] - shape.target_off[k];
-    const int64_t tasks_k = mk > 0 ? std::min<int64_t>((int64_t)1 << D, mk) : 0;
-    const int64_t waves_k = tasks_k * ((nk + 63) / 64);
+    const int64_t tasks_k = mk > 0 ? std::min<int64_t>((int64_t)1 << (D + extra), mk) : 0;
+    const int64_t groups_k = (nk + 63) / 64;
     task_cap += tasks_k;
-    wave_cap += waves_k;
+    wave_cap += tasks_k * groups_k;
     part_cap += tasks_k * nk;
-    const int64_t sub = (mk + ((int64_t)1 << D) - 1) >> D;
-    // per visited node one entry (accepted or opened), a leaf two (accepted by some lanes, opened by others):
-    // at most 3 x the subtree's leaves
-    list_cap[k] = lists && mk > 0 ? 3 * (sub / min_leaf + 2) : 0;
-    list_base[k] = entry_cap;
-    entry_cap += waves_k * list_cap[k];
-    chunk_cap += waves_k * ((list_cap[k] + CHUNK - 1) / CHUNK);
+    entry_cap += lists ? groups_k * 3 * (mk / min_leaf + 3 * tasks_k) : 0;
   }
-  list_base[K] = entry_cap;
   if (part_cap > INT_MAX || wave_cap > INT_MAX / 4) {
     set_error("tree traversal: split depth too large for the target count");
     return OFW_ERR_ARG;
@@ -1440,12 +1622,12 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
   const bool counts = far_counts || near_counts;
   const size_t W1 = (size_t)wave_cap + 1;
   const size_t E = (size_t)std::max<int64_t>(entry_cap, 1);
-  (void)chunk_cap;
   size_t need = al((size_t)std::max<int64_t>(m, 1) * 4) + al((size_t)std::max<int64_t>(n_ex, 1) * 4);
   const int task_words = scan_lb_words((int64_t)t.cap + 1);
   need += 2 * al(((size_t)t.cap + 1) * 4) + al((size_t)std::max<int64_t>(task_cap, 1) * sizeof(EvalTask));
   need += al(4) + al((size_t)task_words * 8);
-  need += 6 * al((size_t)(K + 1) * 4) + al(P * 24) + al(P) + (counts ? 2 * al(P * 4) : 0);
+  need += al((size_t)(K + 1) * 8) + 3 * al((size_t)(K + 1) * 4) + al(P * 24) + al(P) + (counts ? 2 * al(P * 4) : 0);
+  need += al(((size_t)t.cap + 1) * 4) + al((size_t)std::max(box_slots, 1) * B * 6 * 8);  // frontier boxes
   need += al(W1);
   const int plan_words = PLAN_VALUES * (int)((W1 + 255) / 256);
   if (lists) need += al(E * sizeof(NearEntry)) + 2 * al(W1 * 4) + al(ITEM_CLASSES * 4) + 2 * al(W1 * ITEM_CLASSES * 4) +
@@ -1464,10 +1646,11 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
   int32_t* task_ticket = (int32_t*)carve(4);
   unsigned long long* task_status = (unsigned long long*)carve((size_t)task_words * 8);
   EvalTask* tasks = (EvalTask*)carve((size_t)std::max<int64_t>(task_cap, 1) * sizeof(EvalTask));
-  // the host tables (target offsets, list bases, list capacities) side by side: one upload
-  int32_t* toff = (int32_t*)carve((size_t)(K + 1) * 3 * 4);
-  int32_t* d_list_base = toff + (K + 1);
-  int32_t* d_list_cap = toff + 2 * (K + 1);
+  // the host tables side by side, one upload: first target and first group of 64 targets of each tree
+  int32_t* toff = (int32_t*)carve((size_t)(K + 1) * 2 * 4);
+  int32_t* goff = toff + (K + 1);
+  int32_t* node_slot = (int32_t*)carve(((size_t)t.cap + 1) * 4);  // written at the depth-D nodes only
+  double* box_part = (double*)carve((size_t)std::max(box_slots, 1) * B * 6 * 8);
   int32_t* task_first = (int32_t*)carve((size_t)(K + 1) * 4);
   int32_t* wave_off = (int32_t*)carve((size_t)(K + 1) * 4);
   int32_t* part_off = (int32_t*)carve((size_t)(K + 1) * 4);
@@ -1495,9 +1678,10 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
   }
 
   std::vector<int64_t> tables(shape.target_off);
-  if (lists) {
-    tables.insert(tables.end(), list_base.begin(), list_base.end());
-    tables.insert(tables.end(), list_cap.begin(), list_cap.end());
+  int64_t groups_total = 0;
+  for (int k = 0; k <= K; ++k) {
+    tables.push_back(groups_total);
+    if (k < K) groups_total += (shape.target_off[k + 1] - shape.target_off[k] + 63) / 64;
   }
   int st = upload_i32(c, tables, toff);
   if (st) return st;
@@ -1509,12 +1693,16 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
                        ex_ptr, ex_seg, inverse, ex_pos);
   }
   const int32_t* ex_ptr_used = (n_ex > 0) ? ex_ptr : nullptr;
-  hipLaunchKernelGGL(k_task_flags, dim3(grid), dim3(256), 0, c.stream, t, D, flag, task_ticket, task_status,
-                     task_words);
+  if (extra > 0)
+    hipLaunchKernelGGL(k_task_boxes, dim3((unsigned)(box_slots * B)), dim3(256), 0, c.stream, t, D, B, points, toff,
+                       node_slot, box_part);
+  hipLaunchKernelGGL(k_task_flags, dim3(grid), dim3(256), 0, c.stream, t, D, extra, B, node_slot, box_part, flag,
+                     task_ticket, task_status, task_words);
   scan_exclusive_lb(c, t.cap + 1, flag, index, task_ticket, task_status, nullptr, t.counters);
   hipLaunchKernelGGL(k_task_fill, dim3(grid), dim3(256), 0, c.stream, t, flag, index, (int)task_cap, tasks);
   hipLaunchKernelGGL(k_work_plan, dim3(1), dim3(1024), 0, c.stream, t, index, toff, task_first, wave_off, part_off,
-                     (int)wave_cap, (int)part_cap, plan_ticket, plan_status, plan_words);
+                     (int)wave_cap, (int)part_cap, plan_ticket, plan_status, plan_words, tasks, (int)min_leaf,
+                     (int64_t)entry_cap);
   OFW_HIP(hipGetLastError());
   if (c.plan_forked) {
     // the plan ran on the second stream beside the node quantities: the call's stream waits for it here
@@ -1524,7 +1712,7 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
     OFW_HIP(hipStreamWaitEvent(c.stream, c.join_ev, 0));
   }
   OFW_HIP(c.mark());
-  ForestTables ft{K, td.count, (int)m, toff, task_first, wave_off, part_off, d_list_base, d_list_cap, t.counters};
+  ForestTables ft{K, td.count, (int)m, toff, task_first, wave_off, part_off, tasks, t.counters};
   if (wave_cap > 0) {
     dim3 egrid((unsigned)((wave_cap + NEAR_WAVES - 1) / NEAR_WAVES));
     switch (model) {
@@ -1577,15 +1765,15 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
     OFW_HIP(hipGetLastError());
   }
   OFW_HIP(c.mark());
-  if (n > 0) {
+  if (groups_total > 0) {
     if (precision == OFW_FP32)
-      hipLaunchKernelGGL(k_tree_reduce<false>, dim3((unsigned)((n + 63) / 64)), dim3(512), 0, c.stream, ft, (int)n,
-                         part, pflag, pfar, pnear, pnz, (const int32_t*)chunk_off, cpart, cflag, cnear, out, flags,
-                         far_counts, near_counts);
+      hipLaunchKernelGGL(k_tree_reduce<false>, dim3((unsigned)groups_total), dim3(64 * REDUCE_PARTS), 0, c.stream, ft,
+                         goff, part, pflag, pfar, pnear, pnz, (const int32_t*)chunk_off, cpart, cflag, cnear, out,
+                         flags, far_counts, near_counts);
     else
-      hipLaunchKernelGGL(k_tree_reduce<true>, dim3((unsigned)((n + 63) / 64)), dim3(512), 0, c.stream, ft, (int)n,
-                         part, pflag, pfar, pnear, pnz, (const int32_t*)chunk_off, cpart, cflag, cnear, out, flags,
-                         far_counts, near_counts);
+      hipLaunchKernelGGL(k_tree_reduce<true>, dim3((unsigned)groups_total), dim3(64 * REDUCE_PARTS), 0, c.stream, ft,
+                         goff, part, pflag, pfar, pnear, pnz, (const int32_t*)chunk_off, cpart, cflag, cnear, out,
+                         flags, far_counts, near_counts);
   }
   OFW_HIP(hipGetLastError());
   c.stats_i[3] = D;
