@@ -1388,19 +1388,14 @@ __device__ __forceinline__ void slot_box(const double* __restrict__ box_part, in
   }
 }
 
-// A frontier subtree at depth D is split deeper when its tree's targets come within one half-diagonal of its
-// midpoints' box: the walks of the groups near it open most of its nodes.  (A heuristic of the schedule only:
-// every target's accepted nodes and near pairs are the reference's whatever the frontier.)
+// A frontier subtree at depth D is split deeper when the box of its midpoints meets the box of its tree's targets:
+// the walks of the groups among them open most of its nodes.  (A choice of the schedule only: every target's
+// accepted nodes and near pairs are the reference's whatever the frontier.)
 __device__ __forceinline__ bool near_targets(const double* __restrict__ b, const double* __restrict__ tb) {
-  double gap2 = 0.0, diag2 = 0.0;
+  bool meet = true;
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const double g = fmax(0.0, fmax(tb[c] - b[3 + c], b[c] - tb[3 + c]));
-    gap2 += g * g;
-    const double e = b[3 + c] - b[c];
-    diag2 += e * e;
-  }
-  return gap2 <= 0.25 * diag2;
+  for (int c = 0; c < 3; ++c) meet = meet && tb[c] <= b[3 + c] && b[c] <= tb[3 + c];
+  return meet;
 }
 
 // Frontier nodes (pre-order flags; zero past the node count): depth D, or a leaf shallower than D; in a subtree of
