@@ -1632,25 +1632,27 @@ __global__ __launch_bounds__(BL_THREADS) void k_bottom_lists(TreeArrays t, int l
     __syncthreads();
     const bool ties = any_tie != 0;
     if (ties) {  // run ids from the root's lists (coordinates compared for flagged neighbours only)
-      for (int A = 0; A < 3; ++A) {
-        for (int i = threadIdx.x; i < s; i += blockDim.x) {
-          const int x = L[0][A][i];
-          bool eq = false;
-          if (i > 0 && side[x]) {
-            const int y = L[0][A][i - 1];
-            eq = side[y] && cs[A][seg[x]] == cs[A][seg[y]];
-          }
-          pos_new[i] = eq ? 1 : 0;  // equal to the entry before it
+      // the three axes together (the second list buffer, free until the first split, holds the equal flags)
+      for (int idx = threadIdx.x; idx < 3 * s; idx += blockDim.x) {
+        const int A = idx / s, i = idx - A * s;
+        const int x = L[0][A][i];
+        bool eq = false;
+        if (i > 0 && side[x]) {
+          const int y = L[0][A][i - 1];
+          eq = side[y] && cs[A][seg[x]] == cs[A][seg[y]];
         }
-        __syncthreads();
-        for (int i = threadIdx.x; i < s; i += blockDim.x) {
-          const bool member = pos_new[i] || (i + 1 < s && pos_new[i + 1]);
-          int r = i;
-          while (r > 0 && pos_new[r]) --r;
-          rid[A][L[0][A][i]] = (uint16_t)(member ? r : RID_NONE + i);
-        }
-        __syncthreads();
+        L[1][A][i] = eq ? 1 : 0;  // equal to the entry before it
       }
+      __syncthreads();
+      for (int idx = threadIdx.x; idx < 3 * s; idx += blockDim.x) {
+        const int A = idx / s, i = idx - A * s;
+        const uint16_t* __restrict__ eq = L[1][A];
+        const bool member = eq[i] || (i + 1 < s && eq[i + 1]);
+        int r = i;
+        while (r > 0 && eq[r]) --r;
+        rid[A][L[0][A][i]] = (uint16_t)(member ? r : RID_NONE + i);
+      }
+      __syncthreads();
     }
     int cur = 0;
     for (int k = 0;; ++k) {
@@ -1748,16 +1750,43 @@ __global__ __launch_bounds__(BL_THREADS) void k_bottom_lists(TreeArrays t, int l
             const int x = L[cur][A][i];
             int dest = i;
             const int r = rid[A][x];
-            if ((hinfo[j] & 3) == 2 && r < RID_NONE) {
+            // the split axis's lists are in the new order already
+            if ((hinfo[j] & 3) == 2 && r < RID_NONE && ((hinfo[j] >> 2) & 3) != A) {
               const int lo = hlo[j], hi = hhi[j], mid = (lo + hi) / 2;
               const int cl = i < mid ? lo : mid, ch = i < mid ? mid : hi;
+              const uint16_t* __restrict__ Lc = L[cur][A];
+              const uint16_t* __restrict__ rA = rid[A];
+              // the run's ends within the child, four entries a step (their loads issued together)
               int rs = i, re = i + 1;
-              while (rs > cl && rid[A][L[cur][A][rs - 1]] == r) --rs;
-              while (re < ch && rid[A][L[cur][A][re]] == r) ++re;
+              for (;;) {
+                const int r1 = rA[Lc[max(rs - 1, cl)]], r2 = rA[Lc[max(rs - 2, cl)]];
+                const int r3 = rA[Lc[max(rs - 3, cl)]], r4 = rA[Lc[max(rs - 4, cl)]];
+                if (rs - 1 < cl || r1 != r) break;
+                --rs;
+                if (rs - 1 < cl || r2 != r) break;
+                --rs;
+                if (rs - 1 < cl || r3 != r) break;
+                --rs;
+                if (rs - 1 < cl || r4 != r) break;
+                --rs;
+              }
+              for (;;) {
+                const int r1 = rA[Lc[min(re, ch - 1)]], r2 = rA[Lc[min(re + 1, ch - 1)]];
+                const int r3 = rA[Lc[min(re + 2, ch - 1)]], r4 = rA[Lc[min(re + 3, ch - 1)]];
+                if (re >= ch || r1 != r) break;
+                ++re;
+                if (re >= ch || r2 != r) break;
+                ++re;
+                if (re >= ch || r3 != r) break;
+                ++re;
+                if (re >= ch || r4 != r) break;
+                ++re;
+              }
               if (re - rs > 1) {
                 const int p = pos_new[x];
                 int rank = 0;
-                for (int q = rs; q < re; ++q) rank += pos_new[L[cur][A][q]] < p;
+#pragma unroll 4
+                for (int q = rs; q < re; ++q) rank += pos_new[Lc[q]] < p;
                 dest = rs + rank;
               }
             }
