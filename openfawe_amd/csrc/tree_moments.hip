@@ -560,15 +560,26 @@ __device__ __forceinline__ void max_bits(unsigned long long* a, double v) {
 }
 
 // Exact squared bounds of the internal nodes above the cut depth in one pass over the segments (four per
-// lane): each segment's end points against the centre of each such node on its root path; the maximum over
-// a wavefront's segments that share the node, over the workgroup's wavefronts when they all share it, then
-// an integer atomic maximum on the bits (exact in any order: non-negative doubles order as their bits).
+// lane): each segment's end points against the centre of each such node on its root path, maxima combined over
+// the workgroup, then an integer atomic maximum on the bits (exact in any order: non-negative doubles order as
+// their bits).  A workgroup's range of 1,024 consecutive positions meets at most two nodes of a depth when the
+// nodes above the cut are larger than the range: then two lanes walk the root paths of its first and last
+// positions and every depth takes two centres from LDS and two atomics.  Otherwise (a range over several trees,
+// a leaf above the cut, small top nodes) each lane walks its segments' paths and the maxima are combined over the
+// wavefronts that share a node.
+static constexpr int BT_LEVELS = 32;  // the host caps the cut depth at 30
 __global__ __launch_bounds__(256) void k_bound_top(TreeArrays t, MomentForest mf, MomentInputs mi,
                                                    const double* __restrict__ centre, double* __restrict__ r2out) {
 #pragma clang fp contract(off)
   constexpr int J = 4, WAVES = 4;
   __shared__ int s_node[WAVES];  // per wavefront: its one node, -1 nothing live, -2 several nodes
   __shared__ unsigned long long s_bits[WAVES];
+  __shared__ unsigned long long s_bits2[WAVES];
+  __shared__ int s_path[2][BT_LEVELS];     // the internal nodes on the root paths of the range's end positions
+  __shared__ int s_lo[2][BT_LEVELS], s_hi[2][BT_LEVELS];
+  __shared__ double s_cen[2][BT_LEVELS][3];
+  __shared__ int s_depth[2], s_tree[2];    // per path: depths walked (a leaf or the cut stops it), its tree
+  __shared__ int s_fast;
   if (t.counters[C_OVERFLOW]) return;
   const int m = (int)t.m;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -577,22 +588,95 @@ __global__ __launch_bounds__(256) void k_bound_top(TreeArrays t, MomentForest mf
   // the wavefronts of a workgroup run the same iterations and levels (they combine their maxima)
   for (int bb = blockIdx.x * WAVES * 64 * J; bb < m; bb += stride) {
     const int base = bb + wv * 64 * J;
-    int p[J], kk[J], D[J];
-    bool live[J];
+    int kk[J];
     double s[J][3], e[J][3];
 #pragma unroll
     for (int j = 0; j < J; ++j) {
       kk[j] = base + 64 * j + lane;
       const bool valid = kk[j] < m;
-      const int tr = last_le(t.roots_lo, t.K, valid ? kk[j] : min(base, m - 1));
-      D[j] = valid ? mf.cut[tr] : 0;
-      p[j] = valid ? t.tree_first[tr] : -1;
-      live[j] = valid && D[j] > 0;
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
         s[j][q] = valid ? mi.s[q][kk[j]] : 0.0;
         e[j][q] = valid ? mi.end(q, kk[j]) : 0.0;
       }
+    }
+    if (threadIdx.x < 2) {
+      const int pos = threadIdx.x ? min(bb + WAVES * 64 * J, m) - 1 : bb;
+      const int tr = last_le(t.roots_lo, t.K, pos);
+      const int D = min(mf.cut[tr], BT_LEVELS);
+      int p = t.tree_first[tr], lev = 0;
+      for (; lev < D; ++lev) {
+        const int skip = t.p_skip[p], lo = t.p_lo[p], hi = t.p_hi[p];
+        const int hl = t.p_hi[p + 1], sl = t.p_skip[p + 1];
+        const double c0 = centre[3 * (size_t)p], c1 = centre[3 * (size_t)p + 1], c2 = centre[3 * (size_t)p + 2];
+        if (skip == p + 1) break;  // a leaf above the cut
+        s_path[threadIdx.x][lev] = p;
+        s_lo[threadIdx.x][lev] = lo;
+        s_hi[threadIdx.x][lev] = hi;
+        s_cen[threadIdx.x][lev][0] = c0;
+        s_cen[threadIdx.x][lev][1] = c1;
+        s_cen[threadIdx.x][lev][2] = c2;
+        p = pos < hl ? p + 1 : sl;
+      }
+      s_depth[threadIdx.x] = lev == D ? D : -1;
+      s_tree[threadIdx.x] = tr;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      bool fast = s_tree[0] == s_tree[1] && s_depth[0] >= 0 && s_depth[0] == s_depth[1];
+      for (int lev = 0; fast && lev < s_depth[0]; ++lev)
+        fast = s_path[0][lev] == s_path[1][lev] || s_hi[0][lev] == s_lo[1][lev];
+      s_fast = fast ? 1 : 0;
+    }
+    __syncthreads();
+    if (s_fast) {
+      const int D = s_depth[0];
+      for (int lev = 0; lev < D; ++lev) {
+        const int n0 = s_path[0][lev], n1 = s_path[1][lev];
+        const int split = n0 == n1 ? INT_MAX : s_hi[0][lev];  // positions below it are in n0
+        double v0 = 0.0, v1 = 0.0;
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+          if (kk[j] >= m) continue;
+          const int h = kk[j] < split ? 0 : 1;
+          const double c0 = s_cen[h][lev][0], c1 = s_cen[h][lev][1], c2 = s_cen[h][lev][2];
+          const double s0 = s[j][0] - c0, s1 = s[j][1] - c1, s2 = s[j][2] - c2;
+          const double qs = s0 * s0 + s1 * s1 + s2 * s2;
+          const double e0 = e[j][0] - c0, e1 = e[j][1] - c1, e2 = e[j][2] - c2;
+          const double qe = e0 * e0 + e1 * e1 + e2 * e2;
+          const double v = qs > qe ? qs : qe;
+          if (h == 0) v0 = v > v0 ? v : v0;
+          else v1 = v > v1 ? v : v1;
+        }
+        v0 = wave_max(v0);
+        v1 = wave_max(v1);
+        if (lane == 0) {
+          s_bits[wv] = dbits(v0);
+          s_bits2[wv] = dbits(v1);
+        }
+        __syncthreads();
+        if (threadIdx.x < 2 && (threadIdx.x == 0 || n1 != n0)) {
+          unsigned long long mb = 0ull;
+          for (int w = 0; w < WAVES; ++w) {
+            const unsigned long long b = threadIdx.x ? s_bits2[w] : s_bits[w];
+            mb = b > mb ? b : mb;
+          }
+          unsigned long long* a = bits + (threadIdx.x ? n1 : n0);
+          if (mb > *reinterpret_cast<volatile unsigned long long*>(a)) atomicMax(a, mb);
+        }
+        __syncthreads();  // s_bits and s_bits2 are written again at the next depth
+      }
+      continue;
+    }
+    int p[J], D[J];
+    bool live[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const bool valid = kk[j] < m;
+      const int tr = last_le(t.roots_lo, t.K, valid ? kk[j] : min(base, m - 1));
+      D[j] = valid ? mf.cut[tr] : 0;
+      p[j] = valid ? t.tree_first[tr] : -1;
+      live[j] = valid && D[j] > 0;
     }
     for (int lev = 0;; ++lev) {
       bool any = false;
@@ -839,7 +923,8 @@ int tree_moments(Ctx& c, int buf, const TreeArrays& t, const std::vector<int64_t
       hipLaunchKernelGGL(k_internal_level, dim3((unsigned)std::max(1, std::min(grid, (cap + 255) / 256))), dim3(256),
                          0, c.stream, t, mf, d, ns, td.centre, td.core);
     hipLaunchKernelGGL(k_moments_top, dim3(K), dim3(512), 0, c.stream, t, mf, ns, td.centre, td.core);
-    hipLaunchKernelGGL(k_bound_top, dim3((unsigned)std::min<int64_t>(grid, (m + 255) / 256)), dim3(256), 0, c.stream,
+    // one range of 1,024 positions per workgroup
+    hipLaunchKernelGGL(k_bound_top, dim3((unsigned)std::max<int64_t>(1, (m + 1023) / 1024)), dim3(256), 0, c.stream,
                        t, mf, mi, td.centre, ns.r2);
   }
   NodeOutputs no = want_outputs ? td.out : NodeOutputs{};
