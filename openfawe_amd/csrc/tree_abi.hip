@@ -69,7 +69,7 @@ int tree_pipeline_build(Ctx& c, const TreeCall& tc, TreeArrays& t, TreeDevice& t
   if (st) return st;
   OFW_HIP(c.mark());
   // the traversal plan needs the tree only: outside a capture it runs on the second stream beside the node
-  // quantities' last kernels (the subtree kernel fills the device; the top levels are one workgroup)
+  // quantities from their subtree kernel on (the plan's small kernels slow the permute more than the subtrees)
   if (!c.capturing) {
     if (!c.side) {
       if (hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking) != hipSuccess ||

@@ -913,12 +913,12 @@ int tree_moments(Ctx& c, int buf, const TreeArrays& t, const std::vector<int64_t
                        lengths, gamma, radius, model, mi, td.rec64, td.rec32);
     hipLaunchKernelGGL(k_moment_lists, dim3((unsigned)std::max(1, std::min(grid, (cap + 255) / 256))), dim3(256), 0,
                        c.stream, t, mf);
-    hipLaunchKernelGGL(k_moments_bottom, dim3((unsigned)wgs), dim3(BOTTOM_MT), 0, c.stream, t, mf, mi, ns,
-                       td.centre, td.core);
     if (c.fork_pending) {  // a tree call's traversal plan may start on its second stream from here
       c.fork_pending = false;
       OFW_HIP(hipEventRecord(c.fork_ev, c.stream));
     }
+    hipLaunchKernelGGL(k_moments_bottom, dim3((unsigned)wgs), dim3(BOTTOM_MT), 0, c.stream, t, mf, mi, ns,
+                       td.centre, td.core);
     for (int d = cut_max - 1; d >= MB_MAX_CUT; --d)
       hipLaunchKernelGGL(k_internal_level, dim3((unsigned)std::max(1, std::min(grid, (cap + 255) / 256))), dim3(256),
                          0, c.stream, t, mf, d, ns, td.centre, td.core);
