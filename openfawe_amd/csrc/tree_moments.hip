@@ -357,6 +357,51 @@ __global__ __launch_bounds__(256) void k_moment_lists(TreeArrays t, MomentForest
   }
 }
 
+// index of the symmetric pair (j, l) among 00 01 02 11 12 22
+__device__ __forceinline__ int sym6(int j, int l) {
+  const int a = j < l ? j : l, b = j < l ? l : j;
+  return a == 0 ? b : (a == 1 ? 2 + b : 5);
+}
+
+// Field q of a node's far-field record from its moments v (m0 3, m1 9, m2 as 3 x 6 entries j <= l): the
+// contractions of vortex_tree.py:425-447 and the symmetric second moments (the diagonal once, the off-diagonal
+// doubled), by the operations of k_moments_final.
+__device__ __forceinline__ double far_field(const double* v, int q) {
+  const double* m1 = v + 3;
+  const double* m2s = v + 12;
+  if (q < FR_M1) return v[q];
+  if (q < FR_W1) return m1[q - FR_M1];
+  if (q < FR_M2) {
+    const int i = q - FR_W1;  // w1[i] = m1[a][b] - m1[b][a] for (a, b) = (1, 2), (2, 0), (0, 1)
+    const int a = (i + 1) % 3, b = (i + 2) % 3;
+    return m1[3 * a + b] - m1[3 * b + a];
+  }
+  if (q < FR_T2) {
+    const int x = q - FR_M2, u = x % 6;
+    const bool diag = (u == 0 || u == 3 || u == 5);
+    return diag ? m2s[x] : 2.0 * m2s[x];
+  }
+  if (q < FR_W2) {
+    const int i = q - FR_T2;
+    return m2s[6 * i + 0] + m2s[6 * i + 3] + m2s[6 * i + 5];
+  }
+  if (q < FR_PAD0) {
+    const int x = q - FR_W2, i = x / 3, k = x % 3;  // w2[i][k] = m2[a][b][k] - m2[b][a][k]
+    const int a = (i + 1) % 3, b = (i + 2) % 3;
+    return m2s[6 * a + sym6(b, k)] - m2s[6 * b + sym6(a, k)];
+  }
+  return 0.0;
+}
+
+// Where k_moments_bottom writes the final records of its subtree's nodes (on = 0: k_moments_final writes them).
+struct FinalOut {
+  int on;
+  double clearance;
+  double* acc;
+  double* far64;
+  float* far32;
+};
+
 // One workgroup per subtree below the cut depth, all of it in LDS: its leaves (a half wavefront each), its
 // internal nodes level by level from the deepest, then the exact squared bound of each internal node in one
 // sweep over the subtree's segments (each segment against the centres of the internal nodes on its path;
@@ -364,7 +409,7 @@ __global__ __launch_bounds__(256) void k_moment_lists(TreeArrays t, MomentForest
 // exact and independent of the order: non-negative doubles order as their bit patterns).
 __global__ __launch_bounds__(BOTTOM_MT, 4) void k_moments_bottom(TreeArrays t, MomentForest mf, MomentInputs mi,
                                                               NodeScratch ns, double* __restrict__ centre,
-                                                              double* __restrict__ core) {
+                                                              double* __restrict__ core, FinalOut fo) {
 #pragma clang fp contract(off)
   __shared__ double s_box[SUB_CAP * 6], s_cen[SUB_CAP * 3], s_mom[SUB_CAP * 30], s_core[SUB_CAP];
   __shared__ unsigned long long s_r2[SUB_CAP];
@@ -441,13 +486,53 @@ __global__ __launch_bounds__(BOTTOM_MT, 4) void k_moments_bottom(TreeArrays t, M
     }
     __syncthreads();
   }
-  // the subtree's nodes are the pre-order indices r .. r + nn - 1: contiguous rows of every array
-  for (int x = tid; x < 30 * nn; x += BOTTOM_MT) ns.mom[30 * (size_t)r + x] = s_mom[x];
-  for (int x = tid; x < 6 * nn; x += BOTTOM_MT) ns.box[6 * (size_t)r + x] = s_box[x];
+  // the subtree's nodes are the pre-order indices r .. r + nn - 1: contiguous rows of every array.  With fo.on
+  // the nodes' final records are written here, field by field over the workgroup, and the levels above read the
+  // root's quantities only.
+  const int keep = fo.on ? 1 : nn;
+  for (int x = tid; x < 30 * keep; x += BOTTOM_MT) ns.mom[30 * (size_t)r + x] = s_mom[x];
+  for (int x = tid; x < 6 * keep; x += BOTTOM_MT) ns.box[6 * (size_t)r + x] = s_box[x];
   for (int x = tid; x < 3 * nn; x += BOTTOM_MT) centre[3 * (size_t)r + x] = s_cen[x];
   if (tid < nn) {
     core[r + tid] = s_core[tid];
-    ns.r2[r + tid] = __longlong_as_double((long long)s_r2[tid]);
+    if (!fo.on || tid == 0) ns.r2[r + tid] = __longlong_as_double((long long)s_r2[tid]);
+  }
+  if (fo.on) {
+    for (int x = tid; x < FAR_REC * nn; x += BOTTOM_MT) {
+      const int i = x / FAR_REC, q = x - i * FAR_REC;
+      const double v = far_field(s_mom + 30 * i, q);
+      fo.far64[FAR_REC * (size_t)r + x] = v;
+      fo.far32[FAR_REC * (size_t)r + x] = (float)v;
+    }
+    const int tr = t.p_tree[r];
+    for (int x = tid; x < ACC_REC * nn; x += BOTTOM_MT) {
+      const int i = x / ACC_REC, q = x - i * ACC_REC;
+      double v;
+      if (q < A_BOUND) {
+        v = s_cen[3 * i + q];
+      } else if (q == A_BOUND) {
+        v = __builtin_sqrt(__longlong_as_double((long long)s_r2[i])) * (1.0 + 1e-12);
+      } else if (q == A_CLEAR) {
+        v = fo.clearance * s_core[i];
+      } else {
+        // p_lo, p_hi | p_skip, tree | right child (-1 for a leaf), 0
+        int32_t w[2];
+        const int node = r + i;
+        if (q == A_INTS) {
+          w[0] = s_lo[i];
+          w[1] = s_hi[i];
+        } else if (q == A_INTS2) {
+          w[0] = r + s_skip[i];
+          w[1] = tr;
+        } else {
+          w[0] = s_skip[i] != i + 1 ? r + s_skip[i + 1] : -1;
+          w[1] = 0;
+        }
+        (void)node;
+        v = __hiloint2double(w[1], w[0]);
+      }
+      fo.acc[ACC_REC * (size_t)r + x] = v;
+    }
   }
 }
 
@@ -754,20 +839,17 @@ __global__ __launch_bounds__(256) void k_bound_top(TreeArrays t, MomentForest mf
   }
 }
 
-// index of the symmetric pair (j, l) among 00 01 02 11 12 22
-__device__ __forceinline__ int sym6(int j, int l) {
-  const int a = j < l ? j : l, b = j < l ? l : j;
-  return a == 0 ? b : (a == 1 ? 2 + b : 5);
-}
-
 // One thread per node: the bounding radius from the exact squared bound, the contractions
 // (vortex_tree.py:425-447), full outputs (optional) and the packed traversal records.
 __global__ __launch_bounds__(256) void k_moments_final(TreeArrays t, NodeScratch ns, const double* __restrict__ centre,
                                                        const double* __restrict__ core, double clearance,
                                                        NodeOutputs no, double* __restrict__ acc_rec,
-                                                       double* __restrict__ far64, float* __restrict__ far32) {
+                                                       double* __restrict__ far64, float* __restrict__ far32,
+                                                       const int32_t* __restrict__ top_cut) {
   const int count = live_count(t.counters, C_FINAL);
   for (int node = blockIdx.x * blockDim.x + threadIdx.x; node < count; node += gridDim.x * blockDim.x) {
+    // top_cut: only the internal nodes above their tree's cut (k_moments_bottom finished the others)
+    if (top_cut && !(t.p_depth[node] < top_cut[t.p_tree[node]] && !is_leaf(t, node))) continue;
     const double r2 = ns.r2[node];
     double bound;
     {
@@ -917,8 +999,9 @@ int tree_moments(Ctx& c, int buf, const TreeArrays& t, const std::vector<int64_t
       c.fork_pending = false;
       OFW_HIP(hipEventRecord(c.fork_ev, c.stream));
     }
+    const FinalOut fo{want_outputs ? 0 : 1, clearance, td.acc, td.far64, td.far32};
     hipLaunchKernelGGL(k_moments_bottom, dim3((unsigned)wgs), dim3(BOTTOM_MT), 0, c.stream, t, mf, mi, ns,
-                       td.centre, td.core);
+                       td.centre, td.core, fo);
     for (int d = cut_max - 1; d >= MB_MAX_CUT; --d)
       hipLaunchKernelGGL(k_internal_level, dim3((unsigned)std::max(1, std::min(grid, (cap + 255) / 256))), dim3(256),
                          0, c.stream, t, mf, d, ns, td.centre, td.core);
@@ -928,8 +1011,9 @@ int tree_moments(Ctx& c, int buf, const TreeArrays& t, const std::vector<int64_t
                        t, mf, mi, td.centre, ns.r2);
   }
   NodeOutputs no = want_outputs ? td.out : NodeOutputs{};
+  const int32_t* top_cut = (m > 0 && !want_outputs) ? d_table : nullptr;
   hipLaunchKernelGGL(k_moments_final, dim3((unsigned)std::max(1, std::min(grid, (cap + 255) / 256))), dim3(256), 0,
-                     c.stream, t, ns, td.centre, td.core, clearance, no, td.acc, td.far64, td.far32);
+                     c.stream, t, ns, td.centre, td.core, clearance, no, td.acc, td.far64, td.far32, top_cut);
   OFW_HIP(hipGetLastError());
   return OFW_OK;
 }
