@@ -75,6 +75,32 @@ __global__ __launch_bounds__(256) void k_tree_permute(int m, const int32_t* __re
   pack_near32(s, directions, lengths, gamma, radius, model, rec32 + (size_t)k * NEAR32);
 }
 
+// One step of a wavefront maximum of 64-bit patterns: each lane against the lane the DPP control names (rows
+// outside ROW_MASK keep their value).
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ unsigned long long max_dpp(unsigned long long v) {
+  const unsigned lo = (unsigned)v, hi = (unsigned)(v >> 32);
+  const unsigned olo = (unsigned)__builtin_amdgcn_update_dpp((int)lo, (int)lo, CTRL, ROW_MASK, 0xf, false);
+  const unsigned ohi = (unsigned)__builtin_amdgcn_update_dpp((int)hi, (int)hi, CTRL, ROW_MASK, 0xf, false);
+  const unsigned long long o = ((unsigned long long)ohi << 32) | olo;
+  return o > v ? o : v;
+}
+
+// The maximum of a 64-bit pattern over the wavefront (every lane active), by DPP moves: quad swaps, the half-row
+// and row mirrors, then the row broadcasts that carry each row's maximum into the next rows; lane 63 ends with the
+// whole wavefront's.
+__device__ __forceinline__ unsigned long long wave_max_bits(unsigned long long v) {
+  v = max_dpp<0xB1, 0xF>(v);   // quad_perm [1, 0, 3, 2]
+  v = max_dpp<0x4E, 0xF>(v);   // quad_perm [2, 3, 0, 1]
+  v = max_dpp<0x141, 0xF>(v);  // row_half_mirror
+  v = max_dpp<0x140, 0xF>(v);  // row_mirror
+  v = max_dpp<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+  v = max_dpp<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), 63);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
@@ -332,27 +358,47 @@ __device__ __host__ __forceinline__ int halving_depth(int64_t s, int leaf) {
   return d;
 }
 
+// Ranges of k_bound_top: BT_SPAN consecutive positions each, and its table of the internal nodes above the cut
+// that meet a range (per range and depth, slot 0: the node holding the range's first position, slot 1: a node
+// starting inside the range).  The table is not cleared: k_bound_top checks every entry against the tree.
+static constexpr int BT_LEVELS = 32;  // the host caps the cut depth at 30
+static constexpr int BT_SPAN = 1024;
+
 // One pass over the nodes: the subtree roots of k_moments_bottom (the nodes at their tree's cut depth and
-// the leaves above it) and, per tree and level, the internal nodes above min(cut, MB_MAX_CUT) for
-// k_moments_top.  The order of the lists does not matter: every entry is finished on its own.
-__global__ __launch_bounds__(256) void k_moment_lists(TreeArrays t, MomentForest mf) {
+// the leaves above it), per tree and level the internal nodes above min(cut, MB_MAX_CUT) for k_moments_top,
+// and the range table of k_bound_top.  The order of the lists does not matter: every entry is finished on its own.
+__global__ __launch_bounds__(256) void k_moment_lists(TreeArrays t, MomentForest mf, int32_t* __restrict__ btab) {
   const int count = live_count(t.counters, C_FINAL);
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < count; p += gridDim.x * blockDim.x) {
-    const int k = t.p_tree[p], d = t.p_depth[p], cut = mf.cut[k];
-    const bool leaf = is_leaf(t, p);
+  const int lane = threadIdx.x & 63;
+  // whole wavefronts step together: the range table is written by all lanes of a wavefront at once
+  for (int base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
+    const int p = base + (int)threadIdx.x;
+    const bool valid = p < count;
+    const int k = valid ? t.p_tree[p] : 0, d = valid ? t.p_depth[p] : 0, cut = valid ? mf.cut[k] : 0;
+    const bool leaf = valid && is_leaf(t, p);
     // subtree roots appended with one atomic per wavefront
-    const bool root = d == cut || (d < cut && leaf);
+    const bool root = valid && (d == cut || (d < cut && leaf));
     const unsigned long long ballot = __builtin_amdgcn_ballot_w64(root);
     if (ballot) {
-      const int lane = threadIdx.x & 63, leader = __builtin_ctzll(ballot);
-      int base = 0;
-      if (lane == leader) base = atomicAdd(t.counters + C_SUBTREES, __popcll(ballot));
-      base = __shfl(base, leader, 64);
-      if (root) mf.roots[base + __popcll(ballot & ((1ull << lane) - 1ull))] = p;
+      const int leader = __builtin_ctzll(ballot);
+      int rbase = 0;
+      if (lane == leader) rbase = atomicAdd(t.counters + C_SUBTREES, __popcll(ballot));
+      rbase = __shfl(rbase, leader, 64);
+      if (root) mf.roots[rbase + __popcll(ballot & ((1ull << lane) - 1ull))] = p;
     }
-    if (d < min(cut, MB_MAX_CUT) && !leaf) {
+    if (valid && d < min(cut, MB_MAX_CUT) && !leaf) {
       const int slot = atomicAdd(mf.top_count + (size_t)k * MB_MAX_CUT + d, 1);
       mf.top_list[((size_t)k << MB_MAX_CUT) + (1 << d) - 1 + slot] = p;
+    }
+    unsigned long long tops = __builtin_amdgcn_ballot_w64(valid && d < min(cut, BT_LEVELS) && !leaf);
+    const int lo_p = valid ? t.p_lo[p] : 0, hi_p = valid ? t.p_hi[p] : 0;
+    while (tops) {
+      const int l = __builtin_ctzll(tops);
+      tops &= tops - 1ull;
+      const int q = __builtin_amdgcn_readlane(p, l), dq = __builtin_amdgcn_readlane(d, l);
+      const int lo = __builtin_amdgcn_readlane(lo_p, l), hi = __builtin_amdgcn_readlane(hi_p, l);
+      for (int b = lo / BT_SPAN + lane; b <= (hi - 1) / BT_SPAN; b += 64)
+        btab[((size_t)b * BT_LEVELS + dq) * 2 + (lo <= b * BT_SPAN ? 0 : 1)] = q;
     }
   }
 }
@@ -647,23 +693,24 @@ __device__ __forceinline__ void max_bits(unsigned long long* a, double v) {
 // Exact squared bounds of the internal nodes above the cut depth in one pass over the segments (four per
 // lane): each segment's end points against the centre of each such node on its root path, maxima combined over
 // the workgroup, then an integer atomic maximum on the bits (exact in any order: non-negative doubles order as
-// their bits).  A workgroup's range of 1,024 consecutive positions meets at most two nodes of a depth when the
-// nodes above the cut are larger than the range: then two lanes walk the root paths of its first and last
-// positions and every depth takes two centres from LDS and two atomics.  Otherwise (a range over several trees,
-// a leaf above the cut, small top nodes) each lane walks its segments' paths and the maxima are combined over the
-// wavefronts that share a node.
-static constexpr int BT_LEVELS = 32;  // the host caps the cut depth at 30
+// their bits).  A workgroup's range of BT_SPAN consecutive positions meets at most two nodes of a depth when the
+// nodes above the cut are larger than the range: then the range's nodes come from the table of k_moment_lists and
+// every depth takes two centres from LDS and two atomics.  Otherwise (a range over several trees, a leaf above the
+// cut, small top nodes) each lane walks its segments' paths and the maxima are combined over the wavefronts that
+// share a node.
 __global__ __launch_bounds__(256) void k_bound_top(TreeArrays t, MomentForest mf, MomentInputs mi,
-                                                   const double* __restrict__ centre, double* __restrict__ r2out) {
+                                                   const double* __restrict__ centre, double* __restrict__ r2out,
+                                                   const int32_t* __restrict__ btab) {
 #pragma clang fp contract(off)
   constexpr int J = 4, WAVES = 4;
+  static_assert(J * WAVES * 64 == BT_SPAN, "a workgroup's positions are one range of the table");
   __shared__ int s_node[WAVES];  // per wavefront: its one node, -1 nothing live, -2 several nodes
   __shared__ unsigned long long s_bits[WAVES];
-  __shared__ unsigned long long s_bits2[WAVES];
+  __shared__ unsigned long long s_lev_bits[BT_LEVELS][2][WAVES];  // per depth and node: the wavefronts' maxima
   __shared__ int s_path[2][BT_LEVELS];     // the internal nodes on the root paths of the range's end positions
   __shared__ int s_lo[2][BT_LEVELS], s_hi[2][BT_LEVELS];
   __shared__ double s_cen[2][BT_LEVELS][3];
-  __shared__ int s_depth[2], s_tree[2];    // per path: depths walked (a leaf or the cut stops it), its tree
+  __shared__ int s_depth[1], s_tree[2];    // depths to do (the cut), the trees of the range's ends
   __shared__ int s_fast;
   if (t.counters[C_OVERFLOW]) return;
   const int m = (int)t.m;
@@ -685,40 +732,72 @@ __global__ __launch_bounds__(256) void k_bound_top(TreeArrays t, MomentForest mf
         e[j][q] = valid ? mi.end(q, kk[j]) : 0.0;
       }
     }
-    if (threadIdx.x < 2) {
-      const int pos = threadIdx.x ? min(bb + WAVES * 64 * J, m) - 1 : bb;
-      const int tr = last_le(t.roots_lo, t.K, pos);
-      const int D = min(mf.cut[tr], BT_LEVELS);
-      int p = t.tree_first[tr], lev = 0;
-      for (; lev < D; ++lev) {
-        const int skip = t.p_skip[p], lo = t.p_lo[p], hi = t.p_hi[p];
-        const int hl = t.p_hi[p + 1], sl = t.p_skip[p + 1];
-        const double c0 = centre[3 * (size_t)p], c1 = centre[3 * (size_t)p + 1], c2 = centre[3 * (size_t)p + 2];
-        if (skip == p + 1) break;  // a leaf above the cut
-        s_path[threadIdx.x][lev] = p;
-        s_lo[threadIdx.x][lev] = lo;
-        s_hi[threadIdx.x][lev] = hi;
-        s_cen[threadIdx.x][lev][0] = c0;
-        s_cen[threadIdx.x][lev][1] = c1;
-        s_cen[threadIdx.x][lev][2] = c2;
-        p = pos < hl ? p + 1 : sl;
-      }
-      s_depth[threadIdx.x] = lev == D ? D : -1;
-      s_tree[threadIdx.x] = tr;
+    // the range's nodes from the table, every entry checked against the tree (a stale or missing one sends the
+    // range to the per-lane walk); a leaf above the cut or a range over two trees does too
+    const int b0 = bb, b1 = min(bb + WAVES * 64 * J, m) - 1;
+    const int count = live_count(t.counters, C_FINAL);
+    if (threadIdx.x == 0) {
+      const int tr = last_le(t.roots_lo, t.K, b0);
+      s_tree[0] = tr;
+      s_tree[1] = last_le(t.roots_lo, t.K, b1);
+      s_depth[0] = min(mf.cut[tr], BT_LEVELS);
+      s_fast = 1;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-      bool fast = s_tree[0] == s_tree[1] && s_depth[0] >= 0 && s_depth[0] == s_depth[1];
+    if (threadIdx.x < BT_LEVELS) {
+      // one lane per depth: the node of the range's first position, and the node of its last position (the same
+      // node when it holds both, else the table's node starting inside the range)
+      const int lev = threadIdx.x, tr = s_tree[0];
+      if (lev < s_depth[0]) {
+        const int32_t* e = btab + ((size_t)(bb / BT_SPAN) * BT_LEVELS + lev) * 2;
+        const int pa = e[0], pb = e[1];
+        int lo[2] = {0, 0}, hi[2] = {0, 0}, pp[2] = {pa, pb};
+        bool ok[2];
+        double c[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int q = pp[h];
+          ok[h] = q >= 0 && q < count;
+          if (ok[h]) {
+            lo[h] = t.p_lo[q];
+            hi[h] = t.p_hi[q];
+            ok[h] = t.p_depth[q] == lev && t.p_tree[q] == tr && t.p_skip[q] != q + 1;
+#pragma unroll
+            for (int x = 0; x < 3; ++x) c[h][x] = centre[3 * (size_t)q + x];
+          }
+        }
+        ok[0] = ok[0] && lo[0] <= b0 && b0 < hi[0];
+        const bool one = ok[0] && b1 < hi[0];
+        ok[1] = one || (ok[1] && lo[1] <= b1 && b1 < hi[1]);
+        const int h1 = one ? 0 : 1;
+        s_path[0][lev] = pa;
+        s_lo[0][lev] = lo[0];
+        s_hi[0][lev] = hi[0];
+        s_path[1][lev] = pp[h1];
+        s_lo[1][lev] = lo[h1];
+        s_hi[1][lev] = hi[h1];
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+          s_cen[0][lev][x] = c[0][x];
+          s_cen[1][lev][x] = c[h1][x];
+        }
+        if (!(ok[0] && ok[1])) s_fast = 0;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && s_fast) {
+      bool fast = s_tree[0] == s_tree[1];
       for (int lev = 0; fast && lev < s_depth[0]; ++lev)
         fast = s_path[0][lev] == s_path[1][lev] || s_hi[0][lev] == s_lo[1][lev];
       s_fast = fast ? 1 : 0;
     }
     __syncthreads();
     if (s_fast) {
+      // every depth's two wavefront maxima into LDS, then one lane per (depth, node) combines the wavefronts and
+      // takes the atomic: a single round trip to memory for all depths
       const int D = s_depth[0];
       for (int lev = 0; lev < D; ++lev) {
-        const int n0 = s_path[0][lev], n1 = s_path[1][lev];
-        const int split = n0 == n1 ? INT_MAX : s_hi[0][lev];  // positions below it are in n0
+        const int split = s_path[0][lev] == s_path[1][lev] ? INT_MAX : s_hi[0][lev];  // below it: the first node
         double v0 = 0.0, v1 = 0.0;
 #pragma unroll
         for (int j = 0; j < J; ++j) {
@@ -733,24 +812,33 @@ __global__ __launch_bounds__(256) void k_bound_top(TreeArrays t, MomentForest mf
           if (h == 0) v0 = v > v0 ? v : v0;
           else v1 = v > v1 ? v : v1;
         }
-        v0 = wave_max(v0);
-        v1 = wave_max(v1);
-        if (lane == 0) {
-          s_bits[wv] = dbits(v0);
-          s_bits2[wv] = dbits(v1);
+        // a wavefront's 256 positions lie in one node at most depths: one reduction then
+        unsigned long long u0 = 0ull, u1 = 0ull;
+        if (split <= base) {
+          u1 = wave_max_bits(dbits(v1));
+        } else if (split >= base + 64 * J) {
+          u0 = wave_max_bits(dbits(v0));
+        } else {
+          u0 = wave_max_bits(dbits(v0));
+          u1 = wave_max_bits(dbits(v1));
         }
-        __syncthreads();
-        if (threadIdx.x < 2 && (threadIdx.x == 0 || n1 != n0)) {
+        if (lane == 0) {
+          s_lev_bits[lev][0][wv] = u0;
+          s_lev_bits[lev][1][wv] = u1;
+        }
+      }
+      __syncthreads();
+      if ((int)threadIdx.x < 2 * D) {
+        const int lev = threadIdx.x >> 1, h = threadIdx.x & 1;
+        const int n = s_path[h][lev];
+        if (h == 0 || n != s_path[0][lev]) {
           unsigned long long mb = 0ull;
-          for (int w = 0; w < WAVES; ++w) {
-            const unsigned long long b = threadIdx.x ? s_bits2[w] : s_bits[w];
-            mb = b > mb ? b : mb;
-          }
-          unsigned long long* a = bits + (threadIdx.x ? n1 : n0);
+          for (int w = 0; w < WAVES; ++w) mb = s_lev_bits[lev][h][w] > mb ? s_lev_bits[lev][h][w] : mb;
+          unsigned long long* a = bits + n;
           if (mb > *reinterpret_cast<volatile unsigned long long*>(a)) atomicMax(a, mb);
         }
-        __syncthreads();  // s_bits and s_bits2 are written again at the next depth
       }
+      __syncthreads();  // the LDS tables are written again by the next range
       continue;
     }
     int p[J], D[J];
@@ -953,6 +1041,8 @@ int tree_moments(Ctx& c, int buf, const TreeArrays& t, const std::vector<int64_t
   need += al(C * 3 * 8) + al(C * 8);
   need += al(C * ACC_REC * 8) + al(C * FAR_REC * 8) + al(C * FAR_REC * 4);
   need += al(C * 6 * 8) + al(C * 30 * 8) + al(C * 8) + al(C * 4);
+  const size_t nblk = (size_t)std::max<int64_t>(1, (m + BT_SPAN - 1) / BT_SPAN);
+  need += al(nblk * BT_LEVELS * 2 * 4);
   OFW_HIP(c.tree[buf].reserve(need));
   char* base = (char*)c.tree[buf].ptr;
   auto carve = [&](size_t bytes) {
@@ -984,6 +1074,7 @@ int tree_moments(Ctx& c, int buf, const TreeArrays& t, const std::vector<int64_t
   ns.mom = (double*)carve(C * 30 * 8);
   ns.r2 = (double*)carve(C * 8);
   ns.slot = (int32_t*)carve(C * 4);
+  int32_t* btab = (int32_t*)carve(nblk * BT_LEVELS * 2 * 4);
   td.count = cap;
   td.src = SegSource{t.order, starts, directions, lengths, gamma, radius, model};
   int st = upload_i32(c, table, d_table);
@@ -994,7 +1085,7 @@ int tree_moments(Ctx& c, int buf, const TreeArrays& t, const std::vector<int64_t
     hipLaunchKernelGGL(k_tree_permute, dim3((m + 255) / 256), dim3(256), 0, c.stream, m, t.order, starts, directions,
                        lengths, gamma, radius, model, mi, td.rec64, td.rec32);
     hipLaunchKernelGGL(k_moment_lists, dim3((unsigned)std::max(1, std::min(grid, (cap + 255) / 256))), dim3(256), 0,
-                       c.stream, t, mf);
+                       c.stream, t, mf, btab);
     if (c.fork_pending) {  // a tree call's traversal plan may start on its second stream from here
       c.fork_pending = false;
       OFW_HIP(hipEventRecord(c.fork_ev, c.stream));
@@ -1006,9 +1097,8 @@ int tree_moments(Ctx& c, int buf, const TreeArrays& t, const std::vector<int64_t
       hipLaunchKernelGGL(k_internal_level, dim3((unsigned)std::max(1, std::min(grid, (cap + 255) / 256))), dim3(256),
                          0, c.stream, t, mf, d, ns, td.centre, td.core);
     hipLaunchKernelGGL(k_moments_top, dim3(K), dim3(512), 0, c.stream, t, mf, ns, td.centre, td.core);
-    // one range of 1,024 positions per workgroup
-    hipLaunchKernelGGL(k_bound_top, dim3((unsigned)std::max<int64_t>(1, (m + 1023) / 1024)), dim3(256), 0, c.stream,
-                       t, mf, mi, td.centre, ns.r2);
+    // one range of BT_SPAN positions per workgroup
+    hipLaunchKernelGGL(k_bound_top, dim3((unsigned)nblk), dim3(256), 0, c.stream, t, mf, mi, td.centre, ns.r2, btab);
   }
   NodeOutputs no = want_outputs ? td.out : NodeOutputs{};
   const int32_t* top_cut = (m > 0 && !want_outputs) ? d_table : nullptr;
