@@ -11,6 +11,7 @@
 
 #include "common.h"
 #include "internal.h"
+#include "tree.h"
 #include "tree_records.h"
 
 namespace ofw {
@@ -65,6 +66,36 @@ int csr_convert(Ctx& c, int64_t n, int64_t e, const int64_t* ex_ptr, const int64
                      *ptr32);
   if (e > 0)
     hipLaunchKernelGGL(k_i64_to_i32, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, c.stream, e, ex_seg, *seg32);
+  OFW_HIP(hipGetLastError());
+  return OFW_OK;
+}
+
+__global__ void k_ex_seg_checked(int64_t n, int64_t cap, const int64_t* __restrict__ ex_ptr,
+                                 const int64_t* __restrict__ src, int32_t* __restrict__ dst,
+                                 int32_t* __restrict__ counters) {
+  const int64_t e = ex_ptr[n];
+  if (e < 0 || e > cap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      counters[C_EXNEED] = e < 0 ? -1 : (int32_t)(e < (int64_t)INT_MAX ? e : (int64_t)INT_MAX);
+      counters[C_OVERFLOW] = 1;
+    }
+    return;
+  }
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < e; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = (int32_t)src[i];
+}
+
+int csr_convert_checked(Ctx& c, int64_t n, int64_t cap, const int64_t* ex_ptr, const int64_t* ex_seg,
+                        int32_t* counters, int32_t** ptr32, int32_t** seg32) {
+  OFW_HIP(c.exptr32.reserve((size_t)(n + 1) * sizeof(int32_t)));
+  OFW_HIP(c.exseg32.reserve((size_t)(cap > 0 ? cap : 1) * sizeof(int32_t)));
+  *ptr32 = (int32_t*)c.exptr32.ptr;
+  *seg32 = (int32_t*)c.exseg32.ptr;
+  hipLaunchKernelGGL(k_i64_to_i32, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, c.stream, n + 1, ex_ptr,
+                     *ptr32);
+  const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((cap + 255) / 256, 8 * (int64_t)c.cu_count));
+  hipLaunchKernelGGL(k_ex_seg_checked, dim3((unsigned)blocks), dim3(256), 0, c.stream, n, cap, ex_ptr, ex_seg, *seg32,
+                     counters);
   OFW_HIP(hipGetLastError());
   return OFW_OK;
 }

@@ -32,6 +32,7 @@ struct Ctx {
   int nev = 0;               // events recorded by the last call (phase k = ev[k] -> ev[k+1])
   int64_t stats_i[8] = {};
   int64_t list_slots = 0;    // interaction-list work-item slots of the next tree call (0: measure)
+  int64_t ex_cap = 0;        // exclusion entries a tree call assumes without reading the count (0: read it)
   bool capturing = false;    // enqueuing into a graph capture: no events are recorded
   bool sort_build = false;   // build trees with the per-level sorts (set when the list build met equal coordinates)
   void* static_scratch = nullptr;  // this device's copy of the no-allocation scratch (resolved address)
@@ -89,6 +90,10 @@ int csr_to_i32(Ctx& c, int64_t n, const int64_t* ex_ptr, const int64_t* ex_seg, 
 int csr_count(Ctx& c, int64_t n, const int64_t* ex_ptr, int64_t* e_out);
 int csr_convert(Ctx& c, int64_t n, int64_t e, const int64_t* ex_ptr, const int64_t* ex_seg, int32_t** ptr32,
                 int32_t** seg32);
+// csr_convert for at most cap entries, the count read on the device: a larger count converts nothing and is left
+// in counters[C_EXNEED] with counters[C_OVERFLOW] set
+int csr_convert_checked(Ctx& c, int64_t n, int64_t cap, const int64_t* ex_ptr, const int64_t* ex_seg,
+                        int32_t* counters, int32_t** ptr32, int32_t** seg32);
 
 }  // namespace ofw
 

@@ -28,7 +28,8 @@ enum {
   C_STRAND,         // top-phase nodes whose segments stay in the second order buffer
   C_RUNS,           // runs of equal coordinates in the roots' lists (list build)
   C_RUNMEM,         // their members
-  C_COUNT = 18
+  C_EXNEED,         // a call's exclusion count when it exceeds the capacity the call assumed (0: it fits)
+  C_COUNT = 19
 };
 static constexpr int LEVEL_MAX = 64;  // top levels of the list build (a tree of 2^28 segments has at most 18)
 

@@ -96,7 +96,9 @@ int tree_pipeline_build(Ctx& c, const TreeCall& tc, TreeArrays& t, TreeDevice& t
 static int traversal(Ctx& c, const TreeCall& tc, TreeArrays& t, const TreeDevice& td) {
   int32_t *p32 = nullptr, *s32 = nullptr;
   if (tc.ex_ptr && tc.n_ex > 0) {
-    int st = csr_convert(c, tc.n, tc.n_ex, tc.ex_ptr, tc.ex_seg, &p32, &s32);
+    int st = tc.ex_capacity
+                 ? csr_convert_checked(c, tc.n, tc.n_ex, tc.ex_ptr, tc.ex_seg, t.counters, &p32, &s32)
+                 : csr_convert(c, tc.n, tc.n_ex, tc.ex_ptr, tc.ex_seg, &p32, &s32);
     if (st) return st;
   }
   EvalShape shape;
@@ -376,9 +378,18 @@ int ofw_tree_sum(ofw_ctx* x, int precision, int64_t n, const double* points, int
   tc.raise_singular = raise_singular;
   tc.ex_ptr = ex_ptr;
   tc.ex_seg = ex_seg;
+  // the exclusion count is read back once per context; later calls assume the capacity kept from it (with margin)
+  // and the device checks it, so that no call waits for a read-back before its first kernel
+  auto keep_ex_cap = [&](int64_t e) { c.ex_cap = std::max(c.ex_cap, std::min<int64_t>(INT_MAX, e + e / 4 + 1024)); };
   if (ex_ptr) {
-    st = ofw::csr_count(c, n, ex_ptr, &tc.n_ex);
-    if (st) return st;
+    if (c.ex_cap > 0) {
+      tc.n_ex = c.ex_cap;
+      tc.ex_capacity = true;
+    } else {
+      st = ofw::csr_count(c, n, ex_ptr, &tc.n_ex);
+      if (st) return st;
+      keep_ex_cap(tc.n_ex);
+    }
   }
   tc.theta = theta;
   tc.order = expansion_order;
@@ -407,6 +418,17 @@ int ofw_tree_sum(ofw_ctx* x, int precision, int64_t n, const double* points, int
     OFW_HIP(hipMemcpyAsync(counters, t.counters, sizeof(counters), hipMemcpyDeviceToHost, c.stream));
     OFW_HIP(hipStreamSynchronize(c.stream));
     c.stats_i[7] = attempt + 1;  // runs of the pipeline this call took
+    if (tc.ex_capacity && counters[ofw::C_EXNEED] != 0) {
+      // more exclusions than the kept capacity (or a malformed CSR): count them and run again
+      tc.ex_capacity = false;
+      st = ofw::csr_count(c, n, ex_ptr, &tc.n_ex);
+      if (st) {
+        c.sort_build = saved_sort;
+        return st;
+      }
+      keep_ex_cap(tc.n_ex);
+      continue;
+    }
     if (counters[ofw::C_DUP] && !c.sort_build) {
       c.sort_build = true;
       continue;

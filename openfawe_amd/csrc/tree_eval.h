@@ -64,7 +64,8 @@ struct TreeCall {
   const double *gamma = nullptr, *radius = nullptr;
   int model = 0, raise_singular = 0;
   const int64_t *ex_ptr = nullptr, *ex_seg = nullptr;
-  int64_t n_ex = 0;
+  int64_t n_ex = 0;          // exclusion entries, or (ex_capacity) an upper bound the device checks
+  bool ex_capacity = false;
   double theta = 0.5;
   int order = 2;
   int leaf = 32;

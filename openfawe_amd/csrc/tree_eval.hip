@@ -1290,9 +1290,10 @@ __global__ void k_inverse(int m, const int32_t* __restrict__ order, int32_t* __r
 
 // _positions (vortex_tree.py:571-580): per target, the sorted tree positions of its excluded segments
 __global__ void k_ex_positions(int n, int m, const int32_t* __restrict__ ex_ptr, const int32_t* __restrict__ ex_seg,
-                               const int32_t* __restrict__ inverse, int32_t* __restrict__ pos) {
+                               const int32_t* __restrict__ inverse, int32_t* __restrict__ pos,
+                               const int32_t* __restrict__ counters) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
+  if (p >= n || counters[C_OVERFLOW]) return;  // an exclusion count beyond the call's capacity converts nothing
   const int a = ex_ptr[p], b = ex_ptr[p + 1];
   for (int s = a; s < b; ++s) {
     const int seg = ex_seg[s];
@@ -1685,7 +1686,7 @@ int tree_evaluate(Ctx& c, int buf, const TreeArrays& t, const TreeDevice& td, co
     hipLaunchKernelGGL(k_inverse, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, c.stream, (int)m, t.order,
                        inverse);
     hipLaunchKernelGGL(k_ex_positions, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, (int)n, (int)m,
-                       ex_ptr, ex_seg, inverse, ex_pos);
+                       ex_ptr, ex_seg, inverse, ex_pos, t.counters);
   }
   const int32_t* ex_ptr_used = (n_ex > 0) ? ex_ptr : nullptr;
   if (extra > 0)

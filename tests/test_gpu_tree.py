@@ -346,6 +346,30 @@ def test_batch_with_unsplittable_nodes(use_graph):
         batch.close()
 
 
+def test_exclusion_capacity_grows_between_calls():
+    """Calls after the first assume the exclusion capacity kept from earlier calls and check it on the device: a
+    call with more exclusions than that reports the count, runs again with it, and matches the oracle; the same
+    call again runs once and returns the same bits."""
+    case = dict(workloads.sheet(60))
+    n, m = len(case["points"]), len(case["starts"])
+    rng = np.random.default_rng(11)
+    per = 2_000_000 // n + 1  # more entries than any earlier call of the process kept room for
+    seg = (rng.integers(0, m, size=(n, 1)) + np.arange(per)[None, :]) % m
+    case["pairs"] = np.column_stack((np.repeat(np.arange(n), per), seg.ravel())).astype(np.int64)
+    run_tree(workloads.sheet(8), 0.3)  # a small call first: the context's capacity is set
+    out, flagged, st = run_tree(case, 0.3)
+    assert st["attempts"] >= 2
+    ref, fref, far_ref, near_ref = oracle_tree(case, 0.3)
+    assert not flagged and not fref.any()
+    assert np.array_equal(st["far_counts"], far_ref)
+    assert np.array_equal(st["near_counts"], near_ref)
+    scale = np.linalg.norm(ref, axis=1).max()
+    assert np.abs(out - ref).max() <= 1e-10 * scale
+    again, _, st_again = run_tree(case, 0.3)
+    assert st_again["attempts"] == 1
+    assert np.array_equal(out.view(np.uint64), again.view(np.uint64))
+
+
 def test_list_slots_grow_between_calls():
     """A call whose near-field work items exceed the slots kept from the previous, smaller call reports the
     overflow on the device, runs again with its own count, and returns what its first call returned."""
