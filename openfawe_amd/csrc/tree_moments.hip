@@ -769,17 +769,17 @@ __global__ __launch_bounds__(256) void k_bound_top(TreeArrays t, MomentForest mf
         ok[0] = ok[0] && lo[0] <= b0 && b0 < hi[0];
         const bool one = ok[0] && b1 < hi[0];
         ok[1] = one || (ok[1] && lo[1] <= b1 && b1 < hi[1]);
-        const int h1 = one ? 0 : 1;
+        // selects, not indexing: the arrays stay in registers
         s_path[0][lev] = pa;
         s_lo[0][lev] = lo[0];
         s_hi[0][lev] = hi[0];
-        s_path[1][lev] = pp[h1];
-        s_lo[1][lev] = lo[h1];
-        s_hi[1][lev] = hi[h1];
+        s_path[1][lev] = one ? pa : pb;
+        s_lo[1][lev] = one ? lo[0] : lo[1];
+        s_hi[1][lev] = one ? hi[0] : hi[1];
 #pragma unroll
         for (int x = 0; x < 3; ++x) {
           s_cen[0][lev][x] = c[0][x];
-          s_cen[1][lev][x] = c[h1][x];
+          s_cen[1][lev][x] = one ? c[0][x] : c[1][x];
         }
         if (!(ok[0] && ok[1])) s_fast = 0;
       }
