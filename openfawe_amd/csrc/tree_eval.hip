@@ -800,8 +800,9 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) __attribute__((amdgpu_waves_per_eu
   const int lane = threadIdx.x & 63;
   const int works = min(live_count(ft.counters, C_CHUNKS), chunk_slots);
   const int nwaves = live_count(ft.counters, C_WORK);
-  const int stride = gridDim.x * NEAR_WAVES;
-  for (int slot = uniform(blockIdx.x * NEAR_WAVES + (int)(threadIdx.x >> 6)); slot < works; slot += stride) {
+  // one dispatch slot per wavefront (the grid covers every slot): the loop runs once, so nothing stays live
+  // from one item to the next
+  for (int slot = uniform(blockIdx.x * NEAR_WAVES + (int)(threadIdx.x >> 6)); slot < works; slot = works) {
     // the item of this dispatch slot: the classes heaviest first
     int item = -1;
     {
@@ -829,35 +830,11 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) __attribute__((amdgpu_waves_per_eu
     const bool valid = jl < nt;
     const int j = t0 + jl;
     double x0 = 0.0, x1 = 0.0, x2 = 0.0;
-    int cur = 0, cend = 0;
     if (valid) {
       x0 = pts[3 * (size_t)j];
       x1 = pts[3 * (size_t)j + 1];
       x2 = pts[3 * (size_t)j + 2];
-      if (ex_ptr) {
-        cur = ex_ptr[j];
-        cend = ex_ptr[j + 1];
-      }
     }
-    // the next four excluded positions in registers (see k_tree_eval)
-    int nextex = INT_MAX, e1 = INT_MAX, e2 = INT_MAX, e3 = INT_MAX;
-    auto refill = [&]() {
-      nextex = cur < cend ? ex_pos[cur] : INT_MAX;
-      e1 = cur + 1 < cend ? ex_pos[cur + 1] : INT_MAX;
-      e2 = cur + 2 < cend ? ex_pos[cur + 2] : INT_MAX;
-      e3 = cur + 3 < cend ? ex_pos[cur + 3] : INT_MAX;
-    };
-    refill();
-    auto advance_to = [&](int pos) {
-      while (nextex < pos) {
-        ++cur;
-        nextex = e1;
-        e1 = e2;
-        e2 = e3;
-        e3 = INT_MAX;
-        if (nextex == INT_MAX && cur < cend) refill();
-      }
-    };
     acc_t ux = 0, uy = 0, uz = 0;
     int flag = OFW_FLAG_OK, near = 0;
     int sbase = INT_MIN / 2;
@@ -883,35 +860,67 @@ __global__ __launch_bounds__(64 * NEAR_WAVES) __attribute__((amdgpu_waves_per_eu
       }
     }
     wave_lds_sync();
+    // the accepted nodes first, then the opened leaves: the far field's registers and the near field's (the
+    // exclusion cursor, the segment window) are never live together
     for (int e = 0; e < ne; ++e) {
-      const int lo = __builtin_amdgcn_readlane(mine.lo, e), hi = __builtin_amdgcn_readlane(mine.hi, e);
+      const int lo = __builtin_amdgcn_readlane(mine.lo, e);
+      if (lo >= 0) continue;
       const unsigned long long mask =
           ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)mine.mask_hi, e) << 32) |
           (uint32_t)__builtin_amdgcn_readlane((int)mine.mask_lo, e);
       const bool open = valid && ((mask >> lane) & 1ull);
-      if (lo < 0) {
-        // an accepted node: its far field for the lanes that accepted it, as the walk's acceptance test formed
-        // R and its length (vortex_tree.py:526-537)
-        if (open) {
+      // an accepted node: its far field for the lanes that accepted it, as the walk's acceptance test formed
+      // R and its length (vortex_tree.py:526-537)
+      if (open) {
 #pragma clang fp contract(off)
-          const double rx = x0 - s_cen[wv][e][0], ry = x1 - s_cen[wv][e][1], rz = x2 - s_cen[wv][e][2];
-          if (PREC == OFW_FP64) {
-            const double dist = __builtin_sqrt(rx * rx + ry * ry + rz * rz);
-            double vx = 0.0, vy = 0.0, vz = 0.0;
-            far64(rx, ry, rz, dist, (const double*)&s_far[wv][e][0], order, vx, vy, vz);
-            ux += vx;
-            uy += vy;
-            uz += vz;
-          } else {
-            float vx, vy, vz;
-            far32((float)rx, (float)ry, (float)rz, (const float*)&s_far[wv][e][0], order, vx, vy, vz);
-            ux += (acc_t)vx;
-            uy += (acc_t)vy;
-            uz += (acc_t)vz;
-          }
+        const double rx = x0 - s_cen[wv][e][0], ry = x1 - s_cen[wv][e][1], rz = x2 - s_cen[wv][e][2];
+        if (PREC == OFW_FP64) {
+          const double dist = __builtin_sqrt(rx * rx + ry * ry + rz * rz);
+          double vx = 0.0, vy = 0.0, vz = 0.0;
+          far64(rx, ry, rz, dist, (const double*)&s_far[wv][e][0], order, vx, vy, vz);
+          ux += vx;
+          uy += vy;
+          uz += vz;
+        } else {
+          float vx, vy, vz;
+          far32((float)rx, (float)ry, (float)rz, (const float*)&s_far[wv][e][0], order, vx, vy, vz);
+          ux += (acc_t)vx;
+          uy += (acc_t)vy;
+          uz += (acc_t)vz;
         }
-        continue;
       }
+    }
+    int cur = 0, cend = 0;
+    if (valid && ex_ptr) {
+      cur = ex_ptr[j];
+      cend = ex_ptr[j + 1];
+    }
+    // the next four excluded positions in registers (see k_tree_eval)
+    int nextex = INT_MAX, e1 = INT_MAX, e2 = INT_MAX, e3 = INT_MAX;
+    auto refill = [&]() {
+      nextex = cur < cend ? ex_pos[cur] : INT_MAX;
+      e1 = cur + 1 < cend ? ex_pos[cur + 1] : INT_MAX;
+      e2 = cur + 2 < cend ? ex_pos[cur + 2] : INT_MAX;
+      e3 = cur + 3 < cend ? ex_pos[cur + 3] : INT_MAX;
+    };
+    refill();
+    auto advance_to = [&](int pos) {
+      while (nextex < pos) {
+        ++cur;
+        nextex = e1;
+        e1 = e2;
+        e2 = e3;
+        e3 = INT_MAX;
+        if (nextex == INT_MAX && cur < cend) refill();
+      }
+    };
+    for (int e = 0; e < ne; ++e) {
+      const int lo = __builtin_amdgcn_readlane(mine.lo, e), hi = __builtin_amdgcn_readlane(mine.hi, e);
+      if (lo < 0) continue;
+      const unsigned long long mask =
+          ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)mine.mask_hi, e) << 32) |
+          (uint32_t)__builtin_amdgcn_readlane((int)mine.mask_lo, e);
+      const bool open = valid && ((mask >> lane) & 1ull);
       if (open) advance_to(lo);
       if (PREC == OFW_FP64) {
         // FP64: the segment record is wave-uniform (scalar loads); an excluded pair is evaluated, not added
