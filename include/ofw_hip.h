@@ -127,7 +127,8 @@ int ofw_direct_sum_static(ofw_ctx *ctx, int precision, int block, int64_t n, con
 /* Counters of the context's last call (host buffer of `count` int64): [0] direct-sum segment chunks,
  * [1] segments per chunk, [2] traversal subtrees, [3] traversal split depth, [4] tree nodes, [5] tree depth,
  * [6] near-field work items of the traversal, [7] runs of the tree pipeline the last ofw_tree_sum took (2 when the
- * work-item slots kept from the call before were too few).
+ * work-item slots kept from the call before were too few, or the call had more exclusion entries than the
+ * capacity the context kept from its earlier calls).
  * Returns the number written. */
 int ofw_ctx_stats(ofw_ctx *ctx, int64_t *stats_host, int count);
 
@@ -155,7 +156,9 @@ int ofw_tree_nodes(ofw_ctx *ctx, int64_t m, const double *starts, const double *
  * (OFW_ERR_UNSUPPORTED otherwise, as vortex_tree.py:593-594 when a core is present).  The accepted nodes
  * are the reference's (FP64 acceptance test in every precision).  split_depth: depth of the traversal's
  * subtree frontier (-1 = chosen from the target count).  far_counts / near_counts (optional, may be NULL):
- * accepted nodes and evaluated near pairs per target. */
+ * accepted nodes and evaluated near pairs per target.  The context reads the exclusion count ex_ptr[n] back
+ * once and keeps a capacity (a quarter larger); later calls check it on the device instead of reading it,
+ * and a call beyond it runs its pipeline again with the exact count. */
 int ofw_tree_sum(ofw_ctx *ctx, int precision, int64_t n, const double *points, int64_t m, const double *starts,
                  const double *directions, const double *lengths, const double *gamma, const double *radius,
                  int model, int raise_singular, const int64_t *ex_ptr, const int64_t *ex_seg, double theta,
