@@ -105,6 +105,8 @@ int ofw_ctx_destroy(ofw_ctx* x) {
   c.exseg32.release();
   c.vjp.release();
   c.release_side();
+  if (c.host_counters) (void)hipHostFree(c.host_counters);
+  c.host_counters = nullptr;
   for (auto& b : c.tree) b.release();
   for (auto& e : c.ev)
     if (e) (void)hipEventDestroy(e);

@@ -43,6 +43,7 @@ struct Ctx {
   bool fork_pending = false;          // the node quantities record fork_ev once their subtree kernel is enqueued
   bool plan_forked = false;           // the plan is being enqueued on `side`
   hipStream_t plan_main = nullptr;    // the call's stream meanwhile (the null stream is a valid one)
+  int32_t* host_counters = nullptr;   // pinned host copy of a tree call's counters (allocated on first use)
   hipError_t mark() { return (!capturing && nev < 12) ? hipEventRecord(ev[nev++], stream) : hipSuccess; }
   void release_side() {
     if (side) {

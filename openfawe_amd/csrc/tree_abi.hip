@@ -1,5 +1,6 @@
 // C ABI of the tree code: build, node quantities, the tree sum and the batched (graph-replayed) tree sum
 // (include/ofw_hip.h).
+#include <cstring>
 #include <algorithm>
 #include <climits>
 #include <vector>
@@ -415,8 +416,15 @@ int ofw_tree_sum(ofw_ctx* x, int precision, int64_t n, const double* points, int
       c.sort_build = saved_sort;
       return st;
     }
-    OFW_HIP(hipMemcpyAsync(counters, t.counters, sizeof(counters), hipMemcpyDeviceToHost, c.stream));
+    // the counters through pinned memory: a direct copy, not a staged one
+    if (!c.host_counters && hipHostMalloc((void**)&c.host_counters, sizeof(counters), hipHostMallocDefault) != hipSuccess) {
+      c.host_counters = nullptr;
+      (void)hipGetLastError();
+    }
+    int32_t* dst = c.host_counters ? c.host_counters : counters;
+    OFW_HIP(hipMemcpyAsync(dst, t.counters, sizeof(counters), hipMemcpyDeviceToHost, c.stream));
     OFW_HIP(hipStreamSynchronize(c.stream));
+    if (dst != counters) std::memcpy(counters, dst, sizeof(counters));
     c.stats_i[7] = attempt + 1;  // runs of the pipeline this call took
     if (tc.ex_capacity && counters[ofw::C_EXNEED] != 0) {
       // more exclusions than the kept capacity (or a malformed CSR): count them and run again
