@@ -390,11 +390,12 @@ def timed_calls(step, steps, warmup):
     for _ in range(warmup):
         step()
     torch.cuda.synchronize()
+    ctx = V._ctx()  # the library context of the current stream: the same for every call
     phases = []
     t0 = time.perf_counter()
     for _ in range(steps):
         step()
-        phases.append(_lib.timings(V._ctx()))
+        phases.append(_lib.timings(ctx))
     torch.cuda.synchronize()
     ms = 1e3 * (time.perf_counter() - t0) / steps
     return ms, np.mean(np.array(phases), axis=0).tolist()
