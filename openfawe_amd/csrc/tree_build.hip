@@ -74,7 +74,9 @@ __global__ void k_iota(int m, int32_t* __restrict__ order) {
   if (i < m) order[i] = i;
 }
 
-__global__ void k_init_roots(TreeArrays t) {
+// The roots, the counters and the single-pass scans' tickets and status words, by one workgroup (every thread of
+// it calls this).
+__device__ void init_roots_block(const TreeArrays& t) {
   for (int k = threadIdx.x; k < t.K; k += blockDim.x) {
     // the root of an empty tree is no node (lo = -1): it takes no place in the pre-order layout
     const bool empty = t.roots_lo[k] == t.roots_lo[k + 1];
@@ -102,6 +104,8 @@ __global__ void k_init_roots(TreeArrays t) {
     t.counters[C_LEVEL] = t.K;
   }
 }
+
+__global__ void k_init_roots(TreeArrays t) { init_roots_block(t); }
 
 template <typename T>
 __global__ void k_fill_n(int64_t count, T* __restrict__ a, T v) {
@@ -872,16 +876,19 @@ __device__ __forceinline__ int rs_digit(uint32_t key, int tree, int pass) {
   return pass < RS_KEY_PASSES ? (int)((key >> (8 * pass)) & 0xff) : ((tree >> (8 * (pass - RS_KEY_PASSES))) & 0xff);
 }
 
-// Keys and segment ids of the three axes (buffer 0); no segment is in a run yet.
+// Keys and segment ids of the three axes (buffer 0); no segment is in a run yet.  The identity order, and (the
+// first workgroup) the roots, counters and scan words: the list build needs no launch of its own for them.
 __global__ void k_rs_init(TreeArrays t, const double* __restrict__ mx, const double* __restrict__ my,
                           const double* __restrict__ mz) {
   const int m = (int)t.m, A = blockIdx.y;
+  if (blockIdx.x == 0 && blockIdx.y == 0) init_roots_block(t);
   const double* __restrict__ coord = A == 0 ? mx : (A == 1 ? my : mz);
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
     const size_t g = (size_t)A * m + i;
     t.rkey[0][g] = ordered_key32(coord[i]);
     t.rval[0][g] = i;
     if (A == 0) {
+      t.order[i] = i;
       t.tie[i] = 0;
       t.ord_inv[i] = i;
       // the finish's histogram over lo (m + 1 entries)
@@ -2124,8 +2131,11 @@ int tree_build(Ctx& c, TreeArrays& t, const double* mx, const double* my, const 
   int64_t max_root = 0;
   for (int k = 0; k < t.K; ++k) max_root = std::max(max_root, offsets[k + 1] - offsets[k]);
   const int grid = c.cu_count * 2;
-  if (m > 0) hipLaunchKernelGGL(k_iota, dim3((m + 255) / 256), dim3(256), 0, c.stream, m, t.order);
-  hipLaunchKernelGGL(k_init_roots, dim3(1), dim3(256), 0, c.stream, t);
+  const bool list_init = t.lists && m > 0;  // the presort's first kernel takes the order and the roots
+  if (!list_init) {
+    if (m > 0) hipLaunchKernelGGL(k_iota, dim3((m + 255) / 256), dim3(256), 0, c.stream, m, t.order);
+    hipLaunchKernelGGL(k_init_roots, dim3(1), dim3(256), 0, c.stream, t);
+  }
   OFW_HIP(hipGetLastError());
   // top levels: while the largest node of a level can hold more than S_BOTTOM segments.  Level lvl reads the
   // order buffer of its parity and leaves its sorted nodes in the other one.
